@@ -1,0 +1,344 @@
+#!/usr/bin/env python
+"""Benchmark: Laplace-3D FMM wall time at N = 2^20, p = 10 (plus the near-field
+P2P pair rate), on one MI355X.
+
+Workload (the reference-metric configuration): 2^20 sources = targets uniform
+in [0,1]^3, strengths N(0,1), seed 0, order 10 at every level, at most 64
+particles per box, self-interaction excluded; PDE-compressed Taylor
+expansions with translation-class M2L.  A "step" is one full FMM evaluation
+pass (``drive_fmm``: P2M, M2M, P2P, M2L, M2P, P2L, L2L, L2P and the
+reorderings) with the tree, traversal and translation-class data built
+beforehand.
+
+Two forms are timed: the plain FP64 form and the mixed-precision form (FP32
+pair arithmetic with FP64 accumulation in P2P, FP32 MFMA in M2L).  ``value``
+is the mixed form's wall time per pass.  Errors are relative L2 errors
+against the FP64 direct sum on 4096 sampled targets.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks, TFLOP/s
+    "mfma_fp64": 78.6, "mfma_fp32": 157.3, "valu_fp64": 78.6, "valu_fp32": 157.3,
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(n, distribution, seed=0):
+    rng = np.random.default_rng(seed)
+    if distribution == "uniform":
+        pts = rng.random((3, n))
+    elif distribution == "gaussian8":
+        c = np.random.default_rng(0).uniform(0.1, 0.9, (8, 3))
+        pts = np.clip(c[rng.integers(0, 8, n)].T + 0.02*rng.standard_normal((3, n)), 0, 1)
+    else:
+        raise ValueError(distribution)
+    q = rng.standard_normal(n)
+    return pts, q
+
+
+def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision):
+    from functools import partial
+
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import LaplaceKernel
+    knl = LaplaceKernel(3)
+    tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                m2l_translation_override=tr),
+        [knl], exclude_self=True, p2p_precision=p2p_precision)
+    import torch
+    t2s = torch.arange(tree.ntargets, dtype=torch.int32, device=actx.device)
+    return SumpyExpansionWrangler(ti, trav, np.float64,
+                                  fmm_level_to_order=lambda k, a, t, lev: order,
+                                  self_extra_kwargs={"target_to_source": t2s})
+
+
+def time_passes(actx, fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0)*1e3/steps
+    return e0.elapsed_time(e1)/steps, wall, out
+
+
+def p2p_pairs(tree, trav):
+    th, vh = tree.host, trav.host
+    ns = th["box_source_counts_nonchild"].astype(np.int64)
+    nt = th["box_target_counts_nonchild"].astype(np.int64)
+    st, li = vh["neighbor_source_boxes_starts"], vh["neighbor_source_boxes_lists"]
+    per_box = np.add.reduceat(ns[li], st[:-1]) if len(li) else np.zeros(0)
+    per_box = np.where(np.diff(st) > 0, per_box, 0)
+    return int((nt[vh["target_boxes"]]*per_box).sum())
+
+
+def m2l_pairs(trav):
+    return int(len(trav.host["from_sep_siblings_lists"]))
+
+
+def cpu_baseline(tree, trav, q, order, budget_s=20.0):
+    """Oracle (FP64 restatement of the reference's CPU algorithms, OpenMP) on a
+    bounded sample of this workload's stages, extrapolated to the full pass."""
+    import oracle
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_fmm import rscale
+    th, vh = tree.host, trav.host
+    lbs = th["level_start_box_nrs"]
+    qs = np.asarray(q)[th["user_source_ids"]]
+    rng = np.random.default_rng(0)
+    stages = {}
+    # P2P over a sample of target boxes (pairs/s)
+    tb_all = vh["target_boxes"]
+    st1, li1 = vh["neighbor_source_boxes_starts"], vh["neighbor_source_boxes_lists"]
+    pick = np.sort(rng.choice(len(tb_all), size=max(1, len(tb_all)//32), replace=False))
+    sb_starts = np.concatenate([[0], np.cumsum(st1[pick + 1] - st1[pick])]).astype(np.int32)
+    sb_lists = np.concatenate([li1[st1[i]:st1[i + 1]] for i in pick]).astype(np.int32)
+    ns = th["box_source_counts_nonchild"].astype(np.int64)
+    ntc = th["box_target_counts_nonchild"].astype(np.int64)
+    pairs_sample = sum(ntc[tb_all[i]]*ns[li1[st1[i]:st1[i + 1]]].sum() for i in pick)
+    t0 = time.perf_counter()
+    oracle.p2p_csr(3, th, tb_all[pick], sb_starts, sb_lists, th["targets"], th["sources"],
+                   qs, t2s=np.arange(th["targets"].shape[1], dtype=np.int32))
+    dt = time.perf_counter() - t0
+    stages["p2p"] = dt*p2p_pairs(tree, trav)/pairs_sample
+    # M2L at the finest level with list-2 entries, on sampled targets
+    totp = vh["target_or_target_parent_boxes"]
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    st2, li2 = vh["from_sep_siblings_starts"], vh["from_sep_siblings_lists"]
+    cls_ls = vh["from_sep_siblings_translation_classes_level_starts"]
+    vecs = vh["from_sep_siblings_translation_class_to_distance_vector"]
+    lev = max(lv for lv in range(tree.nlevels) if cls_ls[lv + 1] > cls_ls[lv])
+    s0, s1 = lst[lev], lst[lev + 1]
+    rs = rscale(tree, lev, order)
+    t0 = time.perf_counter()
+    data = oracle.m2l_class_data(order, order, vecs[:, cls_ls[lev]:cls_ls[lev + 1]], rs)
+    t_cls = time.perf_counter() - t0
+    ncoef = (order + 1)**2
+    nbl = lbs[lev + 1] - lbs[lev]
+    M = rng.standard_normal((nbl, ncoef))
+    out = np.zeros((nbl, ncoef))
+    nsample = 128
+    idx = np.sort(rng.choice(s1 - s0, size=nsample, replace=False)) + s0
+    m_starts = np.concatenate([[0], np.cumsum(st2[idx + 1] - st2[idx])]).astype(np.int32)
+    m_lists = np.concatenate([li2[st2[i]:st2[i + 1]] for i in idx]).astype(np.int32)
+    m_cls = np.concatenate([vh["from_sep_siblings_translation_classes"][st2[i]:st2[i + 1]]
+                            for i in idx]).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.m2l(order, order, totp[idx], m_starts, m_lists, m_cls, int(cls_ls[lev]), data,
+               rs, rs, M, int(lbs[lev]), out, int(lbs[lev]))
+    dt = time.perf_counter() - t0
+    stages["m2l"] = dt*m2l_pairs(trav)/len(m_lists) + t_cls*0  # class data is setup
+    # P2M and L2P on sampled leaf boxes (per particle)
+    lsb = vh["level_start_source_box_nrs"]
+    lev_f = tree.nlevels - 1
+    sbx = vh["source_boxes"][lsb[lev_f]:lsb[lev_f + 1]]
+    sb_pick = sbx[:max(1, len(sbx)//256)]
+    mp = np.zeros((lbs[lev_f + 1] - lbs[lev_f], ncoef))
+    t0 = time.perf_counter()
+    oracle.p2m(order, sb_pick, th, th["box_centers"], th["sources"], qs, None,
+               rscale(tree, lev_f, order), mp, int(lbs[lev_f]))
+    dt = time.perf_counter() - t0
+    stages["p2m"] = dt*th["sources"].shape[1]/max(1, ns[sb_pick].sum())
+    t0 = time.perf_counter()
+    pot = np.zeros(th["targets"].shape[1])
+    oracle.l2p(order, sb_pick, th, th["box_centers"], th["targets"],
+               rscale(tree, lev_f, order), mp, int(lbs[lev_f]), pot)
+    dt = time.perf_counter() - t0
+    stages["l2p"] = dt*th["targets"].shape[1]/max(1, ntc[sb_pick].sum())
+    # M2M and L2L per translated box
+    lsp = vh["level_start_source_parent_box_nrs"]
+    pb = vh["source_parent_boxes"][lsp[lev_f - 1]:lsp[lev_f]]
+    pb_pick = pb[:max(1, len(pb)//64)]
+    mpp = np.zeros((lbs[lev_f] - lbs[lev_f - 1], ncoef))
+    t0 = time.perf_counter()
+    oracle.m2m(order, order, pb_pick, th["box_child_ids"], th["box_centers"],
+               rscale(tree, lev_f, order), rscale(tree, lev_f - 1, order), mp,
+               int(lbs[lev_f]), mpp, int(lbs[lev_f - 1]))
+    dt = time.perf_counter() - t0
+    stages["m2m+l2l"] = 2*dt*len(vh["source_parent_boxes"])/len(pb_pick)
+    total = sum(stages.values())
+    return {"value": total*1e3, "unit": "ms", "stages_s": stages,
+            "sample": ("oracle stages on sampled boxes of the same tree: P2P on 1/32 of "
+                       "the target boxes, M2L (non-FFT Toeplitz matvec) on 128 finest-level "
+                       "targets, P2M/L2P on 1/256 of the leaves, M2M on 1/64 of the "
+                       "parents; each extrapolated by its work units")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2**20)
+    ap.add_argument("--order", type=int, default=10)
+    ap.add_argument("--max-particles", type=int, default=64)
+    ap.add_argument("--distribution", default="uniform", choices=["uniform", "gaussian8"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nsample", type=int, default=4096)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from sumpy_amd import P2P, profiling
+    from sumpy_amd.array_context import TorchArrayContext
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.kernel import LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+
+    actx = TorchArrayContext("cuda")
+    pts, q = make_workload(args.n, args.distribution)
+    t0 = time.perf_counter()
+    tree, trav = build_tree_and_traversal(actx, pts, None, args.max_particles)
+    t_tree = time.perf_counter() - t0
+    log(f"tree: {tree.nboxes} boxes, {tree.nlevels} levels, built in {t_tree:.2f}s")
+    q_dev = actx.from_numpy(q)
+
+    forms = {
+        "fp64": make_wrangler(actx, tree, trav, args.order, "fp64", "fp64"),
+        "mixed": make_wrangler(actx, tree, trav, args.order, "fp32", "fp32"),
+    }
+    results = {}
+    setup = {}
+    for name, wr in forms.items():
+        t0 = time.perf_counter()
+        wr.multipole_to_local_precompute()
+        pots = drive_fmm(actx, wr, [q_dev])  # also builds the M2L tile schedules
+        torch.cuda.synchronize()
+        setup[name] = time.perf_counter() - t0
+        ms, wall, pots = time_passes(actx, lambda wr=wr: drive_fmm(actx, wr, [q_dev]),
+                                     args.steps, args.warmup)
+        results[name] = {"ms": ms, "wall_ms": wall, "pot": pots[0]}
+        log(f"{name}: {ms:.3f} ms/pass (host wall {wall:.3f} ms), setup {setup[name]:.2f}s")
+
+    # per-kernel spans over one more set of passes
+    spans = {}
+    for name, wr in forms.items():
+        profiling.enable(True)
+        for _ in range(args.steps):
+            drive_fmm(actx, wr, [q_dev])
+        spans[name] = profiling.summary()
+        profiling.enable(False)
+        log(f"{name} kernel spans: "
+            + ", ".join(f"{k}: {v['ms']/args.steps:.3f} ms" for k, v in spans[name].items()))
+
+    # accuracy vs the FP64 direct sum on sampled targets
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(args.n, size=min(args.nsample, args.n), replace=False))
+    knl = LaplaceKernel(3)
+    direct, = P2P([knl], exclude_self=True)(
+        actx, actx.from_numpy(pts[:, sample]), actx.from_numpy(pts), [q_dev],
+        target_to_source=actx.from_numpy(sample.astype(np.int32)))
+    direct = actx.to_numpy(direct)
+    for name in forms:
+        pot = actx.to_numpy(results[name]["pot"])[sample]
+        results[name]["rel_l2_err"] = float(np.linalg.norm(pot - direct)/np.linalg.norm(direct))
+        log(f"{name}: rel. L2 error vs direct on {len(sample)} targets "
+            f"{results[name]['rel_l2_err']:.3e}")
+
+    # near-field pair rate
+    npairs = p2p_pairs(tree, trav)
+    p2p_rate = {}
+    for name, wr in forms.items():
+        ms, _, _ = time_passes(actx, lambda wr=wr: wr.eval_direct(
+            actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
+            trav.neighbor_source_boxes_lists, [q_dev]), args.steps, args.warmup)
+        p2p_rate[name] = npairs/(ms*1e-3)/1e9
+        log(f"{name}: P2P {npairs:.3e} pairs in {ms:.3f} ms = {p2p_rate[name]:.1f} Gpairs/s")
+
+    # roofline of the dominant kernel of the mixed form
+    sp = spans["mixed"]
+    dom = max(sp, key=lambda k: sp[k]["ms"])
+    npair_m2l = m2l_pairs(trav)
+    nl = (args.order + 1)**2
+    if dom.startswith("m2l_apply"):
+        flops = 2.0*nl*nl*npair_m2l*args.steps
+        peak = PEAKS["mfma_fp32" if dom.endswith("fp32") else "mfma_fp64"]
+        bound = "mfma"
+    else:
+        flops = 11.0*npairs*args.steps
+        peak = PEAKS["valu_fp32" if dom.endswith("fp32") else "valu_fp64"]
+        bound = "valu"
+    achieved = flops/(sp[dom]["ms"]*1e-3)/1e12
+    roofline = {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
+                "peak": peak, "unit": "TFLOP/s", "frac": round(achieved/peak, 4),
+                "traffic": None,
+                "launches_per_step": sp[dom]["launches"]//args.steps,
+                "avg_launch_ms": sp[dom]["ms"]/sp[dom]["launches"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        cb = cpu_baseline(tree, trav, q, args.order)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": round(cb["value"], 1), "unit": "ms", "cores": cores, "kind": "port",
+               "sample": cb["sample"],
+               "stages_s": {k: round(v, 3) for k, v in cb["stages_s"].items()}}
+        log(f"cpu baseline: {cb['value']/1e3:.1f} s extrapolated "
+            f"({time.perf_counter() - t0:.1f}s measured)")
+
+    mixed = results["mixed"]
+    line = {
+        "metric": "Laplace-3D FMM wall time N=2^20 p=10 + P2P Ginteractions/s at matched rel error",
+        "value": round(mixed["ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64 mixed (FP32 MFMA M2L, FP32 P2P with FP64 accumulation)",
+        "data": "synthetic",
+        "config": {"workload": f"laplace3d_fmm_{args.distribution}", "n": args.n,
+                   "order": args.order, "max_particles_in_box": args.max_particles,
+                   "nboxes": tree.nboxes, "nlevels": tree.nlevels,
+                   "p2p_pairs": npairs, "m2l_pairs": npair_m2l},
+        "fp64_ms_per_step": round(results["fp64"]["ms"], 4),
+        "speedup_mixed_vs_fp64": round(results["fp64"]["ms"]/mixed["ms"], 3),
+        "rel_l2_err": {k: results[k]["rel_l2_err"] for k in forms},
+        "p2p_ginteractions_per_s": {k: round(v, 2) for k, v in p2p_rate.items()},
+        "kernel_ms_per_step": {f: {k: round(v["ms"]/args.steps, 4) for k, v in spans[f].items()}
+                               for f in forms},
+        "setup_s": {"tree_and_traversal": round(t_tree, 3),
+                    **{f"m2l_precompute_{k}": round(v, 3) for k, v in setup.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,243 @@
+/*
+ * C-ABI of libsumpyhip.so: hand-written CDNA4 (gfx950) kernels for sumpy's
+ * FMM/QBX interaction path.
+ *
+ * Conventions (all entry points):
+ *   - every pointer is a device pointer (caller-owned; the library never
+ *     frees caller memory), except the sumpyhip_expn descriptor itself, which
+ *     is a host struct whose pointer members are device pointers;
+ *   - coordinates are structure-of-arrays: one pointer per axis;
+ *   - real kernels use double arrays; the Helmholtz kernel uses interleaved
+ *     complex128 (re, im) for weights and outputs;
+ *   - "stream" is a hipStream_t; work is enqueued asynchronously on it;
+ *   - return value 0 = success, otherwise an error code;
+ *     sumpyhip_last_error() describes the last failure of the calling thread.
+ *
+ * Kernel ids: 2 = Laplace 2D, 3 = Laplace 3D, 13 = Helmholtz 3D.
+ * Precision: 0 = plain FP64; 1 = FP32 pair math with FP64 accumulation (P2P),
+ * FP32 MFMA (M2L).
+ */
+#ifndef SUMPYHIP_H
+#define SUMPYHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUMPYHIP_OK 0
+#define SUMPYHIP_EINVAL 1
+#define SUMPYHIP_EHIP 2
+#define SUMPYHIP_ENOTIMPL 3
+
+/* Index tables of one expansion (kind, order), see sumpy_amd/expansion. */
+typedef struct sumpyhip_expn {
+  int order;
+  int nfull;                    /* full Taylor identifiers |mi| <= order */
+  int nstored;                  /* stored (PDE-compressed) identifiers */
+  const int *full_ids;          /* 3 x nfull, axis-major */
+  const int *stored_ids;        /* 3 x nstored */
+  const int *stored_in_full;    /* nstored: full index of each stored id */
+  const int *cube_index;        /* (order+1)^3: full index of (i,j,k), -1 if i+j+k>order */
+  const int *proj_row_starts;   /* P (nfull x nstored) as CSR over full rows */
+  const int *proj_cols;
+  const double *proj_vals;
+  const int *projT_col_starts;  /* P as CSC over stored columns */
+  const int *projT_rows;
+  const double *projT_vals;
+  const int *horner_order;      /* nfull: full index at each Horner position */
+} sumpyhip_expn;
+
+const char *sumpyhip_last_error(void);
+int sumpyhip_device_info(int *n_cu, int *gcn_arch_major, int *gcn_arch_minor);
+
+/* {{{ P2P (sumpy/p2p.py) */
+
+/* Dense direct sum, replaces P2P.__call__ (sumpy/p2p.py:267-285):
+ * result[t] = scale * sum_s [w0_s G(d) + w_s . grad_d G(d)], d = t - s,
+ * plus the target gradient when gx/gy/gz are given.  target_to_source may be
+ * NULL; otherwise the pair s == target_to_source[t] is skipped
+ * (exclude_self, sumpy/p2p.py:162-170).  Overwrites pot/grad. */
+int sumpyhip_p2p_dense(int kernel_id, int precision, double k_re, double k_im,
+                       double scale, int ntargets, const double *tx, const double *ty,
+                       const double *tz, int nsources, const double *sx,
+                       const double *sy, const double *sz, const double *w0,
+                       const double *wx, const double *wy, const double *wz,
+                       const int *target_to_source, double *pot, double *gx,
+                       double *gy, double *gz, void *stream);
+
+/* Near-field sum over per-box CSR source lists, replaces P2PFromCSR.__call__
+ * (sumpy/p2p.py:801-833; loop structure :551-670).  Overwrites the results
+ * of the targets in target_boxes.  box_centers (dim x nboxes) give the local
+ * origin of the FP32 form. */
+int sumpyhip_p2p_csr(int kernel_id, int precision, double k_re, double k_im,
+                     double scale, int ntgt_boxes, const int *target_boxes,
+                     const int *source_box_starts, const int *source_box_lists,
+                     const int *box_source_starts, const int *box_source_counts_nonchild,
+                     const int *box_target_starts, const int *box_target_counts_nonchild,
+                     const double *box_centers, int nboxes, const double *tx,
+                     const double *ty, const double *tz, const double *sx,
+                     const double *sy, const double *sz, const double *w0,
+                     const double *wx, const double *wy, const double *wz,
+                     const int *target_to_source, double *pot, double *gx,
+                     double *gy, double *gz, void *stream);
+
+/* }}} */
+
+/* {{{ expansions (Laplace 3D, Taylor / PDE-compressed Taylor) */
+
+/* P2M, replaces P2EFromSingleBox.__call__ (sumpy/p2e.py:155-261):
+ * tgt_expansions[b - tgt_base_ibox, :] = coefficients of the sources of box b
+ * (overwrite).  wx/wy/wz may be NULL (no dipole sources). */
+int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                 const int *source_boxes, const int *box_source_starts,
+                 const int *box_source_counts_nonchild, const double *box_centers,
+                 int nboxes, const double *sx, const double *sy, const double *sz,
+                 const double *w0, const double *wx, const double *wy,
+                 const double *wz, double rscale, double *tgt_expansions,
+                 int tgt_base_ibox, void *stream);
+
+/* P2L from list 4, replaces P2EFromCSR.__call__ (sumpy/p2e.py:268-388),
+ * overwrite. */
+int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                 const int *target_boxes, const int *source_box_starts,
+                 const int *source_box_lists, const int *box_source_starts,
+                 const int *box_source_counts_nonchild, const double *box_centers,
+                 int nboxes, const double *sx, const double *sy, const double *sz,
+                 const double *w0, const double *wx, const double *wy,
+                 const double *wz, double rscale, double *tgt_expansions,
+                 int tgt_base_ibox, void *stream);
+
+/* M2M, replaces E2EFromChildren.__call__ (sumpy/e2e.py:866-978):
+ * tgt[parent] += sum_children T(c_parent - c_child) src[child]. */
+int sumpyhip_m2m(int kernel_id, const sumpyhip_expn *src_expn,
+                 const sumpyhip_expn *tgt_expn, int ntgt_boxes,
+                 const int *target_boxes, const int *box_child_ids,
+                 const double *box_centers, int nboxes, double src_rscale,
+                 double tgt_rscale, const double *src_expansions, int src_base_ibox,
+                 double *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* L2L, replaces E2EFromParent.__call__ (sumpy/e2e.py:985-1083):
+ * tgt[child] += T(c_child - c_parent) src[parent]. */
+int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
+                 const sumpyhip_expn *tgt_expn, int ntgt_boxes,
+                 const int *target_boxes, const int *box_parent_ids,
+                 const double *box_centers, int nboxes, double src_rscale,
+                 double tgt_rscale, const double *src_expansions, int src_base_ibox,
+                 double *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226); overwrites
+ * pot (and the gradient when given) for the targets of each box. */
+int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                 const int *target_boxes, const int *box_target_starts,
+                 const int *box_target_counts_nonchild, const double *box_centers,
+                 int nboxes, const double *tx, const double *ty, const double *tz,
+                 double rscale, double scale, const double *src_expansions,
+                 int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
+                 void *stream);
+
+/* M2P from list 3, replaces E2PFromCSR.__call__ (sumpy/e2p.py:233-343);
+ * accumulates into pot (and the gradient when given). */
+int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                 const int *target_boxes, const int *source_box_starts,
+                 const int *source_box_lists, const int *box_target_starts,
+                 const int *box_target_counts_nonchild, const double *box_centers,
+                 int nboxes, const double *tx, const double *ty, const double *tz,
+                 double rscale, double scale, const double *src_expansions,
+                 int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
+                 void *stream);
+
+/* }}} */
+
+/* {{{ M2L through translation classes */
+
+/* Class data, replaces M2LGenerateTranslationClassesDependentData.__call__
+ * (sumpy/e2e.py:565-669): for class c, out[c, circ(m)] = rscale^|m| d^m G(v_c)
+ * for the derivatives m = s + t the translation needs, in the circulant layout
+ * of the reference's non-FFT M2L (sumpy/expansion/m2l.py:403-537); other
+ * entries are zero.  circ_shape = max_mi + 1 (3 ints). */
+int sumpyhip_m2l_class_data(int kernel_id, int order_src, int order_tgt,
+                            const int *circ_shape, int nclasses,
+                            const double *translation_vectors, int nvectors,
+                            int class_start, double src_rscale, double *out,
+                            void *stream);
+
+/* Preprocessing, replaces M2LPreprocessMultipole.__call__
+ * (sumpy/e2e.py:676-760): pack stored multipoles into rows of kpad entries of
+ * the M2L compute precision (FP64 or FP32), zero padded. */
+int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
+                            const double *src_expansions, int kpad, void *packed,
+                            void *stream);
+
+/* Translation, replaces M2LUsingTranslationClassesDependentData.__call__
+ * (sumpy/e2e.py:310-558) as a class-batched dense contraction on MFMA:
+ * for each tile of 64 target boxes and each class slot of the tile,
+ * Y += A_c X with A_c[t, s] = data_c[off_t[t] + off_s[s]] and X the stacked
+ * packed multipoles of the tile's sources.  The schedule (tiles, slots,
+ * per-slot source boxes) is built on the host.  Overwrites the work rows of
+ * the tile targets. */
+int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_targets,
+                       const int *tile_slot_starts, const int *slot_class,
+                       const int *slot_sources, int nl, int nm, int npad, int kpad,
+                       const int *off_t, const int *off_s, int ncirc,
+                       const double *class_data, const void *packed_src,
+                       int src_base_ibox, void *work, int tgt_base_ibox,
+                       void *stream);
+
+/* Postprocessing, replaces M2LPostprocessLocal.__call__
+ * (sumpy/e2e.py:767-859): locals[b, i] = work[b, i] * ratio^deg_i. */
+int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int npad,
+                             const void *work, const int *tgt_degree, double ratio,
+                             double *tgt_expansions, void *stream);
+
+/* M2L without translation classes, replaces E2EFromCSR.__call__ for
+ * multipole-to-local (sumpy/e2e.py:164-304); overwrites. */
+int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
+                     const int *circ_shape, int nl, int nm, const int *off_t,
+                     const int *off_s, const int *tgt_degree, int ntgt_boxes,
+                     const int *target_boxes, const int *src_box_starts,
+                     const int *src_box_lists, const double *box_centers, int nboxes,
+                     double src_rscale, double tgt_rscale, const double *src_expansions,
+                     int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
+                     void *stream);
+
+/* }}} */
+
+/* {{{ source weights and target outputs */
+
+/* w[a][s] += coeff * strength[s] * (dir ? dir[a][s] : 1): builds the charge
+ * (a = -1 -> w0) and dipole arrays from the reference's source kernels
+ * (AxisSourceDerivative / DirectionalSourceDerivative, sumpy/kernel.py:2288-2507).
+ * Complex when is_complex (interleaved). */
+int sumpyhip_axpy_weights(int is_complex, int n, double coeff, const double *strength,
+                          const double *dir, double *w, void *stream);
+
+/* out[t] = sum_j coeff_j * (mult_j ? x_{mult_j}[t] : 1) * in_j[t]: combines
+ * potential/gradient into TargetPointMultiplier / AxisTargetDerivative results
+ * (sumpy/kernel.py:2330-2588). */
+int sumpyhip_combine_outputs(int is_complex, int n, int nterms, const double *const *ins,
+                             const double *coeffs, const double *const *mults,
+                             double *out, int accumulate, void *stream);
+
+/* }}} */
+
+/* {{{ QBX */
+
+/* 2D Laplace line-Taylor layer potential, replaces LayerPotential.__call__
+ * (sumpy/qbx.py:250-335) for LineTaylorLocalExpansion(LaplaceKernel(2)):
+ * out[t] = scale * sum_s sigma_s Re sum_{i<=order} F^(i)(a) b^i / i!,
+ * a = c_t - s, b = t - c_t (complex notation), F = log (src_kind 0, SLP),
+ * -n/a (1, DLP), -e_axis/a (2+axis, AxisSourceDerivative); tgt_axis >= 0
+ * applies d/da_axis.  nx/ny: source normals (DLP). */
+int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntargets,
+                   const double *tx, const double *ty, const double *cx,
+                   const double *cy, int nsources, const double *sx,
+                   const double *sy, const double *nx, const double *ny,
+                   const double *density, double scale, double *out, void *stream);
+
+/* }}} */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,309 @@
+"""CPU oracle for the FMM/QBX hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package, and only as the checker or as the timed CPU
+baseline.  The product package ``sumpy_amd`` never imports it.
+
+``oracle.c`` holds the FP64 restatements of the reference's CPU algorithms
+(each function cites the reference file:line it follows); ``tables.py`` builds
+the index tables independently of the product's table code.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import tables
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+KNL_LAP2D, KNL_LAP3D, KNL_HELM3D = 2, 3, 13
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype in (np.float64, np.complex128) and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class _PtrArray:
+    """Keeps a ``double*[n]`` array and its referents alive."""
+
+    def __init__(self, arrays):
+        self.arrays = [np.ascontiguousarray(a) for a in arrays]
+        self.c = (_dp * len(self.arrays))(*[_d(a) for a in self.arrays])
+
+
+def _ptrs(arrays):
+    return None if arrays is None else _PtrArray(arrays)
+
+
+def _c(p):
+    return None if p is None else p.c
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# {{{ P2P
+
+def p2p_dense(knl, targets, sources, w0, wd=None, t2s=None, k=0.0, scale=None,
+              want_grad=False):
+    """Direct sum; returns (pot, grad or None).  ``knl`` in {2, 3, 13}."""
+    dim = targets.shape[0]
+    nt, ns = targets.shape[1], sources.shape[1]
+    cdt = np.complex128 if knl == KNL_HELM3D else np.float64
+    if scale is None:
+        scale = {2: -1/(2*np.pi), 3: 1/(4*np.pi), 13: 1/(4*np.pi)}[knl]
+    pot = np.zeros(nt, cdt)
+    grad = [np.zeros(nt, cdt) for _ in range(dim)] if want_grad else None
+    tp, sp = _ptrs(list(targets)), _ptrs(list(sources))
+    w0 = np.ascontiguousarray(w0, dtype=cdt)
+    wdp = _ptrs([np.ascontiguousarray(w, dtype=cdt) for w in wd]) if wd is not None else None
+    gp = _PtrArray(grad) if grad is not None else None
+    if gp is not None:
+        grad = gp.arrays
+    lib().orc_p2p_dense(
+        ctypes.c_int(knl), ctypes.c_int(dim), ctypes.c_double(np.real(k)),
+        ctypes.c_double(np.imag(k)), ctypes.c_double(scale),
+        ctypes.c_long(nt), tp.c, ctypes.c_long(ns), sp.c, _d(w0), _c(wdp),
+        _i(None if t2s is None else i32(t2s)), _d(pot), _c(gp))
+    return pot, grad
+
+
+def p2p_csr(knl, tree_arrays, target_boxes, src_box_starts, src_box_lists,
+            targets, sources, w0, wd=None, t2s=None, k=0.0, scale=None,
+            want_grad=False, pot=None):
+    """P2PFromCSR in the CPU loop order of ``p2p.py:626-670``; overwrites the
+    targets of the listed boxes in *pot* (zeros elsewhere if not given)."""
+    dim = targets.shape[0]
+    nt = targets.shape[1]
+    cdt = np.complex128 if knl == KNL_HELM3D else np.float64
+    if scale is None:
+        scale = {2: -1/(2*np.pi), 3: 1/(4*np.pi), 13: 1/(4*np.pi)}[knl]
+    if pot is None:
+        pot = np.zeros(nt, cdt)
+    grad = [np.zeros(nt, cdt) for _ in range(dim)] if want_grad else None
+    gp = _PtrArray(grad) if grad is not None else None
+    if gp is not None:
+        grad = gp.arrays
+    tp, sp = _ptrs(list(targets)), _ptrs(list(sources))
+    w0 = np.ascontiguousarray(w0, dtype=cdt)
+    wdp = _ptrs([np.ascontiguousarray(w, dtype=cdt) for w in wd]) if wd is not None else None
+    bss, bsc, bts, btc = (i32(tree_arrays[n]) for n in (
+        "box_source_starts", "box_source_counts_nonchild",
+        "box_target_starts", "box_target_counts_nonchild"))
+    tb, sbs, sbl = i32(target_boxes), i32(src_box_starts), i32(src_box_lists)
+    lib().orc_p2p_csr(
+        ctypes.c_int(knl), ctypes.c_int(dim), ctypes.c_double(np.real(k)),
+        ctypes.c_double(np.imag(k)), ctypes.c_double(scale),
+        ctypes.c_long(len(tb)), _i(tb), _i(sbs), _i(sbl), _i(bss), _i(bsc),
+        _i(bts), _i(btc), tp.c, sp.c, _d(w0), _c(wdp),
+        _i(None if t2s is None else i32(t2s)), _d(pot), _c(gp))
+    return pot, grad
+
+# }}}
+
+
+# {{{ Laplace 3D expansions
+
+def lap3d_derivs(x, rscale, maxorder, box):
+    """Scaled derivatives ``rscale^|m| d^m (1/|x|)`` over ``box`` (M0,M1,M2)."""
+    M = i32(box)
+    D = np.zeros(tuple(int(m) + 1 for m in box))
+    lib().orc_lap3d_derivs(_d(f64(x)), ctypes.c_double(rscale),
+                           ctypes.c_int(maxorder), _i(M), _d(D))
+    return D
+
+
+def p2m(order, source_boxes, tree_arrays, centers, sources, w0, wd, rscale,
+        tgt_expansions, tgt_base_ibox, compressed=True):
+    w = tables.wrangler(order, 3, compressed)
+    full = i32(w.full)
+    P = f64(w.projection(rscale))
+    cp, sp = _PtrArray(list(centers)), _PtrArray(list(sources))
+    wdp = _ptrs([f64(a) for a in wd]) if wd is not None else None
+    sb = i32(source_boxes)
+    assert tgt_expansions.flags.c_contiguous
+    lib().orc_p2m(ctypes.c_int(len(w.full)), _i(full), ctypes.c_int(len(w.stored)),
+                  _d(P), ctypes.c_long(len(sb)), _i(sb),
+                  _i(i32(tree_arrays["box_source_starts"])),
+                  _i(i32(tree_arrays["box_source_counts_nonchild"])),
+                  cp.c, sp.c, _d(f64(w0)), _c(wdp), ctypes.c_double(rscale),
+                  _d(tgt_expansions), ctypes.c_int(tgt_base_ibox))
+    return tgt_expansions
+
+
+def m2m(src_order, tgt_order, target_boxes, box_child_ids, centers, src_rscale,
+        tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox,
+        compressed=True):
+    wt = tables.wrangler(tgt_order, 3, compressed)
+    ws = tables.wrangler(src_order, 3, compressed)
+    cp = _PtrArray(list(centers))
+    tb = i32(target_boxes)
+    bci = i32(box_child_ids)
+    lib().orc_m2m(ctypes.c_int(len(wt.full)), _i(i32(wt.full)),
+                  ctypes.c_int(len(wt.stored)), _d(f64(wt.projection(tgt_rscale))),
+                  ctypes.c_int(len(ws.stored)), _i(i32(ws.stored)),
+                  ctypes.c_long(len(tb)), _i(tb), _i(bci),
+                  ctypes.c_long(bci.shape[1]), cp.c, ctypes.c_double(src_rscale),
+                  ctypes.c_double(tgt_rscale), _d(src_expansions),
+                  ctypes.c_int(src_base_ibox), _d(tgt_expansions),
+                  ctypes.c_int(tgt_base_ibox))
+    return tgt_expansions
+
+
+def m2l_class_data(order_src, order_tgt, vectors, src_rscale):
+    """Per-class circulant data (``ntranslation_classes x ncirc``)."""
+    T = tables.m2l_tables(order_src, order_tgt)
+    w2 = T["w2"]
+    P2 = f64(w2.projection(src_rscale))
+    out = np.zeros((vectors.shape[1], T["ncirc"]))
+    for c in range(vectors.shape[1]):
+        row = np.zeros(T["ncirc"])
+        lib().orc_m2l_class_data(
+            _d(f64(vectors[:, c])), ctypes.c_double(src_rscale),
+            ctypes.c_int(order_src + order_tgt), _i(T["max_mi"]),
+            ctypes.c_int(len(w2.full)), _i(i32(w2.full)),
+            ctypes.c_int(len(w2.stored)), _i(i32(w2.stored)), _d(P2),
+            ctypes.c_int(len(T["needed_full_idx"])), _i(T["needed_full_idx"]),
+            _i(T["needed_circ_idx"]), ctypes.c_int(T["ncirc"]), _d(row))
+        out[c] = row
+    return out
+
+
+def m2l(order_src, order_tgt, target_boxes, src_box_starts, src_box_lists,
+        class_lists, class_level_start, class_data, src_rscale, tgt_rscale,
+        src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox):
+    T = tables.m2l_tables(order_src, order_tgt)
+    tb = i32(target_boxes)
+    lib().orc_m2l(ctypes.c_int(T["ncirc"]), ctypes.c_int(len(T["src_circ_idx"])),
+                  _i(T["src_circ_idx"]), ctypes.c_int(len(T["tgt_circ_idx"])),
+                  _i(T["tgt_circ_idx"]), _i(T["tgt_degree"]), ctypes.c_long(len(tb)),
+                  _i(tb), _i(i32(src_box_starts)), _i(i32(src_box_lists)),
+                  _i(i32(class_lists)), ctypes.c_int(class_level_start),
+                  _d(f64(class_data)), ctypes.c_double(src_rscale),
+                  ctypes.c_double(tgt_rscale), _d(src_expansions),
+                  ctypes.c_int(src_base_ibox), _d(tgt_expansions),
+                  ctypes.c_int(tgt_base_ibox))
+    return tgt_expansions
+
+
+def l2l(src_order, tgt_order, target_boxes, box_parent_ids, centers, src_rscale,
+        tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox,
+        compressed=True):
+    ws = tables.wrangler(src_order, 3, compressed)
+    wt = tables.wrangler(tgt_order, 3, compressed)
+    cp = _PtrArray(list(centers))
+    tb = i32(target_boxes)
+    lib().orc_l2l(ctypes.c_int(len(ws.full)), _i(i32(ws.full)),
+                  ctypes.c_int(len(ws.stored)), _d(f64(ws.projection(src_rscale))),
+                  ctypes.c_int(len(wt.stored)), _i(i32(wt.stored)),
+                  ctypes.c_int(src_order), ctypes.c_long(len(tb)), _i(tb),
+                  _i(i32(box_parent_ids)), cp.c, ctypes.c_double(src_rscale),
+                  ctypes.c_double(tgt_rscale), _d(src_expansions),
+                  ctypes.c_int(src_base_ibox), _d(tgt_expansions),
+                  ctypes.c_int(tgt_base_ibox))
+    return tgt_expansions
+
+
+def l2p(order, target_boxes, tree_arrays, centers, targets, rscale,
+        src_expansions, src_base_ibox, pot, grad=None, scale=1/(4*np.pi),
+        compressed=True):
+    w = tables.wrangler(order, 3, compressed)
+    cp, tp = _PtrArray(list(centers)), _PtrArray(list(targets))
+    gp = _PtrArray(grad) if grad is not None else None
+    tb = i32(target_boxes)
+    lib().orc_l2p(ctypes.c_int(len(w.full)), _i(i32(w.full)),
+                  ctypes.c_int(len(w.stored)), _d(f64(w.projection(rscale))),
+                  ctypes.c_long(len(tb)), _i(tb),
+                  _i(i32(tree_arrays["box_target_starts"])),
+                  _i(i32(tree_arrays["box_target_counts_nonchild"])),
+                  cp.c, tp.c, ctypes.c_double(rscale), ctypes.c_double(scale),
+                  _d(src_expansions), ctypes.c_int(src_base_ibox), _d(pot), _c(gp))
+    return pot, (gp.arrays if gp is not None else None)
+
+
+def m2p(order, target_boxes, src_box_starts, src_box_lists, tree_arrays, centers,
+        targets, rscale, src_expansions, src_base_ibox, pot, grad=None,
+        scale=1/(4*np.pi), compressed=True):
+    w = tables.wrangler(order, 3, compressed)
+    cp, tp = _PtrArray(list(centers)), _PtrArray(list(targets))
+    gp = _PtrArray(grad) if grad is not None else None
+    tb = i32(target_boxes)
+    lib().orc_m2p(ctypes.c_int(order), ctypes.c_int(len(w.stored)),
+                  _i(i32(w.stored)), ctypes.c_long(len(tb)), _i(tb),
+                  _i(i32(src_box_starts)), _i(i32(src_box_lists)),
+                  _i(i32(tree_arrays["box_target_starts"])),
+                  _i(i32(tree_arrays["box_target_counts_nonchild"])),
+                  cp.c, tp.c, ctypes.c_double(rscale), ctypes.c_double(scale),
+                  _d(src_expansions), ctypes.c_int(src_base_ibox), _d(pot), _c(gp))
+    return pot, (gp.arrays if gp is not None else None)
+
+
+def p2l(order, target_boxes, src_box_starts, src_box_lists, tree_arrays, centers,
+        sources, w0, wd, rscale, tgt_expansions, tgt_base_ibox, compressed=True):
+    w = tables.wrangler(order, 3, compressed)
+    cp, sp = _PtrArray(list(centers)), _PtrArray(list(sources))
+    wdp = _ptrs([f64(a) for a in wd]) if wd is not None else None
+    tb = i32(target_boxes)
+    lib().orc_p2l(ctypes.c_int(order), ctypes.c_int(len(w.stored)),
+                  _i(i32(w.stored)), ctypes.c_long(len(tb)), _i(tb),
+                  _i(i32(src_box_starts)), _i(i32(src_box_lists)),
+                  _i(i32(tree_arrays["box_source_starts"])),
+                  _i(i32(tree_arrays["box_source_counts_nonchild"])),
+                  cp.c, sp.c, _d(f64(w0)), _c(wdp), ctypes.c_double(rscale),
+                  _d(tgt_expansions), ctypes.c_int(tgt_base_ibox))
+    return tgt_expansions
+
+# }}}
+
+
+def qbx2d(order, srcmode, tgt_axis, targets, centers, radii, sources, normals,
+          density, scale=-1/(2*np.pi)):
+    """Dense 2D Laplace line-Taylor layer potential (``qbx.py:262-308``)."""
+    nt, ns = targets.shape[1], sources.shape[1]
+    out = np.zeros(nt)
+    tp, cp, sp = _PtrArray(list(targets)), _PtrArray(list(centers)), _PtrArray(list(sources))
+    npt = _PtrArray(list(normals)) if normals is not None else None
+    lib().orc_qbx2d(ctypes.c_int(order), ctypes.c_int(srcmode), ctypes.c_int(tgt_axis),
+                    ctypes.c_long(nt), tp.c, cp.c, _d(f64(radii)), ctypes.c_long(ns),
+                    sp.c, _c(npt), _d(f64(density)), ctypes.c_double(scale), _d(out))
+    return out

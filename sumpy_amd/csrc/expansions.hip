@@ -1,0 +1,659 @@
+// Taylor (full and PDE-compressed) expansions of the 3D Laplace kernel:
+// P2M, M2M, L2L, L2P, M2P, P2L.
+//
+// Reference math: sumpy/expansion/multipole.py:70-407 (multipole formation,
+// evaluation, M2M), sumpy/expansion/local.py:189-493 (local formation,
+// evaluation, L2L), sumpy/expansion/__init__.py:563-886 (compressed storage and
+// the PDE projection P), sumpy/derivative_taker.py:195-262 (scaled Laplace
+// derivatives).  The reference emits one symbolic expression per coefficient;
+// here each operator is a per-box kernel that works on the full coefficient
+// set in LDS and applies P (or P^T) from sparse tables:
+//   P2M  c_full[mi] = sum_s w (a/rho)^mi / mi!  (a = centre - source), then
+//        M = P^T c_full;
+//   M2M  embed the child's stored coefficients into the full set, shift the
+//        monomials one axis at a time, sum over children, M += P^T c;
+//   L2L  L_full = P L, shift along each axis, select the stored ids;
+//   L2P  L_full = P L folded with 1/(mi! rho^|mi|) into LDS in Horner order,
+//        then a triple Horner per target (with its gradient);
+//   M2P, P2L  scaled derivatives rho^|m| d^m (1/r) from the recurrence.
+// Roofline: these kernels move little data (tens of MB at N = 2^20) and are
+// bound by latency/VALU; they are small next to P2P and M2L.
+#include "common.h"
+
+namespace sumpyhip {
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int MAXQ = 13;  // max derivative order of M2P/P2L (order <= 12)
+constexpr int TRI = (MAXQ + 1)*(MAXQ + 2)/2;
+
+__device__ __forceinline__ int full_index(const sumpyhip_expn &e, int i, int j, int k) {
+  const int p1 = e.order + 1;
+  if (i < 0 || j < 0 || k < 0 || i + j + k > e.order) return -1;
+  return e.cube_index[(i*p1 + j)*p1 + k];
+}
+
+__device__ __forceinline__ double inv_fact(int n) {
+  double f = 1;
+  for (int i = 2; i <= n; ++i) f *= i;
+  return 1.0/f;
+}
+
+// {{{ P2M
+
+template <bool DIP>
+__global__ __launch_bounds__(WAVE) void p2m_lap3d(
+    sumpyhip_expn e, int nboxes_list, const int *source_boxes, const int *bss,
+    const int *bsc, const double *centers, int nboxes, const double *sx,
+    const double *sy, const double *sz, const double *w0, const double *wx,
+    const double *wy, const double *wz, double rscale, double *out, int base) {
+  extern __shared__ double lds[];
+  const int P1 = e.order + 1;
+  double *pw = lds;                // [3][WAVE][P1]  (a/rho)^i/i!
+  double *sw = pw + 3*WAVE*P1;     // [4][WAVE]
+  double *cf = sw + 4*WAVE;        // [nfull]
+  const int lane = threadIdx.x;
+  constexpr int NPL = 16;          // nfull <= 16*64
+  int fid[NPL][3];
+  for (int q = 0; q < NPL; ++q) {
+    int f = lane + WAVE*q;
+    if (f < e.nfull) {
+      fid[q][0] = e.full_ids[f]; fid[q][1] = e.full_ids[e.nfull + f];
+      fid[q][2] = e.full_ids[2*e.nfull + f];
+    } else {
+      fid[q][0] = fid[q][1] = fid[q][2] = 0;
+    }
+  }
+  const double rinv = 1.0/rscale;
+  for (int ib = blockIdx.x; ib < nboxes_list; ib += gridDim.x) {
+    const int b = source_boxes[ib];
+    const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
+    const int s0 = bss[b], ns = bsc[b];
+    double acc[NPL];
+    for (int q = 0; q < NPL; ++q) acc[q] = 0;
+    for (int sc = 0; sc < ns; sc += WAVE) {
+      __syncthreads();
+      if (lane < ns - sc) {
+        const int s = s0 + sc + lane;
+        const double a[3] = {(c0 - sx[s])*rinv, (c1 - sy[s])*rinv, (c2 - sz[s])*rinv};
+        for (int d = 0; d < 3; ++d) {
+          double *row = pw + (d*WAVE + lane)*P1;
+          double v = 1;
+          row[0] = 1;
+          for (int i = 1; i < P1; ++i) { v *= a[d]/i; row[i] = v; }
+        }
+        sw[lane] = w0[s];
+        if (DIP) {
+          sw[WAVE + lane] = wx[s]*rinv; sw[2*WAVE + lane] = wy[s]*rinv;
+          sw[3*WAVE + lane] = wz[s]*rinv;
+        }
+      }
+      __syncthreads();
+      const int nn = min(WAVE, ns - sc);
+      for (int q = 0; q < NPL; ++q) {
+        if (lane + WAVE*q >= e.nfull) break;
+        const int i = fid[q][0], j = fid[q][1], k = fid[q][2];
+        double v = 0;
+        for (int s = 0; s < nn; ++s) {
+          const double *px = pw + s*P1, *py = pw + (WAVE + s)*P1, *pz = pw + (2*WAVE + s)*P1;
+          double m = px[i]*py[j]*pz[k];
+          v += sw[s]*m;
+          if (DIP) {
+            // d/da_ax of a^mi/mi! = a^(mi - e_ax)/(mi - e_ax)!
+            double dx = i ? px[i - 1]*py[j]*pz[k] : 0.0;
+            double dy = j ? px[i]*py[j - 1]*pz[k] : 0.0;
+            double dz = k ? px[i]*py[j]*pz[k - 1] : 0.0;
+            v += sw[WAVE + s]*dx + sw[2*WAVE + s]*dy + sw[3*WAVE + s]*dz;
+          }
+        }
+        acc[q] += v;
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q < NPL; ++q) {
+      int f = lane + WAVE*q;
+      if (f < e.nfull) cf[f] = acc[q];
+    }
+    __syncthreads();
+    double *o = out + (size_t)(b - base)*e.nstored;
+    for (int s = lane; s < e.nstored; s += WAVE) {
+      double v = 0;
+      for (int q = e.projT_col_starts[s]; q < e.projT_col_starts[s + 1]; ++q)
+        v += e.projT_vals[q]*cf[e.projT_rows[q]];
+      o[s] = v;
+    }
+  }
+}
+
+// }}}
+
+// {{{ M2M
+
+__global__ __launch_bounds__(WAVE) void m2m_lap3d(
+    sumpyhip_expn se, sumpyhip_expn te, int ntgt, const int *target_boxes,
+    const int *child_ids, const double *centers, int nboxes, double src_rscale,
+    double tgt_rscale, const double *src, int sbase, double *tgt, int tbase) {
+  extern __shared__ double lds[];
+  const int nf = te.nfull, P1 = te.order + 1;
+  double *b0 = lds, *b1 = b0 + nf, *acc = b1 + nf, *pw = acc + nf;  // pw[3][P1]
+  const int lane = threadIdx.x;
+  const double ratio = src_rscale/tgt_rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    for (int f = lane; f < nf; f += WAVE) acc[f] = 0;
+    for (int c = 0; c < 8; ++c) {
+      const int ch = child_ids[(size_t)c*nboxes + tb];
+      if (ch == 0) continue;
+      __syncthreads();
+      for (int f = lane; f < nf; f += WAVE) b0[f] = 0;
+      if (lane < 3) {
+        double d = (centers[(size_t)lane*nboxes + tb] - centers[(size_t)lane*nboxes + ch])/tgt_rscale;
+        double v = 1;
+        pw[lane*P1] = 1;
+        for (int i = 1; i < P1; ++i) { v *= d/i; pw[lane*P1 + i] = v; }
+      }
+      __syncthreads();
+      const double *m = src + (size_t)(ch - sbase)*se.nstored;
+      for (int s = lane; s < se.nstored; s += WAVE) {
+        int i = se.stored_ids[s], j = se.stored_ids[se.nstored + s],
+            k = se.stored_ids[2*se.nstored + s];
+        int f = full_index(te, i, j, k);
+        if (f >= 0) b0[f] = m[s]*pow(ratio, (double)(i + j + k));
+      }
+      // shift along y, z, x (sumpy/expansion/multipole.py:331-357)
+      const int axes[3] = {1, 2, 0};
+      double *in = b0, *outb = b1;
+      for (int ax : axes) {
+        __syncthreads();
+        for (int f = lane; f < nf; f += WAVE) {
+          int mi[3] = {te.full_ids[f], te.full_ids[nf + f], te.full_ids[2*nf + f]};
+          double v = 0;
+          const int n = mi[ax];
+          for (int k = 0; k <= n; ++k) {
+            int mk[3] = {mi[0], mi[1], mi[2]};
+            mk[ax] = k;
+            v += in[full_index(te, mk[0], mk[1], mk[2])]*pw[ax*P1 + n - k];
+          }
+          outb[f] = v;
+        }
+        double *t = in; in = outb; outb = t;
+      }
+      __syncthreads();
+      for (int f = lane; f < nf; f += WAVE) acc[f] += in[f];
+    }
+    __syncthreads();
+    double *o = tgt + (size_t)(tb - tbase)*te.nstored;
+    for (int s = lane; s < te.nstored; s += WAVE) {
+      double v = 0;
+      for (int q = te.projT_col_starts[s]; q < te.projT_col_starts[s + 1]; ++q)
+        v += te.projT_vals[q]*acc[te.projT_rows[q]];
+      o[s] += v;
+    }
+    __syncthreads();
+  }
+}
+
+// }}}
+
+// {{{ L2L
+
+__global__ __launch_bounds__(WAVE) void l2l_lap3d(
+    sumpyhip_expn se, sumpyhip_expn te, int ntgt, const int *target_boxes,
+    const int *parent_ids, const double *centers, int nboxes, double src_rscale,
+    double tgt_rscale, const double *src, int sbase, double *tgt, int tbase) {
+  extern __shared__ double lds[];
+  const int nf = se.nfull, P1 = se.order + 1;
+  double *b0 = lds, *b1 = b0 + nf, *pw = b1 + nf;
+  const int lane = threadIdx.x;
+  const double ratio = tgt_rscale/src_rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const int pb = parent_ids[tb];
+    __syncthreads();
+    if (lane < 3) {
+      double d = (centers[(size_t)lane*nboxes + tb] - centers[(size_t)lane*nboxes + pb])/src_rscale;
+      double v = 1;
+      pw[lane*P1] = 1;
+      for (int i = 1; i < P1; ++i) { v *= d/i; pw[lane*P1 + i] = v; }
+    }
+    const double *L = src + (size_t)(pb - sbase)*se.nstored;
+    for (int f = lane; f < nf; f += WAVE) {
+      double v = 0;
+      for (int q = se.proj_row_starts[f]; q < se.proj_row_starts[f + 1]; ++q)
+        v += se.proj_vals[q]*L[se.proj_cols[q]];
+      b0[f] = v;
+    }
+    // shift along x, y, z (sumpy/expansion/local.py:420-453)
+    double *in = b0, *outb = b1;
+    for (int ax = 0; ax < 3; ++ax) {
+      __syncthreads();
+      for (int f = lane; f < nf; f += WAVE) {
+        int mi[3] = {se.full_ids[f], se.full_ids[nf + f], se.full_ids[2*nf + f]};
+        const int deg = mi[0] + mi[1] + mi[2];
+        double v = 0;
+        for (int q = 0; q <= se.order - deg; ++q) {
+          int mk[3] = {mi[0], mi[1], mi[2]};
+          mk[ax] += q;
+          v += in[full_index(se, mk[0], mk[1], mk[2])]*pw[ax*P1 + q];
+        }
+        outb[f] = v;
+      }
+      double *t = in; in = outb; outb = t;
+    }
+    __syncthreads();
+    double *o = tgt + (size_t)(tb - tbase)*te.nstored;
+    for (int s = lane; s < te.nstored; s += WAVE) {
+      int i = te.stored_ids[s], j = te.stored_ids[te.nstored + s],
+          k = te.stored_ids[2*te.nstored + s];
+      int f = full_index(se, i, j, k);
+      if (f >= 0) o[s] += in[f]*pow(ratio, (double)(i + j + k));
+    }
+  }
+}
+
+// }}}
+
+// {{{ L2P
+
+template <bool GRAD>
+__global__ __launch_bounds__(WAVE) void l2p_lap3d(
+    sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
+    const int *btc, const double *centers, int nboxes, const double *tx,
+    const double *ty, const double *tz, double rscale, double scale,
+    const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
+  extern __shared__ double lh[];  // [nfull] in Horner order
+  const int lane = threadIdx.x;
+  const int p = e.order;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const double *L = src + (size_t)(tb - sbase)*e.nstored;
+    __syncthreads();
+    for (int pos = lane; pos < e.nfull; pos += WAVE) {
+      const int f = e.horner_order[pos];
+      double v = 0;
+      for (int q = e.proj_row_starts[f]; q < e.proj_row_starts[f + 1]; ++q)
+        v += e.proj_vals[q]*L[e.proj_cols[q]];
+      const int i = e.full_ids[f], j = e.full_ids[e.nfull + f], k = e.full_ids[2*e.nfull + f];
+      v *= inv_fact(i)*inv_fact(j)*inv_fact(k)*pow(rscale, -(double)(i + j + k));
+      lh[pos] = v;
+    }
+    __syncthreads();
+    const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
+    const int t0 = bts[tb], nt = btc[tb];
+    for (int tc = 0; tc < nt; tc += WAVE) {
+      const bool valid = tc + lane < nt;
+      const int t = t0 + tc + (valid ? lane : 0);
+      const double bx = tx[t] - c0, by = ty[t] - c1, bz = tz[t] - c2;
+      double P = 0, Px = 0, Py = 0, Pz = 0;
+      int pos = 0;
+      for (int i = p; i >= 0; --i) {
+        double S = 0, Sy = 0, Sz = 0;
+        for (int j = p - i; j >= 0; --j) {
+          double R = 0, Rz = 0;
+          for (int k = p - i - j; k >= 0; --k) {
+            if (GRAD) Rz = Rz*bz + R;
+            R = R*bz + lh[pos++];
+          }
+          if (GRAD) { Sy = Sy*by + S; Sz = Sz*by + Rz; }
+          S = S*by + R;
+        }
+        if (GRAD) { Px = Px*bx + P; Py = Py*bx + Sy; Pz = Pz*bx + Sz; }
+        P = P*bx + S;
+      }
+      if (valid) {
+        if (pot) pot[t] = scale*P;
+        if (GRAD) { gx[t] = scale*Px; gy[t] = scale*Py; gz[t] = scale*Pz; }
+      }
+    }
+  }
+}
+
+// }}}
+
+// {{{ scaled Laplace derivatives for M2P/P2L (m_x <= 1 stored, recurrence
+// of sumpy/derivative_taker.py:227-259)
+
+struct Derivs {
+  double v[2][TRI];  // [m_x][tri(m_y + m_z) + m_y]
+};
+
+__device__ __forceinline__ int tri(int s) { return s*(s + 1)/2; }
+
+__device__ __forceinline__ double dget(const Derivs &D, int mx, int my, int mz) {
+  if (mx < 0 || my < 0 || mz < 0) return 0.0;
+  return D.v[mx][tri(my + mz) + my];
+}
+
+// Fills D(m) = rho^|m| d^m (1/r) at x for m_x <= 1, |m| <= q (q <= MAXQ).
+__device__ void lap3d_derivs(double x, double y, double z, double rscale, int q, Derivs &D) {
+  const double xs[3] = {x/rscale, y/rscale, z/rscale};
+  const double r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
+  const double ri = rsqrt(r2s);
+  const double rinv = ri/rscale;  // 1/r = 1/(rho r~)
+  const double ri2 = ri*ri;
+  for (int n = 0; n <= q; ++n) {
+    for (int mx = 0; mx <= 1 && mx <= n; ++mx) {
+      for (int my = 0; my <= n - mx; ++my) {
+        const int mz = n - mx - my;
+        const int mi[3] = {mx, my, mz};
+        double val;
+        if (my <= 1 && mz <= 1) {
+          // direct derivatives of 1/r for m in {0,1}^3
+          if (n == 0) val = rinv;
+          else if (n == 1) val = -xs[mx ? 0 : (my ? 1 : 2)]*ri2*rinv;
+          else if (n == 2) {
+            int a = mx ? 0 : 1, b = mz ? 2 : 1;
+            val = 3.0*xs[a]*xs[b]*ri2*ri2*rinv;
+          } else {
+            val = -15.0*xs[0]*xs[1]*xs[2]*ri2*ri2*ri2*rinv;
+          }
+        } else {
+          const int d = my >= 2 ? 1 : 2;  // first axis with m_d >= 2 (m_x <= 1)
+          double acc = 0;
+          for (int i = 0; i < 3; ++i) {
+            const int nn = mi[i];
+            if (nn == 0) continue;
+            int m1[3] = {mi[0], mi[1], mi[2]}; m1[i] -= 1;
+            int m2[3] = {mi[0], mi[1], mi[2]}; m2[i] -= 2;
+            const double v1 = dget(D, m1[0], m1[1], m1[2]);
+            const double v2 = dget(D, m2[0], m2[1], m2[2]);
+            if (i == d) acc -= (2*nn - 1)*xs[i]*v1 + double((nn - 1)*(nn - 1))*v2;
+            else acc -= 2*nn*xs[i]*v1 + double(nn*(nn - 1))*v2;
+          }
+          val = acc*ri2;
+        }
+        D.v[mx][tri(my + mz) + my] = val;
+      }
+    }
+  }
+}
+
+// D(m + e_ax) for stored m (m_x <= 1): m_x = 2 follows from Laplace's
+// equation, D(2,y,z) = -D(0,y+2,z) - D(0,y,z+2).
+__device__ __forceinline__ double dshift(const Derivs &D, int mx, int my, int mz, int ax) {
+  if (ax == 0) {
+    if (mx == 0) return dget(D, 1, my, mz);
+    return -dget(D, 0, my + 2, mz) - dget(D, 0, my, mz + 2);
+  }
+  if (ax == 1) return dget(D, mx, my + 1, mz);
+  return dget(D, mx, my, mz + 1);
+}
+
+// }}}
+
+// {{{ M2P (list 3), accumulate
+
+template <bool GRAD>
+__global__ __launch_bounds__(WAVE) void m2p_lap3d(
+    sumpyhip_expn e, int ntgt, const int *target_boxes, const int *sstarts,
+    const int *slists, const int *bts, const int *btc, const double *centers,
+    int nboxes, const double *tx, const double *ty, const double *tz, double rscale,
+    double scale, const double *src, int sbase, double *pot, double *gx, double *gy,
+    double *gz) {
+  const int lane = threadIdx.x;
+  Derivs D;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const int t0 = bts[tb], nt = btc[tb];
+    for (int tc = 0; tc < nt; tc += WAVE) {
+      const bool valid = tc + lane < nt;
+      const int t = t0 + tc + (valid ? lane : 0);
+      double P = 0, G[3] = {0, 0, 0};
+      for (int is = sstarts[ib]; is < sstarts[ib + 1]; ++is) {
+        const int sb = slists[is];
+        const double bx = tx[t] - centers[sb], by = ty[t] - centers[nboxes + sb],
+                     bz = tz[t] - centers[2*(size_t)nboxes + sb];
+        lap3d_derivs(bx, by, bz, rscale, e.order + (GRAD ? 1 : 0), D);
+        const double *M = src + (size_t)(sb - sbase)*e.nstored;
+        for (int s = 0; s < e.nstored; ++s) {
+          const int i = e.stored_ids[s], j = e.stored_ids[e.nstored + s],
+                    k = e.stored_ids[2*e.nstored + s];
+          const double m = M[s];
+          P += m*dget(D, i, j, k);
+          if (GRAD)
+            for (int ax = 0; ax < 3; ++ax) G[ax] += m*dshift(D, i, j, k, ax);
+        }
+      }
+      if (valid) {
+        if (pot) pot[t] += scale*P;
+        if (GRAD) {
+          gx[t] += scale*G[0]/rscale; gy[t] += scale*G[1]/rscale; gz[t] += scale*G[2]/rscale;
+        }
+      }
+    }
+  }
+}
+
+// }}}
+
+// {{{ P2L (list 4), overwrite
+
+constexpr int P2L_MAXS = (MAXQ)*(MAXQ);  // (order+1)^2 for order <= 12
+
+template <bool DIP>
+__global__ __launch_bounds__(WAVE) void p2l_lap3d(
+    sumpyhip_expn e, int ntgt, const int *target_boxes, const int *sstarts,
+    const int *slists, const int *bss, const int *bsc, const double *centers,
+    int nboxes, const double *sx, const double *sy, const double *sz, const double *w0,
+    const double *wx, const double *wy, const double *wz, double rscale, double *out,
+    int base) {
+  __shared__ double red[32][WAVE];
+  const int lane = threadIdx.x;
+  Derivs D;
+  double acc[P2L_MAXS];
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
+    for (int s = 0; s < e.nstored; ++s) acc[s] = 0;
+    for (int is = sstarts[ib]; is < sstarts[ib + 1]; ++is) {
+      const int sb = slists[is];
+      const int s0 = bss[sb], ns = bsc[sb];
+      for (int sc = lane; sc < ns; sc += WAVE) {
+        const int s = s0 + sc;
+        lap3d_derivs(c0 - sx[s], c1 - sy[s], c2 - sz[s], rscale, e.order + (DIP ? 1 : 0), D);
+        const double q0 = w0[s];
+        const double qd[3] = {DIP ? wx[s]/rscale : 0.0, DIP ? wy[s]/rscale : 0.0,
+                              DIP ? wz[s]/rscale : 0.0};
+        for (int m = 0; m < e.nstored; ++m) {
+          const int i = e.stored_ids[m], j = e.stored_ids[e.nstored + m],
+                    k = e.stored_ids[2*e.nstored + m];
+          double v = q0*dget(D, i, j, k);
+          if (DIP) for (int ax = 0; ax < 3; ++ax) v += qd[ax]*dshift(D, i, j, k, ax);
+          acc[m] += v;
+        }
+      }
+    }
+    double *o = out + (size_t)(tb - base)*e.nstored;
+    for (int m0 = 0; m0 < e.nstored; m0 += 32) {
+      __syncthreads();
+      for (int m = 0; m < 32; ++m) red[m][lane] = (m0 + m < e.nstored) ? acc[m0 + m] : 0.0;
+      __syncthreads();
+      if (lane < 32 && m0 + lane < e.nstored) {
+        double v = 0;
+        for (int l = 0; l < WAVE; ++l) v += red[lane][l];
+        o[m0 + lane] = v;
+      }
+    }
+  }
+}
+
+// }}}
+
+int grid_for(int n) {
+  const int cap = 32*num_cus();
+  return n < cap ? n : cap;
+}
+
+int check_expn(const sumpyhip_expn *e, const char *what) {
+  SUMPY_REQUIRE(e != nullptr, "%s: null expansion tables", what);
+  SUMPY_REQUIRE(e->order >= 0 && e->order <= 24, "%s: order %d out of range", what, e->order);
+  SUMPY_REQUIRE(e->nfull > 0 && e->nstored > 0 && e->full_ids && e->stored_ids &&
+                    e->cube_index && e->proj_row_starts && e->projT_col_starts &&
+                    e->horner_order,
+                "%s: incomplete expansion tables", what);
+  return SUMPYHIP_OK;
+}
+
+}  // namespace
+}  // namespace sumpyhip
+
+using namespace sumpyhip;
+
+extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                            const int *source_boxes, const int *box_source_starts,
+                            const int *box_source_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *sx,
+                            const double *sy, const double *sz, const double *w0,
+                            const double *wx, const double *wy, const double *wz,
+                            double rscale, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "p2m: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(expn, "p2m");
+  if (rc) return rc;
+  if (nsrc_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(expn->nfull <= 16*64, "p2m: too many coefficients");
+  SUMPY_REQUIRE(!wx || (wy && wz), "p2m: incomplete dipole weights");
+  const size_t lds = sizeof(double)*(3*64*(expn->order + 1) + 4*64 + expn->nfull);
+  SUMPY_REQUIRE(lds <= 64*1024, "p2m: order too large");
+  hipStream_t st = (hipStream_t)stream;
+  if (wx)
+    p2m_lap3d<true><<<grid_for(nsrc_boxes), 64, lds, st>>>(
+        *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,
+        box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,
+        tgt_base_ibox);
+  else
+    p2m_lap3d<false><<<grid_for(nsrc_boxes), 64, lds, st>>>(
+        *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,
+        box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,
+        tgt_base_ibox);
+  return check_launch("p2m");
+}
+
+extern "C" int sumpyhip_m2m(int kernel_id, const sumpyhip_expn *src_expn,
+                            const sumpyhip_expn *tgt_expn, int ntgt_boxes,
+                            const int *target_boxes, const int *box_child_ids,
+                            const double *box_centers, int nboxes, double src_rscale,
+                            double tgt_rscale, const double *src_expansions,
+                            int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "m2m: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(src_expn, "m2m");
+  if (!rc) rc = check_expn(tgt_expn, "m2m");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  const size_t lds = sizeof(double)*(3*tgt_expn->nfull + 3*(tgt_expn->order + 1));
+  SUMPY_REQUIRE(lds <= 64*1024, "m2m: order too large");
+  m2m_lap3d<<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+      *src_expn, *tgt_expn, ntgt_boxes, target_boxes, box_child_ids, box_centers, nboxes,
+      src_rscale, tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox);
+  return check_launch("m2m");
+}
+
+extern "C" int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
+                            const sumpyhip_expn *tgt_expn, int ntgt_boxes,
+                            const int *target_boxes, const int *box_parent_ids,
+                            const double *box_centers, int nboxes, double src_rscale,
+                            double tgt_rscale, const double *src_expansions,
+                            int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "l2l: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(src_expn, "l2l");
+  if (!rc) rc = check_expn(tgt_expn, "l2l");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  const size_t lds = sizeof(double)*(2*src_expn->nfull + 3*(src_expn->order + 1));
+  SUMPY_REQUIRE(lds <= 64*1024, "l2l: order too large");
+  l2l_lap3d<<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+      *src_expn, *tgt_expn, ntgt_boxes, target_boxes, box_parent_ids, box_centers, nboxes,
+      src_rscale, tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox);
+  return check_launch("l2l");
+}
+
+extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *box_target_starts,
+                            const int *box_target_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *tx,
+                            const double *ty, const double *tz, double rscale,
+                            double scale, const double *src_expansions, int src_base_ibox,
+                            double *pot, double *gx, double *gy, double *gz,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "l2p: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(expn, "l2p");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(pot || gx, "l2p: no output");
+  SUMPY_REQUIRE(!gx || (gy && gz), "l2p: incomplete gradient outputs");
+  const size_t lds = sizeof(double)*expn->nfull;
+  SUMPY_REQUIRE(lds <= 64*1024, "l2p: order too large");
+  if (gx)
+    l2p_lap3d<true><<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
+        box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
+        gx, gy, gz);
+  else
+    l2p_lap3d<false><<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
+        box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
+        gx, gy, gz);
+  return check_launch("l2p");
+}
+
+extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *source_box_starts,
+                            const int *source_box_lists, const int *box_target_starts,
+                            const int *box_target_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *tx,
+                            const double *ty, const double *tz, double rscale,
+                            double scale, const double *src_expansions, int src_base_ibox,
+                            double *pot, double *gx, double *gy, double *gz,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "m2p: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(expn, "m2p");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(expn->order + 1 <= MAXQ, "m2p: order %d > %d", expn->order, MAXQ - 1);
+  SUMPY_REQUIRE(expn->nstored == (expn->order + 1)*(expn->order + 1),
+                "m2p: only PDE-compressed expansions are implemented");
+  SUMPY_REQUIRE(!gx || (gy && gz), "m2p: incomplete gradient outputs");
+  if (gx)
+    m2p_lap3d<true><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
+        rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);
+  else
+    m2p_lap3d<false><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
+        rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);
+  return check_launch("m2p");
+}
+
+extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *source_box_starts,
+                            const int *source_box_lists, const int *box_source_starts,
+                            const int *box_source_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *sx,
+                            const double *sy, const double *sz, const double *w0,
+                            const double *wx, const double *wy, const double *wz,
+                            double rscale, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "p2l: only the Laplace 3D kernel is implemented");
+  int rc = check_expn(expn, "p2l");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(expn->order + 1 <= MAXQ, "p2l: order %d > %d", expn->order, MAXQ - 1);
+  SUMPY_REQUIRE(expn->nstored == (expn->order + 1)*(expn->order + 1),
+                "p2l: only PDE-compressed expansions are implemented");
+  SUMPY_REQUIRE(!wx || (wy && wz), "p2l: incomplete dipole weights");
+  if (wx)
+    p2l_lap3d<true><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0,
+        wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);
+  else
+    p2l_lap3d<false><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
+        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0,
+        wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);
+  return check_launch("p2l");
+}

@@ -1,0 +1,389 @@
+// Multipole-to-local translation through translation classes.
+//
+// Reference pipeline (sumpy/fmm.py:723-920): M2LGenerateTranslationClassesDependentData
+// (e2e.py:565-669) -> M2LPreprocessMultipole (e2e.py:676-760) ->
+// M2LUsingTranslationClassesDependentData (e2e.py:310-558) -> M2LPostprocessLocal
+// (e2e.py:767-859); by default with FFTs over a circulant embedding
+// (m2l.py:913-1001).
+//
+// Here the class data keep the reference's non-FFT circulant layout
+// (m2l.py:403-537): data_c[off(m)] = rho^|m| d^m G(v_c), off() row-major in
+// the box [0, max_mi], so that the dense class matrix is
+//   A_c[t, s] = data_c[off(t) + off(s)]          (off is linear in m).
+// The translation of a level is a class-batched dense contraction on MFMA:
+// each block owns a tile of 64 target boxes (grouped on the host by their
+// octant within the parent, so that a tile shares its class set), and for every
+// class slot of the tile it stages data_c and the 64 source multipoles in LDS
+// and accumulates Y(nl x 64) += A_c (nl x nm) X (nm x 64) with
+// v_mfma_f64_16x16x4 (plain form) or v_mfma_f32_16x16x4 (FP32 form), A
+// fragments gathered straight from data_c in LDS.  Roofline: MFMA.
+#include "common.h"
+
+namespace sumpyhip {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// {{{ scaled derivatives of 1/r over a row-major box, degree by degree
+
+// D[(mx*M1 + my)*M2 + mz] = rho^|m| d^m (1/r)(v) for |m| <= N; zero outside.
+// Recurrence of sumpy/derivative_taker.py:227-259 (d = first axis with
+// m_d >= 2); max(m) <= 1 by direct differentiation.  All threads of the block
+// participate; ends with a barrier.
+__device__ void fill_derivs(double *D, int N, int M0, int M1, int M2, const double v[3],
+                            double rscale) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ncirc = M0*M1*M2;
+  for (int i = tid; i < ncirc; i += nt) D[i] = 0;
+  const double xs[3] = {v[0]/rscale, v[1]/rscale, v[2]/rscale};
+  const double r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
+  const double ri = rsqrt(r2s);
+  const double rinv = ri/rscale;
+  const double ri2 = ri*ri;
+  __syncthreads();
+  for (int n = 0; n <= N; ++n) {
+    for (int idx = tid; idx < M0*M1; idx += nt) {
+      const int mx = idx/M1, my = idx % M1, mz = n - mx - my;
+      if (mz < 0 || mz >= M2) continue;
+      const int mi[3] = {mx, my, mz};
+      double val;
+      if (mx <= 1 && my <= 1 && mz <= 1) {
+        if (n == 0) val = rinv;
+        else if (n == 1) val = -xs[mx ? 0 : (my ? 1 : 2)]*ri2*rinv;
+        else if (n == 2) {
+          int a = mx ? 0 : 1, b = mz ? 2 : 1;
+          val = 3.0*xs[a]*xs[b]*ri2*ri2*rinv;
+        } else {
+          val = -15.0*xs[0]*xs[1]*xs[2]*ri2*ri2*ri2*rinv;
+        }
+      } else {
+        const int d = mx >= 2 ? 0 : (my >= 2 ? 1 : 2);
+        double acc = 0;
+        for (int i = 0; i < 3; ++i) {
+          const int nn = mi[i];
+          if (nn == 0) continue;
+          int m1[3] = {mi[0], mi[1], mi[2]}; m1[i] -= 1;
+          const double v1 = D[(m1[0]*M1 + m1[1])*M2 + m1[2]];
+          double v2 = 0;
+          if (nn >= 2) {
+            int m2[3] = {mi[0], mi[1], mi[2]}; m2[i] -= 2;
+            v2 = D[(m2[0]*M1 + m2[1])*M2 + m2[2]];
+          }
+          if (i == d) acc -= (2*nn - 1)*xs[i]*v1 + double((nn - 1)*(nn - 1))*v2;
+          else acc -= 2*nn*xs[i]*v1 + double(nn*(nn - 1))*v2;
+        }
+        val = acc*ri2;
+      }
+      D[(mx*M1 + my)*M2 + mz] = val;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// }}}
+
+__global__ __launch_bounds__(256) void m2l_class_data_kernel(
+    int N, int M0, int M1, int M2, int nclasses, const double *vecs, int nvectors,
+    int class_start, double rscale, double *out) {
+  const size_t ncirc = (size_t)M0*M1*M2;
+  for (int c = blockIdx.x; c < nclasses; c += gridDim.x) {
+    const double v[3] = {vecs[class_start + c], vecs[nvectors + class_start + c],
+                         vecs[2*(size_t)nvectors + class_start + c]};
+    fill_derivs(out + c*ncirc, N, M0, M1, M2, v, rscale);
+  }
+}
+
+template <typename T>
+__global__ void m2l_preprocess_kernel(int nboxes, int nstored, const double *src, int kpad,
+                                      T *packed) {
+  const size_t n = (size_t)nboxes*kpad;
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    const size_t b = i/kpad;
+    const int k = i % kpad;
+    packed[i] = k < nstored ? T(src[b*nstored + k]) : T(0);
+  }
+}
+
+template <typename T>
+__global__ void m2l_postprocess_kernel(int nboxes, int nl, int npad, const T *work,
+                                       const int *deg, double ratio, double *out) {
+  const size_t n = (size_t)nboxes*nl;
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    const size_t b = i/nl;
+    const int k = i % nl;
+    out[i] = double(work[b*npad + k])*pow(ratio, (double)deg[k]);
+  }
+}
+
+// {{{ the MFMA contraction
+
+constexpr int TW = 64;       // targets per tile
+constexpr int MAXRBW = 4;    // row blocks of 16 per wave (npad <= 256)
+
+template <typename T>
+struct Mfma;
+// Operand layouts (checked on gfx950): A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k; accumulator register r of lane l holds column l % 16 and row
+// (l/16) + 4r for the FP64 instruction, 4(l/16) + r for the FP32 one.
+template <>
+struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4*r; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4*(lane >> 4) + r; }
+};
+
+struct ApplyArgs {
+  int ntiles;
+  const int *tile_targets, *tile_slot_starts, *slot_class, *slot_sources;
+  int nl, nm, npad, kpad;
+  const int *off_t, *off_s;
+  int ncirc;
+  const double *class_data;
+  const void *packed;
+  int sbase;
+  void *work;
+  int tbase;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  extern __shared__ unsigned char lds_raw[];
+  T *sD = reinterpret_cast<T *>(lds_raw);                  // [ncirc]
+  T *sX = sD + a.ncirc;                                    // [kpad][TW]
+  int *sOff = reinterpret_cast<int *>(sX + (size_t)a.kpad*TW);  // [kpad]
+  const T *packed = reinterpret_cast<const T *>(a.packed);
+  T *work = reinterpret_cast<T *>(a.work);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nrb = a.npad >> 4;
+  for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : -1;
+  int offT[MAXRBW];
+  for (int r = 0; r < MAXRBW; ++r) {
+    const int row = (wave + 4*r)*16 + (lane & 15);
+    offT[r] = (row < a.nl) ? a.off_t[row] : -1;
+  }
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    acc_t acc[MAXRBW][4];
+    for (int r = 0; r < MAXRBW; ++r)
+      for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
+    for (int slot = a.tile_slot_starts[tile]; slot < a.tile_slot_starts[tile + 1]; ++slot) {
+      const int cls = a.slot_class[slot];
+      const int *srcs = a.slot_sources + (size_t)slot*TW;
+      __syncthreads();
+      const double *D = a.class_data + (size_t)cls*a.ncirc;
+      for (int i = tid; i < a.ncirc; i += 256) sD[i] = T(D[i]);
+      for (int idx = tid; idx < a.kpad*TW; idx += 256) {
+        const int col = idx/a.kpad, k = idx - col*a.kpad;
+        const int sb = srcs[col];
+        sX[k*TW + col] = sb >= 0 ? packed[(size_t)(sb - a.sbase)*a.kpad + k] : T(0);
+      }
+      __syncthreads();
+      for (int k0 = 0; k0 < a.kpad; k0 += 4) {
+        const int kk = k0 + (lane >> 4);
+        const int os = sOff[kk];
+        T bfr[4];
+        for (int cb = 0; cb < 4; ++cb) bfr[cb] = sX[kk*TW + cb*16 + (lane & 15)];
+        for (int r = 0; r < MAXRBW; ++r) {
+          if (wave + 4*r >= nrb) break;
+          const T af = (offT[r] >= 0 && os >= 0) ? sD[offT[r] + os] : T(0);
+          for (int cb = 0; cb < 4; ++cb) acc[r][cb] = Mfma<T>::mma(af, bfr[cb], acc[r][cb]);
+        }
+      }
+    }
+    // epilogue: each lane holds four rows of one column of every block
+    for (int r = 0; r < MAXRBW; ++r) {
+      const int rb = wave + 4*r;
+      if (rb >= nrb) break;
+      for (int cb = 0; cb < 4; ++cb) {
+        const int col = cb*16 + (lane & 15);
+        const int tb = a.tile_targets[(size_t)tile*TW + col];
+        if (tb < 0) continue;
+        T *dst = work + (size_t)(tb - a.tbase)*a.npad;
+        for (int i = 0; i < 4; ++i) {
+          const int row = rb*16 + Mfma<T>::row(lane, i);
+          if (row < a.npad) dst[row] = acc[r][cb][i];
+        }
+      }
+    }
+  }
+}
+
+// }}}
+
+// {{{ M2L without translation classes: derivatives per pair, LDS matvec
+
+__global__ __launch_bounds__(256) void m2l_csr_kernel(
+    int N, int M0, int M1, int M2, int nl, int nm, const int *off_t, const int *off_s,
+    const int *deg, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const double *centers, int nboxes, double src_rscale, double tgt_rscale,
+    const double *src, int sbase, double *tgt, int tbase) {
+  extern __shared__ double lds[];
+  const int ncirc = M0*M1*M2;
+  double *D = lds, *Ms = D + ncirc;
+  const int tid = threadIdx.x;
+  const double ratio = tgt_rscale/src_rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    double acc[4] = {0, 0, 0, 0};
+    for (int is = starts[ib]; is < starts[ib + 1]; ++is) {
+      const int sb = lists[is];
+      __syncthreads();
+      for (int k = tid; k < nm; k += 256) Ms[k] = src[(size_t)(sb - sbase)*nm + k];
+      const double v[3] = {centers[tb] - centers[sb], centers[nboxes + tb] - centers[nboxes + sb],
+                           centers[2*(size_t)nboxes + tb] - centers[2*(size_t)nboxes + sb]};
+      fill_derivs(D, N, M0, M1, M2, v, src_rscale);
+      for (int q = 0; q < 4; ++q) {
+        const int row = tid + 256*q;
+        if (row >= nl) break;
+        const int ot = off_t[row];
+        double s = 0;
+        for (int k = 0; k < nm; ++k) s += Ms[k]*D[ot + off_s[k]];
+        acc[q] += s;
+      }
+    }
+    for (int q = 0; q < 4; ++q) {
+      const int row = tid + 256*q;
+      if (row >= nl) break;
+      tgt[(size_t)(tb - tbase)*nl + row] = acc[q]*pow(ratio, (double)deg[row]);
+    }
+  }
+}
+
+// }}}
+
+template <typename T>
+int set_lds_limit(const void *fn, size_t bytes) {
+  if (bytes <= 64*1024) return SUMPYHIP_OK;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess)
+    return set_error(SUMPYHIP_EHIP, "m2l: cannot raise LDS limit to %zu: %s", bytes,
+                     hipGetErrorString(e));
+  return SUMPYHIP_OK;
+}
+
+}  // namespace
+}  // namespace sumpyhip
+
+using namespace sumpyhip;
+
+extern "C" int sumpyhip_m2l_class_data(int kernel_id, int order_src, int order_tgt,
+                                       const int *circ_shape, int nclasses,
+                                       const double *translation_vectors, int nvectors,
+                                       int class_start, double src_rscale, double *out,
+                                       void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "m2l_class_data: only the Laplace 3D kernel is implemented");
+  SUMPY_REQUIRE(circ_shape != nullptr, "m2l_class_data: null shape");
+  if (nclasses == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(class_start >= 0 && class_start + nclasses <= nvectors,
+                "m2l_class_data: class range outside the vectors");
+  const int grid = nclasses < 1024 ? nclasses : 1024;
+  m2l_class_data_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(
+      order_src + order_tgt, circ_shape[0], circ_shape[1], circ_shape[2], nclasses,
+      translation_vectors, nvectors, class_start, src_rscale, out);
+  return check_launch("m2l_class_data");
+}
+
+extern "C" int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
+                                       const double *src_expansions, int kpad, void *packed,
+                                       void *stream) {
+  SUMPY_REQUIRE(kpad >= nstored, "m2l_preprocess: kpad < nstored");
+  if (nboxes == 0) return SUMPYHIP_OK;
+  const size_t n = (size_t)nboxes*kpad;
+  const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+  if (precision == 0)
+    m2l_preprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
+        nboxes, nstored, src_expansions, kpad, (double *)packed);
+  else
+    m2l_preprocess_kernel<float><<<grid, 256, 0, (hipStream_t)stream>>>(
+        nboxes, nstored, src_expansions, kpad, (float *)packed);
+  return check_launch("m2l_preprocess");
+}
+
+extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int npad,
+                                        const void *work, const int *tgt_degree,
+                                        double ratio, double *tgt_expansions,
+                                        void *stream) {
+  SUMPY_REQUIRE(npad >= nl, "m2l_postprocess: npad < nl");
+  if (nboxes == 0) return SUMPYHIP_OK;
+  const size_t n = (size_t)nboxes*nl;
+  const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+  if (precision == 0)
+    m2l_postprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
+        nboxes, nl, npad, (const double *)work, tgt_degree, ratio, tgt_expansions);
+  else
+    m2l_postprocess_kernel<float><<<grid, 256, 0, (hipStream_t)stream>>>(
+        nboxes, nl, npad, (const float *)work, tgt_degree, ratio, tgt_expansions);
+  return check_launch("m2l_postprocess");
+}
+
+extern "C" int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_targets,
+                                  const int *tile_slot_starts, const int *slot_class,
+                                  const int *slot_sources, int nl, int nm, int npad, int kpad,
+                                  const int *off_t, const int *off_s, int ncirc,
+                                  const double *class_data, const void *packed_src,
+                                  int src_base_ibox, void *work, int tgt_base_ibox,
+                                  void *stream) {
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "m2l_apply: unknown precision %d", precision);
+  SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl && npad <= 16*4*MAXRBW,
+                "m2l_apply: npad %d must be a multiple of 16 in [nl, %d]", npad, 64*MAXRBW);
+  SUMPY_REQUIRE(kpad % 4 == 0 && kpad >= nm, "m2l_apply: kpad %d must be a multiple of 4 >= nm",
+                kpad);
+  if (ntiles == 0) return SUMPYHIP_OK;
+  ApplyArgs a{ntiles, tile_targets, tile_slot_starts, slot_class, slot_sources, nl, nm,
+              npad, kpad, off_t, off_s, ncirc, class_data, packed_src, src_base_ibox,
+              work, tgt_base_ibox};
+  const size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
+  const size_t lds = esz*((size_t)ncirc + (size_t)kpad*TW) + sizeof(int)*kpad;
+  SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
+  const int grid = ntiles;
+  int rc;
+  if (precision == 0) {
+    rc = set_lds_limit<double>((const void *)m2l_apply_kernel<double>, lds);
+    if (rc) return rc;
+    m2l_apply_kernel<double><<<grid, 256, lds, (hipStream_t)stream>>>(a);
+  } else {
+    rc = set_lds_limit<float>((const void *)m2l_apply_kernel<float>, lds);
+    if (rc) return rc;
+    m2l_apply_kernel<float><<<grid, 256, lds, (hipStream_t)stream>>>(a);
+  }
+  return check_launch("m2l_apply");
+}
+
+extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
+                                const int *circ_shape, int nl, int nm, const int *off_t,
+                                const int *off_s, const int *tgt_degree, int ntgt_boxes,
+                                const int *target_boxes, const int *src_box_starts,
+                                const int *src_box_lists, const double *box_centers,
+                                int nboxes, double src_rscale, double tgt_rscale,
+                                const double *src_expansions, int src_base_ibox,
+                                double *tgt_expansions, int tgt_base_ibox, void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "m2l_csr: only the Laplace 3D kernel is implemented");
+  SUMPY_REQUIRE(nl <= 1024, "m2l_csr: too many target coefficients");
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  const size_t ncirc = (size_t)circ_shape[0]*circ_shape[1]*circ_shape[2];
+  const size_t lds = sizeof(double)*(ncirc + nm);
+  SUMPY_REQUIRE(lds <= 160*1024, "m2l_csr: %zu bytes of LDS needed (order too high)", lds);
+  int rc = set_lds_limit<double>((const void *)m2l_csr_kernel, lds);
+  if (rc) return rc;
+  const int grid = ntgt_boxes < 8192 ? ntgt_boxes : 8192;
+  m2l_csr_kernel<<<grid, 256, lds, (hipStream_t)stream>>>(
+      order_src + order_tgt, circ_shape[0], circ_shape[1], circ_shape[2], nl, nm, off_t,
+      off_s, tgt_degree, ntgt_boxes, target_boxes, src_box_starts, src_box_lists,
+      box_centers, nboxes, src_rscale, tgt_rscale, src_expansions, src_base_ibox,
+      tgt_expansions, tgt_base_ibox);
+  return check_launch("m2l_csr");
+}

@@ -1,0 +1,237 @@
+// Error reporting, device queries, source-weight and target-output
+// combination kernels, and the 2D QBX layer potential.
+#include "common.h"
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace sumpyhip {
+
+static thread_local char g_err[1024];
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int num_cus() {
+  static int cached = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!cached) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cached = prop.multiProcessorCount;
+    else
+      cached = 256;
+  }
+  return cached;
+}
+
+namespace {
+
+// {{{ weights and outputs
+
+__global__ void axpy_weights_kernel(int is_complex, int n, double coeff, const double *q,
+                                    const double *dir, double *w) {
+  for (int i = blockIdx.x*blockDim.x + threadIdx.x; i < n; i += gridDim.x*blockDim.x) {
+    const double f = coeff*(dir ? dir[i] : 1.0);
+    if (is_complex) {
+      w[2*i] += f*q[2*i];
+      w[2*i + 1] += f*q[2*i + 1];
+    } else {
+      w[i] += f*q[i];
+    }
+  }
+}
+
+constexpr int MAXTERMS = 8;
+struct CombineArgs {
+  int is_complex, n, nterms, accumulate;
+  const double *ins[MAXTERMS];
+  double coeffs[MAXTERMS];
+  const double *mults[MAXTERMS];
+  double *out;
+};
+
+__global__ void combine_kernel(CombineArgs a) {
+  for (int i = blockIdx.x*blockDim.x + threadIdx.x; i < a.n; i += gridDim.x*blockDim.x) {
+    double re = 0, im = 0;
+    for (int j = 0; j < a.nterms; ++j) {
+      const double f = a.coeffs[j]*(a.mults[j] ? a.mults[j][i] : 1.0);
+      if (a.is_complex) {
+        re += f*a.ins[j][2*i];
+        im += f*a.ins[j][2*i + 1];
+      } else {
+        re += f*a.ins[j][i];
+      }
+    }
+    if (a.is_complex) {
+      if (a.accumulate) { a.out[2*i] += re; a.out[2*i + 1] += im; }
+      else { a.out[2*i] = re; a.out[2*i + 1] = im; }
+    } else {
+      if (a.accumulate) a.out[i] += re;
+      else a.out[i] = re;
+    }
+  }
+}
+
+// }}}
+
+// {{{ QBX: 2D Laplace line-Taylor, complex-analytic closed forms
+
+constexpr int QB = 256;
+
+__global__ __launch_bounds__(QB) void qbx2d_kernel(int order, int src_kind, int tgt_axis,
+                                                   int nt, const double *tx, const double *ty,
+                                                   const double *cx, const double *cy, int ns,
+                                                   const double *sx, const double *sy,
+                                                   const double *nx, const double *ny,
+                                                   const double *dens, double scale,
+                                                   double *out) {
+  __shared__ double s_x[QB], s_y[QB], s_nx[QB], s_ny[QB], s_d[QB];
+  const int tid = threadIdx.x;
+  const int t = min((int)(blockIdx.x*QB + tid), nt - 1);
+  const bool valid = blockIdx.x*QB + tid < nt;
+  const double ccx = cx[t], ccy = cy[t];
+  const double bre = tx[t] - ccx, bim = ty[t] - ccy;  // b = t - c
+  double res = 0;
+  for (int sc = 0; sc < ns; sc += QB) {
+    __syncthreads();
+    if (sc + tid < ns) {
+      const int s = sc + tid;
+      s_x[tid] = sx[s]; s_y[tid] = sy[s]; s_d[tid] = dens[s];
+      if (src_kind == 1) { s_nx[tid] = nx[s]; s_ny[tid] = ny[s]; }
+    }
+    __syncthreads();
+    const int nn = min(QB, ns - sc);
+    for (int j = 0; j < nn; ++j) {
+      // a = c - s; z = b/a
+      const double are = ccx - s_x[j], aim = ccy - s_y[j];
+      const double aa = are*are + aim*aim;
+      const double ia_re = are/aa, ia_im = -aim/aa;  // 1/a
+      const double zre = bre*ia_re - bim*ia_im, zim = bre*ia_im + bim*ia_re;
+      double val;
+      if (src_kind == 0 && tgt_axis < 0) {
+        // log a + sum_{i=1}^p (-1)^{i-1} z^i / i   (real part)
+        double hre = 0, him = 0;
+        for (int i = order; i >= 1; --i) {
+          const double c = ((i & 1) ? 1.0 : -1.0)/i;
+          const double nre = hre*zre - him*zim + c, nim = hre*zim + him*zre;
+          hre = nre; him = nim;
+        }
+        // times z
+        const double sre = hre*zre - him*zim;
+        val = 0.5*log(aa) + sre;
+      } else {
+        // value = Re[ w * sum_{i=0}^p c_i (-z)^i ]
+        //   SLP + d/da: w = 1/a, c_i = 1 (times e_axis factor)
+        //   DLP/axis source deriv: w = -n/a, c_i = 1;  with d/da: w = n/a^2, c_i = i+1
+        double wre, wim;
+        int lin = 0;
+        if (src_kind == 0) {
+          wre = ia_re; wim = ia_im;
+        } else {
+          double nre, nim;
+          if (src_kind == 1) { nre = s_nx[j]; nim = s_ny[j]; }
+          else if (src_kind == 2) { nre = 1; nim = 0; }
+          else { nre = 0; nim = 1; }
+          if (tgt_axis < 0) {
+            wre = -(nre*ia_re - nim*ia_im); wim = -(nre*ia_im + nim*ia_re);
+          } else {
+            const double a2re = ia_re*ia_re - ia_im*ia_im, a2im = 2*ia_re*ia_im;
+            wre = nre*a2re - nim*a2im; wim = nre*a2im + nim*a2re;
+            lin = 1;
+          }
+        }
+        double hre = 0, him = 0;
+        const double mzre = -zre, mzim = -zim;
+        for (int i = order; i >= 0; --i) {
+          const double c = lin ? double(i + 1) : 1.0;
+          const double nre = hre*mzre - him*mzim + c, nim = hre*mzim + him*mzre;
+          hre = nre; him = nim;
+        }
+        double pre = wre*hre - wim*him, pim = wre*him + wim*hre;
+        if (tgt_axis == 1) { const double q = pre; pre = -pim; pim = q; }  // times i
+        val = pre;
+      }
+      res += s_d[j]*val;
+    }
+  }
+  if (valid) out[t] = scale*res;
+}
+
+// }}}
+
+}  // namespace
+}  // namespace sumpyhip
+
+using namespace sumpyhip;
+
+extern "C" const char *sumpyhip_last_error(void) { return g_err; }
+
+extern "C" int sumpyhip_device_info(int *n_cu, int *major, int *minor) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return set_error(SUMPYHIP_EHIP, "device query: %s", hipGetErrorString(e));
+  if (n_cu) *n_cu = prop.multiProcessorCount;
+  if (major) *major = prop.major;
+  if (minor) *minor = prop.minor;
+  return SUMPYHIP_OK;
+}
+
+extern "C" int sumpyhip_axpy_weights(int is_complex, int n, double coeff,
+                                     const double *strength, const double *dir, double *w,
+                                     void *stream) {
+  if (n == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(strength && w, "axpy_weights: null pointer");
+  const int grid = (n + 255)/256 < 4096 ? (n + 255)/256 : 4096;
+  axpy_weights_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(is_complex, n, coeff, strength,
+                                                               dir, w);
+  return check_launch("axpy_weights");
+}
+
+extern "C" int sumpyhip_combine_outputs(int is_complex, int n, int nterms,
+                                        const double *const *ins, const double *coeffs,
+                                        const double *const *mults, double *out,
+                                        int accumulate, void *stream) {
+  SUMPY_REQUIRE(nterms >= 1 && nterms <= MAXTERMS, "combine_outputs: 1..%d terms", MAXTERMS);
+  if (n == 0) return SUMPYHIP_OK;
+  CombineArgs a{};
+  a.is_complex = is_complex; a.n = n; a.nterms = nterms; a.accumulate = accumulate;
+  a.out = out;
+  for (int j = 0; j < nterms; ++j) {
+    a.ins[j] = ins[j];
+    a.coeffs[j] = coeffs[j];
+    a.mults[j] = mults ? mults[j] : nullptr;
+  }
+  const int grid = (n + 255)/256 < 4096 ? (n + 255)/256 : 4096;
+  combine_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(a);
+  return check_launch("combine_outputs");
+}
+
+extern "C" int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntargets,
+                              const double *tx, const double *ty, const double *cx,
+                              const double *cy, int nsources, const double *sx,
+                              const double *sy, const double *nx, const double *ny,
+                              const double *density, double scale, double *out,
+                              void *stream) {
+  SUMPY_REQUIRE(order >= 0 && order <= 64, "qbx2d: order out of range");
+  SUMPY_REQUIRE(src_kind >= 0 && src_kind <= 3, "qbx2d: unknown source kind %d", src_kind);
+  SUMPY_REQUIRE(tgt_axis >= -1 && tgt_axis <= 1, "qbx2d: target axis must be -1, 0 or 1");
+  if (src_kind == 1) SUMPY_REQUIRE(nx && ny, "qbx2d: DLP needs source normals");
+  if (ntargets == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(nsources >= 1, "qbx2d: nsources must be >= 1");
+  const int grid = (ntargets + QB - 1)/QB;
+  qbx2d_kernel<<<grid, QB, 0, (hipStream_t)stream>>>(order, src_kind, tgt_axis, ntargets, tx,
+                                                     ty, cx, cy, nsources, sx, sy, nx, ny,
+                                                     density, scale, out);
+  return check_launch("qbx2d");
+}

@@ -1,0 +1,125 @@
+"""``drive_fmm``: the FMM pass over a wrangler.
+
+Same call contract and stage order as ``boxtree.fmm.drive_fmm`` (the caller
+of ``SumpyExpansionWrangler`` in ``sumpy/test/test_fmm.py:280-282``; the stage
+list is that of the wrangler interface, ``sumpy/fmm.py:543-1086``).  boxtree
+is not part of this build.  Level starts are handed to the wrangler from the
+traversal's host copies, so a pass enqueues all its kernels without a
+device-to-host synchronisation.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+class _PhaseTimer:
+    """Records HIP events around each stage on the current stream."""
+
+    def __init__(self, enabled):
+        self.enabled = enabled
+        self.events = []
+
+    def mark(self, name):
+        if not self.enabled:
+            return
+        import torch
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.events.append((name, ev))
+
+    def result(self):
+        import torch
+        torch.cuda.synchronize()
+        out = {}
+        for (n0, e0), (_, e1) in zip(self.events[:-1], self.events[1:]):
+            out[n0] = out.get(n0, 0.0) + e0.elapsed_time(e1)
+        return out
+
+
+def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
+    """Evaluate the potentials of ``src_weight_vecs`` through ``wrangler``.
+
+    Returns the potentials in user target order.  If *timing_data* is a dict,
+    it receives per-stage GPU times in milliseconds.
+    """
+    trav = wrangler.traversal
+    h = trav.host if hasattr(trav, "host") else {}
+
+    def host(name):
+        return h[name] if name in h else getattr(trav, name)
+
+    timer = _PhaseTimer(timing_data is not None)
+
+    timer.mark("reorder_sources")
+    src_weight_vecs = [wrangler.reorder_sources(w) for w in src_weight_vecs]
+    src_weight_vecs = wrangler.distribute_source_weights(actx, src_weight_vecs, None)
+
+    timer.mark("form_multipoles")
+    mpole_exps = wrangler.form_multipoles(
+        actx, host("level_start_source_box_nrs"), trav.source_boxes, src_weight_vecs)
+
+    timer.mark("coarsen_multipoles")
+    mpole_exps = wrangler.coarsen_multipoles(
+        actx, host("level_start_source_parent_box_nrs"), trav.source_parent_boxes,
+        mpole_exps)
+    wrangler.communicate_mpoles(actx, mpole_exps)
+
+    timer.mark("eval_direct")
+    potentials = wrangler.eval_direct(
+        actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
+        trav.neighbor_source_boxes_lists, src_weight_vecs)
+
+    timer.mark("multipole_to_local")
+    local_exps = wrangler.multipole_to_local(
+        actx, host("level_start_target_or_target_parent_box_nrs"),
+        trav.target_or_target_parent_boxes, trav.from_sep_siblings_starts,
+        trav.from_sep_siblings_lists, mpole_exps)
+
+    timer.mark("eval_multipoles")
+    mpole_result = wrangler.eval_multipoles(
+        actx, trav.target_boxes_sep_smaller_by_source_level,
+        trav.from_sep_smaller_by_level, mpole_exps)
+    potentials = _add(potentials, mpole_result)
+
+    if getattr(trav, "from_sep_close_smaller_starts", None) is not None:
+        potentials = _add(potentials, wrangler.eval_direct(
+            actx, trav.target_boxes, trav.from_sep_close_smaller_starts,
+            trav.from_sep_close_smaller_lists, src_weight_vecs))
+
+    timer.mark("form_locals")
+    local_result = wrangler.form_locals(
+        actx, host("level_start_target_or_target_parent_box_nrs"),
+        trav.target_or_target_parent_boxes, trav.from_sep_bigger_starts,
+        trav.from_sep_bigger_lists, src_weight_vecs)
+    local_exps = local_exps + local_result
+
+    if getattr(trav, "from_sep_close_bigger_starts", None) is not None:
+        potentials = _add(potentials, wrangler.eval_direct(
+            actx, trav.target_boxes, trav.from_sep_close_bigger_starts,
+            trav.from_sep_close_bigger_lists, src_weight_vecs))
+
+    timer.mark("refine_locals")
+    local_exps = wrangler.refine_locals(
+        actx, host("level_start_target_or_target_parent_box_nrs"),
+        trav.target_or_target_parent_boxes, local_exps)
+
+    timer.mark("eval_locals")
+    local_result = wrangler.eval_locals(
+        actx, host("level_start_target_box_nrs"), trav.target_boxes, local_exps)
+    potentials = _add(potentials, local_result)
+
+    timer.mark("reorder_potentials")
+    result = wrangler.reorder_potentials(potentials)
+    result = wrangler.finalize_potentials(actx, result)
+    timer.mark("end")
+    if timing_data is not None:
+        timing_data.update(timer.result())
+    return result
+
+
+def _add(a, b):
+    out = np.empty(len(a), dtype=object)
+    for i in range(len(a)):
+        out[i] = a[i] + b[i]
+    return out

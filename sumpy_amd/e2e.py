@@ -1,0 +1,286 @@
+"""Expansion-to-expansion operators (reference ``sumpy/e2e.py``).
+
+* :class:`E2EFromChildren` -- M2M (``e2e.py:866-978``), accumulates;
+* :class:`E2EFromParent` -- L2L (``e2e.py:985-1083``), accumulates;
+* :class:`E2EFromCSR` -- M2L without translation classes (``e2e.py:164-304``),
+  overwrites;
+* the translation-class M2L pipeline: :class:`M2LGenerateTranslationClassesDependentData`
+  (``e2e.py:565-669``), :class:`M2LPreprocessMultipole` (``:676-760``),
+  :class:`M2LUsingTranslationClassesDependentData` (``:310-558``) and
+  :class:`M2LPostprocessLocal` (``:767-859``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from sumpy_amd import _capi
+from sumpy_amd.expansion import m2l_device_tables
+from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
+from sumpy_amd.profiling import kernel_span
+from sumpy_amd.tools import as_i32
+
+TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
+
+
+class E2EBase:
+    def __init__(self, src_expansion, tgt_expansion, name=None):
+        strip = lambda e: e.with_kernel(  # noqa: E731
+            remove_source_transforms(remove_target_transforms(e.kernel)))
+        self.src_expansion = strip(src_expansion)
+        self.tgt_expansion = strip(tgt_expansion)
+        if self.src_expansion.dim != self.tgt_expansion.dim:
+            raise ValueError("source and target expansions must have same dimensionality")
+        self.name = name or self.default_name
+
+    @property
+    def dim(self):
+        return self.tgt_expansion.dim
+
+    def get_cache_key(self):
+        return (type(self).__name__, self.src_expansion, self.tgt_expansion)
+
+    def _tables(self, actx):
+        return (self.src_expansion.device_tables(actx.device),
+                self.tgt_expansion.device_tables(actx.device))
+
+
+class E2EFromChildren(E2EBase):
+    """M2M: ``tgt[parent] += sum_children T(c_parent - c_child) src[child]``."""
+
+    default_name = "e2e_from_children"
+
+    def __call__(self, actx, **kwargs):
+        st, tt = self._tables(actx)
+        centers = kwargs["centers"].contiguous()
+        tb = as_i32(kwargs["target_boxes"])
+        child = as_i32(kwargs["box_child_ids"])
+        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
+        _capi.call("sumpyhip_m2m", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(child), _capi.ptr(centers),
+                   int(centers.shape[1]), float(kwargs["src_rscale"]),
+                   float(kwargs["tgt_rscale"]), _capi.ptr(src), int(kwargs["src_base_ibox"]),
+                   _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        return tgt
+
+
+class E2EFromParent(E2EBase):
+    """L2L: ``tgt[child] += T(c_child - c_parent) src[parent]``."""
+
+    default_name = "e2e_from_parent"
+
+    def __call__(self, actx, **kwargs):
+        st, tt = self._tables(actx)
+        centers = kwargs["centers"].contiguous()
+        tb = as_i32(kwargs["target_boxes"])
+        parents = as_i32(kwargs["box_parent_ids"])
+        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
+        _capi.call("sumpyhip_l2l", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(parents), _capi.ptr(centers),
+                   int(centers.shape[1]), float(kwargs["src_rscale"]),
+                   float(kwargs["tgt_rscale"]), _capi.ptr(src), int(kwargs["src_base_ibox"]),
+                   _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        return tgt
+
+
+def _m2l_tabs(src_expansion, tgt_expansion, device):
+    return m2l_device_tables(src_expansion.order, tgt_expansion.order,
+                             src_expansion.compressed, str(device))
+
+
+class E2EFromCSR(E2EBase):
+    """M2L without translation classes: kernel derivatives per (target,
+    source) pair, overwrites (reference ``e2e.py:164-304``)."""
+
+    default_name = "e2e_from_csr"
+
+    def __call__(self, actx, **kwargs):
+        from sumpy_amd.expansion.local import LocalExpansionBase
+        from sumpy_amd.expansion.multipole import MultipoleExpansionBase
+        if not (isinstance(self.src_expansion, MultipoleExpansionBase)
+                and isinstance(self.tgt_expansion, LocalExpansionBase)):
+            raise NotImplementedError("E2EFromCSR implements multipole-to-local only")
+        m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+        centers = kwargs["centers"].contiguous()
+        tb = as_i32(kwargs["target_boxes"])
+        starts = as_i32(kwargs["src_box_starts"])
+        lists = as_i32(kwargs["src_box_lists"])
+        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
+        shape = _capi.np_i32(m["shape"])
+        _capi.call("sumpyhip_m2l_csr", kernel_id(self.tgt_expansion.kernel),
+                   self.src_expansion.order, self.tgt_expansion.order,
+                   shape.ctypes.data, m["nl"], m["nm"],
+                   _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), _capi.ptr(m["tgt_degree"]),
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                   _capi.ptr(centers), int(centers.shape[1]), float(kwargs["src_rscale"]),
+                   float(kwargs["tgt_rscale"]), _capi.ptr(src), int(kwargs["src_base_ibox"]),
+                   _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        return tgt
+
+
+class M2LGenerateTranslationClassesDependentData(E2EBase):
+    """Per translation class ``c``: ``data[c, off(m)] = rho^|m| d^m G(v_c)`` in
+    the circulant layout of ``m2l.py:403-537``."""
+
+    default_name = "m2l_generate_translation_classes_dependent_data"
+
+    def __call__(self, actx, **kwargs):
+        vecs = kwargs["m2l_translation_vectors"].contiguous()
+        out = kwargs["m2l_translation_classes_dependent_data"]
+        m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+        ncls = int(kwargs["ntranslation_classes"])
+        if out.shape != (ncls, m["ncirc"]):
+            raise ValueError(f"class data must have shape ({ncls}, {m['ncirc']})")
+        shape = _capi.np_i32(m["shape"])
+        _capi.call("sumpyhip_m2l_class_data", kernel_id(self.src_expansion.kernel),
+                   self.src_expansion.order, self.tgt_expansion.order, shape.ctypes.data,
+                   ncls, _capi.ptr(vecs), int(vecs.shape[1]),
+                   int(kwargs["translation_classes_level_start"]),
+                   float(kwargs["src_rscale"]), _capi.ptr(out), actx.stream_handle)
+        return out
+
+
+class M2LPreprocessMultipole(E2EBase):
+    """Pack the stored multipoles of a level into the M2L compute precision
+    (zero padded to the MFMA K granule)."""
+
+    default_name = "m2l_preprocess_multipole"
+
+    def __call__(self, actx, **kwargs):
+        src = kwargs["src_expansions"]
+        out = kwargs["preprocessed_src_expansions"]
+        tr = self.tgt_expansion.m2l_translation
+        nb = int(src.shape[0])
+        if out.shape[0] != nb:
+            raise ValueError("preprocessed array has the wrong number of boxes")
+        _capi.call("sumpyhip_m2l_preprocess", tr.precision_id, nb, int(src.shape[1]),
+                   _capi.ptr(src), int(out.shape[1]), _capi.ptr(out), actx.stream_handle)
+        return out
+
+
+class M2LUsingTranslationClassesDependentData(E2EBase):
+    """Class-batched dense M2L on MFMA; overwrites the work rows of the
+    target boxes (reference ``e2e.py:310-558``)."""
+
+    default_name = "m2l_using_translation_classes_dependent_data"
+
+    def __init__(self, src_expansion, tgt_expansion, name=None):
+        super().__init__(src_expansion, tgt_expansion, name)
+        self._schedules = {}
+
+    def schedule(self, actx, target_boxes, src_box_starts, src_box_lists, class_lists,
+                 class_level_start, nclasses, centers):
+        """Tile schedule of one level, built on the host and cached."""
+        key = tuple((t.data_ptr(), t.numel(), t._version) for t in
+                    (target_boxes, src_box_starts, src_box_lists, class_lists, centers)) \
+            + (int(class_level_start), int(nclasses))
+        sched = self._schedules.get(key)
+        if sched is None:
+            sched = build_m2l_schedule(
+                actx, target_boxes.cpu().numpy(), src_box_starts.cpu().numpy(),
+                src_box_lists.cpu().numpy(), class_lists.cpu().numpy(),
+                int(class_level_start), int(nclasses), centers.cpu().numpy())
+            self._schedules[key] = sched
+        return sched
+
+    def __call__(self, actx, **kwargs):
+        tr = self.tgt_expansion.m2l_translation
+        m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+        data = kwargs["m2l_translation_classes_dependent_data"]
+        src, work = kwargs["src_expansions"], kwargs["tgt_expansions"]
+        sched = self.schedule(
+            actx, kwargs["target_boxes"], kwargs["src_box_starts"], kwargs["src_box_lists"],
+            kwargs["m2l_translation_classes_lists"],
+            kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"])
+        if sched["ntiles"] == 0:
+            return work
+        flops = 2.0*m["nl"]*m["nm"]*sched["npairs"]
+        with kernel_span(f"m2l_apply_{tr.precision}", flops):
+            _capi.call("sumpyhip_m2l_apply", tr.precision_id, sched["ntiles"],
+                   _capi.ptr(sched["tile_targets"]), _capi.ptr(sched["tile_slot_starts"]),
+                       _capi.ptr(sched["slot_class"]), _capi.ptr(sched["slot_sources"]),
+                       m["nl"], m["nm"], int(work.shape[1]), int(src.shape[1]),
+                       _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), m["ncirc"],
+                       _capi.ptr(data.contiguous()), _capi.ptr(src),
+                       int(kwargs["src_base_ibox"]), _capi.ptr(work),
+                       int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        return work
+
+
+def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_start,
+                       nclasses, centers):
+    """Group a level's target boxes into tiles of :data:`TILE` sharing most of
+    their translation classes (targets sorted by their octant within the
+    parent), and list for each tile the classes present with one source box per
+    tile column (-1 if none)."""
+    nt = len(target_boxes)
+    e0, e1 = int(starts[0]), int(starts[-1])
+    counts = np.diff(starts).astype(np.int64)
+    itgt = np.repeat(np.arange(nt), counts)
+    e_src = lists[e0:e1].astype(np.int64)
+    e_cls = class_lists[e0:e1].astype(np.int64) - class_start
+    if nt == 0:
+        return dict(ntiles=0)
+    # octant parity from centres: the smallest separation of a list-2 pair
+    # along an axis is one box size h
+    tb = target_boxes.astype(np.int64)
+    ct = centers[:, tb]
+    if len(e_src):
+        diff = np.abs(centers[:, tb[itgt]] - centers[:, e_src])
+        h = diff[diff > 1e-12*max(1.0, float(np.abs(centers).max()))].min()
+    else:
+        h = 1.0
+    rel = np.rint((ct - ct[:, :1])/h).astype(np.int64)
+    octant = ((rel[0] & 1) << 2) | ((rel[1] & 1) << 1) | (rel[2] & 1)
+    order = np.argsort(octant, kind="stable")
+    ntiles = (nt + TILE - 1)//TILE
+    padded = np.full(ntiles*TILE, -1, np.int64)
+    padded[:nt] = order
+    tab = np.full((nt + 1, max(nclasses, 1)), -1, np.int32)
+    tab[itgt, e_cls] = e_src
+    rows = np.where(padded >= 0, padded, nt)  # row nt is all -1
+    tabT = tab[rows].reshape(ntiles, TILE, -1)
+    used = (tabT >= 0).any(axis=1)
+    ti, ci = np.nonzero(used)
+    slot_sources = tabT[ti, :, ci].astype(np.int32)
+    tile_slot_starts = np.zeros(ntiles + 1, np.int32)
+    np.cumsum(used.sum(axis=1), out=tile_slot_starts[1:])
+    tile_targets = np.where(padded >= 0, tb[np.maximum(padded, 0)], -1).astype(np.int32)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(actx.device)
+
+    return dict(ntiles=ntiles, tile_targets=dev(tile_targets),
+                tile_slot_starts=dev(tile_slot_starts), slot_class=dev(ci),
+                slot_sources=dev(slot_sources.reshape(-1)), nslots=len(ci),
+                npairs=int(len(e_src)))
+
+
+class M2LPostprocessLocal(E2EBase):
+    """Scale the translated coefficients into FP64 locals:
+    ``L[b, i] = work[b, i] * (tgt_rscale/src_rscale)^|mi_i|``."""
+
+    default_name = "m2l_postprocess_local"
+
+    def __call__(self, actx, **kwargs):
+        tr = self.tgt_expansion.m2l_translation
+        m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+        tgt = kwargs["tgt_expansions"]
+        work = kwargs["tgt_expansions_before_postprocessing"]
+        ratio = float(kwargs["tgt_rscale"])/float(kwargs["src_rscale"])
+        _capi.call("sumpyhip_m2l_postprocess", tr.precision_id, int(tgt.shape[0]),
+                   int(tgt.shape[1]), int(work.shape[1]), _capi.ptr(work),
+                   _capi.ptr(m["tgt_degree"]), ratio, _capi.ptr(tgt), actx.stream_handle)
+        return tgt
+
+
+__all__ = [
+    "E2EFromCSR",
+    "E2EFromChildren",
+    "E2EFromParent",
+    "M2LGenerateTranslationClassesDependentData",
+    "M2LPostprocessLocal",
+    "M2LPreprocessMultipole",
+    "M2LUsingTranslationClassesDependentData",
+]

@@ -1,0 +1,100 @@
+"""Expansion-to-particle operators (reference ``sumpy/e2p.py``):
+:class:`E2PFromSingleBox` (L2P, ``e2p.py:123-226``, overwrites) and
+:class:`E2PFromCSR` (M2P from list 3, ``e2p.py:233-343``, accumulates).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from sumpy_amd import _capi
+from sumpy_amd.array_context import obj_array
+from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
+from sumpy_amd.tools import TargetOutputs, as_component_list, as_i32
+
+
+class E2PBase:
+    def __init__(self, expansion, kernels, name=None):
+        expansion = expansion.with_kernel(remove_source_transforms(expansion.kernel))
+        kernels = [remove_source_transforms(k) for k in kernels]
+        for knl in kernels:
+            if remove_target_transforms(knl) != expansion.kernel:
+                raise ValueError(f"kernel {knl} does not match the expansion's kernel")
+        self.expansion = expansion
+        self.kernels = kernels
+        self.name = name or self.default_name
+        self.dim = expansion.dim
+
+    @property
+    def nresults(self):
+        return len(self.kernels)
+
+    def get_cache_key(self):
+        return (type(self).__name__, self.expansion, tuple(self.kernels))
+
+    def _setup(self, actx, kwargs):
+        targets = as_component_list(kwargs["targets"], self.dim)
+        centers = kwargs["centers"].contiguous()
+        result = kwargs.get("result")
+        nt = targets[0].shape[0]
+        if result is None:
+            result = obj_array([torch.zeros(nt, dtype=torch.float64, device=actx.device)
+                                for _ in self.kernels])
+        outs = TargetOutputs(self.kernels, self.dim)
+        return targets, centers, result, outs
+
+
+class E2PFromSingleBox(E2PBase):
+    """L2P: ``result[t] = scale * sum_mi L_full[mi] (b/rho)^mi/mi!``."""
+
+    default_name = "e2p_from_single_box"
+
+    def __call__(self, actx, **kwargs):
+        targets, centers, result, outs = self._setup(actx, kwargs)
+        nt = targets[0].shape[0]
+        ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
+        tables = self.expansion.device_tables(actx.device)
+        tb = as_i32(kwargs["target_boxes"])
+        bts = as_i32(kwargs["box_target_starts"])
+        btc = as_i32(kwargs["box_target_counts_nonchild"])
+        _capi.call("sumpyhip_l2p", kernel_id(self.expansion.kernel), tables.desc,
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts), _capi.ptr(btc),
+                   _capi.ptr(centers), int(centers.shape[1]),
+                   *[t.data_ptr() for t in targets], float(kwargs["rscale"]),
+                   self.expansion.kernel.get_global_scaling_const(),
+                   _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
+                   *ptrs, actx.stream_handle)
+        # combined target kernels are added to the (zero-initialised) results
+        # of the boxes' targets; plain kernels are written in place
+        outs.finish(actx, targets, accumulate=not outs.direct)
+        return result
+
+
+class E2PFromCSR(E2PBase):
+    """M2P: ``result[t] += scale * sum_src sum_mi M[mi] rho^|mi| d^mi G(t - c)``."""
+
+    default_name = "e2p_from_csr"
+
+    def __call__(self, actx, **kwargs):
+        targets, centers, result, outs = self._setup(actx, kwargs)
+        nt = targets[0].shape[0]
+        ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
+        tables = self.expansion.device_tables(actx.device)
+        tb = as_i32(kwargs["target_boxes"])
+        starts = as_i32(kwargs["source_box_starts"])
+        lists = as_i32(kwargs["source_box_lists"])
+        bts = as_i32(kwargs["box_target_starts"])
+        btc = as_i32(kwargs["box_target_counts_nonchild"])
+        _capi.call("sumpyhip_m2p", kernel_id(self.expansion.kernel), tables.desc,
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                   _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
+                   int(centers.shape[1]), *[t.data_ptr() for t in targets],
+                   float(kwargs["rscale"]),
+                   self.expansion.kernel.get_global_scaling_const(),
+                   _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
+                   *ptrs, actx.stream_handle)
+        outs.finish(actx, targets, accumulate=True)
+        return result
+
+
+__all__ = ["E2PFromCSR", "E2PFromSingleBox"]

@@ -1,0 +1,68 @@
+"""Local expansions (reference ``sumpy/expansion/local.py:71-517``).
+
+``VolumeTaylor*`` locals store ``L_mi = rho^|mi| d^mi G(a)`` for the stored
+identifiers; ``LineTaylorLocalExpansion`` is the QBX expansion along the
+centre-target line (``local.py:116-198``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+from sumpy_amd.expansion import ExpansionBase, VolumeTaylorExpansionMixin
+
+
+class LocalExpansionBase(ExpansionBase):
+    pass
+
+
+@dataclass(frozen=True)
+class LineTaylorLocalExpansion(LocalExpansionBase):
+    """Taylor expansion in the line parameter ``tau`` of ``G(a + tau b)``
+    (``local.py:116-198``); used by :class:`sumpy_amd.qbx.LayerPotential`."""
+
+    def get_coefficient_identifiers(self):
+        return [(i,) for i in range(self.order + 1)]
+
+    def get_storage_index(self, mi):
+        ind, = mi
+        return ind
+
+    @property
+    def m2l_translation(self):
+        raise NotImplementedError
+
+
+@dataclass(frozen=True)
+class _VolumeTaylorLocalBase(VolumeTaylorExpansionMixin, LocalExpansionBase):
+    m2l_translation_override: object = field(default=None, kw_only=True)
+
+    @property
+    def m2l_translation(self):
+        if self.m2l_translation_override is not None:
+            return self.m2l_translation_override
+        from sumpy_amd.expansion.m2l import DefaultM2LTranslationClassFactory
+        return DefaultM2LTranslationClassFactory().get_m2l_translation_class(
+            self.kernel, type(self))()
+
+
+@dataclass(frozen=True)
+class VolumeTaylorLocalExpansion(_VolumeTaylorLocalBase):
+    """Full Taylor local (``local.py:501-504``)."""
+
+    compressed = False
+
+
+@dataclass(frozen=True)
+class LinearPDEConformingVolumeTaylorLocalExpansion(_VolumeTaylorLocalBase):
+    """PDE-compressed Taylor local (``local.py:507-510``)."""
+
+    compressed = True
+
+
+__all__ = [
+    "LineTaylorLocalExpansion",
+    "LinearPDEConformingVolumeTaylorLocalExpansion",
+    "LocalExpansionBase",
+    "VolumeTaylorLocalExpansion",
+]

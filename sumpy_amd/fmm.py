@@ -1,0 +1,596 @@
+"""The FMM wrangler: the drop-in boundary above the operators.
+
+:class:`SumpyTreeIndependentDataForWrangler` (operator factory) and
+:class:`SumpyExpansionWrangler` keep the constructor arguments, attributes and
+method signatures of ``sumpy/fmm.py:99-1115`` (the interface boxtree's
+``drive_fmm`` calls), so :func:`sumpy_amd.drive.drive_fmm` -- or boxtree's --
+runs them unchanged.  Expansion arrays are flat device tensors with per-level
+starts (``build_csr_level_starts``, ``fmm.py:1093-1115``); per-level views are
+row-major ``(nboxes_level, ncoeff)`` reshapes.
+"""
+
+from __future__ import annotations
+
+from functools import cached_property
+
+import numpy as np
+import torch
+
+from sumpy_amd.array_context import obj_array, torch_dtype
+from sumpy_amd.e2e import (
+    E2EFromChildren,
+    E2EFromCSR,
+    E2EFromParent,
+    M2LGenerateTranslationClassesDependentData,
+    M2LPostprocessLocal,
+    M2LPreprocessMultipole,
+    M2LUsingTranslationClassesDependentData,
+)
+from sumpy_amd.e2p import E2PFromCSR, E2PFromSingleBox
+from sumpy_amd.p2e import P2EFromCSR, P2EFromSingleBox
+from sumpy_amd.p2p import P2PFromCSR
+
+
+def build_csr_level_starts(level_orders, order_to_size, level_starts):
+    """Starts of each level's block in a flat per-box array
+    (reference ``fmm.py:1093-1115``)."""
+    result = [0]
+    for lev in range(len(level_orders)):
+        lev_nboxes = level_starts[lev + 1] - level_starts[lev]
+        result.append(result[-1] + order_to_size(level_orders[lev])*lev_nboxes)
+    return result
+
+
+class SumpyTreeIndependentDataForWrangler:
+    """Operator factory (reference ``fmm.py:99-252``).
+
+    Extra keyword ``p2p_precision``: ``"fp64"`` (plain) or ``"fp32"`` for the
+    near field; the M2L precision is chosen by the local expansion's
+    ``m2l_translation`` (``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``).
+    """
+
+    def __init__(self, array_context, multipole_expansion_factory,
+                 local_expansion_factory, target_kernels, exclude_self=False,
+                 use_rscale=None, strength_usage=None, source_kernels=None,
+                 p2p_precision="fp64"):
+        self._setup_actx = array_context
+        self.multipole_expansion_factory = multipole_expansion_factory
+        self.local_expansion_factory = local_expansion_factory
+        self.source_kernels = source_kernels
+        self.target_kernels = target_kernels
+        self.exclude_self = exclude_self
+        self.use_rscale = use_rscale
+        self.strength_usage = strength_usage
+        self.p2p_precision = p2p_precision
+        self._memo = {}
+
+    def _m(self, key, make):
+        v = self._memo.get(key)
+        if v is None:
+            v = make()
+            self._memo[key] = v
+        return v
+
+    def get_base_kernel(self):
+        bases = {k.get_base_kernel() for k in self.target_kernels}
+        if len(bases) != 1:
+            raise ValueError("target kernels must share a base kernel")
+        return bases.pop()
+
+    def multipole_expansion(self, order):
+        def make():
+            if self.use_rscale is None:
+                return self.multipole_expansion_factory(order)
+            return self.multipole_expansion_factory(order, use_rscale=self.use_rscale)
+        return self._m(("mpole", order), make)
+
+    def local_expansion(self, order):
+        def make():
+            if self.use_rscale is None:
+                return self.local_expansion_factory(order)
+            return self.local_expansion_factory(order, use_rscale=self.use_rscale)
+        return self._m(("local", order), make)
+
+    @property
+    def m2l_translation(self):
+        return self.local_expansion(0).m2l_translation
+
+    def p2m(self, tgt_order):
+        return self._m(("p2m", tgt_order), lambda: P2EFromSingleBox(
+            kernels=self.source_kernels, expansion=self.multipole_expansion(tgt_order),
+            strength_usage=self.strength_usage, name="p2m"))
+
+    def p2l(self, tgt_order):
+        return self._m(("p2l", tgt_order), lambda: P2EFromCSR(
+            kernels=self.source_kernels, expansion=self.local_expansion(tgt_order),
+            strength_usage=self.strength_usage, name="p2l"))
+
+    def m2m(self, src_order, tgt_order):
+        return self._m(("m2m", src_order, tgt_order), lambda: E2EFromChildren(
+            self.multipole_expansion(src_order), self.multipole_expansion(tgt_order),
+            name="m2m"))
+
+    def m2l(self, src_order, tgt_order, m2l_use_translation_classes_dependent_data=False):
+        cls = (M2LUsingTranslationClassesDependentData
+               if m2l_use_translation_classes_dependent_data else E2EFromCSR)
+        return self._m(("m2l", src_order, tgt_order, cls), lambda: cls(
+            self.multipole_expansion(src_order), self.local_expansion(tgt_order),
+            name="m2l"))
+
+    def m2l_translation_class_dependent_data_kernel(self, src_order, tgt_order):
+        return self._m(("m2l_data", src_order, tgt_order),
+                       lambda: M2LGenerateTranslationClassesDependentData(
+                           self.multipole_expansion(src_order),
+                           self.local_expansion(tgt_order)))
+
+    def m2l_preprocess_mpole_kernel(self, src_order, tgt_order):
+        return self._m(("m2l_pre", src_order, tgt_order), lambda: M2LPreprocessMultipole(
+            self.multipole_expansion(src_order), self.local_expansion(tgt_order)))
+
+    def m2l_postprocess_local_kernel(self, src_order, tgt_order):
+        return self._m(("m2l_post", src_order, tgt_order), lambda: M2LPostprocessLocal(
+            self.multipole_expansion(src_order), self.local_expansion(tgt_order)))
+
+    def l2l(self, src_order, tgt_order):
+        return self._m(("l2l", src_order, tgt_order), lambda: E2EFromParent(
+            self.local_expansion(src_order), self.local_expansion(tgt_order), name="l2l"))
+
+    def m2p(self, src_order):
+        return self._m(("m2p", src_order), lambda: E2PFromCSR(
+            self.multipole_expansion(src_order), self.target_kernels, name="m2p"))
+
+    def l2p(self, src_order):
+        return self._m(("l2p", src_order), lambda: E2PFromSingleBox(
+            self.local_expansion(src_order), self.target_kernels, name="l2p"))
+
+    def p2p(self):
+        return self._m(("p2p",), lambda: P2PFromCSR(
+            target_kernels=self.target_kernels, source_kernels=self.source_kernels,
+            exclude_self=self.exclude_self, strength_usage=self.strength_usage,
+            name="p2p", precision=self.p2p_precision))
+
+
+class SumpyExpansionWrangler:
+    """Implements boxtree's ``ExpansionWranglerInterface`` with the GPU
+    operators (reference ``fmm.py:264-1088``)."""
+
+    def __init__(self, tree_indep, traversal, dtype, fmm_level_to_order,
+                 source_extra_kwargs=None, kernel_extra_kwargs=None,
+                 self_extra_kwargs=None, translation_classes_data=None,
+                 preprocessed_mpole_dtype=None, *, _disable_translation_classes=False):
+        self.tree_indep = tree_indep
+        self.traversal = traversal
+        self.dtype = np.dtype(dtype)
+        if not callable(fmm_level_to_order):
+            raise TypeError("fmm_level_to_order not passed")
+        base_kernel = tree_indep.get_base_kernel()
+        self.source_extra_kwargs = dict(source_extra_kwargs or {})
+        self.kernel_extra_kwargs = dict(kernel_extra_kwargs or {})
+        self.self_extra_kwargs = dict(self_extra_kwargs or {})
+        self.extra_kwargs = dict(self.source_extra_kwargs)
+        self.extra_kwargs.update(self.kernel_extra_kwargs)
+        kernel_arg_set = frozenset(self.kernel_extra_kwargs.items())
+        self.level_orders = [fmm_level_to_order(base_kernel, kernel_arg_set, self.tree, lev)
+                             for lev in range(self.tree.nlevels)]
+        tr = tree_indep.m2l_translation
+        self.preprocessed_mpole_dtype = (np.dtype(preprocessed_mpole_dtype)
+                                         if preprocessed_mpole_dtype is not None
+                                         else tr.preprocessed_dtype(self.dtype))
+        if _disable_translation_classes or not base_kernel.is_translation_invariant:
+            self.supports_translation_classes = False
+        else:
+            if translation_classes_data is None:
+                from sumpy_amd.tree import TranslationClassesBuilder
+                actx = tree_indep._setup_actx
+                translation_classes_data, _ = TranslationClassesBuilder(actx)(
+                    actx, traversal, self.tree, is_translation_per_level=True)
+            self.supports_translation_classes = True
+        self.translation_classes_data = translation_classes_data
+        self._memo = {}
+
+    @property
+    def tree(self):
+        return self.traversal.tree
+
+    def level_to_rscale(self, level):
+        """``2 r / order`` with ``r`` the box size (reference ``fmm.py:369-386``)."""
+        order = self.level_orders[level]
+        r = self.tree.root_extent*(2**-level)
+        return r*2/order
+
+    # {{{ data vector utilities
+
+    @cached_property
+    def tree_level_start_box_nrs(self):
+        return np.asarray(self.tree.host["level_start_box_nrs"])
+
+    def _expansions_level_starts(self, order_to_size):
+        return build_csr_level_starts(self.level_orders, order_to_size,
+                                      self.tree_level_start_box_nrs)
+
+    @cached_property
+    def _mpole_level_starts(self):
+        return self._expansions_level_starts(
+            lambda order: len(self.tree_indep.multipole_expansion(order)))
+
+    def multipole_expansions_level_starts(self):
+        return self._mpole_level_starts
+
+    @cached_property
+    def _local_level_starts(self):
+        return self._expansions_level_starts(
+            lambda order: len(self.tree_indep.local_expansion(order)))
+
+    def local_expansions_level_starts(self):
+        return self._local_level_starts
+
+    @cached_property
+    def _class_level_starts(self):
+        return self.traversal.host["from_sep_siblings_translation_classes_level_starts"]
+
+    def m2l_translation_class_level_start_box_nrs(self):
+        return self._class_level_starts
+
+    def m2l_translation_classes_dependent_data_level_starts(self):
+        def order_to_size(order):
+            mpole = self.tree_indep.multipole_expansion(order)
+            local = self.tree_indep.local_expansion(order)
+            return local.m2l_translation.translation_classes_dependent_ndata(local, mpole)
+        return build_csr_level_starts(self.level_orders, order_to_size,
+                                      self.m2l_translation_class_level_start_box_nrs())
+
+    def _zeros(self, n, dtype):
+        return torch.zeros(int(n), dtype=torch_dtype(dtype),
+                           device=self.tree_indep._setup_actx.device)
+
+    def multipole_expansion_zeros(self, actx):
+        return self._zeros(self.multipole_expansions_level_starts()[-1], self.dtype)
+
+    def local_expansion_zeros(self, actx):
+        return self._zeros(self.local_expansions_level_starts()[-1], self.dtype)
+
+    def m2l_translation_classes_dependent_data_zeros(self, actx):
+        data_starts = self.m2l_translation_classes_dependent_data_level_starts()
+        cls_starts = self.m2l_translation_class_level_start_box_nrs()
+        result = []
+        for lev in range(self.tree.nlevels):
+            n = data_starts[lev + 1] - data_starts[lev]
+            ncls = cls_starts[lev + 1] - cls_starts[lev]
+            result.append(self._zeros(n, np.float64).reshape(ncls, -1)
+                          if ncls else self._zeros(0, np.float64).reshape(0, 0))
+        return result
+
+    def multipole_expansions_view(self, mpole_exps, level):
+        e0, e1 = self.multipole_expansions_level_starts()[level:level + 2]
+        b0, b1 = self.tree_level_start_box_nrs[level:level + 2]
+        return b0, mpole_exps[e0:e1].reshape(int(b1 - b0), -1)
+
+    def local_expansions_view(self, local_exps, level):
+        e0, e1 = self.local_expansions_level_starts()[level:level + 2]
+        b0, b1 = self.tree_level_start_box_nrs[level:level + 2]
+        return b0, local_exps[e0:e1].reshape(int(b1 - b0), -1)
+
+    def m2l_translation_classes_dependent_data_view(self, data, level):
+        cls_start = self.m2l_translation_class_level_start_box_nrs()[level]
+        return cls_start, data[level]
+
+    def _m2l_level_sizes(self, nexprs):
+        def order_to_size(order):
+            mpole = self.tree_indep.multipole_expansion(order)
+            local = self.tree_indep.local_expansion(order)
+            return nexprs(local.m2l_translation, local, mpole)
+        return build_csr_level_starts(self.level_orders, order_to_size,
+                                      self.tree_level_start_box_nrs)
+
+    def m2l_preproc_mpole_expansions_level_starts(self):
+        return self._m2l_level_sizes(
+            lambda tr, loc, mp: tr.preprocess_multipole_nexprs(loc, mp))
+
+    def m2l_preproc_mpole_expansion_zeros(self, actx, template_ary):
+        starts = self.m2l_preproc_mpole_expansions_level_starts()
+        return self._per_level_zeros(starts)
+
+    def m2l_work_array_level_starts(self):
+        return self._m2l_level_sizes(
+            lambda tr, loc, mp: tr.postprocess_local_nexprs(loc, mp))
+
+    def m2l_work_array_zeros(self, actx, template_ary):
+        return self._per_level_zeros(self.m2l_work_array_level_starts())
+
+    def _per_level_zeros(self, starts):
+        result = []
+        for lev in range(self.tree.nlevels):
+            b0, b1 = self.tree_level_start_box_nrs[lev:lev + 2]
+            n = starts[lev + 1] - starts[lev]
+            result.append(self._zeros(n, self.preprocessed_mpole_dtype).reshape(
+                int(b1 - b0), -1))
+        return result
+
+    def m2l_preproc_mpole_expansions_view(self, mpole_exps, level):
+        return self.tree_level_start_box_nrs[level], mpole_exps[level]
+
+    m2l_work_array_view = m2l_preproc_mpole_expansions_view
+
+    def output_zeros(self, actx):
+        return obj_array([self._zeros(self.tree.ntargets, self.dtype)
+                          for _ in self.tree_indep.target_kernels])
+
+    def reorder_sources(self, source_array):
+        return source_array[self.tree.user_source_ids.long()]
+
+    def reorder_potentials(self, potentials):
+        assert isinstance(potentials, np.ndarray) and potentials.dtype.char == "O"
+        idx = self.tree.sorted_target_ids.long()
+        return obj_array([p[idx] for p in potentials])
+
+    def distribute_source_weights(self, actx, src_weight_vecs, src_idx_all_ranks):
+        return src_weight_vecs
+
+    def gather_potential_results(self, actx, potentials, tgt_idx_all_ranks):
+        return potentials
+
+    def communicate_mpoles(self, actx, mpole_exps, return_stats=False):
+        return None
+
+    @cached_property
+    def max_nsources_in_one_box(self):
+        return int(self.tree.host["box_source_counts_nonchild"].max())
+
+    @cached_property
+    def max_ntargets_in_one_box(self):
+        return int(self.tree.host["box_target_counts_nonchild"].max())
+
+    # }}}
+
+    def box_source_list_kwargs(self):
+        return {"box_source_starts": self.tree.box_source_starts,
+                "box_source_counts_nonchild": self.tree.box_source_counts_nonchild,
+                "sources": self.tree.sources}
+
+    def box_target_list_kwargs(self):
+        return {"box_target_starts": self.tree.box_target_starts,
+                "box_target_counts_nonchild": self.tree.box_target_counts_nonchild,
+                "targets": self.tree.targets}
+
+    # {{{ FMM stages (reference fmm.py:609-1086)
+
+    def form_multipoles(self, actx, level_start_source_box_nrs, source_boxes,
+                        src_weight_vecs):
+        mpoles = self.multipole_expansion_zeros(actx)
+        starts = _host(level_start_source_box_nrs)
+        kwargs = dict(self.extra_kwargs)
+        kwargs.update(self.box_source_list_kwargs())
+        for lev in range(self.tree.nlevels):
+            start, stop = starts[lev:lev + 2]
+            if start == stop:
+                continue
+            p2m = self.tree_indep.p2m(self.level_orders[lev])
+            base, view = self.multipole_expansions_view(mpoles, lev)
+            res = p2m(actx, source_boxes=source_boxes[start:stop],
+                      centers=self.tree.box_centers, strengths=src_weight_vecs,
+                      tgt_expansions=view, tgt_base_ibox=base,
+                      rscale=self.level_to_rscale(lev), **kwargs)
+            assert res is view
+        return mpoles
+
+    def coarsen_multipoles(self, actx, level_start_source_parent_box_nrs,
+                           source_parent_boxes, mpoles):
+        starts = _host(level_start_source_parent_box_nrs)
+        for source_level in range(self.tree.nlevels - 1, 2, -1):
+            target_level = source_level - 1
+            start, stop = starts[target_level:target_level + 2]
+            if start == stop:
+                continue
+            m2m = self.tree_indep.m2m(self.level_orders[source_level],
+                                      self.level_orders[target_level])
+            sbase, sview = self.multipole_expansions_view(mpoles, source_level)
+            tbase, tview = self.multipole_expansions_view(mpoles, target_level)
+            res = m2m(actx, src_expansions=sview, src_base_ibox=sbase,
+                      tgt_expansions=tview, tgt_base_ibox=tbase,
+                      target_boxes=source_parent_boxes[start:stop],
+                      box_child_ids=self.tree.box_child_ids,
+                      centers=self.tree.box_centers,
+                      src_rscale=self.level_to_rscale(source_level),
+                      tgt_rscale=self.level_to_rscale(target_level),
+                      **self.kernel_extra_kwargs)
+            assert res is tview
+        return mpoles
+
+    def eval_direct(self, actx, target_boxes, source_box_starts, source_box_lists,
+                    src_weight_vecs):
+        pot = self.output_zeros(actx)
+        kwargs = dict(self.extra_kwargs)
+        kwargs.update(self.self_extra_kwargs)
+        kwargs.update(self.box_source_list_kwargs())
+        kwargs.update(self.box_target_list_kwargs())
+        res = self.tree_indep.p2p()(
+            actx, target_boxes=target_boxes, source_box_starts=source_box_starts,
+            source_box_lists=source_box_lists, strength=src_weight_vecs, result=pot,
+            max_nsources_in_one_box=self.max_nsources_in_one_box,
+            max_ntargets_in_one_box=self.max_ntargets_in_one_box,
+            box_centers=self.tree.box_centers, **kwargs)
+        for a, b in zip(pot, res):
+            assert a is b
+        return pot
+
+    def multipole_to_local_precompute(self):
+        memo = self._memo.get("m2l_precompute")
+        if memo is not None:
+            return memo
+        actx = self.tree_indep._setup_actx
+        data = self.m2l_translation_classes_dependent_data_zeros(actx)
+        result = []
+        for lev in range(self.tree.nlevels):
+            order = self.level_orders[lev]
+            cls_start, view = self.m2l_translation_classes_dependent_data_view(data, lev)
+            ncls = view.shape[0]
+            if ncls == 0:
+                result.append(view)
+                continue
+            vecs = self.translation_classes_data \
+                .from_sep_siblings_translation_class_to_distance_vector
+            self.tree_indep.m2l_translation_class_dependent_data_kernel(order, order)(
+                actx, src_rscale=self.level_to_rscale(lev),
+                translation_classes_level_start=cls_start, ntranslation_classes=ncls,
+                m2l_translation_classes_dependent_data=view,
+                m2l_translation_vectors=vecs, ntranslation_vectors=vecs.shape[1],
+                **self.kernel_extra_kwargs)
+            result.append(view)
+        self._memo["m2l_precompute"] = result
+        return result
+
+    def _add_m2l_precompute_kwargs(self, kwargs, lev):
+        if not self.supports_translation_classes:
+            return
+        data = self.multipole_to_local_precompute()
+        cls_start, view = self.m2l_translation_classes_dependent_data_view(data, lev)
+        kwargs["m2l_translation_classes_dependent_data"] = view
+        kwargs["translation_classes_level_start"] = cls_start
+        kwargs["m2l_translation_classes_lists"] = \
+            self.translation_classes_data.from_sep_siblings_translation_classes
+
+    def multipole_to_local(self, actx, level_start_target_box_nrs, target_boxes,
+                           src_box_starts, src_box_lists, mpole_exps):
+        local_exps = self.local_expansion_zeros(actx)
+        starts = _host(level_start_target_box_nrs)
+        use_pre = self.supports_translation_classes
+        if use_pre:
+            pre_mpoles = self.m2l_preproc_mpole_expansion_zeros(actx, mpole_exps)
+            work = self.m2l_work_array_zeros(actx, local_exps)
+        tr_classes = self.m2l_translation_class_level_start_box_nrs()
+        for lev in range(self.tree.nlevels):
+            start, stop = starts[lev:lev + 2]
+            if start == stop:
+                continue
+            order = self.level_orders[lev]
+            if use_pre:
+                if tr_classes[lev] == tr_classes[lev + 1]:
+                    continue
+                _, smv = self.multipole_expansions_view(mpole_exps, lev)
+                self.tree_indep.m2l_preprocess_mpole_kernel(order, order)(
+                    actx, src_expansions=smv, preprocessed_src_expansions=pre_mpoles[lev],
+                    src_rscale=self.level_to_rscale(lev), **self.kernel_extra_kwargs)
+                sbase, sview = self.m2l_preproc_mpole_expansions_view(pre_mpoles, lev)
+                tbase, tview = self.m2l_work_array_view(work, lev)
+            else:
+                sbase, sview = self.multipole_expansions_view(mpole_exps, lev)
+                tbase, tview = self.local_expansions_view(local_exps, lev)
+            m2l = self.tree_indep.m2l(order, order, self.supports_translation_classes)
+            kwargs = dict(
+                src_expansions=sview, src_base_ibox=sbase, tgt_expansions=tview,
+                tgt_base_ibox=tbase, target_boxes=target_boxes[start:stop],
+                src_box_starts=src_box_starts[start:stop + 1], src_box_lists=src_box_lists,
+                centers=self.tree.box_centers, src_rscale=self.level_to_rscale(lev),
+                tgt_rscale=self.level_to_rscale(lev), **self.kernel_extra_kwargs)
+            self._add_m2l_precompute_kwargs(kwargs, lev)
+            if "m2l_translation_classes_dependent_data" in kwargs and \
+                    kwargs["m2l_translation_classes_dependent_data"].numel() == 0:
+                continue
+            m2l(actx, **kwargs)
+            if use_pre:
+                _, lview = self.local_expansions_view(local_exps, lev)
+                self.tree_indep.m2l_postprocess_local_kernel(order, order)(
+                    actx, tgt_expansions=lview, tgt_expansions_before_postprocessing=tview,
+                    src_rscale=self.level_to_rscale(lev), tgt_rscale=self.level_to_rscale(lev),
+                    **self.kernel_extra_kwargs)
+        return local_exps
+
+    def eval_multipoles(self, actx, target_boxes_by_source_level, source_boxes_by_level,
+                        mpole_exps):
+        pot = self.output_zeros(actx)
+        kwargs = dict(self.kernel_extra_kwargs)
+        kwargs.update(self.box_target_list_kwargs())
+        for lev, ssn in enumerate(source_boxes_by_level):
+            if len(target_boxes_by_source_level[lev]) == 0:
+                continue
+            m2p = self.tree_indep.m2p(self.level_orders[lev])
+            base, view = self.multipole_expansions_view(mpole_exps, lev)
+            res = m2p(actx, src_expansions=view, src_base_ibox=base,
+                      target_boxes=target_boxes_by_source_level[lev],
+                      source_box_starts=ssn.starts, source_box_lists=ssn.lists,
+                      centers=self.tree.box_centers, result=pot,
+                      rscale=self.level_to_rscale(lev), **kwargs)
+            for a, b in zip(pot, res):
+                assert a is b
+        return pot
+
+    def form_locals(self, actx, level_start_target_or_target_parent_box_nrs,
+                    target_or_target_parent_boxes, starts, lists, src_weight_vecs):
+        local_exps = self.local_expansion_zeros(actx)
+        lstarts = _host(level_start_target_or_target_parent_box_nrs)
+        kwargs = dict(self.extra_kwargs)
+        kwargs.update(self.box_source_list_kwargs())
+        for lev in range(self.tree.nlevels):
+            start, stop = lstarts[lev:lev + 2]
+            if start == stop:
+                continue
+            p2l = self.tree_indep.p2l(self.level_orders[lev])
+            base, view = self.local_expansions_view(local_exps, lev)
+            res = p2l(actx, target_boxes=target_or_target_parent_boxes[start:stop],
+                      source_box_starts=starts[start:stop + 1], source_box_lists=lists,
+                      centers=self.tree.box_centers, strengths=src_weight_vecs,
+                      tgt_expansions=view, tgt_base_ibox=base,
+                      rscale=self.level_to_rscale(lev), **kwargs)
+            assert res is view
+        return local_exps
+
+    def refine_locals(self, actx, level_start_target_or_target_parent_box_nrs,
+                      target_or_target_parent_boxes, local_exps):
+        lstarts = _host(level_start_target_or_target_parent_box_nrs)
+        for target_lev in range(1, self.tree.nlevels):
+            start, stop = lstarts[target_lev:target_lev + 2]
+            if start == stop:
+                continue
+            source_lev = target_lev - 1
+            l2l = self.tree_indep.l2l(self.level_orders[source_lev],
+                                      self.level_orders[target_lev])
+            sbase, sview = self.local_expansions_view(local_exps, source_lev)
+            tbase, tview = self.local_expansions_view(local_exps, target_lev)
+            res = l2l(actx, src_expansions=sview, src_base_ibox=sbase,
+                      tgt_expansions=tview, tgt_base_ibox=tbase,
+                      target_boxes=target_or_target_parent_boxes[start:stop],
+                      box_parent_ids=self.tree.box_parent_ids,
+                      centers=self.tree.box_centers,
+                      src_rscale=self.level_to_rscale(source_lev),
+                      tgt_rscale=self.level_to_rscale(target_lev),
+                      **self.kernel_extra_kwargs)
+            assert res is tview
+        return local_exps
+
+    def eval_locals(self, actx, level_start_target_box_nrs, target_boxes, local_exps):
+        pot = self.output_zeros(actx)
+        starts = _host(level_start_target_box_nrs)
+        kwargs = dict(self.kernel_extra_kwargs)
+        kwargs.update(self.box_target_list_kwargs())
+        for lev in range(self.tree.nlevels):
+            start, stop = starts[lev:lev + 2]
+            if start == stop:
+                continue
+            l2p = self.tree_indep.l2p(self.level_orders[lev])
+            base, view = self.local_expansions_view(local_exps, lev)
+            res = l2p(actx, src_expansions=view, src_base_ibox=base,
+                      target_boxes=target_boxes[start:stop], centers=self.tree.box_centers,
+                      result=pot, rscale=self.level_to_rscale(lev), **kwargs)
+            for a, b in zip(pot, res):
+                assert a is b
+        return pot
+
+    def finalize_potentials(self, actx, potentials):
+        return potentials
+
+    # }}}
+
+
+def _host(a):
+    """Level starts on the host (the reference syncs with ``to_numpy`` here,
+    ``fmm.py:614``); host copies kept by the traversal are used when given."""
+    if isinstance(a, torch.Tensor):
+        return a.cpu().numpy()
+    return np.asarray(a)
+
+
+__all__ = [
+    "SumpyExpansionWrangler",
+    "SumpyTreeIndependentDataForWrangler",
+    "build_csr_level_starts",
+]

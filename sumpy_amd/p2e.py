@@ -1,0 +1,95 @@
+"""Particle-to-expansion operators (reference ``sumpy/p2e.py``):
+:class:`P2EFromSingleBox` (P2M, ``p2e.py:155-261``) and :class:`P2EFromCSR`
+(P2L from list 4, ``p2e.py:268-388``).  Both overwrite the rows of the boxes
+they process.
+"""
+
+from __future__ import annotations
+
+from sumpy_amd import _capi
+from sumpy_amd.expansion.local import LocalExpansionBase
+from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
+from sumpy_amd.tools import (
+    KernelComputation,
+    SourceWeights,
+    as_component_list,
+    as_i32,
+    as_strength_list,
+)
+
+
+class P2EBase(KernelComputation):
+    def __init__(self, expansion, kernels=None, name=None, strength_usage=None):
+        if kernels is None:
+            kernels = [remove_target_transforms(expansion.kernel)]
+        expansion = expansion.with_kernel(
+            remove_source_transforms(remove_target_transforms(expansion.kernel)))
+        for knl in kernels:
+            if remove_source_transforms(knl) != expansion.kernel:
+                raise ValueError(f"kernel {knl} does not match the expansion's kernel")
+        super().__init__([], kernels, strength_usage, None, name)
+        self.expansion = expansion
+        self.dim = expansion.dim
+
+    def get_cache_key(self):
+        return (type(self).__name__, self.name, self.expansion,
+                tuple(self.source_kernels), tuple(self.strength_usage))
+
+    def _common(self, actx, kwargs):
+        sources = as_component_list(kwargs.pop("sources"), self.dim)
+        centers = kwargs.pop("centers")
+        _capi.require_gpu(centers)
+        rscale = float(kwargs.pop("rscale"))
+        strengths = as_strength_list(kwargs.pop("strengths"), self.strength_count)
+        weights = SourceWeights(actx, self.source_kernels, self.strength_usage, strengths,
+                                kwargs, self.dim, self.expansion.is_complex_valued)
+        tgt = kwargs.pop("tgt_expansions")
+        _capi.require_gpu(tgt)
+        if not tgt.is_contiguous():
+            raise ValueError("tgt_expansions must be contiguous")
+        tables = self.expansion.device_tables(actx.device)
+        return sources, centers.contiguous(), rscale, weights, tgt, tables
+
+
+class P2EFromSingleBox(P2EBase):
+    """P2M (reference ``p2e.py:155-261``)."""
+
+    default_name = "p2e_from_single_box"
+
+    def __call__(self, actx, **kwargs):
+        sources, centers, rscale, weights, tgt, tables = self._common(actx, kwargs)
+        source_boxes = as_i32(kwargs.pop("source_boxes"))
+        bss = as_i32(kwargs.pop("box_source_starts"))
+        bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
+        _capi.call("sumpyhip_p2m", kernel_id(self.expansion.kernel), tables.desc,
+                   int(source_boxes.shape[0]), _capi.ptr(source_boxes), _capi.ptr(bss),
+                   _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
+                   *[s.data_ptr() for s in sources], *weights.ptrs(3), rscale,
+                   _capi.ptr(tgt), int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+        return tgt
+
+
+class P2EFromCSR(P2EBase):
+    """P2L from list 4 (reference ``p2e.py:268-388``)."""
+
+    default_name = "p2e_from_csr"
+
+    def __call__(self, actx, **kwargs):
+        if not isinstance(self.expansion, LocalExpansionBase):
+            raise TypeError("P2EFromCSR forms local expansions")
+        sources, centers, rscale, weights, tgt, tables = self._common(actx, kwargs)
+        tb = as_i32(kwargs.pop("target_boxes"))
+        starts = as_i32(kwargs.pop("source_box_starts"))
+        lists = as_i32(kwargs.pop("source_box_lists"))
+        bss = as_i32(kwargs.pop("box_source_starts"))
+        bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
+        _capi.call("sumpyhip_p2l", kernel_id(self.expansion.kernel), tables.desc,
+                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                   _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
+                   int(centers.shape[1]), *[s.data_ptr() for s in sources],
+                   *weights.ptrs(3), rscale, _capi.ptr(tgt),
+                   int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+        return tgt
+
+
+__all__ = ["P2EFromCSR", "P2EFromSingleBox"]

@@ -1,0 +1,105 @@
+"""QBX layer potential (reference ``sumpy/qbx.py:85-335``).
+
+:class:`LayerPotential` is the dense direct QBX sum over all sources for a
+:class:`~sumpy_amd.expansion.local.LineTaylorLocalExpansion` of the 2D Laplace
+kernel: ``result_k[t] = scale sum_s sigma_s sum_{i<=p} coeff_i(a, b)/i!``
+with ``a = c_t - s``, ``b = t - c_t`` (``qbx.py:262-308``).  The HIP kernel
+evaluates the line-Taylor sum in closed complex-analytic form.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from sumpy_amd import _capi
+from sumpy_amd.array_context import obj_array
+from sumpy_amd.expansion.local import LineTaylorLocalExpansion
+from sumpy_amd.kernel import (
+    AxisSourceDerivative,
+    AxisTargetDerivative,
+    DirectionalSourceDerivative,
+    LaplaceKernel,
+)
+from sumpy_amd.tools import KernelComputation, as_component_list, as_strength_list
+
+
+def _src_kind(knl):
+    if isinstance(knl, LaplaceKernel):
+        return 0, None
+    if isinstance(knl, DirectionalSourceDerivative) and isinstance(knl.inner_kernel, LaplaceKernel):
+        return 1, knl.dir_vec_name
+    if isinstance(knl, AxisSourceDerivative) and isinstance(knl.inner_kernel, LaplaceKernel):
+        return 2 + knl.axis, None
+    raise NotImplementedError(f"unsupported QBX source kernel: {knl}")
+
+
+def _tgt_axis(knl):
+    if isinstance(knl, LaplaceKernel):
+        return -1
+    if isinstance(knl, AxisTargetDerivative) and isinstance(knl.inner_kernel, LaplaceKernel):
+        return knl.axis
+    raise NotImplementedError(f"unsupported QBX target kernel: {knl}")
+
+
+class LayerPotentialBase(KernelComputation):
+    def __init__(self, expansion, strength_usage=None, value_dtypes=None, name=None,
+                 device=None, source_kernels=None, target_kernels=None):
+        if source_kernels is None:
+            source_kernels = [expansion.kernel]
+        if target_kernels is None:
+            target_kernels = [expansion.kernel]
+        super().__init__(target_kernels, source_kernels, strength_usage, value_dtypes, name)
+        self.expansion = expansion
+        self.dim = expansion.dim
+        if not isinstance(expansion, LineTaylorLocalExpansion) or self.dim != 2:
+            raise NotImplementedError(
+                "LayerPotential is implemented for the 2D LineTaylorLocalExpansion")
+        for k in source_kernels:
+            _src_kind(k)
+        for k in target_kernels:
+            _tgt_axis(k)
+
+    def get_cache_key(self):
+        return (type(self).__name__, self.expansion, tuple(self.target_kernels),
+                tuple(self.source_kernels), tuple(self.strength_usage))
+
+
+class LayerPotential(LayerPotentialBase):
+    """Direct applier for the layer potential (``qbx.py:250-335``)."""
+
+    default_name = "qbx_apply"
+
+    def __call__(self, actx, targets, sources, centers, strengths, expansion_radii,
+                 **kwargs):
+        tgt = as_component_list(targets, 2)
+        src = as_component_list(sources, 2)
+        ctr = as_component_list(centers, 2)
+        dens = [s.to(torch.float64).contiguous()
+                for s in as_strength_list(strengths, self.strength_count)]
+        nt, ns = tgt[0].shape[0], src[0].shape[0]
+        scale = LaplaceKernel(2).get_global_scaling_const()
+        results = []
+        for tk in self.target_kernels:
+            ax = _tgt_axis(tk)
+            out = torch.zeros(nt, dtype=torch.float64, device=actx.device)
+            tmp = None
+            for j, sk in enumerate(self.source_kernels):
+                kind, dname = _src_kind(sk)
+                nrm = as_component_list(kwargs[dname], 2) if dname else [None, None]
+                dst = out if j == 0 else (tmp if tmp is not None else torch.empty_like(out))
+                tmp = dst if j else None
+                _capi.call("sumpyhip_qbx2d", self.expansion.order, kind, ax, nt,
+                           tgt[0].data_ptr(), tgt[1].data_ptr(), ctr[0].data_ptr(),
+                           ctr[1].data_ptr(), ns, src[0].data_ptr(), src[1].data_ptr(),
+                           _capi.ptr(nrm[0]), _capi.ptr(nrm[1]),
+                           dens[self.strength_usage[j]].data_ptr(), scale, dst.data_ptr(),
+                           actx.stream_handle)
+                if j:
+                    _capi.call("sumpyhip_combine_outputs", 0, nt, 1,
+                               _capi.ptr_array([dst.data_ptr()]), _capi.f64_array([1.0]),
+                               None, out.data_ptr(), 1, actx.stream_handle)
+            results.append(out)
+        return obj_array(results)
+
+
+__all__ = ["LayerPotential", "LayerPotentialBase"]

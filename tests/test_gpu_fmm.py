@@ -1,0 +1,162 @@
+"""End-to-end FMM on the GPU: parity with the oracle-driven FMM (same tree,
+same stages, FP64 to rounding), accuracy against the FP64 direct sum, and the
+reference's FMM identities (``sumpy/test/test_fmm.py``)."""
+
+from functools import partial
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle_fmm import oracle_fmm
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b))/np.linalg.norm(np.asarray(b))
+
+
+def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
+            m2l_precision="fp64", p2p_precision="fp64", targets=None,
+            use_classes=True, target_kernels=None, source_kernels=None,
+            source_extra_kwargs=None, strength_usage=None, tree_trav=None):
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import LaplaceKernel
+    from sumpy_amd.tree import FMMTraversalBuilder, TreeBuilder
+    knl = LaplaceKernel(3)
+    if tree_trav is None:
+        tree, _ = TreeBuilder(actx)(actx, actx.from_numpy(sources),
+                                    targets=None if targets is None else actx.from_numpy(targets),
+                                    max_particles_in_box=max_particles)
+        trav, _ = FMMTraversalBuilder(actx)(actx, tree)
+    else:
+        tree, trav = tree_trav
+    tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
+    tree_indep = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                m2l_translation_override=tr),
+        target_kernels or [knl], exclude_self=exclude_self, source_kernels=source_kernels,
+        strength_usage=strength_usage, p2p_precision=p2p_precision)
+    self_kw = {}
+    if exclude_self:
+        self_kw["target_to_source"] = actx.from_numpy(
+            np.arange(tree.ntargets, dtype=np.int32))
+    wr = SumpyExpansionWrangler(tree_indep, trav, np.float64,
+                                fmm_level_to_order=lambda k, a, t, lev: order,
+                                self_extra_kwargs=self_kw,
+                                source_extra_kwargs=source_extra_kwargs,
+                                _disable_translation_classes=not use_classes)
+    ws = weights if isinstance(weights, (list, tuple)) else [weights]
+    pots = drive_fmm(actx, wr, [actx.from_numpy(w) for w in ws])
+    return [actx.to_numpy(p) for p in pots], tree, trav
+
+
+@pytest.fixture(scope="module")
+def uniform_case():
+    rng = np.random.default_rng(0)
+    n = 4096
+    return rng.random((3, n)), rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_fmm_matches_oracle_fmm(actx, uniform_case, use_classes):
+    src, q = uniform_case
+    order = 6
+    (pot,), tree, trav = run_fmm(actx, src, q, order, use_classes=use_classes)
+    ref = oracle_fmm(tree, trav, q, order, exclude_self=True)
+    assert rel(pot, ref) < 1e-12
+
+
+def test_fmm_clustered_matches_oracle_fmm(actx):
+    """Uneven tree (lists 3 and 4 populated): 8 Gaussian clusters."""
+    rng = np.random.default_rng(1)
+    n = 6000
+    c = rng.uniform(0.1, 0.9, (8, 3))
+    src = np.clip(c[rng.integers(0, 8, n)].T + 0.02*rng.standard_normal((3, n)), 0, 1)
+    q = rng.standard_normal(n)
+    order = 5
+    (pot,), tree, trav = run_fmm(actx, src, q, order, max_particles=20)
+    assert sum(len(x[2]) for x in trav.host["sep_smaller"]) > 0
+    assert len(trav.host["from_sep_bigger_lists"]) > 0
+    ref = oracle_fmm(tree, trav, q, order, exclude_self=True)
+    assert rel(pot, ref) < 1e-12
+
+
+def test_fmm_error_vs_direct_converges(actx, uniform_case):
+    src, q = uniform_case
+    direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(src.shape[1], dtype=np.int32))
+    errs = []
+    for order in [2, 4, 6, 8]:
+        (pot,), _, _ = run_fmm(actx, src, q, order)
+        errs.append(rel(pot, direct))
+    assert all(b < a for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 3e-5
+
+
+def test_fmm_mixed_precision_accuracy(actx, uniform_case):
+    """FP32 M2L + FP32 near field: error within 2x of the FP64 form's (order 8)."""
+    src, q = uniform_case
+    direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(src.shape[1], dtype=np.int32))
+    (p64,), _, _ = run_fmm(actx, src, q, 8)
+    (p32,), _, _ = run_fmm(actx, src, q, 8, m2l_precision="fp32", p2p_precision="fp32")
+    e64, e32 = rel(p64, direct), rel(p32, direct)
+    assert e32 <= 2*e64 + 1e-7, (e64, e32)
+
+
+def test_fmm_source_target_derivative_identity(actx):
+    """Source/target derivative identity of
+    ``test_fmm.py:605-665``: d/dx0 with a plain source equals minus the plain
+    target with a d/dy0 source."""
+    from sumpy_amd.kernel import AxisSourceDerivative, AxisTargetDerivative, LaplaceKernel
+    knl = LaplaceKernel(3)
+    rng = np.random.default_rng(12)
+    src = rng.standard_normal((3, 3000))*0.4
+    q = rng.random(3000)
+    (p0,), tree, trav = run_fmm(actx, src, q, 8, target_kernels=[AxisTargetDerivative(0, knl)],
+                                source_kernels=[knl])
+    (p1,), _, _ = run_fmm(actx, src, q, 8, target_kernels=[knl],
+                          source_kernels=[AxisSourceDerivative(0, knl)],
+                          tree_trav=(tree, trav))
+    assert np.linalg.norm(p0 + p1)/np.linalg.norm(p0) < 1e-5
+    direct, grad = oracle.p2p_dense(3, src, src, q, t2s=np.arange(3000, dtype=np.int32),
+                                    want_grad=True)
+    assert rel(p0, grad[0]) < 1e-5
+
+
+def test_fmm_linearity_single_and_double_layer(actx):
+    """One FMM with strength_usage [0, 1] equals the sum of two FMMs
+    (``test_fmm.py:386-469``)."""
+    from sumpy_amd.kernel import DirectionalSourceDerivative, LaplaceKernel
+    knl = LaplaceKernel(3)
+    dk = DirectionalSourceDerivative(knl, "dir_vec")
+    rng = np.random.default_rng(44)
+    n = 3000
+    src = rng.random((3, n))
+    w = [rng.random(n), rng.random(n)]
+    dirv = rng.standard_normal((3, n))
+    (pa,), tree, trav = run_fmm(actx, src, w[0], 5, exclude_self=True)
+    dkw = {"dir_vec": actx.from_numpy(dirv[:, tree.host["user_source_ids"]])}
+    (pb,), _, _ = run_fmm(actx, src, [w[1]], 5, source_kernels=[dk],
+                          source_extra_kwargs=dkw, tree_trav=(tree, trav))
+    (pc,), _, _ = run_fmm(actx, src, w, 5, source_kernels=[knl, dk], strength_usage=[0, 1],
+                          source_extra_kwargs=dkw, tree_trav=(tree, trav))
+    assert rel(pc, pa + pb) < 1e-12
+
+
+def test_fmm_separate_targets_matches_oracle_fmm(actx):
+    """Sources and targets are different point sets (``test_fmm.py:158-196``)."""
+    rng = np.random.default_rng(15)
+    src = rng.standard_normal((3, 3000))*0.3
+    tgt = rng.standard_normal((3, 1000))*0.3 + np.array([[0.1], [0], [0]])
+    q = rng.random(3000)
+    (pot,), tree, trav = run_fmm(actx, src, q, 6, exclude_self=False, targets=tgt)
+    ref = oracle_fmm(tree, trav, q, 6, exclude_self=False)
+    assert rel(pot, ref) < 1e-12
+    direct, _ = oracle.p2p_dense(3, tgt, src, q)
+    assert rel(pot, direct) < 1e-3

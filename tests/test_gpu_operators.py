@@ -1,0 +1,441 @@
+"""Parity of every GPU operator with the CPU oracle on identical inputs.
+
+Tolerances: FP64 forms agree with the FP64 oracle to rounding (relative
+1e-12 on sums, 1e-11 on expansion coefficients); FP32 forms to 2e-6
+relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm((a - b).ravel())/np.linalg.norm(b.ravel())
+
+
+def dev(actx, a):
+    return actx.from_numpy(np.ascontiguousarray(a))
+
+
+@pytest.fixture(scope="module")
+def lap3d():
+    from sumpy_amd.kernel import LaplaceKernel
+    return LaplaceKernel(3)
+
+
+# {{{ P2P dense
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("exclude_self", [False, True])
+def test_p2p_dense_laplace3d(actx, lap3d, precision, exclude_self):
+    from sumpy_amd import P2P
+    rng = np.random.default_rng(42)
+    n = 700
+    pts = rng.random((3, n))
+    q = rng.standard_normal(n)
+    if exclude_self:
+        tgt, t2s = pts, np.arange(n, dtype=np.int32)
+    else:
+        tgt, t2s = rng.random((3, 500)), None
+    kw = {"target_to_source": dev(actx, t2s)} if exclude_self else {}
+    p2p = P2P([lap3d], exclude_self=exclude_self, precision=precision)
+    pot, = p2p(actx, dev(actx, tgt), dev(actx, pts), [dev(actx, q)], **kw)
+    ref, _ = oracle.p2p_dense(3, tgt, pts, q, t2s=t2s)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert rel(actx.to_numpy(pot), ref) < tol
+
+
+def test_p2p_dense_derivatives(actx, lap3d):
+    """Source and target derivative wrappers (charge + dipole, pot + grad)."""
+    from sumpy_amd import P2P
+    from sumpy_amd.kernel import (
+        AxisSourceDerivative,
+        AxisTargetDerivative,
+        DirectionalSourceDerivative,
+        TargetPointMultiplier,
+    )
+    rng = np.random.default_rng(3)
+    ns, nt = 400, 300
+    src, tgt = rng.random((3, ns)), rng.random((3, nt)) + 1.5
+    q1, q2 = rng.standard_normal(ns), rng.standard_normal(ns)
+    nrm = rng.standard_normal((3, ns))
+    skn = [lap3d, DirectionalSourceDerivative(lap3d, "dir"), AxisSourceDerivative(1, lap3d)]
+    tkn = [lap3d, AxisTargetDerivative(0, lap3d), TargetPointMultiplier(2, lap3d),
+           AxisTargetDerivative(2, TargetPointMultiplier(2, lap3d))]
+    p2p = P2P(tkn, exclude_self=False, source_kernels=skn, strength_usage=[0, 1, 0])
+    res = p2p(actx, dev(actx, tgt), dev(actx, src), [dev(actx, q1), dev(actx, q2)],
+              dir=dev(actx, nrm))
+    # oracle: w0 = q1, dipole w = -q2 n - q1 e_y
+    wd = [-q2*nrm[0], -q2*nrm[1] - q1, -q2*nrm[2]]
+    pot, grad = oracle.p2p_dense(3, tgt, src, q1, wd=wd, want_grad=True)
+    expect = [pot, grad[0], tgt[2]*pot, pot + tgt[2]*grad[2]]
+    for r, e in zip(res, expect):
+        assert rel(actx.to_numpy(r), e) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_p2p_dense_helmholtz3d(actx, precision):
+    from sumpy_amd import P2P
+    from sumpy_amd.kernel import AxisTargetDerivative, HelmholtzKernel
+    knl = HelmholtzKernel(3)
+    rng = np.random.default_rng(5)
+    ns, nt = 500, 400
+    src, tgt = rng.random((3, ns)), rng.random((3, nt))
+    q = rng.standard_normal(ns)
+    p2p = P2P([knl, AxisTargetDerivative(1, knl)], exclude_self=False, precision=precision)
+    pot, gy = p2p(actx, dev(actx, tgt), dev(actx, src), [dev(actx, q)], k=5.0)
+    ref, grad = oracle.p2p_dense(13, tgt, src, q.astype(np.complex128), k=5.0,
+                                 want_grad=True)
+    tol = 1e-12 if precision == "fp64" else 5e-6
+    assert rel(actx.to_numpy(pot), ref) < tol
+    assert rel(actx.to_numpy(gy), grad[1]) < tol
+
+
+def test_p2p_dense_laplace2d(actx):
+    from sumpy_amd import P2P
+    from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative, LaplaceKernel
+    knl = LaplaceKernel(2)
+    rng = np.random.default_rng(6)
+    src, tgt = rng.random((2, 300)), rng.random((2, 200)) + 1.2
+    q = rng.standard_normal(300)
+    nrm = rng.standard_normal((2, 300))
+    p2p = P2P([knl, AxisTargetDerivative(0, knl)], exclude_self=False,
+              source_kernels=[DirectionalSourceDerivative(knl, "n")])
+    pot, gx = p2p(actx, dev(actx, tgt), dev(actx, src), [dev(actx, q)], n=dev(actx, nrm))
+    ref, grad = oracle.p2p_dense(2, tgt, src, np.zeros(300), wd=[-q*nrm[0], -q*nrm[1]],
+                                 want_grad=True)
+    assert rel(actx.to_numpy(pot), ref) < 1e-12
+    assert rel(actx.to_numpy(gx), grad[0]) < 1e-12
+
+# }}}
+
+
+# {{{ tree fixture
+
+@pytest.fixture(scope="module")
+def small_tree(actx):
+    from sumpy_amd.tree import build_tree_and_traversal
+    rng = np.random.default_rng(15)
+    src = rng.standard_normal((3, 3000))*0.3
+    tree, trav = build_tree_and_traversal(actx, src, None, max_particles_in_box=30)
+    return tree, trav
+
+
+def _level_sizes(tree):
+    return tree.host["level_start_box_nrs"]
+
+# }}}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_p2p_csr(actx, lap3d, small_tree, precision):
+    from sumpy_amd import P2PFromCSR
+    tree, trav = small_tree
+    th, vh = tree.host, trav.host
+    rng = np.random.default_rng(1)
+    q = rng.standard_normal(th["sources"].shape[1])
+    t2s = np.arange(th["targets"].shape[1], dtype=np.int32)
+    p2p = P2PFromCSR([lap3d], exclude_self=True, precision=precision)
+    pot, = p2p(actx, targets=tree.targets, sources=tree.sources,
+               target_boxes=trav.target_boxes,
+               source_box_starts=trav.neighbor_source_boxes_starts,
+               source_box_lists=trav.neighbor_source_boxes_lists,
+               box_source_starts=tree.box_source_starts,
+               box_source_counts_nonchild=tree.box_source_counts_nonchild,
+               box_target_starts=tree.box_target_starts,
+               box_target_counts_nonchild=tree.box_target_counts_nonchild,
+               strength=[dev(actx, q)], max_nsources_in_one_box=30,
+               max_ntargets_in_one_box=30, target_to_source=dev(actx, t2s))
+    ref, _ = oracle.p2p_csr(3, th, vh["target_boxes"], vh["neighbor_source_boxes_starts"],
+                            vh["neighbor_source_boxes_lists"], th["targets"], th["sources"],
+                            q, t2s=t2s)
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert rel(actx.to_numpy(pot), ref) < tol
+
+
+# {{{ expansions
+
+def _rscale(tree, lev, order):
+    return tree.root_extent*2**-lev*2/order
+
+
+@pytest.mark.parametrize("order", [3, 6, 10])
+def test_p2m(actx, lap3d, small_tree, order):
+    from sumpy_amd import P2EFromSingleBox
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    tree, trav = small_tree
+    th, vh = tree.host, trav.host
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal(th["sources"].shape[1])
+    expn = LinearPDEConformingVolumeTaylorMultipoleExpansion(lap3d, order)
+    lev = tree.nlevels - 1
+    lsb = vh["level_start_source_box_nrs"]
+    lbs = th["level_start_box_nrs"]
+    sb = vh["source_boxes"][lsb[lev]:lsb[lev + 1]]
+    nbl = lbs[lev + 1] - lbs[lev]
+    rs = _rscale(tree, lev, order)
+    out = torch.zeros((nbl, len(expn)), dtype=torch.float64, device=actx.device)
+    P2EFromSingleBox(expn)(actx, source_boxes=dev(actx, sb),
+                           box_source_starts=tree.box_source_starts,
+                           box_source_counts_nonchild=tree.box_source_counts_nonchild,
+                           centers=tree.box_centers, sources=tree.sources,
+                           strengths=[dev(actx, q)], tgt_expansions=out,
+                           tgt_base_ibox=int(lbs[lev]), rscale=rs)
+    ref = np.zeros((nbl, len(expn)))
+    oracle.p2m(order, sb, th, th["box_centers"], th["sources"], q, None, rs, ref, int(lbs[lev]))
+    assert rel(actx.to_numpy(out), ref) < 1e-12
+
+
+def _random_expansions(rng, nboxes, ncoeff):
+    return rng.standard_normal((nboxes, ncoeff))
+
+
+@pytest.mark.parametrize("order", [4, 10])
+def test_m2m_l2l(actx, lap3d, small_tree, order):
+    from sumpy_amd import E2EFromChildren, E2EFromParent
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    tree, trav = small_tree
+    th, vh = tree.host, trav.host
+    rng = np.random.default_rng(4)
+    lbs = th["level_start_box_nrs"]
+    mp = LinearPDEConformingVolumeTaylorMultipoleExpansion(lap3d, order)
+    lc = LinearPDEConformingVolumeTaylorLocalExpansion(lap3d, order)
+    tl = tree.nlevels - 2
+    sl = tl + 1
+    n_t, n_s = lbs[tl + 1] - lbs[tl], lbs[sl + 1] - lbs[sl]
+    src = _random_expansions(rng, n_s, len(mp))
+    tgt0 = _random_expansions(rng, n_t, len(mp))
+    parents = vh["source_parent_boxes"]
+    lsp = vh["level_start_source_parent_box_nrs"]
+    pb = parents[lsp[tl]:lsp[tl + 1]]
+    rs_s, rs_t = _rscale(tree, sl, order), _rscale(tree, tl, order)
+    out = dev(actx, tgt0.copy())
+    E2EFromChildren(mp, mp)(actx, src_expansions=dev(actx, src), src_base_ibox=int(lbs[sl]),
+                            tgt_expansions=out, tgt_base_ibox=int(lbs[tl]),
+                            target_boxes=dev(actx, pb), box_child_ids=tree.box_child_ids,
+                            centers=tree.box_centers, src_rscale=rs_s, tgt_rscale=rs_t)
+    ref = tgt0.copy()
+    oracle.m2m(order, order, pb, th["box_child_ids"], th["box_centers"], rs_s, rs_t, src,
+               int(lbs[sl]), ref, int(lbs[tl]))
+    assert rel(actx.to_numpy(out), ref) < 1e-11
+
+    # L2L from level tl to level sl
+    lsrc = _random_expansions(rng, n_t, len(lc))
+    ltgt0 = _random_expansions(rng, n_s, len(lc))
+    totp = vh["target_or_target_parent_boxes"]
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    cb = totp[lst[sl]:lst[sl + 1]]
+    out = dev(actx, ltgt0.copy())
+    E2EFromParent(lc, lc)(actx, src_expansions=dev(actx, lsrc), src_base_ibox=int(lbs[tl]),
+                          tgt_expansions=out, tgt_base_ibox=int(lbs[sl]),
+                          target_boxes=dev(actx, cb), box_parent_ids=tree.box_parent_ids,
+                          centers=tree.box_centers, src_rscale=rs_t, tgt_rscale=rs_s)
+    ref = ltgt0.copy()
+    oracle.l2l(order, order, cb, th["box_parent_ids"], th["box_centers"], rs_t, rs_s, lsrc,
+               int(lbs[tl]), ref, int(lbs[sl]))
+    assert rel(actx.to_numpy(out), ref) < 1e-11
+
+
+@pytest.mark.parametrize("order", [5, 10])
+def test_l2p_m2p_p2l(actx, lap3d, small_tree, order):
+    from sumpy_amd import E2PFromCSR, E2PFromSingleBox, P2EFromCSR
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.kernel import AxisTargetDerivative
+    tree, trav = small_tree
+    th, vh = tree.host, trav.host
+    rng = np.random.default_rng(7)
+    lbs = th["level_start_box_nrs"]
+    mp = LinearPDEConformingVolumeTaylorMultipoleExpansion(lap3d, order)
+    lc = LinearPDEConformingVolumeTaylorLocalExpansion(lap3d, order)
+    nt = th["targets"].shape[1]
+    tkn = [lap3d, AxisTargetDerivative(0, lap3d), AxisTargetDerivative(1, lap3d),
+           AxisTargetDerivative(2, lap3d)]
+
+    # L2P at the finest level
+    lev = tree.nlevels - 1
+    L = _random_expansions(rng, lbs[lev + 1] - lbs[lev], len(lc))
+    ltb = vh["level_start_target_box_nrs"]
+    tb = vh["target_boxes"][ltb[lev]:ltb[lev + 1]]
+    rs = _rscale(tree, lev, order)
+    res = E2PFromSingleBox(lc, tkn)(
+        actx, src_expansions=dev(actx, L), src_base_ibox=int(lbs[lev]),
+        target_boxes=dev(actx, tb), centers=tree.box_centers,
+        result=[torch.zeros(nt, dtype=torch.float64, device=actx.device) for _ in tkn],
+        rscale=rs, box_target_starts=tree.box_target_starts,
+        box_target_counts_nonchild=tree.box_target_counts_nonchild, targets=tree.targets)
+    pot, grad = np.zeros(nt), [np.zeros(nt) for _ in range(3)]
+    oracle.l2p(order, tb, th, th["box_centers"], th["targets"], rs, L, int(lbs[lev]), pot,
+               grad)
+    for r, e in zip(res, [pot, *grad]):
+        assert rel(actx.to_numpy(r), e) < 1e-11
+
+    # M2P over list 3, all source levels
+    pot_gpu = [torch.zeros(nt, dtype=torch.float64, device=actx.device) for _ in tkn]
+    pot, grad = np.zeros(nt), [np.zeros(nt) for _ in range(3)]
+    found = False
+    for slev, (tbs, st, li) in enumerate(vh["sep_smaller"]):
+        if len(tbs) == 0:
+            continue
+        found = True
+        M = _random_expansions(rng, lbs[slev + 1] - lbs[slev], len(mp))
+        rs = _rscale(tree, slev, order)
+        E2PFromCSR(mp, tkn)(
+            actx, src_expansions=dev(actx, M), src_base_ibox=int(lbs[slev]),
+            target_boxes=dev(actx, tbs), source_box_starts=dev(actx, st),
+            source_box_lists=dev(actx, li), centers=tree.box_centers, result=pot_gpu,
+            rscale=rs, box_target_starts=tree.box_target_starts,
+            box_target_counts_nonchild=tree.box_target_counts_nonchild, targets=tree.targets)
+        oracle.m2p(order, tbs, st, li, th, th["box_centers"], th["targets"], rs, M,
+                   int(lbs[slev]), pot, grad)
+    assert found
+    for r, e in zip(pot_gpu, [pot, *grad]):
+        assert rel(actx.to_numpy(r), e) < 1e-11
+
+    # P2L over list 4, every level with entries
+    q = rng.standard_normal(th["sources"].shape[1])
+    totp = vh["target_or_target_parent_boxes"]
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    starts, lists = vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"]
+    checked = 0
+    for lev in range(tree.nlevels):
+        s0, s1 = lst[lev], lst[lev + 1]
+        if starts[s1] == starts[s0]:
+            continue
+        nbl = lbs[lev + 1] - lbs[lev]
+        rs = _rscale(tree, lev, order)
+        out = torch.zeros((nbl, len(lc)), dtype=torch.float64, device=actx.device)
+        P2EFromCSR(lc)(actx, target_boxes=dev(actx, totp[s0:s1]),
+                       source_box_starts=dev(actx, starts[s0:s1 + 1]),
+                       source_box_lists=dev(actx, lists), centers=tree.box_centers,
+                       sources=tree.sources, strengths=[dev(actx, q)],
+                       box_source_starts=tree.box_source_starts,
+                       box_source_counts_nonchild=tree.box_source_counts_nonchild,
+                       tgt_expansions=out, tgt_base_ibox=int(lbs[lev]), rscale=rs)
+        ref = np.zeros((nbl, len(lc)))
+        oracle.p2l(order, totp[s0:s1], starts[s0:s1 + 1], lists, th, th["box_centers"],
+                   th["sources"], q, None, rs, ref, int(lbs[lev]))
+        assert rel(actx.to_numpy(out), ref) < 1e-11
+        checked += 1
+    assert checked
+
+# }}}
+
+
+# {{{ M2L
+
+@pytest.mark.parametrize("order", [3, 6, 10])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_m2l_translation_classes(actx, lap3d, small_tree, order, precision):
+    from sumpy_amd import (
+        E2EFromCSR,
+        M2LGenerateTranslationClassesDependentData,
+        M2LPostprocessLocal,
+        M2LPreprocessMultipole,
+        M2LUsingTranslationClassesDependentData,
+    )
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    tree, trav = small_tree
+    th, vh = tree.host, trav.host
+    rng = np.random.default_rng(11)
+    tr = VolumeTaylorM2LWithDenseClassMatrices(precision)
+    mp = LinearPDEConformingVolumeTaylorMultipoleExpansion(lap3d, order)
+    lc = LinearPDEConformingVolumeTaylorLocalExpansion(lap3d, order,
+                                                       m2l_translation_override=tr)
+    lbs = th["level_start_box_nrs"]
+    lev = tree.nlevels - 1
+    nbl = lbs[lev + 1] - lbs[lev]
+    cls_ls = vh["from_sep_siblings_translation_classes_level_starts"]
+    c0, c1 = int(cls_ls[lev]), int(cls_ls[lev + 1])
+    vecs = vh["from_sep_siblings_translation_class_to_distance_vector"]
+    rs = _rscale(tree, lev, order)
+
+    data = torch.zeros((c1 - c0, tr.translation_classes_dependent_ndata(lc, mp)),
+                       dtype=torch.float64, device=actx.device)
+    M2LGenerateTranslationClassesDependentData(mp, lc)(
+        actx, src_rscale=rs, translation_classes_level_start=c0, ntranslation_classes=c1 - c0,
+        m2l_translation_classes_dependent_data=data,
+        m2l_translation_vectors=dev(actx, vecs), ntranslation_vectors=vecs.shape[1])
+    ref_data = oracle.m2l_class_data(order, order, vecs[:, c0:c1], rs)
+    assert rel(actx.to_numpy(data), ref_data) < 1e-12
+
+    M = _random_expansions(rng, nbl, len(mp))
+    pdt = torch.float64 if precision == "fp64" else torch.float32
+    pre = torch.zeros((nbl, tr.preprocess_multipole_nexprs(lc, mp)), dtype=pdt,
+                      device=actx.device)
+    M2LPreprocessMultipole(mp, lc)(actx, src_expansions=dev(actx, M),
+                                   preprocessed_src_expansions=pre, src_rscale=rs)
+    work = torch.zeros((nbl, tr.postprocess_local_nexprs(lc, mp)), dtype=pdt,
+                       device=actx.device)
+    totp = vh["target_or_target_parent_boxes"]
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    s0, s1 = lst[lev], lst[lev + 1]
+    M2LUsingTranslationClassesDependentData(mp, lc)(
+        actx, src_expansions=pre, src_base_ibox=int(lbs[lev]), tgt_expansions=work,
+        tgt_base_ibox=int(lbs[lev]), target_boxes=dev(actx, totp[s0:s1]),
+        src_box_starts=dev(actx, vh["from_sep_siblings_starts"][s0:s1 + 1]),
+        src_box_lists=dev(actx, vh["from_sep_siblings_lists"]), centers=tree.box_centers,
+        src_rscale=rs, tgt_rscale=rs, m2l_translation_classes_dependent_data=data,
+        translation_classes_level_start=c0,
+        m2l_translation_classes_lists=dev(actx, vh["from_sep_siblings_translation_classes"]))
+    L = torch.zeros((nbl, len(lc)), dtype=torch.float64, device=actx.device)
+    M2LPostprocessLocal(mp, lc)(actx, tgt_expansions=L, tgt_expansions_before_postprocessing=work,
+                                src_rscale=rs, tgt_rscale=rs)
+    ref = np.zeros((nbl, len(lc)))
+    oracle.m2l(order, order, totp[s0:s1], vh["from_sep_siblings_starts"][s0:s1 + 1],
+               vh["from_sep_siblings_lists"], vh["from_sep_siblings_translation_classes"], c0,
+               ref_data, rs, rs, M, int(lbs[lev]), ref, int(lbs[lev]))
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert rel(actx.to_numpy(L), ref) < tol
+
+    # M2L without translation classes
+    L2 = torch.zeros((nbl, len(lc)), dtype=torch.float64, device=actx.device)
+    E2EFromCSR(mp, lc)(actx, src_expansions=dev(actx, M), src_base_ibox=int(lbs[lev]),
+                       tgt_expansions=L2, tgt_base_ibox=int(lbs[lev]),
+                       target_boxes=dev(actx, totp[s0:s1]),
+                       src_box_starts=dev(actx, vh["from_sep_siblings_starts"][s0:s1 + 1]),
+                       src_box_lists=dev(actx, vh["from_sep_siblings_lists"]),
+                       centers=tree.box_centers, src_rscale=rs, tgt_rscale=rs)
+    assert rel(actx.to_numpy(L2), ref) < 1e-12
+
+# }}}
+
+
+# {{{ QBX
+
+@pytest.mark.parametrize("src_kind,tgt_axis", [(0, -1), (1, -1), (0, 0), (1, 1)])
+def test_qbx2d(actx, src_kind, tgt_axis):
+    from sumpy_amd.expansion.local import LineTaylorLocalExpansion
+    from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative, LaplaceKernel
+    from sumpy_amd.qbx import LayerPotential
+    knl = LaplaceKernel(2)
+    n = 512
+    t = np.linspace(0, 2*np.pi, n, endpoint=False)
+    r = 1 + 0.25*np.cos(5*t)
+    dr = -1.25*np.sin(5*t)
+    pts = np.array([r*np.cos(t), r*np.sin(t)])
+    tang = np.array([dr*np.cos(t) - r*np.sin(t), dr*np.sin(t) + r*np.cos(t)])
+    nrm = np.array([tang[1], -tang[0]])/np.linalg.norm(tang, axis=0)
+    h = 2*np.pi/n*np.linalg.norm(tang, axis=0)
+    ctr = pts - 2*h*nrm
+    rng = np.random.default_rng(0)
+    dens = np.sin(3*t) + 0.01*rng.standard_normal(n)
+    skn = knl if src_kind == 0 else DirectionalSourceDerivative(knl, "nrm")
+    tkn = knl if tgt_axis < 0 else AxisTargetDerivative(tgt_axis, knl)
+    lp = LayerPotential(LineTaylorLocalExpansion(knl, 6), source_kernels=[skn],
+                        target_kernels=[tkn])
+    out, = lp(actx, dev(actx, pts), dev(actx, pts), dev(actx, ctr), [dev(actx, dens)],
+              dev(actx, np.full(n, 2*h.mean())), nrm=dev(actx, nrm))
+    ref = oracle.qbx2d(6, src_kind, tgt_axis, pts, ctr, np.full(n, 1.0), pts, nrm, dens)
+    assert rel(actx.to_numpy(out), ref) < 1e-12
+
+# }}}
