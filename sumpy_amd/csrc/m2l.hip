@@ -159,53 +159,143 @@ struct ApplyArgs {
   int tbase;
 };
 
+int set_lds_limit(const void *fn, size_t bytes);
+
+// 16-byte vector type of each compute precision (global loads of X)
 template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  typedef double2 type;
+  static constexpr int N = 2;
+};
+template <>
+struct Vec16<float> {
+  typedef float4 type;
+  static constexpr int N = 4;
+};
+
+// Per-instance limits: RBW row blocks of 16 per wave (npad <= 64 RBW); the
+// largest PDE-compressed order that fits sets the class-data size.
+template <int RBW>
+struct Limits {
+  static constexpr int PMAX = RBW == 1 ? 7 : RBW == 2 ? 10 : RBW == 3 ? 12 : 15;
+  static constexpr int MAXK = 64*RBW;
+  static constexpr int MAXCIRC = 3*(2*PMAX + 1)*(2*PMAX + 1);
+  static constexpr int DPT = (MAXCIRC + 255)/256;   // class-data values per thread
+};
+
+template <typename T, int RBW>
+struct Apply {
+  static constexpr int VN = Vec16<T>::N;
+  static constexpr int XS = TW + (sizeof(T) == 4 ? 16 : 8);   // padded LDS row of X
+  static constexpr int XPT = (Limits<RBW>::MAXK/VN*TW + 255)/256;  // X vectors per thread
+  typedef typename Vec16<T>::type vec_t;
+};
+
+// One block per tile (4 waves), persistent over tiles.  Per class slot:
+//   stage   : data_c (ncirc) and X (kpad x 64, LDS row stride XS) in LDS;
+//   compute : wave w owns row blocks w, w+4, ... and all four 16-column
+//             blocks; per 4-deep k step it reads 4 B fragments and gathers
+//             one A fragment per row block from data_c.
+// The next slot's data are loaded into registers before the MFMA loop of the
+// current slot, so global-memory latency overlaps the contraction; LDS is
+// rewritten between two barriers.
+template <typename T, int RBW>
 __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   typedef typename Mfma<T>::acc_t acc_t;
+  typedef Apply<T, RBW> C;
+  typedef typename C::vec_t vec_t;
+  constexpr int DPT = Limits<RBW>::DPT;
   extern __shared__ unsigned char lds_raw[];
-  T *sD = reinterpret_cast<T *>(lds_raw);                  // [ncirc]
-  T *sX = sD + a.ncirc;                                    // [kpad][TW]
-  int *sOff = reinterpret_cast<int *>(sX + (size_t)a.kpad*TW);  // [kpad]
+  T *sD = reinterpret_cast<T *>(lds_raw);                       // [ncirc]
+  T *sX = sD + ((a.ncirc + 3) & ~3);                            // [kpad][XS]
+  int *sOff = reinterpret_cast<int *>(sX + (size_t)a.kpad*C::XS);  // [kpad]
   const T *packed = reinterpret_cast<const T *>(a.packed);
   T *work = reinterpret_cast<T *>(a.work);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int nrb = a.npad >> 4;
+  const int kq = a.kpad/C::VN;          // vectors per source row
+  const int nxv = kq*TW;                // vectors per X tile
   for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : -1;
-  int offT[MAXRBW];
-  for (int r = 0; r < MAXRBW; ++r) {
+  int offT[RBW];
+  for (int r = 0; r < RBW; ++r) {
     const int row = (wave + 4*r)*16 + (lane & 15);
     offT[r] = (row < a.nl) ? a.off_t[row] : -1;
   }
-  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    acc_t acc[MAXRBW][4];
-    for (int r = 0; r < MAXRBW; ++r)
-      for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
-    for (int slot = a.tile_slot_starts[tile]; slot < a.tile_slot_starts[tile + 1]; ++slot) {
-      const int cls = a.slot_class[slot];
-      const int *srcs = a.slot_sources + (size_t)slot*TW;
-      __syncthreads();
-      const double *D = a.class_data + (size_t)cls*a.ncirc;
-      for (int i = tid; i < a.ncirc; i += 256) sD[i] = T(D[i]);
-      for (int idx = tid; idx < a.kpad*TW; idx += 256) {
-        const int col = idx/a.kpad, k = idx - col*a.kpad;
+
+  double dreg[DPT];
+  vec_t xreg[C::XPT];
+  // global -> registers for one slot
+  auto fetch = [&](int slot) {
+    const double *D = a.class_data + (size_t)a.slot_class[slot]*a.ncirc;
+    for (int i = 0; i < DPT; ++i) {
+      const int e = tid + 256*i;
+      dreg[i] = e < a.ncirc ? D[e] : 0.0;
+    }
+    const int *srcs = a.slot_sources + (size_t)slot*TW;
+    for (int i = 0; i < C::XPT; ++i) {
+      const int v = tid + 256*i;
+      vec_t x = {};
+      if (v < nxv) {
+        const int col = v & (TW - 1), q = v >> 6;
         const int sb = srcs[col];
-        sX[k*TW + col] = sb >= 0 ? packed[(size_t)(sb - a.sbase)*a.kpad + k] : T(0);
+        if (sb >= 0)
+          x = *reinterpret_cast<const vec_t *>(packed + (size_t)(sb - a.sbase)*a.kpad + q*C::VN);
       }
+      xreg[i] = x;
+    }
+  };
+  // registers -> LDS
+  auto stage = [&]() {
+    for (int i = 0; i < DPT; ++i) {
+      const int e = tid + 256*i;
+      if (e < a.ncirc) sD[e] = T(dreg[i]);
+    }
+    for (int i = 0; i < C::XPT; ++i) {
+      const int v = tid + 256*i;
+      if (v < nxv) {
+        const int col = v & (TW - 1), q = v >> 6;
+        const T *xv = reinterpret_cast<const T *>(&xreg[i]);
+        for (int j = 0; j < C::VN; ++j) sX[(q*C::VN + j)*C::XS + col] = xv[j];
+      }
+    }
+  };
+
+  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    acc_t acc[RBW][4];
+    for (int r = 0; r < RBW; ++r)
+      for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
+    const int s0 = a.tile_slot_starts[tile], s1 = a.tile_slot_starts[tile + 1];
+    if (s0 < s1) {
+      fetch(s0);
       __syncthreads();
+      stage();
+      __syncthreads();
+    }
+    for (int slot = s0; slot < s1; ++slot) {
+      if (slot + 1 < s1) fetch(slot + 1);
+      #pragma unroll 2
       for (int k0 = 0; k0 < a.kpad; k0 += 4) {
         const int kk = k0 + (lane >> 4);
         const int os = sOff[kk];
+        const T *xr = sX + kk*C::XS + (lane & 15);
         T bfr[4];
-        for (int cb = 0; cb < 4; ++cb) bfr[cb] = sX[kk*TW + cb*16 + (lane & 15)];
-        for (int r = 0; r < MAXRBW; ++r) {
+        for (int cb = 0; cb < 4; ++cb) bfr[cb] = xr[cb*16];
+        for (int r = 0; r < RBW; ++r) {
           if (wave + 4*r >= nrb) break;
           const T af = (offT[r] >= 0 && os >= 0) ? sD[offT[r] + os] : T(0);
           for (int cb = 0; cb < 4; ++cb) acc[r][cb] = Mfma<T>::mma(af, bfr[cb], acc[r][cb]);
         }
       }
+      __syncthreads();
+      if (slot + 1 < s1) {
+        stage();
+        __syncthreads();
+      }
     }
     // epilogue: each lane holds four rows of one column of every block
-    for (int r = 0; r < MAXRBW; ++r) {
+    for (int r = 0; r < RBW; ++r) {
       const int rb = wave + 4*r;
       if (rb >= nrb) break;
       for (int cb = 0; cb < 4; ++cb) {
@@ -222,7 +312,32 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   }
 }
 
-// }}}
+template <typename T, int RBW>
+int launch_apply(const ApplyArgs &a, hipStream_t st) {
+  const size_t lds = sizeof(T)*((size_t)((a.ncirc + 3) & ~3) + (size_t)a.kpad*Apply<T, RBW>::XS) +
+                     sizeof(int)*a.kpad;
+  SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
+  int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW>, lds);
+  if (rc) return rc;
+  m2l_apply_kernel<T, RBW><<<a.ntiles, 256, lds, st>>>(a);
+  return check_launch("m2l_apply");
+}
+
+template <typename T>
+int dispatch_apply(const ApplyArgs &a, hipStream_t st) {
+  const int nrb = a.npad/16;
+  if (nrb <= 4 && a.kpad <= Limits<1>::MAXK && a.ncirc <= Limits<1>::MAXCIRC)
+    return launch_apply<T, 1>(a, st);
+  if (nrb <= 8 && a.kpad <= Limits<2>::MAXK && a.ncirc <= Limits<2>::MAXCIRC)
+    return launch_apply<T, 2>(a, st);
+  if (nrb <= 12 && a.kpad <= Limits<3>::MAXK && a.ncirc <= Limits<3>::MAXCIRC)
+    return launch_apply<T, 3>(a, st);
+  if (nrb <= 16 && a.kpad <= Limits<4>::MAXK && a.ncirc <= Limits<4>::MAXCIRC)
+    return launch_apply<T, 4>(a, st);
+  return set_error(SUMPYHIP_ENOTIMPL,
+                   "m2l_apply: npad %d / kpad %d / ncirc %d exceed the supported sizes",
+                   a.npad, a.kpad, a.ncirc);
+}
 
 // {{{ M2L without translation classes: derivatives per pair, LDS matvec
 
@@ -265,7 +380,6 @@ __global__ __launch_bounds__(256) void m2l_csr_kernel(
 
 // }}}
 
-template <typename T>
 int set_lds_limit(const void *fn, size_t bytes) {
   if (bytes <= 64*1024) return SUMPYHIP_OK;
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -338,29 +452,16 @@ extern "C" int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_tar
                                   int src_base_ibox, void *work, int tgt_base_ibox,
                                   void *stream) {
   SUMPY_REQUIRE(precision == 0 || precision == 1, "m2l_apply: unknown precision %d", precision);
-  SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl && npad <= 16*4*MAXRBW,
-                "m2l_apply: npad %d must be a multiple of 16 in [nl, %d]", npad, 64*MAXRBW);
+  SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl, "m2l_apply: npad %d must be a multiple of 16 >= nl",
+                npad);
   SUMPY_REQUIRE(kpad % 4 == 0 && kpad >= nm, "m2l_apply: kpad %d must be a multiple of 4 >= nm",
                 kpad);
   if (ntiles == 0) return SUMPYHIP_OK;
   ApplyArgs a{ntiles, tile_targets, tile_slot_starts, slot_class, slot_sources, nl, nm,
               npad, kpad, off_t, off_s, ncirc, class_data, packed_src, src_base_ibox,
               work, tgt_base_ibox};
-  const size_t esz = precision == 0 ? sizeof(double) : sizeof(float);
-  const size_t lds = esz*((size_t)ncirc + (size_t)kpad*TW) + sizeof(int)*kpad;
-  SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
-  const int grid = ntiles;
-  int rc;
-  if (precision == 0) {
-    rc = set_lds_limit<double>((const void *)m2l_apply_kernel<double>, lds);
-    if (rc) return rc;
-    m2l_apply_kernel<double><<<grid, 256, lds, (hipStream_t)stream>>>(a);
-  } else {
-    rc = set_lds_limit<float>((const void *)m2l_apply_kernel<float>, lds);
-    if (rc) return rc;
-    m2l_apply_kernel<float><<<grid, 256, lds, (hipStream_t)stream>>>(a);
-  }
-  return check_launch("m2l_apply");
+  hipStream_t st = (hipStream_t)stream;
+  return precision == 0 ? dispatch_apply<double>(a, st) : dispatch_apply<float>(a, st);
 }
 
 extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
@@ -377,7 +478,7 @@ extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
   const size_t ncirc = (size_t)circ_shape[0]*circ_shape[1]*circ_shape[2];
   const size_t lds = sizeof(double)*(ncirc + nm);
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_csr: %zu bytes of LDS needed (order too high)", lds);
-  int rc = set_lds_limit<double>((const void *)m2l_csr_kernel, lds);
+  int rc = set_lds_limit((const void *)m2l_csr_kernel, lds);
   if (rc) return rc;
   const int grid = ntgt_boxes < 8192 ? ntgt_boxes : 8192;
   m2l_csr_kernel<<<grid, 256, lds, (hipStream_t)stream>>>(
