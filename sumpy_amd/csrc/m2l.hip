@@ -213,38 +213,37 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   int *sOff = reinterpret_cast<int *>(sX + (size_t)a.kpad*C::XS);  // [kpad]
   const T *packed = reinterpret_cast<const T *>(a.packed);
   T *work = reinterpret_cast<T *>(a.work);
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int nrb = a.npad >> 4;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kq = a.kpad/C::VN;          // vectors per source row
-  const int nxv = kq*TW;                // vectors per X tile
-  for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : -1;
+  // Padding rows (>= nl) and columns (>= nm) read an arbitrary valid entry of
+  // data_c: the matching X rows are zero (preprocessing pads with zeros) and
+  // the padding output rows are never read back.
+  for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : 0;
   int offT[RBW];
   for (int r = 0; r < RBW; ++r) {
     const int row = (wave + 4*r)*16 + (lane & 15);
-    offT[r] = (row < a.nl) ? a.off_t[row] : -1;
+    offT[r] = (row < a.nl) ? a.off_t[row] : 0;
   }
 
   double dreg[DPT];
   vec_t xreg[C::XPT];
-  // global -> registers for one slot
+  // global -> registers for one slot; every load is unconditional (clamped
+  // address, then a select) so all of them are in flight together
   auto fetch = [&](int slot) {
     const double *D = a.class_data + (size_t)a.slot_class[slot]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      dreg[i] = e < a.ncirc ? D[e] : 0.0;
+      dreg[i] = D[e < a.ncirc ? e : 0];
     }
-    const int *srcs = a.slot_sources + (size_t)slot*TW;
+    const int sb = a.slot_sources[(size_t)slot*TW + lane];
+    const T *row = packed + (size_t)((sb >= 0 ? sb : a.sbase) - a.sbase)*a.kpad;
     for (int i = 0; i < C::XPT; ++i) {
-      const int v = tid + 256*i;
-      vec_t x = {};
-      if (v < nxv) {
-        const int col = v & (TW - 1), q = v >> 6;
-        const int sb = srcs[col];
-        if (sb >= 0)
-          x = *reinterpret_cast<const vec_t *>(packed + (size_t)(sb - a.sbase)*a.kpad + q*C::VN);
-      }
-      xreg[i] = x;
+      const int q = wave + 4*i;
+      xreg[i] = *reinterpret_cast<const vec_t *>(row + (q < kq ? q : 0)*C::VN);
     }
+    if (sb < 0)
+      for (int i = 0; i < C::XPT; ++i) xreg[i] = vec_t{};
   };
   // registers -> LDS
   auto stage = [&]() {
@@ -253,15 +252,16 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
       if (e < a.ncirc) sD[e] = T(dreg[i]);
     }
     for (int i = 0; i < C::XPT; ++i) {
-      const int v = tid + 256*i;
-      if (v < nxv) {
-        const int col = v & (TW - 1), q = v >> 6;
+      const int q = wave + 4*i;
+      if (q < kq) {
         const T *xv = reinterpret_cast<const T *>(&xreg[i]);
-        for (int j = 0; j < C::VN; ++j) sX[(q*C::VN + j)*C::XS + col] = xv[j];
+        for (int j = 0; j < C::VN; ++j) sX[(q*C::VN + j)*C::XS + lane] = xv[j];
       }
     }
   };
 
+  const T *xlane = sX + (lane >> 4)*C::XS + (lane & 15);
+  const int *olane = sOff + (lane >> 4);
   for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     acc_t acc[RBW][4];
     for (int r = 0; r < RBW; ++r)
@@ -275,16 +275,21 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
     }
     for (int slot = s0; slot < s1; ++slot) {
       if (slot + 1 < s1) fetch(slot + 1);
+      int os_next = olane[0];
       #pragma unroll 2
       for (int k0 = 0; k0 < a.kpad; k0 += 4) {
-        const int kk = k0 + (lane >> 4);
-        const int os = sOff[kk];
-        const T *xr = sX + kk*C::XS + (lane & 15);
+        // the column offset of the next k step is loaded one step ahead so
+        // that the A gather does not wait on a dependent LDS read
+        const int os = os_next;
+        os_next = olane[k0 + 4 < a.kpad ? k0 + 4 : 0];
+        const T *xr = xlane + k0*C::XS;
         T bfr[4];
+        #pragma unroll
         for (int cb = 0; cb < 4; ++cb) bfr[cb] = xr[cb*16];
+        #pragma unroll
         for (int r = 0; r < RBW; ++r) {
-          if (wave + 4*r >= nrb) break;
-          const T af = (offT[r] >= 0 && os >= 0) ? sD[offT[r] + os] : T(0);
+          const T af = sD[offT[r] + os];
+          #pragma unroll
           for (int cb = 0; cb < 4; ++cb) acc[r][cb] = Mfma<T>::mma(af, bfr[cb], acc[r][cb]);
         }
       }
@@ -295,6 +300,7 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
       }
     }
     // epilogue: each lane holds four rows of one column of every block
+    const int nrb = a.npad >> 4;
     for (int r = 0; r < RBW; ++r) {
       const int rb = wave + 4*r;
       if (rb >= nrb) break;
@@ -303,10 +309,7 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
         const int tb = a.tile_targets[(size_t)tile*TW + col];
         if (tb < 0) continue;
         T *dst = work + (size_t)(tb - a.tbase)*a.npad;
-        for (int i = 0; i < 4; ++i) {
-          const int row = rb*16 + Mfma<T>::row(lane, i);
-          if (row < a.npad) dst[row] = acc[r][cb][i];
-        }
+        for (int i = 0; i < 4; ++i) dst[rb*16 + Mfma<T>::row(lane, i)] = acc[r][cb][i];
       }
     }
   }
