@@ -443,6 +443,11 @@ __global__ __launch_bounds__(WAVE) void p2l_lap3d(
   double acc[P2L_MAXS];
   for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
     const int tb = target_boxes[ib];
+    if (sstarts[ib] == sstarts[ib + 1]) {
+      // empty list 4: the expansion is overwritten with zeros
+      for (int m = lane; m < e.nstored; m += WAVE) out[(size_t)(tb - base)*e.nstored + m] = 0.0;
+      continue;
+    }
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
     for (int s = 0; s < e.nstored; ++s) acc[s] = 0;
     for (int is = sstarts[ib]; is < sstarts[ib + 1]; ++is) {

@@ -520,9 +520,14 @@ class SumpyExpansionWrangler:
         lstarts = _host(level_start_target_or_target_parent_box_nrs)
         kwargs = dict(self.extra_kwargs)
         kwargs.update(self.box_source_list_kwargs())
+        host_starts = self._host_copy(starts, "from_sep_bigger_starts")
         for lev in range(self.tree.nlevels):
             start, stop = lstarts[lev:lev + 2]
             if start == stop:
+                continue
+            if host_starts is not None and host_starts[start] == host_starts[stop]:
+                # no list-4 sources at this level: the zero-initialised
+                # expansions are what P2L would write
                 continue
             p2l = self.tree_indep.p2l(self.level_orders[lev])
             base, view = self.local_expansions_view(local_exps, lev)
@@ -577,6 +582,19 @@ class SumpyExpansionWrangler:
 
     def finalize_potentials(self, actx, potentials):
         return potentials
+
+    def _host_copy(self, ary, name):
+        """Host copy of a traversal array when *ary* is that array (no
+        device synchronisation); None otherwise."""
+        trav = self.traversal
+        host = getattr(trav, "host", None)
+        if host is None or name not in host:
+            return None
+        try:
+            same = ary is getattr(trav, name)
+        except AttributeError:
+            same = False
+        return host[name] if same else None
 
     # }}}
 
