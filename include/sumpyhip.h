@@ -68,7 +68,8 @@ int sumpyhip_p2p_dense(int kernel_id, int precision, double k_re, double k_im,
 /* Near-field sum over per-box CSR source lists, replaces P2PFromCSR.__call__
  * (sumpy/p2p.py:801-833; loop structure :551-670).  Overwrites the results
  * of the targets in target_boxes.  box_centers (dim x nboxes) give the local
- * origin of the FP32 form. */
+ * origin of the FP32 form; when box_centers is NULL the first target of each
+ * target box is that origin (positions stay FP64 in the FP64 form). */
 int sumpyhip_p2p_csr(int kernel_id, int precision, double k_re, double k_im,
                      double scale, int ntgt_boxes, const int *target_boxes,
                      const int *source_box_starts, const int *source_box_lists,
@@ -174,14 +175,17 @@ int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
  * Y += A_c X with A_c[t, s] = data_c[off_t[t] + off_s[s]] and X the stacked
  * packed multipoles of the tile's sources.  The schedule (tiles, slots,
  * per-slot source boxes) is built on the host.  Overwrites the work rows of
- * the tile targets. */
+ * the tile targets.  With nsplit > 1 the slots of each tile are split over
+ * nsplit blocks; partial (device, (nsplit - 1) * ntiles * 64 * npad entries of
+ * the compute precision) holds the sums of splits 1.., which are then added
+ * to the work rows in a fixed order (deterministic). */
 int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_targets,
                        const int *tile_slot_starts, const int *slot_class,
                        const int *slot_sources, int nl, int nm, int npad, int kpad,
                        const int *off_t, const int *off_s, int ncirc,
                        const double *class_data, const void *packed_src,
                        int src_base_ibox, void *work, int tgt_base_ibox,
-                       void *stream);
+                       int nsplit, void *partial, void *stream);
 
 /* Postprocessing, replaces M2LPostprocessLocal.__call__
  * (sumpy/e2e.py:767-859): locals[b, i] = work[b, i] * ratio^deg_i. */

@@ -157,6 +157,8 @@ struct ApplyArgs {
   int sbase;
   void *work;
   int tbase;
+  int nsplit;      // slot ranges of a tile split over this many blocks
+  void *partial;   // (nsplit - 1) x ntiles*64 x npad sums of splits 1..
 };
 
 int set_lds_limit(const void *fn, size_t bytes);
@@ -188,12 +190,18 @@ struct Limits {
 template <typename T, int RBW>
 struct Apply {
   static constexpr int VN = Vec16<T>::N;
-  static constexpr int XS = TW + (sizeof(T) == 4 ? 16 : 8);   // padded LDS row of X
+  // padded LDS row of X (FP64: 4 doubles, so that two blocks fit in one CU)
+  static constexpr int XS = TW + (sizeof(T) == 4 ? 16 : 4);
   static constexpr int XPT = (Limits<RBW>::MAXK/VN*TW + 255)/256;  // X vectors per thread
   typedef typename Vec16<T>::type vec_t;
 };
 
-// One block per tile (4 waves), persistent over tiles.  Per class slot:
+// One block per (tile, split) item (4 waves); split j of a tile covers slots
+// [n j / nsplit, n (j + 1) / nsplit) of its n slots and writes work (j = 0) or
+// partial[j - 1] (the sums are added by m2l_reduce_splits).  Consecutive block
+// ids go to different XCDs, so items are mapped XCD-major: an XCD works on a
+// contiguous range of tiles, which the schedule orders spatially, and its L2
+// holds their source multipoles.  Per class slot:
 //   stage   : data_c (ncirc) and X (kpad x 64, LDS row stride XS) in LDS;
 //   compute : wave w owns row blocks w, w+4, ... and all four 16-column
 //             blocks; per 4-deep k step it reads 4 B fragments and gathers
@@ -262,11 +270,17 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
 
   const T *xlane = sX + (lane >> 4)*C::XS + (lane & 15);
   const int *olane = sOff + (lane >> 4);
-  for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+  const int nitems = a.ntiles*a.nsplit;
+  const int per_xcd = gridDim.x >> 3;   // the grid is a multiple of 8
+  const int item = (blockIdx.x & 7)*per_xcd + (blockIdx.x >> 3);
+  if (item < nitems) {
+    const int tile = item/a.nsplit, split = item - tile*a.nsplit;
     acc_t acc[RBW][4];
     for (int r = 0; r < RBW; ++r)
       for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
-    const int s0 = a.tile_slot_starts[tile], s1 = a.tile_slot_starts[tile + 1];
+    const int t0 = a.tile_slot_starts[tile], nsl = a.tile_slot_starts[tile + 1] - t0;
+    const int s0 = t0 + (int)((long long)nsl*split/a.nsplit);
+    const int s1 = t0 + (int)((long long)nsl*(split + 1)/a.nsplit);
     if (s0 < s1) {
       fetch(s0);
       __syncthreads();
@@ -308,10 +322,31 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
         const int col = cb*16 + (lane & 15);
         const int tb = a.tile_targets[(size_t)tile*TW + col];
         if (tb < 0) continue;
-        T *dst = work + (size_t)(tb - a.tbase)*a.npad;
+        T *dst = split == 0
+            ? work + (size_t)(tb - a.tbase)*a.npad
+            : reinterpret_cast<T *>(a.partial) +
+                  ((size_t)(split - 1)*a.ntiles*TW + (size_t)tile*TW + col)*a.npad;
         for (int i = 0; i < 4; ++i) dst[rb*16 + Mfma<T>::row(lane, i)] = acc[r][cb][i];
       }
     }
+  }
+}
+
+// work[target] += sum over splits 1.. of partial[split - 1][tile position]
+template <typename T>
+__global__ __launch_bounds__(256) void m2l_reduce_splits(int ntiles, int nsplit, int npad,
+                                                         const int *tile_targets,
+                                                         const T *partial, T *work,
+                                                         int tbase) {
+  const size_t npos = (size_t)ntiles*TW, n = npos*npad;
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    const size_t pos = i/npad;
+    const int tb = tile_targets[pos];
+    if (tb < 0) continue;
+    T v = 0;
+    for (int j = 0; j < nsplit - 1; ++j) v += partial[(size_t)j*n + i];
+    work[(size_t)(tb - tbase)*npad + (i - pos*npad)] += v;
   }
 }
 
@@ -322,8 +357,18 @@ int launch_apply(const ApplyArgs &a, hipStream_t st) {
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
   int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW>, lds);
   if (rc) return rc;
-  m2l_apply_kernel<T, RBW><<<a.ntiles, 256, lds, st>>>(a);
-  return check_launch("m2l_apply");
+  const long long nitems = (long long)a.ntiles*a.nsplit;
+  SUMPY_REQUIRE(nitems < (1ll << 30), "m2l_apply: too many tiles x splits");
+  const int grid = (int)((nitems + 7)/8*8);
+  m2l_apply_kernel<T, RBW><<<grid, 256, lds, st>>>(a);
+  rc = check_launch("m2l_apply");
+  if (rc || a.nsplit == 1) return rc;
+  const size_t n = (size_t)a.ntiles*TW*a.npad;
+  const int rgrid = (int)((n + 255)/256 < 16384 ? (n + 255)/256 : 16384);
+  m2l_reduce_splits<T><<<rgrid, 256, 0, st>>>(a.ntiles, a.nsplit, a.npad, a.tile_targets,
+                                              reinterpret_cast<const T *>(a.partial),
+                                              reinterpret_cast<T *>(a.work), a.tbase);
+  return check_launch("m2l_reduce_splits");
 }
 
 template <typename T>
@@ -453,8 +498,10 @@ extern "C" int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_tar
                                   const int *off_t, const int *off_s, int ncirc,
                                   const double *class_data, const void *packed_src,
                                   int src_base_ibox, void *work, int tgt_base_ibox,
-                                  void *stream) {
+                                  int nsplit, void *partial, void *stream) {
   SUMPY_REQUIRE(precision == 0 || precision == 1, "m2l_apply: unknown precision %d", precision);
+  SUMPY_REQUIRE(nsplit >= 1, "m2l_apply: nsplit %d < 1", nsplit);
+  SUMPY_REQUIRE(nsplit == 1 || partial != nullptr, "m2l_apply: nsplit > 1 needs a partial buffer");
   SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl, "m2l_apply: npad %d must be a multiple of 16 >= nl",
                 npad);
   SUMPY_REQUIRE(kpad % 4 == 0 && kpad >= nm, "m2l_apply: kpad %d must be a multiple of 4 >= nm",
@@ -462,7 +509,7 @@ extern "C" int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_tar
   if (ntiles == 0) return SUMPYHIP_OK;
   ApplyArgs a{ntiles, tile_targets, tile_slot_starts, slot_class, slot_sources, nl, nm,
               npad, kpad, off_t, off_s, ncirc, class_data, packed_src, src_base_ibox,
-              work, tgt_base_ibox};
+              work, tgt_base_ibox, nsplit, partial};
   hipStream_t st = (hipStream_t)stream;
   return precision == 0 ? dispatch_apply<double>(a, st) : dispatch_apply<float>(a, st);
 }

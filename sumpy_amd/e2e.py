@@ -22,6 +22,7 @@ from sumpy_amd.profiling import kernel_span
 from sumpy_amd.tools import as_i32
 
 TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
+MIN_SLOTS_PER_SPLIT = 12
 
 
 class E2EBase:
@@ -195,16 +196,23 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"])
         if sched["ntiles"] == 0:
             return work
+        ntiles, nsplit = sched["ntiles"], sched["nsplit"]
+        partial = None
+        if nsplit > 1:
+            partial = torch.empty((nsplit - 1)*ntiles*TILE*int(work.shape[1]),
+                                  dtype=work.dtype, device=work.device)
         flops = 2.0*m["nl"]*m["nm"]*sched["npairs"]
         with kernel_span(f"m2l_apply_{tr.precision}", flops):
-            _capi.call("sumpyhip_m2l_apply", tr.precision_id, sched["ntiles"],
-                   _capi.ptr(sched["tile_targets"]), _capi.ptr(sched["tile_slot_starts"]),
+            _capi.call("sumpyhip_m2l_apply", tr.precision_id, ntiles,
+                       _capi.ptr(sched["tile_targets"]), _capi.ptr(sched["tile_slot_starts"]),
                        _capi.ptr(sched["slot_class"]), _capi.ptr(sched["slot_sources"]),
                        m["nl"], m["nm"], int(work.shape[1]), int(src.shape[1]),
                        _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), m["ncirc"],
                        _capi.ptr(data.contiguous()), _capi.ptr(src),
                        int(kwargs["src_base_ibox"]), _capi.ptr(work),
-                       int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+                       int(kwargs["tgt_base_ibox"]), nsplit,
+                       _capi.ptr(partial) if partial is not None else None,
+                       actx.stream_handle)
         return work
 
 
@@ -234,9 +242,19 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
     rel = np.rint((ct - ct[:, :1])/h).astype(np.int64)
     octant = ((rel[0] & 1) << 2) | ((rel[1] & 1) << 1) | (rel[2] & 1)
     order = np.argsort(octant, kind="stable")
-    ntiles = (nt + TILE - 1)//TILE
+    # tiles: chunks of TILE targets of one octant, in box order; chunk j of
+    # every octant before chunk j + 1, so that a contiguous range of tiles
+    # covers one region of space (the apply kernel hands each XCD such a range)
+    cnt = np.bincount(octant, minlength=8)
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    nchunk = (cnt + TILE - 1)//TILE
+    chunks = [(j, o) for o in range(8) for j in range(nchunk[o])]
+    chunks.sort()
+    ntiles = len(chunks)
     padded = np.full(ntiles*TILE, -1, np.int64)
-    padded[:nt] = order
+    for it, (j, o) in enumerate(chunks):
+        sel = order[first[o] + j*TILE:first[o] + min((j + 1)*TILE, cnt[o])]
+        padded[it*TILE:it*TILE + len(sel)] = sel
     tab = np.full((nt + 1, max(nclasses, 1)), -1, np.int32)
     tab[itgt, e_cls] = e_src
     rows = np.where(padded >= 0, padded, nt)  # row nt is all -1
@@ -254,7 +272,21 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
     return dict(ntiles=ntiles, tile_targets=dev(tile_targets),
                 tile_slot_starts=dev(tile_slot_starts), slot_class=dev(ci),
                 slot_sources=dev(slot_sources.reshape(-1)), nslots=len(ci),
-                npairs=int(len(e_src)))
+                npairs=int(len(e_src)),
+                nsplit=choose_nsplit(ntiles, len(ci), actx.device))
+
+
+def choose_nsplit(ntiles, nslots, device):
+    """Number of blocks per tile: enough (tile, split) items for about four
+    blocks per CU, with at least :data:`MIN_SLOTS_PER_SPLIT` class slots per
+    split (each split costs one extra write and read of its partial sums)."""
+    if ntiles == 0:
+        return 1
+    ncu = torch.cuda.get_device_properties(device).multi_processor_count \
+        if torch.device(device).type == "cuda" else 256
+    want = -(-4*ncu//ntiles)
+    most = max(1, nslots//(ntiles*MIN_SLOTS_PER_SPLIT))
+    return int(max(1, min(want, most)))
 
 
 class M2LPostprocessLocal(E2EBase):
