@@ -231,6 +231,78 @@ def test_m2l_toeplitz_equals_naive_sum():
     assert T["ncirc"] == 3*(2*order + 1)**2
 
 
+def _local_setup(nsrc=20, ntgt=12, seed=5):
+    """Three boxes: 0 = parent target box at the origin (size 1), 1 = its child
+    at (1/4, -1/4, 1/4), 2 = a far source box at (3, 0.5, -0.7)."""
+    rng = np.random.default_rng(seed)
+    centers = np.array([[0.0, 0.25, 3.0], [0.0, -0.25, 0.5], [0.0, 0.25, -0.7]])
+    src = centers[:, 2:3] + 0.3*(rng.random((3, nsrc)) - 0.5)
+    tgt = centers[:, 1:2] + 0.2*(rng.random((3, ntgt)) - 0.5)
+    q = rng.standard_normal(nsrc)
+    wd = [rng.standard_normal(nsrc) for _ in range(3)]
+
+    def arrays(tgt_box):
+        cnt = np.zeros(3, np.int32)
+        cnt[tgt_box] = ntgt
+        return {"box_source_starts": np.zeros(3, np.int32),
+                "box_source_counts_nonchild": np.array([0, 0, nsrc], np.int32),
+                "box_target_starts": np.zeros(3, np.int32),
+                "box_target_counts_nonchild": cnt}
+    return centers, src, tgt, q, wd, arrays
+
+
+@pytest.mark.parametrize("compressed", [True, False])
+@pytest.mark.parametrize("dipole", [False, True])
+def test_p2l_l2p_converge_and_l2l_is_exact(compressed, dipole):
+    """P2L + L2P of a far cluster converges to the direct field with the
+    order; L2L re-expands the truncated (harmonic) Taylor polynomial exactly,
+    so L2P after L2L equals L2P of the parent expansion at the same points."""
+    centers, src, tgt, q, wd, arrays = _local_setup()
+    wdd = np.array(wd) if dipole else None
+    direct, _ = oracle.p2p_dense(3, tgt, src, q, wd=wdd)
+    errs = []
+    for order in (3, 6, 9):
+        w = tables.wrangler(order, 3, compressed)
+        rs0, rs1 = 2.0/order, 1.0/order
+        L0 = np.zeros((3, len(w.stored)))
+        oracle.p2l(order, [0], [0, 1], [2], arrays(0), centers, src, q,
+                   wd if dipole else None, rs0, L0, 0, compressed=compressed)
+        pot0 = np.zeros(tgt.shape[1])
+        oracle.l2p(order, [0], arrays(0), centers, tgt, rs0, L0, 0, pot0,
+                   compressed=compressed)
+        L1 = np.zeros((3, len(w.stored)))
+        oracle.l2l(order, order, [1], np.zeros(3, np.int32), centers, rs0, rs1, L0, 0,
+                   L1, 0, compressed=compressed)
+        pot1 = np.zeros(tgt.shape[1])
+        oracle.l2p(order, [1], arrays(1), centers, tgt, rs1, L1, 0, pot1,
+                   compressed=compressed)
+        assert rel(pot1, pot0) < 1e-12
+        errs.append(rel(pot0, direct))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4, errs
+
+
+def test_m2p_matches_derivative_sum_and_direct():
+    """M2P of a box's multipole equals sum_i M_i rho^|m_i| d^m_i G (derivatives
+    pinned by the sympy fixture) and converges to the direct field."""
+    centers, src, tgt, q, wd, arrays = _local_setup()
+    direct, _ = oracle.p2p_dense(3, tgt, src, q)
+    errs = []
+    for order in (3, 6, 9):
+        w = tables.wrangler(order)
+        rs = 1.0/order
+        M = np.zeros((3, len(w.stored)))
+        oracle.p2m(order, [2], arrays(1), centers, src, q, None, rs, M, 0)
+        pot = np.zeros(tgt.shape[1])
+        oracle.m2p(order, [1], [0, 1], [2], arrays(1), centers, tgt, rs, M, 0, pot)
+        expl = []
+        for t in range(tgt.shape[1]):
+            D = oracle.lap3d_derivs(tgt[:, t] - centers[:, 2], rs, order, (order,)*3)
+            expl.append(sum(M[2, i]*D[mi] for i, mi in enumerate(w.stored))/(4*np.pi))
+        assert rel(pot, expl) < 1e-12
+        errs.append(rel(pot, direct))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4, errs
+
+
 @pytest.mark.parametrize("order", [4, 8])
 def test_oracle_fmm_converges_to_direct(order):
     import sys
