@@ -20,6 +20,17 @@ namespace {
 
 constexpr int KLAP2 = 2, KLAP3 = 3, KHELM3 = 13;
 
+// precision-typed math (a plain rsqrt/log/exp call on a float argument
+// resolves to the double overload).  FP32 rsqrt is the bare v_rsq_f32 (1 ulp;
+// no rescaling of subnormal inputs, which a squared distance between
+// distinct points of a box neighbourhood does not reach).
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt(x); }
+__device__ __forceinline__ float log_t(float x) { return logf(x); }
+__device__ __forceinline__ double log_t(double x) { return log(x); }
+__device__ __forceinline__ float exp_t(float x) { return expf(x); }
+__device__ __forceinline__ double exp_t(double x) { return exp(x); }
+
 // {{{ pair evaluators (unscaled G; d = target - source)
 
 template <int KNL, bool DIP, bool GRAD, typename T>
@@ -27,7 +38,7 @@ __device__ __forceinline__ void pair_real(T dx, T dy, T dz, T w0, T wx, T wy, T 
                                           T &pot, T &g0, T &g1, T &g2) {
   if (KNL == KLAP3) {
     T r2 = dx*dx + dy*dy + dz*dz;
-    T ri = rsqrt(r2);
+    T ri = rsqrt_t(r2);
     pot += w0*ri;
     if (DIP || GRAD) {
       T ri3 = ri*ri*ri;
@@ -44,7 +55,7 @@ __device__ __forceinline__ void pair_real(T dx, T dy, T dz, T w0, T wx, T wy, T 
     }
   } else {  // Laplace 2D, G = log r
     T r2 = dx*dx + dy*dy;
-    pot += w0*(T(0.5)*log(r2));
+    pot += w0*(T(0.5)*log_t(r2));
     if (DIP || GRAD) {
       T inv = T(1)/r2;
       T wd = DIP ? (wx*dx + wy*dy) : T(0);
@@ -69,11 +80,11 @@ __device__ __forceinline__ void pair_helm(T dx, T dy, T dz, T kre, T kim,
                                           T w0r, T w0i, const T *wd, T *acc) {
   // acc: pot(re,im), g0(re,im), g1, g2
   T r2 = dx*dx + dy*dy + dz*dz;
-  T ri = rsqrt(r2);
+  T ri = rsqrt_t(r2);
   T r = r2*ri;
   T s, c;
   sincos_t(kre*r, &s, &c);
-  T damp = (kim != T(0)) ? exp(-kim*r) : T(1);
+  T damp = (kim != T(0)) ? exp_t(-kim*r) : T(1);
   T er = c*damp*ri, ei = s*damp*ri;  // e^{ikr}/r
   acc[0] += w0r*er - w0i*ei;
   acc[1] += w0r*ei + w0i*er;
@@ -134,72 +145,168 @@ struct P2PArgs {
 
 constexpr int WAVE = 64;
 
-// {{{ CSR near field, real kernels
+// {{{ CSR near field, real kernels: one 256-thread block per target box
 
+constexpr int NB = 256;     // threads per block
+constexpr int MAXB = 256;   // source boxes staged per pass
+
+// Sources per LDS chunk: (x, y, z, w0) [+ (wx, wy, wz, 0)] vectors, <= 48 KB.
+template <typename T, bool DIP>
+struct Chunk {
+  static constexpr int N = (DIP ? 48 : 32)*1024/(int)(sizeof(T)*4*(DIP ? 2 : 1));
+};
+
+// Pair value with the exclude-self pair masked (skip): every term carries a
+// factor ri (3D) or inv/lg (2D) that the mask sets to zero.
 template <int KNL, bool DIP, bool GRAD, typename T>
-__global__ __launch_bounds__(WAVE) void p2p_csr_real(P2PArgs a) {
+__device__ __forceinline__ void pair_masked(T dx, T dy, T dz, T w0, T wx, T wy, T wz, bool skip,
+                                            T &pot, T &g0, T &g1, T &g2) {
+  if (KNL == KLAP3) {
+    const T r2 = dx*dx + dy*dy + dz*dz;
+    T ri = rsqrt_t(r2);
+    ri = skip ? T(0) : ri;
+    pot += w0*ri;
+    if (DIP || GRAD) {
+      const T ri3 = ri*ri*ri;
+      const T wd = DIP ? (wx*dx + wy*dy + wz*dz) : T(0);
+      if (DIP) pot -= wd*ri3;
+      if (GRAD) {
+        const T f = -w0*ri3;
+        g0 += f*dx; g1 += f*dy; g2 += f*dz;
+        if (DIP) {
+          const T c = T(3)*wd*ri3*ri*ri;
+          g0 += c*dx - wx*ri3; g1 += c*dy - wy*ri3; g2 += c*dz - wz*ri3;
+        }
+      }
+    }
+  } else {  // Laplace 2D, G = log r
+    const T r2 = dx*dx + dy*dy;
+    T lg = T(0.5)*log_t(r2);
+    lg = skip ? T(0) : lg;
+    pot += w0*lg;
+    if (DIP || GRAD) {
+      T inv = T(1)/r2;
+      inv = skip ? T(0) : inv;
+      const T wd = DIP ? (wx*dx + wy*dy) : T(0);
+      if (DIP) pot += wd*inv;
+      if (GRAD) {
+        g0 += w0*dx*inv; g1 += w0*dy*inv;
+        if (DIP) {
+          const T c = T(2)*wd*inv*inv;
+          g0 += wx*inv - c*dx; g1 += wy*inv - c*dy;
+        }
+      }
+    }
+  }
+}
+
+// Per target box: the source boxes of its list are staged MAXB at a time
+// (starts, counts, exclusive offsets); their particles, concatenated, are
+// loaded into LDS in chunks of Chunk::N (each thread finds its box by binary
+// search over the offsets, so all loads of a chunk are independent).  G lanes
+// share a target (G = the largest power of two <= 64 with nt*G <= 256) and
+// take every G-th source; their sums are combined with cross-lane adds.  The
+// exclude-self pair is found by its position in the concatenation and masked
+// with a select.  FP32: coordinates relative to the box centre (or the first
+// target), FP32 sums of 16 pairs folded into FP64 accumulators.
+template <int KNL, bool DIP, bool GRAD, bool SELF, typename T>
+__global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
   constexpr int DIM = KNL == KLAP3 ? 3 : 2;
   constexpr bool F32 = sizeof(T) == 4;
-  __shared__ T s_c[DIM][WAVE];
-  __shared__ T s_w[DIP ? 4 : 1][WAVE];
-  __shared__ int s_idx[WAVE];
-  const int lane = threadIdx.x;
+  constexpr int CH = Chunk<T, DIP>::N;
+  constexpr int FOLD = F32 ? 16 : (1 << 20);
+  __shared__ v4 s_src[CH];
+  __shared__ v4 s_dip[DIP ? CH : 1];
+  __shared__ int s_bs[MAXB], s_bn[MAXB], s_boff[MAXB + 1];
+  const int tid = threadIdx.x;
   for (int ib = blockIdx.x; ib < a.ntgt_boxes; ib += gridDim.x) {
     const int tb = a.target_boxes[ib];
     const int t0 = a.box_tgt_starts[tb], nt = a.box_tgt_counts[tb];
     if (nt == 0) continue;
     const int sb0 = a.src_box_starts[ib], sb1 = a.src_box_starts[ib + 1];
     int G = 1;
-    while (G < WAVE && nt*(2*G) <= WAVE) G *= 2;
-    const int tpc = WAVE/G;
+    while (G < 64 && nt*2*G <= NB) G *= 2;
+    const int tpp = NB/G;
     double org[3] = {0, 0, 0};
     if (F32) {
-      // local origin: the target box centre, else its first target
-      const double *tc[3] = {a.tx, a.ty, a.tz};
+      const double *tcrd[3] = {a.tx, a.ty, a.tz};
       for (int d = 0; d < DIM; ++d)
-        org[d] = a.centers ? a.centers[(size_t)d*a.nboxes + tb] : tc[d][t0];
+        org[d] = a.centers ? a.centers[(size_t)d*a.nboxes + tb] : tcrd[d][t0];
     }
-    for (int tc = 0; tc < nt; tc += tpc) {
-      const int lt = lane/G, sub = lane % G;
+    for (int tc = 0; tc < nt; tc += tpp) {
+      const int lt = tid/G, sub = tid % G;
       const bool tvalid = tc + lt < nt;
       const int t = t0 + tc + (tvalid ? lt : 0);
       const T px = T(a.tx[t] - org[0]), py = T(a.ty[t] - org[1]),
               pz = DIM == 3 ? T(a.tz[t] - org[2]) : T(0);
-      const int self = a.t2s ? a.t2s[t] : -1;
+      const int self = SELF ? a.t2s[t] : -1;
       double pot = 0, g0 = 0, g1 = 0, g2 = 0;
-      for (int isb = sb0; isb < sb1; ++isb) {
-        const int sbx = a.src_box_lists[isb];
-        const int s0 = a.box_src_starts[sbx], ns = a.box_src_counts[sbx];
-        for (int sc = 0; sc < ns; sc += WAVE) {
-          __syncthreads();
-          if (lane < ns - sc) {
-            const int s = s0 + sc + lane;
-            s_c[0][lane] = T(a.sx[s] - org[0]);
-            s_c[1][lane] = T(a.sy[s] - org[1]);
-            if (DIM == 3) s_c[DIM - 1][lane] = T(a.sz[s] - org[2]);
-            s_w[0][lane] = T(a.w0[s]);
-            if constexpr (DIP) {
-              s_w[1][lane] = T(a.wx[s]);
-              s_w[2][lane] = T(a.wy[s]);
-              if (DIM == 3) s_w[3][lane] = T(a.wz[s]);
+      for (int bb = sb0; bb < sb1; bb += MAXB) {
+        const int nbx = min(MAXB, sb1 - bb);
+        __syncthreads();
+        if (tid < nbx) {
+          const int sbx = a.src_box_lists[bb + tid];
+          s_bs[tid] = a.box_src_starts[sbx];
+          s_bn[tid] = a.box_src_counts[sbx];
+        }
+        __syncthreads();
+        if (tid < 64) {  // exclusive scan of the counts (4 per lane)
+          int v[4], loc = 0;
+          for (int i = 0; i < 4; ++i) {
+            const int k = 4*tid + i;
+            v[i] = k < nbx ? s_bn[k] : 0;
+            loc += v[i];
+          }
+          int incl = loc;
+          for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (tid >= o) incl += y;
+          }
+          int run = incl - loc;
+          for (int i = 0; i < 4; ++i) { s_boff[4*tid + i] = run; run += v[i]; }
+          if (tid == 63) s_boff[MAXB] = run;
+        }
+        __syncthreads();
+        const int total = s_boff[nbx];
+        int pself = -1;
+        if (SELF) {
+          for (int k = 0; k < nbx; ++k)
+            if ((unsigned)(self - s_bs[k]) < (unsigned)s_bn[k]) pself = s_boff[k] + self - s_bs[k];
+        }
+        for (int c0 = 0; c0 < total; c0 += CH) {
+          const int nn = min(CH, total - c0);
+          if (c0 > 0) __syncthreads();
+          for (int p = tid; p < nn; p += NB) {
+            const int q = c0 + p;
+            int lo = 0, hi = nbx;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_boff[mid] <= q) lo = mid; else hi = mid;
             }
-            s_idx[lane] = s;
+            const int s = s_bs[lo] + q - s_boff[lo];
+            s_src[p] = v4{T(a.sx[s] - org[0]), T(a.sy[s] - org[1]),
+                          DIM == 3 ? T(a.sz[s] - org[2]) : T(0), T(a.w0[s])};
+            if constexpr (DIP)
+              s_dip[p] = v4{T(a.wx[s]), T(a.wy[s]), DIM == 3 ? T(a.wz[s]) : T(0), T(0)};
           }
           __syncthreads();
-          const int nn = min(WAVE, ns - sc);
-          T tp = 0, t0g = 0, t1g = 0, t2g = 0;
-          for (int j = sub; j < nn; j += G) {
-            T w0 = s_w[0][j];
-            if (s_idx[j] == self) continue;
-            T dx = px - s_c[0][j], dy = py - s_c[1][j];
-            T dz = DIM == 3 ? pz - s_c[DIM - 1][j] : T(0);
-            pair_real<KNL, DIP, GRAD, T>(dx, dy, dz, w0, DIP ? s_w[1][j] : T(0),
-                                         DIP ? s_w[2][j] : T(0),
-                                         (DIP && DIM == 3) ? s_w[3][j] : T(0), tp, t0g,
-                                         t1g, t2g);
+          const int jself = pself - c0;
+          for (int j0 = sub; j0 < nn; j0 += G*FOLD) {
+            const int jend = min(nn, j0 + G*FOLD);
+            T tp = 0, t0g = 0, t1g = 0, t2g = 0;
+            #pragma unroll 4
+            for (int j = j0; j < jend; j += G) {
+              const v4 sv = s_src[j];
+              v4 dv = v4{0, 0, 0, 0};
+              if constexpr (DIP) dv = s_dip[j];
+              pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y, DIM == 3 ? pz - sv.z : T(0),
+                                             sv.w, dv.x, dv.y, dv.z, SELF && j == jself, tp,
+                                             t0g, t1g, t2g);
+            }
+            pot += double(tp);
+            if (GRAD) { g0 += double(t0g); g1 += double(t1g); g2 += double(t2g); }
           }
-          pot += double(tp);
-          if (GRAD) { g0 += double(t0g); g1 += double(t1g); g2 += double(t2g); }
         }
       }
       pot = group_sum(pot, G);
@@ -422,15 +529,23 @@ __global__ __launch_bounds__(DB) void p2p_dense_helm(P2PArgs a) {
 
 // {{{ dispatch
 
+template <int KNL, bool DIP, bool GRAD, bool SELF>
+void launch_csr_real(int prec, const P2PArgs &a, hipStream_t st) {
+  const int grid = a.ntgt_boxes;
+  if (prec == 0) p2p_csr_real<KNL, DIP, GRAD, SELF, double><<<grid, NB, 0, st>>>(a);
+  else p2p_csr_real<KNL, DIP, GRAD, SELF, float><<<grid, NB, 0, st>>>(a);
+}
+
 template <bool DIP, bool GRAD>
 int launch_csr(int knl, int prec, const P2PArgs &a, hipStream_t st) {
   const int grid = a.ntgt_boxes < 16*num_cus() ? a.ntgt_boxes : 16*num_cus();
+  const bool self = a.t2s != nullptr;
   if (knl == KLAP3) {
-    if (prec == 0) p2p_csr_real<KLAP3, DIP, GRAD, double><<<grid, WAVE, 0, st>>>(a);
-    else p2p_csr_real<KLAP3, DIP, GRAD, float><<<grid, WAVE, 0, st>>>(a);
+    if (self) launch_csr_real<KLAP3, DIP, GRAD, true>(prec, a, st);
+    else launch_csr_real<KLAP3, DIP, GRAD, false>(prec, a, st);
   } else if (knl == KLAP2) {
-    if (prec == 0) p2p_csr_real<KLAP2, DIP, GRAD, double><<<grid, WAVE, 0, st>>>(a);
-    else p2p_csr_real<KLAP2, DIP, GRAD, float><<<grid, WAVE, 0, st>>>(a);
+    if (self) launch_csr_real<KLAP2, DIP, GRAD, true>(prec, a, st);
+    else launch_csr_real<KLAP2, DIP, GRAD, false>(prec, a, st);
   } else {
     if (prec == 0) p2p_csr_helm<DIP, GRAD, double><<<grid, WAVE, 0, st>>>(a);
     else p2p_csr_helm<DIP, GRAD, float><<<grid, WAVE, 0, st>>>(a);
