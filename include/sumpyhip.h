@@ -126,6 +126,23 @@ int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
                  double tgt_rscale, const double *src_expansions, int src_base_ibox,
                  double *tgt_expansions, int tgt_base_ibox, void *stream);
 
+/* Octree level translation as dense products, replaces E2EFromChildren.__call__
+ * (M2M, sumpy/e2e.py:866-978) and E2EFromParent.__call__ (L2L,
+ * sumpy/e2e.py:985-1083) when every child of the level sits at one of eight
+ * fixed offsets from its parent: for each group of 16 target boxes and each
+ * octant o with bit o set in group_octant_mask[g],
+ *   tgt[b] += T_o src[group_inputs[g][o][j]]  (b = group_targets[g][j];
+ *   -1 entries are skipped), accumulate.
+ * T_frag holds the eight n_out x n_in FP64 matrices in MFMA A-fragment order:
+ * T_frag[((o*nrb + rb)*ksteps + ks)*64 + l] = T_o[16 rb + l%16][4 ks + l/16]
+ * (nrb = ceil(n_out/16), ksteps = ceil(n_in/4), zero padded).  A target box
+ * may appear in one group only. */
+int sumpyhip_octant_translate(int ngroups, const int *group_targets,
+                              const int *group_inputs, const int *group_octant_mask,
+                              int n_out, int n_in, const double *T_frag,
+                              const double *src_expansions, int src_base_ibox,
+                              double *tgt_expansions, int tgt_base_ibox, void *stream);
+
 /* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226); overwrites
  * pot (and the gradient when given) for the targets of each box. */
 int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,

@@ -22,6 +22,7 @@ from sumpy_amd.profiling import kernel_span
 from sumpy_amd.tools import as_i32
 
 TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
+GROUP = 16  # target boxes per octant-translation group (matches the HIP kernel)
 MIN_SLOTS_PER_SPLIT = 12
 
 
@@ -47,10 +48,157 @@ class E2EBase:
                 self.tgt_expansion.device_tables(actx.device))
 
 
-class E2EFromChildren(E2EBase):
+class _OctantTranslation(E2EBase):
+    """Shared machinery of M2M and L2L: when every child of the level sits at
+    one of eight fixed offsets from its parent (an octree), the level's
+    translation is eight fixed matrices applied to gathered coefficient
+    vectors (``sumpyhip_octant_translate``, MFMA).  The matrices are the
+    per-box kernel applied to unit vectors, so both paths compute the same map;
+    other geometries use the per-box kernel."""
+
+    def __init__(self, src_expansion, tgt_expansion, name=None,
+                 use_octant_matrices=True):
+        super().__init__(src_expansion, tgt_expansion, name)
+        self.use_octant_matrices = use_octant_matrices
+        self._host = {}
+        self._plans = {}
+        self._mats = {}
+
+    def _host_copy(self, t):
+        key = (t.data_ptr(), t.numel(), t._version, tuple(t.shape), str(t.dtype))
+        v = self._host.get(key)
+        if v is None:
+            v = t.cpu().numpy()
+            self._host[key] = v
+        return v
+
+    def _matrices(self, actx, offsets, present, src_rscale, tgt_rscale):
+        key = (offsets.tobytes(), present.tobytes(), float(src_rscale), float(tgt_rscale))
+        frag = self._mats.get(key)
+        if frag is None:
+            T = self._unit_matrices(actx, offsets, present, src_rscale, tgt_rscale)
+            frag = _fragment_order(T)
+            self._mats[key] = frag
+        return frag
+
+    def _unit_matrices(self, actx, offsets, present, src_rscale, tgt_rscale):
+        """T[o] (n_out x n_in) from the per-box kernel on a synthetic level:
+        box 0 unused, boxes 1..8n are the 'big' boxes (parents), 8n+1..16n the
+        'small' ones (children), one pair per (octant, unit vector)."""
+        st, tt = self._tables(actx)
+        n_in, n_out = st.table.nstored, tt.table.nstored
+        no = 8*n_in
+        nb = 1 + 2*no
+        par = 1 + np.arange(no, dtype=np.int32)
+        chd = 1 + no + np.arange(no, dtype=np.int32)
+        centers = np.zeros((3, nb))
+        for o in range(8):
+            centers[:, chd[o*n_in:(o + 1)*n_in]] = self._child_offset(offsets[o])[:, None]
+        eye = torch.eye(n_in, dtype=torch.float64, device=actx.device).repeat(8, 1)
+        out = torch.zeros((no, n_out), dtype=torch.float64, device=actx.device)
+        self._unit_call(actx, st, tt, par, chd, nb, actx.from_numpy(centers), src_rscale,
+                        tgt_rscale, eye, out)
+        T = out.view(8, n_in, n_out).transpose(1, 2)
+        return T*torch.from_numpy(present.astype(np.float64)).to(actx.device)[:, None, None]
+
+    def _launch_octant(self, actx, plan, frag, src, sbase, tgt, tbase):
+        st, tt = self._tables(actx)
+        flops = 2.0*st.table.nstored*tt.table.nstored*plan["npairs"]
+        with kernel_span(self.span_name, flops):
+            _capi.call("sumpyhip_octant_translate", plan["ngroups"],
+                       _capi.ptr(plan["targets"]), _capi.ptr(plan["inputs"]),
+                       _capi.ptr(plan["mask"]), tt.table.nstored, st.table.nstored, _capi.ptr(frag),
+                       _capi.ptr(src), int(sbase), _capi.ptr(tgt), int(tbase),
+                       actx.stream_handle)
+        return tgt
+
+
+def _fragment_order(T):
+    """(8, n_out, n_in) -> flat MFMA A-fragment order
+    ``[((o*nrb + rb)*ksteps + ks)*64 + l] = T[o, 16 rb + l%16, 4 ks + l/16]``."""
+    no, n_out, n_in = T.shape
+    nrb, ks = -(-n_out//16), -(-n_in//4)
+    P = torch.zeros((no, nrb*16, ks*4), dtype=torch.float64, device=T.device)
+    P[:, :n_out, :n_in] = T
+    return P.view(no, nrb, 16, ks, 4).permute(0, 1, 3, 4, 2).contiguous().view(-1)
+
+
+def _common_offsets(d, labels, rel_tol=1e-10):
+    """Per label 0..7 the common offset of its pairs (columns of *d*), or None
+    when the offsets of some label differ by more than *rel_tol*."""
+    offsets = np.zeros((8, 3))
+    present = np.zeros(8, bool)
+    for o in range(8):
+        sel = labels == o
+        if not sel.any():
+            continue
+        do = d[:, sel]
+        d0 = do[:, 0]
+        if np.abs(do - d0[:, None]).max() > rel_tol*max(np.abs(d0).max(), 1e-300):
+            return None
+        offsets[o] = d0
+        present[o] = True
+    return offsets, present
+
+
+def _groups(actx, targets, inputs, labels):
+    """Groups of GROUP targets sharing one label each (inputs[i] feeds target
+    i through octant labels[i]); M2M passes all eight octants per target."""
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(actx.device)
+    return dev(targets), dev(inputs)
+
+
+class E2EFromChildren(_OctantTranslation):
     """M2M: ``tgt[parent] += sum_children T(c_parent - c_child) src[child]``."""
 
     default_name = "e2e_from_children"
+    span_name = "m2m"
+
+    @staticmethod
+    def _child_offset(d):
+        return -d  # the parent sits at the origin, d = c_parent - c_child
+
+    def _unit_call(self, actx, st, tt, par, chd, nb, centers, src_rscale, tgt_rscale,
+                   src, out):
+        n_in = st.table.nstored
+        child = np.zeros((8, nb), np.int32)
+        for o in range(8):
+            child[o, par[o*n_in:(o + 1)*n_in]] = chd[o*n_in:(o + 1)*n_in]
+        child = actx.from_numpy(child)
+        par_d = actx.from_numpy(par)
+        _capi.call("sumpyhip_m2m", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
+                   len(par), _capi.ptr(par_d), _capi.ptr(child), _capi.ptr(centers), nb,
+                   float(src_rscale), float(tgt_rscale), _capi.ptr(src), int(chd[0]),
+                   _capi.ptr(out), int(par[0]), actx.stream_handle)
+
+    def _plan(self, tb_t, child_t, centers_t):
+        key = tuple((t.data_ptr(), t.numel(), t._version) for t in (tb_t, child_t, centers_t))
+        if key in self._plans:
+            return self._plans[key]
+        tb = self._host_copy(tb_t).astype(np.int64)
+        child = self._host_copy(child_t)
+        centers = self._host_copy(centers_t)
+        ch = child[:, tb].astype(np.int64)  # (8, n); 0 = no child
+        has = ch != 0
+        d = np.concatenate([centers[:, tb[has[o]]] - centers[:, ch[o, has[o]]]
+                            for o in range(8)], axis=1)
+        labels = np.concatenate([np.full(int(has[o].sum()), o) for o in range(8)])
+        common = _common_offsets(d, labels) if len(labels) else None
+        plan = None
+        if common is not None:
+            n = len(tb)
+            ng = -(-n//GROUP)
+            targets = np.full(ng*GROUP, -1, np.int64)
+            targets[:n] = tb
+            inputs = np.full((8, ng*GROUP), -1, np.int64)
+            inputs[:, :n] = np.where(has, ch, -1)
+            inputs = inputs.reshape(8, ng, GROUP).transpose(1, 0, 2)
+            mask = ((inputs >= 0).any(axis=2)*(1 << np.arange(8))[None, :]).sum(axis=1)
+            plan = dict(ngroups=ng, targets=targets, inputs=inputs, mask=mask,
+                        offsets=common[0], present=common[1], npairs=int(has.sum()))
+        self._plans[key] = plan
+        return plan
 
     def __call__(self, actx, **kwargs):
         st, tt = self._tables(actx)
@@ -58,18 +206,83 @@ class E2EFromChildren(E2EBase):
         tb = as_i32(kwargs["target_boxes"])
         child = as_i32(kwargs["box_child_ids"])
         src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
-        _capi.call("sumpyhip_m2m", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
-                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(child), _capi.ptr(centers),
-                   int(centers.shape[1]), float(kwargs["src_rscale"]),
-                   float(kwargs["tgt_rscale"]), _capi.ptr(src), int(kwargs["src_base_ibox"]),
-                   _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        if self.use_octant_matrices and tb.numel():
+            plan = self._plan(tb, child, centers)
+            if plan is not None:
+                if "dev" not in plan:
+                    plan["dev"] = dict(zip(("targets", "inputs"), _groups(
+                        actx, plan["targets"], plan["inputs"], None)),
+                        mask=actx.from_numpy(plan["mask"].astype(np.int32)))
+                frag = self._matrices(actx, plan["offsets"], plan["present"],
+                                      kwargs["src_rscale"], kwargs["tgt_rscale"])
+                return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
+                                                      npairs=plan["npairs"]),
+                                           frag, src, kwargs["src_base_ibox"], tgt,
+                                           kwargs["tgt_base_ibox"])
+        with kernel_span(self.span_name, 0.0):
+            _capi.call("sumpyhip_m2m", kernel_id(self.tgt_expansion.kernel), st.desc,
+                       tt.desc, int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(child),
+                       _capi.ptr(centers), int(centers.shape[1]), float(kwargs["src_rscale"]),
+                       float(kwargs["tgt_rscale"]), _capi.ptr(src),
+                       int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
+                       int(kwargs["tgt_base_ibox"]), actx.stream_handle)
         return tgt
 
 
-class E2EFromParent(E2EBase):
+class E2EFromParent(_OctantTranslation):
     """L2L: ``tgt[child] += T(c_child - c_parent) src[parent]``."""
 
     default_name = "e2e_from_parent"
+    span_name = "l2l"
+
+    @staticmethod
+    def _child_offset(d):
+        return d  # the parent sits at the origin, d = c_child - c_parent
+
+    def _unit_call(self, actx, st, tt, par, chd, nb, centers, src_rscale, tgt_rscale,
+                   src, out):
+        parent = np.zeros(nb, np.int32)
+        parent[chd] = par
+        parent = actx.from_numpy(parent)
+        chd_d = actx.from_numpy(chd)
+        _capi.call("sumpyhip_l2l", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
+                   len(chd), _capi.ptr(chd_d), _capi.ptr(parent), _capi.ptr(centers), nb,
+                   float(src_rscale), float(tgt_rscale), _capi.ptr(src), int(par[0]),
+                   _capi.ptr(out), int(chd[0]), actx.stream_handle)
+
+    def _plan(self, tb_t, parent_t, centers_t):
+        key = tuple((t.data_ptr(), t.numel(), t._version) for t in (tb_t, parent_t, centers_t))
+        if key in self._plans:
+            return self._plans[key]
+        tb = self._host_copy(tb_t).astype(np.int64)
+        parent = self._host_copy(parent_t).astype(np.int64)
+        centers = self._host_copy(centers_t)
+        pb = parent[tb]
+        d = centers[:, tb] - centers[:, pb]
+        labels = ((d[0] > 0)*4 + (d[1] > 0)*2 + (d[2] > 0)).astype(np.int64)
+        common = _common_offsets(d, labels) if len(tb) and (pb != tb).all() else None
+        plan = None
+        if common is not None:
+            order = np.argsort(labels, kind="stable")
+            cnt = np.bincount(labels, minlength=8)
+            first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+            tg, ip, mk = [], [], []
+            for o in range(8):
+                sel = order[first[o]:first[o] + cnt[o]]
+                for c0 in range(0, len(sel), GROUP):
+                    part = sel[c0:c0 + GROUP]
+                    t = np.full(GROUP, -1, np.int64)
+                    t[:len(part)] = tb[part]
+                    i = np.full((8, GROUP), -1, np.int64)
+                    i[o, :len(part)] = pb[part]
+                    tg.append(t)
+                    ip.append(i)
+                    mk.append(1 << o)
+            plan = dict(ngroups=len(tg), targets=np.concatenate(tg),
+                        inputs=np.stack(ip), mask=np.array(mk), offsets=common[0],
+                        present=common[1], npairs=len(tb))
+        self._plans[key] = plan
+        return plan
 
     def __call__(self, actx, **kwargs):
         st, tt = self._tables(actx)
@@ -77,11 +290,26 @@ class E2EFromParent(E2EBase):
         tb = as_i32(kwargs["target_boxes"])
         parents = as_i32(kwargs["box_parent_ids"])
         src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
-        _capi.call("sumpyhip_l2l", kernel_id(self.tgt_expansion.kernel), st.desc, tt.desc,
-                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(parents), _capi.ptr(centers),
-                   int(centers.shape[1]), float(kwargs["src_rscale"]),
-                   float(kwargs["tgt_rscale"]), _capi.ptr(src), int(kwargs["src_base_ibox"]),
-                   _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+        if self.use_octant_matrices and tb.numel():
+            plan = self._plan(tb, parents, centers)
+            if plan is not None:
+                if "dev" not in plan:
+                    plan["dev"] = dict(zip(("targets", "inputs"), _groups(
+                        actx, plan["targets"], plan["inputs"], None)),
+                        mask=actx.from_numpy(plan["mask"].astype(np.int32)))
+                frag = self._matrices(actx, plan["offsets"], plan["present"],
+                                      kwargs["src_rscale"], kwargs["tgt_rscale"])
+                return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
+                                                      npairs=plan["npairs"]),
+                                           frag, src, kwargs["src_base_ibox"], tgt,
+                                           kwargs["tgt_base_ibox"])
+        with kernel_span(self.span_name, 0.0):
+            _capi.call("sumpyhip_l2l", kernel_id(self.tgt_expansion.kernel), st.desc,
+                       tt.desc, int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(parents),
+                       _capi.ptr(centers), int(centers.shape[1]), float(kwargs["src_rscale"]),
+                       float(kwargs["tgt_rscale"]), _capi.ptr(src),
+                       int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
+                       int(kwargs["tgt_base_ibox"]), actx.stream_handle)
         return tgt
 
 

@@ -196,8 +196,10 @@ def _random_expansions(rng, nboxes, ncoeff):
     return rng.standard_normal((nboxes, ncoeff))
 
 
+@pytest.mark.parametrize("octant", [True, False])
 @pytest.mark.parametrize("order", [4, 10])
-def test_m2m_l2l(actx, lap3d, small_tree, order):
+def test_m2m_l2l(actx, lap3d, small_tree, order, octant):
+    """Both the octant-matrix (MFMA) path and the per-box kernels."""
     from sumpy_amd import E2EFromChildren, E2EFromParent
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
     from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
@@ -217,10 +219,12 @@ def test_m2m_l2l(actx, lap3d, small_tree, order):
     pb = parents[lsp[tl]:lsp[tl + 1]]
     rs_s, rs_t = _rscale(tree, sl, order), _rscale(tree, tl, order)
     out = dev(actx, tgt0.copy())
-    E2EFromChildren(mp, mp)(actx, src_expansions=dev(actx, src), src_base_ibox=int(lbs[sl]),
-                            tgt_expansions=out, tgt_base_ibox=int(lbs[tl]),
-                            target_boxes=dev(actx, pb), box_child_ids=tree.box_child_ids,
-                            centers=tree.box_centers, src_rscale=rs_s, tgt_rscale=rs_t)
+    m2m = E2EFromChildren(mp, mp, use_octant_matrices=octant)
+    m2m(actx, src_expansions=dev(actx, src), src_base_ibox=int(lbs[sl]),
+        tgt_expansions=out, tgt_base_ibox=int(lbs[tl]),
+        target_boxes=dev(actx, pb), box_child_ids=tree.box_child_ids,
+        centers=tree.box_centers, src_rscale=rs_s, tgt_rscale=rs_t)
+    assert (m2m._plans and next(iter(m2m._plans.values())) is not None) == octant
     ref = tgt0.copy()
     oracle.m2m(order, order, pb, th["box_child_ids"], th["box_centers"], rs_s, rs_t, src,
                int(lbs[sl]), ref, int(lbs[tl]))
@@ -233,10 +237,12 @@ def test_m2m_l2l(actx, lap3d, small_tree, order):
     lst = vh["level_start_target_or_target_parent_box_nrs"]
     cb = totp[lst[sl]:lst[sl + 1]]
     out = dev(actx, ltgt0.copy())
-    E2EFromParent(lc, lc)(actx, src_expansions=dev(actx, lsrc), src_base_ibox=int(lbs[tl]),
-                          tgt_expansions=out, tgt_base_ibox=int(lbs[sl]),
-                          target_boxes=dev(actx, cb), box_parent_ids=tree.box_parent_ids,
-                          centers=tree.box_centers, src_rscale=rs_t, tgt_rscale=rs_s)
+    l2l = E2EFromParent(lc, lc, use_octant_matrices=octant)
+    l2l(actx, src_expansions=dev(actx, lsrc), src_base_ibox=int(lbs[tl]),
+        tgt_expansions=out, tgt_base_ibox=int(lbs[sl]),
+        target_boxes=dev(actx, cb), box_parent_ids=tree.box_parent_ids,
+        centers=tree.box_centers, src_rscale=rs_t, tgt_rscale=rs_s)
+    assert (l2l._plans and next(iter(l2l._plans.values())) is not None) == octant
     ref = ltgt0.copy()
     oracle.l2l(order, order, cb, th["box_parent_ids"], th["box_centers"], rs_t, rs_s, lsrc,
                int(lbs[tl]), ref, int(lbs[sl]))
