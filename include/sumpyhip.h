@@ -45,6 +45,7 @@ typedef struct sumpyhip_expn {
   const int *projT_rows;
   const double *projT_vals;
   const int *horner_order;      /* nfull: full index at each Horner position */
+  int proj_nnz;                 /* entries of P (proj_row_starts[nfull]) */
 } sumpyhip_expn;
 
 const char *sumpyhip_last_error(void);

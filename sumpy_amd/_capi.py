@@ -32,7 +32,7 @@ class ExpnDesc(ctypes.Structure):
         ("proj_row_starts", ctypes.c_void_p), ("proj_cols", ctypes.c_void_p),
         ("proj_vals", ctypes.c_void_p), ("projT_col_starts", ctypes.c_void_p),
         ("projT_rows", ctypes.c_void_p), ("projT_vals", ctypes.c_void_p),
-        ("horner_order", ctypes.c_void_p),
+        ("horner_order", ctypes.c_void_p), ("proj_nnz", ctypes.c_int),
     ]
 
 

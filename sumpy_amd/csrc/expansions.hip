@@ -41,88 +41,125 @@ __device__ __forceinline__ double inv_fact(int n) {
 
 // {{{ P2M
 
-template <bool DIP>
-__global__ __launch_bounds__(WAVE) void p2m_lap3d(
+constexpr int P2M_NT = 128;   // threads per block, one source box at a time
+constexpr int P2M_SCH = 64;   // sources per LDS chunk
+
+// Number of (i, j, k-range) tasks when each pair (i, j), i + j <= p, is cut
+// into pieces of at most kt consecutive k (k <= p - i - j).
+__host__ __device__ inline int p2m_ntasks(int p, int kt) {
+  int n = 0;
+  for (int s = 0; s <= p; ++s) n += (s + 1)*((p - s + kt)/kt);
+  return n;
+}
+
+// Task t: pairs in order of (i + j, i), pieces in order of k.
+__device__ inline bool p2m_task(int p, int kt, int t, int &ti, int &tj, int &k0, int &k1) {
+  int n = 0;
+  for (int s = 0; s <= p; ++s) {
+    const int pieces = (p - s + kt)/kt;
+    if (t < n + (s + 1)*pieces) {
+      const int q = t - n;
+      ti = q/pieces;
+      tj = s - ti;
+      k0 = (q - ti*pieces)*kt;
+      k1 = min(k0 + kt, p - s + 1);
+      return true;
+    }
+    n += (s + 1)*pieces;
+  }
+  return false;
+}
+
+// C_full[i,j,k] = sum_s w0 X_i Y_j Z_k + wx X_{i-1} Y_j Z_k + wy X_i Y_{j-1} Z_k
+//                 + wz X_i Y_j Z_{k-1},   X_i = (a_x/rho)^i / i!,  a = centre - source
+// (sumpy/expansion/multipole.py:78-114; dipole weights pre-divided by rho),
+// then M = P^T C_full.  Each thread owns an (i, j, k-range) task and keeps its
+// KT sums in registers; the per-source power tables X, Y, Z live in LDS.
+template <bool DIP, int KT>
+__global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
     sumpyhip_expn e, int nboxes_list, const int *source_boxes, const int *bss,
     const int *bsc, const double *centers, int nboxes, const double *sx,
     const double *sy, const double *sz, const double *w0, const double *wx,
     const double *wy, const double *wz, double rscale, double *out, int base) {
   extern __shared__ double lds[];
-  const int P1 = e.order + 1;
-  double *pw = lds;                // [3][WAVE][P1]  (a/rho)^i/i!
-  double *sw = pw + 3*WAVE*P1;     // [4][WAVE]
-  double *cf = sw + 4*WAVE;        // [nfull]
-  const int lane = threadIdx.x;
-  constexpr int NPL = 16;          // nfull <= 16*64
-  int fid[NPL][3];
-  for (int q = 0; q < NPL; ++q) {
-    int f = lane + WAVE*q;
-    if (f < e.nfull) {
-      fid[q][0] = e.full_ids[f]; fid[q][1] = e.full_ids[e.nfull + f];
-      fid[q][2] = e.full_ids[2*e.nfull + f];
-    } else {
-      fid[q][0] = fid[q][1] = fid[q][2] = 0;
-    }
-  }
+  const int p = e.order, P1 = p + 1;
+  double *pw = lds;                       // [SCH][3][P1]
+  double *sw = pw + P2M_SCH*3*P1;         // [SCH][4]
+  double *cf = sw + P2M_SCH*4;            // [nfull]
+  const int tid = threadIdx.x;
+  const int ntasks = p2m_ntasks(p, KT);
+  const int nrounds = (ntasks + P2M_NT - 1)/P2M_NT;
   const double rinv = 1.0/rscale;
   for (int ib = blockIdx.x; ib < nboxes_list; ib += gridDim.x) {
     const int b = source_boxes[ib];
     const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
     const int s0 = bss[b], ns = bsc[b];
-    double acc[NPL];
-    for (int q = 0; q < NPL; ++q) acc[q] = 0;
-    for (int sc = 0; sc < ns; sc += WAVE) {
-      __syncthreads();
-      if (lane < ns - sc) {
-        const int s = s0 + sc + lane;
-        const double a[3] = {(c0 - sx[s])*rinv, (c1 - sy[s])*rinv, (c2 - sz[s])*rinv};
-        for (int d = 0; d < 3; ++d) {
-          double *row = pw + (d*WAVE + lane)*P1;
-          double v = 1;
-          row[0] = 1;
-          for (int i = 1; i < P1; ++i) { v *= a[d]/i; row[i] = v; }
-        }
-        sw[lane] = w0[s];
-        if (DIP) {
-          sw[WAVE + lane] = wx[s]*rinv; sw[2*WAVE + lane] = wy[s]*rinv;
-          sw[3*WAVE + lane] = wz[s]*rinv;
-        }
-      }
-      __syncthreads();
-      const int nn = min(WAVE, ns - sc);
-      for (int q = 0; q < NPL; ++q) {
-        if (lane + WAVE*q >= e.nfull) break;
-        const int i = fid[q][0], j = fid[q][1], k = fid[q][2];
-        double v = 0;
-        for (int s = 0; s < nn; ++s) {
-          const double *px = pw + s*P1, *py = pw + (WAVE + s)*P1, *pz = pw + (2*WAVE + s)*P1;
-          double m = px[i]*py[j]*pz[k];
-          v += sw[s]*m;
+    for (int round = 0; round < nrounds; ++round) {
+      int ti = 0, tj = 0, k0 = 0, k1 = 0;
+      const bool has = p2m_task(p, KT, tid + round*P2M_NT, ti, tj, k0, k1);
+      double acc[KT];
+      for (int kk = 0; kk < KT; ++kk) acc[kk] = 0;
+      for (int sc = 0; sc < ns; sc += P2M_SCH) {
+        const int nn = min(P2M_SCH, ns - sc);
+        __syncthreads();
+        if (tid < nn) {
+          const int s = s0 + sc + tid;
+          const double a[3] = {(c0 - sx[s])*rinv, (c1 - sy[s])*rinv, (c2 - sz[s])*rinv};
+          double *row = pw + tid*3*P1;
+          for (int d = 0; d < 3; ++d) {
+            double v = 1;
+            row[d*P1] = 1;
+            for (int q = 1; q < P1; ++q) { v *= a[d]/q; row[d*P1 + q] = v; }
+          }
+          sw[tid*4] = w0[s];
           if (DIP) {
-            // d/da_ax of a^mi/mi! = a^(mi - e_ax)/(mi - e_ax)!
-            double dx = i ? px[i - 1]*py[j]*pz[k] : 0.0;
-            double dy = j ? px[i]*py[j - 1]*pz[k] : 0.0;
-            double dz = k ? px[i]*py[j]*pz[k - 1] : 0.0;
-            v += sw[WAVE + s]*dx + sw[2*WAVE + s]*dy + sw[3*WAVE + s]*dz;
+            sw[tid*4 + 1] = wx[s]*rinv; sw[tid*4 + 2] = wy[s]*rinv; sw[tid*4 + 3] = wz[s]*rinv;
           }
         }
-        acc[q] += v;
+        __syncthreads();
+        if (has) {
+          for (int s = 0; s < nn; ++s) {
+            const double *X = pw + s*3*P1, *Y = X + P1, *Z = Y + P1;
+            const double xi = X[ti], yj = Y[tj];
+            double bco = sw[s*4]*xi*yj, cco = 0;
+            if (DIP) {
+              bco += sw[s*4 + 1]*(ti ? X[ti - 1] : 0.0)*yj + sw[s*4 + 2]*xi*(tj ? Y[tj - 1] : 0.0);
+              cco = sw[s*4 + 3]*xi*yj;
+            }
+            #pragma unroll
+            for (int kk = 0; kk < KT; ++kk) {
+              const int k = k0 + kk;
+              if (k < k1) {
+                acc[kk] += bco*Z[k];
+                if (DIP && k > 0) acc[kk] += cco*Z[k - 1];
+              }
+            }
+          }
+        }
       }
-    }
-    __syncthreads();
-    for (int q = 0; q < NPL; ++q) {
-      int f = lane + WAVE*q;
-      if (f < e.nfull) cf[f] = acc[q];
+      if (has)
+        for (int kk = 0; kk < KT; ++kk) {
+          const int k = k0 + kk;
+          if (k < k1) cf[e.cube_index[(ti*P1 + tj)*P1 + k]] = acc[kk];
+        }
     }
     __syncthreads();
     double *o = out + (size_t)(b - base)*e.nstored;
-    for (int s = lane; s < e.nstored; s += WAVE) {
+    for (int s = tid; s < e.nstored; s += P2M_NT) {
       double v = 0;
       for (int q = e.projT_col_starts[s]; q < e.projT_col_starts[s + 1]; ++q)
         v += e.projT_vals[q]*cf[e.projT_rows[q]];
       o[s] = v;
     }
   }
+}
+
+// smallest of the supported piece lengths that needs at most one round
+inline int p2m_kt(int p) {
+  const int kts[] = {2, 3, 4, 6, 12};
+  for (int kt : kts)
+    if (p2m_ntasks(p, kt) <= P2M_NT) return kt;
+  return 12;
 }
 
 // }}}
@@ -255,27 +292,72 @@ __global__ __launch_bounds__(WAVE) void l2l_lap3d(
 
 // {{{ L2P
 
+// Per block (once): the Horner-ordered coefficient factors
+// h[pos] = rho^-|m| / m! of L_full = P L (sumpy/expansion/local.py:284-309)
+// and P's rows in Horner order, in LDS.  Per box: L (stored) to LDS, then
+// lh[pos] = h[pos] (P L)[m(pos)], then a triple Horner per target (one lane
+// per target) with its gradient.
 template <bool GRAD>
 __global__ __launch_bounds__(WAVE) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
     const double *ty, const double *tz, double rscale, double scale,
     const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
-  extern __shared__ double lh[];  // [nfull] in Horner order
+  extern __shared__ double sh[];
+  const int nf = e.nfull, nst = e.nstored;
+  double *lh = sh;                 // [nfull] Horner-ordered coefficients of a box
+  double *hs = lh + nf;            // [nfull] factors
+  double *Ls = hs + nf;            // [nstored]
+  int *rs = reinterpret_cast<int *>(Ls + nst);   // [nfull + 1] row starts (Horner order)
+  const int nnz = e.proj_nnz;
+  double *pv = reinterpret_cast<double *>(rs + ((nf + 2) & ~1));  // [nnz] values
+  int *pc = reinterpret_cast<int *>(pv + nnz);   // [nnz] columns
   const int lane = threadIdx.x;
   const int p = e.order;
+  const double rinv = 1.0/rscale;
+  {  // row starts of P in Horner order: each lane scans a contiguous range
+    const int ch = (nf + WAVE - 1)/WAVE, p0 = min(nf, lane*ch), p1 = min(nf, p0 + ch);
+    int loc = 0;
+    for (int pos = p0; pos < p1; ++pos) {
+      const int f = e.horner_order[pos];
+      loc += e.proj_row_starts[f + 1] - e.proj_row_starts[f];
+    }
+    int incl = loc;
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int y = __shfl_up(incl, o, WAVE);
+      if (lane >= o) incl += y;
+    }
+    int n = incl - loc;
+    for (int pos = p0; pos < p1; ++pos) {
+      rs[pos] = n;
+      const int f = e.horner_order[pos];
+      n += e.proj_row_starts[f + 1] - e.proj_row_starts[f];
+    }
+    if (lane == WAVE - 1) rs[nf] = incl;
+  }
+  __syncthreads();
+  for (int pos = lane; pos < nf; pos += WAVE) {
+    const int f = e.horner_order[pos];
+    const int i = e.full_ids[f], j = e.full_ids[nf + f], k = e.full_ids[2*nf + f];
+    double h = inv_fact(i)*inv_fact(j)*inv_fact(k);
+    for (int q = 0; q < i + j + k; ++q) h *= rinv;
+    hs[pos] = h;
+    int d = rs[pos];
+    for (int q = e.proj_row_starts[f]; q < e.proj_row_starts[f + 1]; ++q, ++d) {
+      pv[d] = e.proj_vals[q];
+      pc[d] = e.proj_cols[q];
+    }
+  }
   for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
     const int tb = target_boxes[ib];
-    const double *L = src + (size_t)(tb - sbase)*e.nstored;
+    const double *L = src + (size_t)(tb - sbase)*nst;
     __syncthreads();
-    for (int pos = lane; pos < e.nfull; pos += WAVE) {
-      const int f = e.horner_order[pos];
+    for (int q = lane; q < nst; q += WAVE) Ls[q] = L[q];
+    __syncthreads();
+    for (int pos = lane; pos < nf; pos += WAVE) {
       double v = 0;
-      for (int q = e.proj_row_starts[f]; q < e.proj_row_starts[f + 1]; ++q)
-        v += e.proj_vals[q]*L[e.proj_cols[q]];
-      const int i = e.full_ids[f], j = e.full_ids[e.nfull + f], k = e.full_ids[2*e.nfull + f];
-      v *= inv_fact(i)*inv_fact(j)*inv_fact(k)*pow(rscale, -(double)(i + j + k));
-      lh[pos] = v;
+      for (int q = rs[pos]; q < rs[pos + 1]; ++q) v += pv[q]*Ls[pc[q]];
+      lh[pos] = v*hs[pos];
     }
     __syncthreads();
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
@@ -516,21 +598,32 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
   int rc = check_expn(expn, "p2m");
   if (rc) return rc;
   if (nsrc_boxes == 0) return SUMPYHIP_OK;
-  SUMPY_REQUIRE(expn->nfull <= 16*64, "p2m: too many coefficients");
   SUMPY_REQUIRE(!wx || (wy && wz), "p2m: incomplete dipole weights");
-  const size_t lds = sizeof(double)*(3*64*(expn->order + 1) + 4*64 + expn->nfull);
+  const size_t lds = sizeof(double)*(3*P2M_SCH*(expn->order + 1) + 4*P2M_SCH + expn->nfull);
   SUMPY_REQUIRE(lds <= 64*1024, "p2m: order too large");
   hipStream_t st = (hipStream_t)stream;
-  if (wx)
-    p2m_lap3d<true><<<grid_for(nsrc_boxes), 64, lds, st>>>(
-        *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,
-        box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,
-        tgt_base_ibox);
-  else
-    p2m_lap3d<false><<<grid_for(nsrc_boxes), 64, lds, st>>>(
-        *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,
-        box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,
-        tgt_base_ibox);
+  const int grid = grid_for(nsrc_boxes);
+#define SUMPY_P2M(KT)                                                                     \
+  do {                                                                                   \
+    if (wx)                                                                              \
+      p2m_lap3d<true, KT><<<grid, P2M_NT, lds, st>>>(                                    \
+          *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild, \
+          box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,       \
+          tgt_base_ibox);                                                                \
+    else                                                                                 \
+      p2m_lap3d<false, KT><<<grid, P2M_NT, lds, st>>>(                                   \
+          *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild, \
+          box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,       \
+          tgt_base_ibox);                                                                \
+  } while (0)
+  switch (p2m_kt(expn->order)) {
+    case 2: SUMPY_P2M(2); break;
+    case 3: SUMPY_P2M(3); break;
+    case 4: SUMPY_P2M(4); break;
+    case 6: SUMPY_P2M(6); break;
+    default: SUMPY_P2M(12); break;
+  }
+#undef SUMPY_P2M
   return check_launch("p2m");
 }
 
@@ -588,7 +681,10 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(pot || gx, "l2p: no output");
   SUMPY_REQUIRE(!gx || (gy && gz), "l2p: incomplete gradient outputs");
-  const size_t lds = sizeof(double)*expn->nfull;
+  SUMPY_REQUIRE(expn->proj_nnz > 0, "l2p: empty projection");
+  const size_t lds = sizeof(double)*(2*expn->nfull + expn->nstored) +
+                     sizeof(int)*((expn->nfull + 2) & ~1) +
+                     (sizeof(double) + sizeof(int))*(size_t)expn->proj_nnz;
   SUMPY_REQUIRE(lds <= 64*1024, "l2p: order too large");
   if (gx)
     l2p_lap3d<true><<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(

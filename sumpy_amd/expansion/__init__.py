@@ -70,7 +70,8 @@ class DeviceExpnTables:
             self.cube_index.data_ptr(), self.proj_row_starts.data_ptr(),
             self.proj_cols.data_ptr(), self.proj_vals.data_ptr(),
             self.projT_col_starts.data_ptr(), self.projT_rows.data_ptr(),
-            self.projT_vals.data_ptr(), self.horner_order.data_ptr())
+            self.projT_vals.data_ptr(), self.horner_order.data_ptr(),
+            int(self.proj_vals.numel()))
 
 
 _DEVICE_TABLES: dict = {}
