@@ -42,7 +42,17 @@ __device__ __forceinline__ double inv_fact(int n) {
 // {{{ P2M
 
 constexpr int P2M_NT = 128;   // threads per block, one source box at a time
-constexpr int P2M_SCH = 64;   // sources per LDS chunk
+constexpr int P2M_SCH = 32;   // sources per LDS chunk
+
+// 1/q for q < 64 (power tables without divisions)
+__constant__ double c_inv[64] = {
+    0.0, 1.0, 1.0/2, 1.0/3, 1.0/4, 1.0/5, 1.0/6, 1.0/7, 1.0/8, 1.0/9, 1.0/10, 1.0/11,
+    1.0/12, 1.0/13, 1.0/14, 1.0/15, 1.0/16, 1.0/17, 1.0/18, 1.0/19, 1.0/20, 1.0/21,
+    1.0/22, 1.0/23, 1.0/24, 1.0/25, 1.0/26, 1.0/27, 1.0/28, 1.0/29, 1.0/30, 1.0/31,
+    1.0/32, 1.0/33, 1.0/34, 1.0/35, 1.0/36, 1.0/37, 1.0/38, 1.0/39, 1.0/40, 1.0/41,
+    1.0/42, 1.0/43, 1.0/44, 1.0/45, 1.0/46, 1.0/47, 1.0/48, 1.0/49, 1.0/50, 1.0/51,
+    1.0/52, 1.0/53, 1.0/54, 1.0/55, 1.0/56, 1.0/57, 1.0/58, 1.0/59, 1.0/60, 1.0/61,
+    1.0/62, 1.0/63};
 
 // Number of (i, j, k-range) tasks when each pair (i, j), i + j <= p, is cut
 // into pieces of at most kt consecutive k (k <= p - i - j).
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
           for (int d = 0; d < 3; ++d) {
             double v = 1;
             row[d*P1] = 1;
-            for (int q = 1; q < P1; ++q) { v *= a[d]/q; row[d*P1 + q] = v; }
+            for (int q = 1; q < P1; ++q) { v *= a[d]*c_inv[q]; row[d*P1 + q] = v; }
           }
           sw[tid*4] = w0[s];
           if (DIP) {
@@ -292,39 +302,47 @@ __global__ __launch_bounds__(WAVE) void l2l_lap3d(
 
 // {{{ L2P
 
-// Per block (once): the Horner-ordered coefficient factors
-// h[pos] = rho^-|m| / m! of L_full = P L (sumpy/expansion/local.py:284-309)
-// and P's rows in Horner order, in LDS.  Per box: L (stored) to LDS, then
-// lh[pos] = h[pos] (P L)[m(pos)], then a triple Horner per target (one lane
-// per target) with its gradient.
+constexpr int L2P_NW = 4;  // waves per block, each on its own target box
+
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per block (once): the Horner-ordered factors h[pos] = rho^-|m| / m! of
+// L_full = P L (sumpy/expansion/local.py:284-309) and P's rows in Horner
+// order, in LDS.  Each wave then takes its own boxes: L (stored) to LDS,
+// lh[pos] = h[pos] (P L)[m(pos)], and a triple Horner per target (one lane
+// per target) with its gradient.  Four boxes per block are in flight at once.
 template <bool GRAD>
-__global__ __launch_bounds__(WAVE) void l2p_lap3d(
+__global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
     const double *ty, const double *tz, double rscale, double scale,
     const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
   extern __shared__ double sh[];
-  const int nf = e.nfull, nst = e.nstored;
-  double *lh = sh;                 // [nfull] Horner-ordered coefficients of a box
-  double *hs = lh + nf;            // [nfull] factors
-  double *Ls = hs + nf;            // [nstored]
-  int *rs = reinterpret_cast<int *>(Ls + nst);   // [nfull + 1] row starts (Horner order)
-  const int nnz = e.proj_nnz;
-  double *pv = reinterpret_cast<double *>(rs + ((nf + 2) & ~1));  // [nnz] values
-  int *pc = reinterpret_cast<int *>(pv + nnz);   // [nnz] columns
-  const int lane = threadIdx.x;
+  const int nf = e.nfull, nst = e.nstored, nnz = e.proj_nnz;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double *hs = sh;                          // [nfull] factors
+  double *pv = hs + nf;                     // [nnz] values of P (Horner row order)
+  double *lh = pv + nnz + w*(nf + nst);     // per wave: [nfull] Horner coefficients
+  double *Ls = lh + nf;                     //           [nstored] L of the box
+  int *rs = reinterpret_cast<int *>(pv + nnz + L2P_NW*(nf + nst));  // [nfull + 1]
+  int *pc = rs + nf + 1;                    // [nnz] columns of P
   const int p = e.order;
   const double rinv = 1.0/rscale;
-  {  // row starts of P in Horner order: each lane scans a contiguous range
-    const int ch = (nf + WAVE - 1)/WAVE, p0 = min(nf, lane*ch), p1 = min(nf, p0 + ch);
+  if (w == 0) {  // row starts of P in Horner order: each lane scans a contiguous range
+    const int ch = (nf + 63)/64, p0 = min(nf, lane*ch), p1 = min(nf, p0 + ch);
     int loc = 0;
     for (int pos = p0; pos < p1; ++pos) {
       const int f = e.horner_order[pos];
       loc += e.proj_row_starts[f + 1] - e.proj_row_starts[f];
     }
     int incl = loc;
-    for (int o = 1; o < WAVE; o <<= 1) {
-      const int y = __shfl_up(incl, o, WAVE);
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(incl, o, 64);
       if (lane >= o) incl += y;
     }
     int n = incl - loc;
@@ -333,10 +351,10 @@ __global__ __launch_bounds__(WAVE) void l2p_lap3d(
       const int f = e.horner_order[pos];
       n += e.proj_row_starts[f + 1] - e.proj_row_starts[f];
     }
-    if (lane == WAVE - 1) rs[nf] = incl;
+    if (lane == 63) rs[nf] = incl;
   }
   __syncthreads();
-  for (int pos = lane; pos < nf; pos += WAVE) {
+  for (int pos = tid; pos < nf; pos += 64*L2P_NW) {
     const int f = e.horner_order[pos];
     const int i = e.full_ids[f], j = e.full_ids[nf + f], k = e.full_ids[2*nf + f];
     double h = inv_fact(i)*inv_fact(j)*inv_fact(k);
@@ -348,21 +366,22 @@ __global__ __launch_bounds__(WAVE) void l2p_lap3d(
       pc[d] = e.proj_cols[q];
     }
   }
-  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+  __syncthreads();
+  for (int ib = blockIdx.x*L2P_NW + w; ib < ntgt; ib += gridDim.x*L2P_NW) {
     const int tb = target_boxes[ib];
     const double *L = src + (size_t)(tb - sbase)*nst;
-    __syncthreads();
-    for (int q = lane; q < nst; q += WAVE) Ls[q] = L[q];
-    __syncthreads();
-    for (int pos = lane; pos < nf; pos += WAVE) {
+    wave_sync();
+    for (int q = lane; q < nst; q += 64) Ls[q] = L[q];
+    wave_sync();
+    for (int pos = lane; pos < nf; pos += 64) {
       double v = 0;
       for (int q = rs[pos]; q < rs[pos + 1]; ++q) v += pv[q]*Ls[pc[q]];
       lh[pos] = v*hs[pos];
     }
-    __syncthreads();
+    wave_sync();
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
     const int t0 = bts[tb], nt = btc[tb];
-    for (int tc = 0; tc < nt; tc += WAVE) {
+    for (int tc = 0; tc < nt; tc += 64) {
       const bool valid = tc + lane < nt;
       const int t = t0 + tc + (valid ? lane : 0);
       const double bx = tx[t] - c0, by = ty[t] - c1, bz = tz[t] - c2;
@@ -602,7 +621,7 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
   const size_t lds = sizeof(double)*(3*P2M_SCH*(expn->order + 1) + 4*P2M_SCH + expn->nfull);
   SUMPY_REQUIRE(lds <= 64*1024, "p2m: order too large");
   hipStream_t st = (hipStream_t)stream;
-  const int grid = grid_for(nsrc_boxes);
+  const int grid = nsrc_boxes;  // one box per block: many boxes in flight per CU
 #define SUMPY_P2M(KT)                                                                     \
   do {                                                                                   \
     if (wx)                                                                              \
@@ -682,17 +701,19 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   SUMPY_REQUIRE(pot || gx, "l2p: no output");
   SUMPY_REQUIRE(!gx || (gy && gz), "l2p: incomplete gradient outputs");
   SUMPY_REQUIRE(expn->proj_nnz > 0, "l2p: empty projection");
-  const size_t lds = sizeof(double)*(2*expn->nfull + expn->nstored) +
-                     sizeof(int)*((expn->nfull + 2) & ~1) +
-                     (sizeof(double) + sizeof(int))*(size_t)expn->proj_nnz;
+  const size_t lds = sizeof(double)*(expn->nfull + expn->proj_nnz +
+                                     L2P_NW*(size_t)(expn->nfull + expn->nstored)) +
+                     sizeof(int)*(expn->nfull + 1 + (size_t)expn->proj_nnz);
   SUMPY_REQUIRE(lds <= 64*1024, "l2p: order too large");
+  const int grid = (ntgt_boxes + L2P_NW - 1)/L2P_NW < 8*num_cus()
+                       ? (ntgt_boxes + L2P_NW - 1)/L2P_NW : 8*num_cus();
   if (gx)
-    l2p_lap3d<true><<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+    l2p_lap3d<true><<<grid, 64*L2P_NW, lds, (hipStream_t)stream>>>(
         *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
         box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
         gx, gy, gz);
   else
-    l2p_lap3d<false><<<grid_for(ntgt_boxes), 64, lds, (hipStream_t)stream>>>(
+    l2p_lap3d<false><<<grid, 64*L2P_NW, lds, (hipStream_t)stream>>>(
         *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
         box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
         gx, gy, gz);
