@@ -128,6 +128,7 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
         }
         __syncthreads();
         if (has) {
+          #pragma unroll 4
           for (int s = 0; s < nn; ++s) {
             const double *X = pw + s*3*P1, *Y = X + P1, *Z = Y + P1;
             const double xi = X[ti], yj = Y[tj];
@@ -316,7 +317,7 @@ __device__ __forceinline__ void wave_sync() {
 // order, in LDS.  Each wave then takes its own boxes: L (stored) to LDS,
 // lh[pos] = h[pos] (P L)[m(pos)], and a triple Horner per target (one lane
 // per target) with its gradient.  Four boxes per block are in flight at once.
-template <bool GRAD>
+template <bool GRAD, int KMAX>
 __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
   double *Ls = lh + nf;                     //           [nstored] L of the box
   int *rs = reinterpret_cast<int *>(pv + nnz + L2P_NW*(nf + nst));  // [nfull + 1]
   int *pc = rs + nf + 1;                    // [nnz] columns of P
-  const int p = e.order;
+  const int p = e.order;  // p + 1 <= KMAX
   const double rinv = 1.0/rscale;
   if (w == 0) {  // row starts of P in Horner order: each lane scans a contiguous range
     const int ch = (nf + 63)/64, p0 = min(nf, lane*ch), p1 = min(nf, p0 + ch);
@@ -390,11 +391,20 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
       for (int i = p; i >= 0; --i) {
         double S = 0, Sy = 0, Sz = 0;
         for (int j = p - i; j >= 0; --j) {
+          // the (i, j) run of n coefficients: KMAX loads issued together
+          const int n = p - i - j + 1;
+          double c[KMAX];
+          #pragma unroll
+          for (int q = 0; q < KMAX; ++q) c[q] = lh[pos + (q < n ? q : n - 1)];
           double R = 0, Rz = 0;
-          for (int k = p - i - j; k >= 0; --k) {
-            if (GRAD) Rz = Rz*bz + R;
-            R = R*bz + lh[pos++];
+          #pragma unroll
+          for (int q = 0; q < KMAX; ++q) {
+            if (q < n) {
+              if (GRAD) Rz = Rz*bz + R;
+              R = R*bz + c[q];
+            }
           }
+          pos += n;
           if (GRAD) { Sy = Sy*by + S; Sz = Sz*by + Rz; }
           S = S*by + R;
         }
@@ -707,16 +717,20 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   SUMPY_REQUIRE(lds <= 64*1024, "l2p: order too large");
   const int grid = (ntgt_boxes + L2P_NW - 1)/L2P_NW < 8*num_cus()
                        ? (ntgt_boxes + L2P_NW - 1)/L2P_NW : 8*num_cus();
-  if (gx)
-    l2p_lap3d<true><<<grid, 64*L2P_NW, lds, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
-        box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
-        gx, gy, gz);
-  else
-    l2p_lap3d<false><<<grid, 64*L2P_NW, lds, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,
-        box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,
-        gx, gy, gz);
+  hipStream_t st = (hipStream_t)stream;
+#define SUMPY_L2P_LAUNCH(G, K)                                                             \
+  l2p_lap3d<G, K><<<grid, 64*L2P_NW, lds, st>>>(                                           \
+      *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,      \
+      box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,  \
+      gx, gy, gz)
+  if (expn->order < 8) {
+    if (gx) SUMPY_L2P_LAUNCH(true, 8); else SUMPY_L2P_LAUNCH(false, 8);
+  } else if (expn->order < 12) {
+    if (gx) SUMPY_L2P_LAUNCH(true, 12); else SUMPY_L2P_LAUNCH(false, 12);
+  } else {
+    if (gx) SUMPY_L2P_LAUNCH(true, 25); else SUMPY_L2P_LAUNCH(false, 25);
+  }
+#undef SUMPY_L2P_LAUNCH
   return check_launch("l2p");
 }
 
