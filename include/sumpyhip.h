@@ -187,23 +187,37 @@ int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
                             const double *src_expansions, int kpad, void *packed,
                             void *stream);
 
+/* One tree level of sumpyhip_m2l_apply: tile schedule (built on the host),
+ * class data (FP64, nclasses x ncirc), packed source multipoles (compute
+ * precision, kpad per box), work rows (compute precision, npad per box) and,
+ * when nsplit > 1, a partial buffer of (nsplit - 1) * ntiles * 64 * npad
+ * entries of the compute precision. */
+typedef struct sumpyhip_m2l_level {
+  int ntiles, nsplit;
+  const int *tile_targets;      /* ntiles x 64 target boxes (-1: none) */
+  const int *tile_slot_starts;  /* ntiles + 1 */
+  const int *slot_class;        /* nslots: class of the slot (level-relative) */
+  const int *slot_sources;      /* nslots x 64 source boxes (-1: none) */
+  const double *class_data;
+  const void *packed_src;
+  int src_base_ibox;
+  void *work;
+  int tgt_base_ibox;
+  void *partial;
+} sumpyhip_m2l_level;
+
 /* Translation, replaces M2LUsingTranslationClassesDependentData.__call__
- * (sumpy/e2e.py:310-558) as a class-batched dense contraction on MFMA:
- * for each tile of 64 target boxes and each class slot of the tile,
+ * (sumpy/e2e.py:310-558) as a class-batched dense contraction on MFMA, for up
+ * to 32 levels of the same orders in one launch: for each tile of 64 target
+ * boxes and each class slot of the tile,
  * Y += A_c X with A_c[t, s] = data_c[off_t[t] + off_s[s]] and X the stacked
- * packed multipoles of the tile's sources.  The schedule (tiles, slots,
- * per-slot source boxes) is built on the host.  Overwrites the work rows of
- * the tile targets.  With nsplit > 1 the slots of each tile are split over
- * nsplit blocks; partial (device, (nsplit - 1) * ntiles * 64 * npad entries of
- * the compute precision) holds the sums of splits 1.., which are then added
- * to the work rows in a fixed order (deterministic). */
-int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_targets,
-                       const int *tile_slot_starts, const int *slot_class,
-                       const int *slot_sources, int nl, int nm, int npad, int kpad,
-                       const int *off_t, const int *off_s, int ncirc,
-                       const double *class_data, const void *packed_src,
-                       int src_base_ibox, void *work, int tgt_base_ibox,
-                       int nsplit, void *partial, void *stream);
+ * packed multipoles of the tile's sources.  Overwrites the work rows of the
+ * tile targets.  The slots of a tile are split over nsplit blocks; splits
+ * 1.. write partial sums that are then added to the work rows in a fixed
+ * order (deterministic). */
+int sumpyhip_m2l_apply(int precision, int nlevels, const sumpyhip_m2l_level *levels,
+                       int nl, int nm, int npad, int kpad, const int *off_t,
+                       const int *off_s, int ncirc, void *stream);
 
 /* Postprocessing, replaces M2LPostprocessLocal.__call__
  * (sumpy/e2e.py:767-859): locals[b, i] = work[b, i] * ratio^deg_i. */

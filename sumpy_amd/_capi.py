@@ -36,6 +36,18 @@ class ExpnDesc(ctypes.Structure):
     ]
 
 
+class M2LLevel(ctypes.Structure):
+    """Mirror of ``struct sumpyhip_m2l_level``."""
+    _fields_ = [
+        ("ntiles", ctypes.c_int), ("nsplit", ctypes.c_int),
+        ("tile_targets", ctypes.c_void_p), ("tile_slot_starts", ctypes.c_void_p),
+        ("slot_class", ctypes.c_void_p), ("slot_sources", ctypes.c_void_p),
+        ("class_data", ctypes.c_void_p), ("packed_src", ctypes.c_void_p),
+        ("src_base_ibox", ctypes.c_int), ("work", ctypes.c_void_p),
+        ("tgt_base_ibox", ctypes.c_int), ("partial", ctypes.c_void_p),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -60,8 +72,7 @@ _SIGNATURES = {
     "sumpyhip_octant_translate": [_I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P],
     "sumpyhip_m2l_class_data": [_I, _I, _I, _P, _I, _P, _I, _I, _D, _P, _P],
     "sumpyhip_m2l_preprocess": [_I, _I, _I, _P, _I, _P, _P],
-    "sumpyhip_m2l_apply": [_I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _I,
-                           _P, _P, _I, _P, _I, _I, _P, _P],
+    "sumpyhip_m2l_apply": [_I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _P],
     "sumpyhip_m2l_postprocess": [_I, _I, _I, _I, _P, _P, _D, _P, _P],
     "sumpyhip_m2l_csr": [_I, _I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P,
                          _I, _D, _D, _P, _I, _P, _I, _P],

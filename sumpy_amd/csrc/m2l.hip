@@ -17,6 +17,8 @@
 // and accumulates Y(nl x 64) += A_c (nl x nm) X (nm x 64) with
 // v_mfma_f64_16x16x4 (plain form) or v_mfma_f32_16x16x4 (FP32 form), A
 // fragments gathered straight from data_c in LDS.  Roofline: MFMA.
+#include <algorithm>
+
 #include "common.h"
 
 namespace sumpyhip {
@@ -146,19 +148,25 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4*(lane >> 4) + r; }
 };
 
-struct ApplyArgs {
-  int ntiles;
+// One tree level of a (possibly multi-level) launch.
+struct LevelArgs {
+  int ntiles, nsplit;   // slot ranges of a tile split over nsplit blocks
+  int item_end;         // cumulative (tile, split) items up to this level
+  int sbase, tbase;
   const int *tile_targets, *tile_slot_starts, *slot_class, *slot_sources;
-  int nl, nm, npad, kpad;
-  const int *off_t, *off_s;
-  int ncirc;
   const double *class_data;
   const void *packed;
-  int sbase;
   void *work;
-  int tbase;
-  int nsplit;      // slot ranges of a tile split over this many blocks
-  void *partial;   // (nsplit - 1) x ntiles*64 x npad sums of splits 1..
+  void *partial;        // (nsplit - 1) x ntiles*64 x npad sums of splits 1..
+};
+
+constexpr int M2L_MAXLEV = 32;
+
+struct ApplyArgs {
+  int nlev;
+  int nl, nm, npad, kpad, ncirc;
+  const int *off_t, *off_s;
+  LevelArgs lv[M2L_MAXLEV];
 };
 
 int set_lds_limit(const void *fn, size_t bytes);
@@ -219,11 +227,18 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   T *sD = reinterpret_cast<T *>(lds_raw);                       // [ncirc]
   T *sX = sD + ((a.ncirc + 3) & ~3);                            // [kpad][XS]
   int *sOff = reinterpret_cast<int *>(sX + (size_t)a.kpad*C::XS);  // [kpad]
-  const T *packed = reinterpret_cast<const T *>(a.packed);
-  T *work = reinterpret_cast<T *>(a.work);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kq = a.kpad/C::VN;          // vectors per source row
+  // (tile, split) item of this block and its level; items are mapped
+  // XCD-major (the grid is a multiple of 8)
+  const int per_xcd = gridDim.x >> 3;
+  const int item = (blockIdx.x & 7)*per_xcd + (blockIdx.x >> 3);
+  int lev = 0;
+  while (lev < a.nlev - 1 && item >= a.lv[lev].item_end) ++lev;
+  const LevelArgs &L = a.lv[lev];
+  const T *packed = reinterpret_cast<const T *>(L.packed);
+  T *work = reinterpret_cast<T *>(L.work);
   // Padding rows (>= nl) and columns (>= nm) read an arbitrary valid entry of
   // data_c: the matching X rows are zero (preprocessing pads with zeros) and
   // the padding output rows are never read back.
@@ -239,13 +254,13 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   // global -> registers for one slot; every load is unconditional (clamped
   // address, then a select) so all of them are in flight together
   auto fetch = [&](int slot) {
-    const double *D = a.class_data + (size_t)a.slot_class[slot]*a.ncirc;
+    const double *D = L.class_data + (size_t)L.slot_class[slot]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
       dreg[i] = D[e < a.ncirc ? e : 0];
     }
-    const int sb = a.slot_sources[(size_t)slot*TW + lane];
-    const T *row = packed + (size_t)((sb >= 0 ? sb : a.sbase) - a.sbase)*a.kpad;
+    const int sb = L.slot_sources[(size_t)slot*TW + lane];
+    const T *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*a.kpad;
     for (int i = 0; i < C::XPT; ++i) {
       const int q = wave + 4*i;
       xreg[i] = *reinterpret_cast<const vec_t *>(row + (q < kq ? q : 0)*C::VN);
@@ -270,17 +285,15 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
 
   const T *xlane = sX + (lane >> 4)*C::XS + (lane & 15);
   const int *olane = sOff + (lane >> 4);
-  const int nitems = a.ntiles*a.nsplit;
-  const int per_xcd = gridDim.x >> 3;   // the grid is a multiple of 8
-  const int item = (blockIdx.x & 7)*per_xcd + (blockIdx.x >> 3);
-  if (item < nitems) {
-    const int tile = item/a.nsplit, split = item - tile*a.nsplit;
+  if (item < L.item_end) {
+    const int li = item - (lev ? a.lv[lev - 1].item_end : 0);
+    const int tile = li/L.nsplit, split = li - tile*L.nsplit;
     acc_t acc[RBW][4];
     for (int r = 0; r < RBW; ++r)
       for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
-    const int t0 = a.tile_slot_starts[tile], nsl = a.tile_slot_starts[tile + 1] - t0;
-    const int s0 = t0 + (int)((long long)nsl*split/a.nsplit);
-    const int s1 = t0 + (int)((long long)nsl*(split + 1)/a.nsplit);
+    const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
+    const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
+    const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
     if (s0 < s1) {
       fetch(s0);
       __syncthreads();
@@ -320,33 +333,36 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
       if (rb >= nrb) break;
       for (int cb = 0; cb < 4; ++cb) {
         const int col = cb*16 + (lane & 15);
-        const int tb = a.tile_targets[(size_t)tile*TW + col];
+        const int tb = L.tile_targets[(size_t)tile*TW + col];
         if (tb < 0) continue;
         T *dst = split == 0
-            ? work + (size_t)(tb - a.tbase)*a.npad
-            : reinterpret_cast<T *>(a.partial) +
-                  ((size_t)(split - 1)*a.ntiles*TW + (size_t)tile*TW + col)*a.npad;
+            ? work + (size_t)(tb - L.tbase)*a.npad
+            : reinterpret_cast<T *>(L.partial) +
+                  ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
         for (int i = 0; i < 4; ++i) dst[rb*16 + Mfma<T>::row(lane, i)] = acc[r][cb][i];
       }
     }
   }
 }
 
-// work[target] += sum over splits 1.. of partial[split - 1][tile position]
+// work[target] += sum over splits 1.. of partial[split - 1][tile position],
+// for every level with nsplit > 1 (blockIdx.y = level)
 template <typename T>
-__global__ __launch_bounds__(256) void m2l_reduce_splits(int ntiles, int nsplit, int npad,
-                                                         const int *tile_targets,
-                                                         const T *partial, T *work,
-                                                         int tbase) {
-  const size_t npos = (size_t)ntiles*TW, n = npos*npad;
+__global__ __launch_bounds__(256) void m2l_reduce_splits(ApplyArgs a) {
+  const LevelArgs &L = a.lv[blockIdx.y];
+  if (L.nsplit < 2) return;
+  const int npad = a.npad;
+  const T *partial = reinterpret_cast<const T *>(L.partial);
+  T *work = reinterpret_cast<T *>(L.work);
+  const size_t npos = (size_t)L.ntiles*TW, n = npos*npad;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
     const size_t pos = i/npad;
-    const int tb = tile_targets[pos];
+    const int tb = L.tile_targets[pos];
     if (tb < 0) continue;
     T v = 0;
-    for (int j = 0; j < nsplit - 1; ++j) v += partial[(size_t)j*n + i];
-    work[(size_t)(tb - tbase)*npad + (i - pos*npad)] += v;
+    for (int j = 0; j < L.nsplit - 1; ++j) v += partial[(size_t)j*n + i];
+    work[(size_t)(tb - L.tbase)*npad + (i - pos*npad)] += v;
   }
 }
 
@@ -357,17 +373,17 @@ int launch_apply(const ApplyArgs &a, hipStream_t st) {
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
   int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW>, lds);
   if (rc) return rc;
-  const long long nitems = (long long)a.ntiles*a.nsplit;
-  SUMPY_REQUIRE(nitems < (1ll << 30), "m2l_apply: too many tiles x splits");
+  const long long nitems = a.lv[a.nlev - 1].item_end;
   const int grid = (int)((nitems + 7)/8*8);
   m2l_apply_kernel<T, RBW><<<grid, 256, lds, st>>>(a);
   rc = check_launch("m2l_apply");
-  if (rc || a.nsplit == 1) return rc;
-  const size_t n = (size_t)a.ntiles*TW*a.npad;
-  const int rgrid = (int)((n + 255)/256 < 16384 ? (n + 255)/256 : 16384);
-  m2l_reduce_splits<T><<<rgrid, 256, 0, st>>>(a.ntiles, a.nsplit, a.npad, a.tile_targets,
-                                              reinterpret_cast<const T *>(a.partial),
-                                              reinterpret_cast<T *>(a.work), a.tbase);
+  if (rc) return rc;
+  size_t nmax = 0;
+  for (int l = 0; l < a.nlev; ++l)
+    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*TW*a.npad);
+  if (nmax == 0) return SUMPYHIP_OK;
+  const int rgrid = (int)((nmax + 255)/256 < 8192 ? (nmax + 255)/256 : 8192);
+  m2l_reduce_splits<T><<<dim3(rgrid, a.nlev), 256, 0, st>>>(a);
   return check_launch("m2l_reduce_splits");
 }
 
@@ -492,24 +508,38 @@ extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int n
   return check_launch("m2l_postprocess");
 }
 
-extern "C" int sumpyhip_m2l_apply(int precision, int ntiles, const int *tile_targets,
-                                  const int *tile_slot_starts, const int *slot_class,
-                                  const int *slot_sources, int nl, int nm, int npad, int kpad,
-                                  const int *off_t, const int *off_s, int ncirc,
-                                  const double *class_data, const void *packed_src,
-                                  int src_base_ibox, void *work, int tgt_base_ibox,
-                                  int nsplit, void *partial, void *stream) {
+extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
+                                  const sumpyhip_m2l_level *levels, int nl, int nm, int npad,
+                                  int kpad, const int *off_t, const int *off_s, int ncirc,
+                                  void *stream) {
   SUMPY_REQUIRE(precision == 0 || precision == 1, "m2l_apply: unknown precision %d", precision);
-  SUMPY_REQUIRE(nsplit >= 1, "m2l_apply: nsplit %d < 1", nsplit);
-  SUMPY_REQUIRE(nsplit == 1 || partial != nullptr, "m2l_apply: nsplit > 1 needs a partial buffer");
   SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl, "m2l_apply: npad %d must be a multiple of 16 >= nl",
                 npad);
   SUMPY_REQUIRE(kpad % 4 == 0 && kpad >= nm, "m2l_apply: kpad %d must be a multiple of 4 >= nm",
                 kpad);
-  if (ntiles == 0) return SUMPYHIP_OK;
-  ApplyArgs a{ntiles, tile_targets, tile_slot_starts, slot_class, slot_sources, nl, nm,
-              npad, kpad, off_t, off_s, ncirc, class_data, packed_src, src_base_ibox,
-              work, tgt_base_ibox, nsplit, partial};
+  SUMPY_REQUIRE(nlevels >= 0 && nlevels <= M2L_MAXLEV, "m2l_apply: %d levels (at most %d)",
+                nlevels, M2L_MAXLEV);
+  ApplyArgs a{};
+  a.nl = nl; a.nm = nm; a.npad = npad; a.kpad = kpad; a.ncirc = ncirc;
+  a.off_t = off_t; a.off_s = off_s;
+  long long items = 0;
+  for (int l = 0; l < nlevels; ++l) {
+    const sumpyhip_m2l_level &in = levels[l];
+    if (in.ntiles == 0) continue;
+    SUMPY_REQUIRE(in.ntiles > 0 && in.nsplit >= 1, "m2l_apply: bad tiles/splits");
+    SUMPY_REQUIRE(in.nsplit == 1 || in.partial != nullptr,
+                  "m2l_apply: nsplit > 1 needs a partial buffer");
+    items += (long long)in.ntiles*in.nsplit;
+    SUMPY_REQUIRE(items < (1ll << 30), "m2l_apply: too many tiles x splits");
+    LevelArgs &o = a.lv[a.nlev++];
+    o.ntiles = in.ntiles; o.nsplit = in.nsplit; o.item_end = (int)items;
+    o.sbase = in.src_base_ibox; o.tbase = in.tgt_base_ibox;
+    o.tile_targets = in.tile_targets; o.tile_slot_starts = in.tile_slot_starts;
+    o.slot_class = in.slot_class; o.slot_sources = in.slot_sources;
+    o.class_data = in.class_data; o.packed = in.packed_src; o.work = in.work;
+    o.partial = in.partial;
+  }
+  if (a.nlev == 0) return SUMPYHIP_OK;
   hipStream_t st = (hipStream_t)stream;
   return precision == 0 ? dispatch_apply<double>(a, st) : dispatch_apply<float>(a, st);
 }

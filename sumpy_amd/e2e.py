@@ -12,6 +12,8 @@
 
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -413,9 +415,9 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             self._schedules[key] = sched
         return sched
 
-    def __call__(self, actx, **kwargs):
-        tr = self.tgt_expansion.m2l_translation
-        m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+    def level_entry(self, actx, **kwargs):
+        """The launch description of one level (for :func:`m2l_apply_levels`),
+        or None when the level has no target tiles."""
         data = kwargs["m2l_translation_classes_dependent_data"]
         src, work = kwargs["src_expansions"], kwargs["tgt_expansions"]
         sched = self.schedule(
@@ -423,25 +425,57 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             kwargs["m2l_translation_classes_lists"],
             kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"])
         if sched["ntiles"] == 0:
-            return work
-        ntiles, nsplit = sched["ntiles"], sched["nsplit"]
+            return None
+        return dict(op=self, sched=sched, data=data.contiguous(), src=src,
+                    sbase=int(kwargs["src_base_ibox"]), work=work,
+                    tbase=int(kwargs["tgt_base_ibox"]))
+
+    def __call__(self, actx, **kwargs):
+        entry = self.level_entry(actx, **kwargs)
+        if entry is not None:
+            m2l_apply_levels(actx, [entry])
+        return kwargs["tgt_expansions"]
+
+
+def m2l_apply_levels(actx, entries):
+    """One launch of the MFMA contraction over several levels (same
+    translation operator orders), largest level first."""
+    entries = [e for e in entries if e is not None]
+    if not entries:
+        return
+    op = entries[0]["op"]
+    tr = op.tgt_expansion.m2l_translation
+    m = _m2l_tabs(op.src_expansion, op.tgt_expansion, actx.device)
+    for e in entries[1:]:
+        if (e["op"].src_expansion.order, e["op"].tgt_expansion.order,
+                e["op"].tgt_expansion.m2l_translation.precision) != \
+                (op.src_expansion.order, op.tgt_expansion.order, tr.precision):
+            raise ValueError("m2l_apply_levels: levels must share orders and precision")
+    entries = sorted(entries, key=lambda e: -e["sched"]["npairs"])
+    npad, kpad = int(entries[0]["work"].shape[1]), int(entries[0]["src"].shape[1])
+    levels, keep = [], []
+    npairs = 0
+    for e in entries:
+        sched, work = e["sched"], e["work"]
+        if int(work.shape[1]) != npad or int(e["src"].shape[1]) != kpad:
+            raise ValueError("m2l_apply_levels: inconsistent row padding")
         partial = None
-        if nsplit > 1:
-            partial = torch.empty((nsplit - 1)*ntiles*TILE*int(work.shape[1]),
+        if sched["nsplit"] > 1:
+            partial = torch.empty((sched["nsplit"] - 1)*sched["ntiles"]*TILE*npad,
                                   dtype=work.dtype, device=work.device)
-        flops = 2.0*m["nl"]*m["nm"]*sched["npairs"]
-        with kernel_span(f"m2l_apply_{tr.precision}", flops):
-            _capi.call("sumpyhip_m2l_apply", tr.precision_id, ntiles,
-                       _capi.ptr(sched["tile_targets"]), _capi.ptr(sched["tile_slot_starts"]),
-                       _capi.ptr(sched["slot_class"]), _capi.ptr(sched["slot_sources"]),
-                       m["nl"], m["nm"], int(work.shape[1]), int(src.shape[1]),
-                       _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), m["ncirc"],
-                       _capi.ptr(data.contiguous()), _capi.ptr(src),
-                       int(kwargs["src_base_ibox"]), _capi.ptr(work),
-                       int(kwargs["tgt_base_ibox"]), nsplit,
-                       _capi.ptr(partial) if partial is not None else None,
-                       actx.stream_handle)
-        return work
+            keep.append(partial)
+        npairs += sched["npairs"]
+        levels.append(_capi.M2LLevel(
+            sched["ntiles"], sched["nsplit"], _capi.ptr(sched["tile_targets"]),
+            _capi.ptr(sched["tile_slot_starts"]), _capi.ptr(sched["slot_class"]),
+            _capi.ptr(sched["slot_sources"]), _capi.ptr(e["data"]), _capi.ptr(e["src"]),
+            e["sbase"], _capi.ptr(work), e["tbase"], _capi.ptr(partial)))
+    arr = (_capi.M2LLevel*len(levels))(*levels)
+    flops = 2.0*m["nl"]*m["nm"]*npairs
+    with kernel_span(f"m2l_apply_{tr.precision}", flops):
+        _capi.call("sumpyhip_m2l_apply", tr.precision_id, len(levels), ctypes.addressof(arr),
+                   m["nl"], m["nm"], npad, kpad, _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]),
+                   m["ncirc"], actx.stream_handle)
 
 
 def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_start,

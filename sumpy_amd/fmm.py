@@ -451,6 +451,12 @@ class SumpyExpansionWrangler:
 
     def multipole_to_local(self, actx, level_start_target_box_nrs, target_boxes,
                            src_box_starts, src_box_lists, mpole_exps):
+        """M2L of every level.  With translation classes the levels are
+        preprocessed one by one, contracted in a single MFMA launch
+        (:func:`sumpy_amd.e2e.m2l_apply_levels`, levels of equal order
+        batched) and postprocessed one by one; the result equals the
+        reference's per-level loop (``fmm.py:777-920``)."""
+        from sumpy_amd.e2e import m2l_apply_levels
         local_exps = self.local_expansion_zeros(actx)
         starts = _host(level_start_target_box_nrs)
         use_pre = self.supports_translation_classes
@@ -458,6 +464,7 @@ class SumpyExpansionWrangler:
             pre_mpoles = self.m2l_preproc_mpole_expansion_zeros(actx, mpole_exps)
             work = self.m2l_work_array_zeros(actx, local_exps)
         tr_classes = self.m2l_translation_class_level_start_box_nrs()
+        batches = {}   # order -> [(lev, entry, work view)]
         for lev in range(self.tree.nlevels):
             start, stop = starts[lev:lev + 2]
             if start == stop:
@@ -486,8 +493,14 @@ class SumpyExpansionWrangler:
             if "m2l_translation_classes_dependent_data" in kwargs and \
                     kwargs["m2l_translation_classes_dependent_data"].numel() == 0:
                 continue
-            m2l(actx, **kwargs)
             if use_pre:
+                batches.setdefault(order, []).append(
+                    (lev, m2l.level_entry(actx, **kwargs), tview))
+            else:
+                m2l(actx, **kwargs)
+        for order, levs in batches.items():
+            m2l_apply_levels(actx, [entry for _, entry, _ in levs])
+            for lev, _, tview in levs:
                 _, lview = self.local_expansions_view(local_exps, lev)
                 self.tree_indep.m2l_postprocess_local_kernel(order, order)(
                     actx, tgt_expansions=lview, tgt_expansions_before_postprocessing=tview,
