@@ -230,13 +230,19 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kq = a.kpad/C::VN;          // vectors per source row
-  // (tile, split) item of this block and its level; items are mapped
-  // XCD-major (the grid is a multiple of 8)
-  const int per_xcd = gridDim.x >> 3;
-  const int item = (blockIdx.x & 7)*per_xcd + (blockIdx.x >> 3);
-  int lev = 0;
-  while (lev < a.nlev - 1 && item >= a.lv[lev].item_end) ++lev;
-  const LevelArgs &L = a.lv[lev];
+  // (tile, split) item of this block and its level.  Consecutive block ids
+  // go to different XCDs: XCD x takes the x-th eighth of every level's items
+  // (a contiguous, spatially ordered range), levels in order.
+  const int xcd = blockIdx.x & 7;
+  int r = blockIdx.x >> 3, lev = 0, li = -1;
+  for (; lev < a.nlev; ++lev) {
+    const int n = a.lv[lev].item_end - (lev ? a.lv[lev - 1].item_end : 0);
+    const int per = (n + 7)/8, lo = min(xcd*per, n), cnt = min(lo + per, n) - lo;
+    if (r < cnt) { li = lo + r; break; }
+    r -= cnt;
+  }
+  if (li < 0) return;
+  const LevelArgs L = a.lv[lev];
   const T *packed = reinterpret_cast<const T *>(L.packed);
   T *work = reinterpret_cast<T *>(L.work);
   // Padding rows (>= nl) and columns (>= nm) read an arbitrary valid entry of
@@ -285,8 +291,7 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
 
   const T *xlane = sX + (lane >> 4)*C::XS + (lane & 15);
   const int *olane = sOff + (lane >> 4);
-  if (item < L.item_end) {
-    const int li = item - (lev ? a.lv[lev - 1].item_end : 0);
+  {
     const int tile = li/L.nsplit, split = li - tile*L.nsplit;
     acc_t acc[RBW][4];
     for (int r = 0; r < RBW; ++r)
@@ -373,8 +378,10 @@ int launch_apply(const ApplyArgs &a, hipStream_t st) {
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
   int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW>, lds);
   if (rc) return rc;
-  const long long nitems = a.lv[a.nlev - 1].item_end;
-  const int grid = (int)((nitems + 7)/8*8);
+  long long per_xcd = 0;  // items of XCD 0 (the largest share)
+  for (int l = 0; l < a.nlev; ++l)
+    per_xcd += (a.lv[l].item_end - (l ? a.lv[l - 1].item_end : 0) + 7)/8;
+  const int grid = (int)(8*per_xcd);
   m2l_apply_kernel<T, RBW><<<grid, 256, lds, st>>>(a);
   rc = check_launch("m2l_apply");
   if (rc) return rc;
