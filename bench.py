@@ -34,6 +34,25 @@ PEAKS = {  # MI355X published dense peaks, TFLOP/s
 }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_m2l_laplace3d.json")
+
+
+def measured_traffic(form, args):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), when they were taken on this workload; else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    c = t.get("config", {})
+    if (c.get("n"), c.get("order"), c.get("max_particles_in_box"), c.get("distribution")) != \
+            (args.n, args.order, args.max_particles, args.distribution):
+        return None
+    return t.get("bytes_per_launch", {}).get(form)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -202,6 +221,9 @@ def main():
     ap.add_argument("--distribution", default="uniform", choices=["uniform", "gaussian8"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nsample", type=int, default=4096)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="one warm-up and one measured pass per form, then a 256 MiB "
+                         "device copy (counter calibration); for rocprofv3 --pmc runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -229,6 +251,20 @@ def main():
         "fp64": make_wrangler(actx, tree, trav, args.order, "fp64", "fp64"),
         "mixed": make_wrangler(actx, tree, trav, args.order, "fp32", "fp32"),
     }
+    if args.traffic_probe:
+        for name, wr in forms.items():
+            wr.multipole_to_local_precompute()
+            for _ in range(2):
+                drive_fmm(actx, wr, [q_dev])
+            torch.cuda.synchronize()
+            log(f"{name}: probe passes done")
+        src = torch.ones(2**25, dtype=torch.float64, device=actx.device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy of 256 MiB done")
+        return
+
     results = {}
     setup = {}
     for name, wr in forms.items():
@@ -293,7 +329,7 @@ def main():
     achieved = flops/(sp[dom]["ms"]*1e-3)/1e12
     roofline = {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
                 "peak": peak, "unit": "TFLOP/s", "frac": round(achieved/peak, 4),
-                "traffic": None,
+                "traffic": measured_traffic("fp32" if dom.endswith("fp32") else "fp64", args),
                 "launches_per_step": sp[dom]["launches"]//args.steps,
                 "avg_launch_ms": sp[dom]["ms"]/sp[dom]["launches"]}
 
