@@ -26,7 +26,6 @@ from sumpy_amd.tools import as_i32
 TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
 GROUP = 16  # target boxes per octant-translation group (matches the HIP kernel)
 MIN_SLOTS_PER_SPLIT = 12
-ITEMS_PER_CU = 8  # (tile, split) items per CU in one M2L launch
 
 
 class E2EBase:
@@ -454,7 +453,7 @@ def m2l_apply_levels(actx, entries):
             raise ValueError("m2l_apply_levels: levels must share orders and precision")
     entries = sorted(entries, key=lambda e: -e["sched"]["npairs"])
     npad, kpad = int(entries[0]["work"].shape[1]), int(entries[0]["src"].shape[1])
-    nsplits = choose_nsplits([e["sched"] for e in entries], actx.device)
+    nsplits = [e["sched"]["nsplit"] for e in entries]
     levels, keep = [], []
     npairs = 0
     for e, nsplit in zip(entries, nsplits):
@@ -538,23 +537,6 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
                 slot_sources=dev(slot_sources.reshape(-1)), nslots=len(ci),
                 npairs=int(len(e_src)),
                 nsplit=choose_nsplit(ntiles, len(ci), actx.device))
-
-
-def choose_nsplits(scheds, device):
-    """Splits per level for one launch over all *scheds*: the slots of a tile
-    are cut into pieces of about S = max(MIN_SLOTS_PER_SPLIT, total slots /
-    (ITEMS_PER_CU CUs)) slots, so that the launch has several items per CU and
-    no item is much longer than the others (each split > 0 costs one write
-    and read of its partial sums)."""
-    ncu = torch.cuda.get_device_properties(device).multi_processor_count \
-        if torch.device(device).type == "cuda" else 256
-    total = sum(s["nslots"] for s in scheds)
-    target = max(MIN_SLOTS_PER_SPLIT, -(-total//(ITEMS_PER_CU*ncu)))
-    out = []
-    for s in scheds:
-        per_tile = -(-s["nslots"]//max(1, s["ntiles"]))
-        out.append(int(max(1, -(-per_tile//target))))
-    return out
 
 
 def choose_nsplit(ntiles, nslots, device):
