@@ -270,6 +270,23 @@ int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntargets,
                    const double *sy, const double *nx, const double *ny,
                    const double *density, double scale, double *out, void *stream);
 
+/* Adjoint of sumpyhip_qbx2d for the single (src_kind 0) and double (1) layer
+ * without target derivatives: with g = grad_out (per target), the gradients of
+ * sum_t g_t out_t with respect to density, source positions, source normals
+ * (DLP), target positions and expansion centres.  Any output pair may be
+ * NULL.  Two kernels: one parallel over targets (d/dt, d/dc), one over
+ * sources (d/dsigma, d/ds, d/dn).  No reference counterpart (sumpy has no
+ * adjoints); the math is the a- and b-derivative of the line-Taylor sum of
+ * sumpy/expansion/local.py:132-185. */
+int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *tx,
+                           const double *ty, const double *cx, const double *cy,
+                           int nsources, const double *sx, const double *sy,
+                           const double *nx, const double *ny, const double *density,
+                           double scale, const double *grad_out, double *d_density,
+                           double *d_sx, double *d_sy, double *d_nx, double *d_ny,
+                           double *d_tx, double *d_ty, double *d_cx, double *d_cy,
+                           void *stream);
+
 /* }}} */
 
 #ifdef __cplusplus

@@ -70,6 +70,8 @@ _SIGNATURES = {
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
     "sumpyhip_octant_translate": [_I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P],
+    "sumpyhip_qbx2d_adjoint": [_I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P,
+                               _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     "sumpyhip_m2l_class_data": [_I, _I, _I, _P, _I, _P, _I, _I, _D, _P, _P],
     "sumpyhip_m2l_preprocess": [_I, _I, _I, _P, _I, _P, _P],
     "sumpyhip_m2l_apply": [_I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _P],

@@ -168,6 +168,149 @@ __global__ __launch_bounds__(QB) void qbx2d_kernel(int order, int src_kind, int 
 
 // }}}
 
+// {{{ QBX adjoints (gradients of sum_t g_t out_t)
+
+// Line-Taylor sums of the 2D Laplace SLP / DLP and their a- and b-derivatives
+// (complex notation, a = c - s, b = t - c, u = -b/a, p = order):
+//   SLP: F = log a + sum_{i=1}^p (-1)^{i-1} (b/a)^i / i,
+//        F_a = (1/a) sum_{i<=p} u^i,        F_b = (1/a) sum_{i<p} u^i
+//   DLP: F = n G,  G = -(1/a) sum_{i<=p} u^i,
+//        G_a = (1/a^2) sum_{i<=p} (i+1) u^i,  G_b = (1/a^2) sum_{i<p} (i+1) u^i
+// The kernel value is Re F; for a real 2-vector x entering through a complex
+// variable w, d Re F / dx = (Re F_w, -Im F_w).
+struct QbxTerms {
+  double fre;            // Re F
+  double fa_re, fa_im;   // F_a
+  double fb_re, fb_im;   // F_b
+  double g_re, g_im;     // DLP: G (= dF/dn)
+};
+
+__device__ __forceinline__ QbxTerms qbx_terms(int order, bool dlp, double are, double aim,
+                                              double bre, double bim, double nre, double nim) {
+  const double aa = are*are + aim*aim;
+  const double ia_re = are/aa, ia_im = -aim/aa;  // 1/a
+  const double zre = bre*ia_re - bim*ia_im, zim = bre*ia_im + bim*ia_re;
+  const double ure = -zre, uim = -zim;
+  // Horner sums of sum_{i<=p} c_i u^i and sum_{i<p} c_i u^i (c_i = 1 or i + 1)
+  double s0re = 0, s0im = 0, s1re = 0, s1im = 0;  // i <= p and i < p
+  for (int i = order; i >= 0; --i) {
+    const double c = dlp ? double(i + 1) : 1.0;
+    double tre = s0re*ure - s0im*uim + c, tim = s0re*uim + s0im*ure;
+    s0re = tre; s0im = tim;
+    if (i < order) {
+      tre = s1re*ure - s1im*uim + c; tim = s1re*uim + s1im*ure;
+      s1re = tre; s1im = tim;
+    }
+  }
+  QbxTerms r;
+  if (!dlp) {
+    r.fa_re = ia_re*s0re - ia_im*s0im; r.fa_im = ia_re*s0im + ia_im*s0re;
+    r.fb_re = ia_re*s1re - ia_im*s1im; r.fb_im = ia_re*s1im + ia_im*s1re;
+    // Re F = log|a| + Re sum_{i=1}^p (-1)^{i-1} z^i / i
+    double hre = 0, him = 0;
+    for (int i = order; i >= 1; --i) {
+      const double c = ((i & 1) ? 1.0 : -1.0)/i;
+      const double tre = hre*zre - him*zim + c, tim = hre*zim + him*zre;
+      hre = tre; him = tim;
+    }
+    r.fre = 0.5*log(aa) + (hre*zre - him*zim);
+    r.g_re = r.g_im = 0;
+  } else {
+    // sum_{i<=p} u^i for G itself
+    double hre = 0, him = 0;
+    for (int i = order; i >= 0; --i) {
+      const double tre = hre*ure - him*uim + 1.0, tim = hre*uim + him*ure;
+      hre = tre; him = tim;
+    }
+    r.g_re = -(ia_re*hre - ia_im*him); r.g_im = -(ia_re*him + ia_im*hre);
+    r.fre = nre*r.g_re - nim*r.g_im;
+    const double a2re = ia_re*ia_re - ia_im*ia_im, a2im = 2*ia_re*ia_im;  // 1/a^2
+    const double gare = a2re*s0re - a2im*s0im, gaim = a2re*s0im + a2im*s0re;
+    const double gbre = a2re*s1re - a2im*s1im, gbim = a2re*s1im + a2im*s1re;
+    r.fa_re = nre*gare - nim*gaim; r.fa_im = nre*gaim + nim*gare;
+    r.fb_re = nre*gbre - nim*gbim; r.fb_im = nre*gbim + nim*gbre;
+  }
+  return r;
+}
+
+// per target: d/dt and d/dc of g_t out_t (b = t - c, a = c - s)
+__global__ __launch_bounds__(QB) void qbx2d_grad_targets(
+    int order, int dlp, int nt, const double *tx, const double *ty, const double *cx,
+    const double *cy, int ns, const double *sx, const double *sy, const double *nx,
+    const double *ny, const double *dens, const double *g, double scale, double *dtx,
+    double *dty, double *dcx, double *dcy) {
+  __shared__ double s_x[QB], s_y[QB], s_nx[QB], s_ny[QB], s_d[QB];
+  const int tid = threadIdx.x;
+  const int t = min((int)(blockIdx.x*QB + tid), nt - 1);
+  const bool valid = blockIdx.x*QB + tid < nt;
+  const double ccx = cx[t], ccy = cy[t];
+  const double bre = tx[t] - ccx, bim = ty[t] - ccy;
+  double ab_re = 0, ab_im = 0, aa_re = 0, aa_im = 0;  // sum sigma F_b, sum sigma F_a
+  for (int sc = 0; sc < ns; sc += QB) {
+    __syncthreads();
+    if (sc + tid < ns) {
+      const int s = sc + tid;
+      s_x[tid] = sx[s]; s_y[tid] = sy[s]; s_d[tid] = dens[s];
+      if (dlp) { s_nx[tid] = nx[s]; s_ny[tid] = ny[s]; }
+    }
+    __syncthreads();
+    const int nn = min(QB, ns - sc);
+    for (int j = 0; j < nn; ++j) {
+      const QbxTerms q = qbx_terms(order, dlp, ccx - s_x[j], ccy - s_y[j], bre, bim,
+                                   dlp ? s_nx[j] : 0.0, dlp ? s_ny[j] : 0.0);
+      ab_re += s_d[j]*q.fb_re; ab_im += s_d[j]*q.fb_im;
+      aa_re += s_d[j]*q.fa_re; aa_im += s_d[j]*q.fa_im;
+    }
+  }
+  if (valid) {
+    const double w = scale*g[t];
+    if (dtx) { dtx[t] = w*ab_re; dty[t] = -w*ab_im; }
+    if (dcx) { dcx[t] = w*(aa_re - ab_re); dcy[t] = -w*(aa_im - ab_im); }
+  }
+}
+
+// per source: d/dsigma, d/ds and (DLP) d/dn of sum_t g_t out_t
+__global__ __launch_bounds__(QB) void qbx2d_grad_sources(
+    int order, int dlp, int ns, const double *sx, const double *sy, const double *nx,
+    const double *ny, const double *dens, int nt, const double *tx, const double *ty,
+    const double *cx, const double *cy, const double *g, double scale, double *ddens,
+    double *dsx, double *dsy, double *dnx, double *dny) {
+  __shared__ double s_tx[QB], s_ty[QB], s_cx[QB], s_cy[QB], s_g[QB];
+  const int tid = threadIdx.x;
+  const int s = min((int)(blockIdx.x*QB + tid), ns - 1);
+  const bool valid = blockIdx.x*QB + tid < ns;
+  const double px = sx[s], py = sy[s];
+  const double nre = dlp ? nx[s] : 0.0, nim = dlp ? ny[s] : 0.0;
+  double val = 0, fa_re = 0, fa_im = 0, g_re = 0, g_im = 0;
+  for (int tc = 0; tc < nt; tc += QB) {
+    __syncthreads();
+    if (tc + tid < nt) {
+      const int t = tc + tid;
+      s_tx[tid] = tx[t]; s_ty[tid] = ty[t]; s_cx[tid] = cx[t]; s_cy[tid] = cy[t];
+      s_g[tid] = g[t];
+    }
+    __syncthreads();
+    const int nn = min(QB, nt - tc);
+    for (int j = 0; j < nn; ++j) {
+      const QbxTerms q = qbx_terms(order, dlp, s_cx[j] - px, s_cy[j] - py, s_tx[j] - s_cx[j],
+                                   s_ty[j] - s_cy[j], nre, nim);
+      const double w = s_g[j];
+      val += w*q.fre;
+      fa_re += w*q.fa_re; fa_im += w*q.fa_im;
+      g_re += w*q.g_re; g_im += w*q.g_im;
+    }
+  }
+  if (valid) {
+    const double sig = scale*dens[s];
+    if (ddens) ddens[s] = scale*val;
+    // d/ds = -d/da
+    if (dsx) { dsx[s] = -sig*fa_re; dsy[s] = sig*fa_im; }
+    if (dlp && dnx) { dnx[s] = sig*g_re; dny[s] = -sig*g_im; }
+  }
+}
+
+// }}}
+
 }  // namespace
 }  // namespace sumpyhip
 
@@ -234,4 +377,37 @@ extern "C" int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntarget
                                                      ty, cx, cy, nsources, sx, sy, nx, ny,
                                                      density, scale, out);
   return check_launch("qbx2d");
+}
+
+extern "C" int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *tx,
+                                      const double *ty, const double *cx, const double *cy,
+                                      int nsources, const double *sx, const double *sy,
+                                      const double *nx, const double *ny,
+                                      const double *density, double scale,
+                                      const double *grad_out, double *d_density, double *d_sx,
+                                      double *d_sy, double *d_nx, double *d_ny, double *d_tx,
+                                      double *d_ty, double *d_cx, double *d_cy, void *stream) {
+  SUMPY_REQUIRE(order >= 0 && order <= 64, "qbx2d_adjoint: order out of range");
+  SUMPY_REQUIRE(src_kind == 0 || src_kind == 1,
+                "qbx2d_adjoint: single (0) or double (1) layer only, got %d", src_kind);
+  if (src_kind == 1) SUMPY_REQUIRE(nx && ny, "qbx2d_adjoint: DLP needs source normals");
+  SUMPY_REQUIRE(!d_sx == !d_sy && !d_nx == !d_ny && !d_tx == !d_ty && !d_cx == !d_cy,
+                "qbx2d_adjoint: gradient outputs come in (x, y) pairs");
+  if (ntargets == 0 || nsources == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(grad_out && density, "qbx2d_adjoint: null input");
+  hipStream_t st = (hipStream_t)stream;
+  if (d_tx || d_cx) {
+    qbx2d_grad_targets<<<(ntargets + QB - 1)/QB, QB, 0, st>>>(
+        order, src_kind, ntargets, tx, ty, cx, cy, nsources, sx, sy, nx, ny, density,
+        grad_out, scale, d_tx, d_ty, d_cx, d_cy);
+    int rc = check_launch("qbx2d_grad_targets");
+    if (rc) return rc;
+  }
+  if (d_density || d_sx || d_nx) {
+    qbx2d_grad_sources<<<(nsources + QB - 1)/QB, QB, 0, st>>>(
+        order, src_kind, nsources, sx, sy, nx, ny, density, ntargets, tx, ty, cx, cy,
+        grad_out, scale, d_density, d_sx, d_sy, d_nx, d_ny);
+    return check_launch("qbx2d_grad_sources");
+  }
+  return SUMPYHIP_OK;
 }
