@@ -371,6 +371,199 @@ __global__ __launch_bounds__(256) void m2l_reduce_splits(ApplyArgs a) {
   }
 }
 
+// {{{ split-BF16 contraction (x = hi + lo in bf16; hi*hi + hi*lo + lo*hi, FP32 sums)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int KB = 32;        // k per v_mfma_f32_16x16x32_bf16
+constexpr int XB = 128 + 8;   // LDS row of one target column (bf16 entries, padded)
+
+__device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
+  hi = static_cast<__bf16>(v);
+  lo = static_cast<__bf16>(v - static_cast<float>(hi));
+}
+
+// Same block/item structure as m2l_apply_kernel; per class slot the class
+// data (FP32 in LDS) and the tile's 64 source columns (hi and lo bf16 rows,
+// k contiguous) are staged, and every 32-deep k step of a wave gathers 8
+// class-data values per A row, splits them, and issues 3 bf16 MFMAs per
+// (row block, column block).  Operand layout of v_mfma_f32_16x16x32_bf16:
+// A row = lane % 16, B column = lane % 16, the 8 k of a lane are
+// 8 (lane / 16) + e in both; accumulator register r = row 4 (lane / 16) + r.
+template <int RBW>
+__global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
+  constexpr int DPT = Limits<RBW>::DPT;
+  constexpr int XPT = (128/4*TW + 255)/256;   // float4 loads per thread (kpad <= 128)
+  extern __shared__ unsigned char lds_raw[];
+  float *sD = reinterpret_cast<float *>(lds_raw);                        // [ncirc]
+  __bf16 *sXh = reinterpret_cast<__bf16 *>(sD + ((a.ncirc + 3) & ~3));  // [64][XB]
+  __bf16 *sXl = sXh + TW*XB;                                             // [64][XB]
+  int *sOff = reinterpret_cast<int *>(sXl + TW*XB);                      // [kpad]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int kq = a.kpad/4;  // float4 per source row
+  const int xcd = blockIdx.x & 7;
+  int r = blockIdx.x >> 3, lev = 0, li = -1;
+  for (; lev < a.nlev; ++lev) {
+    const int n = a.lv[lev].item_end - (lev ? a.lv[lev - 1].item_end : 0);
+    const int per = (n + 7)/8, lo = min(xcd*per, n), cnt = min(lo + per, n) - lo;
+    if (r < cnt) { li = lo + r; break; }
+    r -= cnt;
+  }
+  if (li < 0) return;
+  const LevelArgs L = a.lv[lev];
+  const float *packed = reinterpret_cast<const float *>(L.packed);
+  float *work = reinterpret_cast<float *>(L.work);
+  for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : 0;
+  int offT[RBW];
+  for (int q = 0; q < RBW; ++q) {
+    const int row = (wave + 4*q)*16 + (lane & 15);
+    offT[q] = (row < a.nl) ? a.off_t[row] : 0;
+  }
+  double dreg[DPT];
+  float4 xreg[XPT];
+  auto fetch = [&](int slot) {
+    const double *D = L.class_data + (size_t)L.slot_class[slot]*a.ncirc;
+    for (int i = 0; i < DPT; ++i) {
+      const int e = tid + 256*i;
+      dreg[i] = D[e < a.ncirc ? e : 0];
+    }
+    const int sb = L.slot_sources[(size_t)slot*TW + lane];
+    const float *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*a.kpad;
+    for (int i = 0; i < XPT; ++i) {
+      const int q = wave + 4*i;
+      xreg[i] = *reinterpret_cast<const float4 *>(row + (q < kq ? q : 0)*4);
+    }
+    if (sb < 0)
+      for (int i = 0; i < XPT; ++i) xreg[i] = float4{0, 0, 0, 0};
+  };
+  auto stage = [&]() {
+    for (int i = 0; i < DPT; ++i) {
+      const int e = tid + 256*i;
+      if (e < a.ncirc) sD[e] = float(dreg[i]);
+    }
+    for (int i = 0; i < XPT; ++i) {
+      const int q = wave + 4*i;
+      if (q < kq) {
+        const float v[4] = {xreg[i].x, xreg[i].y, xreg[i].z, xreg[i].w};
+        for (int j = 0; j < 4; ++j) {
+          __bf16 h, l;
+          split_bf16(v[j], h, l);
+          sXh[lane*XB + q*4 + j] = h;
+          sXl[lane*XB + q*4 + j] = l;
+        }
+      }
+    }
+  };
+  {
+    const int tile = li/L.nsplit, split = li - tile*L.nsplit;
+    typedef float v4f_t __attribute__((ext_vector_type(4)));
+    v4f_t acc[RBW][4];
+    for (int q = 0; q < RBW; ++q)
+      for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
+    const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
+    const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
+    const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
+    if (s0 < s1) {
+      fetch(s0);
+      __syncthreads();
+      stage();
+      __syncthreads();
+    }
+    const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
+    for (int slot = s0; slot < s1; ++slot) {
+      if (slot + 1 < s1) fetch(slot + 1);
+      for (int k0 = 0; k0 < a.kpad; k0 += KB) {
+        // B fragments: 8 consecutive k of one column (16-byte LDS reads)
+        bf16x8 bh[4], bl[4];
+        #pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          const int col = cb*16 + (lane & 15);
+          bh[cb] = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
+          bl[cb] = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
+        }
+        int os[8];
+        #pragma unroll
+        for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
+        #pragma unroll
+        for (int q = 0; q < RBW; ++q) {
+          bf16x8 ah, al;
+          #pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            __bf16 h, l;
+            split_bf16(sD[offT[q] + os[e]], h, l);
+            ah[e] = h;
+            al[e] = l;
+          }
+          #pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[q][cb], 0, 0, 0);
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[q][cb], 0, 0, 0);
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[q][cb], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
+      if (slot + 1 < s1) {
+        stage();
+        __syncthreads();
+      }
+    }
+    const int nrb = a.npad >> 4;
+    for (int q = 0; q < RBW; ++q) {
+      const int rb = wave + 4*q;
+      if (rb >= nrb) break;
+      for (int cb = 0; cb < 4; ++cb) {
+        const int col = cb*16 + (lane & 15);
+        const int tb = L.tile_targets[(size_t)tile*TW + col];
+        if (tb < 0) continue;
+        float *dst = split == 0
+            ? work + (size_t)(tb - L.tbase)*a.npad
+            : reinterpret_cast<float *>(L.partial) +
+                  ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
+        for (int i = 0; i < 4; ++i) dst[rb*16 + 4*(lane >> 4) + i] = acc[q][cb][i];
+      }
+    }
+  }
+}
+
+template <int RBW>
+int launch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
+  SUMPY_REQUIRE(a.kpad % KB == 0 && a.kpad <= 128,
+                "m2l_apply (bf16x3): kpad %d must be a multiple of 32 and <= 128", a.kpad);
+  const size_t lds = sizeof(float)*((size_t)((a.ncirc + 3) & ~3)) + 2*sizeof(__bf16)*TW*XB +
+                     sizeof(int)*a.kpad;
+  SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
+  int rc = set_lds_limit((const void *)m2l_apply_bf16_kernel<RBW>, lds);
+  if (rc) return rc;
+  long long per_xcd = 0;
+  for (int l = 0; l < a.nlev; ++l)
+    per_xcd += (a.lv[l].item_end - (l ? a.lv[l - 1].item_end : 0) + 7)/8;
+  m2l_apply_bf16_kernel<RBW><<<(int)(8*per_xcd), 256, lds, st>>>(a);
+  return check_launch("m2l_apply_bf16");
+}
+
+int dispatch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
+  const int nrb = a.npad/16;
+  int rc;
+  if (nrb <= 4 && a.ncirc <= Limits<1>::MAXCIRC) rc = launch_apply_bf16<1>(a, st);
+  else if (nrb <= 8 && a.ncirc <= Limits<2>::MAXCIRC) rc = launch_apply_bf16<2>(a, st);
+  else
+    return set_error(SUMPYHIP_ENOTIMPL,
+                     "m2l_apply (bf16x3): npad %d / ncirc %d exceed the supported sizes",
+                     a.npad, a.ncirc);
+  if (rc) return rc;
+  size_t nmax = 0;
+  for (int l = 0; l < a.nlev; ++l)
+    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*TW*a.npad);
+  if (nmax == 0) return SUMPYHIP_OK;
+  const int rgrid = (int)((nmax + 255)/256 < 8192 ? (nmax + 255)/256 : 8192);
+  m2l_reduce_splits<float><<<dim3(rgrid, a.nlev), 256, 0, st>>>(a);
+  return check_launch("m2l_reduce_splits");
+}
+
+// }}}
+
 template <typename T, int RBW>
 int launch_apply(const ApplyArgs &a, hipStream_t st) {
   const size_t lds = sizeof(T)*((size_t)((a.ncirc + 3) & ~3) + (size_t)a.kpad*Apply<T, RBW>::XS) +
@@ -489,6 +682,7 @@ extern "C" int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
   if (nboxes == 0) return SUMPYHIP_OK;
   const size_t n = (size_t)nboxes*kpad;
   const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+  SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_preprocess: unknown precision");
   if (precision == 0)
     m2l_preprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
         nboxes, nstored, src_expansions, kpad, (double *)packed);
@@ -506,6 +700,7 @@ extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int n
   if (nboxes == 0) return SUMPYHIP_OK;
   const size_t n = (size_t)nboxes*nl;
   const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+  SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_postprocess: unknown precision");
   if (precision == 0)
     m2l_postprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
         nboxes, nl, npad, (const double *)work, tgt_degree, ratio, tgt_expansions);
@@ -519,7 +714,7 @@ extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
                                   const sumpyhip_m2l_level *levels, int nl, int nm, int npad,
                                   int kpad, const int *off_t, const int *off_s, int ncirc,
                                   void *stream) {
-  SUMPY_REQUIRE(precision == 0 || precision == 1, "m2l_apply: unknown precision %d", precision);
+  SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_apply: unknown precision %d", precision);
   SUMPY_REQUIRE(npad % 16 == 0 && npad >= nl, "m2l_apply: npad %d must be a multiple of 16 >= nl",
                 npad);
   SUMPY_REQUIRE(kpad % 4 == 0 && kpad >= nm, "m2l_apply: kpad %d must be a multiple of 4 >= nm",
@@ -548,6 +743,7 @@ extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
   }
   if (a.nlev == 0) return SUMPYHIP_OK;
   hipStream_t st = (hipStream_t)stream;
+  if (precision == 2) return dispatch_apply_bf16(a, st);
   return precision == 0 ? dispatch_apply<double>(a, st) : dispatch_apply<float>(a, st);
 }
 

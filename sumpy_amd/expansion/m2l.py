@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-PRECISIONS = {"fp64": 0, "fp32": 1}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
 def _pad(n: int, m: int) -> int:
@@ -39,7 +39,9 @@ class VolumeTaylorM2LWithDenseClassMatrices(M2LTranslationBase):
       the circulant layout of ``m2l.py:403-537`` (FP64);
     * preprocessed multipoles: the stored coefficients packed into ``kpad``
       entries of the compute precision (``precision`` = ``"fp64"`` for the
-      plain form or ``"fp32"`` for FP32 MFMA);
+      plain form, ``"fp32"`` for FP32 MFMA, ``"bf16x3"`` for split-BF16 MFMA:
+      operands split as x = hi + lo in bf16, products hi*hi + hi*lo + lo*hi
+      accumulated in FP32; FP32 rows padded to 32);
     * the translation writes ``npad`` entries of the compute precision per
       target box, which the postprocessing scales into FP64 locals.
     """
@@ -67,7 +69,7 @@ class VolumeTaylorM2LWithDenseClassMatrices(M2LTranslationBase):
         return int(np.prod(m2l_circulant_shape(src_expansion.table, tgt_expansion.table)))
 
     def preprocess_multipole_nexprs(self, tgt_expansion, src_expansion):
-        return _pad(len(src_expansion), 4)
+        return _pad(len(src_expansion), 32 if self.precision == "bf16x3" else 4)
 
     def postprocess_local_nexprs(self, tgt_expansion, src_expansion):
         return _pad(len(tgt_expansion), 16)

@@ -99,12 +99,15 @@ def test_fmm_error_vs_direct_converges(actx, uniform_case):
     assert errs[-1] < 3e-5
 
 
-def test_fmm_mixed_precision_accuracy(actx, uniform_case):
-    """FP32 M2L + FP32 near field: error within 2x of the FP64 form's (order 8)."""
+@pytest.mark.parametrize("m2l_precision", ["fp32", "bf16x3"])
+def test_fmm_mixed_precision_accuracy(actx, uniform_case, m2l_precision):
+    """FP32 / split-BF16 M2L + FP32 near field: error within 2x of the FP64
+    form's (order 8)."""
     src, q = uniform_case
     direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(src.shape[1], dtype=np.int32))
     (p64,), _, _ = run_fmm(actx, src, q, 8)
-    (p32,), _, _ = run_fmm(actx, src, q, 8, m2l_precision="fp32", p2p_precision="fp32")
+    (p32,), _, _ = run_fmm(actx, src, q, 8, m2l_precision=m2l_precision,
+                           p2p_precision="fp32")
     e64, e32 = rel(p64, direct), rel(p32, direct)
     assert e32 <= 2*e64 + 1e-7, (e64, e32)
 
