@@ -31,6 +31,8 @@ sys.path.insert(0, ROOT)
 
 PEAKS = {  # MI355X published dense peaks, TFLOP/s
     "mfma_fp64": 78.6, "mfma_fp32": 157.3, "valu_fp64": 78.6, "valu_fp32": 157.3,
+    # split BF16: three bf16 products (2.5 PFLOP/s dense) per algorithmic product
+    "mfma_bf16x3": 2500.0/3,
 }
 
 
@@ -256,7 +258,8 @@ def main():
 
     forms = {
         "fp64": make_wrangler(actx, tree, trav, args.order, "fp64", "fp64"),
-        "mixed": make_wrangler(actx, tree, trav, args.order, "fp32", "fp32"),
+        "mixed_fp32": make_wrangler(actx, tree, trav, args.order, "fp32", "fp32"),
+        "mixed_bf16x3": make_wrangler(actx, tree, trav, args.order, "bf16x3", "fp32"),
     }
     if args.traffic_probe:
         for name, wr in forms.items():
@@ -321,14 +324,21 @@ def main():
         p2p_rate[name] = npairs/(ms*1e-3)/1e9
         log(f"{name}: P2P {npairs:.3e} pairs in {ms:.3f} ms = {p2p_rate[name]:.1f} Gpairs/s")
 
-    # roofline of the dominant kernel of the mixed form
-    sp = spans["mixed"]
+    # the reported mixed form: the fastest whose error is within 2x of FP64's
+    ok = [f for f in forms if f != "fp64"
+          and results[f]["rel_l2_err"] <= 2*results["fp64"]["rel_l2_err"]]
+    best = min(ok or [f for f in forms if f != "fp64"], key=lambda f: results[f]["ms"])
+    log(f"reported mixed form: {best}")
+
+    # roofline of the dominant kernel of the reported form
+    sp = spans[best]
     dom = max(sp, key=lambda k: sp[k]["ms"])
     npair_m2l = m2l_pairs(trav)
     nl = (args.order + 1)**2
     if dom.startswith("m2l_apply"):
         flops = 2.0*nl*nl*npair_m2l*args.steps
-        peak = PEAKS["mfma_fp32" if dom.endswith("fp32") else "mfma_fp64"]
+        peak = PEAKS[{"fp32": "mfma_fp32", "fp64": "mfma_fp64",
+                      "bf16x3": "mfma_bf16x3"}[dom.rsplit("_", 1)[1]]]
         bound = "mfma"
     else:
         flops = 11.0*npairs*args.steps
@@ -337,7 +347,7 @@ def main():
     achieved = flops/(sp[dom]["ms"]*1e-3)/1e12
     roofline = {"bound": bound, "kernel": dom, "achieved": round(achieved, 3),
                 "peak": peak, "unit": "TFLOP/s", "frac": round(achieved/peak, 4),
-                "traffic": measured_traffic("fp32" if dom.endswith("fp32") else "fp64", args),
+                "traffic": measured_traffic(dom.rsplit("_", 1)[1], args),
                 "launches_per_step": sp[dom]["launches"]//args.steps,
                 "avg_launch_ms": sp[dom]["ms"]/sp[dom]["launches"]}
 
@@ -352,7 +362,7 @@ def main():
         log(f"cpu baseline: {cb['value']/1e3:.1f} s extrapolated "
             f"({time.perf_counter() - t0:.1f}s measured)")
 
-    mixed = results["mixed"]
+    mixed = results[best]
     line = {
         "metric": "Laplace-3D FMM wall time N=2^20 p=10 + P2P Ginteractions/s at matched rel error",
         "value": round(mixed["ms"], 4),
@@ -364,14 +374,20 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64 mixed (FP32 MFMA M2L, FP32 P2P with FP64 accumulation)",
+        "dtype": {"mixed_fp32": "f32/f64 mixed (FP32 MFMA M2L, FP32 P2P with FP64 "
+                                "accumulation)",
+                  "mixed_bf16x3": "bf16x3/f32/f64 mixed (split-BF16 MFMA M2L with FP32 sums, "
+                                  "FP32 P2P with FP64 accumulation)"}[best],
         "data": "synthetic",
         "config": {"workload": f"laplace3d_fmm_{args.distribution}", "n": args.n,
                    "order": args.order, "max_particles_in_box": args.max_particles,
                    "nboxes": tree.nboxes, "nlevels": tree.nlevels,
                    "p2p_pairs": npairs, "m2l_pairs": npair_m2l},
+        "form": best,
         "fp64_ms_per_step": round(results["fp64"]["ms"], 4),
+        "ms_per_step_by_form": {k: round(v["ms"], 4) for k, v in results.items()},
         "speedup_mixed_vs_fp64": round(results["fp64"]["ms"]/mixed["ms"], 3),
+        "error_ratio_mixed_vs_fp64": round(mixed["rel_l2_err"]/results["fp64"]["rel_l2_err"], 3),
         "rel_l2_err": {k: results[k]["rel_l2_err"] for k in forms},
         "p2p_ginteractions_per_s": {k: round(v, 2) for k, v in p2p_rate.items()},
         "kernel_ms_per_step": {f: {k: round(v["ms"]/args.steps, 4) for k, v in spans[f].items()}
