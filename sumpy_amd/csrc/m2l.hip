@@ -395,7 +395,8 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   constexpr int DPT = Limits<RBW>::DPT;
   constexpr int XPT = (128/4*TW + 255)/256;   // float4 loads per thread (kpad <= 128)
   extern __shared__ unsigned char lds_raw[];
-  float *sD = reinterpret_cast<float *>(lds_raw);                        // [ncirc]
+  // class data split once per slot: word = hi | lo << 16 (bf16 bit patterns)
+  unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
   __bf16 *sXh = reinterpret_cast<__bf16 *>(sD + ((a.ncirc + 3) & ~3));  // [64][XB]
   __bf16 *sXl = sXh + TW*XB;                                             // [64][XB]
   int *sOff = reinterpret_cast<int *>(sXl + TW*XB);                      // [kpad]
@@ -440,7 +441,12 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   auto stage = [&]() {
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      if (e < a.ncirc) sD[e] = float(dreg[i]);
+      if (e < a.ncirc) {
+        __bf16 h, l;
+        split_bf16(float(dreg[i]), h, l);
+        sD[e] = (unsigned)__builtin_bit_cast(unsigned short, h) |
+                ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+      }
     }
     for (int i = 0; i < XPT; ++i) {
       const int q = wave + 4*i;
@@ -487,14 +493,19 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
         for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
         #pragma unroll
         for (int q = 0; q < RBW; ++q) {
-          bf16x8 ah, al;
+          // gather 8 packed words, then regroup the low (hi) and high (lo)
+          // halves of consecutive words into the two bf16x8 operands
+          unsigned w[8];
           #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            __bf16 h, l;
-            split_bf16(sD[offT[q] + os[e]], h, l);
-            ah[e] = h;
-            al[e] = l;
+          for (int e = 0; e < 8; ++e) w[e] = sD[offT[q] + os[e]];
+          typedef unsigned u4 __attribute__((ext_vector_type(4)));
+          u4 hw, lw;
+          #pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            hw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x05040100u);
+            lw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x07060302u);
           }
+          const bf16x8 ah = __builtin_bit_cast(bf16x8, hw), al = __builtin_bit_cast(bf16x8, lw);
           #pragma unroll
           for (int cb = 0; cb < 4; ++cb) {
             acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[q][cb], 0, 0, 0);
@@ -531,8 +542,8 @@ template <int RBW>
 int launch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
   SUMPY_REQUIRE(a.kpad % KB == 0 && a.kpad <= 128,
                 "m2l_apply (bf16x3): kpad %d must be a multiple of 32 and <= 128", a.kpad);
-  const size_t lds = sizeof(float)*((size_t)((a.ncirc + 3) & ~3)) + 2*sizeof(__bf16)*TW*XB +
-                     sizeof(int)*a.kpad;
+  const size_t lds = sizeof(unsigned)*((size_t)((a.ncirc + 3) & ~3)) +
+                     2*sizeof(__bf16)*TW*XB + sizeof(int)*a.kpad;
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
   int rc = set_lds_limit((const void *)m2l_apply_bf16_kernel<RBW>, lds);
   if (rc) return rc;
