@@ -393,7 +393,9 @@ __device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
 template <int RBW>
 __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   constexpr int DPT = Limits<RBW>::DPT;
-  constexpr int XPT = (128/4*TW + 255)/256;   // float4 loads per thread (kpad <= 128)
+  // a thread loads 8 consecutive k (two float4) of its column for chunks
+  // m = wave + 4 i, i < 4 (kpad <= 128)
+  constexpr int XPT = 8;
   extern __shared__ unsigned char lds_raw[];
   // class data split once per slot: word = hi | lo << 16 (bf16 bit patterns)
   unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   int *sOff = reinterpret_cast<int *>(sXl + TW*XB);                      // [kpad]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int kq = a.kpad/4;  // float4 per source row
+  const int kq = a.kpad/8;  // 8-wide k chunks per source row
   const int xcd = blockIdx.x & 7;
   int r = blockIdx.x >> 3, lev = 0, li = -1;
   for (; lev < a.nlev; ++lev) {
@@ -431,9 +433,11 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     }
     const int sb = L.slot_sources[(size_t)slot*TW + lane];
     const float *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*a.kpad;
-    for (int i = 0; i < XPT; ++i) {
-      const int q = wave + 4*i;
-      xreg[i] = *reinterpret_cast<const float4 *>(row + (q < kq ? q : 0)*4);
+    for (int i = 0; i < XPT/2; ++i) {
+      const int m = wave + 4*i;
+      const float4 *src = reinterpret_cast<const float4 *>(row + (m < kq ? m : 0)*8);
+      xreg[2*i] = src[0];
+      xreg[2*i + 1] = src[1];
     }
     if (sb < 0)
       for (int i = 0; i < XPT; ++i) xreg[i] = float4{0, 0, 0, 0};
@@ -448,16 +452,23 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
                 ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
       }
     }
-    for (int i = 0; i < XPT; ++i) {
-      const int q = wave + 4*i;
-      if (q < kq) {
-        const float v[4] = {xreg[i].x, xreg[i].y, xreg[i].z, xreg[i].w};
-        for (int j = 0; j < 4; ++j) {
-          __bf16 h, l;
-          split_bf16(v[j], h, l);
-          sXh[lane*XB + q*4 + j] = h;
-          sXl[lane*XB + q*4 + j] = l;
+    for (int i = 0; i < XPT/2; ++i) {
+      const int m = wave + 4*i;
+      if (m < kq) {
+        const float v[8] = {xreg[2*i].x, xreg[2*i].y, xreg[2*i].z, xreg[2*i].w,
+                            xreg[2*i + 1].x, xreg[2*i + 1].y, xreg[2*i + 1].z,
+                            xreg[2*i + 1].w};
+        bf16x8 h, l;
+        for (int j = 0; j < 8; ++j) {
+          __bf16 hh, ll;
+          split_bf16(v[j], hh, ll);
+          h[j] = hh;
+          l[j] = ll;
         }
+        // one 16-byte store each (the column stride, 17 x 16 bytes, spreads
+        // the 64 lanes over the bank groups)
+        *reinterpret_cast<bf16x8 *>(sXh + lane*XB + m*8) = h;
+        *reinterpret_cast<bf16x8 *>(sXl + lane*XB + m*8) = l;
       }
     }
   };
@@ -479,7 +490,9 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
     for (int slot = s0; slot < s1; ++slot) {
       if (slot + 1 < s1) fetch(slot + 1);
-      for (int k0 = 0; k0 < a.kpad; k0 += KB) {
+      #pragma unroll
+      for (int k0 = 0; k0 < 128; k0 += KB) {
+        if (k0 >= a.kpad) break;
         // B fragments: 8 consecutive k of one column (16-byte LDS reads)
         bf16x8 bh[4], bl[4];
         #pragma unroll
