@@ -244,7 +244,7 @@ def main():
 
     from sumpy_amd import P2P, profiling
     from sumpy_amd.array_context import TorchArrayContext
-    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.drive import capture_drive_fmm, drive_fmm
     from sumpy_amd.kernel import LaplaceKernel
     from sumpy_amd.tree import build_tree_and_traversal
 
@@ -286,8 +286,17 @@ def main():
         setup[name] = time.perf_counter() - t0
         ms, wall, pots = time_passes(actx, lambda wr=wr: drive_fmm(actx, wr, [q_dev]),
                                      args.steps, args.warmup)
-        results[name] = {"ms": ms, "wall_ms": wall, "pot": pots[0]}
-        log(f"{name}: {ms:.3f} ms/pass (host wall {wall:.3f} ms), setup {setup[name]:.2f}s")
+        results[name] = {"eager_ms": ms, "wall_ms": wall, "pot": pots[0]}
+        # the same pass captured in a HIP graph; used when every replay is
+        # bitwise equal to the eager pass
+        graph = capture_drive_fmm(actx, wr, [q_dev])
+        gms, _, gout = time_passes(actx, graph.replay, args.steps, args.warmup)
+        same = bool(torch.equal(gout[0], pots[0]))
+        results[name].update(graph_ms=gms, graph_ok=same,
+                             ms=gms if same else ms)
+        log(f"{name}: {ms:.3f} ms/pass eager (host wall {wall:.3f} ms), {gms:.3f} ms/pass "
+            f"as a HIP graph (replay {'==' if same else '!='} eager), setup "
+            f"{setup[name]:.2f}s")
 
     # per-kernel spans over one more set of passes
     spans = {}
@@ -386,6 +395,8 @@ def main():
         "form": best,
         "fp64_ms_per_step": round(results["fp64"]["ms"], 4),
         "ms_per_step_by_form": {k: round(v["ms"], 4) for k, v in results.items()},
+        "eager_ms_per_step_by_form": {k: round(v["eager_ms"], 4) for k, v in results.items()},
+        "hip_graph": all(v["graph_ok"] for v in results.values()),
         "speedup_mixed_vs_fp64": round(results["fp64"]["ms"]/mixed["ms"], 3),
         "error_ratio_mixed_vs_fp64": round(mixed["rel_l2_err"]/results["fp64"]["rel_l2_err"], 3),
         "rel_l2_err": {k: results[k]["rel_l2_err"] for k in forms},

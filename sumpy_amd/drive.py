@@ -123,3 +123,42 @@ def _add(a, b):
     for i in range(len(a)):
         out[i] = a[i] + b[i]
     return out
+
+
+class FMMGraph:
+    """One FMM pass captured in a HIP graph (``torch.cuda.CUDAGraph``).
+
+    The level loops of the pass are host loops that launch one kernel per
+    level and stage (``sumpy/fmm.py:619-1056``); replaying the captured graph
+    removes their launch overhead.  :meth:`replay` re-runs the pass on the
+    current contents of the weight tensors given at capture (update them in
+    place) and returns the potential tensors, which every replay overwrites.
+    The pass is run twice eagerly before capture, so that every per-level plan
+    and tile schedule is built (and cached) outside the captured region; all
+    device memory the pass allocates comes from the graph's private pool.
+    """
+
+    def __init__(self, actx, wrangler, src_weight_vecs):
+        import torch
+        self.actx, self.wrangler = actx, wrangler
+        self.weights = list(src_weight_vecs)
+        side = torch.cuda.Stream(actx.device)
+        side.wait_stream(torch.cuda.current_stream(actx.device))
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                drive_fmm(actx, wrangler, self.weights)
+        torch.cuda.current_stream(actx.device).wait_stream(side)
+        torch.cuda.synchronize(actx.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.outputs = drive_fmm(actx, wrangler, self.weights)
+
+    def replay(self):
+        self.graph.replay()
+        return self.outputs
+
+
+def capture_drive_fmm(actx, wrangler, src_weight_vecs):
+    """Capture :func:`drive_fmm` of *wrangler* on *src_weight_vecs* into a
+    :class:`FMMGraph`."""
+    return FMMGraph(actx, wrangler, src_weight_vecs)

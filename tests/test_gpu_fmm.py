@@ -20,7 +20,8 @@ def rel(a, b):
 def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
             m2l_precision="fp64", p2p_precision="fp64", targets=None,
             use_classes=True, target_kernels=None, source_kernels=None,
-            source_extra_kwargs=None, strength_usage=None, tree_trav=None):
+            source_extra_kwargs=None, strength_usage=None, tree_trav=None,
+            return_wrangler=False):
     from sumpy_amd.drive import drive_fmm
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
     from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
@@ -54,6 +55,8 @@ def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
                                 _disable_translation_classes=not use_classes)
     ws = weights if isinstance(weights, (list, tuple)) else [weights]
     pots = drive_fmm(actx, wr, [actx.from_numpy(w) for w in ws])
+    if return_wrangler:
+        return [actx.to_numpy(p) for p in pots], wr, trav
     return [actx.to_numpy(p) for p in pots], tree, trav
 
 
@@ -164,3 +167,29 @@ def test_fmm_separate_targets_matches_oracle_fmm(actx):
     assert rel(pot, ref) < 1e-12
     direct, _ = oracle.p2p_dense(3, tgt, src, q)
     assert rel(pot, direct) < 1e-3
+
+
+@pytest.mark.parametrize("m2l_precision,p2p_precision", [("fp64", "fp64"), ("bf16x3", "fp32")])
+def test_fmm_graph_replays_match_eager(actx, uniform_case, m2l_precision, p2p_precision):
+    """A captured FMM pass, replayed several times with eager work and
+    synchronisations in between (and new weights written in place), matches
+    the eager pass every time."""
+    import torch
+
+    from sumpy_amd.drive import capture_drive_fmm, drive_fmm
+    src, q = uniform_case
+    (_,), wr, _ = run_fmm(actx, src, q, 6, m2l_precision=m2l_precision,
+                          p2p_precision=p2p_precision, return_wrangler=True)
+    w = actx.from_numpy(q.copy())
+    g = capture_drive_fmm(actx, wr, [w])
+    rng = np.random.default_rng(9)
+    for it in range(4):
+        if it == 2:
+            w.copy_(actx.from_numpy(rng.standard_normal(q.shape)))
+        out, = g.replay()
+        got = actx.to_numpy(out).copy()
+        torch.cuda.synchronize()
+        ref, = drive_fmm(actx, wr, [w])
+        ref = actx.to_numpy(ref)
+        torch.cuda.synchronize()
+        assert np.array_equal(got, ref), (it, np.abs(got - ref).max())
