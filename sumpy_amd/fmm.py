@@ -534,6 +534,9 @@ class SumpyExpansionWrangler:
         kwargs = dict(self.extra_kwargs)
         kwargs.update(self.box_source_list_kwargs())
         host_starts = self._host_copy(starts, "from_sep_bigger_starts")
+        # host-side knowledge that the result is all zeros (no list-4 work);
+        # drive_fmm then skips adding it
+        self.last_form_locals_was_empty = host_starts is not None
         for lev in range(self.tree.nlevels):
             start, stop = lstarts[lev:lev + 2]
             if start == stop:
@@ -542,6 +545,7 @@ class SumpyExpansionWrangler:
                 # no list-4 sources at this level: the zero-initialised
                 # expansions are what P2L would write
                 continue
+            self.last_form_locals_was_empty = False
             p2l = self.tree_indep.p2l(self.level_orders[lev])
             base, view = self.local_expansions_view(local_exps, lev)
             res = p2l(actx, target_boxes=target_or_target_parent_boxes[start:stop],
