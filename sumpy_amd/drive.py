@@ -65,23 +65,10 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
         mpole_exps)
     wrangler.communicate_mpoles(actx, mpole_exps)
 
-    # The near field (list 1) depends on nothing but the weights: it runs on
-    # a second stream, overlapping the multipole stages, and is joined before
-    # the first sum of potentials.
     timer.mark("eval_direct")
-    side = _side_stream(actx)
-    if side is not None:
-        import torch
-        main = torch.cuda.current_stream(actx.device)
-        side.wait_stream(main)
-        with torch.cuda.stream(side):
-            potentials = wrangler.eval_direct(
-                actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
-                trav.neighbor_source_boxes_lists, src_weight_vecs)
-    else:
-        potentials = wrangler.eval_direct(
-            actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
-            trav.neighbor_source_boxes_lists, src_weight_vecs)
+    potentials = wrangler.eval_direct(
+        actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
+        trav.neighbor_source_boxes_lists, src_weight_vecs)
 
     timer.mark("multipole_to_local")
     local_exps = wrangler.multipole_to_local(
@@ -93,10 +80,6 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     mpole_result = wrangler.eval_multipoles(
         actx, trav.target_boxes_sep_smaller_by_source_level,
         trav.from_sep_smaller_by_level, mpole_exps)
-    if side is not None:
-        main.wait_stream(side)
-        for p in potentials:
-            p.record_stream(main)
     potentials = _add(potentials, mpole_result)
 
     if getattr(trav, "from_sep_close_smaller_starts", None) is not None:
@@ -135,22 +118,6 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     if timing_data is not None:
         timing_data.update(timer.result())
     return result
-
-
-_SIDE = {}
-
-
-def _side_stream(actx):
-    """A second (default-priority) stream on the context's device, or None
-    off the GPU."""
-    import torch
-    dev = getattr(actx, "device", None)
-    if dev is None or torch.device(dev).type != "cuda":
-        return None
-    key = str(torch.device(dev))
-    if key not in _SIDE:
-        _SIDE[key] = torch.cuda.Stream(torch.device(dev))
-    return _SIDE[key]
 
 
 def _add(a, b):
