@@ -504,7 +504,6 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
         int os[8];
         #pragma unroll
         for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
-        bf16x8 ah[RBW], al[RBW];
         #pragma unroll
         for (int q = 0; q < RBW; ++q) {
           // gather 8 packed words, then regroup the low (hi) and high (lo)
@@ -519,26 +518,14 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
             hw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x05040100u);
             lw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x07060302u);
           }
-          ah[q] = __builtin_bit_cast(bf16x8, hw);
-          al[q] = __builtin_bit_cast(bf16x8, lw);
+          const bf16x8 ah = __builtin_bit_cast(bf16x8, hw), al = __builtin_bit_cast(bf16x8, lw);
+          #pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[q][cb], 0, 0, 0);
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[q][cb], 0, 0, 0);
+            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[q][cb], 0, 0, 0);
+          }
         }
-        // small products first; RBW x 4 independent accumulators between two
-        // updates of the same one
-        #pragma unroll
-        for (int q = 0; q < RBW; ++q)
-          #pragma unroll
-          for (int cb = 0; cb < 4; ++cb)
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[cb], acc[q][cb], 0, 0, 0);
-        #pragma unroll
-        for (int q = 0; q < RBW; ++q)
-          #pragma unroll
-          for (int cb = 0; cb < 4; ++cb)
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[cb], acc[q][cb], 0, 0, 0);
-        #pragma unroll
-        for (int q = 0; q < RBW; ++q)
-          #pragma unroll
-          for (int cb = 0; cb < 4; ++cb)
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[cb], acc[q][cb], 0, 0, 0);
       }
       __syncthreads();
       if (slot + 1 < s1) {
