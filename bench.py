@@ -219,6 +219,79 @@ def cpu_baseline(tree, trav, q, order, budget_s=20.0):
             "measured_s": measured}
 
 
+def p2p_dense_bench(actx, n, steps):
+    """Dense P2P, n sources x n targets uniform in [0,1]^3, N(0,1) strengths:
+    Laplace 3D and Helmholtz 3D (k = 5), plain FP64 and FP32 pair math with
+    FP64 accumulation; pair rate and the FP32 form's error against FP64."""
+    import torch
+
+    from sumpy_amd import P2P
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    rng = np.random.default_rng(0)
+    src = actx.from_numpy(rng.random((3, n)))
+    tgt = actx.from_numpy(rng.random((3, n)))
+    q = actx.from_numpy(rng.standard_normal(n))
+    out = {}
+    for name, knl, kw in (("laplace3d", LaplaceKernel(3), {}),
+                          ("helmholtz3d_k5", HelmholtzKernel(3), {"k": 5.0})):
+        res = {}
+        for prec in ("fp64", "fp32"):
+            op = P2P([knl], exclude_self=False, precision=prec)
+            ms, _, pot = time_passes(actx, lambda op=op: op(actx, tgt, src, [q], **kw),
+                                     steps, 1)
+            res[prec] = (ms, pot[0])
+        p64, p32 = res["fp64"][1], res["fp32"][1]
+        err = float(torch.linalg.vector_norm(p32 - p64)/torch.linalg.vector_norm(p64))
+        out[name] = {"gpairs_per_s": {k: round(n*n/(v[0]*1e-3)/1e9, 1) for k, v in res.items()},
+                     "ms": {k: round(v[0], 3) for k, v in res.items()},
+                     "fp32_rel_l2_err_vs_fp64": err}
+        log(f"dense P2P {name} {n}x{n}: " + ", ".join(
+            f"{k} {v} Gpairs/s" for k, v in out[name]["gpairs_per_s"].items())
+            + f", FP32 rel. err {err:.2e}")
+    return out
+
+
+def qbx_bench(actx, steps):
+    """2D Laplace double-layer QBX on the starfish r = 1 + 0.25 cos 5t: 1024
+    Gauss-Legendre panels of 16 nodes (2^14 nodes = sources = targets), one
+    centre per node at half a panel length inside, order 6, density
+    sin 3t + 0.01 N(0,1); time of the layer potential and of the gradient of
+    sum pot^2 with respect to density and node positions (adjoint kernels)."""
+    import torch
+
+    from sumpy_amd.autograd import qbx_layer_potential
+    npanels, porder = 1024, 16
+    x, _ = np.polynomial.legendre.leggauss(porder)
+    t = (np.arange(npanels)[:, None] + (x[None, :] + 1)/2).ravel()*2*np.pi/npanels
+    r = 1 + 0.25*np.cos(5*t)
+    X = torch.tensor(np.array([r*np.cos(t), r*np.sin(t)]), device=actx.device,
+                     requires_grad=True)
+    rng = np.random.default_rng(0)
+    sigma = torch.tensor(np.sin(3*t) + 0.01*rng.standard_normal(t.size), device=actx.device,
+                         requires_grad=True)
+    radius = 0.5*2*np.pi/npanels
+
+    def forward():
+        tan = torch.roll(X, -1, 1) - torch.roll(X, 1, 1)
+        n = torch.stack([tan[1], -tan[0]])/tan.norm(dim=0)
+        return qbx_layer_potential(X, X, X - radius*n, sigma*(2*np.pi/t.size), 6, normals=n)
+
+    def fwd_bwd():
+        X.grad = None
+        sigma.grad = None
+        (forward()**2).sum().backward()
+        return sigma.grad
+
+    with torch.no_grad():
+        ms_f, _, _ = time_passes(actx, forward, steps, 1)
+    ms_fb, _, _ = time_passes(actx, fwd_bwd, steps, 1)
+    npairs = t.size**2
+    log(f"QBX starfish DLP 2^14: forward {ms_f:.3f} ms, forward + adjoint {ms_fb:.3f} ms")
+    return {"nodes": int(t.size), "order": 6, "forward_ms": round(ms_f, 3),
+            "forward_and_adjoint_ms": round(ms_fb, 3),
+            "forward_gpairs_per_s": round(npairs/(ms_f*1e-3)/1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +303,8 @@ def main():
     ap.add_argument("--distribution", default="uniform", choices=["uniform", "gaussian8"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nsample", type=int, default=4096)
+    ap.add_argument("--p2p-dense-n", type=int, default=2**18,
+                    help="dense P2P size (sources = targets; 0 skips it)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one warm-up and one measured pass per form, then a 256 MiB "
                          "device copy (counter calibration); for rocprofv3 --pmc runs")
@@ -360,6 +435,9 @@ def main():
                 "launches_per_step": sp[dom]["launches"]//args.steps,
                 "avg_launch_ms": sp[dom]["ms"]/sp[dom]["launches"]}
 
+    dense = p2p_dense_bench(actx, args.p2p_dense_n, 3) if args.p2p_dense_n else None
+    qbx = qbx_bench(actx, 3)
+
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
@@ -407,6 +485,8 @@ def main():
                     **{f"m2l_precompute_{k}": round(v, 3) for k, v in setup.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "p2p_dense": dense,
+        "qbx_starfish_dlp": qbx,
     }
     print(json.dumps(line), flush=True)
 

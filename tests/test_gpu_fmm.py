@@ -193,3 +193,17 @@ def test_fmm_graph_replays_match_eager(actx, uniform_case, m2l_precision, p2p_pr
         ref = actx.to_numpy(ref)
         torch.cuda.synchronize()
         assert np.array_equal(got, ref), (it, np.abs(got - ref).max())
+
+
+def test_laplace3d_fmm_2e14_order6_vs_direct(actx):
+    """2^14 sources = targets uniform in [0,1]^3, strengths N(0,1), seed 0, at
+    most 30 particles per box, order 6, self-interaction excluded, plain FP64:
+    the FMM against the FP64 direct sum (the oracle's)."""
+    rng = np.random.default_rng(0)
+    n = 2**14
+    src, q = rng.random((3, n)), rng.standard_normal(n)
+    direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(n, dtype=np.int32))
+    (pot,), _, _ = run_fmm(actx, src, q, 6, max_particles=30)
+    err = rel(pot, direct)
+    print(f"rel. L2 error, order 6: {err:.3e}")
+    assert err < 3e-4
