@@ -295,23 +295,36 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
           for (int j0 = sub; j0 < nn; j0 += G*FOLD) {
             const int jend = min(nn, j0 + G*FOLD);
             T tp = 0, t0g = 0, t1g = 0, t2g = 0;
-            // this lane's own (excluded) pair, if it falls in this block, is
-            // skipped by splitting the loop there: no per-pair mask
-            const int js = (SELF && jself >= j0 && jself < jend && (jself - j0) % G == 0)
-                               ? jself : jend;
-            auto run = [&](int jb, int je) {
+            if constexpr (F32) {
+              // FP32: this lane's own (excluded) pair, if it falls in this
+              // block, is skipped by splitting the loop there (no per-pair
+              // mask; the FP32 loop is VALU-issue bound)
+              const int js = (SELF && jself >= j0 && jself < jend && (jself - j0) % G == 0)
+                                 ? jself : jend;
+              auto run = [&](int jb, int je) {
+                #pragma unroll 4
+                for (int j = jb; j < je; j += G) {
+                  const v4 sv = s_src[j];
+                  v4 dv = v4{0, 0, 0, 0};
+                  if constexpr (DIP) dv = s_dip[j];
+                  pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y,
+                                                 DIM == 3 ? pz - sv.z : T(0), sv.w, dv.x, dv.y,
+                                                 dv.z, false, tp, t0g, t1g, t2g);
+                }
+              };
+              run(j0, js);
+              run(js + G, jend);
+            } else {
               #pragma unroll 4
-              for (int j = jb; j < je; j += G) {
+              for (int j = j0; j < jend; j += G) {
                 const v4 sv = s_src[j];
                 v4 dv = v4{0, 0, 0, 0};
                 if constexpr (DIP) dv = s_dip[j];
                 pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y,
                                                DIM == 3 ? pz - sv.z : T(0), sv.w, dv.x, dv.y,
-                                               dv.z, false, tp, t0g, t1g, t2g);
+                                               dv.z, SELF && j == jself, tp, t0g, t1g, t2g);
               }
-            };
-            run(j0, js);
-            run(js + G, jend);
+            }
             pot += double(tp);
             if (GRAD) { g0 += double(t0g); g1 += double(t1g); g2 += double(t2g); }
           }
