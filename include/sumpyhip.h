@@ -188,7 +188,8 @@ int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
                             void *stream);
 
 /* One tree level of sumpyhip_m2l_apply: tile schedule (built on the host),
- * class data (FP64, nclasses x ncirc), packed source multipoles (compute
+ * class data (nclasses x ncirc: FP64, or at precision 2 the packed words of
+ * sumpyhip_m2l_pack_bf16_class_data), packed source multipoles (compute
  * precision, kpad per box), work rows (compute precision, npad per box) and,
  * when nsplit > 1, a partial buffer of (nsplit - 1) * ntiles * 64 * npad
  * entries of the compute precision. */
@@ -198,7 +199,7 @@ typedef struct sumpyhip_m2l_level {
   const int *tile_slot_starts;  /* ntiles + 1 */
   const int *slot_class;        /* nslots: class of the slot (level-relative) */
   const int *slot_sources;      /* nslots x 64 source boxes (-1: none) */
-  const double *class_data;
+  const void *class_data;
   const void *packed_src;
   int src_base_ibox;
   void *work;
@@ -218,6 +219,15 @@ typedef struct sumpyhip_m2l_level {
 int sumpyhip_m2l_apply(int precision, int nlevels, const sumpyhip_m2l_level *levels,
                        int nl, int nm, int npad, int kpad, const int *off_t,
                        const int *off_s, int ncirc, void *stream);
+
+/* Split-BF16 form (precision 2) support: the class data as packed words
+ * bf16(x) | bf16(x - bf16(x)) << 16, the input of sumpyhip_m2l_apply at
+ * precision 2 (level class_data pointers then point to these words).  No
+ * reference counterpart (a layout of this build's split-BF16 contraction).
+ * At precision 2, sumpyhip_m2l_preprocess writes rows of kpad hi then kpad lo
+ * bf16 values (the same bytes as kpad FP32 values). */
+int sumpyhip_m2l_pack_bf16_class_data(long long n, const double *class_data, void *packed,
+                                      void *stream);
 
 /* Postprocessing, replaces M2LPostprocessLocal.__call__
  * (sumpy/e2e.py:767-859): locals[b, i] = work[b, i] * ratio^deg_i. */

@@ -24,8 +24,7 @@ namespace sumpyhip {
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int MAXQ = 13;  // max derivative order of M2P/P2L (order <= 12)
-constexpr int TRI = (MAXQ + 1)*(MAXQ + 2)/2;
+constexpr int MAX_LIST_ORDER = 12;  // M2P / P2L orders (compile-time shells)
 
 __device__ __forceinline__ int full_index(const sumpyhip_expn &e, int i, int j, int k) {
   const int p1 = e.order + 1;
@@ -421,116 +420,219 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
 
 // }}}
 
-// {{{ scaled Laplace derivatives for M2P/P2L (m_x <= 1 stored, recurrence
-// of sumpy/derivative_taker.py:227-259)
+// {{{ scaled Laplace derivatives for M2P/P2L, one shell |m| = n at a time
 
-struct Derivs {
-  double v[2][TRI];  // [m_x][tri(m_y + m_z) + m_y]
-};
+// The values D(m) = rho^|m| d^m (1/r), m_x <= 1, |m| <= Q, are produced by
+// the recurrence of sumpy/derivative_taker.py:227-259 shell by shell: shell n
+// (the 2n + 1 values with |m| = n) needs only shells n - 1 and n - 2, so with
+// the order a template parameter the three live shells stay in registers and
+// every index is a compile-time constant.  Shell n holds m at
+// m_x ? n + 1 + m_y : m_y, and the concatenation of shells 0..n puts m at
+// n^2 + that index (the PDE-compressed stored set is exactly m_x <= 1).
 
-__device__ __forceinline__ int tri(int s) { return s*(s + 1)/2; }
+__host__ __device__ constexpr int sidx(int n, int mx, int my) { return mx ? n + 1 + my : my; }
 
-__device__ __forceinline__ double dget(const Derivs &D, int mx, int my, int mz) {
-  if (mx < 0 || my < 0 || mz < 0) return 0.0;
-  return D.v[mx][tri(my + mz) + my];
+__device__ __forceinline__ int shell_pos(int i, int j, int k) {
+  const int n = i + j + k;
+  return n*n + sidx(n, i, j);
 }
 
-// Fills D(m) = rho^|m| d^m (1/r) at x for m_x <= 1, |m| <= q (q <= MAXQ).
-__device__ void lap3d_derivs(double x, double y, double z, double rscale, int q, Derivs &D) {
-  const double xs[3] = {x/rscale, y/rscale, z/rscale};
-  const double r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
-  const double ri = rsqrt(r2s);
-  const double rinv = ri/rscale;  // 1/r = 1/(rho r~)
-  const double ri2 = ri*ri;
-  for (int n = 0; n <= q; ++n) {
-    for (int mx = 0; mx <= 1 && mx <= n; ++mx) {
-      for (int my = 0; my <= n - mx; ++my) {
-        const int mz = n - mx - my;
-        const int mi[3] = {mx, my, mz};
-        double val;
-        if (my <= 1 && mz <= 1) {
-          // direct derivatives of 1/r for m in {0,1}^3
-          if (n == 0) val = rinv;
-          else if (n == 1) val = -xs[mx ? 0 : (my ? 1 : 2)]*ri2*rinv;
-          else if (n == 2) {
-            int a = mx ? 0 : 1, b = mz ? 2 : 1;
-            val = 3.0*xs[a]*xs[b]*ri2*ri2*rinv;
-          } else {
-            val = -15.0*xs[0]*xs[1]*xs[2]*ri2*ri2*ri2*rinv;
-          }
+struct LapGeo {
+  double xs[3];     // x / rho
+  double ri2, rinv; // 1 / |x / rho|^2, 1 / |x|
+  __device__ LapGeo(double x, double y, double z, double rscale) {
+    const double inv = 1.0/rscale;
+    xs[0] = x*inv; xs[1] = y*inv; xs[2] = z*inv;
+    const double r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
+    const double ri = rsqrt(r2s);
+    ri2 = ri*ri;
+    rinv = ri*inv;
+  }
+};
+
+// shell N from shells N - 1 (p1) and N - 2 (p2)
+template <int N>
+__device__ __forceinline__ void lap3d_shell(const LapGeo &g, const double *p1, const double *p2,
+                                            double *cur) {
+  #pragma unroll
+  for (int mx = 0; mx <= 1; ++mx) {
+    #pragma unroll
+    for (int my = 0; my <= N - mx; ++my) {
+      const int mz = N - mx - my;
+      double val;
+      if (my <= 1 && mz <= 1) {
+        // m in {0,1}^3: closed forms
+        if (N == 0) {
+          val = g.rinv;
+        } else if (N == 1) {
+          val = -g.xs[mx ? 0 : (my ? 1 : 2)]*g.ri2*g.rinv;
+        } else if (N == 2) {
+          const int a = mx ? 0 : 1, b = mz ? 2 : 1;
+          val = 3.0*g.xs[a]*g.xs[b]*(g.ri2*g.ri2)*g.rinv;
         } else {
-          const int d = my >= 2 ? 1 : 2;  // first axis with m_d >= 2 (m_x <= 1)
-          double acc = 0;
-          for (int i = 0; i < 3; ++i) {
-            const int nn = mi[i];
-            if (nn == 0) continue;
-            int m1[3] = {mi[0], mi[1], mi[2]}; m1[i] -= 1;
-            int m2[3] = {mi[0], mi[1], mi[2]}; m2[i] -= 2;
-            const double v1 = dget(D, m1[0], m1[1], m1[2]);
-            const double v2 = dget(D, m2[0], m2[1], m2[2]);
-            if (i == d) acc -= (2*nn - 1)*xs[i]*v1 + double((nn - 1)*(nn - 1))*v2;
-            else acc -= 2*nn*xs[i]*v1 + double(nn*(nn - 1))*v2;
-          }
-          val = acc*ri2;
+          val = -15.0*g.xs[0]*g.xs[1]*g.xs[2]*(g.ri2*g.ri2*g.ri2)*g.rinv;
         }
-        D.v[mx][tri(my + mz) + my] = val;
+      } else {
+        // d: the first axis with m_d >= 2 (m_x <= 1, so y or z)
+        const int d = my >= 2 ? 1 : 2;
+        double acc = 0.0;
+        if (mx == 1) acc += 2.0*g.xs[0]*p1[sidx(N - 1, 0, my)];
+        if (my >= 1) {
+          const double c1 = d == 1 ? 2*my - 1 : 2*my;
+          const double c2 = d == 1 ? (my - 1)*(my - 1) : my*(my - 1);
+          acc += c1*g.xs[1]*p1[sidx(N - 1, mx, my - 1)];
+          if (my >= 2) acc += c2*p2[sidx(N - 2, mx, my - 2)];
+        }
+        if (mz >= 1) {
+          const double c1 = d == 2 ? 2*mz - 1 : 2*mz;
+          const double c2 = d == 2 ? (mz - 1)*(mz - 1) : mz*(mz - 1);
+          acc += c1*g.xs[2]*p1[sidx(N - 1, mx, my)];
+          if (mz >= 2) acc += c2*p2[sidx(N - 2, mx, my)];
+        }
+        val = -acc*g.ri2;
       }
+      cur[sidx(N, mx, my)] = val;
     }
   }
 }
 
-// D(m + e_ax) for stored m (m_x <= 1): m_x = 2 follows from Laplace's
-// equation, D(2,y,z) = -D(0,y+2,z) - D(0,y,z+2).
-__device__ __forceinline__ double dshift(const Derivs &D, int mx, int my, int mz, int ax) {
-  if (ax == 0) {
-    if (mx == 0) return dget(D, 1, my, mz);
-    return -dget(D, 0, my + 2, mz) - dget(D, 0, my, mz + 2);
+// Runs shells N..Q, handing each to f.template shell<N>(cur, prev) (prev =
+// shell N - 1).
+template <int N, int Q, bool DONE = (N > Q)>
+struct LapShells {
+  template <class F>
+  static __device__ __forceinline__ void run(const LapGeo &g, const double *p2,
+                                             const double *p1, F &f) {
+    double cur[2*N + 1];
+    lap3d_shell<N>(g, p1, p2, cur);
+    f.template shell<N>(cur, p1);
+    LapShells<N + 1, Q>::run(g, p1, cur, f);
   }
-  if (ax == 1) return dget(D, mx, my + 1, mz);
-  return dget(D, mx, my, mz + 1);
+};
+
+template <int N, int Q>
+struct LapShells<N, Q, true> {
+  template <class F>
+  static __device__ __forceinline__ void run(const LapGeo &, const double *, const double *,
+                                             F &) {}
+};
+
+template <int Q, class F>
+__device__ __forceinline__ void lap3d_shells(const LapGeo &g, F &f) {
+  const double none[1] = {0.0};
+  LapShells<0, Q>::run(g, none, none, f);
+}
+
+// D(m + e_x) for m = (mx, my, .) of shell N - 1, from shell N (m_x = 2 by
+// Laplace's equation: D(2, y, z) = -D(0, y + 2, z) - D(0, y, z + 2))
+template <int N>
+__device__ __forceinline__ double shift_x(const double *cur, int mx, int my) {
+  return mx == 0 ? cur[sidx(N, 1, my)] : -cur[sidx(N, 0, my + 2)] - cur[sidx(N, 0, my)];
 }
 
 // }}}
 
 // {{{ M2P (list 3), accumulate
 
-template <bool GRAD>
-__global__ __launch_bounds__(WAVE) void m2p_lap3d(
-    sumpyhip_expn e, int ntgt, const int *target_boxes, const int *sstarts,
-    const int *slists, const int *bts, const int *btc, const double *centers,
-    int nboxes, const double *tx, const double *ty, const double *tz, double rscale,
-    double scale, const double *src, int sbase, double *pot, double *gx, double *gy,
+constexpr int LW = 4;  // waves per block of the list 3 / 4 kernels
+
+// M2P consumer: P += sum_m M(m) D(m) (shells <= ORD); with GRAD also
+// G_ax += sum_m M(m) D(m + e_ax) from the next shell.
+template <int ORD, bool GRAD>
+struct M2PSum {
+  const double *M;  // this lane's source box, shell order (LDS)
+  double P = 0, G[3] = {0, 0, 0};
+  template <int N>
+  __device__ __forceinline__ void shell(const double *cur, const double *prev) {
+    if (N <= ORD) {
+      #pragma unroll
+      for (int i = 0; i < 2*N + 1; ++i) P += M[N*N + i]*cur[i];
+    }
+    if (GRAD && N >= 1) {
+      constexpr int n = N - 1;
+      #pragma unroll
+      for (int mx = 0; mx <= 1; ++mx) {
+        #pragma unroll
+        for (int my = 0; my <= n - mx; ++my) {
+          const double m = M[n*n + sidx(n, mx, my)];
+          G[0] += m*shift_x<N>(cur, mx, my);
+          G[1] += m*cur[sidx(N, mx, my + 1)];
+          G[2] += m*cur[sidx(N, mx, my)];
+        }
+      }
+    }
+  }
+};
+
+// One block of LW waves per target box.  Lanes are targets; when the box has
+// at most 32 (16) targets a wave's lanes form 2 (4) groups of 32 (16), each on
+// its own source box.  Every wave stages the multipoles of its current source
+// boxes in LDS in shell order; the partial sums of waves and groups are added
+// in LDS at the end.
+template <int ORD, bool GRAD>
+__global__ __launch_bounds__(64*LW) void m2p_lap3d(
+    sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
+    const int *__restrict__ sstarts, const int *__restrict__ slists,
+    const int *__restrict__ bts, const int *__restrict__ btc,
+    const double *__restrict__ centers, int nboxes, const double *__restrict__ tx,
+    const double *__restrict__ ty, const double *__restrict__ tz, double rscale, double scale,
+    const double *__restrict__ src, int sbase, double *pot, double *gx, double *gy,
     double *gz) {
-  const int lane = threadIdx.x;
-  Derivs D;
+  constexpr int NS = (ORD + 1)*(ORD + 1);
+  constexpr int NC = GRAD ? 4 : 1;
+  __shared__ int sPos[NS];
+  __shared__ double sM[LW][4][NS];
+  __shared__ double red[NC][LW][64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int s = tid; s < NS; s += 64*LW)
+    sPos[s] = shell_pos(e.stored_ids[s], e.stored_ids[NS + s], e.stored_ids[2*NS + s]);
+  __syncthreads();
   for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int is0 = sstarts[ib], is1 = sstarts[ib + 1];
+    if (is0 == is1) continue;
     const int tb = target_boxes[ib];
     const int t0 = bts[tb], nt = btc[tb];
-    for (int tc = 0; tc < nt; tc += WAVE) {
-      const bool valid = tc + lane < nt;
-      const int t = t0 + tc + (valid ? lane : 0);
-      double P = 0, G[3] = {0, 0, 0};
-      for (int is = sstarts[ib]; is < sstarts[ib + 1]; ++is) {
-        const int sb = slists[is];
-        const double bx = tx[t] - centers[sb], by = ty[t] - centers[nboxes + sb],
-                     bz = tz[t] - centers[2*(size_t)nboxes + sb];
-        lap3d_derivs(bx, by, bz, rscale, e.order + (GRAD ? 1 : 0), D);
-        const double *M = src + (size_t)(sb - sbase)*e.nstored;
-        for (int s = 0; s < e.nstored; ++s) {
-          const int i = e.stored_ids[s], j = e.stored_ids[e.nstored + s],
-                    k = e.stored_ids[2*e.nstored + s];
-          const double m = M[s];
-          P += m*dget(D, i, j, k);
-          if (GRAD)
-            for (int ax = 0; ax < 3; ++ax) G[ax] += m*dshift(D, i, j, k, ax);
+    const int G = nt <= 16 ? 16 : nt <= 32 ? 32 : 64, ngrp = 64/G;
+    const int grp = lane/G, tl = lane - grp*G;
+    for (int tc = 0; tc < nt; tc += G) {
+      const int t = t0 + tc + (tc + tl < nt ? tl : 0);
+      const double x = tx[t], y = ty[t], z = tz[t];
+      M2PSum<ORD, GRAD> f;
+      f.M = sM[wave][grp];
+      for (int base = is0 + wave*ngrp; base < is1; base += LW*ngrp) {
+        wave_sync();
+        for (int v = lane; v < ngrp*NS; v += 64) {
+          const int g2 = v/NS, s = v - g2*NS;
+          if (base + g2 < is1)
+            sM[wave][g2][sPos[s]] = src[(size_t)(slists[base + g2] - sbase)*NS + s];
+        }
+        wave_sync();
+        if (base + grp < is1) {
+          const int sb = slists[base + grp];
+          const LapGeo g(x - centers[sb], y - centers[nboxes + sb],
+                         z - centers[2*(size_t)nboxes + sb], rscale);
+          lap3d_shells<ORD + (GRAD ? 1 : 0)>(g, f);
         }
       }
-      if (valid) {
-        if (pot) pot[t] += scale*P;
+      red[0][wave][lane] = f.P;
+      if (GRAD)
+        for (int c = 0; c < 3; ++c) red[1 + c][wave][lane] = f.G[c];
+      __syncthreads();
+      if (tid < G && tc + tid < nt) {
+        double v[NC];
+        for (int c = 0; c < NC; ++c) {
+          v[c] = 0;
+          for (int w = 0; w < LW; ++w)
+            for (int g2 = 0; g2 < ngrp; ++g2) v[c] += red[c][w][g2*G + tid];
+        }
+        const int tt = t0 + tc + tid;
+        if (pot) pot[tt] += scale*v[0];
         if (GRAD) {
-          gx[t] += scale*G[0]/rscale; gy[t] += scale*G[1]/rscale; gz[t] += scale*G[2]/rscale;
+          gx[tt] += scale*v[1]/rscale; gy[tt] += scale*v[2]/rscale; gz[tt] += scale*v[3]/rscale;
         }
       }
+      __syncthreads();
     }
   }
 }
@@ -539,56 +641,113 @@ __global__ __launch_bounds__(WAVE) void m2p_lap3d(
 
 // {{{ P2L (list 4), overwrite
 
-constexpr int P2L_MAXS = (MAXQ)*(MAXQ);  // (order+1)^2 for order <= 12
+// P2L consumer: acc(m) += q0 D(m) (+ sum_ax qd_ax D(m + e_ax) for dipoles),
+// acc in shell order.
+template <int ORD, bool DIP>
+struct P2LSum {
+  double acc[(ORD + 1)*(ORD + 1)];
+  double q0, qd[3];
+  template <int N>
+  __device__ __forceinline__ void shell(const double *cur, const double *prev) {
+    if (N <= ORD) {
+      #pragma unroll
+      for (int i = 0; i < 2*N + 1; ++i) acc[N*N + i] += q0*cur[i];
+    }
+    if (DIP && N >= 1) {
+      constexpr int n = N - 1;
+      #pragma unroll
+      for (int mx = 0; mx <= 1; ++mx) {
+        #pragma unroll
+        for (int my = 0; my <= n - mx; ++my)
+          acc[n*n + sidx(n, mx, my)] += qd[0]*shift_x<N>(cur, mx, my) +
+                                        qd[1]*cur[sidx(N, mx, my + 1)] +
+                                        qd[2]*cur[sidx(N, mx, my)];
+      }
+    }
+  }
+};
 
-template <bool DIP>
-__global__ __launch_bounds__(WAVE) void p2l_lap3d(
-    sumpyhip_expn e, int ntgt, const int *target_boxes, const int *sstarts,
-    const int *slists, const int *bss, const int *bsc, const double *centers,
-    int nboxes, const double *sx, const double *sy, const double *sz, const double *w0,
-    const double *wx, const double *wy, const double *wz, double rscale, double *out,
+// One wave per target box (LW boxes per block): the sources of the box's
+// list-4 boxes are concatenated (box offsets from a wave scan in LDS, 64
+// boxes at a time) and spread over the lanes; every lane keeps the
+// (ORD + 1)^2 sums in registers, and the wave adds them in LDS, 16
+// coefficients at a time.
+template <int ORD, bool DIP>
+__global__ __launch_bounds__(64*LW) void p2l_lap3d(
+    sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
+    const int *__restrict__ sstarts, const int *__restrict__ slists,
+    const int *__restrict__ bss, const int *__restrict__ bsc,
+    const double *__restrict__ centers, int nboxes, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ w0, const double *__restrict__ wx,
+    const double *__restrict__ wy, const double *__restrict__ wz, double rscale, double *out,
     int base) {
-  __shared__ double red[32][WAVE];
-  const int lane = threadIdx.x;
-  Derivs D;
-  double acc[P2L_MAXS];
-  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+  constexpr int NS = (ORD + 1)*(ORD + 1);
+  constexpr int RC = 16;  // coefficients per reduction round
+  __shared__ int sStored[NS];  // shell position -> stored index
+  __shared__ int sOff[LW][64];
+  __shared__ double red[LW][RC][64 + 1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int s = tid; s < NS; s += 64*LW)
+    sStored[shell_pos(e.stored_ids[s], e.stored_ids[NS + s], e.stored_ids[2*NS + s])] = s;
+  __syncthreads();
+  int *off = sOff[wave];
+  for (int ib = blockIdx.x*LW + wave; ib < ntgt; ib += gridDim.x*LW) {
     const int tb = target_boxes[ib];
-    if (sstarts[ib] == sstarts[ib + 1]) {
+    const int is0 = sstarts[ib], is1 = sstarts[ib + 1];
+    double *o = out + (size_t)(tb - base)*NS;
+    if (is0 == is1) {
       // empty list 4: the expansion is overwritten with zeros
-      for (int m = lane; m < e.nstored; m += WAVE) out[(size_t)(tb - base)*e.nstored + m] = 0.0;
+      for (int m = lane; m < NS; m += 64) o[m] = 0.0;
       continue;
     }
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
-    for (int s = 0; s < e.nstored; ++s) acc[s] = 0;
-    for (int is = sstarts[ib]; is < sstarts[ib + 1]; ++is) {
-      const int sb = slists[is];
-      const int s0 = bss[sb], ns = bsc[sb];
-      for (int sc = lane; sc < ns; sc += WAVE) {
-        const int s = s0 + sc;
-        lap3d_derivs(c0 - sx[s], c1 - sy[s], c2 - sz[s], rscale, e.order + (DIP ? 1 : 0), D);
-        const double q0 = w0[s];
-        const double qd[3] = {DIP ? wx[s]/rscale : 0.0, DIP ? wy[s]/rscale : 0.0,
-                              DIP ? wz[s]/rscale : 0.0};
-        for (int m = 0; m < e.nstored; ++m) {
-          const int i = e.stored_ids[m], j = e.stored_ids[e.nstored + m],
-                    k = e.stored_ids[2*e.nstored + m];
-          double v = q0*dget(D, i, j, k);
-          if (DIP) for (int ax = 0; ax < 3; ++ax) v += qd[ax]*dshift(D, i, j, k, ax);
-          acc[m] += v;
+    P2LSum<ORD, DIP> f;
+    #pragma unroll
+    for (int m = 0; m < NS; ++m) f.acc[m] = 0.0;
+    for (int cb = is0; cb < is1; cb += 64) {
+      const int nb = min(64, is1 - cb);
+      // exclusive scan of the source counts of boxes cb .. cb + nb - 1
+      const int c = lane < nb ? bsc[slists[cb + lane]] : 0;
+      int v = c;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+      }
+      const int total = __shfl(v, 63, 64);
+      wave_sync();
+      if (lane < nb) off[lane] = v - c;
+      wave_sync();
+      for (int jj = lane; jj < total; jj += 64) {
+        int lo = 0, hi = nb - 1;  // largest b with off[b] <= jj
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (off[mid] <= jj) lo = mid; else hi = mid - 1;
         }
+        const int s = bss[slists[cb + lo]] + (jj - off[lo]);
+        f.q0 = w0[s];
+        if (DIP) {
+          f.qd[0] = wx[s]/rscale; f.qd[1] = wy[s]/rscale; f.qd[2] = wz[s]/rscale;
+        }
+        const LapGeo g(c0 - sx[s], c1 - sy[s], c2 - sz[s], rscale);
+        lap3d_shells<ORD + (DIP ? 1 : 0)>(g, f);
       }
     }
-    double *o = out + (size_t)(tb - base)*e.nstored;
-    for (int m0 = 0; m0 < e.nstored; m0 += 32) {
-      __syncthreads();
-      for (int m = 0; m < 32; ++m) red[m][lane] = (m0 + m < e.nstored) ? acc[m0 + m] : 0.0;
-      __syncthreads();
-      if (lane < 32 && m0 + lane < e.nstored) {
-        double v = 0;
-        for (int l = 0; l < WAVE; ++l) v += red[lane][l];
-        o[m0 + lane] = v;
-      }
+    #pragma unroll
+    for (int m0 = 0; m0 < NS; m0 += RC) {
+      wave_sync();
+      #pragma unroll
+      for (int m = 0; m < RC; ++m)
+        if (m0 + m < NS) red[wave][m][lane] = f.acc[m0 + m];
+      wave_sync();
+      // lane: coefficient m0 + lane / 4, part lane % 4 of the 64 lane sums
+      const int m = lane >> 2, part = lane & 3;
+      double v = 0;
+      for (int q = part; q < 64; q += 4) v += red[wave][m][q];
+      v += __shfl_xor(v, 1, 4);
+      v += __shfl_xor(v, 2, 4);
+      if (part == 0 && m0 + m < NS) o[sStored[m0 + m]] = v;
     }
   }
 }
@@ -747,20 +906,31 @@ extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   int rc = check_expn(expn, "m2p");
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
-  SUMPY_REQUIRE(expn->order + 1 <= MAXQ, "m2p: order %d > %d", expn->order, MAXQ - 1);
+  SUMPY_REQUIRE(expn->order <= MAX_LIST_ORDER, "m2p: order %d > %d", expn->order,
+                MAX_LIST_ORDER);
   SUMPY_REQUIRE(expn->nstored == (expn->order + 1)*(expn->order + 1),
                 "m2p: only PDE-compressed expansions are implemented");
   SUMPY_REQUIRE(!gx || (gy && gz), "m2p: incomplete gradient outputs");
-  if (gx)
-    m2p_lap3d<true><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
-        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
-        rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);
-  else
-    m2p_lap3d<false><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
-        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
-        rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);
+  const int grid = ntgt_boxes < (1 << 30) ? ntgt_boxes : (1 << 30);
+#define SUMPY_M2P(ORD)                                                                       \
+  case ORD:                                                                                  \
+    if (gx)                                                                                  \
+      m2p_lap3d<ORD, true><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                         \
+          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
+          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
+          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);                    \
+    else                                                                                     \
+      m2p_lap3d<ORD, false><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                        \
+          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
+          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
+          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);                    \
+    break;
+  switch (expn->order) {
+    SUMPY_M2P(0) SUMPY_M2P(1) SUMPY_M2P(2) SUMPY_M2P(3) SUMPY_M2P(4) SUMPY_M2P(5)
+    SUMPY_M2P(6) SUMPY_M2P(7) SUMPY_M2P(8) SUMPY_M2P(9) SUMPY_M2P(10) SUMPY_M2P(11)
+    SUMPY_M2P(12)
+  }
+#undef SUMPY_M2P
   return check_launch("m2p");
 }
 
@@ -777,19 +947,30 @@ extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   int rc = check_expn(expn, "p2l");
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
-  SUMPY_REQUIRE(expn->order + 1 <= MAXQ, "p2l: order %d > %d", expn->order, MAXQ - 1);
+  SUMPY_REQUIRE(expn->order <= MAX_LIST_ORDER, "p2l: order %d > %d", expn->order,
+                MAX_LIST_ORDER);
   SUMPY_REQUIRE(expn->nstored == (expn->order + 1)*(expn->order + 1),
                 "p2l: only PDE-compressed expansions are implemented");
   SUMPY_REQUIRE(!wx || (wy && wz), "p2l: incomplete dipole weights");
-  if (wx)
-    p2l_lap3d<true><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
-        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0,
-        wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);
-  else
-    p2l_lap3d<false><<<grid_for(ntgt_boxes), 64, 0, (hipStream_t)stream>>>(
-        *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
-        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0,
-        wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);
+  const int grid = (ntgt_boxes + LW - 1)/LW;
+#define SUMPY_P2L(ORD)                                                                       \
+  case ORD:                                                                                  \
+    if (wx)                                                                                  \
+      p2l_lap3d<ORD, true><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                         \
+          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
+          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
+          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
+    else                                                                                     \
+      p2l_lap3d<ORD, false><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                        \
+          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
+          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
+          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
+    break;
+  switch (expn->order) {
+    SUMPY_P2L(0) SUMPY_P2L(1) SUMPY_P2L(2) SUMPY_P2L(3) SUMPY_P2L(4) SUMPY_P2L(5)
+    SUMPY_P2L(6) SUMPY_P2L(7) SUMPY_P2L(8) SUMPY_P2L(9) SUMPY_P2L(10) SUMPY_P2L(11)
+    SUMPY_P2L(12)
+  }
+#undef SUMPY_P2L
   return check_launch("p2l");
 }

@@ -154,7 +154,7 @@ struct LevelArgs {
   int item_end;         // cumulative (tile, split) items up to this level
   int sbase, tbase;
   const int *tile_targets, *tile_slot_starts, *slot_class, *slot_sources;
-  const double *class_data;
+  const void *class_data;     // FP64 values, or packed (hi | lo) words at precision 2
   const void *packed;
   void *work;
   void *partial;        // (nsplit - 1) x ntiles*64 x npad sums of splits 1..
@@ -260,7 +260,8 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   // global -> registers for one slot; every load is unconditional (clamped
   // address, then a select) so all of them are in flight together
   auto fetch = [&](int slot) {
-    const double *D = L.class_data + (size_t)L.slot_class[slot]*a.ncirc;
+    const double *D =
+        static_cast<const double *>(L.class_data) + (size_t)L.slot_class[slot]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
       dreg[i] = D[e < a.ncirc ? e : 0];
@@ -374,6 +375,7 @@ __global__ __launch_bounds__(256) void m2l_reduce_splits(ApplyArgs a) {
 // {{{ split-BF16 contraction (x = hi + lo in bf16; hi*hi + hi*lo + lo*hi, FP32 sums)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr int KB = 32;        // k per v_mfma_f32_16x16x32_bf16
 constexpr int XB = 128 + 8;   // LDS row of one target column (bf16 entries, padded)
@@ -383,28 +385,57 @@ __device__ __forceinline__ void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
   lo = static_cast<__bf16>(v - static_cast<float>(hi));
 }
 
-// Same block/item structure as m2l_apply_kernel; per class slot the class
-// data (FP32 in LDS) and the tile's 64 source columns (hi and lo bf16 rows,
-// k contiguous) are staged, and every 32-deep k step of a wave gathers 8
-// class-data values per A row, splits them, and issues 3 bf16 MFMAs per
-// (row block, column block).  Operand layout of v_mfma_f32_16x16x32_bf16:
-// A row = lane % 16, B column = lane % 16, the 8 k of a lane are
-// 8 (lane / 16) + e in both; accumulator register r = row 4 (lane / 16) + r.
+__device__ __forceinline__ unsigned bf16_bits(__bf16 h) {
+  return (unsigned)__builtin_bit_cast(unsigned short, h);
+}
+
+// class data -> packed words hi | lo << 16 (bf16 bit patterns), once per launch
+__global__ void m2l_pack_bf16_class_data(size_t n, const double *data, unsigned *out) {
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    __bf16 h, l;
+    split_bf16(float(data[i]), h, l);
+    out[i] = bf16_bits(h) | (bf16_bits(l) << 16);
+  }
+}
+
+// stored multipoles -> rows of kpad hi then kpad lo bf16 (zero padded)
+__global__ void m2l_preprocess_bf16_kernel(int nboxes, int nstored, const double *src, int kpad,
+                                           unsigned short *packed) {
+  const size_t n = (size_t)nboxes*kpad;
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    const size_t b = i/kpad;
+    const int k = i % kpad;
+    __bf16 h, l;
+    split_bf16(k < nstored ? float(src[b*nstored + k]) : 0.f, h, l);
+    packed[b*2*kpad + k] = (unsigned short)bf16_bits(h);
+    packed[b*2*kpad + kpad + k] = (unsigned short)bf16_bits(l);
+  }
+}
+
+// Same block/item structure as m2l_apply_kernel.  Inputs come pre-split:
+// class data as packed (hi | lo) words, multipole rows as kpad hi then kpad lo
+// bf16, so staging is a plain copy.  The next two slots are fetched into two
+// register sets while a slot is contracted (each fetch has two slots of MFMA
+// work to land); per slot the class words and the tile's 64 hi / lo columns
+// (k contiguous) are staged in LDS, and every 32-deep k step of a wave
+// gathers 8 class words per A row, regroups their halves with byte permutes
+// and issues 3 bf16 MFMAs per (row block, column block).  Operand layout of
+// v_mfma_f32_16x16x32_bf16: A row = lane % 16, B column = lane % 16, the 8 k
+// of a lane are 8 (lane / 16) + e in both; accumulator register r = row
+// 4 (lane / 16) + r.
 template <int RBW>
 __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   constexpr int DPT = Limits<RBW>::DPT;
-  // a thread loads 8 consecutive k (two float4) of its column for chunks
-  // m = wave + 4 i, i < 4 (kpad <= 128)
-  constexpr int XPT = 8;
   extern __shared__ unsigned char lds_raw[];
-  // class data split once per slot: word = hi | lo << 16 (bf16 bit patterns)
   unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
   __bf16 *sXh = reinterpret_cast<__bf16 *>(sD + ((a.ncirc + 3) & ~3));  // [64][XB]
   __bf16 *sXl = sXh + TW*XB;                                             // [64][XB]
   int *sOff = reinterpret_cast<int *>(sXl + TW*XB);                      // [kpad]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int kq = a.kpad/8;  // 8-wide k chunks per source row
+  const int kq = a.kpad/8;  // 8-wide k chunks per row
   const int xcd = blockIdx.x & 7;
   int r = blockIdx.x >> 3, lev = 0, li = -1;
   for (; lev < a.nlev; ++lev) {
@@ -415,7 +446,8 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   }
   if (li < 0) return;
   const LevelArgs L = a.lv[lev];
-  const float *packed = reinterpret_cast<const float *>(L.packed);
+  const unsigned short *packed = reinterpret_cast<const unsigned short *>(L.packed);
+  const unsigned *cwords = reinterpret_cast<const unsigned *>(L.class_data);
   float *work = reinterpret_cast<float *>(L.work);
   for (int k = tid; k < a.kpad; k += 256) sOff[k] = k < a.nm ? a.off_s[k] : 0;
   int offT[RBW];
@@ -423,130 +455,122 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     const int row = (wave + 4*q)*16 + (lane & 15);
     offT[q] = (row < a.nl) ? a.off_t[row] : 0;
   }
-  double dreg[DPT];
-  float4 xreg[XPT];
-  auto fetch = [&](int slot) {
-    const double *D = L.class_data + (size_t)L.slot_class[slot]*a.ncirc;
+  // one slot's staging data: DPT class words and, for column `lane`, the hi
+  // and lo 8-chunks m = wave + 4 i (i < 4)
+  struct Regs {
+    unsigned d[DPT];
+    u4 xh[4], xl[4];
+  };
+  auto fetch = [&](int slot, Regs &g) {
+    const unsigned *D = cwords + (size_t)L.slot_class[slot]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      dreg[i] = D[e < a.ncirc ? e : 0];
+      g.d[i] = D[e < a.ncirc ? e : 0];
     }
     const int sb = L.slot_sources[(size_t)slot*TW + lane];
-    const float *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*a.kpad;
-    for (int i = 0; i < XPT/2; ++i) {
-      const int m = wave + 4*i;
-      const float4 *src = reinterpret_cast<const float4 *>(row + (m < kq ? m : 0)*8);
-      xreg[2*i] = src[0];
-      xreg[2*i + 1] = src[1];
+    const unsigned short *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*2*a.kpad;
+    for (int i = 0; i < 4; ++i) {
+      const int m = wave + 4*i, mm = m < kq ? m : 0;
+      g.xh[i] = *reinterpret_cast<const u4 *>(row + mm*8);
+      g.xl[i] = *reinterpret_cast<const u4 *>(row + a.kpad + mm*8);
     }
     if (sb < 0)
-      for (int i = 0; i < XPT; ++i) xreg[i] = float4{0, 0, 0, 0};
+      for (int i = 0; i < 4; ++i) g.xh[i] = g.xl[i] = u4{0, 0, 0, 0};
   };
-  auto stage = [&]() {
+  auto stage = [&](const Regs &g) {
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      if (e < a.ncirc) {
-        __bf16 h, l;
-        split_bf16(float(dreg[i]), h, l);
-        sD[e] = (unsigned)__builtin_bit_cast(unsigned short, h) |
-                ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
-      }
+      if (e < a.ncirc) sD[e] = g.d[i];
     }
-    for (int i = 0; i < XPT/2; ++i) {
+    for (int i = 0; i < 4; ++i) {
       const int m = wave + 4*i;
       if (m < kq) {
-        const float v[8] = {xreg[2*i].x, xreg[2*i].y, xreg[2*i].z, xreg[2*i].w,
-                            xreg[2*i + 1].x, xreg[2*i + 1].y, xreg[2*i + 1].z,
-                            xreg[2*i + 1].w};
-        bf16x8 h, l;
-        for (int j = 0; j < 8; ++j) {
-          __bf16 hh, ll;
-          split_bf16(v[j], hh, ll);
-          h[j] = hh;
-          l[j] = ll;
-        }
-        // one 16-byte store each (the column stride, 17 x 16 bytes, spreads
-        // the 64 lanes over the bank groups)
-        *reinterpret_cast<bf16x8 *>(sXh + lane*XB + m*8) = h;
-        *reinterpret_cast<bf16x8 *>(sXl + lane*XB + m*8) = l;
+        *reinterpret_cast<u4 *>(sXh + lane*XB + m*8) = g.xh[i];
+        *reinterpret_cast<u4 *>(sXl + lane*XB + m*8) = g.xl[i];
       }
     }
   };
-  {
-    const int tile = li/L.nsplit, split = li - tile*L.nsplit;
-    typedef float v4f_t __attribute__((ext_vector_type(4)));
-    v4f_t acc[RBW][4];
-    for (int q = 0; q < RBW; ++q)
-      for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
-    const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
-    const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
-    const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
-    if (s0 < s1) {
-      fetch(s0);
-      __syncthreads();
-      stage();
-      __syncthreads();
-    }
-    const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
-    for (int slot = s0; slot < s1; ++slot) {
-      if (slot + 1 < s1) fetch(slot + 1);
+  typedef float v4f_t __attribute__((ext_vector_type(4)));
+  v4f_t acc[RBW][4];
+  for (int q = 0; q < RBW; ++q)
+    for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
+  const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
+  auto contract = [&]() {
+    #pragma unroll
+    for (int k0 = 0; k0 < 128; k0 += KB) {
+      if (k0 >= a.kpad) break;
+      int os[8];
       #pragma unroll
-      for (int k0 = 0; k0 < 128; k0 += KB) {
-        if (k0 >= a.kpad) break;
-        // B fragments: 8 consecutive k of one column (16-byte LDS reads)
-        bf16x8 bh[4], bl[4];
+      for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
+      bf16x8 ah[RBW], al[RBW];
+      #pragma unroll
+      for (int q = 0; q < RBW; ++q) {
+        unsigned w[8];
         #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-          const int col = cb*16 + (lane & 15);
-          bh[cb] = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
-          bl[cb] = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
+        for (int e = 0; e < 8; ++e) w[e] = sD[offT[q] + os[e]];
+        u4 hw, lw;
+        #pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          hw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x05040100u);
+          lw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x07060302u);
         }
-        int os[8];
-        #pragma unroll
-        for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
-        #pragma unroll
-        for (int q = 0; q < RBW; ++q) {
-          // gather 8 packed words, then regroup the low (hi) and high (lo)
-          // halves of consecutive words into the two bf16x8 operands
-          unsigned w[8];
-          #pragma unroll
-          for (int e = 0; e < 8; ++e) w[e] = sD[offT[q] + os[e]];
-          typedef unsigned u4 __attribute__((ext_vector_type(4)));
-          u4 hw, lw;
-          #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            hw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x05040100u);
-            lw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x07060302u);
-          }
-          const bf16x8 ah = __builtin_bit_cast(bf16x8, hw), al = __builtin_bit_cast(bf16x8, lw);
-          #pragma unroll
-          for (int cb = 0; cb < 4; ++cb) {
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[cb], acc[q][cb], 0, 0, 0);
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[cb], acc[q][cb], 0, 0, 0);
-            acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[cb], acc[q][cb], 0, 0, 0);
-          }
-        }
+        ah[q] = __builtin_bit_cast(bf16x8, hw);
+        al[q] = __builtin_bit_cast(bf16x8, lw);
       }
-      __syncthreads();
-      if (slot + 1 < s1) {
-        stage();
-        __syncthreads();
-      }
-    }
-    const int nrb = a.npad >> 4;
-    for (int q = 0; q < RBW; ++q) {
-      const int rb = wave + 4*q;
-      if (rb >= nrb) break;
+      #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
         const int col = cb*16 + (lane & 15);
-        const int tb = L.tile_targets[(size_t)tile*TW + col];
-        if (tb < 0) continue;
-        float *dst = split == 0
-            ? work + (size_t)(tb - L.tbase)*a.npad
-            : reinterpret_cast<float *>(L.partial) +
-                  ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
-        for (int i = 0; i < 4; ++i) dst[rb*16 + 4*(lane >> 4) + i] = acc[q][cb][i];
+        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
+        #pragma unroll
+        for (int q = 0; q < RBW; ++q) {
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh, acc[q][cb], 0, 0, 0);
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl, acc[q][cb], 0, 0, 0);
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh, acc[q][cb], 0, 0, 0);
+        }
       }
+    }
+  };
+  const int tile = li/L.nsplit, split = li - tile*L.nsplit;
+  const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
+  const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
+  const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
+  if (s0 < s1) {
+    Regs ga, gb;
+    fetch(s0, ga);
+    stage(ga);
+    __syncthreads();
+    if (s0 + 1 < s1) fetch(s0 + 1, ga);
+    if (s0 + 2 < s1) fetch(s0 + 2, gb);
+    // ga holds slot + 1 and gb slot + 2 at the top of every pair of slots
+    for (int slot = s0; slot < s1; slot += 2) {
+      contract();
+      __syncthreads();
+      if (slot + 1 >= s1) break;
+      stage(ga);
+      __syncthreads();
+      if (slot + 3 < s1) fetch(slot + 3, ga);
+      contract();
+      __syncthreads();
+      if (slot + 2 >= s1) break;
+      stage(gb);
+      __syncthreads();
+      if (slot + 4 < s1) fetch(slot + 4, gb);
+    }
+  }
+  const int nrb = a.npad >> 4;
+  for (int q = 0; q < RBW; ++q) {
+    const int rb = wave + 4*q;
+    if (rb >= nrb) break;
+    for (int cb = 0; cb < 4; ++cb) {
+      const int col = cb*16 + (lane & 15);
+      const int tb = L.tile_targets[(size_t)tile*TW + col];
+      if (tb < 0) continue;
+      float *dst = split == 0
+          ? work + (size_t)(tb - L.tbase)*a.npad
+          : reinterpret_cast<float *>(L.partial) +
+                ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
+      for (int i = 0; i < 4; ++i) dst[rb*16 + 4*(lane >> 4) + i] = acc[q][cb][i];
     }
   }
 }
@@ -710,6 +734,9 @@ extern "C" int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
   if (precision == 0)
     m2l_preprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
         nboxes, nstored, src_expansions, kpad, (double *)packed);
+  else if (precision == 2)
+    m2l_preprocess_bf16_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(
+        nboxes, nstored, src_expansions, kpad, (unsigned short *)packed);
   else
     m2l_preprocess_kernel<float><<<grid, 256, 0, (hipStream_t)stream>>>(
         nboxes, nstored, src_expansions, kpad, (float *)packed);
@@ -794,4 +821,14 @@ extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
       box_centers, nboxes, src_rscale, tgt_rscale, src_expansions, src_base_ibox,
       tgt_expansions, tgt_base_ibox);
   return check_launch("m2l_csr");
+}
+
+extern "C" int sumpyhip_m2l_pack_bf16_class_data(long long n, const double *class_data,
+                                                 void *packed, void *stream) {
+  SUMPY_REQUIRE(n >= 0, "m2l_pack_bf16_class_data: negative size");
+  if (n == 0) return SUMPYHIP_OK;
+  const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+  m2l_pack_bf16_class_data<<<grid, 256, 0, (hipStream_t)stream>>>(
+      (size_t)n, class_data, (unsigned *)packed);
+  return check_launch("m2l_pack_bf16_class_data");
 }

@@ -399,6 +399,22 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
     def __init__(self, src_expansion, tgt_expansion, name=None):
         super().__init__(src_expansion, tgt_expansion, name)
         self._schedules = {}
+        self._packed_data = {}
+
+    def packed_class_data(self, actx, data):
+        """The split-BF16 form's class data: one 32-bit word bf16(x) |
+        bf16(x - bf16(x)) << 16 per value, made once per class-data tensor
+        (and version) and cached."""
+        key = (data.data_ptr(), data.numel(), data._version)
+        out = self._packed_data.get(key)
+        if out is None:
+            out = torch.empty(data.shape, dtype=torch.int32, device=data.device)
+            _capi.call("sumpyhip_m2l_pack_bf16_class_data", int(data.numel()), _capi.ptr(data),
+                       _capi.ptr(out), actx.stream_handle)
+            if len(self._packed_data) >= 64:
+                self._packed_data.clear()
+            self._packed_data[key] = out
+        return out
 
     def schedule(self, actx, target_boxes, src_box_starts, src_box_lists, class_lists,
                  class_level_start, nclasses, centers):
@@ -466,10 +482,13 @@ def m2l_apply_levels(actx, entries):
                                   dtype=work.dtype, device=work.device)
             keep.append(partial)
         npairs += sched["npairs"]
+        data = e["data"] if tr.precision != "bf16x3" else \
+            e["op"].packed_class_data(actx, e["data"])
+        keep.append(data)
         levels.append(_capi.M2LLevel(
             sched["ntiles"], sched["nsplit"], _capi.ptr(sched["tile_targets"]),
             _capi.ptr(sched["tile_slot_starts"]), _capi.ptr(sched["slot_class"]),
-            _capi.ptr(sched["slot_sources"]), _capi.ptr(e["data"]), _capi.ptr(e["src"]),
+            _capi.ptr(sched["slot_sources"]), _capi.ptr(data), _capi.ptr(e["src"]),
             e["sbase"], _capi.ptr(work), e["tbase"], _capi.ptr(partial)))
     arr = (_capi.M2LLevel*len(levels))(*levels)
     flops = 2.0*m["nl"]*m["nm"]*npairs

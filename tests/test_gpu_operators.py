@@ -249,7 +249,7 @@ def test_m2m_l2l(actx, lap3d, small_tree, order, octant):
     assert rel(actx.to_numpy(out), ref) < 1e-11
 
 
-@pytest.mark.parametrize("order", [5, 10])
+@pytest.mark.parametrize("order", [2, 5, 10, 12])
 def test_l2p_m2p_p2l(actx, lap3d, small_tree, order):
     from sumpy_amd import E2PFromCSR, E2PFromSingleBox, P2EFromCSR
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
@@ -305,32 +305,44 @@ def test_l2p_m2p_p2l(actx, lap3d, small_tree, order):
     for r, e in zip(pot_gpu, [pot, *grad]):
         assert rel(actx.to_numpy(r), e) < 1e-11
 
-    # P2L over list 4, every level with entries
+    # P2L over list 4, every level with entries; charges, then a charge plus
+    # an x dipole and a y dipole (the dipole weights as SourceWeights forms them)
+    from sumpy_amd.kernel import AxisSourceDerivative
+    from sumpy_amd.tools import SourceWeights
     q = rng.standard_normal(th["sources"].shape[1])
+    q2 = rng.standard_normal(th["sources"].shape[1])
     totp = vh["target_or_target_parent_boxes"]
     lst = vh["level_start_target_or_target_parent_box_nrs"]
     starts, lists = vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"]
-    checked = 0
-    for lev in range(tree.nlevels):
-        s0, s1 = lst[lev], lst[lev + 1]
-        if starts[s1] == starts[s0]:
-            continue
-        nbl = lbs[lev + 1] - lbs[lev]
-        rs = _rscale(tree, lev, order)
-        out = torch.zeros((nbl, len(lc)), dtype=torch.float64, device=actx.device)
-        P2EFromCSR(lc)(actx, target_boxes=dev(actx, totp[s0:s1]),
-                       source_box_starts=dev(actx, starts[s0:s1 + 1]),
-                       source_box_lists=dev(actx, lists), centers=tree.box_centers,
-                       sources=tree.sources, strengths=[dev(actx, q)],
-                       box_source_starts=tree.box_source_starts,
-                       box_source_counts_nonchild=tree.box_source_counts_nonchild,
-                       tgt_expansions=out, tgt_base_ibox=int(lbs[lev]), rscale=rs)
-        ref = np.zeros((nbl, len(lc)))
-        oracle.p2l(order, totp[s0:s1], starts[s0:s1 + 1], lists, th, th["box_centers"],
-                   th["sources"], q, None, rs, ref, int(lbs[lev]))
-        assert rel(actx.to_numpy(out), ref) < 1e-11
-        checked += 1
-    assert checked
+    for skn in ([lap3d], [lap3d, AxisSourceDerivative(0, lap3d),
+                          AxisSourceDerivative(1, lap3d)]):
+        strengths = [dev(actx, q), dev(actx, q2), dev(actx, q)][:len(skn)]
+        sw = SourceWeights(actx, skn, list(range(len(skn))), strengths, {}, 3, False)
+        w0 = actx.to_numpy(sw.w0)
+        wd = None if sw.wd is None else [actx.to_numpy(w) for w in sw.wd]
+        assert (wd is None) == (len(skn) == 1)
+        checked = 0
+        for lev in range(tree.nlevels):
+            s0, s1 = lst[lev], lst[lev + 1]
+            if starts[s1] == starts[s0]:
+                continue
+            nbl = lbs[lev + 1] - lbs[lev]
+            rs = _rscale(tree, lev, order)
+            out = torch.zeros((nbl, len(lc)), dtype=torch.float64, device=actx.device)
+            P2EFromCSR(lc, kernels=skn, strength_usage=list(range(len(skn))))(
+                actx, target_boxes=dev(actx, totp[s0:s1]),
+                source_box_starts=dev(actx, starts[s0:s1 + 1]),
+                source_box_lists=dev(actx, lists), centers=tree.box_centers,
+                sources=tree.sources, strengths=strengths,
+                box_source_starts=tree.box_source_starts,
+                box_source_counts_nonchild=tree.box_source_counts_nonchild,
+                tgt_expansions=out, tgt_base_ibox=int(lbs[lev]), rscale=rs)
+            ref = np.zeros((nbl, len(lc)))
+            oracle.p2l(order, totp[s0:s1], starts[s0:s1 + 1], lists, th, th["box_centers"],
+                       th["sources"], w0, wd, rs, ref, int(lbs[lev]))
+            assert rel(actx.to_numpy(out), ref) < 1e-11
+            checked += 1
+        assert checked
 
 # }}}
 
