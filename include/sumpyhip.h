@@ -137,12 +137,18 @@ int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
  * T_frag holds the eight n_out x n_in FP64 matrices in MFMA A-fragment order:
  * T_frag[((o*nrb + rb)*ksteps + ks)*64 + l] = T_o[16 rb + l%16][4 ks + l/16]
  * (nrb = ceil(n_out/16), ksteps = ceil(n_in/4), zero padded).  A target box
- * may appear in one group only. */
+ * may appear in one group only.  items: the nitems values g * 8 + o of the
+ * octants present (bit o of group_octant_mask[g]), in group order.  partial:
+ * workspace of ngroups * 8 * 16 * n_out doubles, through which the octants'
+ * sums are added in octant order; NULL is allowed when every target has an
+ * input in at most one octant (L2L). */
 int sumpyhip_octant_translate(int ngroups, const int *group_targets,
                               const int *group_inputs, const int *group_octant_mask,
-                              int n_out, int n_in, const double *T_frag,
+                              int nitems, const int *items, int n_out, int n_in,
+                              const double *T_frag,
                               const double *src_expansions, int src_base_ibox,
-                              double *tgt_expansions, int tgt_base_ibox, void *stream);
+                              double *tgt_expansions, int tgt_base_ibox, void *partial,
+                              void *stream);
 
 /* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226); overwrites
  * pot (and the gradient when given) for the targets of each box. */
@@ -182,10 +188,13 @@ int sumpyhip_m2l_class_data(int kernel_id, int order_src, int order_tgt,
 
 /* Preprocessing, replaces M2LPreprocessMultipole.__call__
  * (sumpy/e2e.py:676-760): pack stored multipoles into rows of kpad entries of
- * the M2L compute precision (FP64 or FP32), zero padded. */
+ * the M2L compute precision (precision 0: FP64, 1: FP32, 2: kpad bf16 high
+ * parts then kpad bf16 low parts).  Entry k of a row holds stored coefficient
+ * k_source[k] (zero where -1); k_source NULL: entry k holds coefficient k
+ * (zero past nstored). */
 int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
-                            const double *src_expansions, int kpad, void *packed,
-                            void *stream);
+                            const double *src_expansions, int kpad, const int *k_source,
+                            void *packed, void *stream);
 
 /* One tree level of sumpyhip_m2l_apply: tile schedule (built on the host),
  * class data (nclasses x ncirc: FP64, or at precision 2 the packed words of
@@ -220,20 +229,26 @@ int sumpyhip_m2l_apply(int precision, int nlevels, const sumpyhip_m2l_level *lev
                        int nl, int nm, int npad, int kpad, const int *off_t,
                        const int *off_s, int ncirc, void *stream);
 
-/* Split-BF16 form (precision 2) support: the class data as packed words
- * bf16(x) | bf16(x - bf16(x)) << 16, the input of sumpyhip_m2l_apply at
- * precision 2 (level class_data pointers then point to these words).  No
- * reference counterpart (a layout of this build's split-BF16 contraction).
- * At precision 2, sumpyhip_m2l_preprocess writes rows of kpad hi then kpad lo
- * bf16 values (the same bytes as kpad FP32 values). */
-int sumpyhip_m2l_pack_bf16_class_data(long long n, const double *class_data, void *packed,
-                                      void *stream);
+/* Split-BF16 form (precision 2) support, no reference counterpart (a layout
+ * of this library's split-BF16 contraction): the class data of nclasses
+ * classes (circulant shape circ_shape, as written by sumpyhip_m2l_class_data)
+ * as packed words bf16(x) | bf16(x - bf16(x)) << 16, the value of multi-index
+ * (mx, my, mz) at word mx S0 + my S1 + mz of a class's ncirc_packed words
+ * (zero in the gaps).  At precision 2, sumpyhip_m2l_apply takes these words
+ * as class_data, with off_t / off_s computed with the same strides: off_t has
+ * npad entries (pass nl = npad) and off_s kpad entries (pass nm = kpad), and
+ * ncirc = ncirc_packed.  Row and k orders are free (sumpyhip_m2l_preprocess's
+ * k_source, sumpyhip_m2l_postprocess's row_pos). */
+int sumpyhip_m2l_pack_bf16_class_data(int nclasses, const int *circ_shape, int S0, int S1,
+                                      int ncirc_packed, const double *class_data,
+                                      void *packed, void *stream);
 
 /* Postprocessing, replaces M2LPostprocessLocal.__call__
- * (sumpy/e2e.py:767-859): locals[b, i] = work[b, i] * ratio^deg_i. */
+ * (sumpy/e2e.py:767-859): locals[b, i] = work[b, row_pos[i]] * ratio^deg_i
+ * (row_pos NULL: row i). */
 int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int npad,
-                             const void *work, const int *tgt_degree, double ratio,
-                             double *tgt_expansions, void *stream);
+                             const void *work, const int *tgt_degree, const int *row_pos,
+                             double ratio, double *tgt_expansions, void *stream);
 
 /* M2L without translation classes, replaces E2EFromCSR.__call__ for
  * multipole-to-local (sumpy/e2e.py:164-304); overwrites. */

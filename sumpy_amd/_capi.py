@@ -69,14 +69,15 @@ _SIGNATURES = {
                      _P, _P, _P, _P, _P],
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
-    "sumpyhip_octant_translate": [_I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P],
+    "sumpyhip_octant_translate": [_I, _P, _P, _P, _I, _P, _I, _I, _P, _P, _I, _P, _I, _P,
+                                  _P],
     "sumpyhip_qbx2d_adjoint": [_I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P,
                                _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
-    "sumpyhip_m2l_pack_bf16_class_data": [ctypes.c_longlong, _P, _P, _P],
+    "sumpyhip_m2l_pack_bf16_class_data": [_I, _P, _I, _I, _I, _P, _P, _P],
     "sumpyhip_m2l_class_data": [_I, _I, _I, _P, _I, _P, _I, _I, _D, _P, _P],
-    "sumpyhip_m2l_preprocess": [_I, _I, _I, _P, _I, _P, _P],
+    "sumpyhip_m2l_preprocess": [_I, _I, _I, _P, _I, _P, _P, _P],
     "sumpyhip_m2l_apply": [_I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _P],
-    "sumpyhip_m2l_postprocess": [_I, _I, _I, _I, _P, _P, _D, _P, _P],
+    "sumpyhip_m2l_postprocess": [_I, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "sumpyhip_m2l_csr": [_I, _I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P,
                          _I, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_axpy_weights": [_I, _I, _D, _P, _P, _P, _P],

@@ -641,37 +641,61 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
 
 // {{{ P2L (list 4), overwrite
 
-// P2L consumer: acc(m) += q0 D(m) (+ sum_ax qd_ax D(m + e_ax) for dipoles),
-// acc in shell order.
+// P2L consumer: every lane holds one source; the values q0 D(m) (+ sum_ax
+// qd_ax D(m + e_ax) for dipoles) are produced in shell order, written to the
+// wave's LDS buffer (one row of 64 lanes per coefficient) and, every 16
+// coefficients, summed over the lanes: lane l then adds the sum for
+// coefficient 16 k + l / 4 to its register acc[k].
 template <int ORD, bool DIP>
 struct P2LSum {
-  double acc[(ORD + 1)*(ORD + 1)];
+  static constexpr int NS = (ORD + 1)*(ORD + 1);
+  static constexpr int NCH = (NS + 15)/16;
+  static constexpr int BS = 64 + 1;  // LDS row stride (doubles)
+  double acc[NCH];
   double q0, qd[3];
+  double *buf;  // [16][BS]
+  int lane;
+  __device__ __forceinline__ void put(int p, double v) {
+    buf[(p & 15)*BS + lane] = v;
+    if ((p & 15) == 15 || p == NS - 1) {
+      wave_sync();
+      const double *r = buf + (lane >> 2)*BS + (lane & 3)*16;
+      double t = 0;
+      #pragma unroll
+      for (int q = 0; q < 16; ++q) t += r[q];
+      t += __shfl_xor(t, 1, 4);
+      t += __shfl_xor(t, 2, 4);
+      acc[p >> 4] += t;
+      wave_sync();
+    }
+  }
   template <int N>
   __device__ __forceinline__ void shell(const double *cur, const double *prev) {
-    if (N <= ORD) {
+    if (!DIP && N <= ORD) {
       #pragma unroll
-      for (int i = 0; i < 2*N + 1; ++i) acc[N*N + i] += q0*cur[i];
+      for (int i = 0; i < 2*N + 1; ++i) put(N*N + i, q0*cur[i]);
     }
     if (DIP && N >= 1) {
       constexpr int n = N - 1;
       #pragma unroll
       for (int mx = 0; mx <= 1; ++mx) {
         #pragma unroll
-        for (int my = 0; my <= n - mx; ++my)
-          acc[n*n + sidx(n, mx, my)] += qd[0]*shift_x<N>(cur, mx, my) +
-                                        qd[1]*cur[sidx(N, mx, my + 1)] +
-                                        qd[2]*cur[sidx(N, mx, my)];
+        for (int my = 0; my <= n - mx; ++my) {
+          const int i = sidx(n, mx, my);
+          put(n*n + i, q0*prev[i] + qd[0]*shift_x<N>(cur, mx, my) +
+                           qd[1]*cur[sidx(N, mx, my + 1)] + qd[2]*cur[sidx(N, mx, my)]);
+        }
       }
     }
   }
 };
 
-// One wave per target box (LW boxes per block): the sources of the box's
-// list-4 boxes are concatenated (box offsets from a wave scan in LDS, 64
-// boxes at a time) and spread over the lanes; every lane keeps the
-// (ORD + 1)^2 sums in registers, and the wave adds them in LDS, 16
-// coefficients at a time.
+// One wave per target box (LW boxes per block).  The sources of the box's
+// list-4 boxes are concatenated (box offsets and first sources from a wave
+// scan, 64 boxes at a time, in LDS) and taken 64 at a time, one per lane;
+// lanes past the end carry zero weights.  The lane sums are reduced through
+// LDS as the coefficients are produced (P2LSum), so a lane needs only the
+// live derivative shells and (ORD + 1)^2 / 16 accumulators.
 template <int ORD, bool DIP>
 __global__ __launch_bounds__(64*LW) void p2l_lap3d(
     sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
@@ -682,17 +706,17 @@ __global__ __launch_bounds__(64*LW) void p2l_lap3d(
     const double *__restrict__ w0, const double *__restrict__ wx,
     const double *__restrict__ wy, const double *__restrict__ wz, double rscale, double *out,
     int base) {
-  constexpr int NS = (ORD + 1)*(ORD + 1);
-  constexpr int RC = 16;  // coefficients per reduction round
+  using Sum = P2LSum<ORD, DIP>;
+  constexpr int NS = Sum::NS;
   __shared__ int sStored[NS];  // shell position -> stored index
-  __shared__ int sOff[LW][64];
-  __shared__ double red[LW][RC][64 + 1];
+  __shared__ int sOff[LW][64], sFirst[LW][64];
+  __shared__ double sBuf[LW][16*Sum::BS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int s = tid; s < NS; s += 64*LW)
     sStored[shell_pos(e.stored_ids[s], e.stored_ids[NS + s], e.stored_ids[2*NS + s])] = s;
   __syncthreads();
-  int *off = sOff[wave];
+  int *off = sOff[wave], *first = sFirst[wave];
   for (int ib = blockIdx.x*LW + wave; ib < ntgt; ib += gridDim.x*LW) {
     const int tb = target_boxes[ib];
     const int is0 = sstarts[ib], is1 = sstarts[ib + 1];
@@ -703,13 +727,16 @@ __global__ __launch_bounds__(64*LW) void p2l_lap3d(
       continue;
     }
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
-    P2LSum<ORD, DIP> f;
+    Sum f;
+    f.buf = sBuf[wave];
+    f.lane = lane;
     #pragma unroll
-    for (int m = 0; m < NS; ++m) f.acc[m] = 0.0;
+    for (int k = 0; k < Sum::NCH; ++k) f.acc[k] = 0.0;
     for (int cb = is0; cb < is1; cb += 64) {
       const int nb = min(64, is1 - cb);
       // exclusive scan of the source counts of boxes cb .. cb + nb - 1
-      const int c = lane < nb ? bsc[slists[cb + lane]] : 0;
+      const int sb = lane < nb ? slists[cb + lane] : 0;
+      const int c = lane < nb ? bsc[sb] : 0;
       int v = c;
       for (int d = 1; d < 64; d <<= 1) {
         const int u = __shfl_up(v, d, 64);
@@ -717,37 +744,31 @@ __global__ __launch_bounds__(64*LW) void p2l_lap3d(
       }
       const int total = __shfl(v, 63, 64);
       wave_sync();
-      if (lane < nb) off[lane] = v - c;
+      if (lane < nb) { off[lane] = v - c; first[lane] = bss[sb]; }
       wave_sync();
-      for (int jj = lane; jj < total; jj += 64) {
+      for (int j0 = 0; j0 < total; j0 += 64) {
+        const int jj = min(j0 + lane, total - 1);
         int lo = 0, hi = nb - 1;  // largest b with off[b] <= jj
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
           if (off[mid] <= jj) lo = mid; else hi = mid - 1;
         }
-        const int s = bss[slists[cb + lo]] + (jj - off[lo]);
-        f.q0 = w0[s];
+        const int s = first[lo] + (jj - off[lo]);
+        const bool live = j0 + lane < total;
+        f.q0 = live ? w0[s] : 0.0;
         if (DIP) {
-          f.qd[0] = wx[s]/rscale; f.qd[1] = wy[s]/rscale; f.qd[2] = wz[s]/rscale;
+          f.qd[0] = live ? wx[s]/rscale : 0.0;
+          f.qd[1] = live ? wy[s]/rscale : 0.0;
+          f.qd[2] = live ? wz[s]/rscale : 0.0;
         }
         const LapGeo g(c0 - sx[s], c1 - sy[s], c2 - sz[s], rscale);
         lap3d_shells<ORD + (DIP ? 1 : 0)>(g, f);
       }
     }
     #pragma unroll
-    for (int m0 = 0; m0 < NS; m0 += RC) {
-      wave_sync();
-      #pragma unroll
-      for (int m = 0; m < RC; ++m)
-        if (m0 + m < NS) red[wave][m][lane] = f.acc[m0 + m];
-      wave_sync();
-      // lane: coefficient m0 + lane / 4, part lane % 4 of the 64 lane sums
-      const int m = lane >> 2, part = lane & 3;
-      double v = 0;
-      for (int q = part; q < 64; q += 4) v += red[wave][m][q];
-      v += __shfl_xor(v, 1, 4);
-      v += __shfl_xor(v, 2, 4);
-      if (part == 0 && m0 + m < NS) o[sStored[m0 + m]] = v;
+    for (int k = 0; k < Sum::NCH; ++k) {
+      const int m = 16*k + (lane >> 2);
+      if ((lane & 3) == 0 && m < NS) o[sStored[m]] = f.acc[k];
     }
   }
 }

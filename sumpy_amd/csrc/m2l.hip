@@ -97,27 +97,40 @@ __global__ __launch_bounds__(256) void m2l_class_data_kernel(
   }
 }
 
+// packed row entry k holds stored coefficient ksrc[k] (-1: zero; ksrc null:
+// entry k holds coefficient k)
+__device__ __forceinline__ int k_source_of(const int *ksrc, int k, int nstored) {
+  return ksrc ? ksrc[k] : (k < nstored ? k : -1);
+}
+
 template <typename T>
 __global__ void m2l_preprocess_kernel(int nboxes, int nstored, const double *src, int kpad,
-                                      T *packed) {
+                                      const int *ksrc, T *packed) {
   const size_t n = (size_t)nboxes*kpad;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
     const size_t b = i/kpad;
-    const int k = i % kpad;
-    packed[i] = k < nstored ? T(src[b*nstored + k]) : T(0);
+    const int k = k_source_of(ksrc, i % kpad, nstored);
+    packed[i] = k >= 0 ? T(src[b*nstored + k]) : T(0);
   }
 }
 
+constexpr int MAXDEG = 64;  // ratio^d table of the postprocessing (d = |mi| <= 2 order)
+
+// locals[b, i] = work[b, pos[i]] ratio^deg[i]  (pos null: pos[i] = i)
 template <typename T>
-__global__ void m2l_postprocess_kernel(int nboxes, int nl, int npad, const T *work,
-                                       const int *deg, double ratio, double *out) {
+__global__ __launch_bounds__(256) void m2l_postprocess_kernel(
+    int nboxes, int nl, int npad, const T *work, const int *deg, const int *pos, double ratio,
+    double *out) {
+  __shared__ double pw[MAXDEG];
+  if (threadIdx.x < MAXDEG) pw[threadIdx.x] = pow(ratio, (double)threadIdx.x);
+  __syncthreads();
   const size_t n = (size_t)nboxes*nl;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
     const size_t b = i/nl;
     const int k = i % nl;
-    out[i] = double(work[b*npad + k])*pow(ratio, (double)deg[k]);
+    out[i] = double(work[b*npad + (pos ? pos[k] : k)])*pw[deg[k]];
   }
 }
 
@@ -389,26 +402,35 @@ __device__ __forceinline__ unsigned bf16_bits(__bf16 h) {
   return (unsigned)__builtin_bit_cast(unsigned short, h);
 }
 
-// class data -> packed words hi | lo << 16 (bf16 bit patterns), once per launch
-__global__ void m2l_pack_bf16_class_data(size_t n, const double *data, unsigned *out) {
+// class data (circulant shape M0 x M1 x M2) -> packed words hi | lo << 16
+// (bf16 bit patterns) at mx S0 + my S1 + mz, zero in the gaps
+__global__ void m2l_pack_bf16_class_data(int nclasses, int M0, int M1, int M2, int S0, int S1,
+                                         int ncirc_p, const double *data, unsigned *out) {
+  const size_t n = (size_t)nclasses*ncirc_p, ncirc = (size_t)M0*M1*M2;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
+    const size_t c = i/ncirc_p;
+    const int w = i - c*ncirc_p;
+    const int mx = w/S0, r = w - mx*S0, my = r/S1, mz = r - my*S1;
+    double v = 0.0;
+    if (mx < M0 && my < M1 && mz < M2) v = data[c*ncirc + ((size_t)mx*M1 + my)*M2 + mz];
     __bf16 h, l;
-    split_bf16(float(data[i]), h, l);
+    split_bf16(float(v), h, l);
     out[i] = bf16_bits(h) | (bf16_bits(l) << 16);
   }
 }
 
-// stored multipoles -> rows of kpad hi then kpad lo bf16 (zero padded)
+// stored multipoles -> rows of kpad hi then kpad lo bf16 (entry k holds
+// coefficient ksrc[k], zero when -1)
 __global__ void m2l_preprocess_bf16_kernel(int nboxes, int nstored, const double *src, int kpad,
-                                           unsigned short *packed) {
+                                           const int *ksrc, unsigned short *packed) {
   const size_t n = (size_t)nboxes*kpad;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
     const size_t b = i/kpad;
-    const int k = i % kpad;
+    const int k = i % kpad, ks = k_source_of(ksrc, k, nstored);
     __bf16 h, l;
-    split_bf16(k < nstored ? float(src[b*nstored + k]) : 0.f, h, l);
+    split_bf16(ks >= 0 ? float(src[b*nstored + ks]) : 0.f, h, l);
     packed[b*2*kpad + k] = (unsigned short)bf16_bits(h);
     packed[b*2*kpad + kpad + k] = (unsigned short)bf16_bits(l);
   }
@@ -425,9 +447,11 @@ __global__ void m2l_preprocess_bf16_kernel(int nboxes, int nstored, const double
 // v_mfma_f32_16x16x32_bf16: A row = lane % 16, B column = lane % 16, the 8 k
 // of a lane are 8 (lane / 16) + e in both; accumulator register r = row
 // 4 (lane / 16) + r.
+constexpr int BF16_MAXCIRC = 2560;  // packed class words per slot (LDS)
+
 template <int RBW>
 __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
-  constexpr int DPT = Limits<RBW>::DPT;
+  constexpr int DPT = (BF16_MAXCIRC + 255)/256;
   extern __shared__ unsigned char lds_raw[];
   unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
   __bf16 *sXh = reinterpret_cast<__bf16 *>(sD + ((a.ncirc + 3) & ~3));  // [64][XB]
@@ -457,17 +481,28 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   }
   // one slot's staging data: DPT class words and, for column `lane`, the hi
   // and lo 8-chunks m = wave + 4 i (i < 4)
-  struct Regs {
-    unsigned d[DPT];
+  // one slot's multipole columns: for column `lane`, the hi and lo 8-chunks
+  // m = wave + 4 i (i < 4)
+  struct XRegs {
     u4 xh[4], xl[4];
   };
-  auto fetch = [&](int slot, Regs &g) {
-    const unsigned *D = cwords + (size_t)L.slot_class[slot]*a.ncirc;
+  // slot range of this block (computed here so that loads can be clamped to
+  // it: every fetch is unconditional, which keeps the memory-counter waits
+  // exact)
+  const int tile = li/L.nsplit, split = li - tile*L.nsplit;
+  const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
+  const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
+  const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
+  // source box of this lane's column in a slot (loaded two fetches ahead)
+  auto meta = [&](int slot) { return L.slot_sources[(size_t)min(slot, s1 - 1)*TW + lane]; };
+  auto fetch_d = [&](int slot, unsigned (&d)[DPT]) {
+    const unsigned *D = cwords + (size_t)L.slot_class[min(slot, s1 - 1)]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      g.d[i] = D[e < a.ncirc ? e : 0];
+      d[i] = D[e < a.ncirc ? e : 0];
     }
-    const int sb = L.slot_sources[(size_t)slot*TW + lane];
+  };
+  auto fetch_x = [&](int sb, XRegs &g) {
     const unsigned short *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*2*a.kpad;
     for (int i = 0; i < 4; ++i) {
       const int m = wave + 4*i, mm = m < kq ? m : 0;
@@ -477,10 +512,10 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     if (sb < 0)
       for (int i = 0; i < 4; ++i) g.xh[i] = g.xl[i] = u4{0, 0, 0, 0};
   };
-  auto stage = [&](const Regs &g) {
+  auto stage = [&](const unsigned (&d)[DPT], const XRegs &g) {
     for (int i = 0; i < DPT; ++i) {
       const int e = tid + 256*i;
-      if (e < a.ncirc) sD[e] = g.d[i];
+      if (e < a.ncirc) sD[e] = d[i];
     }
     for (int i = 0; i < 4; ++i) {
       const int m = wave + 4*i;
@@ -496,6 +531,8 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
   const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
   auto contract = [&]() {
+    // (the run-time bound keeps the compiler from hoisting later steps'
+    // fragments, which would cost a wave per SIMD)
     #pragma unroll
     for (int k0 = 0; k0 < 128; k0 += KB) {
       if (k0 >= a.kpad) break;
@@ -531,31 +568,44 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
       }
     }
   };
-  const int tile = li/L.nsplit, split = li - tile*L.nsplit;
-  const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
-  const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
-  const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
   if (s0 < s1) {
-    Regs ga, gb;
-    fetch(s0, ga);
-    stage(ga);
+    // the multipole columns (from HBM) are fetched two slots ahead, the class
+    // words (few, shared by the tiles, from L2) one slot ahead; each slot's
+    // class words are issued before the columns of the slot after next, so
+    // that waiting for them leaves those columns in flight
+    unsigned dn[DPT];
+    XRegs ga, gb;
+    fetch_d(s0, dn);
+    fetch_x(meta(s0), ga);
+    stage(dn, ga);
     __syncthreads();
-    if (s0 + 1 < s1) fetch(s0 + 1, ga);
-    if (s0 + 2 < s1) fetch(s0 + 2, gb);
-    // ga holds slot + 1 and gb slot + 2 at the top of every pair of slots
+    const int m1 = meta(s0 + 1), m2 = meta(s0 + 2);
+    fetch_d(s0 + 1, dn);
+    fetch_x(m1, ga);
+    int m3 = meta(s0 + 3);
+    fetch_x(m2, gb);
+    int m4 = meta(s0 + 4);
+    // at the top of every pair of slots: ga holds the columns of slot + 1, gb
+    // those of slot + 2, dn the class words of slot + 1, and m3 / m4 the
+    // source boxes of slots slot + 3 / slot + 4 (fetches past the range
+    // reload the last slot and are not used)
     for (int slot = s0; slot < s1; slot += 2) {
       contract();
       __syncthreads();
       if (slot + 1 >= s1) break;
-      stage(ga);
+      stage(dn, ga);
       __syncthreads();
-      if (slot + 3 < s1) fetch(slot + 3, ga);
+      fetch_d(slot + 2, dn);
+      fetch_x(m3, ga);
+      m3 = meta(slot + 5);
       contract();
       __syncthreads();
       if (slot + 2 >= s1) break;
-      stage(gb);
+      stage(dn, gb);
       __syncthreads();
-      if (slot + 4 < s1) fetch(slot + 4, gb);
+      fetch_d(slot + 3, dn);
+      fetch_x(m4, gb);
+      m4 = meta(slot + 6);
     }
   }
   const int nrb = a.npad >> 4;
@@ -594,8 +644,8 @@ int launch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
 int dispatch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
   const int nrb = a.npad/16;
   int rc;
-  if (nrb <= 4 && a.ncirc <= Limits<1>::MAXCIRC) rc = launch_apply_bf16<1>(a, st);
-  else if (nrb <= 8 && a.ncirc <= Limits<2>::MAXCIRC) rc = launch_apply_bf16<2>(a, st);
+  if (nrb <= 4 && a.ncirc <= BF16_MAXCIRC) rc = launch_apply_bf16<1>(a, st);
+  else if (nrb <= 8 && a.ncirc <= BF16_MAXCIRC) rc = launch_apply_bf16<2>(a, st);
   else
     return set_error(SUMPYHIP_ENOTIMPL,
                      "m2l_apply (bf16x3): npad %d / ncirc %d exceed the supported sizes",
@@ -724,8 +774,8 @@ extern "C" int sumpyhip_m2l_class_data(int kernel_id, int order_src, int order_t
 }
 
 extern "C" int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
-                                       const double *src_expansions, int kpad, void *packed,
-                                       void *stream) {
+                                       const double *src_expansions, int kpad,
+                                       const int *k_source, void *packed, void *stream) {
   SUMPY_REQUIRE(kpad >= nstored, "m2l_preprocess: kpad < nstored");
   if (nboxes == 0) return SUMPYHIP_OK;
   const size_t n = (size_t)nboxes*kpad;
@@ -733,20 +783,20 @@ extern "C" int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
   SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_preprocess: unknown precision");
   if (precision == 0)
     m2l_preprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
-        nboxes, nstored, src_expansions, kpad, (double *)packed);
+        nboxes, nstored, src_expansions, kpad, k_source, (double *)packed);
   else if (precision == 2)
     m2l_preprocess_bf16_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(
-        nboxes, nstored, src_expansions, kpad, (unsigned short *)packed);
+        nboxes, nstored, src_expansions, kpad, k_source, (unsigned short *)packed);
   else
     m2l_preprocess_kernel<float><<<grid, 256, 0, (hipStream_t)stream>>>(
-        nboxes, nstored, src_expansions, kpad, (float *)packed);
+        nboxes, nstored, src_expansions, kpad, k_source, (float *)packed);
   return check_launch("m2l_preprocess");
 }
 
 extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int npad,
                                         const void *work, const int *tgt_degree,
-                                        double ratio, double *tgt_expansions,
-                                        void *stream) {
+                                        const int *row_pos, double ratio,
+                                        double *tgt_expansions, void *stream) {
   SUMPY_REQUIRE(npad >= nl, "m2l_postprocess: npad < nl");
   if (nboxes == 0) return SUMPYHIP_OK;
   const size_t n = (size_t)nboxes*nl;
@@ -754,10 +804,10 @@ extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int n
   SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_postprocess: unknown precision");
   if (precision == 0)
     m2l_postprocess_kernel<double><<<grid, 256, 0, (hipStream_t)stream>>>(
-        nboxes, nl, npad, (const double *)work, tgt_degree, ratio, tgt_expansions);
+        nboxes, nl, npad, (const double *)work, tgt_degree, row_pos, ratio, tgt_expansions);
   else
     m2l_postprocess_kernel<float><<<grid, 256, 0, (hipStream_t)stream>>>(
-        nboxes, nl, npad, (const float *)work, tgt_degree, ratio, tgt_expansions);
+        nboxes, nl, npad, (const float *)work, tgt_degree, row_pos, ratio, tgt_expansions);
   return check_launch("m2l_postprocess");
 }
 
@@ -823,12 +873,20 @@ extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,
   return check_launch("m2l_csr");
 }
 
-extern "C" int sumpyhip_m2l_pack_bf16_class_data(long long n, const double *class_data,
-                                                 void *packed, void *stream) {
-  SUMPY_REQUIRE(n >= 0, "m2l_pack_bf16_class_data: negative size");
-  if (n == 0) return SUMPYHIP_OK;
+extern "C" int sumpyhip_m2l_pack_bf16_class_data(int nclasses, const int *circ_shape, int S0,
+                                                 int S1, int ncirc_packed,
+                                                 const double *class_data, void *packed,
+                                                 void *stream) {
+  SUMPY_REQUIRE(nclasses >= 0 && circ_shape, "m2l_pack_bf16_class_data: bad arguments");
+  const int M0 = circ_shape[0], M1 = circ_shape[1], M2 = circ_shape[2];
+  SUMPY_REQUIRE(M0 > 0 && M1 > 0 && M2 > 0 && S1 >= M2 && S0 >= (M1 - 1)*S1 + M2 &&
+                    ncirc_packed >= (M0 - 1)*S0 + (M1 - 1)*S1 + M2,
+                "m2l_pack_bf16_class_data: strides (%d, %d) / size %d do not hold shape "
+                "(%d, %d, %d)", S0, S1, ncirc_packed, M0, M1, M2);
+  if (nclasses == 0) return SUMPYHIP_OK;
+  const size_t n = (size_t)nclasses*ncirc_packed;
   const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
   m2l_pack_bf16_class_data<<<grid, 256, 0, (hipStream_t)stream>>>(
-      (size_t)n, class_data, (unsigned *)packed);
+      nclasses, M0, M1, M2, S0, S1, ncirc_packed, class_data, (unsigned *)packed);
   return check_launch("m2l_pack_bf16_class_data");
 }

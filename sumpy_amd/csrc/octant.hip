@@ -106,6 +106,87 @@ __global__ __launch_bounds__(64*NWAVE) void octant_translate_kernel(OctArgs a) {
   }
 }
 
+// Split form for n_out <= 128 (one row block per wave) and n_in <= 128: one
+// block per (group, octant), so the octants of a group run side by side.  A
+// block loads its whole A fragments (one row block per wave) into registers
+// while it stages the octant's 16 inputs (zero padded to 128 coefficients),
+// then runs 32 MFMA k steps.  With a partial buffer the sums are written per
+// octant and added to the targets by octant_reduce in octant order (M2M: a
+// parent has inputs in several octants); without one, each target column
+// with an input in the block's octant is added to directly (L2L: a child has
+// its parent in exactly one octant).
+constexpr int KS = 32;                           // k steps (n_in <= 128)
+constexpr int SPT = (GW*KS*4 + 64*NWAVE - 1)/(64*NWAVE);  // staged inputs per thread
+
+__global__ __launch_bounds__(64*NWAVE) void octant_translate_split(OctArgs a, const int *items,
+                                                                   double *partial) {
+  extern __shared__ double sB[];  // [KS*4][BS]
+  const int item = items[blockIdx.x];
+  const int g = item >> 3, o = item & 7;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool active = wave < a.nrb;
+  const int rb = active ? wave : 0;
+  const int *inputs = a.grp_inputs + ((size_t)g*8 + o)*GW;
+  double A[KS];
+  const double *Tw = a.T + ((size_t)o*a.nrb + rb)*a.ksteps*64 + lane;
+  #pragma unroll
+  for (int ks = 0; ks < KS; ++ks) A[ks] = Tw[(size_t)min(ks, a.ksteps - 1)*64];
+  #pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    const int idx = tid + 64*NWAVE*i;
+    const int j = idx/(KS*4), k = idx - j*(KS*4);
+    if (j < GW) {
+      const int box = inputs[j];
+      sB[k*BS + j] = (box >= 0 && k < a.n_in) ? a.src[(size_t)(box - a.sbase)*a.n_in + k] : 0.0;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  v4d acc = v4d{0, 0, 0, 0};
+  #pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const double b = sB[(ks*4 + (lane >> 4))*BS + (lane & 15)];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[ks], b, acc, 0, 0, 0);
+  }
+  // register i of lane l: row (l/16) + 4i of column l%16
+  const int col = lane & 15;
+  if (partial) {
+    double *dst = partial + (((size_t)g*8 + o)*GW + col)*a.n_out;
+    for (int i = 0; i < 4; ++i) {
+      const int row = rb*16 + (lane >> 4) + 4*i;
+      if (row < a.n_out) dst[row] = acc[i];
+    }
+  } else {
+    const int tb = a.grp_targets[(size_t)g*GW + col];
+    if (tb >= 0 && inputs[col] >= 0) {
+      double *dst = a.tgt + (size_t)(tb - a.tbase)*a.n_out;
+      for (int i = 0; i < 4; ++i) {
+        const int row = rb*16 + (lane >> 4) + 4*i;
+        if (row < a.n_out) dst[row] += acc[i];
+      }
+    }
+  }
+}
+
+// tgt[target] += sum over the group's octants (in order) of the partial sums
+__global__ __launch_bounds__(256) void octant_reduce(OctArgs a, const double *partial) {
+  const size_t n = (size_t)a.ngroups*GW*a.n_out;
+  for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x*blockDim.x) {
+    const size_t gj = i/a.n_out;
+    const int row = i - gj*a.n_out;
+    const int g = gj/GW, j = gj - (size_t)g*GW;
+    const int tb = a.grp_targets[gj];
+    if (tb < 0) continue;
+    const int mask = a.grp_mask[g];
+    double s = 0;
+    for (int o = 0; o < 8; ++o)
+      if ((mask >> o) & 1) s += partial[(((size_t)g*8 + o)*GW + j)*a.n_out + row];
+    a.tgt[(size_t)(tb - a.tbase)*a.n_out + row] += s;
+  }
+}
+
 template <int RBW>
 int launch_octant(const OctArgs &a, hipStream_t st) {
   const size_t lds = sizeof(double)*(size_t)a.ksteps*4*BS;
@@ -128,10 +209,11 @@ using namespace sumpyhip;
 
 extern "C" int sumpyhip_octant_translate(int ngroups, const int *group_targets,
                                          const int *group_inputs, const int *group_octant_mask,
-                                         int n_out, int n_in, const double *T_frag,
+                                         int nitems, const int *items, int n_out, int n_in,
+                                         const double *T_frag,
                                          const double *src_expansions, int src_base_ibox,
                                          double *tgt_expansions, int tgt_base_ibox,
-                                         void *stream) {
+                                         void *partial, void *stream) {
   SUMPY_REQUIRE(ngroups >= 0 && n_out > 0 && n_in > 0, "octant_translate: bad sizes");
   SUMPY_REQUIRE(n_in <= 1024, "octant_translate: n_in %d > 1024", n_in);
   if (ngroups == 0) return SUMPYHIP_OK;
@@ -143,6 +225,20 @@ extern "C" int sumpyhip_octant_translate(int ngroups, const int *group_targets,
             tgt_expansions, tgt_base_ibox};
   hipStream_t st = (hipStream_t)stream;
   const int rbw = (a.nrb + NWAVE - 1)/NWAVE;
+  if (rbw == 1 && a.ksteps <= KS) {
+    SUMPY_REQUIRE(items && nitems >= 0 && nitems <= 8*ngroups,
+                  "octant_translate: %d items for %d groups", nitems, ngroups);
+    if (nitems == 0) return SUMPYHIP_OK;
+    const size_t lds = sizeof(double)*(size_t)KS*4*BS;
+    double *part = static_cast<double *>(partial);
+    octant_translate_split<<<nitems, 64*NWAVE, lds, st>>>(a, items, part);
+    int rc = check_launch("octant_translate");
+    if (rc || !part) return rc;
+    const size_t n = (size_t)ngroups*GW*n_out;
+    const int grid = (int)((n + 255)/256 < 65536 ? (n + 255)/256 : 65536);
+    octant_reduce<<<grid, 256, 0, st>>>(a, part);
+    return check_launch("octant_reduce");
+  }
   switch (rbw) {
     case 1: return launch_octant<1>(a, st);
     case 2: return launch_octant<2>(a, st);

@@ -80,7 +80,9 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     mpole_result = wrangler.eval_multipoles(
         actx, trav.target_boxes_sep_smaller_by_source_level,
         trav.from_sep_smaller_by_level, mpole_exps)
-    potentials = _add(potentials, mpole_result)
+    # (a wrangler that knows its list-3 result is all zeros says so)
+    if not getattr(wrangler, "last_eval_multipoles_was_empty", False):
+        potentials = _add(potentials, mpole_result)
 
     if getattr(trav, "from_sep_close_smaller_starts", None) is not None:
         potentials = _add(potentials, wrangler.eval_direct(

@@ -106,12 +106,17 @@ class _OctantTranslation(E2EBase):
     def _launch_octant(self, actx, plan, frag, src, sbase, tgt, tbase):
         st, tt = self._tables(actx)
         flops = 2.0*st.table.nstored*tt.table.nstored*plan["npairs"]
+        partial = None
+        if self.octant_inputs_per_target > 1:
+            partial = torch.empty(plan["ngroups"]*8*GROUP*tt.table.nstored,
+                                  dtype=torch.float64, device=tgt.device)
         with kernel_span(self.span_name, flops):
             _capi.call("sumpyhip_octant_translate", plan["ngroups"],
                        _capi.ptr(plan["targets"]), _capi.ptr(plan["inputs"]),
-                       _capi.ptr(plan["mask"]), tt.table.nstored, st.table.nstored, _capi.ptr(frag),
-                       _capi.ptr(src), int(sbase), _capi.ptr(tgt), int(tbase),
-                       actx.stream_handle)
+                       _capi.ptr(plan["mask"]), int(plan["items"].shape[0]),
+                       _capi.ptr(plan["items"]), tt.table.nstored, st.table.nstored,
+                       _capi.ptr(frag), _capi.ptr(src), int(sbase), _capi.ptr(tgt), int(tbase),
+                       _capi.ptr(partial), actx.stream_handle)
         return tgt
 
 
@@ -143,6 +148,12 @@ def _common_offsets(d, labels, rel_tol=1e-10):
     return offsets, present
 
 
+def _octant_items(mask):
+    """g * 8 + o for every octant o present in group g, in group order."""
+    g, o = np.nonzero((np.asarray(mask)[:, None] >> np.arange(8)[None, :]) & 1)
+    return (g*8 + o).astype(np.int32)
+
+
 def _groups(actx, targets, inputs, labels):
     """Groups of GROUP targets sharing one label each (inputs[i] feeds target
     i through octant labels[i]); M2M passes all eight octants per target."""
@@ -156,6 +167,7 @@ class E2EFromChildren(_OctantTranslation):
 
     default_name = "e2e_from_children"
     span_name = "m2m"
+    octant_inputs_per_target = 8  # a parent sums its children's octants
 
     @staticmethod
     def _child_offset(d):
@@ -214,7 +226,8 @@ class E2EFromChildren(_OctantTranslation):
                 if "dev" not in plan:
                     plan["dev"] = dict(zip(("targets", "inputs"), _groups(
                         actx, plan["targets"], plan["inputs"], None)),
-                        mask=actx.from_numpy(plan["mask"].astype(np.int32)))
+                        mask=actx.from_numpy(plan["mask"].astype(np.int32)),
+                        items=actx.from_numpy(_octant_items(plan["mask"])))
                 frag = self._matrices(actx, plan["offsets"], plan["present"],
                                       kwargs["src_rscale"], kwargs["tgt_rscale"])
                 return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
@@ -236,6 +249,7 @@ class E2EFromParent(_OctantTranslation):
 
     default_name = "e2e_from_parent"
     span_name = "l2l"
+    octant_inputs_per_target = 1  # a child has its parent in one octant
 
     @staticmethod
     def _child_offset(d):
@@ -298,7 +312,8 @@ class E2EFromParent(_OctantTranslation):
                 if "dev" not in plan:
                     plan["dev"] = dict(zip(("targets", "inputs"), _groups(
                         actx, plan["targets"], plan["inputs"], None)),
-                        mask=actx.from_numpy(plan["mask"].astype(np.int32)))
+                        mask=actx.from_numpy(plan["mask"].astype(np.int32)),
+                        items=actx.from_numpy(_octant_items(plan["mask"])))
                 frag = self._matrices(actx, plan["offsets"], plan["present"],
                                       kwargs["src_rscale"], kwargs["tgt_rscale"])
                 return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
@@ -318,6 +333,47 @@ class E2EFromParent(_OctantTranslation):
 def _m2l_tabs(src_expansion, tgt_expansion, device):
     return m2l_device_tables(src_expansion.order, tgt_expansion.order,
                              src_expansion.compressed, str(device))
+
+
+_BF16_TABS = {}
+
+
+def _bf16_tabs(src_expansion, tgt_expansion, device):
+    """Device tables of the split-BF16 contraction's LDS layout: the tuned
+    layout of :mod:`sumpy_amd.m2l_layout` for equal compressed orders, else
+    the plain circulant layout (stored row and k order)."""
+    key = (src_expansion.order, tgt_expansion.order, src_expansion.compressed, str(device))
+    tabs = _BF16_TABS.get(key)
+    if tabs is not None:
+        return tabs
+    from sumpy_amd import m2l_layout
+    m = _m2l_tabs(src_expansion, tgt_expansion, device)
+    nl, nm = m["nl"], m["nm"]
+    npad, kpad = -(-nl//16)*16, -(-nm//32)*32
+    shape = tuple(int(x) for x in m["shape"])
+    if src_expansion.order == tgt_expansion.order and src_expansion.compressed:
+        lay = m2l_layout.layout(src_expansion.order)
+        off_t, off_s, ks = lay.off_t, lay.off_s, lay.ks
+        row_pos, strides, ncirc = lay.row_pos, lay.strides, lay.ncirc
+    else:
+        off_t = np.zeros(npad, np.int32)
+        off_t[:nl] = m["off_t"].cpu().numpy()
+        off_s = np.zeros(kpad, np.int32)
+        off_s[:nm] = m["off_s"].cpu().numpy()
+        ks = np.where(np.arange(kpad) < nm, np.arange(kpad), -1)
+        row_pos = np.arange(nl)
+        strides, ncirc = (shape[1]*shape[2], shape[2]), m["ncirc"]
+    if len(off_t) != npad or len(off_s) != kpad:
+        raise ValueError("split-BF16 layout does not match the expansion sizes")
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(device)
+
+    tabs = dict(off_t=dev(off_t), off_s=dev(off_s), k_src=dev(ks), row_pos=dev(row_pos),
+                strides=tuple(int(x) for x in strides), ncirc=int(ncirc), npad=npad,
+                kpad=kpad, shape=_capi.np_i32(shape))
+    _BF16_TABS[key] = tabs
+    return tabs
 
 
 class E2EFromCSR(E2EBase):
@@ -385,8 +441,14 @@ class M2LPreprocessMultipole(E2EBase):
         nb = int(src.shape[0])
         if out.shape[0] != nb:
             raise ValueError("preprocessed array has the wrong number of boxes")
+        k_src = None
+        if tr.precision == "bf16x3":
+            k_src = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)["k_src"]
+            if int(k_src.shape[0]) != int(out.shape[1]):
+                raise ValueError("preprocessed rows do not match the split-BF16 layout")
         _capi.call("sumpyhip_m2l_preprocess", tr.precision_id, nb, int(src.shape[1]),
-                   _capi.ptr(src), int(out.shape[1]), _capi.ptr(out), actx.stream_handle)
+                   _capi.ptr(src), int(out.shape[1]), _capi.ptr(k_src), _capi.ptr(out),
+                   actx.stream_handle)
         return out
 
 
@@ -403,14 +465,17 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
 
     def packed_class_data(self, actx, data):
         """The split-BF16 form's class data: one 32-bit word bf16(x) |
-        bf16(x - bf16(x)) << 16 per value, made once per class-data tensor
-        (and version) and cached."""
+        bf16(x - bf16(x)) << 16 per value, at the strides of the LDS layout,
+        made once per class-data tensor (and version) and cached."""
         key = (data.data_ptr(), data.numel(), data._version)
         out = self._packed_data.get(key)
         if out is None:
-            out = torch.empty(data.shape, dtype=torch.int32, device=data.device)
-            _capi.call("sumpyhip_m2l_pack_bf16_class_data", int(data.numel()), _capi.ptr(data),
-                       _capi.ptr(out), actx.stream_handle)
+            tabs = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)
+            ncls = int(data.shape[0])
+            out = torch.empty((ncls, tabs["ncirc"]), dtype=torch.int32, device=data.device)
+            _capi.call("sumpyhip_m2l_pack_bf16_class_data", ncls, tabs["shape"].ctypes.data,
+                       *tabs["strides"], tabs["ncirc"], _capi.ptr(data), _capi.ptr(out),
+                       actx.stream_handle)
             if len(self._packed_data) >= 64:
                 self._packed_data.clear()
             self._packed_data[key] = out
@@ -492,10 +557,18 @@ def m2l_apply_levels(actx, entries):
             e["sbase"], _capi.ptr(work), e["tbase"], _capi.ptr(partial)))
     arr = (_capi.M2LLevel*len(levels))(*levels)
     flops = 2.0*m["nl"]*m["nm"]*npairs
+    if tr.precision == "bf16x3":
+        tabs = _bf16_tabs(op.src_expansion, op.tgt_expansion, actx.device)
+        if (tabs["npad"], tabs["kpad"]) != (npad, kpad):
+            raise ValueError("m2l_apply_levels: padding differs from the split-BF16 layout")
+        geom = (npad, kpad, npad, kpad, _capi.ptr(tabs["off_t"]), _capi.ptr(tabs["off_s"]),
+                tabs["ncirc"])
+    else:
+        geom = (m["nl"], m["nm"], npad, kpad, _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]),
+                m["ncirc"])
     with kernel_span(f"m2l_apply_{tr.precision}", flops):
         _capi.call("sumpyhip_m2l_apply", tr.precision_id, len(levels), ctypes.addressof(arr),
-                   m["nl"], m["nm"], npad, kpad, _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]),
-                   m["ncirc"], actx.stream_handle)
+                   *geom, actx.stream_handle)
 
 
 def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_start,
@@ -583,9 +656,13 @@ class M2LPostprocessLocal(E2EBase):
         tgt = kwargs["tgt_expansions"]
         work = kwargs["tgt_expansions_before_postprocessing"]
         ratio = float(kwargs["tgt_rscale"])/float(kwargs["src_rscale"])
+        row_pos = None
+        if tr.precision == "bf16x3":
+            row_pos = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)["row_pos"]
         _capi.call("sumpyhip_m2l_postprocess", tr.precision_id, int(tgt.shape[0]),
                    int(tgt.shape[1]), int(work.shape[1]), _capi.ptr(work),
-                   _capi.ptr(m["tgt_degree"]), ratio, _capi.ptr(tgt), actx.stream_handle)
+                   _capi.ptr(m["tgt_degree"]), _capi.ptr(row_pos), ratio, _capi.ptr(tgt),
+                   actx.stream_handle)
         return tgt
 
 

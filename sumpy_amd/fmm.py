@@ -287,24 +287,37 @@ class SumpyExpansionWrangler:
             lambda tr, loc, mp: tr.preprocess_multipole_nexprs(loc, mp))
 
     def m2l_preproc_mpole_expansion_zeros(self, actx, template_ary):
+        # preprocessing writes every entry of every box: no clearing needed
         starts = self.m2l_preproc_mpole_expansions_level_starts()
-        return self._per_level_zeros(starts)
+        return self._per_level(starts, clear=False)
 
     def m2l_work_array_level_starts(self):
         return self._m2l_level_sizes(
             lambda tr, loc, mp: tr.postprocess_local_nexprs(loc, mp))
 
     def m2l_work_array_zeros(self, actx, template_ary):
-        return self._per_level_zeros(self.m2l_work_array_level_starts())
+        return self._per_level(self.m2l_work_array_level_starts(), clear=True)
 
-    def _per_level_zeros(self, starts):
+    def _per_level(self, starts, clear):
+        """Per-level views of one allocation (cleared with one fill)."""
+        alloc = torch.zeros if clear else torch.empty
+        flat = alloc(int(starts[-1]), dtype=torch_dtype(self.preprocessed_mpole_dtype),
+                     device=self.tree_indep._setup_actx.device)
         result = []
         for lev in range(self.tree.nlevels):
             b0, b1 = self.tree_level_start_box_nrs[lev:lev + 2]
-            n = starts[lev + 1] - starts[lev]
-            result.append(self._zeros(n, self.preprocessed_mpole_dtype).reshape(
+            result.append(flat[int(starts[lev]):int(starts[lev + 1])].reshape(
                 int(b1 - b0), -1))
         return result
+
+    def _cached_zeros(self, name, make):
+        """A zero result that is never written, kept for the stages that have
+        no work for this traversal (the driver then skips adding it)."""
+        z = self._memo.get(name)
+        if z is None:
+            z = make()
+            self._memo[name] = z
+        return z
 
     def m2l_preproc_mpole_expansions_view(self, mpole_exps, level):
         return self.tree_level_start_box_nrs[level], mpole_exps[level]
@@ -510,6 +523,12 @@ class SumpyExpansionWrangler:
 
     def eval_multipoles(self, actx, target_boxes_by_source_level, source_boxes_by_level,
                         mpole_exps):
+        # host-side knowledge that the result is all zeros (no list-3 work);
+        # drive_fmm then skips adding it
+        self.last_eval_multipoles_was_empty = all(
+            len(t) == 0 for t in target_boxes_by_source_level)
+        if self.last_eval_multipoles_was_empty:
+            return self._cached_zeros("zero_potentials", lambda: self.output_zeros(actx))
         pot = self.output_zeros(actx)
         kwargs = dict(self.kernel_extra_kwargs)
         kwargs.update(self.box_target_list_kwargs())
@@ -529,14 +548,19 @@ class SumpyExpansionWrangler:
 
     def form_locals(self, actx, level_start_target_or_target_parent_box_nrs,
                     target_or_target_parent_boxes, starts, lists, src_weight_vecs):
-        local_exps = self.local_expansion_zeros(actx)
         lstarts = _host(level_start_target_or_target_parent_box_nrs)
         kwargs = dict(self.extra_kwargs)
         kwargs.update(self.box_source_list_kwargs())
         host_starts = self._host_copy(starts, "from_sep_bigger_starts")
         # host-side knowledge that the result is all zeros (no list-4 work);
         # drive_fmm then skips adding it
-        self.last_form_locals_was_empty = host_starts is not None
+        self.last_form_locals_was_empty = host_starts is not None and all(
+            lstarts[lev] == lstarts[lev + 1]
+            or host_starts[lstarts[lev]] == host_starts[lstarts[lev + 1]]
+            for lev in range(self.tree.nlevels))
+        if self.last_form_locals_was_empty:
+            return self._cached_zeros("zero_locals", lambda: self.local_expansion_zeros(actx))
+        local_exps = self.local_expansion_zeros(actx)
         for lev in range(self.tree.nlevels):
             start, stop = lstarts[lev:lev + 2]
             if start == stop:
@@ -545,7 +569,6 @@ class SumpyExpansionWrangler:
                 # no list-4 sources at this level: the zero-initialised
                 # expansions are what P2L would write
                 continue
-            self.last_form_locals_was_empty = False
             p2l = self.tree_indep.p2l(self.level_orders[lev])
             base, view = self.local_expansions_view(local_exps, lev)
             res = p2l(actx, target_boxes=target_or_target_parent_boxes[start:stop],
