@@ -316,7 +316,36 @@ __device__ __forceinline__ void wave_sync() {
 // order, in LDS.  Each wave then takes its own boxes: L (stored) to LDS,
 // lh[pos] = h[pos] (P L)[m(pos)], and a triple Horner per target (one lane
 // per target) with its gradient.  Four boxes per block are in flight at once.
-template <bool GRAD, int KMAX>
+// Triple Horner of the local expansion at (bx, by, bz) over the Horner-ordered
+// coefficients lh (runs of n = p - i - j + 1 coefficients for i = p..0,
+// j = p - i..0), with its gradient; ORD = p at compile time, so every
+// coefficient read has a constant offset and no row is padded.
+template <int ORD, bool GRAD>
+__device__ __forceinline__ void horner_t(const double *lh, double bx, double by, double bz,
+                                         double &P, double &Px, double &Py, double &Pz) {
+  int pos = 0;
+  #pragma unroll
+  for (int i = ORD; i >= 0; --i) {
+    double S = 0, Sy = 0, Sz = 0;
+    #pragma unroll
+    for (int j = ORD - i; j >= 0; --j) {
+      const int n = ORD - i - j + 1;
+      double R = 0, Rz = 0;
+      #pragma unroll
+      for (int q = 0; q < n; ++q) {
+        if (GRAD) Rz = Rz*bz + R;
+        R = R*bz + lh[pos + q];
+      }
+      pos += n;
+      if (GRAD) { Sy = Sy*by + S; Sz = Sz*by + Rz; }
+      S = S*by + R;
+    }
+    if (GRAD) { Px = Px*bx + P; Py = Py*bx + Sy; Pz = Pz*bx + Sz; }
+    P = P*bx + S;
+  }
+}
+
+template <bool GRAD, int KMAX, int ORD = -1>
 __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
@@ -386,6 +415,9 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
       const int t = t0 + tc + (valid ? lane : 0);
       const double bx = tx[t] - c0, by = ty[t] - c1, bz = tz[t] - c2;
       double P = 0, Px = 0, Py = 0, Pz = 0;
+      if constexpr (ORD >= 0) {
+        horner_t<ORD, GRAD>(lh, bx, by, bz, P, Px, Py, Pz);
+      } else {
       int pos = 0;
       for (int i = p; i >= 0; --i) {
         double S = 0, Sy = 0, Sz = 0;
@@ -409,6 +441,7 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
         }
         if (GRAD) { Px = Px*bx + P; Py = Py*bx + Sy; Pz = Pz*bx + Sz; }
         P = P*bx + S;
+      }
       }
       if (valid) {
         if (pot) pot[t] = scale*P;
@@ -898,11 +931,25 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   const int grid = (ntgt_boxes + L2P_NW - 1)/L2P_NW < 8*num_cus()
                        ? (ntgt_boxes + L2P_NW - 1)/L2P_NW : 8*num_cus();
   hipStream_t st = (hipStream_t)stream;
-#define SUMPY_L2P_LAUNCH(G, K)                                                             \
-  l2p_lap3d<G, K><<<grid, 64*L2P_NW, lds, st>>>(                                           \
+#define SUMPY_L2P_LAUNCH(G, K, ...)                                                        \
+  l2p_lap3d<G, K, ##__VA_ARGS__><<<grid, 64*L2P_NW, lds, st>>>(                            \
       *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,      \
       box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,  \
       gx, gy, gz)
+#define SUMPY_L2P_ORD(O)                                                                    \
+  case O:                                                                                   \
+    SUMPY_L2P_LAUNCH(false, 8, O);                                                          \
+    return check_launch("l2p");
+  // potentials: compile-time Horner for the usual orders (with gradients the
+  // unrolled form would hold every coefficient in registers); otherwise the
+  // generic loop
+  if (!gx) switch (expn->order) {
+    SUMPY_L2P_ORD(1) SUMPY_L2P_ORD(2) SUMPY_L2P_ORD(3) SUMPY_L2P_ORD(4) SUMPY_L2P_ORD(5)
+    SUMPY_L2P_ORD(6) SUMPY_L2P_ORD(7) SUMPY_L2P_ORD(8) SUMPY_L2P_ORD(9) SUMPY_L2P_ORD(10)
+    SUMPY_L2P_ORD(11) SUMPY_L2P_ORD(12)
+    default: break;
+  }
+#undef SUMPY_L2P_ORD
   if (expn->order < 8) {
     if (gx) SUMPY_L2P_LAUNCH(true, 8); else SUMPY_L2P_LAUNCH(false, 8);
   } else if (expn->order < 12) {

@@ -219,14 +219,16 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
   __shared__ v4 s_src[CH];
   __shared__ v4 s_dip[DIP ? CH : 1];
   __shared__ int s_bs[MAXB], s_bn[MAXB], s_boff[MAXB + 1];
+  __shared__ double s_red[GRAD ? 4 : 1][NB];
   const int tid = threadIdx.x;
   for (int ib = blockIdx.x; ib < a.ntgt_boxes; ib += gridDim.x) {
     const int tb = a.target_boxes[ib];
     const int t0 = a.box_tgt_starts[tb], nt = a.box_tgt_counts[tb];
     if (nt == 0) continue;
     const int sb0 = a.src_box_starts[ib], sb1 = a.src_box_starts[ib + 1];
-    int G = 1;
-    while (G < 64 && nt*2*G <= NB) G *= 2;
+    // G consecutive threads per target (any G, so that nt G fills the block:
+    // the lane sums are added through LDS)
+    const int G = max(1, min(64, NB/nt));
     const int tpp = NB/G;
     double org[3] = {0, 0, 0};
     if (F32) {
@@ -330,9 +332,17 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
           }
         }
       }
-      pot = group_sum(pot, G);
-      if (GRAD) { g0 = group_sum(g0, G); g1 = group_sum(g1, G); g2 = group_sum(g2, G); }
+      __syncthreads();  // the previous pass's sums have been read
+      s_red[0][tid] = pot;
+      if constexpr (GRAD) { s_red[1][tid] = g0; s_red[2][tid] = g1; s_red[3][tid] = g2; }
+      __syncthreads();
       if (sub == 0 && tvalid) {
+        const int q0 = lt*G;
+        pot = 0; g0 = g1 = g2 = 0;
+        for (int q = q0; q < q0 + G; ++q) {
+          pot += s_red[0][q];
+          if constexpr (GRAD) { g0 += s_red[1][q]; g1 += s_red[2][q]; g2 += s_red[3][q]; }
+        }
         if (a.pot) a.pot[t] = a.scale*pot;
         if (GRAD) {
           a.gx[t] = a.scale*g0; a.gy[t] = a.scale*g1;
