@@ -38,6 +38,13 @@ __device__ __forceinline__ double inv_fact(int n) {
   return 1.0/f;
 }
 
+// LDS written by some lanes of a wave and read by others of the same wave
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // {{{ P2M
 
 constexpr int P2M_NT = 128;   // threads per block, one source box at a time
@@ -92,8 +99,9 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
     const double *wy, const double *wz, double rscale, double *out, int base) {
   extern __shared__ double lds[];
   const int p = e.order, P1 = p + 1;
-  double *pw = lds;                       // [SCH][3][P1]
-  double *sw = pw + P2M_SCH*3*P1;         // [SCH][4]
+  const int PW = P1 + KT;                 // power rows zero padded by KT
+  double *pw = lds;                       // [SCH][3][PW]
+  double *sw = pw + P2M_SCH*3*PW;         // [SCH][4]
   double *cf = sw + P2M_SCH*4;            // [nfull]
   const int tid = threadIdx.x;
   const int ntasks = p2m_ntasks(p, KT);
@@ -114,11 +122,12 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
         if (tid < nn) {
           const int s = s0 + sc + tid;
           const double a[3] = {(c0 - sx[s])*rinv, (c1 - sy[s])*rinv, (c2 - sz[s])*rinv};
-          double *row = pw + tid*3*P1;
+          double *row = pw + tid*3*PW;
           for (int d = 0; d < 3; ++d) {
             double v = 1;
-            row[d*P1] = 1;
-            for (int q = 1; q < P1; ++q) { v *= a[d]*c_inv[q]; row[d*P1 + q] = v; }
+            row[d*PW] = 1;
+            for (int q = 1; q < P1; ++q) { v *= a[d]*c_inv[q]; row[d*PW + q] = v; }
+            for (int q = P1; q < PW; ++q) row[d*PW + q] = 0.0;
           }
           sw[tid*4] = w0[s];
           if (DIP) {
@@ -127,9 +136,11 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
         }
         __syncthreads();
         if (has) {
+          // branch-free: the KT powers past the task's range read zero
+          // padding or feed sums that are not stored
           #pragma unroll 4
           for (int s = 0; s < nn; ++s) {
-            const double *X = pw + s*3*P1, *Y = X + P1, *Z = Y + P1;
+            const double *X = pw + s*3*PW, *Y = X + PW, *Z = Y + PW;
             const double xi = X[ti], yj = Y[tj];
             double bco = sw[s*4]*xi*yj, cco = 0;
             if (DIP) {
@@ -139,10 +150,8 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
             #pragma unroll
             for (int kk = 0; kk < KT; ++kk) {
               const int k = k0 + kk;
-              if (k < k1) {
-                acc[kk] += bco*Z[k];
-                if (DIP && k > 0) acc[kk] += cco*Z[k - 1];
-              }
+              acc[kk] += bco*Z[k];
+              if (DIP) acc[kk] += cco*(k > 0 ? Z[k - 1] : 0.0);
             }
           }
         }
@@ -159,6 +168,122 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
       double v = 0;
       for (int q = e.projT_col_starts[s]; q < e.projT_col_starts[s + 1]; ++q)
         v += e.projT_vals[q]*cf[e.projT_rows[q]];
+      o[s] = v;
+    }
+  }
+}
+
+// P2M with the order at compile time: one wave per source box (4 boxes per
+// block), lane = (i, j) pair(s) of the full Taylor set, accumulating
+// C[i, j, k] for all k <= ORD in registers (entries past ORD - i - j are not
+// stored).  Per chunk of 32 sources the lanes tabulate X_q = (a_x/rho)^q / q!
+// and Y_q in LDS (one source per lane); per source a lane gathers its X_i and
+// Y_j, and runs the Z powers of the (broadcast) source itself.  The stored
+// coefficients are then M = P^T C (sumpy/expansion/multipole.py:78-114).
+constexpr int P2M_NW = 4;
+
+template <int ORD, bool DIP>
+__global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
+    sumpyhip_expn e, int nboxes_list, const int *__restrict__ source_boxes,
+    const int *__restrict__ bss, const int *__restrict__ bsc,
+    const double *__restrict__ centers, int nboxes, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz,
+    const double *__restrict__ w0, const double *__restrict__ wx,
+    const double *__restrict__ wy, const double *__restrict__ wz, double rscale, double *out,
+    int base) {
+  constexpr int P1 = ORD + 1;
+  constexpr int NP = P1*(P1 + 1)/2;  // pairs i + j <= ORD
+  constexpr int PPL = (NP + 63)/64;  // pairs per lane
+  constexpr int NF = P1*(P1 + 1)*(P1 + 2)/6;
+  constexpr int CHK = 32;            // sources per table chunk
+  __shared__ double sX[P2M_NW][CHK][P1], sY[P2M_NW][CHK][P1];
+  __shared__ double sZ[P2M_NW][CHK][DIP ? 4 : 2];  // a_z / rho, w0 (wx, wy / rho)
+  __shared__ double sWz[P2M_NW][DIP ? CHK : 1];    // wz / rho
+  __shared__ double sC[P2M_NW][NF];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // this lane's pairs, in order of i + j, then i
+  int pi[PPL], pj[PPL];
+  #pragma unroll
+  for (int r = 0; r < PPL; ++r) {
+    int q = lane + 64*r, d = 0;
+    while (d <= ORD && q >= d + 1) { q -= d + 1; ++d; }
+    pi[r] = d <= ORD ? q : 0;
+    pj[r] = d <= ORD ? d - q : 0;
+    if (d > ORD) pi[r] = -1;
+  }
+  const double rinv = 1.0/rscale;
+  for (int ib = blockIdx.x*P2M_NW + w; ib < nboxes_list; ib += gridDim.x*P2M_NW) {
+    const int b = source_boxes[ib];
+    const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
+    const int s0 = bss[b], ns = bsc[b];
+    double acc[PPL][P1];
+    #pragma unroll
+    for (int r = 0; r < PPL; ++r)
+      #pragma unroll
+      for (int k = 0; k < P1; ++k) acc[r][k] = 0.0;
+    for (int sc = 0; sc < ns; sc += CHK) {
+      const int nn = min(CHK, ns - sc);
+      wave_sync();
+      if (lane < nn) {
+        const int s = s0 + sc + lane;
+        const double ax = (c0 - sx[s])*rinv, ay = (c1 - sy[s])*rinv;
+        double xv = 1, yv = 1;
+        #pragma unroll
+        for (int q = 0; q < P1; ++q) {
+          sX[w][lane][q] = xv;
+          sY[w][lane][q] = yv;
+          xv *= ax*(1.0/(q + 1));
+          yv *= ay*(1.0/(q + 1));
+        }
+        sZ[w][lane][0] = (c2 - sz[s])*rinv;
+        sZ[w][lane][1] = w0[s];
+        if (DIP) {
+          sZ[w][lane][2 % (DIP ? 4 : 2)] = wx[s]*rinv;
+          sZ[w][lane][3 % (DIP ? 4 : 2)] = wy[s]*rinv;
+          sWz[w][lane % (DIP ? CHK : 1)] = wz[s]*rinv;
+        }
+      }
+      wave_sync();
+      for (int t = 0; t < nn; ++t) {
+        const double az = sZ[w][t][0], q0 = sZ[w][t][1];
+        #pragma unroll
+        for (int r = 0; r < PPL; ++r) {
+          const int i = pi[r] < 0 ? 0 : pi[r], j = pj[r];
+          const double xi = sX[w][t][i], yj = sY[w][t][j];
+          double bco = q0*xi*yj, cco = 0.0;
+          if (DIP) {
+            const double qx = sZ[w][t][2 % (DIP ? 4 : 2)], qy = sZ[w][t][3 % (DIP ? 4 : 2)];
+            bco += qx*(i ? sX[w][t][i - 1] : 0.0)*yj + qy*xi*(j ? sY[w][t][j - 1] : 0.0);
+            cco = sWz[w][t % (DIP ? CHK : 1)]*xi*yj;
+          }
+          double z = 1.0, zprev = 0.0;
+          #pragma unroll
+          for (int k = 0; k < P1; ++k) {
+            acc[r][k] += bco*z;
+            if (DIP) acc[r][k] += cco*zprev;
+            zprev = z;
+            z *= az*(1.0/(k + 1));
+          }
+        }
+      }
+    }
+    // C to LDS (full index order), then M = P^T C
+    wave_sync();
+    #pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+      if (pi[r] < 0) continue;
+      const int i = pi[r], j = pj[r];
+      #pragma unroll
+      for (int k = 0; k < P1; ++k)
+        if (k <= ORD - i - j) sC[w][e.cube_index[(i*P1 + j)*P1 + k]] = acc[r][k];
+    }
+    wave_sync();
+    double *o = out + (size_t)(b - base)*e.nstored;
+    for (int s = lane; s < e.nstored; s += 64) {
+      double v = 0;
+      for (int q = e.projT_col_starts[s]; q < e.projT_col_starts[s + 1]; ++q)
+        v += e.projT_vals[q]*sC[w][e.projT_rows[q]];
       o[s] = v;
     }
   }
@@ -304,12 +429,6 @@ __global__ __launch_bounds__(WAVE) void l2l_lap3d(
 
 constexpr int L2P_NW = 4;  // waves per block, each on its own target box
 
-// LDS written by some lanes of a wave and read by others of the same wave
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Per block (once): the Horner-ordered factors h[pos] = rho^-|m| / m! of
 // L_full = P L (sumpy/expansion/local.py:284-309) and P's rows in Horner
@@ -841,7 +960,8 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
   if (rc) return rc;
   if (nsrc_boxes == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(!wx || (wy && wz), "p2m: incomplete dipole weights");
-  const size_t lds = sizeof(double)*(3*P2M_SCH*(expn->order + 1) + 4*P2M_SCH + expn->nfull);
+  const int kt = p2m_kt(expn->order);
+  const size_t lds = sizeof(double)*(3*P2M_SCH*(expn->order + 1 + kt) + 4*P2M_SCH + expn->nfull);
   SUMPY_REQUIRE(lds <= 64*1024, "p2m: order too large");
   hipStream_t st = (hipStream_t)stream;
   const int grid = nsrc_boxes;  // one box per block: many boxes in flight per CU
@@ -858,7 +978,31 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,       \
           tgt_base_ibox);                                                                \
   } while (0)
-  switch (p2m_kt(expn->order)) {
+  if (expn->nfull == (expn->order + 1)*(expn->order + 2)*(expn->order + 3)/6) {
+    // pair kernel for the usual orders
+    const int g2 = (nsrc_boxes + P2M_NW - 1)/P2M_NW;
+#define SUMPY_P2M_PAIRS(O)                                                                   \
+  case O:                                                                                    \
+    if (wx)                                                                                  \
+      p2m_lap3d_pairs<O, true><<<g2, 64*P2M_NW, 0, st>>>(                                    \
+          *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
+          box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
+          tgt_base_ibox);                                                                    \
+    else                                                                                     \
+      p2m_lap3d_pairs<O, false><<<g2, 64*P2M_NW, 0, st>>>(                                   \
+          *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
+          box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
+          tgt_base_ibox);                                                                    \
+    return check_launch("p2m");
+    switch (expn->order) {
+      SUMPY_P2M_PAIRS(1) SUMPY_P2M_PAIRS(2) SUMPY_P2M_PAIRS(3) SUMPY_P2M_PAIRS(4)
+      SUMPY_P2M_PAIRS(5) SUMPY_P2M_PAIRS(6) SUMPY_P2M_PAIRS(7) SUMPY_P2M_PAIRS(8)
+      SUMPY_P2M_PAIRS(9) SUMPY_P2M_PAIRS(10) SUMPY_P2M_PAIRS(11) SUMPY_P2M_PAIRS(12)
+      default: break;
+    }
+#undef SUMPY_P2M_PAIRS
+  }
+  switch (kt) {
     case 2: SUMPY_P2M(2); break;
     case 3: SUMPY_P2M(3); break;
     case 4: SUMPY_P2M(4); break;
