@@ -795,31 +795,38 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
 
 // P2L consumer: every lane holds one source; the values q0 D(m) (+ sum_ax
 // qd_ax D(m + e_ax) for dipoles) are produced in shell order, written to the
-// wave's LDS buffer (one row of 64 lanes per coefficient) and, every 16
-// coefficients, summed over the lanes: lane l then adds the sum for
-// coefficient 16 k + l / 4 to its register acc[k].
+// wave's LDS ring (one row of 64 lanes per coefficient, two chunks of 16
+// rows) and, every 16 coefficients, summed in quarters: lane l adds lanes
+// 16 (l % 4) .. 16 (l % 4) + 15 of coefficient 16 k + l / 4 to its register
+// acc[k].  The four quarters are added once per target box (finish()).
 template <int ORD, bool DIP>
 struct P2LSum {
   static constexpr int NS = (ORD + 1)*(ORD + 1);
   static constexpr int NCH = (NS + 15)/16;
   static constexpr int BS = 64 + 1;  // LDS row stride (doubles)
+  static constexpr int RING = 32;    // rows: two chunks
   double acc[NCH];
   double q0, qd[3];
-  double *buf;  // [16][BS]
+  double *buf;  // [RING][BS]
   int lane;
   __device__ __forceinline__ void put(int p, double v) {
-    buf[(p & 15)*BS + lane] = v;
+    buf[(p & (RING - 1))*BS + lane] = v;
     if ((p & 15) == 15 || p == NS - 1) {
+      // the other chunk's rows were read before this chunk's first write
+      // (previous flush), so one wave barrier per flush suffices
       wave_sync();
-      const double *r = buf + (lane >> 2)*BS + (lane & 3)*16;
+      const double *r = buf + ((p & 16) + (lane >> 2))*BS + (lane & 3)*16;
       double t = 0;
       #pragma unroll
       for (int q = 0; q < 16; ++q) t += r[q];
-      t += __shfl_xor(t, 1, 4);
-      t += __shfl_xor(t, 2, 4);
       acc[p >> 4] += t;
-      wave_sync();
     }
+  }
+  __device__ __forceinline__ double finish(int k) const {
+    double t = acc[k];
+    t += __shfl_xor(t, 1, 4);
+    t += __shfl_xor(t, 2, 4);
+    return t;
   }
   template <int N>
   __device__ __forceinline__ void shell(const double *cur, const double *prev) {
@@ -842,14 +849,16 @@ struct P2LSum {
   }
 };
 
-// One wave per target box (LW boxes per block).  The sources of the box's
+// One wave per target box (P2L_LW boxes per block).  The sources of the box's
 // list-4 boxes are concatenated (box offsets and first sources from a wave
 // scan, 64 boxes at a time, in LDS) and taken 64 at a time, one per lane;
 // lanes past the end carry zero weights.  The lane sums are reduced through
 // LDS as the coefficients are produced (P2LSum), so a lane needs only the
 // live derivative shells and (ORD + 1)^2 / 16 accumulators.
+constexpr int P2L_LW = 2;  // waves (target boxes) per block of the P2L kernel
+
 template <int ORD, bool DIP>
-__global__ __launch_bounds__(64*LW) void p2l_lap3d(
+__global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
     sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
     const int *__restrict__ sstarts, const int *__restrict__ slists,
     const int *__restrict__ bss, const int *__restrict__ bsc,
@@ -861,15 +870,15 @@ __global__ __launch_bounds__(64*LW) void p2l_lap3d(
   using Sum = P2LSum<ORD, DIP>;
   constexpr int NS = Sum::NS;
   __shared__ int sStored[NS];  // shell position -> stored index
-  __shared__ int sOff[LW][64], sFirst[LW][64];
-  __shared__ double sBuf[LW][16*Sum::BS];
+  __shared__ int sOff[P2L_LW][64], sFirst[P2L_LW][64];
+  __shared__ double sBuf[P2L_LW][Sum::RING*Sum::BS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int s = tid; s < NS; s += 64*LW)
+  for (int s = tid; s < NS; s += 64*P2L_LW)
     sStored[shell_pos(e.stored_ids[s], e.stored_ids[NS + s], e.stored_ids[2*NS + s])] = s;
   __syncthreads();
   int *off = sOff[wave], *first = sFirst[wave];
-  for (int ib = blockIdx.x*LW + wave; ib < ntgt; ib += gridDim.x*LW) {
+  for (int ib = blockIdx.x*P2L_LW + wave; ib < ntgt; ib += gridDim.x*P2L_LW) {
     const int tb = target_boxes[ib];
     const int is0 = sstarts[ib], is1 = sstarts[ib + 1];
     double *o = out + (size_t)(tb - base)*NS;
@@ -898,29 +907,42 @@ __global__ __launch_bounds__(64*LW) void p2l_lap3d(
       wave_sync();
       if (lane < nb) { off[lane] = v - c; first[lane] = bss[sb]; }
       wave_sync();
-      for (int j0 = 0; j0 < total; j0 += 64) {
-        const int jj = min(j0 + lane, total - 1);
+      // source of concatenated position j (clamped to the last one)
+      auto source_at = [&](int j) {
+        const int jj = min(j, total - 1);
         int lo = 0, hi = nb - 1;  // largest b with off[b] <= jj
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
           if (off[mid] <= jj) lo = mid; else hi = mid - 1;
         }
-        const int s = first[lo] + (jj - off[lo]);
+        return first[lo] + (jj - off[lo]);
+      };
+      // the next batch's source data are loaded while this batch's shells run
+      double nx, ny, nz, nq, ndx = 0, ndy = 0, ndz = 0;
+      auto load = [&](int s) {
+        nx = sx[s]; ny = sy[s]; nz = sz[s]; nq = w0[s];
+        if (DIP) { ndx = wx[s]; ndy = wy[s]; ndz = wz[s]; }
+      };
+      if (total > 0) load(source_at(lane));
+      for (int j0 = 0; j0 < total; j0 += 64) {
         const bool live = j0 + lane < total;
-        f.q0 = live ? w0[s] : 0.0;
+        const double x = nx, y = ny, z = nz;
+        f.q0 = live ? nq : 0.0;
         if (DIP) {
-          f.qd[0] = live ? wx[s]/rscale : 0.0;
-          f.qd[1] = live ? wy[s]/rscale : 0.0;
-          f.qd[2] = live ? wz[s]/rscale : 0.0;
+          f.qd[0] = live ? ndx/rscale : 0.0;
+          f.qd[1] = live ? ndy/rscale : 0.0;
+          f.qd[2] = live ? ndz/rscale : 0.0;
         }
-        const LapGeo g(c0 - sx[s], c1 - sy[s], c2 - sz[s], rscale);
+        load(source_at(j0 + 64 + lane));
+        const LapGeo g(c0 - x, c1 - y, c2 - z, rscale);
         lap3d_shells<ORD + (DIP ? 1 : 0)>(g, f);
       }
     }
     #pragma unroll
     for (int k = 0; k < Sum::NCH; ++k) {
       const int m = 16*k + (lane >> 2);
-      if ((lane & 3) == 0 && m < NS) o[sStored[m]] = f.acc[k];
+      const double v = f.finish(k);
+      if ((lane & 3) == 0 && m < NS) o[sStored[m]] = v;
     }
   }
 }
@@ -1164,16 +1186,16 @@ extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   SUMPY_REQUIRE(expn->nstored == (expn->order + 1)*(expn->order + 1),
                 "p2l: only PDE-compressed expansions are implemented");
   SUMPY_REQUIRE(!wx || (wy && wz), "p2l: incomplete dipole weights");
-  const int grid = (ntgt_boxes + LW - 1)/LW;
+  const int grid = (ntgt_boxes + P2L_LW - 1)/P2L_LW;
 #define SUMPY_P2L(ORD)                                                                       \
   case ORD:                                                                                  \
     if (wx)                                                                                  \
-      p2l_lap3d<ORD, true><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                         \
+      p2l_lap3d<ORD, true><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                         \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
           w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
     else                                                                                     \
-      p2l_lap3d<ORD, false><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                        \
+      p2l_lap3d<ORD, false><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                        \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
           w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
