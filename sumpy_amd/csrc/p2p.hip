@@ -219,6 +219,7 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
   __shared__ v4 s_src[CH];
   __shared__ v4 s_dip[DIP ? CH : 1];
   __shared__ int s_bs[MAXB], s_bn[MAXB], s_boff[MAXB + 1];
+  __shared__ int s_ktb;  // position of the target box in the source-box chunk, or -1
   __shared__ double s_red[GRAD ? 4 : 1][NB];
   const int tid = threadIdx.x;
   for (int ib = blockIdx.x; ib < a.ntgt_boxes; ib += gridDim.x) {
@@ -247,10 +248,13 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
       for (int bb = sb0; bb < sb1; bb += MAXB) {
         const int nbx = min(MAXB, sb1 - bb);
         __syncthreads();
+        if (tid == 0) s_ktb = -1;
+        __syncthreads();
         if (tid < nbx) {
           const int sbx = a.src_box_lists[bb + tid];
           s_bs[tid] = a.box_src_starts[sbx];
           s_bn[tid] = a.box_src_counts[sbx];
+          if (SELF && sbx == tb) s_ktb = tid;
         }
         __syncthreads();
         if (tid < 64) {  // exclusive scan of the counts (4 per lane)
@@ -273,8 +277,14 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
         const int total = s_boff[nbx];
         int pself = -1;
         if (SELF) {
-          for (int k = 0; k < nbx; ++k)
-            if ((unsigned)(self - s_bs[k]) < (unsigned)s_bn[k]) pself = s_boff[k] + self - s_bs[k];
+          // a target's own source usually lies in its own box; else search
+          const int kt = s_ktb;
+          if (kt >= 0 && (unsigned)(self - s_bs[kt]) < (unsigned)s_bn[kt]) {
+            pself = s_boff[kt] + self - s_bs[kt];
+          } else {
+            for (int k = 0; k < nbx; ++k)
+              if ((unsigned)(self - s_bs[k]) < (unsigned)s_bn[k]) pself = s_boff[k] + self - s_bs[k];
+          }
         }
         for (int c0 = 0; c0 < total; c0 += CH) {
           const int nn = min(CH, total - c0);
