@@ -65,10 +65,13 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
         mpole_exps)
     wrangler.communicate_mpoles(actx, mpole_exps)
 
-    timer.mark("eval_direct")
-    potentials = wrangler.eval_direct(
-        actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
-        trav.neighbor_source_boxes_lists, src_weight_vecs)
+    # (per-stage timing needs one stream: no overlap then)
+    side = _side_stream(actx) if _OVERLAP_NEAR_FIELD and timing_data is None else None
+    if side is None:
+        timer.mark("eval_direct")
+        potentials = wrangler.eval_direct(
+            actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
+            trav.neighbor_source_boxes_lists, src_weight_vecs)
 
     timer.mark("multipole_to_local")
     local_exps = wrangler.multipole_to_local(
@@ -76,18 +79,33 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
         trav.target_or_target_parent_boxes, trav.from_sep_siblings_starts,
         trav.from_sep_siblings_lists, mpole_exps)
 
+    if side is not None:
+        # The near field (list 1) depends on nothing but the weights.  It
+        # runs on a second stream once the M2L contraction has finished (run
+        # beside it, the two compete for LDS and issue slots), so that it fills
+        # the GPU while the latency-bound downward pass (lists 3 and 4, L2L,
+        # L2P) runs; its potentials are joined at the end and summed in the
+        # reference's order.
+        import torch
+        main = torch.cuda.current_stream(actx.device)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            potentials = wrangler.eval_direct(
+                actx, trav.target_boxes, trav.neighbor_source_boxes_starts,
+                trav.neighbor_source_boxes_lists, src_weight_vecs)
+
     timer.mark("eval_multipoles")
     mpole_result = wrangler.eval_multipoles(
         actx, trav.target_boxes_sep_smaller_by_source_level,
         trav.from_sep_smaller_by_level, mpole_exps)
     # (a wrangler that knows its list-3 result is all zeros says so)
-    if not getattr(wrangler, "last_eval_multipoles_was_empty", False):
-        potentials = _add(potentials, mpole_result)
+    add_mpole = not getattr(wrangler, "last_eval_multipoles_was_empty", False)
 
+    close_smaller = None
     if getattr(trav, "from_sep_close_smaller_starts", None) is not None:
-        potentials = _add(potentials, wrangler.eval_direct(
+        close_smaller = wrangler.eval_direct(
             actx, trav.target_boxes, trav.from_sep_close_smaller_starts,
-            trav.from_sep_close_smaller_lists, src_weight_vecs))
+            trav.from_sep_close_smaller_lists, src_weight_vecs)
 
     timer.mark("form_locals")
     local_result = wrangler.form_locals(
@@ -98,10 +116,11 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     if not getattr(wrangler, "last_form_locals_was_empty", False):
         local_exps = local_exps + local_result
 
+    close_bigger = None
     if getattr(trav, "from_sep_close_bigger_starts", None) is not None:
-        potentials = _add(potentials, wrangler.eval_direct(
+        close_bigger = wrangler.eval_direct(
             actx, trav.target_boxes, trav.from_sep_close_bigger_starts,
-            trav.from_sep_close_bigger_lists, src_weight_vecs))
+            trav.from_sep_close_bigger_lists, src_weight_vecs)
 
     timer.mark("refine_locals")
     local_exps = wrangler.refine_locals(
@@ -111,6 +130,19 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     timer.mark("eval_locals")
     local_result = wrangler.eval_locals(
         actx, host("level_start_target_box_nrs"), trav.target_boxes, local_exps)
+
+    if side is not None:
+        main.wait_stream(side)
+        for pot in potentials:
+            pot.record_stream(main)
+        for w in src_weight_vecs:
+            w.record_stream(side)
+    if add_mpole:
+        potentials = _add(potentials, mpole_result)
+    if close_smaller is not None:
+        potentials = _add(potentials, close_smaller)
+    if close_bigger is not None:
+        potentials = _add(potentials, close_bigger)
     potentials = _add(potentials, local_result)
 
     timer.mark("reorder_potentials")
@@ -120,6 +152,23 @@ def drive_fmm(actx, wrangler, src_weight_vecs, timing_data=None):
     if timing_data is not None:
         timing_data.update(timer.result())
     return result
+
+
+_OVERLAP_NEAR_FIELD = True
+_SIDE = {}
+
+
+def _side_stream(actx):
+    """A second (default-priority) stream on the context's device, or None
+    off the GPU."""
+    import torch
+    dev = getattr(actx, "device", None)
+    if dev is None or torch.device(dev).type != "cuda":
+        return None
+    key = str(torch.device(dev))
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(torch.device(dev))
+    return _SIDE[key]
 
 
 def _add(a, b):
