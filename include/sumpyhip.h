@@ -14,6 +14,10 @@
  *     sumpyhip_last_error() describes the last failure of the calling thread.
  *
  * Kernel ids: 2 = Laplace 2D, 3 = Laplace 3D, 13 = Helmholtz 3D.
+ * P2M, M2M, L2L and L2P are driven by the expansion tables alone: for the
+ * Helmholtz kernel (real k) the caller passes one real plane of the complex
+ * coefficients (and of the weights / outputs) at a time, with tables whose PDE
+ * projection holds (k rscale)^2 for the expansion's rscale.
  * Precision: 0 = plain FP64; 1 = FP32 pair math with FP64 accumulation (P2P),
  * FP32 MFMA (M2L).
  */
@@ -170,6 +174,46 @@ int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                  double rscale, double scale, const double *src_expansions,
                  int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
                  void *stream);
+
+/* }}} */
+
+/* {{{ Helmholtz-3D (G = e^{ik|d|}/|d|, sumpy/kernel.py:754-844), real k */
+
+/* M2P from list 3 for the Helmholtz kernel, replaces E2PFromCSR.__call__
+ * (sumpy/e2p.py:233-343) for HelmholtzKernel(3); accumulates.  Multipoles,
+ * pot and the gradient are complex, interleaved (re, im) doubles; expn holds
+ * the stored identifiers of the PDE-compressed expansion (at most 256). */
+int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+                           const int *target_boxes, const int *source_box_starts,
+                           const int *source_box_lists, const int *box_target_starts,
+                           const int *box_target_counts_nonchild,
+                           const double *box_centers, int nboxes, const double *tx,
+                           const double *ty, const double *tz, double rscale,
+                           double scale, const void *src_expansions, int src_base_ibox,
+                           void *pot, void *gx, void *gy, void *gz, void *stream);
+
+/* P2L from list 4 for the Helmholtz kernel, replaces P2EFromCSR.__call__
+ * (sumpy/p2e.py:268-388) for HelmholtzKernel(3); overwrites the rows of the
+ * target boxes.  Weights and expansions are complex, interleaved. */
+int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+                           const int *target_boxes, const int *source_box_starts,
+                           const int *source_box_lists, const int *box_source_starts,
+                           const int *box_source_counts_nonchild,
+                           const double *box_centers, int nboxes, const double *sx,
+                           const double *sy, const double *sz, const void *w0,
+                           const void *wx, const void *wy, const void *wz, double rscale,
+                           void *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* Class data for the Helmholtz kernel, replaces
+ * M2LGenerateTranslationClassesDependentData.__call__ (sumpy/e2e.py:565-669)
+ * for HelmholtzKernel(3): out is nclasses x 2 x ncirc doubles, the real then
+ * the imaginary plane of rscale^|m| d^m G(v_c) in the circulant layout of
+ * sumpyhip_m2l_class_data.  The two planes are the classes 2c and 2c + 1 of
+ * the real embedding that sumpyhip_m2l_apply contracts. */
+int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt, const int *circ_shape,
+                                      int nclasses, const double *translation_vectors,
+                                      int nvectors, int class_start, double src_rscale,
+                                      double k, double *out, void *stream);
 
 /* }}} */
 

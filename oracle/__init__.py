@@ -154,10 +154,12 @@ def lap3d_derivs(x, rscale, maxorder, box):
 
 
 def p2m(order, source_boxes, tree_arrays, centers, sources, w0, wd, rscale,
-        tgt_expansions, tgt_base_ibox, compressed=True):
-    w = tables.wrangler(order, 3, compressed)
+        tgt_expansions, tgt_base_ibox, compressed=True, k=0.0):
+    """Real weights; *k* != 0 uses the Helmholtz PDE projection (apply to the
+    real and imaginary parts of complex weights separately)."""
+    w = tables.wrangler(order, 3, compressed, helmholtz=bool(k))
     full = i32(w.full)
-    P = f64(w.projection(rscale))
+    P = f64(w.projection(rscale, k))
     cp, sp = _PtrArray(list(centers)), _PtrArray(list(sources))
     wdp = _ptrs([f64(a) for a in wd]) if wd is not None else None
     sb = i32(source_boxes)
@@ -173,14 +175,14 @@ def p2m(order, source_boxes, tree_arrays, centers, sources, w0, wd, rscale,
 
 def m2m(src_order, tgt_order, target_boxes, box_child_ids, centers, src_rscale,
         tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox,
-        compressed=True):
-    wt = tables.wrangler(tgt_order, 3, compressed)
-    ws = tables.wrangler(src_order, 3, compressed)
+        compressed=True, k=0.0):
+    wt = tables.wrangler(tgt_order, 3, compressed, helmholtz=bool(k))
+    ws = tables.wrangler(src_order, 3, compressed, helmholtz=bool(k))
     cp = _PtrArray(list(centers))
     tb = i32(target_boxes)
     bci = i32(box_child_ids)
     lib().orc_m2m(ctypes.c_int(len(wt.full)), _i(i32(wt.full)),
-                  ctypes.c_int(len(wt.stored)), _d(f64(wt.projection(tgt_rscale))),
+                  ctypes.c_int(len(wt.stored)), _d(f64(wt.projection(tgt_rscale, k))),
                   ctypes.c_int(len(ws.stored)), _i(i32(ws.stored)),
                   ctypes.c_long(len(tb)), _i(tb), _i(bci),
                   ctypes.c_long(bci.shape[1]), cp.c, ctypes.c_double(src_rscale),
@@ -228,13 +230,13 @@ def m2l(order_src, order_tgt, target_boxes, src_box_starts, src_box_lists,
 
 def l2l(src_order, tgt_order, target_boxes, box_parent_ids, centers, src_rscale,
         tgt_rscale, src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox,
-        compressed=True):
-    ws = tables.wrangler(src_order, 3, compressed)
-    wt = tables.wrangler(tgt_order, 3, compressed)
+        compressed=True, k=0.0):
+    ws = tables.wrangler(src_order, 3, compressed, helmholtz=bool(k))
+    wt = tables.wrangler(tgt_order, 3, compressed, helmholtz=bool(k))
     cp = _PtrArray(list(centers))
     tb = i32(target_boxes)
     lib().orc_l2l(ctypes.c_int(len(ws.full)), _i(i32(ws.full)),
-                  ctypes.c_int(len(ws.stored)), _d(f64(ws.projection(src_rscale))),
+                  ctypes.c_int(len(ws.stored)), _d(f64(ws.projection(src_rscale, k))),
                   ctypes.c_int(len(wt.stored)), _i(i32(wt.stored)),
                   ctypes.c_int(src_order), ctypes.c_long(len(tb)), _i(tb),
                   _i(i32(box_parent_ids)), cp.c, ctypes.c_double(src_rscale),
@@ -246,13 +248,13 @@ def l2l(src_order, tgt_order, target_boxes, box_parent_ids, centers, src_rscale,
 
 def l2p(order, target_boxes, tree_arrays, centers, targets, rscale,
         src_expansions, src_base_ibox, pot, grad=None, scale=1/(4*np.pi),
-        compressed=True):
-    w = tables.wrangler(order, 3, compressed)
+        compressed=True, k=0.0):
+    w = tables.wrangler(order, 3, compressed, helmholtz=bool(k))
     cp, tp = _PtrArray(list(centers)), _PtrArray(list(targets))
     gp = _PtrArray(grad) if grad is not None else None
     tb = i32(target_boxes)
     lib().orc_l2p(ctypes.c_int(len(w.full)), _i(i32(w.full)),
-                  ctypes.c_int(len(w.stored)), _d(f64(w.projection(rscale))),
+                  ctypes.c_int(len(w.stored)), _d(f64(w.projection(rscale, k))),
                   ctypes.c_long(len(tb)), _i(tb),
                   _i(i32(tree_arrays["box_target_starts"])),
                   _i(i32(tree_arrays["box_target_counts_nonchild"])),

@@ -977,7 +977,10 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
                             const double *wx, const double *wy, const double *wz,
                             double rscale, double *tgt_expansions, int tgt_base_ibox,
                             void *stream) {
-  SUMPY_REQUIRE(kernel_id == 3, "p2m: only the Laplace 3D kernel is implemented");
+  // Helmholtz (13): one real plane of complex coefficients, with projection
+  // tables built for (k rscale)^2 -- the kernel itself is table-driven
+  SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
+                "p2m: only the Laplace and Helmholtz 3D kernels are implemented");
   int rc = check_expn(expn, "p2m");
   if (rc) return rc;
   if (nsrc_boxes == 0) return SUMPYHIP_OK;
@@ -1042,7 +1045,10 @@ extern "C" int sumpyhip_m2m(int kernel_id, const sumpyhip_expn *src_expn,
                             double tgt_rscale, const double *src_expansions,
                             int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
                             void *stream) {
-  SUMPY_REQUIRE(kernel_id == 3, "m2m: only the Laplace 3D kernel is implemented");
+  // Helmholtz (13): one real plane of complex coefficients, with projection
+  // tables built for (k rscale)^2 -- the kernel itself is table-driven
+  SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
+                "m2m: only the Laplace and Helmholtz 3D kernels are implemented");
   int rc = check_expn(src_expn, "m2m");
   if (!rc) rc = check_expn(tgt_expn, "m2m");
   if (rc) return rc;
@@ -1062,7 +1068,10 @@ extern "C" int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
                             double tgt_rscale, const double *src_expansions,
                             int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
                             void *stream) {
-  SUMPY_REQUIRE(kernel_id == 3, "l2l: only the Laplace 3D kernel is implemented");
+  // Helmholtz (13): one real plane of complex coefficients, with projection
+  // tables built for (k rscale)^2 -- the kernel itself is table-driven
+  SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
+                "l2l: only the Laplace and Helmholtz 3D kernels are implemented");
   int rc = check_expn(src_expn, "l2l");
   if (!rc) rc = check_expn(tgt_expn, "l2l");
   if (rc) return rc;
@@ -1083,7 +1092,10 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
                             double scale, const double *src_expansions, int src_base_ibox,
                             double *pot, double *gx, double *gy, double *gz,
                             void *stream) {
-  SUMPY_REQUIRE(kernel_id == 3, "l2p: only the Laplace 3D kernel is implemented");
+  // Helmholtz (13): one real plane of complex coefficients, with projection
+  // tables built for (k rscale)^2 -- the kernel itself is table-driven
+  SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
+                "l2p: only the Laplace and Helmholtz 3D kernels are implemented");
   int rc = check_expn(expn, "l2p");
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;

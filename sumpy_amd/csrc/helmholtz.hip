@@ -1,0 +1,439 @@
+// The kernel-dependent parts of the Helmholtz-3D FMM: translation-class data,
+// M2P (list 3) and P2L (list 4) for G = e^{ik|d|}/|d| (reference
+// sumpy/kernel.py:754-844), PDE-compressed Taylor expansions.
+//
+// Scaled derivatives.  With d = rho xi and kappa = k rho,
+//   rho^|m| d^m G(d) = (1/rho) d_xi^m H(xi),   H(xi) = e^{i kappa |xi|}/|xi|.
+// H is radial: H = F(s), s = |xi|^2/2, with F^(q) = g_q = ((1/r) d/dr)^q H, and
+//   g_0 = e^{i kappa r}/r,  g_1 = e^{i kappa r}(i kappa r - 1)/r^3,
+//   g_q = (-(2q - 1) g_{q-1} - kappa^2 g_{q-2})/r^2
+// (the radial recurrence of sumpy/derivative_taker.py:354-382, after Tausch).
+// Since s separates over the axes, the chain rule gives
+//   d^(a,b,c) H = sum_{i,j,l} h(a,i) h(b,j) h(c,l) x^{2i-a} y^{2j-b} z^{2l-c} g_{i+j+l}
+// with h(n,i) = n!/((2i-n)! (n-i)! 2^{n-i}), ceil(n/2) <= i <= n (the
+// coefficients of d^n/dx^n F(x^2/2) in the F^(i)).  This is the sum the
+// reference's RadialDerivativeTaker recursion (derivative_taker.py:290-346)
+// expands to; here it is evaluated directly from a table of h.
+//
+// Complex values are interleaved (re, im) doubles, torch's complex128 layout.
+// The kernel-independent stages (P2M, M2M, L2L, L2P) are the real kernels of
+// expansions.hip and octant.hip applied to the real and imaginary planes with
+// projection tables built for (k rho)^2; the M2L contraction is the real MFMA
+// contraction of m2l.hip on the real embedding [Lr; Li] = [Ar -Ai; Ai Ar][Mr; Mi].
+#include "common.h"
+
+namespace sumpyhip {
+namespace {
+
+constexpr int HELM_MAXN = 52;  // derivative orders handled (class data: 2 p <= 48)
+constexpr int HN = HELM_MAXN + 1;
+
+// h(n, i) for n, i <= nmax into tab[n*(nmax + 1) + i] (zero outside
+// ceil(n/2)..n); block-cooperative, ends with a barrier
+__device__ void herm_table(double *tab, int nmax) {
+  const int hs = nmax + 1;
+  for (int n = threadIdx.x; n <= nmax; n += blockDim.x) {
+    for (int i = 0; i <= nmax; ++i) tab[n*hs + i] = 0;
+    double h = 1;  // h(n, ceil(n/2)): the product of the odd numbers <= n
+    for (int o = 3; o <= n; o += 2) h *= o;
+    for (int i = (n + 1)/2; i <= n; ++i) {
+      tab[n*hs + i] = h;
+      h *= 2.0*(n - i)/double((2*i + 2 - n)*(2*i + 1 - n));
+    }
+  }
+  __syncthreads();
+}
+
+// g_0..g_n of e^{i kappa r}/r at r into (gre, gim)[q*stride]
+__device__ void helm_radial(double kappa, double r, int n, double *gre, double *gim,
+                            int stride) {
+  const double ri = 1.0/r, ri2 = ri*ri;
+  double sn, cs;
+  sincos(kappa*r, &sn, &cs);
+  double ar = cs*ri, ai = sn*ri;
+  gre[0] = ar;
+  gim[0] = ai;
+  if (n < 1) return;
+  const double kr = kappa*r;
+  // e^{i kr}(i kr - 1)/r^3
+  double br = (-cs - sn*kr)*ri2*ri, bi = (cs*kr - sn)*ri2*ri;
+  gre[stride] = br;
+  gim[stride] = bi;
+  const double k2 = kappa*kappa;
+  for (int q = 2; q <= n; ++q) {
+    const double f = -(2*q - 1);
+    const double cr = (f*br - k2*ar)*ri2, ci = (f*bi - k2*ai)*ri2;
+    gre[q*stride] = cr;
+    gim[q*stride] = ci;
+    ar = br; ai = bi; br = cr; bi = ci;
+  }
+}
+
+// d^(a,b,c) H from the powers xp[e*ps] = x^e (yp, zp likewise), g_q at
+// (gre, gim)[q*gs] and the h table of row stride hs
+__device__ __forceinline__ cplx helm_deriv(int a, int b, int c, const double *xp,
+                                           const double *yp, const double *zp, int ps,
+                                           const double *gre, const double *gim, int gs,
+                                           const double *ht, int hs) {
+  double rr = 0, ri = 0;
+  for (int i = (a + 1)/2; i <= a; ++i) {
+    double sr = 0, si = 0;
+    for (int j = (b + 1)/2; j <= b; ++j) {
+      double tr = 0, ti = 0;
+      for (int l = (c + 1)/2; l <= c; ++l) {
+        const double f = ht[c*hs + l]*zp[(2*l - c)*ps];
+        const int q = (i + j + l)*gs;
+        tr += f*gre[q];
+        ti += f*gim[q];
+      }
+      const double f = ht[b*hs + j]*yp[(2*j - b)*ps];
+      sr += f*tr;
+      si += f*ti;
+    }
+    const double f = ht[a*hs + i]*xp[(2*i - a)*ps];
+    rr += f*sr;
+    ri += f*si;
+  }
+  return {rr, ri};
+}
+
+// {{{ translation-class data
+
+// out[c][0][off(m)] + i out[c][1][off(m)] = rho^|m| d^m G(v_c) for |m| <= N in
+// the row-major box (M0, M1, M2); zero elsewhere
+__global__ __launch_bounds__(256) void helm_class_data_kernel(
+    int N, int M0, int M1, int M2, int nclasses, const double *vecs, int nvectors,
+    int class_start, double rscale, double k, double *out) {
+  __shared__ double ht[HN*HN];
+  __shared__ double sp[3][HN], sg[2][HN];
+  herm_table(ht, N);
+  const size_t ncirc = (size_t)M0*M1*M2;
+  const double rinv = 1.0/rscale;
+  for (int c = blockIdx.x; c < nclasses; c += gridDim.x) {
+    __syncthreads();  // the previous class is done with sp / sg
+    if (threadIdx.x < 4) {
+      const size_t v = class_start + c;
+      const double x = vecs[v]*rinv, y = vecs[nvectors + v]*rinv,
+                   z = vecs[2*(size_t)nvectors + v]*rinv;
+      if (threadIdx.x < 3) {
+        const double xi = threadIdx.x == 0 ? x : (threadIdx.x == 1 ? y : z);
+        double pw = 1;
+        for (int e = 0; e <= N; ++e) {
+          sp[threadIdx.x][e] = pw;
+          pw *= xi;
+        }
+      } else {
+        helm_radial(k*rscale, sqrt(x*x + y*y + z*z), N, sg[0], sg[1], 1);
+      }
+    }
+    __syncthreads();
+    double *o = out + (size_t)c*2*ncirc;
+    for (size_t e = threadIdx.x; e < ncirc; e += blockDim.x) {
+      const int mz = int(e % M2), my = int((e/M2) % M1), mx = int(e/((size_t)M2*M1));
+      double re = 0, im = 0;
+      if (mx + my + mz <= N) {
+        const cplx d = helm_deriv(mx, my, mz, sp[0], sp[1], sp[2], 1, sg[0], sg[1], 1, ht, N + 1);
+        re = d.re*rinv;
+        im = d.im*rinv;
+      }
+      o[e] = re;
+      o[ncirc + e] = im;
+    }
+  }
+}
+
+// }}}
+
+// {{{ M2P
+
+// One wave per target box; lane = target.  Per list-3 source box the wave
+// stages the box's multipole (complex, stored order) in LDS, each lane fills
+// its powers of xi = (t - c)/rho and g_q in its own LDS column, and
+//   pot += scale sum_s M_s rho^|s| d^s G(t - c)
+//   grad_i += scale sum_s M_s rho^|s| d^{s + e_i} G(t - c) / rho
+// (reference sumpy/e2p.py:233-343, accumulates).
+template <bool GRAD>
+__global__ __launch_bounds__(64) void m2p_helm3d(
+    sumpyhip_expn e, double k, int ntgt, const int *target_boxes, const int *starts,
+    const int *lists, const int *bts, const int *btc, const double *centers, int nboxes,
+    const double *tx, const double *ty, const double *tz, double rscale, double scale,
+    const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
+  extern __shared__ double sh[];
+  const int p = e.order, nst = e.nstored;
+  const int NQ = p + 1 + (GRAD ? 1 : 0);  // highest derivative order
+  const int lane = threadIdx.x;
+  double *ht = sh;                      // [(NQ+1)^2]
+  double *sM = ht + (NQ + 1)*(NQ + 1);  // [2*nst]
+  double *tab = sM + 2*nst;             // [5*(NQ+1)][64]: x, y, z powers, g re, g im
+  int *sid = reinterpret_cast<int *>(tab + 5*(NQ + 1)*64);  // [3*nst]
+  herm_table(ht, NQ);
+  for (int q = lane; q < 3*nst; q += 64) sid[q] = e.stored_ids[q];
+  double *xp = tab + lane, *yp = xp + (NQ + 1)*64, *zp = yp + (NQ + 1)*64;
+  double *gre = zp + (NQ + 1)*64, *gim = gre + (NQ + 1)*64;
+  const double rinv = 1.0/rscale, kappa = k*rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const int t0 = bts[tb], nt = btc[tb];
+    for (int tc = 0; tc < nt; tc += 64) {
+      const bool valid = tc + lane < nt;
+      const int t = t0 + tc + (valid ? lane : 0);
+      const double X = tx[t], Y = ty[t], Z = tz[t];
+      double pr = 0, pi = 0, ar[3] = {0, 0, 0}, ai[3] = {0, 0, 0};
+      for (int q = starts[ib]; q < starts[ib + 1]; ++q) {
+        const int sb = lists[q];
+        __syncthreads();
+        const double *M = src + 2*(size_t)(sb - sbase)*nst;
+        for (int u = lane; u < 2*nst; u += 64) sM[u] = M[u];
+        const double x = (X - centers[sb])*rinv, y = (Y - centers[nboxes + sb])*rinv,
+                     z = (Z - centers[2*(size_t)nboxes + sb])*rinv;
+        double px = 1, py = 1, pz = 1;
+        for (int u = 0; u <= NQ; ++u) {
+          xp[u*64] = px; yp[u*64] = py; zp[u*64] = pz;
+          px *= x; py *= y; pz *= z;
+        }
+        helm_radial(kappa, sqrt(x*x + y*y + z*z), NQ, gre, gim, 64);
+        __syncthreads();
+        for (int s = 0; s < nst; ++s) {
+          const int a = sid[s], b = sid[nst + s], c = sid[2*nst + s];
+          const double mr = sM[2*s], mi = sM[2*s + 1];
+          const cplx d = helm_deriv(a, b, c, xp, yp, zp, 64, gre, gim, 64, ht, NQ + 1);
+          pr += mr*d.re - mi*d.im;
+          pi += mr*d.im + mi*d.re;
+          if (GRAD) {
+            #pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+              const cplx g = helm_deriv(a + (ax == 0), b + (ax == 1), c + (ax == 2), xp, yp,
+                                        zp, 64, gre, gim, 64, ht, NQ + 1);
+              ar[ax] += mr*g.re - mi*g.im;
+              ai[ax] += mr*g.im + mi*g.re;
+            }
+          }
+        }
+      }
+      if (valid) {
+        const double f = scale*rinv;
+        if (pot) {
+          pot[2*(size_t)t] += f*pr;
+          pot[2*(size_t)t + 1] += f*pi;
+        }
+        if (GRAD) {
+          double *g[3] = {gx, gy, gz};
+          #pragma unroll
+          for (int ax = 0; ax < 3; ++ax) {
+            g[ax][2*(size_t)t] += f*rinv*ar[ax];
+            g[ax][2*(size_t)t + 1] += f*rinv*ai[ax];
+          }
+        }
+      }
+    }
+  }
+}
+
+// }}}
+
+// {{{ P2L
+
+constexpr int P2LH_CH = 64;  // sources staged per chunk
+
+// One block per target box; thread = stored local coefficient t.  The list-4
+// sources are staged in chunks: per source the powers of xi = (c - y)/rho,
+// g_q and the weights; then
+//   L_t = sum_y [w0 rho^|t| d^t G(c - y) + sum_i w_i rho^|t| d^{t + e_i} G(c - y)/rho]
+// (reference sumpy/p2e.py:268-388; overwrites the box's row).
+template <bool DIP>
+__global__ __launch_bounds__(256) void p2l_helm3d(
+    sumpyhip_expn e, double k, int ntgt, const int *target_boxes, const int *starts,
+    const int *lists, const int *bss, const int *bsc, const double *centers, int nboxes,
+    const double *sx, const double *sy, const double *sz, const double *w0,
+    const double *wx, const double *wy, const double *wz, double rscale, double *tgt,
+    int tbase) {
+  extern __shared__ double sh[];
+  const int p = e.order, nst = e.nstored;
+  const int NQ = p + (DIP ? 1 : 0);
+  const int tid = threadIdx.x;
+  double *ht = sh;                                  // [(NQ+1)^2]
+  double *tab = ht + (NQ + 1)*(NQ + 1);             // [5*(NQ+1)][CH]
+  double *sw = tab + 5*(NQ + 1)*P2LH_CH;            // [(DIP ? 8 : 2)][CH]
+  herm_table(ht, NQ);
+  const double *xp = tab + 0*(NQ + 1)*P2LH_CH, *yp = tab + 1*(NQ + 1)*P2LH_CH,
+               *zp = tab + 2*(NQ + 1)*P2LH_CH, *gre = tab + 3*(NQ + 1)*P2LH_CH,
+               *gim = tab + 4*(NQ + 1)*P2LH_CH;
+  const bool own = tid < nst;
+  const int a = own ? e.stored_ids[tid] : 0, b = own ? e.stored_ids[nst + tid] : 0,
+            c = own ? e.stored_ids[2*nst + tid] : 0;
+  const double rinv = 1.0/rscale, kappa = k*rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const double cx = centers[tb], cy = centers[nboxes + tb],
+                 cz = centers[2*(size_t)nboxes + tb];
+    double lr = 0, li = 0;
+    for (int q = starts[ib]; q < starts[ib + 1]; ++q) {
+      const int sbox = lists[q];
+      const int s0 = bss[sbox], ns = bsc[sbox];
+      for (int sc = 0; sc < ns; sc += P2LH_CH) {
+        const int n = min(P2LH_CH, ns - sc);
+        __syncthreads();
+        if (tid < n) {
+          const int s = s0 + sc + tid;
+          const double x = (cx - sx[s])*rinv, y = (cy - sy[s])*rinv, z = (cz - sz[s])*rinv;
+          double px = 1, py = 1, pz = 1;
+          for (int u = 0; u <= NQ; ++u) {
+            tab[(0*(NQ + 1) + u)*P2LH_CH + tid] = px;
+            tab[(1*(NQ + 1) + u)*P2LH_CH + tid] = py;
+            tab[(2*(NQ + 1) + u)*P2LH_CH + tid] = pz;
+            px *= x; py *= y; pz *= z;
+          }
+          helm_radial(kappa, sqrt(x*x + y*y + z*z), NQ, tab + 3*(NQ + 1)*P2LH_CH + tid,
+                      tab + 4*(NQ + 1)*P2LH_CH + tid, P2LH_CH);
+          sw[tid] = w0[2*(size_t)s];
+          sw[P2LH_CH + tid] = w0[2*(size_t)s + 1];
+          if (DIP) {
+            const double *wd[3] = {wx, wy, wz};
+            #pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+              sw[(2 + 2*ax)*P2LH_CH + tid] = wd[ax][2*(size_t)s];
+              sw[(3 + 2*ax)*P2LH_CH + tid] = wd[ax][2*(size_t)s + 1];
+            }
+          }
+        }
+        __syncthreads();
+        if (own) {
+          for (int j = 0; j < n; ++j) {
+            const cplx d = helm_deriv(a, b, c, xp + j, yp + j, zp + j, P2LH_CH, gre + j,
+                                      gim + j, P2LH_CH, ht, NQ + 1);
+            const double qr = sw[j], qi = sw[P2LH_CH + j];
+            lr += qr*d.re - qi*d.im;
+            li += qr*d.im + qi*d.re;
+            if (DIP) {
+              #pragma unroll
+              for (int ax = 0; ax < 3; ++ax) {
+                const cplx g = helm_deriv(a + (ax == 0), b + (ax == 1), c + (ax == 2),
+                                          xp + j, yp + j, zp + j, P2LH_CH, gre + j, gim + j,
+                                          P2LH_CH, ht, NQ + 1);
+                const double ur = sw[(2 + 2*ax)*P2LH_CH + j]*rinv,
+                             ui = sw[(3 + 2*ax)*P2LH_CH + j]*rinv;
+                lr += ur*g.re - ui*g.im;
+                li += ur*g.im + ui*g.re;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (own) {
+      double *o = tgt + 2*((size_t)(tb - tbase)*nst + tid);
+      o[0] = lr*rinv;
+      o[1] = li*rinv;
+    }
+  }
+}
+
+// }}}
+
+int check_helm_expn(const sumpyhip_expn *e, const char *what) {
+  SUMPY_REQUIRE(e != nullptr && e->stored_ids != nullptr, "%s: null expansion tables", what);
+  SUMPY_REQUIRE(e->order >= 0 && e->order + 2 <= HELM_MAXN, "%s: order %d out of range",
+                what, e->order);
+  SUMPY_REQUIRE(e->nstored > 0 && e->nstored <= 256, "%s: %d stored coefficients (at most 256)",
+                what, e->nstored);
+  return SUMPYHIP_OK;
+}
+
+}  // namespace
+}  // namespace sumpyhip
+
+using namespace sumpyhip;
+
+extern "C" int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt,
+                                                 const int *circ_shape, int nclasses,
+                                                 const double *translation_vectors,
+                                                 int nvectors, int class_start,
+                                                 double src_rscale, double k, double *out,
+                                                 void *stream) {
+  SUMPY_REQUIRE(circ_shape != nullptr, "m2l_class_data_helmholtz: null shape");
+  const int N = order_src + order_tgt;
+  SUMPY_REQUIRE(N >= 0 && N <= HELM_MAXN, "m2l_class_data_helmholtz: order %d > %d", N,
+                HELM_MAXN);
+  SUMPY_REQUIRE(src_rscale > 0, "m2l_class_data_helmholtz: rscale must be positive");
+  if (nclasses == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(class_start >= 0 && class_start + nclasses <= nvectors,
+                "m2l_class_data_helmholtz: class range outside the vectors");
+  const int grid = nclasses < 1024 ? nclasses : 1024;
+  helm_class_data_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(
+      N, circ_shape[0], circ_shape[1], circ_shape[2], nclasses, translation_vectors,
+      nvectors, class_start, src_rscale, k, out);
+  return check_launch("m2l_class_data_helmholtz");
+}
+
+extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+                                      const int *target_boxes, const int *source_box_starts,
+                                      const int *source_box_lists,
+                                      const int *box_target_starts,
+                                      const int *box_target_counts_nonchild,
+                                      const double *box_centers, int nboxes,
+                                      const double *tx, const double *ty, const double *tz,
+                                      double rscale, double scale,
+                                      const void *src_expansions, int src_base_ibox,
+                                      void *pot, void *gx, void *gy, void *gz,
+                                      void *stream) {
+  int rc = check_helm_expn(expn, "m2p_helmholtz");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(pot || gx, "m2p_helmholtz: no output");
+  SUMPY_REQUIRE(!gx || (gy && gz), "m2p_helmholtz: incomplete gradient outputs");
+  const int NQ = expn->order + 1 + (gx ? 1 : 0);
+  const size_t lds = sizeof(double)*((NQ + 1)*(NQ + 1) + 2*expn->nstored + 5*(NQ + 1)*64) +
+                     sizeof(int)*3*expn->nstored;
+  SUMPY_REQUIRE(lds <= 64*1024, "m2p_helmholtz: order too large");
+  const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
+  hipStream_t st = (hipStream_t)stream;
+  const double *src = static_cast<const double *>(src_expansions);
+  if (gx)
+    m2p_helm3d<true><<<grid, 64, lds, st>>>(
+        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
+        rscale, scale, src, src_base_ibox, (double *)pot, (double *)gx, (double *)gy,
+        (double *)gz);
+  else
+    m2p_helm3d<false><<<grid, 64, lds, st>>>(
+        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
+        rscale, scale, src, src_base_ibox, (double *)pot, nullptr, nullptr, nullptr);
+  return check_launch("m2p_helmholtz");
+}
+
+extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+                                      const int *target_boxes, const int *source_box_starts,
+                                      const int *source_box_lists,
+                                      const int *box_source_starts,
+                                      const int *box_source_counts_nonchild,
+                                      const double *box_centers, int nboxes,
+                                      const double *sx, const double *sy, const double *sz,
+                                      const void *w0, const void *wx, const void *wy,
+                                      const void *wz, double rscale, void *tgt_expansions,
+                                      int tgt_base_ibox, void *stream) {
+  int rc = check_helm_expn(expn, "p2l_helmholtz");
+  if (rc) return rc;
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(w0 != nullptr, "p2l_helmholtz: null charge weights");
+  SUMPY_REQUIRE(!wx || (wy && wz), "p2l_helmholtz: incomplete dipole weights");
+  const int NQ = expn->order + (wx ? 1 : 0);
+  const size_t lds = sizeof(double)*((NQ + 1)*(NQ + 1) + 5*(NQ + 1)*P2LH_CH + 8*P2LH_CH);
+  SUMPY_REQUIRE(lds <= 64*1024, "p2l_helmholtz: order too large");
+  const int nt = (expn->nstored + 63)/64*64;
+  const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
+  hipStream_t st = (hipStream_t)stream;
+  const double *c0 = static_cast<const double *>(w0);
+  if (wx)
+    p2l_helm3d<true><<<grid, nt, lds, st>>>(
+        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, c0,
+        (const double *)wx, (const double *)wy, (const double *)wz, rscale,
+        (double *)tgt_expansions, tgt_base_ibox);
+  else
+    p2l_helm3d<false><<<grid, nt, lds, st>>>(
+        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, c0,
+        nullptr, nullptr, nullptr, rscale, (double *)tgt_expansions, tgt_base_ibox);
+  return check_launch("p2l_helmholtz");
+}

@@ -21,7 +21,7 @@ from sumpy_amd import _capi
 from sumpy_amd.expansion import m2l_device_tables
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.profiling import kernel_span
-from sumpy_amd.tools import as_i32
+from sumpy_amd.tools import as_i32, expansion_k, planes, set_planes
 
 TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
 GROUP = 16  # target boxes per octant-translation group (matches the HIP kernel)
@@ -45,9 +45,14 @@ class E2EBase:
     def get_cache_key(self):
         return (type(self).__name__, self.src_expansion, self.tgt_expansion)
 
-    def _tables(self, actx):
-        return (self.src_expansion.device_tables(actx.device),
-                self.tgt_expansion.device_tables(actx.device))
+    def _tables(self, actx, src_rscale=None, tgt_rscale=None, k=0.0):
+        """Device tables; for Helmholtz (*k* nonzero) those of the two
+        expansion scales, whose PDE projections hold ``(k rscale)^2``."""
+        if not k:
+            return (self.src_expansion.device_tables(actx.device),
+                    self.tgt_expansion.device_tables(actx.device))
+        return (self.src_expansion.tables_for(actx.device, k, src_rscale),
+                self.tgt_expansion.tables_for(actx.device, k, tgt_rscale))
 
 
 class _OctantTranslation(E2EBase):
@@ -74,20 +79,35 @@ class _OctantTranslation(E2EBase):
             self._host[key] = v
         return v
 
-    def _matrices(self, actx, offsets, present, src_rscale, tgt_rscale):
-        key = (offsets.tobytes(), present.tobytes(), float(src_rscale), float(tgt_rscale))
+    def _matrices(self, actx, offsets, present, src_rscale, tgt_rscale, k=0.0):
+        key = (offsets.tobytes(), present.tobytes(), float(src_rscale), float(tgt_rscale),
+               float(k))
         frag = self._mats.get(key)
         if frag is None:
-            T = self._unit_matrices(actx, offsets, present, src_rscale, tgt_rscale)
+            T = self._unit_matrices(actx, offsets, present, src_rscale, tgt_rscale, k)
             frag = _fragment_order(T)
             self._mats[key] = frag
         return frag
 
-    def _unit_matrices(self, actx, offsets, present, src_rscale, tgt_rscale):
+    def __call__(self, actx, **kwargs):
+        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
+        k = expansion_k(self.tgt_expansion.kernel, kwargs)
+        if not tgt.is_complex():
+            return self._apply(actx, kwargs, src, tgt, k)
+        # complex (Helmholtz) expansions: the translation is real-linear with
+        # real matrices, applied to the real and the imaginary plane
+        sr, si = planes(src)
+        tr, ti = planes(tgt)
+        self._apply(actx, kwargs, sr, tr, k)
+        self._apply(actx, kwargs, si, ti, k)
+        set_planes(tgt, tr, ti)
+        return tgt
+
+    def _unit_matrices(self, actx, offsets, present, src_rscale, tgt_rscale, k=0.0):
         """T[o] (n_out x n_in) from the per-box kernel on a synthetic level:
         box 0 unused, boxes 1..8n are the 'big' boxes (parents), 8n+1..16n the
         'small' ones (children), one pair per (octant, unit vector)."""
-        st, tt = self._tables(actx)
+        st, tt = self._tables(actx, src_rscale, tgt_rscale, k)
         n_in, n_out = st.table.nstored, tt.table.nstored
         no = 8*n_in
         nb = 1 + 2*no
@@ -214,12 +234,11 @@ class E2EFromChildren(_OctantTranslation):
         self._plans[key] = plan
         return plan
 
-    def __call__(self, actx, **kwargs):
-        st, tt = self._tables(actx)
+    def _apply(self, actx, kwargs, src, tgt, k):
+        st, tt = self._tables(actx, kwargs["src_rscale"], kwargs["tgt_rscale"], k)
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
         child = as_i32(kwargs["box_child_ids"])
-        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
         if self.use_octant_matrices and tb.numel():
             plan = self._plan(tb, child, centers)
             if plan is not None:
@@ -229,7 +248,7 @@ class E2EFromChildren(_OctantTranslation):
                         mask=actx.from_numpy(plan["mask"].astype(np.int32)),
                         items=actx.from_numpy(_octant_items(plan["mask"])))
                 frag = self._matrices(actx, plan["offsets"], plan["present"],
-                                      kwargs["src_rscale"], kwargs["tgt_rscale"])
+                                      kwargs["src_rscale"], kwargs["tgt_rscale"], k)
                 return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
                                                       npairs=plan["npairs"]),
                                            frag, src, kwargs["src_base_ibox"], tgt,
@@ -300,12 +319,11 @@ class E2EFromParent(_OctantTranslation):
         self._plans[key] = plan
         return plan
 
-    def __call__(self, actx, **kwargs):
-        st, tt = self._tables(actx)
+    def _apply(self, actx, kwargs, src, tgt, k):
+        st, tt = self._tables(actx, kwargs["src_rscale"], kwargs["tgt_rscale"], k)
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
         parents = as_i32(kwargs["box_parent_ids"])
-        src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
         if self.use_octant_matrices and tb.numel():
             plan = self._plan(tb, parents, centers)
             if plan is not None:
@@ -315,7 +333,7 @@ class E2EFromParent(_OctantTranslation):
                         mask=actx.from_numpy(plan["mask"].astype(np.int32)),
                         items=actx.from_numpy(_octant_items(plan["mask"])))
                 frag = self._matrices(actx, plan["offsets"], plan["present"],
-                                      kwargs["src_rscale"], kwargs["tgt_rscale"])
+                                      kwargs["src_rscale"], kwargs["tgt_rscale"], k)
                 return self._launch_octant(actx, dict(plan["dev"], ngroups=plan["ngroups"],
                                                       npairs=plan["npairs"]),
                                            frag, src, kwargs["src_base_ibox"], tgt,
@@ -388,6 +406,10 @@ class E2EFromCSR(E2EBase):
         if not (isinstance(self.src_expansion, MultipoleExpansionBase)
                 and isinstance(self.tgt_expansion, LocalExpansionBase)):
             raise NotImplementedError("E2EFromCSR implements multipole-to-local only")
+        if self.tgt_expansion.is_complex_valued:
+            raise NotImplementedError(
+                "Helmholtz M2L runs through translation classes "
+                "(M2LUsingTranslationClassesDependentData)")
         m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
@@ -417,9 +439,21 @@ class M2LGenerateTranslationClassesDependentData(E2EBase):
         out = kwargs["m2l_translation_classes_dependent_data"]
         m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
         ncls = int(kwargs["ntranslation_classes"])
+        shape = _capi.np_i32(m["shape"])
+        if self.tgt_expansion.is_complex_valued:
+            # real and imaginary plane per class: classes 2c, 2c + 1 of the
+            # real embedding the contraction uses
+            if out.shape != (ncls, 2*m["ncirc"]):
+                raise ValueError(f"class data must have shape ({ncls}, {2*m['ncirc']})")
+            _capi.call("sumpyhip_m2l_class_data_helmholtz", self.src_expansion.order,
+                       self.tgt_expansion.order, shape.ctypes.data, ncls, _capi.ptr(vecs),
+                       int(vecs.shape[1]), int(kwargs["translation_classes_level_start"]),
+                       float(kwargs["src_rscale"]),
+                       expansion_k(self.tgt_expansion.kernel, kwargs), _capi.ptr(out),
+                       actx.stream_handle)
+            return out
         if out.shape != (ncls, m["ncirc"]):
             raise ValueError(f"class data must have shape ({ncls}, {m['ncirc']})")
-        shape = _capi.np_i32(m["shape"])
         _capi.call("sumpyhip_m2l_class_data", kernel_id(self.src_expansion.kernel),
                    self.src_expansion.order, self.tgt_expansion.order, shape.ctypes.data,
                    ncls, _capi.ptr(vecs), int(vecs.shape[1]),
@@ -441,6 +475,13 @@ class M2LPreprocessMultipole(E2EBase):
         nb = int(src.shape[0])
         if out.shape[0] != nb:
             raise ValueError("preprocessed array has the wrong number of boxes")
+        if src.is_complex():
+            # rows Mr, Mi, -Mi per box (the real embedding of the complex
+            # contraction); the packing kernel then sees 3 nb real boxes
+            re, im = planes(src)
+            src = torch.stack((re, im, -im), dim=1).reshape(3*nb, -1)
+            out = out.view(3*nb, -1)
+            nb = 3*nb
         k_src = None
         if tr.precision == "bf16x3":
             k_src = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)["k_src"]
@@ -449,7 +490,7 @@ class M2LPreprocessMultipole(E2EBase):
         _capi.call("sumpyhip_m2l_preprocess", tr.precision_id, nb, int(src.shape[1]),
                    _capi.ptr(src), int(out.shape[1]), _capi.ptr(k_src), _capi.ptr(out),
                    actx.stream_handle)
-        return out
+        return kwargs["preprocessed_src_expansions"]
 
 
 class M2LUsingTranslationClassesDependentData(E2EBase):
@@ -507,9 +548,61 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"])
         if sched["ntiles"] == 0:
             return None
+        sbase, tbase = int(kwargs["src_base_ibox"]), int(kwargs["tgt_base_ibox"])
+        if self.tgt_expansion.is_complex_valued:
+            sched = self._complex_schedule(actx, sched, sbase, tbase)
+            nb_s, nb_t = int(src.shape[0]), int(work.shape[0])
+            return dict(op=self, sched=sched, data=data.contiguous().view(2*data.shape[0], -1),
+                        src=src.view(3*nb_s, -1), sbase=0, work=work.view(2*nb_t, -1),
+                        tbase=0)
         return dict(op=self, sched=sched, data=data.contiguous(), src=src,
-                    sbase=int(kwargs["src_base_ibox"]), work=work,
-                    tbase=int(kwargs["tgt_base_ibox"]))
+                    sbase=sbase, work=work, tbase=tbase)
+
+    def _complex_schedule(self, actx, sched, sbase, tbase):
+        """The schedule of the real embedding: tile T becomes the tiles of
+        the target rows Lr (2 (t - tbase)) and Li (2 (t - tbase) + 1); class
+        slot c of a tile becomes the slots of Ar (class 2c) and Ai (2c + 1),
+        fed by the source rows 3 (s - sbase) + (0: Mr, 1: Mi, 2: -Mi):
+        Lr = Ar Mr + Ai (-Mi), Li = Ar Mi + Ai Mr."""
+        key = ("complex", id(sched), sbase, tbase)
+        out = self._schedules.get(key)
+        if out is not None:
+            return out
+        h = sched["host"]
+        tt, tss, sc, ss = (h["tile_targets"], h["tile_slot_starts"], h["slot_class"],
+                           h["slot_sources"].reshape(-1, TILE))
+        ntiles = sched["ntiles"]
+
+        def vrow(ids, base, mult, off):
+            return np.where(ids >= 0, mult*(ids.astype(np.int64) - base) + off, -1)
+
+        tiles, starts, classes, sources = [], [0], [], []
+        for t in range(ntiles):
+            tgt = tt[t*TILE:(t + 1)*TILE]
+            sl = range(tss[t], tss[t + 1])
+            for part in (0, 1):
+                tiles.append(vrow(tgt, tbase, 2, part))
+                for q in sl:
+                    c, srcs = int(sc[q]), ss[q]
+                    classes += [2*c, 2*c + 1]
+                    if part == 0:
+                        sources += [vrow(srcs, sbase, 3, 0), vrow(srcs, sbase, 3, 2)]
+                    else:
+                        sources += [vrow(srcs, sbase, 3, 1), vrow(srcs, sbase, 3, 0)]
+                starts.append(starts[-1] + 2*len(sl))
+
+        def dev(a):
+            return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(actx.device)
+
+        nslots = starts[-1]
+        out = dict(ntiles=2*ntiles, tile_targets=dev(np.concatenate(tiles)),
+                   tile_slot_starts=dev(np.array(starts)), slot_class=dev(np.array(classes)),
+                   slot_sources=dev(np.concatenate(sources) if sources
+                                    else np.zeros(0, np.int32)),
+                   nslots=nslots, npairs=4*sched["npairs"],
+                   nsplit=choose_nsplit(2*ntiles, nslots, actx.device))
+        self._schedules[key] = out
+        return out
 
     def __call__(self, actx, **kwargs):
         entry = self.level_entry(actx, **kwargs)
@@ -628,7 +721,10 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
                 tile_slot_starts=dev(tile_slot_starts), slot_class=dev(ci),
                 slot_sources=dev(slot_sources.reshape(-1)), nslots=len(ci),
                 npairs=int(len(e_src)),
-                nsplit=choose_nsplit(ntiles, len(ci), actx.device))
+                nsplit=choose_nsplit(ntiles, len(ci), actx.device),
+                host=dict(tile_targets=tile_targets, tile_slot_starts=tile_slot_starts,
+                          slot_class=ci.astype(np.int32),
+                          slot_sources=slot_sources.reshape(-1)))
 
 
 def choose_nsplit(ntiles, nslots, device):
@@ -659,10 +755,21 @@ class M2LPostprocessLocal(E2EBase):
         row_pos = None
         if tr.precision == "bf16x3":
             row_pos = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)["row_pos"]
-        _capi.call("sumpyhip_m2l_postprocess", tr.precision_id, int(tgt.shape[0]),
-                   int(tgt.shape[1]), int(work.shape[1]), _capi.ptr(work),
-                   _capi.ptr(m["tgt_degree"]), _capi.ptr(row_pos), ratio, _capi.ptr(tgt),
+        out = tgt
+        nb = int(tgt.shape[0])
+        if tgt.is_complex():
+            # work rows Lr, Li per box (the real embedding)
+            work = work.reshape(2*nb, -1)
+            out = torch.empty((2*nb, int(tgt.shape[1])), dtype=torch.float64,
+                              device=tgt.device)
+            nb = 2*nb
+        _capi.call("sumpyhip_m2l_postprocess", tr.precision_id, nb,
+                   int(out.shape[1]), int(work.shape[1]), _capi.ptr(work),
+                   _capi.ptr(m["tgt_degree"]), _capi.ptr(row_pos), ratio, _capi.ptr(out),
                    actx.stream_handle)
+        if out is not tgt:
+            v = out.view(nb//2, 2, -1)
+            set_planes(tgt, v[:, 0], v[:, 1])
         return tgt
 
 

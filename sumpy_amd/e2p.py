@@ -10,7 +10,14 @@ import torch
 from sumpy_amd import _capi
 from sumpy_amd.array_context import obj_array
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
-from sumpy_amd.tools import TargetOutputs, as_component_list, as_i32
+from sumpy_amd.tools import (
+    TargetOutputs,
+    as_component_list,
+    as_i32,
+    expansion_k,
+    planes,
+    set_planes,
+)
 
 
 class E2PBase:
@@ -38,7 +45,8 @@ class E2PBase:
         result = kwargs.get("result")
         nt = targets[0].shape[0]
         if result is None:
-            result = obj_array([torch.zeros(nt, dtype=torch.float64, device=actx.device)
+            dt = torch.complex128 if self.expansion.is_complex_valued else torch.float64
+            result = obj_array([torch.zeros(nt, dtype=dt, device=actx.device)
                                 for _ in self.kernels])
         outs = TargetOutputs(self.kernels, self.dim)
         return targets, centers, result, outs
@@ -53,17 +61,37 @@ class E2PFromSingleBox(E2PBase):
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
-        tables = self.expansion.device_tables(actx.device)
+        rscale = float(kwargs["rscale"])
+        k = expansion_k(self.expansion.kernel, kwargs)
+        tables = self.expansion.tables_for(actx.device, k, rscale)
         tb = as_i32(kwargs["target_boxes"])
         bts = as_i32(kwargs["box_target_starts"])
         btc = as_i32(kwargs["box_target_counts_nonchild"])
-        _capi.call("sumpyhip_l2p", kernel_id(self.expansion.kernel), tables.desc,
-                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts), _capi.ptr(btc),
-                   _capi.ptr(centers), int(centers.shape[1]),
-                   *[t.data_ptr() for t in targets], float(kwargs["rscale"]),
-                   self.expansion.kernel.get_global_scaling_const(),
-                   _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
-                   *ptrs, actx.stream_handle)
+        src = kwargs["src_expansions"]
+
+        def l2p(loc, out_ptrs):
+            _capi.call("sumpyhip_l2p", kernel_id(self.expansion.kernel), tables.desc,
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts), _capi.ptr(btc),
+                       _capi.ptr(centers), int(centers.shape[1]),
+                       *[t.data_ptr() for t in targets], rscale,
+                       self.expansion.kernel.get_global_scaling_const(),
+                       _capi.ptr(loc), int(kwargs["src_base_ibox"]), *out_ptrs,
+                       actx.stream_handle)
+
+        if not src.is_complex():
+            l2p(src, ptrs)
+        else:
+            # complex locals (Helmholtz): evaluation is real-linear in the
+            # coefficients with a real projection; each plane of the locals
+            # gives the same plane of the outputs (the targets of other boxes
+            # keep their values)
+            bufs = outs._bufs + [None]*(4 - len(outs._bufs))
+            pl = [planes(b) if b is not None else (None, None) for b in bufs]
+            for part, loc in enumerate(planes(src)):
+                l2p(loc, [None if b is None else p[part].data_ptr() for b, p in zip(bufs, pl)])
+            for b, (re, im) in zip(bufs, pl):
+                if b is not None:
+                    set_planes(b, re, im)
         # combined target kernels are added to the (zero-initialised) results
         # of the boxes' targets; plain kernels are written in place
         outs.finish(actx, targets, accumulate=not outs.direct)
@@ -85,6 +113,18 @@ class E2PFromCSR(E2PBase):
         lists = as_i32(kwargs["source_box_lists"])
         bts = as_i32(kwargs["box_target_starts"])
         btc = as_i32(kwargs["box_target_counts_nonchild"])
+        if self.expansion.is_complex_valued:
+            _capi.call("sumpyhip_m2p_helmholtz", tables.desc,
+                       expansion_k(self.expansion.kernel, kwargs),
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
+                       int(centers.shape[1]), *[t.data_ptr() for t in targets],
+                       float(kwargs["rscale"]),
+                       self.expansion.kernel.get_global_scaling_const(),
+                       _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
+                       *ptrs, actx.stream_handle)
+            outs.finish(actx, targets, accumulate=True)
+            return result
         _capi.call("sumpyhip_m2p", kernel_id(self.expansion.kernel), tables.desc,
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                    _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),

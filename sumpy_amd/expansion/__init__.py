@@ -125,10 +125,10 @@ class VolumeTaylorExpansionMixin:
         if self.kernel.dim != 3:
             raise NotImplementedError(
                 "volume Taylor expansions are implemented for 3D kernels")
-        if not isinstance(self.kernel.get_base_kernel(), LaplaceKernel):
+        if not isinstance(self.kernel.get_base_kernel(), (LaplaceKernel, HelmholtzKernel)):
             raise NotImplementedError(
                 "volume Taylor expansions on the GPU are implemented for the "
-                "Laplace kernel")
+                "Laplace and Helmholtz kernels")
         return get_terms_table(self.order, self.kernel.dim, self.compressed)
 
     def get_coefficient_identifiers(self):
@@ -141,7 +141,15 @@ class VolumeTaylorExpansionMixin:
         return self.table.stored_index[tuple(mi)]
 
     def device_tables(self, device, k2rscale2=0.0) -> DeviceExpnTables:
-        return device_tables(self.order, self.compressed, device, k2rscale2)
+        """Device tables; *k2rscale2* = ``(k rscale)^2`` enters the PDE
+        projection of compressed Helmholtz expansions (0 for Laplace)."""
+        if not self.compressed:
+            k2rscale2 = 0.0
+        return device_tables(self.order, self.compressed, device, float(k2rscale2))
+
+    def tables_for(self, device, k, rscale) -> DeviceExpnTables:
+        """Device tables at expansion scale *rscale* for wavenumber *k*."""
+        return self.device_tables(device, (k*rscale)**2 if k else 0.0)
 
 
 __all__ = [
@@ -197,7 +205,3 @@ def m2l_device_tables(src_order: int, tgt_order: int, compressed: bool, device: 
                 nl=tgt.nstored, nm=src.nstored)
 
 
-def _helmholtz_guard(kernel):
-    if isinstance(kernel.get_base_kernel(), HelmholtzKernel):
-        raise NotImplementedError(
-            "Helmholtz expansions are not implemented on the GPU")

@@ -6,6 +6,8 @@ they process.
 
 from __future__ import annotations
 
+import torch
+
 from sumpy_amd import _capi
 from sumpy_amd.expansion.local import LocalExpansionBase
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
@@ -15,6 +17,9 @@ from sumpy_amd.tools import (
     as_component_list,
     as_i32,
     as_strength_list,
+    expansion_k,
+    planes,
+    set_planes,
 )
 
 
@@ -40,6 +45,7 @@ class P2EBase(KernelComputation):
         centers = kwargs.pop("centers")
         _capi.require_gpu(centers)
         rscale = float(kwargs.pop("rscale"))
+        self._k = expansion_k(self.expansion.kernel, kwargs)
         strengths = as_strength_list(kwargs.pop("strengths"), self.strength_count)
         weights = SourceWeights(actx, self.source_kernels, self.strength_usage, strengths,
                                 kwargs, self.dim, self.expansion.is_complex_valued)
@@ -47,7 +53,7 @@ class P2EBase(KernelComputation):
         _capi.require_gpu(tgt)
         if not tgt.is_contiguous():
             raise ValueError("tgt_expansions must be contiguous")
-        tables = self.expansion.device_tables(actx.device)
+        tables = self.expansion.tables_for(actx.device, self._k, rscale)
         return sources, centers.contiguous(), rscale, weights, tgt, tables
 
 
@@ -61,11 +67,27 @@ class P2EFromSingleBox(P2EBase):
         source_boxes = as_i32(kwargs.pop("source_boxes"))
         bss = as_i32(kwargs.pop("box_source_starts"))
         bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
-        _capi.call("sumpyhip_p2m", kernel_id(self.expansion.kernel), tables.desc,
-                   int(source_boxes.shape[0]), _capi.ptr(source_boxes), _capi.ptr(bss),
-                   _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
-                   *[s.data_ptr() for s in sources], *weights.ptrs(3), rscale,
-                   _capi.ptr(tgt), int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+        base = int(kwargs.pop("tgt_base_ibox"))
+
+        def p2m(wptrs, out):
+            _capi.call("sumpyhip_p2m", kernel_id(self.expansion.kernel), tables.desc,
+                       int(source_boxes.shape[0]), _capi.ptr(source_boxes), _capi.ptr(bss),
+                       _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
+                       *[s.data_ptr() for s in sources], *wptrs, rscale,
+                       _capi.ptr(out), base, actx.stream_handle)
+
+        if not tgt.is_complex():
+            p2m(weights.ptrs(3), tgt)
+            return tgt
+        # complex strengths (Helmholtz): the moments are real-linear in the
+        # weights and the projection is real, so the real and imaginary
+        # planes are formed one after the other
+        w = [weights.w0] + (list(weights.wd) if weights.wd is not None else [])
+        re, im = planes(tgt)
+        for part, out in ((0, re), (1, im)):
+            ws = [torch.view_as_real(x)[..., part].contiguous() for x in w]
+            p2m([t.data_ptr() for t in ws] + [None]*(4 - len(ws)), out)
+        set_planes(tgt, re, im)
         return tgt
 
 
@@ -83,6 +105,14 @@ class P2EFromCSR(P2EBase):
         lists = as_i32(kwargs.pop("source_box_lists"))
         bss = as_i32(kwargs.pop("box_source_starts"))
         bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
+        if tgt.is_complex():
+            _capi.call("sumpyhip_p2l_helmholtz", tables.desc, self._k,
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
+                       int(centers.shape[1]), *[s.data_ptr() for s in sources],
+                       *weights.ptrs(3), rscale, _capi.ptr(tgt),
+                       int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+            return tgt
         _capi.call("sumpyhip_p2l", kernel_id(self.expansion.kernel), tables.desc,
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                    _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),

@@ -218,3 +218,30 @@ def kernel_k(kernel, kwargs):
     if isinstance(k, torch.Tensor):
         k = k.item()
     return complex(k)
+
+
+def expansion_k(kernel, kwargs):
+    """Real Helmholtz parameter of an FMM expansion (0 for Laplace).
+
+    The Helmholtz expansions here split complex coefficients into a real and
+    an imaginary plane that the real translation kernels process one after
+    the other; that needs the PDE projection (the ``(k rho)^2`` terms) to be
+    real, hence a real ``k``."""
+    k = kernel_k(kernel, kwargs)
+    if k.imag != 0:
+        raise NotImplementedError(
+            "Helmholtz expansions on the GPU need a real wavenumber")
+    return k.real
+
+
+def planes(z: torch.Tensor):
+    """Contiguous real and imaginary planes of a complex tensor."""
+    v = torch.view_as_real(z)
+    return v[..., 0].contiguous(), v[..., 1].contiguous()
+
+
+def set_planes(z: torch.Tensor, re: torch.Tensor, im: torch.Tensor):
+    """Write real and imaginary planes back into the complex tensor *z*."""
+    v = torch.view_as_real(z)
+    v[..., 0].copy_(re)
+    v[..., 1].copy_(im)

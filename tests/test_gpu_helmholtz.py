@@ -1,0 +1,138 @@
+"""Helmholtz-3D FMM on the GPU (``HelmholtzKernel(3)``, real k, complex
+strengths): translation-class data against the oracle's restatement of the
+reference's radial recursion, the whole pass against the oracle-driven FMM
+(same tree, every stage from the oracle; lists 3 and 4 populated on the
+sphere), and accuracy against the FP64 direct sum for the plain and the
+complex64-compute forms."""
+
+from functools import partial
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import helmholtz as oh
+from oracle_fmm import oracle_fmm
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b))/np.linalg.norm(np.asarray(b))
+
+
+def sphere(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((3, n))
+    x /= np.linalg.norm(x, axis=0)
+    return x*(1 + 1e-3*rng.standard_normal(n)), (rng.standard_normal(n)
+                                                 + 1j*rng.standard_normal(n))
+
+
+def run_helm(actx, src, q, order, k, max_particles=30, m2l_precision="fp64",
+             p2p_precision="fp64"):
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import FMMTraversalBuilder, TreeBuilder
+    knl = HelmholtzKernel(3)
+    tree, _ = TreeBuilder(actx)(actx, actx.from_numpy(src), max_particles_in_box=max_particles)
+    trav, _ = FMMTraversalBuilder(actx)(actx, tree)
+    tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                m2l_translation_override=tr),
+        [knl], exclude_self=True, p2p_precision=p2p_precision)
+    wr = SumpyExpansionWrangler(
+        ti, trav, np.complex128, fmm_level_to_order=lambda kn, a, t, lev: order,
+        kernel_extra_kwargs={"k": k},
+        self_extra_kwargs={"target_to_source": actx.from_numpy(
+            np.arange(src.shape[1], dtype=np.int32))})
+    pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+    return actx.to_numpy(pot), tree, trav
+
+
+@pytest.mark.parametrize("order", [4, 12])
+def test_class_data(actx, order):
+    from sumpy_amd.e2e import M2LGenerateTranslationClassesDependentData
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.expansion.terms import get_terms_table, m2l_circulant_shape
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    import torch
+    rng = np.random.default_rng(order)
+    vecs = rng.integers(-3, 4, (3, 40)).astype(np.float64)
+    vecs = vecs[:, np.abs(vecs).max(axis=0) >= 2]*0.25
+    ncls = vecs.shape[1]
+    rs, k = 0.25*2/order, 7.0
+    tab = get_terms_table(order, 3, True)
+    shape = m2l_circulant_shape(tab, tab)
+    ncirc = int(np.prod(shape))
+
+    def gpu(knl, kk, width):
+        op = M2LGenerateTranslationClassesDependentData(
+            LinearPDEConformingVolumeTaylorMultipoleExpansion(knl, order),
+            LinearPDEConformingVolumeTaylorLocalExpansion(knl, order))
+        out = torch.zeros((ncls, width), dtype=torch.float64, device=actx.device)
+        op(actx, src_rscale=rs, translation_classes_level_start=0, ntranslation_classes=ncls,
+           m2l_translation_classes_dependent_data=out,
+           m2l_translation_vectors=actx.from_numpy(vecs), ntranslation_vectors=ncls, k=kk)
+        return actx.to_numpy(out)
+
+    got = gpu(HelmholtzKernel(3), k, 2*ncirc).reshape(ncls, 2, ncirc)
+    ref = oh.class_data(order, order, vecs, rs, k, shape)
+    scale = np.abs(ref).max(axis=2, keepdims=True)
+    assert np.abs(got - ref).max() <= 1e-10*scale.max()
+    assert (np.abs(got - ref)/scale).max() < 1e-9
+    # k = 0: the Laplace class data (the stable recurrence of
+    # derivative_taker.py:227-259), imaginary plane zero.  The radial sum has
+    # alternating terms up to ~1e7 times the result at derivative order 24, as
+    # the reference's Helmholtz recursion has, so the bound is relative to
+    # each class's largest value.
+    z = gpu(HelmholtzKernel(3), 0.0, 2*ncirc).reshape(ncls, 2, ncirc)
+    lap = gpu(LaplaceKernel(3), None, ncirc)
+    lscale = np.abs(lap).max(axis=1, keepdims=True)
+    assert (np.abs(z[:, 0] - lap)/lscale).max() < (1e-14 if order <= 4 else 1e-9)
+    assert np.abs(z[:, 1]).max() == 0
+
+
+@pytest.mark.parametrize("case", ["uniform", "sphere"])
+def test_fmm_matches_oracle_fmm(actx, case):
+    if case == "uniform":
+        rng = np.random.default_rng(1)
+        src = rng.random((3, 1500))
+        q = rng.standard_normal(1500) + 1j*rng.standard_normal(1500)
+        k = 2.0
+    else:
+        src, q = sphere(1500, 2)
+        k = 1.5
+    order = 6
+    pot, tree, trav = run_helm(actx, src, q, order, k)
+    if case == "sphere":
+        vh = trav.host
+        assert sum(len(t) for t, _, _ in vh["sep_smaller"]) > 0
+        assert vh["from_sep_bigger_starts"][-1] > 0
+    ref = oracle_fmm(tree, trav, q, order, exclude_self=True, k=k)
+    assert rel(pot, ref) < 1e-11
+
+
+def test_fmm_accuracy(actx):
+    """k = 10 on the jittered sphere: the plain form converges with the order
+    (Taylor truncation at k h ~ 5 on the coarsest interacting boxes gives
+    3e-2 at p = 6, 3e-5 at p = 12), and the complex64-compute form (FP32 M2L
+    and near field) stays within twice the plain form's error."""
+    src, q = sphere(6000, 3)
+    k = 10.0
+    direct, _ = oracle.p2p_dense(13, src, src, q, t2s=np.arange(src.shape[1]), k=k)
+    e = {}
+    for order, precision in ((6, "fp64"), (12, "fp64"), (12, "fp32")):
+        pot, _, _ = run_helm(actx, src, q, order, k, max_particles=64,
+                             m2l_precision=precision, p2p_precision=precision)
+        e[order, precision] = rel(pot, direct)
+    assert e[12, "fp64"] < e[6, "fp64"]/100
+    assert e[12, "fp64"] < 1e-4, e
+    assert e[12, "fp32"] <= 2*e[12, "fp64"], e
