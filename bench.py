@@ -72,15 +72,16 @@ def make_workload(n, distribution, seed=0):
     return pts, q
 
 
-def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision):
+def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None):
+    """Laplace-3D wrangler, or Helmholtz-3D (complex) when *k* is given."""
     from functools import partial
 
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
     from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
     from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
     from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
-    from sumpy_amd.kernel import LaplaceKernel
-    knl = LaplaceKernel(3)
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    knl = LaplaceKernel(3) if k is None else HelmholtzKernel(3)
     tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
     ti = SumpyTreeIndependentDataForWrangler(
         actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
@@ -89,8 +90,9 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision):
         [knl], exclude_self=True, p2p_precision=p2p_precision)
     import torch
     t2s = torch.arange(tree.ntargets, dtype=torch.int32, device=actx.device)
-    return SumpyExpansionWrangler(ti, trav, np.float64,
-                                  fmm_level_to_order=lambda k, a, t, lev: order,
+    return SumpyExpansionWrangler(ti, trav, np.float64 if k is None else np.complex128,
+                                  fmm_level_to_order=lambda kn, a, t, lev: order,
+                                  kernel_extra_kwargs=None if k is None else {"k": k},
                                   self_extra_kwargs={"target_to_source": t2s})
 
 
@@ -292,6 +294,126 @@ def qbx_bench(actx, steps):
             "forward_gpairs_per_s": round(npairs/(ms_f*1e-3)/1e9, 1)}
 
 
+def make_helmholtz_batch(nproblems=64, seed=0):
+    """64 problems, N_i = round(2^U(10, 16)) points on the unit sphere with
+    1e-3 radial jitter, complex N(0, 1) strengths."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(2**rng.uniform(10, 16, nproblems)).astype(np.int64)
+    pts, qs = [], []
+    for n in sizes:
+        x = rng.standard_normal((3, n))
+        x /= np.linalg.norm(x, axis=0)
+        pts.append(x*(1 + 1e-3*rng.standard_normal(n)))
+        qs.append(rng.standard_normal(n) + 1j*rng.standard_normal(n))
+    return pts, np.concatenate(qs)
+
+
+def helmholtz_batched(args):
+    """Batched Helmholtz-3D FMM (k = 10, p = 12): the 64 problems of
+    :func:`make_helmholtz_batch` as one forest, one launch per stage for all
+    of them; complex128 plain form against the complex64-compute form (FP32
+    MFMA M2L, FP32 near field with FP64 sums), both replayed as HIP graphs."""
+    import torch
+
+    from sumpy_amd import P2P, profiling
+    from sumpy_amd.array_context import TorchArrayContext
+    from sumpy_amd.drive import capture_drive_fmm, drive_fmm
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import build_forest
+
+    actx = TorchArrayContext("cuda")
+    k, order = 10.0, args.order
+    pts, q = make_helmholtz_batch()
+    t0 = time.perf_counter()
+    tree, trav, offs = build_forest(actx, pts, args.max_particles)
+    t_tree = time.perf_counter() - t0
+    log(f"forest: {len(pts)} problems, {offs[-1]} points, {tree.nboxes} boxes, "
+        f"{tree.nlevels} levels, built in {t_tree:.2f}s")
+    q_dev = actx.from_numpy(q)
+    forms = {"complex128": make_wrangler(actx, tree, trav, order, "fp64", "fp64", k),
+             "complex64": make_wrangler(actx, tree, trav, order, "fp32", "fp32", k)}
+    results = {}
+    for name, wr in forms.items():
+        t0 = time.perf_counter()
+        wr.multipole_to_local_precompute()
+        pots = drive_fmm(actx, wr, [q_dev])
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        ms, wall, pots = time_passes(actx, lambda wr=wr: drive_fmm(actx, wr, [q_dev]),
+                                     args.steps, args.warmup)
+        graph = capture_drive_fmm(actx, wr, [q_dev])
+        gms, _, gout = time_passes(actx, graph.replay, args.steps, args.warmup)
+        same = bool(torch.equal(gout[0], pots[0]))
+        results[name] = dict(eager_ms=ms, graph_ms=gms, graph_ok=same,
+                             ms=gms if same else ms, pot=pots[0], setup_s=setup)
+        log(f"{name}: {ms:.3f} ms/pass eager, {gms:.3f} ms/pass as a HIP graph "
+            f"(replay {'==' if same else '!='} eager), setup {setup:.2f}s")
+        profiling.enable(True)
+        for _ in range(args.steps):
+            drive_fmm(actx, wr, [q_dev])
+        results[name]["spans"] = profiling.summary()
+        profiling.enable(False)
+        log(f"{name} kernel spans: " + ", ".join(
+            f"{a}: {b['ms']/args.steps:.3f} ms" for a, b in results[name]["spans"].items()))
+    # accuracy: direct sums over each sampled target's own problem
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(offs[-1], size=min(args.nsample, offs[-1]), replace=False))
+    prob = np.searchsorted(offs, sample, side="right") - 1
+    knl = HelmholtzKernel(3)
+    direct = np.zeros(len(sample), np.complex128)
+    p2p = P2P([knl], exclude_self=True, value_dtypes=[np.complex128])
+    for i in np.unique(prob):
+        sel = np.nonzero(prob == i)[0]
+        loc = (sample[sel] - offs[i]).astype(np.int32)
+        d, = p2p(actx, actx.from_numpy(pts[i][:, loc]), actx.from_numpy(pts[i]),
+                 [actx.from_numpy(q[offs[i]:offs[i + 1]])],
+                 target_to_source=actx.from_numpy(loc), k=k)
+        direct[sel] = actx.to_numpy(d)
+    for name in forms:
+        pot = actx.to_numpy(results[name]["pot"])[sample]
+        results[name]["rel_l2_err"] = float(np.linalg.norm(pot - direct)/np.linalg.norm(direct))
+        log(f"{name}: rel. L2 error vs direct on {len(sample)} targets "
+            f"{results[name]['rel_l2_err']:.3e}")
+    mixed, plain = results["complex64"], results["complex128"]
+    sp = mixed["spans"]
+    npair = m2l_pairs(trav)
+    nl = (order + 1)**2
+    m2l = sp.get("m2l_apply_fp32")
+    roofline = None
+    if m2l:
+        achieved = 8.0*nl*nl*npair*args.steps/(m2l["ms"]*1e-3)/1e12
+        roofline = {"bound": "mfma", "kernel": "m2l_apply_fp32 (complex as a real 2x2 "
+                    "block product)", "achieved": round(achieved, 3),
+                    "peak": PEAKS["mfma_fp32"], "unit": "TFLOP/s",
+                    "frac": round(achieved/PEAKS["mfma_fp32"], 4), "traffic": None}
+    line = {
+        "metric": "Helmholtz-3D batched FMM wall time, 64 problems, k=10, p=12",
+        "value": round(mixed["ms"], 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(mixed["ms"], 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64/c128 mixed (FP32 MFMA M2L on the real embedding, FP32 near field "
+                 "with FP64 sums, FP64 expansions)",
+        "data": "synthetic",
+        "config": {"workload": "helmholtz3d_batched_sphere", "problems": len(pts),
+                   "n_total": int(offs[-1]), "k": k, "order": order,
+                   "max_particles_in_box": args.max_particles, "nboxes": tree.nboxes,
+                   "nlevels": tree.nlevels, "m2l_pairs": npair,
+                   "p2p_pairs": p2p_pairs(tree, trav)},
+        "complex128_ms_per_step": round(plain["ms"], 4),
+        "eager_ms_per_step_by_form": {a: round(b["eager_ms"], 4) for a, b in results.items()},
+        "hip_graph": all(b["graph_ok"] for b in results.values()),
+        "speedup_complex64_vs_complex128": round(plain["ms"]/mixed["ms"], 3),
+        "rel_l2_err": {a: b["rel_l2_err"] for a, b in results.items()},
+        "kernel_ms_per_step": {a: {c: round(d["ms"]/args.steps, 4) for c, d in b["spans"].items()}
+                               for a, b in results.items()},
+        "setup_s": {"forest": round(t_tree, 3),
+                    **{f"m2l_precompute_{a}": round(b["setup_s"], 3) for a, b in results.items()}},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,12 +430,20 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="one warm-up and one measured pass per form, then a 256 MiB "
                          "device copy (counter calibration); for rocprofv3 --pmc runs")
+    ap.add_argument("--workload", default="laplace", choices=["laplace", "helmholtz-batched"],
+                    help="laplace: the headline N=2^20 p=10 FMM; helmholtz-batched: 64 "
+                         "Helmholtz problems (k=10, p=12) in one pass")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "helmholtz-batched":
+        if args.order == 10:
+            args.order = 12
+        helmholtz_batched(args)
+        return
 
     import torch
 

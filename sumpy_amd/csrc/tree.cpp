@@ -103,10 +103,25 @@ struct Builder {
 
 extern "C" {
 
+void *sumpytree_build_in_box(int dim, long nsrc, const double *src, long ntgt,
+                             const double *tgt, int max_particles_in_box, int max_levels,
+                             const double *box_min, double box_extent);
+
 // Builds the tree and traversal.  src/tgt are SoA (dim x n, row-major);
 // tgt == nullptr means the targets are the sources.
 void *sumpytree_build(int dim, long nsrc, const double *src, long ntgt,
                       const double *tgt, int max_particles_in_box, int max_levels) {
+  return sumpytree_build_in_box(dim, nsrc, src, ntgt, tgt, max_particles_in_box, max_levels,
+                                nullptr, 0);
+}
+
+// As sumpytree_build, with the root box given (box_min, cubic extent; every
+// particle must lie inside) instead of the particles' bounding box: trees of
+// several problems built in one root box share box sizes and centre lattice
+// level by level, so they can be merged into one forest.
+void *sumpytree_build_in_box(int dim, long nsrc, const double *src, long ntgt,
+                             const double *tgt, int max_particles_in_box, int max_levels,
+                             const double *box_min, double box_extent) {
   Result *res = new Result;
   res->dim = dim;
   if (dim != 2 && dim != 3) { res->error = "dim must be 2 or 3"; return res; }
@@ -126,6 +141,16 @@ void *sumpytree_build(int dim, long nsrc, const double *src, long ntgt,
   for (int d = 0; d < dim; ++d) ext = std::max(ext, hi[d] - lo[d]);
   if (ext == 0) ext = 1;
   ext *= 1 + 1e-4;
+  if (box_min) {
+    for (int d = 0; d < dim; ++d) {
+      if (lo[d] < box_min[d] || hi[d] >= box_min[d] + box_extent) {
+        res->error = "particles outside the given root box";
+        return res;
+      }
+      lo[d] = box_min[d];
+    }
+    ext = box_extent;
+  }
   res->root_extent = ext;
   for (int d = 0; d < dim; ++d) res->bbox_min[d] = lo[d];
   // }}}

@@ -35,6 +35,10 @@ def _treelib():
         lib.sumpytree_build.argtypes = [
             ctypes.c_int, ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
             ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        lib.sumpytree_build_in_box.restype = ctypes.c_void_p
+        lib.sumpytree_build_in_box.argtypes = [
+            ctypes.c_int, ctypes.c_long, ctypes.c_void_p, ctypes.c_long,
+            ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
         lib.sumpytree_error.restype = ctypes.c_char_p
         lib.sumpytree_error.argtypes = [ctypes.c_void_p]
         lib.sumpytree_scalars.argtypes = [
@@ -171,17 +175,22 @@ _TRAV_INT_FIELDS = [
     "from_sep_siblings_translation_classes_level_starts"]
 
 
-def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=30,
-                             max_levels=None):
-    """Builds tree + traversal + translation classes in one host pass."""
+def _build_host(sources, targets=None, max_particles_in_box=30, max_levels=None,
+                root_box=None):
+    """One host pass of the C++ builder: (scalars, tree arrays, traversal
+    arrays).  *root_box* = (box_min, extent) fixes the root box."""
     src = np.ascontiguousarray(_to_host_coords(sources))
     dim, nsrc = src.shape
     tgt = None if targets is None else np.ascontiguousarray(_to_host_coords(targets))
     lib = _treelib()
-    h = lib.sumpytree_build(
-        dim, nsrc, src.ctypes.data, 0 if tgt is None else tgt.shape[1],
-        None if tgt is None else tgt.ctypes.data, int(max_particles_in_box),
-        int(max_levels or 0))
+    common = (dim, nsrc, src.ctypes.data, 0 if tgt is None else tgt.shape[1],
+              None if tgt is None else tgt.ctypes.data, int(max_particles_in_box),
+              int(max_levels or 0))
+    if root_box is None:
+        h = lib.sumpytree_build(*common)
+    else:
+        bmin = np.ascontiguousarray(root_box[0], np.float64)
+        h = lib.sumpytree_build_in_box(*common, bmin.ctypes.data, float(root_box[1]))
     try:
         err = lib.sumpytree_error(h)
         if err:
@@ -209,26 +218,33 @@ def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=3
                 _get_int(h, f"target_boxes_sep_smaller_by_source_level_{lev}"),
                 _get_int(h, f"from_sep_smaller_by_level_starts_{lev}"),
                 _get_int(h, f"from_sep_smaller_by_level_lists_{lev}")))
+        vh["sep_smaller"] = sep_smaller
     finally:
         lib.sumpytree_free(h)
+    return dict(dim=dim, nlevels=nlevels, root_extent=root_extent, bb_min=bb_min,
+                same=tgt is None), th, vh
 
-    tree = Tree(dimensions=dim, nlevels=nlevels, root_extent=root_extent,
-                bounding_box=(bb_min, bb_min + root_extent),
-                sources_are_targets=tgt is None, host=th)
+
+def _wrap(actx, sc, th, vh):
+    """Tree / traversal / translation-class objects over the host arrays."""
+    dim, nlevels = sc["dim"], sc["nlevels"]
+    tree = Tree(dimensions=dim, nlevels=nlevels, root_extent=sc["root_extent"],
+                bounding_box=(sc["bb_min"], sc["bb_min"] + sc["root_extent"]),
+                sources_are_targets=sc["same"], host=th)
     dev = {k: actx.from_numpy(v) for k, v in th.items()}
-    if tgt is None:
+    if sc["same"]:
         dev["targets"] = dev["sources"]
     tree.__dict__["_dev"] = dev
 
+    sep_smaller = vh["sep_smaller"]
     trav = FMMTraversalInfo(tree=tree, host=vh)
     tdev = {k: actx.from_numpy(v) for k, v in vh.items()
-            if not k.startswith("from_sep_siblings_translation")}
+            if not k.startswith("from_sep_siblings_translation") and k != "sep_smaller"}
     tdev["target_boxes_sep_smaller_by_source_level"] = [
         actx.from_numpy(t) for t, _, _ in sep_smaller]
     tdev["from_sep_smaller_by_level"] = [
         _LevelCSR(actx.from_numpy(s), actx.from_numpy(li)) for _, s, li in sep_smaller]
     trav.__dict__["_dev"] = tdev
-    trav.host["sep_smaller"] = sep_smaller
 
     tcd = TranslationClassesData(
         from_sep_siblings_translation_classes=actx.from_numpy(
@@ -239,6 +255,199 @@ def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=3
             vh["from_sep_siblings_translation_classes_level_starts"]))
     trav.__dict__["_translation_classes"] = tcd
     return tree, trav
+
+
+def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=30,
+                             max_levels=None):
+    """Builds tree + traversal + translation classes in one host pass."""
+    sc, th, vh = _build_host(sources, targets, max_particles_in_box, max_levels)
+    return _wrap(actx, sc, th, vh)
+
+
+# {{{ forests: many independent problems in one tree
+
+def build_forest(actx, sources_list, max_particles_in_box=30, max_levels=None):
+    """One tree and traversal over several independent problems (sources =
+    targets in each), so that every FMM stage processes all of them in one
+    launch.
+
+    Each problem gets its own octree, all in one common root box (the union
+    of the problems' bounding boxes): box sizes, expansion scales and the
+    translation-class vectors then agree level by level.  The trees are merged
+    level-major -- the boxes of level l of problem 0, then of problem 1, ... --
+    and every list of the traversal is the concatenation of the problems'
+    lists in that order, with box numbers mapped; no list couples two
+    problems.  Particles are concatenated problem by problem: user index
+    ``offsets[i] + j`` is particle ``j`` of problem ``i``.
+
+    Returns ``(tree, traversal, offsets)``."""
+    pts = [np.ascontiguousarray(_to_host_coords(s)) for s in sources_list]
+    if not pts:
+        raise ValueError("no problems")
+    dim = pts[0].shape[0]
+    lo = np.min([p.min(axis=1) for p in pts if p.shape[1]], axis=0)
+    hi = np.max([p.max(axis=1) for p in pts if p.shape[1]], axis=0)
+    ext = float((hi - lo).max())*(1 + 1e-4) or 1.0
+    parts = [_build_host(p, None, max_particles_in_box, max_levels, (lo, ext))
+             for p in pts]
+    sc, th, vh = _merge_forest(parts, dim)
+    sc["bb_min"] = lo
+    tree, trav = _wrap(actx, sc, th, vh)
+    offsets = np.concatenate([[0], np.cumsum([p.shape[1] for p in pts])]).astype(np.int64)
+    return tree, trav, offsets
+
+
+def _merge_forest(parts, dim):
+    nlev = max(sc["nlevels"] for sc, _, _ in parts)
+    np_ = len(parts)
+    lbs = [np.concatenate([th["level_start_box_nrs"],
+                           np.full(nlev + 1 - len(th["level_start_box_nrs"]),
+                                   th["level_start_box_nrs"][-1])]) for _, th, _ in parts]
+    # new box number: level-major, then problem, then the old order
+    cnt = np.array([[lb[l + 1] - lb[l] for l in range(nlev)] for lb in lbs])  # (np, nlev)
+    lstart = np.concatenate([[0], np.cumsum(cnt.sum(axis=0))])
+    first = lstart[:-1][None, :] + np.concatenate(
+        [np.zeros((1, nlev), np.int64), np.cumsum(cnt, axis=0)[:-1]], axis=0)
+    maps = []
+    for i, (_, th, _) in enumerate(parts):
+        lev = th["box_levels"].astype(np.int64)
+        old = np.arange(len(lev))
+        maps.append((first[i, lev] + old - lbs[i][lev]).astype(np.int64))
+    nboxes = int(lstart[-1])
+    nsrc = [th["sources"].shape[1] for _, th, _ in parts]
+    soff = np.concatenate([[0], np.cumsum(nsrc)]).astype(np.int64)
+
+    def m(i, a):
+        return maps[i][np.asarray(a, np.int64)]
+
+    th = {}
+    th["level_start_box_nrs"] = lstart.astype(np.int32)
+    for name in ("box_source_starts", "box_target_starts"):
+        out = np.zeros(nboxes, np.int64)
+        for i, (_, t, _) in enumerate(parts):
+            out[maps[i]] = t[name] + soff[i]
+        th[name] = out.astype(np.int32)
+    for name in ("box_source_counts_cumul", "box_source_counts_nonchild",
+                 "box_target_counts_cumul", "box_target_counts_nonchild", "box_levels",
+                 "box_flags"):
+        out = np.zeros(nboxes, np.int64)
+        for i, (_, t, _) in enumerate(parts):
+            out[maps[i]] = t[name]
+        th[name] = out.astype(np.int32)
+    par = np.zeros(nboxes, np.int64)
+    child = np.zeros((2**dim, nboxes), np.int64)
+    centers = np.zeros((dim, nboxes))
+    for i, (_, t, _) in enumerate(parts):
+        par[maps[i]] = m(i, t["box_parent_ids"])
+        c = t["box_child_ids"].astype(np.int64)
+        child[:, maps[i]] = np.where(c != 0, maps[i][c], 0)
+        centers[:, maps[i]] = t["box_centers"]
+    th["box_parent_ids"] = par.astype(np.int32)
+    th["box_child_ids"] = child.astype(np.int32)
+    th["box_centers"] = centers
+    th["user_source_ids"] = np.concatenate(
+        [t["user_source_ids"] + soff[i] for i, (_, t, _) in enumerate(parts)]).astype(np.int32)
+    th["sorted_target_ids"] = np.concatenate(
+        [t["sorted_target_ids"] + soff[i] for i, (_, t, _) in enumerate(parts)]).astype(np.int32)
+    th["sources"] = np.concatenate([t["sources"] for _, t, _ in parts], axis=1)
+    th["targets"] = th["sources"]
+
+    def level_list(name, starts_name):
+        """A box list sorted by level: level by level, problem by problem."""
+        out, st = [], [0]
+        for lev in range(nlev):
+            for i, (_, _, v) in enumerate(parts):
+                ls = v[starts_name]
+                if lev + 1 < len(ls):
+                    out.append(m(i, v[name][ls[lev]:ls[lev + 1]]))
+            st.append(sum(len(o) for o in out))
+        return np.concatenate(out).astype(np.int32), np.array(st, np.int32)
+
+    def csr_rows(starts_name, lists_name, owner_starts):
+        """The CSR rows of a list keyed by a level-sorted box list, in the
+        merged order of that box list."""
+        out_starts, out_lists, n = [0], [], 0
+        for lev in range(nlev):
+            for i, (_, _, v) in enumerate(parts):
+                ls = v[owner_starts]
+                if lev + 1 >= len(ls):
+                    continue
+                s = v[starts_name]
+                r0, r1 = ls[lev], ls[lev + 1]
+                seg = v[lists_name][s[r0]:s[r1]]
+                out_lists.append(m(i, seg))
+                out_starts.extend((s[r0 + 1:r1 + 1] - s[r0] + n).tolist())
+                n += int(s[r1] - s[r0])
+        lists = np.concatenate(out_lists) if out_lists else np.zeros(0, np.int64)
+        return np.array(out_starts, np.int32), lists.astype(np.int32)
+
+    vh = {}
+    for name, starts in (("source_boxes", "level_start_source_box_nrs"),
+                         ("source_parent_boxes", "level_start_source_parent_box_nrs"),
+                         ("target_boxes", "level_start_target_box_nrs"),
+                         ("target_or_target_parent_boxes",
+                          "level_start_target_or_target_parent_box_nrs")):
+        vh[name], vh[starts] = level_list(name, starts)
+    vh["neighbor_source_boxes_starts"], vh["neighbor_source_boxes_lists"] = csr_rows(
+        "neighbor_source_boxes_starts", "neighbor_source_boxes_lists",
+        "level_start_target_box_nrs")
+    for pre in ("from_sep_siblings", "from_sep_bigger"):
+        vh[pre + "_starts"], vh[pre + "_lists"] = csr_rows(
+            pre + "_starts", pre + "_lists", "level_start_target_or_target_parent_box_nrs")
+    # translation classes: one class per distinct vector per level, numbered
+    # in order of first appearance over the problems
+    cls_all, vec_levels, cls_starts = [], [], [0]
+    for lev in range(nlev):
+        key_to_id, vecs_lev = {}, []
+        for i, (sc, _, v) in enumerate(parts):
+            ls = v["level_start_target_or_target_parent_box_nrs"]
+            if lev + 1 >= len(ls):
+                continue
+            cs = v["from_sep_siblings_translation_classes_level_starts"]
+            vec = v["from_sep_siblings_translation_class_to_distance_vector"]
+            h = sc["root_extent"]*2.0**-lev
+            s = v["from_sep_siblings_starts"]
+            seg = v["from_sep_siblings_translation_classes"][s[ls[lev]]:s[ls[lev + 1]]]
+            # this problem's classes of the level -> merged ids (by vector)
+            c0 = int(cs[lev]) if lev < len(cs) - 1 else 0
+            c1 = int(cs[lev + 1]) if lev < len(cs) - 1 else 0
+            lut = np.empty(c1 - c0, np.int64)
+            for c in range(c0, c1):
+                key = tuple(np.rint(vec[:, c]/h).astype(np.int64).tolist())
+                cid = key_to_id.get(key)
+                if cid is None:
+                    cid = len(key_to_id)
+                    key_to_id[key] = cid
+                    vecs_lev.append(vec[:, c])
+                lut[c - c0] = cid
+            cls_all.append(lut[seg.astype(np.int64) - c0] + cls_starts[-1])
+        cls_starts.append(cls_starts[-1] + len(key_to_id))
+        vec_levels.extend(vecs_lev)
+    vh["from_sep_siblings_translation_classes"] = (
+        np.concatenate(cls_all) if cls_all else np.zeros(0)).astype(np.int32)
+    vh["from_sep_siblings_translation_classes_level_starts"] = np.array(cls_starts, np.int32)
+    vh["from_sep_siblings_translation_class_to_distance_vector"] = (
+        np.array(vec_levels).T.reshape(dim, -1) if vec_levels else np.zeros((dim, 0)))
+    # list 3, by source level
+    sep = []
+    for lev in range(nlev):
+        tb, st, li, n = [], [0], [], 0
+        for i, (_, _, v) in enumerate(parts):
+            if lev >= len(v["sep_smaller"]):
+                continue
+            t, s, lst = v["sep_smaller"][lev]
+            tb.append(m(i, t))
+            li.append(m(i, lst[s[0]:s[-1]]))
+            st.extend((s[1:] - s[0] + n).tolist())
+            n += int(s[-1] - s[0])
+        sep.append((np.concatenate(tb).astype(np.int32) if tb else np.zeros(0, np.int32),
+                    np.array(st, np.int32),
+                    np.concatenate(li).astype(np.int32) if li else np.zeros(0, np.int32)))
+    vh["sep_smaller"] = sep
+    sc = dict(dim=dim, nlevels=nlev, root_extent=parts[0][0]["root_extent"], same=True)
+    return sc, th, vh
+
+# }}}
 
 
 class TreeBuilder:

@@ -29,6 +29,26 @@ def sphere(n, seed):
                                                  + 1j*rng.standard_normal(n))
 
 
+def helm_wrangler(actx, tree, trav, order, k, m2l_precision="fp64", p2p_precision="fp64"):
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    knl = HelmholtzKernel(3)
+    tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                m2l_translation_override=tr),
+        [knl], exclude_self=True, p2p_precision=p2p_precision)
+    return SumpyExpansionWrangler(
+        ti, trav, np.complex128, fmm_level_to_order=lambda kn, a, t, lev: order,
+        kernel_extra_kwargs={"k": k},
+        self_extra_kwargs={"target_to_source": actx.from_numpy(
+            np.arange(tree.ntargets, dtype=np.int32))})
+
+
 def run_helm(actx, src, q, order, k, max_particles=30, m2l_precision="fp64",
              p2p_precision="fp64"):
     from sumpy_amd.drive import drive_fmm
@@ -136,3 +156,50 @@ def test_fmm_accuracy(actx):
     assert e[12, "fp64"] < e[6, "fp64"]/100
     assert e[12, "fp64"] < 1e-4, e
     assert e[12, "fp32"] <= 2*e[12, "fp64"], e
+
+
+@pytest.fixture(scope="module")
+def forest_case(actx):
+    from sumpy_amd.tree import build_forest
+    rng = np.random.default_rng(11)
+    sizes = [300, 40, 1200, 700, 1, 900]
+    pts = []
+    for n in sizes:
+        x = rng.standard_normal((3, n))
+        pts.append(x/np.linalg.norm(x, axis=0)*(1 + 1e-3*rng.standard_normal(n)))
+    tree, trav, offs = build_forest(actx, pts, max_particles_in_box=30)
+    q = rng.standard_normal(offs[-1]) + 1j*rng.standard_normal(offs[-1])
+    return pts, tree, trav, offs, q
+
+
+def test_forest_matches_oracle_fmm(actx, forest_case):
+    """Six independent problems in one pass (one launch per stage for all of
+    them) against the oracle FMM over the same forest."""
+    from sumpy_amd.drive import drive_fmm
+    pts, tree, trav, offs, q = forest_case
+    order, k = 6, 2.5
+    wr = helm_wrangler(actx, tree, trav, order, k)
+    pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+    ref = oracle_fmm(tree, trav, q, order, exclude_self=True, k=k)
+    assert rel(actx.to_numpy(pot), ref) < 1e-11
+
+
+def test_forest_graph_replays(actx, forest_case):
+    """The batched pass captured in a HIP graph: every replay (with eager work
+    and synchronisation between replays, new weights written in place)
+    equals the eager pass."""
+    import torch
+    from sumpy_amd.drive import capture_drive_fmm, drive_fmm
+    pts, tree, trav, offs, q = forest_case
+    wr = helm_wrangler(actx, tree, trav, 8, 5.0, m2l_precision="fp32", p2p_precision="fp32")
+    w = actx.from_numpy(q)
+    g = capture_drive_fmm(actx, wr, [w])
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        qn = rng.standard_normal(len(q)) + 1j*rng.standard_normal(len(q))
+        w.copy_(actx.from_numpy(qn))
+        out, = g.replay()
+        got = actx.to_numpy(out).copy()
+        eager, = drive_fmm(actx, wr, [actx.from_numpy(qn)])
+        torch.cuda.synchronize()
+        assert np.array_equal(got, actx.to_numpy(eager))
