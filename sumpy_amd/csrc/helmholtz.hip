@@ -229,6 +229,146 @@ __global__ __launch_bounds__(64) void m2p_helm3d(
   }
 }
 
+// {{{ M2P, polynomial form (compile-time order, potential only)
+
+// Stored multi-indices have m_x = a <= 1, so d^(a,b,c) H = x^a sum_{j,l}
+// h(b,j) h(c,l) y^{2j-b} z^{2l-c} g_{a+j+l}, and the whole list-3 sum of one
+// source box regroups into
+//   sum_s M_s d^s H = sum_a x^a sum_n g_{a+n} R_{a,n}(y, z),
+//   R_{a,n} = sum_{beta + gamma <= n} C[a,n,beta,gamma] y^beta z^gamma,
+//   C[a,n,beta,gamma] = sum_{j=beta}^{n-gamma} M_(a, 2j-beta, 2(n-j)-gamma)
+//                       h(2j-beta, j) h(2(n-j)-gamma, n-j)   (|m| <= p only).
+// The coefficients depend on the multipole only: the wave forms them once per
+// source box in LDS, then every lane evaluates the polynomials at its target
+// by Horner's rule with compile-time offsets (broadcast LDS reads).
+// Coefficient order: a, then n, then beta, then gamma.
+
+__host__ __device__ constexpr int tet(int n) { return n*(n + 1)*(n + 2)/6; }
+
+template <int ORD>
+struct M2PPoly {
+  static constexpr int N0 = tet(ORD + 1);   // coefficients with a = 0
+  static constexpr int NC = N0 + tet(ORD);  // all
+};
+
+// stored index of (a, b, c) in the PDE-compressed order (degree d = a+b+c:
+// first a = 0 with b = 0..d, then a = 1 with b = 0..d-1)
+__device__ __forceinline__ int stored_index(int a, int b, int c) {
+  const int d = a + b + c;
+  return d*d + (a ? d + 1 + b : b);
+}
+
+template <int ORD>
+__device__ __forceinline__ void m2p_poly_eval(const double2 *C, double x, double y, double z,
+                                              const double *gr, const double *gi,
+                                              double &pr, double &pi) {
+  #pragma unroll
+  for (int a = 0; a <= 1; ++a) {
+    const int base = a ? M2PPoly<ORD>::N0 : 0;
+    double ar = 0, ai = 0;
+    #pragma unroll
+    for (int n = 0; n <= ORD - a; ++n) {
+      double Rr = 0, Ri = 0;
+      #pragma unroll
+      for (int b = n; b >= 0; --b) {
+        double Sr = 0, Si = 0;
+        #pragma unroll
+        for (int c = n - b; c >= 0; --c) {
+          const double2 v = C[base + tet(n) + b*(n + 1) - b*(b - 1)/2 + c];
+          Sr = Sr*z + v.x;
+          Si = Si*z + v.y;
+        }
+        Rr = Rr*y + Sr;
+        Ri = Ri*y + Si;
+      }
+      ar += gr[a + n]*Rr - gi[a + n]*Ri;
+      ai += gr[a + n]*Ri + gi[a + n]*Rr;
+    }
+    if (a == 0) { pr += ar; pi += ai; } else { pr += x*ar; pi += x*ai; }
+  }
+}
+
+template <int ORD>
+__global__ __launch_bounds__(64) void m2p_helm3d_poly(
+    double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const int *bts, const int *btc, const double *centers, int nboxes, const double *tx,
+    const double *ty, const double *tz, double rscale, double scale, const double2 *src,
+    int sbase, double2 *pot) {
+  constexpr int NC = M2PPoly<ORD>::NC, NS = (ORD + 1)*(ORD + 1), HS = ORD + 1;
+  __shared__ double2 sC[NC];
+  __shared__ double2 sM[NS];
+  __shared__ double ht[HS*HS];
+  __shared__ unsigned dec[NC];
+  herm_table(ht, ORD);
+  const int lane = threadIdx.x;
+  for (int i = lane; i < NC; i += 64) {  // (a, n, beta, gamma) of coefficient i
+    const int a = i >= M2PPoly<ORD>::N0, r = a ? i - M2PPoly<ORD>::N0 : i;
+    int n = 0;
+    while (tet(n + 1) <= r) ++n;
+    int q = r - tet(n), b = 0;
+    while (q >= n + 1 - b) { q -= n + 1 - b; ++b; }
+    dec[i] = a | (n << 1) | (b << 8) | (q << 16);
+  }
+  const double rinv = 1.0/rscale, kappa = k*rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const int t0 = bts[tb], nt = btc[tb];
+    for (int tc = 0; tc < nt; tc += 64) {
+      const bool valid = tc + lane < nt;
+      const int t = t0 + tc + (valid ? lane : 0);
+      const double X = tx[t], Y = ty[t], Z = tz[t];
+      double pr = 0, pi = 0;
+      for (int qq = starts[ib]; qq < starts[ib + 1]; ++qq) {
+        const int sb = lists[qq];
+        __syncthreads();  // previous source box done with sC / sM
+        const double2 *M = src + (size_t)(sb - sbase)*NS;
+        for (int u = lane; u < NS; u += 64) sM[u] = M[u];
+        __syncthreads();
+        for (int i = lane; i < NC; i += 64) {
+          const unsigned d = dec[i];
+          const int a = d & 1, n = (d >> 1) & 127, be = (d >> 8) & 255, ga = d >> 16;
+          double vr = 0, vi = 0;
+          for (int j = be; j <= n - ga; ++j) {
+            const int b = 2*j - be, c = 2*(n - j) - ga;
+            if (a + b + c > ORD) continue;
+            const double w = ht[b*HS + j]*ht[c*HS + (n - j)];
+            const double2 m = sM[stored_index(a, b, c)];
+            vr += w*m.x;
+            vi += w*m.y;
+          }
+          sC[i] = make_double2(vr, vi);
+        }
+        const double x = (X - centers[sb])*rinv, y = (Y - centers[nboxes + sb])*rinv,
+                     z = (Z - centers[2*(size_t)nboxes + sb])*rinv;
+        double gr[ORD + 1], gi[ORD + 1];
+        {
+          const double r = sqrt(x*x + y*y + z*z), ri = 1.0/r, ri2 = ri*ri;
+          double sn, cs;
+          sincos(kappa*r, &sn, &cs);
+          gr[0] = cs*ri;
+          gi[0] = sn*ri;
+          if (ORD >= 1) {
+            const double kr = kappa*r;
+            gr[1] = (-cs - sn*kr)*ri2*ri;
+            gi[1] = (cs*kr - sn)*ri2*ri;
+          }
+          #pragma unroll
+          for (int q = 2; q <= ORD; ++q) {
+            gr[q] = (-(2*q - 1)*gr[q - 1] - kappa*kappa*gr[q - 2])*ri2;
+            gi[q] = (-(2*q - 1)*gi[q - 1] - kappa*kappa*gi[q - 2])*ri2;
+          }
+        }
+        __syncthreads();
+        m2p_poly_eval<ORD>(sC, x, y, z, gr, gi, pr, pi);
+      }
+      if (valid) {
+        pot[t].x += scale*rinv*pr;
+        pot[t].y += scale*rinv*pi;
+      }
+    }
+  }
+}
+
 // }}}
 
 // {{{ P2L
@@ -388,6 +528,24 @@ extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int n
   const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
   hipStream_t st = (hipStream_t)stream;
   const double *src = static_cast<const double *>(src_expansions);
+  if (!gx && pot && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
+    // potentials of the usual orders: the polynomial form
+#define SUMPY_M2P_POLY(O)                                                                  \
+  case O:                                                                                  \
+    m2p_helm3d_poly<O><<<grid, 64, 0, st>>>(                                               \
+        k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
+        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
+        rscale, scale, static_cast<const double2 *>(src_expansions), src_base_ibox,        \
+        static_cast<double2 *>(pot));                                                      \
+    return check_launch("m2p_helmholtz");
+    switch (expn->order) {
+      SUMPY_M2P_POLY(1) SUMPY_M2P_POLY(2) SUMPY_M2P_POLY(3) SUMPY_M2P_POLY(4)
+      SUMPY_M2P_POLY(5) SUMPY_M2P_POLY(6) SUMPY_M2P_POLY(7) SUMPY_M2P_POLY(8)
+      SUMPY_M2P_POLY(9) SUMPY_M2P_POLY(10) SUMPY_M2P_POLY(11) SUMPY_M2P_POLY(12)
+      default: break;
+    }
+#undef SUMPY_M2P_POLY
+  }
   if (gx)
     m2p_helm3d<true><<<grid, 64, lds, st>>>(
         *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,

@@ -533,7 +533,8 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             sched = build_m2l_schedule(
                 actx, target_boxes.cpu().numpy(), src_box_starts.cpu().numpy(),
                 src_box_lists.cpu().numpy(), class_lists.cpu().numpy(),
-                int(class_level_start), int(nclasses), centers.cpu().numpy())
+                int(class_level_start), int(nclasses), centers.cpu().numpy(),
+                tile=TILE//2 if self.tgt_expansion.is_complex_valued else TILE)
             self._schedules[key] = sched
         return sched
 
@@ -559,48 +560,41 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
                     sbase=sbase, work=work, tbase=tbase)
 
     def _complex_schedule(self, actx, sched, sbase, tbase):
-        """The schedule of the real embedding: tile T becomes the tiles of
-        the target rows Lr (2 (t - tbase)) and Li (2 (t - tbase) + 1); class
-        slot c of a tile becomes the slots of Ar (class 2c) and Ai (2c + 1),
-        fed by the source rows 3 (s - sbase) + (0: Mr, 1: Mi, 2: -Mi):
+        """The schedule of the real embedding.  A tile of TILE/2 complex
+        targets becomes one tile of TILE columns: column 2j is the row Lr
+        (2 (t_j - tbase)), column 2j + 1 the row Li.  Class slot c becomes the
+        slots of Ar (class 2c) and Ai (2c + 1), fed by the source rows
+        3 (s - sbase) + (0: Mr, 1: Mi, 2: -Mi):
         Lr = Ar Mr + Ai (-Mi), Li = Ar Mi + Ai Mr."""
         key = ("complex", id(sched), sbase, tbase)
         out = self._schedules.get(key)
         if out is not None:
             return out
         h = sched["host"]
-        tt, tss, sc, ss = (h["tile_targets"], h["tile_slot_starts"], h["slot_class"],
-                           h["slot_sources"].reshape(-1, TILE))
-        ntiles = sched["ntiles"]
+        half = TILE//2
+        tt = h["tile_targets"].astype(np.int64).reshape(-1, half)
+        ss = h["slot_sources"].astype(np.int64).reshape(-1, half)
+        sc = h["slot_class"].astype(np.int64)
 
-        def vrow(ids, base, mult, off):
-            return np.where(ids >= 0, mult*(ids.astype(np.int64) - base) + off, -1)
+        def rows(ids, base, mult, offs):
+            r = np.where(ids[..., None] >= 0,
+                         mult*(ids[..., None] - base) + np.array(offs), -1)
+            return r.reshape(ids.shape[0], -1)
 
-        tiles, starts, classes, sources = [], [0], [], []
-        for t in range(ntiles):
-            tgt = tt[t*TILE:(t + 1)*TILE]
-            sl = range(tss[t], tss[t + 1])
-            for part in (0, 1):
-                tiles.append(vrow(tgt, tbase, 2, part))
-                for q in sl:
-                    c, srcs = int(sc[q]), ss[q]
-                    classes += [2*c, 2*c + 1]
-                    if part == 0:
-                        sources += [vrow(srcs, sbase, 3, 0), vrow(srcs, sbase, 3, 2)]
-                    else:
-                        sources += [vrow(srcs, sbase, 3, 1), vrow(srcs, sbase, 3, 0)]
-                starts.append(starts[-1] + 2*len(sl))
+        vt = rows(tt, tbase, 2, (0, 1))                      # (ntiles, TILE)
+        vs = np.stack([rows(ss, sbase, 3, (0, 1)),           # Ar <- (Mr, Mi)
+                       rows(ss, sbase, 3, (2, 0))], axis=1)  # Ai <- (-Mi, Mr)
+        vc = np.stack([2*sc, 2*sc + 1], axis=1)
 
         def dev(a):
             return torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(actx.device)
 
-        nslots = starts[-1]
-        out = dict(ntiles=2*ntiles, tile_targets=dev(np.concatenate(tiles)),
-                   tile_slot_starts=dev(np.array(starts)), slot_class=dev(np.array(classes)),
-                   slot_sources=dev(np.concatenate(sources) if sources
-                                    else np.zeros(0, np.int32)),
+        ntiles, nslots = sched["ntiles"], 2*len(sc)
+        out = dict(ntiles=ntiles, tile_targets=dev(vt.reshape(-1)),
+                   tile_slot_starts=dev(2*h["tile_slot_starts"].astype(np.int64)),
+                   slot_class=dev(vc.reshape(-1)), slot_sources=dev(vs.reshape(-1)),
                    nslots=nslots, npairs=4*sched["npairs"],
-                   nsplit=choose_nsplit(2*ntiles, nslots, actx.device))
+                   nsplit=choose_nsplit(ntiles, nslots, actx.device))
         self._schedules[key] = out
         return out
 
@@ -665,11 +659,16 @@ def m2l_apply_levels(actx, entries):
 
 
 def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_start,
-                       nclasses, centers):
-    """Group a level's target boxes into tiles of :data:`TILE` sharing most of
-    their translation classes (targets sorted by their octant within the
-    parent), and list for each tile the classes present with one source box per
-    tile column (-1 if none)."""
+                       nclasses, centers, tile=TILE):
+    """Group a level's target boxes into tiles of *tile* targets sharing most
+    of their translation classes, and list for each tile the classes present
+    with one source box per tile column (-1 if none).
+
+    Targets are taken octant by octant (their octant within the parent fixes
+    which classes they can have).  Within an octant they stay in box order,
+    unless sorting them by their set of classes cuts the class slots of the
+    level by 15 % or more (sparse levels, e.g. points on a surface): then
+    targets with equal class sets share tiles."""
     nt = len(target_boxes)
     e0, e1 = int(starts[0]), int(starts[-1])
     counts = np.diff(starts).astype(np.int64)
@@ -689,24 +688,35 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
         h = 1.0
     rel = np.rint((ct - ct[:, :1])/h).astype(np.int64)
     octant = ((rel[0] & 1) << 2) | ((rel[1] & 1) << 1) | (rel[2] & 1)
-    order = np.argsort(octant, kind="stable")
-    # tiles: chunks of TILE targets of one octant, in box order; chunk j of
-    # every octant before chunk j + 1, so that a contiguous range of tiles
-    # covers one region of space (the apply kernel hands each XCD such a range)
-    cnt = np.bincount(octant, minlength=8)
-    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    nchunk = (cnt + TILE - 1)//TILE
-    chunks = [(j, o) for o in range(8) for j in range(nchunk[o])]
-    chunks.sort()
-    ntiles = len(chunks)
-    padded = np.full(ntiles*TILE, -1, np.int64)
-    for it, (j, o) in enumerate(chunks):
-        sel = order[first[o] + j*TILE:first[o] + min((j + 1)*TILE, cnt[o])]
-        padded[it*TILE:it*TILE + len(sel)] = sel
     tab = np.full((nt + 1, max(nclasses, 1)), -1, np.int32)
     tab[itgt, e_cls] = e_src
-    rows = np.where(padded >= 0, padded, nt)  # row nt is all -1
-    tabT = tab[rows].reshape(ntiles, TILE, -1)
+    member = tab >= 0  # row nt is all False
+    cnt = np.bincount(octant, minlength=8)
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    nchunk = (cnt + tile - 1)//tile
+    # tiles: chunks of targets of one octant; chunk j of every octant before
+    # chunk j + 1, so that a contiguous range of tiles covers one region of
+    # space (the apply kernel hands each XCD such a range)
+    chunks = sorted((j, o) for o in range(8) for j in range(nchunk[o]))
+    ntiles = len(chunks)
+
+    def tiles_of(order):
+        padded = np.full(ntiles*tile, -1, np.int64)
+        for it, (j, o) in enumerate(chunks):
+            sel = order[first[o] + j*tile:first[o] + min((j + 1)*tile, cnt[o])]
+            padded[it*tile:it*tile + len(sel)] = sel
+        rows = np.where(padded >= 0, padded, nt)
+        return padded, rows, int(member[rows].reshape(ntiles, tile, -1).any(axis=1).sum())
+
+    padded, rows, nslots = tiles_of(np.argsort(octant, kind="stable"))
+    if len(e_src) < 0.5*nslots*tile:
+        packed = np.packbits(member[:nt], axis=1)
+        order = np.lexsort([packed[:, i] for i in range(packed.shape[1] - 1, -1, -1)]
+                           + [octant])
+        cand = tiles_of(order)
+        if cand[2] <= nslots/1.15:
+            padded, rows, nslots = cand
+    tabT = tab[rows].reshape(ntiles, tile, -1)
     used = (tabT >= 0).any(axis=1)
     ti, ci = np.nonzero(used)
     slot_sources = tabT[ti, :, ci].astype(np.int32)
