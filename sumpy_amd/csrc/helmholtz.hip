@@ -470,6 +470,135 @@ __global__ __launch_bounds__(256) void p2l_helm3d(
 
 // }}}
 
+// {{{ P2L, moment form (compile-time order, charges only)
+
+// The transpose of the polynomial M2P: with xi = (c - y)/rho,
+//   L_(a,b,c) = sum_{j,l} h(b,j) h(c,l) Phi_a(j + l, 2j - b, 2l - c),
+//   Phi_a(n, beta, gamma) = sum_y q_y g_{a+n}(|xi_y|) x_y^a y_y^beta z_y^gamma,
+// (beta + gamma <= n <= p - a), then times 1/rho.  Thread r owns the moment row
+// (a, n, beta) -- (p+1)^2 rows, gamma = 0..n-beta -- in registers while the
+// box's list-4 sources stream through LDS; the coefficients are assembled
+// from the moments at the end.
+
+template <int ORD>
+__global__ __launch_bounds__(256) void p2l_helm3d_moments(
+    double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const int *bss, const int *bsc, const double *centers, int nboxes, const double *sx,
+    const double *sy, const double *sz, const double2 *w0, double rscale, double2 *tgt,
+    int tbase) {
+  constexpr int NS = (ORD + 1)*(ORD + 1), NC = M2PPoly<ORD>::NC, HS = ORD + 1;
+  constexpr int CH = P2LH_CH, NP = ORD + 1;
+  __shared__ double2 sQG[NP][CH];   // q g_q per source
+  __shared__ double sY[NP][CH], sZ[NP][CH], sX[CH];
+  __shared__ double2 sPhi[NC];
+  __shared__ double ht[HS*HS];
+  herm_table(ht, ORD);
+  const int tid = threadIdx.x;
+  // moment row of this thread
+  int ra = 0, rn = 0, rb = 0, rlen = 0, rbase = 0;
+  if (tid < NS) {
+    constexpr int R0 = (ORD + 1)*(ORD + 2)/2;
+    ra = tid >= R0;
+    int r = ra ? tid - R0 : tid;
+    while (r > rn) { r -= rn + 1; ++rn; }
+    rb = r;
+    rlen = rn - rb + 1;
+    rbase = (ra ? M2PPoly<ORD>::N0 : 0) + tet(rn) + rb*(rn + 1) - rb*(rb - 1)/2;
+  }
+  const double rinv = 1.0/rscale, kappa = k*rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    const double cx = centers[tb], cy = centers[nboxes + tb],
+                 cz = centers[2*(size_t)nboxes + tb];
+    double pr[NP], pi[NP];
+    #pragma unroll
+    for (int g = 0; g < NP; ++g) { pr[g] = 0; pi[g] = 0; }
+    for (int q = starts[ib]; q < starts[ib + 1]; ++q) {
+      const int sbox = lists[q];
+      const int s0 = bss[sbox], ns = bsc[sbox];
+      for (int sc = 0; sc < ns; sc += CH) {
+        const int n = min(CH, ns - sc);
+        __syncthreads();
+        if (tid < n) {
+          const int si = s0 + sc + tid;
+          const double x = (cx - sx[si])*rinv, y = (cy - sy[si])*rinv, z = (cz - sz[si])*rinv;
+          const double r = sqrt(x*x + y*y + z*z), ri = 1.0/r, ri2 = ri*ri;
+          double sn, cs;
+          sincos(kappa*r, &sn, &cs);
+          const double2 w = w0[si];
+          double ar = cs*ri, ai = sn*ri;
+          sQG[0][tid] = make_double2(w.x*ar - w.y*ai, w.x*ai + w.y*ar);
+          double br = 0, bi = 0;
+          if (ORD >= 1) {
+            const double kr = kappa*r;
+            br = (-cs - sn*kr)*ri2*ri;
+            bi = (cs*kr - sn)*ri2*ri;
+            sQG[1][tid] = make_double2(w.x*br - w.y*bi, w.x*bi + w.y*br);
+          }
+          #pragma unroll
+          for (int g = 2; g <= ORD; ++g) {
+            const double f = -(2*g - 1);
+            const double cr = (f*br - kappa*kappa*ar)*ri2, ci = (f*bi - kappa*kappa*ai)*ri2;
+            sQG[g][tid] = make_double2(w.x*cr - w.y*ci, w.x*ci + w.y*cr);
+            ar = br; ai = bi; br = cr; bi = ci;
+          }
+          double py = 1, pz = 1;
+          #pragma unroll
+          for (int e = 0; e <= ORD; ++e) {
+            sY[e][tid] = py;
+            sZ[e][tid] = pz;
+            py *= y;
+            pz *= z;
+          }
+          sX[tid] = x;
+        }
+        __syncthreads();
+        if (tid < NS) {
+          for (int j = 0; j < n; ++j) {
+            const double2 qg = sQG[ra + rn][j];
+            const double m = (ra ? sX[j] : 1.0)*sY[rb][j];
+            const double wr = qg.x*m, wi = qg.y*m;
+            #pragma unroll
+            for (int g = 0; g < NP; ++g) {
+              if (g < rlen) {
+                const double zg = sZ[g][j];
+                pr[g] += wr*zg;
+                pi[g] += wi*zg;
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < NS) {
+      #pragma unroll
+      for (int g = 0; g < NP; ++g)
+        if (g < rlen) sPhi[rbase + g] = make_double2(pr[g], pi[g]);
+    }
+    __syncthreads();
+    if (tid < NS) {
+      // coefficient tid = (a, b, c) in the stored order
+      int d = 0;
+      while ((d + 1)*(d + 1) <= tid) ++d;
+      const int o = tid - d*d, a = o > d, b = a ? o - d - 1 : o, c = d - a - b;
+      const int base = a ? M2PPoly<ORD>::N0 : 0;
+      double lr = 0, li = 0;
+      for (int j = (b + 1)/2; j <= b; ++j)
+        for (int l = (c + 1)/2; l <= c; ++l) {
+          const int nn = j + l, be = 2*j - b, ga = 2*l - c;
+          const double2 v = sPhi[base + tet(nn) + be*(nn + 1) - be*(be - 1)/2 + ga];
+          const double f = ht[b*HS + j]*ht[c*HS + l];
+          lr += f*v.x;
+          li += f*v.y;
+        }
+      tgt[(size_t)(tb - tbase)*NS + tid] = make_double2(lr*rinv, li*rinv);
+    }
+  }
+}
+
+// }}}
+
 int check_helm_expn(const sumpyhip_expn *e, const char *what) {
   SUMPY_REQUIRE(e != nullptr && e->stored_ids != nullptr, "%s: null expansion tables", what);
   SUMPY_REQUIRE(e->order >= 0 && e->order + 2 <= HELM_MAXN, "%s: order %d out of range",
@@ -582,6 +711,24 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int n
   const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
   hipStream_t st = (hipStream_t)stream;
   const double *c0 = static_cast<const double *>(w0);
+  if (!wx && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
+    // charges at the usual orders: the moment form
+#define SUMPY_P2L_MOM(O)                                                                   \
+  case O:                                                                                  \
+    p2l_helm3d_moments<O><<<grid, nt, 0, st>>>(                                            \
+        k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
+        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
+        static_cast<const double2 *>(w0), rscale, static_cast<double2 *>(tgt_expansions), \
+        tgt_base_ibox);                                                                    \
+    return check_launch("p2l_helmholtz");
+    switch (expn->order) {
+      SUMPY_P2L_MOM(1) SUMPY_P2L_MOM(2) SUMPY_P2L_MOM(3) SUMPY_P2L_MOM(4)
+      SUMPY_P2L_MOM(5) SUMPY_P2L_MOM(6) SUMPY_P2L_MOM(7) SUMPY_P2L_MOM(8)
+      SUMPY_P2L_MOM(9) SUMPY_P2L_MOM(10) SUMPY_P2L_MOM(11) SUMPY_P2L_MOM(12)
+      default: break;
+    }
+#undef SUMPY_P2L_MOM
+  }
   if (wx)
     p2l_helm3d<true><<<grid, nt, lds, st>>>(
         *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
