@@ -13,6 +13,7 @@
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -354,6 +355,18 @@ def _m2l_tabs(src_expansion, tgt_expansion, device):
 
 
 _BF16_TABS = {}
+_SCHEDULES = {}  # id(list tensor) -> (weak ref, {key: M2L tile schedule})
+
+
+def _schedule_cache(lists):
+    """The schedules made for the list tensor *lists*, dropped with it."""
+    k = id(lists)
+    ent = _SCHEDULES.get(k)
+    if ent is None or ent[0]() is not lists:
+        ent = (weakref.ref(lists), {})
+        _SCHEDULES[k] = ent
+        weakref.finalize(lists, _SCHEDULES.pop, k, None)
+    return ent[1]
 
 
 def _bf16_tabs(src_expansion, tgt_expansion, device):
@@ -525,17 +538,20 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
     def schedule(self, actx, target_boxes, src_box_starts, src_box_lists, class_lists,
                  class_level_start, nclasses, centers):
         """Tile schedule of one level, built on the host and cached."""
+        tile = TILE//2 if self.tgt_expansion.is_complex_valued else TILE
         key = tuple((t.data_ptr(), t.numel(), t._version) for t in
                     (target_boxes, src_box_starts, src_box_lists, class_lists, centers)) \
-            + (int(class_level_start), int(nclasses))
-        sched = self._schedules.get(key)
+            + (int(class_level_start), int(nclasses), tile, str(actx.device))
+        # shared by every operator instance (the schedule depends on the lists
+        # only), held while the traversal's list tensor lives
+        per_lists = _schedule_cache(src_box_lists)
+        sched = per_lists.get(key)
         if sched is None:
             sched = build_m2l_schedule(
                 actx, target_boxes.cpu().numpy(), src_box_starts.cpu().numpy(),
                 src_box_lists.cpu().numpy(), class_lists.cpu().numpy(),
-                int(class_level_start), int(nclasses), centers.cpu().numpy(),
-                tile=TILE//2 if self.tgt_expansion.is_complex_valued else TILE)
-            self._schedules[key] = sched
+                int(class_level_start), int(nclasses), centers.cpu().numpy(), tile=tile)
+            per_lists[key] = sched
         return sched
 
     def level_entry(self, actx, **kwargs):
@@ -666,9 +682,9 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
 
     Targets are taken octant by octant (their octant within the parent fixes
     which classes they can have).  Within an octant they stay in box order,
-    unless sorting them by their set of classes cuts the class slots of the
-    level by 15 % or more (sparse levels, e.g. points on a surface): then
-    targets with equal class sets share tiles."""
+    unless grouping them by their sets of classes (:func:`_class_set_order`)
+    cuts the class slots of the level by 15 % or more (sparse levels, e.g.
+    points on a surface)."""
     nt = len(target_boxes)
     e0, e1 = int(starts[0]), int(starts[-1])
     counts = np.diff(starts).astype(np.int64)
@@ -710,10 +726,7 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
 
     padded, rows, nslots = tiles_of(np.argsort(octant, kind="stable"))
     if len(e_src) < 0.5*nslots*tile:
-        packed = np.packbits(member[:nt], axis=1)
-        order = np.lexsort([packed[:, i] for i in range(packed.shape[1] - 1, -1, -1)]
-                           + [octant])
-        cand = tiles_of(order)
+        cand = tiles_of(_class_set_order(member[:nt], octant, tile))
         if cand[2] <= nslots/1.15:
             padded, rows, nslots = cand
     tabT = tab[rows].reshape(ntiles, tile, -1)
@@ -735,6 +748,40 @@ def build_m2l_schedule(actx, target_boxes, starts, lists, class_lists, class_sta
                 host=dict(tile_targets=tile_targets, tile_slot_starts=tile_slot_starts,
                           slot_class=ci.astype(np.int32),
                           slot_sources=slot_sources.reshape(-1)))
+
+
+def _class_set_order(member, octant, tile, window=1024):
+    """Targets octant by octant, grouped into tiles of *tile* that share
+    classes: sorted by class set, then each tile grown greedily from its first
+    free target by the target (among the next *window* free ones) that adds
+    the fewest classes to the tile's set."""
+    packed = np.packbits(member, axis=1)
+    base = np.lexsort([packed[:, i] for i in range(packed.shape[1] - 1, -1, -1)] + [octant])
+    out = []
+    for o in range(8):
+        seq = base[octant[base] == o]
+        M = member[seq]
+        Mi = M.astype(np.int16)
+        free = np.ones(len(seq), bool)
+        ptr = 0
+        while True:
+            while ptr < len(seq) and not free[ptr]:
+                ptr += 1
+            if ptr >= len(seq):
+                break
+            grp = [ptr]
+            free[ptr] = False
+            U = M[ptr].copy()
+            while len(grp) < tile:
+                cand = np.nonzero(free[ptr:ptr + window])[0] + ptr
+                if len(cand) == 0:
+                    break
+                j = cand[np.argmin(Mi[cand] @ ~U)]
+                free[j] = False
+                grp.append(j)
+                U |= M[j]
+            out.extend(seq[grp])
+    return np.array(out, np.int64)
 
 
 def choose_nsplit(ntiles, nslots, device):
