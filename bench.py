@@ -87,13 +87,19 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None)
         actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
         partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
                 m2l_translation_override=tr),
-        [knl], exclude_self=True, p2p_precision=p2p_precision)
+        [knl], exclude_self=True, p2p_precision=p2p_precision, autotune=True)
     import torch
     t2s = torch.arange(tree.ntargets, dtype=torch.int32, device=actx.device)
     return SumpyExpansionWrangler(ti, trav, np.float64 if k is None else np.complex128,
                                   fmm_level_to_order=lambda kn, a, t, lev: order,
                                   kernel_extra_kwargs=None if k is None else {"k": k},
                                   self_extra_kwargs={"target_to_source": t2s})
+
+
+def tuned_policies(wr):
+    """The M2L split policies (items per CU, slots per split) the autotuner
+    chose for this wrangler's passes."""
+    return sorted({tuple(v) for v in wr.tree_indep.m2l_tuner._memo.values()})
 
 
 def time_passes(actx, fn, steps, warmup):
@@ -409,6 +415,7 @@ def helmholtz_batched(args):
         "setup_s": {"forest": round(t_tree, 3),
                     **{f"m2l_precompute_{a}": round(b["setup_s"], 3) for a, b in results.items()}},
         "roofline": roofline,
+        "m2l_split_policy": {a: tuned_policies(b) for a, b in forms.items()},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -614,6 +621,7 @@ def main():
         "setup_s": {"tree_and_traversal": round(t_tree, 3),
                     **{f"m2l_precompute_{k}": round(v, 3) for k, v in setup.items()}},
         "roofline": roofline,
+        "m2l_split_policy": {f: tuned_policies(wr) for f, wr in forms.items()},
         "cpu_baseline": cpu,
         "p2p_dense": dense,
         "qbx_starfish_dlp": qbx,

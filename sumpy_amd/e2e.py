@@ -621,9 +621,11 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         return kwargs["tgt_expansions"]
 
 
-def m2l_apply_levels(actx, entries):
+def m2l_apply_levels(actx, entries, policy=None):
     """One launch of the MFMA contraction over several levels (same
-    translation operator orders), largest level first."""
+    translation operator orders), largest level first.  *policy* = (items
+    per CU, minimum slots per split) overrides the split rule of the
+    schedules (:mod:`sumpy_amd.autotune`)."""
     entries = [e for e in entries if e is not None]
     if not entries:
         return
@@ -637,7 +639,9 @@ def m2l_apply_levels(actx, entries):
             raise ValueError("m2l_apply_levels: levels must share orders and precision")
     entries = sorted(entries, key=lambda e: -e["sched"]["npairs"])
     npad, kpad = int(entries[0]["work"].shape[1]), int(entries[0]["src"].shape[1])
-    nsplits = [e["sched"]["nsplit"] for e in entries]
+    nsplits = [e["sched"]["nsplit"] if policy is None else
+               choose_nsplit(e["sched"]["ntiles"], e["sched"]["nslots"], actx.device, *policy)
+               for e in entries]
     levels, keep = [], []
     npairs = 0
     for e, nsplit in zip(entries, nsplits):
@@ -784,16 +788,17 @@ def _class_set_order(member, octant, tile, window=1024):
     return np.array(out, np.int64)
 
 
-def choose_nsplit(ntiles, nslots, device):
-    """Number of blocks per tile: enough (tile, split) items for about four
-    blocks per CU, with at least :data:`MIN_SLOTS_PER_SPLIT` class slots per
-    split (each split costs one extra write and read of its partial sums)."""
+def choose_nsplit(ntiles, nslots, device, per_cu=4, min_slots=MIN_SLOTS_PER_SPLIT):
+    """Number of blocks per tile: enough (tile, split) items for about
+    *per_cu* blocks per CU, with at least *min_slots* class slots per split
+    (each split costs one extra write and read of its partial sums).  The
+    defaults are the fixed rule; :mod:`sumpy_amd.autotune` may pick others."""
     if ntiles == 0:
         return 1
     ncu = torch.cuda.get_device_properties(device).multi_processor_count \
         if torch.device(device).type == "cuda" else 256
-    want = -(-4*ncu//ntiles)
-    most = max(1, nslots//(ntiles*MIN_SLOTS_PER_SPLIT))
+    want = -(-per_cu*ncu//ntiles)
+    most = max(1, nslots//(ntiles*min_slots))
     return int(max(1, min(want, most)))
 
 

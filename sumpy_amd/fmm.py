@@ -44,15 +44,18 @@ def build_csr_level_starts(level_orders, order_to_size, level_starts):
 class SumpyTreeIndependentDataForWrangler:
     """Operator factory (reference ``fmm.py:99-252``).
 
-    Extra keyword ``p2p_precision``: ``"fp64"`` (plain) or ``"fp32"`` for the
-    near field; the M2L precision is chosen by the local expansion's
-    ``m2l_translation`` (``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``).
+    Extra keywords: ``p2p_precision``, ``"fp64"`` (plain) or ``"fp32"`` for
+    the near field (the M2L precision is chosen by the local expansion's
+    ``m2l_translation``, ``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``);
+    ``autotune``, time the M2L contraction's split policies on the first pass
+    of each kind of tree and keep the fastest (cached on disk,
+    :mod:`sumpy_amd.autotune`).
     """
 
     def __init__(self, array_context, multipole_expansion_factory,
                  local_expansion_factory, target_kernels, exclude_self=False,
                  use_rscale=None, strength_usage=None, source_kernels=None,
-                 p2p_precision="fp64"):
+                 p2p_precision="fp64", autotune=False):
         self._setup_actx = array_context
         self.multipole_expansion_factory = multipole_expansion_factory
         self.local_expansion_factory = local_expansion_factory
@@ -62,7 +65,13 @@ class SumpyTreeIndependentDataForWrangler:
         self.use_rscale = use_rscale
         self.strength_usage = strength_usage
         self.p2p_precision = p2p_precision
+        self.autotune = autotune
         self._memo = {}
+
+    @cached_property
+    def m2l_tuner(self):
+        from sumpy_amd.autotune import M2LSplitTuner
+        return M2LSplitTuner()
 
     def _m(self, key, make):
         v = self._memo.get(key)
@@ -512,7 +521,13 @@ class SumpyExpansionWrangler:
             else:
                 m2l(actx, **kwargs)
         for order, levs in batches.items():
-            m2l_apply_levels(actx, [entry for _, entry, _ in levs])
+            ents = [entry for _, entry, _ in levs]
+            policy = None
+            if self.tree_indep.autotune:
+                policy = self.tree_indep.m2l_tuner.policy(
+                    actx, ents, str(self.tree_indep.get_base_kernel()), order,
+                    self.tree_indep.m2l_translation.precision)
+            m2l_apply_levels(actx, ents, policy=policy)
             for lev, _, tview in levs:
                 _, lview = self.local_expansions_view(local_exps, lev)
                 self.tree_indep.m2l_postprocess_local_kernel(order, order)(
