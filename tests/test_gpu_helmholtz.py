@@ -203,3 +203,132 @@ def test_forest_graph_replays(actx, forest_case):
         eager, = drive_fmm(actx, wr, [actx.from_numpy(qn)])
         torch.cuda.synchronize()
         assert np.array_equal(got, actx.to_numpy(eager))
+
+
+@pytest.mark.parametrize("order", [3, 8])
+def test_list_operators_with_derivatives(actx, order):
+    """The Helmholtz expansion operators one by one against the oracle, with
+    target gradients (M2P direct form, L2P on planes) and dipole sources (P2L
+    direct form, P2M on planes), on a tree whose lists 3 and 4 are populated."""
+    import torch
+
+    from sumpy_amd import E2PFromCSR, E2PFromSingleBox, P2EFromCSR, P2EFromSingleBox
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.kernel import AxisSourceDerivative, AxisTargetDerivative, HelmholtzKernel
+    from sumpy_amd.tools import SourceWeights
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl, k = HelmholtzKernel(3), 3.5
+    src, _ = sphere(2500, 21)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    th, vh = tree.host, trav.host
+    lbs = th["level_start_box_nrs"]
+    rng = np.random.default_rng(order)
+    nt = th["targets"].shape[1]
+    mp = LinearPDEConformingVolumeTaylorMultipoleExpansion(knl, order)
+    lc = LinearPDEConformingVolumeTaylorLocalExpansion(knl, order)
+    n = (order + 1)**2
+    tkn = [knl] + [AxisTargetDerivative(a, knl) for a in range(3)]
+
+    def rscale(lev):
+        return tree.root_extent*2**-lev*2/order
+
+    def crand(*shape):
+        return rng.standard_normal(shape) + 1j*rng.standard_normal(shape)
+
+    def zeros():
+        return [torch.zeros(nt, dtype=torch.complex128, device=actx.device) for _ in tkn]
+
+    # M2P with gradients (direct form) and potential only (polynomial form)
+    for kernels in (tkn, [knl]):
+        got, pot = zeros()[:len(kernels)], np.zeros(nt, np.complex128)
+        grad = [np.zeros(nt, np.complex128) for _ in range(3)] if len(kernels) > 1 else None
+        found = False
+        for slev, (tbs, st, li) in enumerate(vh["sep_smaller"]):
+            if len(tbs) == 0:
+                continue
+            found = True
+            M = crand(lbs[slev + 1] - lbs[slev], n)
+            E2PFromCSR(mp, kernels)(
+                actx, src_expansions=actx.from_numpy(M), src_base_ibox=int(lbs[slev]),
+                target_boxes=actx.from_numpy(tbs), source_box_starts=actx.from_numpy(st),
+                source_box_lists=actx.from_numpy(li), centers=tree.box_centers, result=got,
+                rscale=rscale(slev), box_target_starts=tree.box_target_starts,
+                box_target_counts_nonchild=tree.box_target_counts_nonchild,
+                targets=tree.targets, k=k)
+            oh.m2p(order, tbs, st, li, th, th["box_centers"], th["targets"], rscale(slev), k,
+                   M, int(lbs[slev]), pot, grad)
+        assert found
+        for r, e in zip(got, [pot] + (grad or [])):
+            assert rel(actx.to_numpy(r), e) < 1e-11
+
+    # L2P with gradients at the finest level (real kernels on the two planes)
+    lev = tree.nlevels - 1
+    L = crand(lbs[lev + 1] - lbs[lev], n)
+    ltb = vh["level_start_target_box_nrs"]
+    tb = vh["target_boxes"][ltb[lev]:ltb[lev + 1]]
+    got = E2PFromSingleBox(lc, tkn)(
+        actx, src_expansions=actx.from_numpy(L), src_base_ibox=int(lbs[lev]),
+        target_boxes=actx.from_numpy(tb), centers=tree.box_centers, result=zeros(),
+        rscale=rscale(lev), box_target_starts=tree.box_target_starts,
+        box_target_counts_nonchild=tree.box_target_counts_nonchild, targets=tree.targets, k=k)
+    parts = []
+    for part in (L.real, L.imag):
+        pot, grad = np.zeros(nt), [np.zeros(nt) for _ in range(3)]
+        oracle.l2p(order, tb, th, th["box_centers"], th["targets"], rscale(lev),
+                   np.ascontiguousarray(part), int(lbs[lev]), pot, grad, k=k)
+        parts.append([pot, *grad])
+    for r, re_, im_ in zip(got, *parts):
+        assert rel(actx.to_numpy(r), re_ + 1j*im_) < 1e-11
+
+    # P2L (direct form with dipoles, moment form with charges) and P2M (planes)
+    q, q2 = crand(th["sources"].shape[1]), crand(th["sources"].shape[1])
+    totp = vh["target_or_target_parent_boxes"]
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    starts, lists = vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"]
+    for skn in ([knl], [knl, AxisSourceDerivative(0, knl), AxisSourceDerivative(2, knl)]):
+        strengths = [actx.from_numpy(q), actx.from_numpy(q2), actx.from_numpy(q)][:len(skn)]
+        sw = SourceWeights(actx, skn, list(range(len(skn))), strengths, {}, 3, True)
+        w0 = actx.to_numpy(sw.w0)
+        wd = None if sw.wd is None else [actx.to_numpy(w) for w in sw.wd]
+        checked = 0
+        for lev in range(tree.nlevels):
+            s0, s1 = lst[lev], lst[lev + 1]
+            if starts[s1] == starts[s0]:
+                continue
+            nbl = lbs[lev + 1] - lbs[lev]
+            out = torch.zeros((nbl, n), dtype=torch.complex128, device=actx.device)
+            P2EFromCSR(lc, kernels=skn, strength_usage=list(range(len(skn))))(
+                actx, target_boxes=actx.from_numpy(totp[s0:s1]),
+                source_box_starts=actx.from_numpy(starts[s0:s1 + 1]),
+                source_box_lists=actx.from_numpy(lists), centers=tree.box_centers,
+                sources=tree.sources, strengths=strengths,
+                box_source_starts=tree.box_source_starts,
+                box_source_counts_nonchild=tree.box_source_counts_nonchild,
+                tgt_expansions=out, tgt_base_ibox=int(lbs[lev]), rscale=rscale(lev), k=k)
+            ref = np.zeros((nbl, n), np.complex128)
+            oh.p2l(order, totp[s0:s1], starts[s0:s1 + 1], lists, th, th["box_centers"],
+                   th["sources"], w0, wd, rscale(lev), k, ref, int(lbs[lev]))
+            assert rel(actx.to_numpy(out), ref) < 1e-11
+            checked += 1
+        assert checked
+        # P2M at the finest level with the same weights
+        lsb = vh["level_start_source_box_nrs"]
+        sb = vh["source_boxes"][lsb[lev]:lsb[lev + 1]]
+        nbl = lbs[lev + 1] - lbs[lev]
+        out = torch.zeros((nbl, n), dtype=torch.complex128, device=actx.device)
+        P2EFromSingleBox(mp, kernels=skn, strength_usage=list(range(len(skn))))(
+            actx, source_boxes=actx.from_numpy(sb), box_source_starts=tree.box_source_starts,
+            box_source_counts_nonchild=tree.box_source_counts_nonchild,
+            centers=tree.box_centers, sources=tree.sources, strengths=strengths,
+            tgt_expansions=out, tgt_base_ibox=int(lbs[lev]), rscale=rscale(lev), k=k)
+        ref = []
+        for part in ("real", "imag"):
+            r = np.zeros((nbl, n))
+            oracle.p2m(order, sb, th, th["box_centers"], th["sources"],
+                       np.ascontiguousarray(getattr(w0, part)),
+                       None if wd is None else [np.ascontiguousarray(getattr(w, part))
+                                                for w in wd],
+                       rscale(lev), r, int(lbs[lev]), k=k)
+            ref.append(r)
+        assert rel(actx.to_numpy(out), ref[0] + 1j*ref[1]) < 1e-11

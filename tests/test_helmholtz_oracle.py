@@ -102,3 +102,79 @@ def test_oracle_helmholtz_fmm_converges(case):
         pot = oracle_fmm(tree, trav, q, order, exclude_self=True, k=k)
         errs.append(np.linalg.norm(pot - ref)/np.linalg.norm(ref))
     assert errs[1] < 5e-5 and errs[1] < errs[0]/20, errs
+
+
+@pytest.mark.parametrize("use_dipoles", [False, True])
+def test_p2l_then_l2p_converges_to_direct(use_dipoles):
+    """Oracle P2L (charges and dipoles) followed by L2P on the real and
+    imaginary planes against the direct Helmholtz sum: the error falls
+    geometrically with the order (pins the P2L dipole sign and scaling and the
+    (k rho)^2 projection used by L2P)."""
+    rng = np.random.default_rng(8)
+    k = 3.0
+    ns, nt = 40, 30
+    src = rng.uniform(-0.5, 0.5, (3, ns)) + np.array([[3.0], [0.5], [0.0]])
+    tgt = rng.uniform(-0.5, 0.5, (3, nt))
+    q = rng.standard_normal(ns) + 1j*rng.standard_normal(ns)
+    wd = ([rng.standard_normal(ns) + 1j*rng.standard_normal(ns) for _ in range(3)]
+          if use_dipoles else None)
+    ref, _ = oracle.p2p_dense(13, tgt, src, q, wd=wd, k=k)
+    # one target box (id 0) with center 0 and every target; one source box (id 1)
+    tree = {"box_source_starts": np.array([0, 0], np.int32),
+            "box_source_counts_nonchild": np.array([0, ns], np.int32),
+            "box_target_starts": np.array([0, 0], np.int32),
+            "box_target_counts_nonchild": np.array([nt, 0], np.int32)}
+    centers = np.zeros((3, 2))
+    errs = []
+    for order in (4, 8, 12):
+        rho = 1.0/order
+        n = (order + 1)**2
+        L = np.zeros((1, n), np.complex128)
+        oh.p2l(order, np.array([0]), np.array([0, 1]), np.array([1]), tree, centers, src,
+               q, wd, rho, k, L, 0)
+        parts = []
+        for part in (L.real, L.imag):
+            pot = np.zeros(nt)
+            oracle.l2p(order, np.array([0], np.int32), tree, centers, tgt, rho,
+                       np.ascontiguousarray(part), 0, pot, k=k)
+            parts.append(pot)
+        pot = parts[0] + 1j*parts[1]
+        errs.append(np.linalg.norm(pot - ref)/np.linalg.norm(ref))
+    assert errs[2] < 1e-5 and errs[2] < errs[1] < errs[0], errs
+
+
+def test_p2m_then_m2p_gradient_converges_to_direct():
+    """Oracle P2M (real and imaginary planes, (k rho)^2 projection) followed
+    by M2P with target gradients against the direct Helmholtz potential and
+    gradient of a far cluster."""
+    rng = np.random.default_rng(9)
+    k = 2.5
+    ns, nt = 50, 20
+    src = rng.uniform(-0.5, 0.5, (3, ns))
+    tgt = rng.uniform(-0.5, 0.5, (3, nt)) + np.array([[0.0], [3.2], [0.4]])
+    q = rng.standard_normal(ns) + 1j*rng.standard_normal(ns)
+    ref, gref = oracle.p2p_dense(13, tgt, src, q, k=k, want_grad=True)
+    tree = {"box_source_starts": np.array([0, 0], np.int32),
+            "box_source_counts_nonchild": np.array([ns, 0], np.int32),
+            "box_target_starts": np.array([0, 0], np.int32),
+            "box_target_counts_nonchild": np.array([0, nt], np.int32)}
+    centers = np.zeros((3, 2))
+    centers[:, 1] = [0.0, 3.2, 0.4]
+    errs = []
+    for order in (4, 8, 12):
+        rho = 1.0/order
+        n = (order + 1)**2
+        M = np.zeros((1, n), np.complex128)
+        for part, w in (("real", q.real), ("imag", q.imag)):
+            out = np.zeros((1, n))
+            oracle.p2m(order, np.array([0], np.int32), tree, centers, src,
+                       np.ascontiguousarray(w), None, rho, out, 0, k=k)
+            M += out if part == "real" else 1j*out
+        pot = np.zeros(nt, np.complex128)
+        grad = [np.zeros(nt, np.complex128) for _ in range(3)]
+        oh.m2p(order, np.array([1]), np.array([0, 1]), np.array([0]), tree, centers, tgt,
+               rho, k, M, 0, pot, grad)
+        e = [np.linalg.norm(pot - ref)/np.linalg.norm(ref)]
+        e += [np.linalg.norm(g - gr)/np.linalg.norm(gr) for g, gr in zip(grad, gref)]
+        errs.append(max(e))
+    assert errs[2] < 1e-5 and errs[2] < errs[1] < errs[0], errs
