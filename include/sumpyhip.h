@@ -70,6 +70,24 @@ int sumpyhip_p2p_dense(int kernel_id, int precision, double k_re, double k_im,
                        const int *target_to_source, double *pot, double *gx,
                        double *gy, double *gz, void *stream);
 
+/* Interaction matrix entries, replaces P2PMatrixGenerator.__call__ and
+ * P2PMatrixSubsetGenerator.__call__ (sumpy/p2p.py:292-477): entry e is the
+ * pair (tgtindices[e], srcindices[e]), or (e / nsources, e % nsources) for
+ * the full row-major ntargets x nsources matrix (both index arrays NULL,
+ * nentries = ntargets * nsources).  pot[e] = scale * the pair's kernel value
+ * with the source's unit-strength weights w0 (and dipole weights wx/wy/wz,
+ * NULL when no source derivative), the gradient outputs its target gradient
+ * (NULL when not needed); a self pair (target_to_source[t] == s, when given)
+ * is zero.  Complex (Helmholtz) weights and outputs are interleaved. */
+int sumpyhip_p2p_matrix(int kernel_id, double k_re, double k_im, double scale,
+                        int ntargets, const double *tx, const double *ty,
+                        const double *tz, int nsources, const double *sx,
+                        const double *sy, const double *sz, const void *w0,
+                        const void *wx, const void *wy, const void *wz,
+                        const int *target_to_source, long long nentries,
+                        const int *tgtindices, const int *srcindices, void *pot,
+                        void *gx, void *gy, void *gz, void *stream);
+
 /* Near-field sum over per-box CSR source lists, replaces P2PFromCSR.__call__
  * (sumpy/p2p.py:801-833; loop structure :551-670).  Overwrites the results
  * of the targets in target_boxes.  box_centers (dim x nboxes) give the local
@@ -326,6 +344,19 @@ int sumpyhip_combine_outputs(int is_complex, int n, int nterms, const double *co
 /* }}} */
 
 /* {{{ QBX */
+
+/* Layer-potential matrix entries, replaces LayerPotentialMatrixGenerator.__call__
+ * and LayerPotentialMatrixSubsetGenerator.__call__ (sumpy/qbx.py:342-539):
+ * entry e of the pair (tgtindices[e], srcindices[e]) (or of the full
+ * row-major matrix, index arrays NULL) is scale times the term of
+ * sumpyhip_qbx2d for a unit density; accumulate = 1 adds to out (one call per
+ * source kernel). */
+int sumpyhip_qbx2d_matrix(int order, int src_kind, int tgt_axis, int ntargets,
+                          const double *tx, const double *ty, const double *cx,
+                          const double *cy, int nsources, const double *sx,
+                          const double *sy, const double *nx, const double *ny,
+                          long long nentries, const int *tgtindices, const int *srcindices,
+                          double scale, int accumulate, double *out, void *stream);
 
 /* 2D Laplace line-Taylor layer potential, replaces LayerPotential.__call__
  * (sumpy/qbx.py:250-335) for LineTaylorLocalExpansion(LaplaceKernel(2)):

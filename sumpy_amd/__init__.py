@@ -17,7 +17,12 @@ from sumpy_amd.e2e import (
 )
 from sumpy_amd.e2p import E2PFromCSR, E2PFromSingleBox
 from sumpy_amd.p2e import P2EFromCSR, P2EFromSingleBox
-from sumpy_amd.p2p import P2P, P2PFromCSR
+from sumpy_amd.p2p import P2P, P2PFromCSR, P2PMatrixGenerator, P2PMatrixSubsetGenerator
+from sumpy_amd.qbx import (
+    LayerPotential,
+    LayerPotentialMatrixGenerator,
+    LayerPotentialMatrixSubsetGenerator,
+)
 
 __all__ = [
     "P2P",
@@ -33,4 +38,9 @@ __all__ = [
     "P2EFromCSR",
     "P2EFromSingleBox",
     "P2PFromCSR",
+    "P2PMatrixGenerator",
+    "P2PMatrixSubsetGenerator",
+    "LayerPotential",
+    "LayerPotentialMatrixGenerator",
+    "LayerPotentialMatrixSubsetGenerator",
 ]

@@ -51,6 +51,7 @@ class M2LLevel(ctypes.Structure):
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
+_L = ctypes.c_longlong
 
 _SIGNATURES = {
     "sumpyhip_device_info": [_P, _P, _P],
@@ -85,6 +86,10 @@ _SIGNATURES = {
     "sumpyhip_m2l_postprocess": [_I, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "sumpyhip_m2l_csr": [_I, _I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P,
                          _I, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_p2p_matrix": [_I, _D, _D, _D, _I, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P,
+                            _P, _L, _P, _P, _P, _P, _P, _P, _P],
+    "sumpyhip_qbx2d_matrix": [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _L, _P, _P,
+                              _D, _I, _P, _P],
     "sumpyhip_axpy_weights": [_I, _I, _D, _P, _P, _P, _P],
     "sumpyhip_combine_outputs": [_I, _I, _I, _P, _P, _P, _P, _I, _P],
     "sumpyhip_qbx2d": [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P,

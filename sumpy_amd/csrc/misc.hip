@@ -87,6 +87,57 @@ __global__ void combine_kernel(CombineArgs a) {
 
 constexpr int QB = 256;
 
+// One (target, source) term of the line-Taylor sum: a = c - s, b = t - c,
+// z = b/a; source kind 0 = SLP, 1 = DLP with normal n, 2/3 = x/y source
+// derivative; target axis -1 = potential, 0/1 = x/y target derivative.
+__device__ __forceinline__ double qbx2d_pair(int order, int src_kind, int tgt_axis,
+                                             double are, double aim, double bre, double bim,
+                                             double nx, double ny) {
+  const double aa = are*are + aim*aim;
+  const double ia_re = are/aa, ia_im = -aim/aa;  // 1/a
+  const double zre = bre*ia_re - bim*ia_im, zim = bre*ia_im + bim*ia_re;
+  if (src_kind == 0 && tgt_axis < 0) {
+    // log a + sum_{i=1}^p (-1)^{i-1} z^i / i   (real part)
+    double hre = 0, him = 0;
+    for (int i = order; i >= 1; --i) {
+      const double c = ((i & 1) ? 1.0 : -1.0)/i;
+      const double nre = hre*zre - him*zim + c, nim = hre*zim + him*zre;
+      hre = nre; him = nim;
+    }
+    return 0.5*log(aa) + (hre*zre - him*zim);  // times z
+  }
+  // value = Re[ w * sum_{i=0}^p c_i (-z)^i ]
+  //   SLP + d/da: w = 1/a, c_i = 1 (times e_axis factor)
+  //   DLP/axis source deriv: w = -n/a, c_i = 1;  with d/da: w = n/a^2, c_i = i+1
+  double wre, wim;
+  int lin = 0;
+  if (src_kind == 0) {
+    wre = ia_re; wim = ia_im;
+  } else {
+    double nre, nim;
+    if (src_kind == 1) { nre = nx; nim = ny; }
+    else if (src_kind == 2) { nre = 1; nim = 0; }
+    else { nre = 0; nim = 1; }
+    if (tgt_axis < 0) {
+      wre = -(nre*ia_re - nim*ia_im); wim = -(nre*ia_im + nim*ia_re);
+    } else {
+      const double a2re = ia_re*ia_re - ia_im*ia_im, a2im = 2*ia_re*ia_im;
+      wre = nre*a2re - nim*a2im; wim = nre*a2im + nim*a2re;
+      lin = 1;
+    }
+  }
+  double hre = 0, him = 0;
+  const double mzre = -zre, mzim = -zim;
+  for (int i = order; i >= 0; --i) {
+    const double c = lin ? double(i + 1) : 1.0;
+    const double nre = hre*mzre - him*mzim + c, nim = hre*mzim + him*mzre;
+    hre = nre; him = nim;
+  }
+  double pre = wre*hre - wim*him, pim = wre*him + wim*hre;
+  if (tgt_axis == 1) pre = -pim;  // times i
+  return pre;
+}
+
 __global__ __launch_bounds__(QB) void qbx2d_kernel(int order, int src_kind, int tgt_axis,
                                                    int nt, const double *tx, const double *ty,
                                                    const double *cx, const double *cy, int ns,
@@ -111,59 +162,34 @@ __global__ __launch_bounds__(QB) void qbx2d_kernel(int order, int src_kind, int 
     __syncthreads();
     const int nn = min(QB, ns - sc);
     for (int j = 0; j < nn; ++j) {
-      // a = c - s; z = b/a
-      const double are = ccx - s_x[j], aim = ccy - s_y[j];
-      const double aa = are*are + aim*aim;
-      const double ia_re = are/aa, ia_im = -aim/aa;  // 1/a
-      const double zre = bre*ia_re - bim*ia_im, zim = bre*ia_im + bim*ia_re;
-      double val;
-      if (src_kind == 0 && tgt_axis < 0) {
-        // log a + sum_{i=1}^p (-1)^{i-1} z^i / i   (real part)
-        double hre = 0, him = 0;
-        for (int i = order; i >= 1; --i) {
-          const double c = ((i & 1) ? 1.0 : -1.0)/i;
-          const double nre = hre*zre - him*zim + c, nim = hre*zim + him*zre;
-          hre = nre; him = nim;
-        }
-        // times z
-        const double sre = hre*zre - him*zim;
-        val = 0.5*log(aa) + sre;
-      } else {
-        // value = Re[ w * sum_{i=0}^p c_i (-z)^i ]
-        //   SLP + d/da: w = 1/a, c_i = 1 (times e_axis factor)
-        //   DLP/axis source deriv: w = -n/a, c_i = 1;  with d/da: w = n/a^2, c_i = i+1
-        double wre, wim;
-        int lin = 0;
-        if (src_kind == 0) {
-          wre = ia_re; wim = ia_im;
-        } else {
-          double nre, nim;
-          if (src_kind == 1) { nre = s_nx[j]; nim = s_ny[j]; }
-          else if (src_kind == 2) { nre = 1; nim = 0; }
-          else { nre = 0; nim = 1; }
-          if (tgt_axis < 0) {
-            wre = -(nre*ia_re - nim*ia_im); wim = -(nre*ia_im + nim*ia_re);
-          } else {
-            const double a2re = ia_re*ia_re - ia_im*ia_im, a2im = 2*ia_re*ia_im;
-            wre = nre*a2re - nim*a2im; wim = nre*a2im + nim*a2re;
-            lin = 1;
-          }
-        }
-        double hre = 0, him = 0;
-        const double mzre = -zre, mzim = -zim;
-        for (int i = order; i >= 0; --i) {
-          const double c = lin ? double(i + 1) : 1.0;
-          const double nre = hre*mzre - him*mzim + c, nim = hre*mzim + him*mzre;
-          hre = nre; him = nim;
-        }
-        double pre = wre*hre - wim*him, pim = wre*him + wim*hre;
-        if (tgt_axis == 1) { const double q = pre; pre = -pim; pim = q; }  // times i
-        val = pre;
-      }
+      const double val = qbx2d_pair(order, src_kind, tgt_axis, ccx - s_x[j], ccy - s_y[j],
+                                    bre, bim, src_kind == 1 ? s_nx[j] : 0.0,
+                                    src_kind == 1 ? s_ny[j] : 0.0);
       res += s_d[j]*val;
     }
   }
   if (valid) out[t] = scale*res;
+}
+
+// Matrix entries (reference LayerPotentialMatrixGenerator and
+// LayerPotentialMatrixSubsetGenerator, sumpy/qbx.py:342-539): entry e is the
+// pair (tidx[e], sidx[e]), or (e / ns, e % ns) for the full row-major matrix;
+// out[e] (+)= scale * term (accumulate: one call per source kernel).
+__global__ __launch_bounds__(QB) void qbx2d_matrix_kernel(
+    int order, int src_kind, int tgt_axis, int ns, long long nentries, const int *tidx,
+    const int *sidx, const double *tx, const double *ty, const double *cx, const double *cy,
+    const double *sx, const double *sy, const double *nx, const double *ny, double scale,
+    int accumulate, double *out) {
+  for (long long e = blockIdx.x*(long long)QB + threadIdx.x; e < nentries;
+       e += (long long)gridDim.x*QB) {
+    int t, s;
+    if (tidx) { t = tidx[e]; s = sidx[e]; }
+    else { t = (int)(e/ns); s = (int)(e - (long long)t*ns); }
+    const double val = qbx2d_pair(order, src_kind, tgt_axis, cx[t] - sx[s], cy[t] - sy[s],
+                                  tx[t] - cx[t], ty[t] - cy[t],
+                                  src_kind == 1 ? nx[s] : 0.0, src_kind == 1 ? ny[s] : 0.0);
+    out[e] = (accumulate ? out[e] : 0.0) + scale*val;
+  }
 }
 
 // }}}
@@ -410,4 +436,29 @@ extern "C" int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, con
     return check_launch("qbx2d_grad_sources");
   }
   return SUMPYHIP_OK;
+}
+
+extern "C" int sumpyhip_qbx2d_matrix(int order, int src_kind, int tgt_axis, int ntargets,
+                                     const double *tx, const double *ty, const double *cx,
+                                     const double *cy, int nsources, const double *sx,
+                                     const double *sy, const double *nx, const double *ny,
+                                     long long nentries, const int *tgtindices,
+                                     const int *srcindices, double scale, int accumulate,
+                                     double *out, void *stream) {
+  SUMPY_REQUIRE(order >= 0 && order <= 64, "qbx2d_matrix: order out of range");
+  SUMPY_REQUIRE(src_kind >= 0 && src_kind <= 3, "qbx2d_matrix: unknown source kind %d",
+                src_kind);
+  SUMPY_REQUIRE(tgt_axis >= -1 && tgt_axis <= 1, "qbx2d_matrix: target axis must be -1, 0 or 1");
+  if (src_kind == 1) SUMPY_REQUIRE(nx && ny, "qbx2d_matrix: DLP needs source normals");
+  SUMPY_REQUIRE((tgtindices == nullptr) == (srcindices == nullptr),
+                "qbx2d_matrix: give both index arrays or neither");
+  SUMPY_REQUIRE(tgtindices || nentries == (long long)ntargets*nsources,
+                "qbx2d_matrix: the full matrix has ntargets * nsources entries");
+  if (nentries == 0) return SUMPYHIP_OK;
+  const long long nb = (nentries + QB - 1)/QB;
+  const int grid = (int)(nb < 65536 ? nb : 65536);
+  qbx2d_matrix_kernel<<<grid, QB, 0, (hipStream_t)stream>>>(
+      order, src_kind, tgt_axis, nsources, nentries, tgtindices, srcindices, tx, ty, cx, cy,
+      sx, sy, nx, ny, scale, accumulate, out);
+  return check_launch("qbx2d_matrix");
 }

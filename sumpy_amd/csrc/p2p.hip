@@ -625,6 +625,92 @@ int check_common(int knl, int prec, const P2PArgs &a) {
 
 // }}}
 
+// {{{ interaction matrices: one entry per thread
+
+// Entry e is the pair (itgt, isrc) = (tidx[e], sidx[e]), or (e / ns, e % ns)
+// for the full row-major matrix (tidx null).  The entry holds the unscaled
+// kernel of the pair with the source's unit-strength weights (charge w0 and
+// dipole wd, the source kernels folded as in the direct sums) times scale:
+// its potential and, with GRAD, its target gradient.  A self pair (exclude
+// self, t2s[itgt] == isrc) is zero.  Reference: P2PMatrixGenerator and
+// P2PMatrixSubsetGenerator (sumpy/p2p.py:292-477).
+struct MatArgs {
+  double scale, kre, kim;
+  int nt, ns;
+  long long nentries;
+  const double *tx, *ty, *tz, *sx, *sy, *sz;
+  const double *w0, *wx, *wy, *wz;
+  const int *t2s, *tidx, *sidx;
+  double *pot, *gx, *gy, *gz;
+};
+
+__device__ __forceinline__ void entry_pair(const MatArgs &a, long long e, int &it, int &is) {
+  if (a.tidx) {
+    it = a.tidx[e];
+    is = a.sidx[e];
+  } else {
+    it = (int)(e/a.ns);
+    is = (int)(e - (long long)it*a.ns);
+  }
+}
+
+template <int KNL, bool DIP, bool GRAD>
+__global__ __launch_bounds__(256) void p2p_matrix_real(MatArgs a) {
+  for (long long e = blockIdx.x*256ll + threadIdx.x; e < a.nentries;
+       e += (long long)gridDim.x*256) {
+    int it, is;
+    entry_pair(a, e, it, is);
+    double pot = 0, g0 = 0, g1 = 0, g2 = 0;
+    if (!(a.t2s && a.t2s[it] == is)) {
+      const double dx = a.tx[it] - a.sx[is], dy = a.ty[it] - a.sy[is];
+      const double dz = KNL == KLAP3 ? a.tz[it] - a.sz[is] : 0.0;
+      pair_real<KNL, DIP, GRAD, double>(dx, dy, dz, a.w0[is], DIP ? a.wx[is] : 0.0,
+                                        DIP ? a.wy[is] : 0.0,
+                                        DIP && KNL == KLAP3 ? a.wz[is] : 0.0, pot, g0, g1, g2);
+    }
+    if (a.pot) a.pot[e] = a.scale*pot;
+    if (GRAD) {
+      a.gx[e] = a.scale*g0;
+      a.gy[e] = a.scale*g1;
+      if (KNL == KLAP3) a.gz[e] = a.scale*g2;
+    }
+  }
+}
+
+template <bool DIP, bool GRAD>
+__global__ __launch_bounds__(256) void p2p_matrix_helm(MatArgs a) {
+  for (long long e = blockIdx.x*256ll + threadIdx.x; e < a.nentries;
+       e += (long long)gridDim.x*256) {
+    int it, is;
+    entry_pair(a, e, it, is);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!(a.t2s && a.t2s[it] == is)) {
+      double wd[6] = {0, 0, 0, 0, 0, 0};
+      if (DIP) {
+        wd[0] = a.wx[2*(size_t)is]; wd[1] = a.wx[2*(size_t)is + 1];
+        wd[2] = a.wy[2*(size_t)is]; wd[3] = a.wy[2*(size_t)is + 1];
+        wd[4] = a.wz[2*(size_t)is]; wd[5] = a.wz[2*(size_t)is + 1];
+      }
+      pair_helm<DIP, GRAD, double>(a.tx[it] - a.sx[is], a.ty[it] - a.sy[is],
+                                   a.tz[it] - a.sz[is], a.kre, a.kim, a.w0[2*(size_t)is],
+                                   a.w0[2*(size_t)is + 1], wd, acc);
+    }
+    if (a.pot) {
+      a.pot[2*e] = a.scale*acc[0];
+      a.pot[2*e + 1] = a.scale*acc[1];
+    }
+    if (GRAD) {
+      double *g[3] = {a.gx, a.gy, a.gz};
+      for (int b = 0; b < 3; ++b) {
+        g[b][2*e] = a.scale*acc[2 + 2*b];
+        g[b][2*e + 1] = a.scale*acc[3 + 2*b];
+      }
+    }
+  }
+}
+
+// }}}
+
 }  // namespace
 }  // namespace sumpyhip
 
@@ -697,4 +783,51 @@ extern "C" int sumpyhip_p2p_csr(int kernel_id, int precision, double k_re, doubl
   if (dip) return launch_csr<true, false>(kernel_id, precision, a, st);
   if (grad) return launch_csr<false, true>(kernel_id, precision, a, st);
   return launch_csr<false, false>(kernel_id, precision, a, st);
+}
+
+extern "C" int sumpyhip_p2p_matrix(int kernel_id, double k_re, double k_im, double scale,
+                                   int ntargets, const double *tx, const double *ty,
+                                   const double *tz, int nsources, const double *sx,
+                                   const double *sy, const double *sz, const void *w0,
+                                   const void *wx, const void *wy, const void *wz,
+                                   const int *target_to_source, long long nentries,
+                                   const int *tgtindices, const int *srcindices, void *pot,
+                                   void *gx, void *gy, void *gz, void *stream) {
+  SUMPY_REQUIRE(kernel_id == 2 || kernel_id == 3 || kernel_id == 13,
+                "p2p_matrix: unknown kernel %d", kernel_id);
+  const int dim = kernel_id == 2 ? 2 : 3;
+  SUMPY_REQUIRE(tx && ty && sx && sy && w0 && (dim == 2 || (tz && sz)),
+                "p2p_matrix: null coordinates or weights");
+  SUMPY_REQUIRE(pot || gx, "p2p_matrix: no output");
+  SUMPY_REQUIRE(!gx || (gy && (dim == 2 || gz)), "p2p_matrix: incomplete gradient outputs");
+  SUMPY_REQUIRE(!wx || (wy && (dim == 2 || wz)), "p2p_matrix: incomplete dipole weights");
+  SUMPY_REQUIRE((tgtindices == nullptr) == (srcindices == nullptr),
+                "p2p_matrix: give both index arrays or neither");
+  SUMPY_REQUIRE(tgtindices || nentries == (long long)ntargets*nsources,
+                "p2p_matrix: the full matrix has ntargets * nsources entries");
+  if (nentries == 0) return SUMPYHIP_OK;
+  MatArgs a{scale, k_re, k_im, ntargets, nsources, nentries, tx, ty, tz, sx, sy, sz,
+            (const double *)w0, (const double *)wx, (const double *)wy, (const double *)wz,
+            target_to_source, tgtindices, srcindices, (double *)pot, (double *)gx,
+            (double *)gy, (double *)gz};
+  const long long nb = (nentries + 255)/256;
+  const int grid = (int)(nb < 65536 ? nb : 65536);
+  hipStream_t st = (hipStream_t)stream;
+  const bool dip = wx != nullptr, grad = gx != nullptr;
+#define SUMPY_MAT(LAUNCH)                                                  \
+  if (dip && grad) LAUNCH(true, true);                                     \
+  else if (dip) LAUNCH(true, false);                                       \
+  else if (grad) LAUNCH(false, true);                                      \
+  else LAUNCH(false, false);
+#define SUMPY_L3(D, G) p2p_matrix_real<KLAP3, D, G><<<grid, 256, 0, st>>>(a)
+#define SUMPY_L2(D, G) p2p_matrix_real<KLAP2, D, G><<<grid, 256, 0, st>>>(a)
+#define SUMPY_H3(D, G) p2p_matrix_helm<D, G><<<grid, 256, 0, st>>>(a)
+  if (kernel_id == 3) { SUMPY_MAT(SUMPY_L3) }
+  else if (kernel_id == 2) { SUMPY_MAT(SUMPY_L2) }
+  else { SUMPY_MAT(SUMPY_H3) }
+#undef SUMPY_L3
+#undef SUMPY_L2
+#undef SUMPY_H3
+#undef SUMPY_MAT
+  return check_launch("p2p_matrix");
 }

@@ -14,6 +14,7 @@ from __future__ import annotations
 import torch
 
 from sumpy_amd import _capi
+from sumpy_amd.array_context import obj_array
 from sumpy_amd.kernel import (
     kernel_id,
     remove_source_transforms,
@@ -140,4 +141,88 @@ class P2PFromCSR(P2PBase):
         return result
 
 
-__all__ = ["P2P", "P2PFromCSR"]
+class _P2PMatrixBase(P2PBase):
+    """Matrix entries: each source kernel at unit strength (the reference's
+    ``get_strength_or_not`` returns 1), scaled per target kernel; with
+    ``exclude_self`` the self pairs are zero."""
+
+    def _entries(self, actx, targets, sources, tgtindices, srcindices, kwargs):
+        tgt = as_component_list(targets, self.dim)
+        src = as_component_list(sources, self.dim)
+        nt, ns = tgt[0].shape[0], src[0].shape[0]
+        ones = torch.ones(ns, dtype=torch.float64, device=actx.device)
+        is_complex = self.base_kernel.is_complex_valued
+        weights = SourceWeights(actx, self.source_kernels, list(range(len(self.source_kernels))),
+                                [ones]*len(self.source_kernels), kwargs, self.dim, is_complex)
+        t2s = None
+        if self.exclude_self:
+            t2s = kwargs.get("target_to_source")
+            if t2s is None:
+                raise ValueError("exclude_self=True needs 'target_to_source'")
+            t2s = as_i32(t2s)
+        k = kernel_k(self.base_kernel, kwargs)
+        if tgtindices is None:
+            nent, ti, si = nt*ns, None, None
+        else:
+            ti, si = as_i32(tgtindices), as_i32(srcindices)
+            if ti.shape != si.shape:
+                raise ValueError("tgtindices and srcindices must have the same length")
+            nent = int(ti.shape[0])
+        outs = TargetOutputs(self.target_kernels, self.dim)
+        dt = torch.complex128 if is_complex else torch.float64
+        raw = [torch.empty(nent, dtype=dt, device=actx.device) if need else None
+               for need in [outs.need_pot] + [outs.need_grad]*self.dim]
+        ptrs = [None if r is None else torch.view_as_real(r).data_ptr() if is_complex
+                else r.data_ptr() for r in raw] + [None]*(3 - self.dim)
+        tp = [t.data_ptr() for t in tgt] + [None]*(3 - self.dim)
+        sp = [x.data_ptr() for x in src] + [None]*(3 - self.dim)
+        _capi.call("sumpyhip_p2p_matrix", kernel_id(self.base_kernel), k.real, k.imag,
+                   self.base_kernel.get_global_scaling_const(), nt, *tp, ns, *sp,
+                   *weights.ptrs(3), _capi.ptr(t2s), nent, _capi.ptr(ti), _capi.ptr(si),
+                   *ptrs, actx.stream_handle)
+        # target kernels: linear combinations of the potential and gradient
+        # entries, with target-coordinate factors where the kernel has them
+        row = None
+        results = []
+        for ts, vdt in zip(outs.terms, self.value_dtypes):
+            acc = None
+            for t in ts:
+                v = raw[0 if t.deriv < 0 else 1 + t.deriv]*t.coeff
+                if t.mult_axis >= 0:
+                    if row is None:
+                        row = (torch.arange(nt, device=actx.device).repeat_interleave(ns)
+                               if ti is None else ti.long())
+                    v = v*tgt[t.mult_axis][row]
+                acc = v if acc is None else acc + v
+            results.append(acc.to(_torch_value_dtype(vdt)))
+        return results, nt, ns
+
+
+def _torch_value_dtype(d):
+    from sumpy_amd.tools import _torch_dtype
+    return _torch_dtype(d)
+
+
+class P2PMatrixGenerator(_P2PMatrixBase):
+    """The ntargets x nsources interaction matrix of each target kernel
+    (reference ``p2p.py:292-345``)."""
+
+    default_name = "p2p_matrix"
+
+    def __call__(self, actx, targets, sources, **kwargs):
+        res, nt, ns = self._entries(actx, targets, sources, None, None, kwargs)
+        return obj_array([r.view(nt, ns) for r in res])
+
+
+class P2PMatrixSubsetGenerator(_P2PMatrixBase):
+    """The entries at the index pairs (tgtindices[i], srcindices[i])
+    (reference ``p2p.py:351-477``)."""
+
+    default_name = "p2p_subset"
+
+    def __call__(self, actx, targets, sources, *, tgtindices, srcindices, **kwargs):
+        res, _, _ = self._entries(actx, targets, sources, tgtindices, srcindices, kwargs)
+        return obj_array(res)
+
+
+__all__ = ["P2P", "P2PFromCSR", "P2PMatrixGenerator", "P2PMatrixSubsetGenerator"]

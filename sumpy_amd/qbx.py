@@ -102,4 +102,64 @@ class LayerPotential(LayerPotentialBase):
         return obj_array(results)
 
 
-__all__ = ["LayerPotential", "LayerPotentialBase"]
+class _LayerPotentialMatrixBase(LayerPotentialBase):
+    """Entries of the layer-potential matrix: each source kernel at unit
+    density, summed (reference ``qbx.py:342-539``)."""
+
+    def _entries(self, actx, targets, sources, centers, tgtindices, srcindices, kwargs):
+        tgt = as_component_list(targets, 2)
+        src = as_component_list(sources, 2)
+        ctr = as_component_list(centers, 2)
+        nt, ns = tgt[0].shape[0], src[0].shape[0]
+        if tgtindices is None:
+            nent, ti, si = nt*ns, None, None
+        else:
+            from sumpy_amd.tools import as_i32
+            ti, si = as_i32(tgtindices), as_i32(srcindices)
+            if ti.shape != si.shape:
+                raise ValueError("tgtindices and srcindices must have the same length")
+            nent = int(ti.shape[0])
+        scale = LaplaceKernel(2).get_global_scaling_const()
+        results = []
+        for tk in self.target_kernels:
+            ax = _tgt_axis(tk)
+            out = torch.empty(nent, dtype=torch.float64, device=actx.device)
+            for j, sk in enumerate(self.source_kernels):
+                kind, dname = _src_kind(sk)
+                nrm = as_component_list(kwargs[dname], 2) if dname else [None, None]
+                _capi.call("sumpyhip_qbx2d_matrix", self.expansion.order, kind, ax, nt,
+                           tgt[0].data_ptr(), tgt[1].data_ptr(), ctr[0].data_ptr(),
+                           ctr[1].data_ptr(), ns, src[0].data_ptr(), src[1].data_ptr(),
+                           _capi.ptr(nrm[0]), _capi.ptr(nrm[1]), nent, _capi.ptr(ti),
+                           _capi.ptr(si), scale, int(j > 0), out.data_ptr(),
+                           actx.stream_handle)
+            results.append(out)
+        return results, nt, ns
+
+
+class LayerPotentialMatrixGenerator(_LayerPotentialMatrixBase):
+    """The ntargets x nsources matrix of each target kernel
+    (reference ``qbx.py:342-407``)."""
+
+    default_name = "qbx_matrix"
+
+    def __call__(self, actx, targets, sources, centers, expansion_radii, **kwargs):
+        res, nt, ns = self._entries(actx, targets, sources, centers, None, None, kwargs)
+        return obj_array([r.view(nt, ns) for r in res])
+
+
+class LayerPotentialMatrixSubsetGenerator(_LayerPotentialMatrixBase):
+    """The entries at the index pairs (tgtindices[i], srcindices[i])
+    (reference ``qbx.py:412-539``)."""
+
+    default_name = "qbx_subset"
+
+    def __call__(self, actx, targets, sources, centers, expansion_radii, tgtindices,
+                 srcindices, **kwargs):
+        res, _, _ = self._entries(actx, targets, sources, centers, tgtindices, srcindices,
+                                  kwargs)
+        return obj_array(res)
+
+
+__all__ = ["LayerPotential", "LayerPotentialBase", "LayerPotentialMatrixGenerator",
+           "LayerPotentialMatrixSubsetGenerator"]
