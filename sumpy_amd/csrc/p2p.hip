@@ -304,29 +304,43 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
           }
           __syncthreads();
           const int jself = pself - c0;
-          for (int j0 = sub; j0 < nn; j0 += G*FOLD) {
-            const int jend = min(nn, j0 + G*FOLD);
-            T tp = 0, t0g = 0, t1g = 0, t2g = 0;
-            if constexpr (F32) {
-              // FP32: this lane's own (excluded) pair, if it falls in this
-              // block, is skipped by splitting the loop there (no per-pair
-              // mask; the FP32 loop is VALU-issue bound)
-              const int js = (SELF && jself >= j0 && jself < jend && (jself - j0) % G == 0)
-                                 ? jself : jend;
-              auto run = [&](int jb, int je) {
+          if constexpr (F32) {
+            // FP32: lane sub takes a contiguous run of the chunk's sources,
+            // [sub L, sub L + cnt), L = ceil(nn / G) rounded up to odd (the G
+            // runs of a wave then start on different LDS banks), so the
+            // unrolled loop reads at constant offsets.  The lane's own
+            // (excluded) pair is skipped by splitting the loop there (no
+            // per-pair mask; the FP32 loop is VALU-issue bound).
+            const int L = ((nn + G - 1)/G) | 1;
+            const int cnt = max(0, min(L, nn - sub*L));
+            const int iself = (SELF && jself >= sub*L && jself < sub*L + cnt) ? jself - sub*L : -1;
+            const v4 *bs = s_src + sub*L;
+            const v4 *bd = s_dip + (DIP ? sub*L : 0);
+            for (int i0 = 0; i0 < cnt; i0 += FOLD) {
+              const int i1 = min(cnt, i0 + FOLD);
+              T tp = 0, t0g = 0, t1g = 0, t2g = 0;
+              const int is = (iself >= i0 && iself < i1) ? iself : i1;
+              auto run = [&](int ib, int ie) {
                 #pragma unroll 4
-                for (int j = jb; j < je; j += G) {
-                  const v4 sv = s_src[j];
+                for (int i = ib; i < ie; ++i) {
+                  const v4 sv = bs[i];
                   v4 dv = v4{0, 0, 0, 0};
-                  if constexpr (DIP) dv = s_dip[j];
+                  if constexpr (DIP) dv = bd[i];
                   pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y,
                                                  DIM == 3 ? pz - sv.z : T(0), sv.w, dv.x, dv.y,
                                                  dv.z, false, tp, t0g, t1g, t2g);
                 }
               };
-              run(j0, js);
-              run(js + G, jend);
-            } else {
+              run(i0, is);
+              run(is + 1, i1);
+              pot += double(tp);
+              if (GRAD) { g0 += double(t0g); g1 += double(t1g); g2 += double(t2g); }
+            }
+          } else
+          for (int j0 = sub; j0 < nn; j0 += G*FOLD) {
+            const int jend = min(nn, j0 + G*FOLD);
+            T tp = 0, t0g = 0, t1g = 0, t2g = 0;
+            {
               #pragma unroll 4
               for (int j = j0; j < jend; j += G) {
                 const v4 sv = s_src[j];
