@@ -488,9 +488,16 @@ __global__ __launch_bounds__(256) void p2l_helm3d_moments(
     int tbase) {
   constexpr int NS = (ORD + 1)*(ORD + 1), NC = M2PPoly<ORD>::NC, HS = ORD + 1;
   constexpr int CH = P2LH_CH, NP = ORD + 1;
-  __shared__ double2 sQG[NP][CH];   // q g_q per source
-  __shared__ double sY[NP][CH], sZ[NP][CH], sX[CH];
-  __shared__ double2 sPhi[NC];
+  // the source staging (q g_q, powers) and, after the last source, the
+  // moments share one LDS buffer
+  constexpr int STAGE = NP*CH*2 + 2*NP*CH + CH;  // doubles
+  constexpr int BUF = STAGE > 2*NC ? STAGE : 2*NC;
+  __shared__ __align__(16) double sbuf[BUF];
+  double2 (*sQG)[CH] = reinterpret_cast<double2 (*)[CH]>(sbuf);
+  double (*sY)[CH] = reinterpret_cast<double (*)[CH]>(sbuf + NP*CH*2);
+  double (*sZ)[CH] = reinterpret_cast<double (*)[CH]>(sbuf + NP*CH*3);
+  double *sX = sbuf + NP*CH*4;
+  double2 *sPhi = reinterpret_cast<double2 *>(sbuf);
   __shared__ double ht[HS*HS];
   herm_table(ht, ORD);
   const int tid = threadIdx.x;
@@ -554,6 +561,7 @@ __global__ __launch_bounds__(256) void p2l_helm3d_moments(
         }
         __syncthreads();
         if (tid < NS) {
+          #pragma unroll 2
           for (int j = 0; j < n; ++j) {
             const double2 qg = sQG[ra + rn][j];
             const double m = (ra ? sX[j] : 1.0)*sY[rb][j];
