@@ -14,6 +14,15 @@ additions on top of its operators, built from the same HIP kernels:
   layer potential (``sumpy/qbx.py:250-335``); its backward pass is
   ``sumpyhip_qbx2d_adjoint`` (density, sources, normals, targets, centres).
 
+* :func:`fmm_potential` -- a whole FMM pass (``drive_fmm`` over a
+  :class:`~sumpy_amd.fmm.SumpyExpansionWrangler`) for sources = targets with
+  self-interaction excluded.  The interaction matrix is then symmetric
+  (G(t - s) = G(s - t)), so the adjoint for the strengths is one more FMM
+  pass with the conjugate kernel (the same wrangler for Laplace; k -> -k for
+  a real Helmholtz wavenumber, conj G_k = G_{-k}); every expansion is
+  recomputed by that pass, nothing is stored.  It agrees with the exact
+  transpose of the discrete FMM operator to within the FMM's accuracy.
+
 Complex outputs follow PyTorch's convention: for a real loss L the incoming
 gradient is dL/dRe y + i dL/dIm y, and the gradient of a real input x is
 sum_t Re(conj(g_t) dy_t/dx).
@@ -204,3 +213,65 @@ def qbx_layer_potential(targets, sources, centers, density, order, normals=None)
 
 
 __all__ = ["p2p_potential", "qbx_layer_potential"]
+
+
+# {{{ FMM (strength gradient by the adjoint pass)
+
+class _FMMPotential(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, strength, actx, wrangler, adjoint_wrangler):
+        from sumpy_amd.drive import drive_fmm
+        ctx.actx, ctx.adjoint = actx, adjoint_wrangler
+        ctx.real_input = not strength.is_complex()
+        pot, = drive_fmm(actx, wrangler, [strength.detach().contiguous()])
+        return pot
+
+    @staticmethod
+    def backward(ctx, g):
+        from sumpy_amd.drive import drive_fmm
+        grad, = drive_fmm(ctx.actx, ctx.adjoint, [g.contiguous()])
+        if ctx.real_input and grad.is_complex():
+            grad = grad.real
+        return grad.contiguous(), None, None, None
+
+
+def fmm_potential(actx, wrangler, strength, adjoint_wrangler=None):
+    """Differentiable FMM potential of *strength* (user order) for one target
+    kernel, sources = targets, self excluded (the wrangler's
+    ``target_to_source``).  *adjoint_wrangler* evaluates the adjoint pass; by
+    default the wrangler itself (real kernels).  For the Helmholtz kernel pass
+    a wrangler over the same tree with the wavenumber negated
+    (:func:`helmholtz_adjoint_wrangler`)."""
+    tree = wrangler.tree
+    if not tree.sources_are_targets:
+        raise NotImplementedError("fmm_potential: sources and targets must coincide")
+    if len(wrangler.tree_indep.target_kernels) != 1:
+        raise ValueError("fmm_potential: one target kernel")
+    if adjoint_wrangler is None:
+        if wrangler.tree_indep.get_base_kernel().is_complex_valued:
+            raise ValueError("fmm_potential: complex kernels need an adjoint wrangler")
+        adjoint_wrangler = wrangler
+    return _FMMPotential.apply(strength, actx, wrangler, adjoint_wrangler)
+
+
+def helmholtz_adjoint_wrangler(wrangler):
+    """A wrangler for the adjoint pass of a real-wavenumber Helmholtz FMM:
+    same tree, traversal, orders and options, with k -> -k (its translation
+    class data are generated for -k)."""
+    from sumpy_amd.fmm import SumpyExpansionWrangler
+    kw = dict(wrangler.kernel_extra_kwargs)
+    name = wrangler.tree_indep.get_base_kernel().get_args()[0]
+    k = complex(kw[name])
+    if k.imag:
+        raise NotImplementedError("the adjoint wrangler needs a real wavenumber")
+    kw[name] = -k.real
+    orders = list(wrangler.level_orders)
+    return SumpyExpansionWrangler(
+        wrangler.tree_indep, wrangler.traversal, wrangler.dtype,
+        fmm_level_to_order=lambda kn, a, t, lev: orders[lev],
+        source_extra_kwargs=wrangler.source_extra_kwargs, kernel_extra_kwargs=kw,
+        self_extra_kwargs=wrangler.self_extra_kwargs,
+        translation_classes_data=wrangler.translation_classes_data,
+        preprocessed_mpole_dtype=wrangler.preprocessed_mpole_dtype)
+
+# }}}

@@ -153,3 +153,62 @@ def test_qbx_adjoint_matches_forward_derivative(actx):
         e[s] = 1
         ref = float(qbx_layer_potential(X, X, ctr, e, 5, normals=n)[7])
         assert math.isclose(float(sigma.grad[s]), ref, rel_tol=1e-12, abs_tol=1e-14)
+
+
+@pytest.mark.parametrize("kernel", ["laplace", "helmholtz"])
+def test_fmm_strength_adjoint_fd(actx, kernel):
+    """Strength gradient of a whole FMM pass (sources = targets, self
+    excluded) by the adjoint pass, against central differences of the FMM
+    itself on 64 random strengths.  The FMM is linear in the strengths, so
+    the differences are its exact transpose applied to the loss weights; the
+    adjoint pass (the FMM of the conjugate kernel) equals it to within the
+    FMM's accuracy at order 10 (tolerance 1e-5 relative)."""
+    from functools import partial
+
+    from sumpy_amd.autograd import fmm_potential, helmholtz_adjoint_wrangler
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    rng = np.random.default_rng(12)
+    n = 3000
+    src = rng.random((3, n))
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    knl = LaplaceKernel(3) if kernel == "laplace" else HelmholtzKernel(3)
+    cplx = kernel == "helmholtz"
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl), [knl], exclude_self=True)
+    wr = SumpyExpansionWrangler(
+        ti, trav, np.complex128 if cplx else np.float64,
+        fmm_level_to_order=lambda kn, a, t, lev: 10,
+        kernel_extra_kwargs={"k": 3.0} if cplx else None,
+        self_extra_kwargs={"target_to_source": actx.from_numpy(np.arange(n, dtype=np.int32))})
+    adj = helmholtz_adjoint_wrangler(wr) if cplx else None
+    q = rng.standard_normal(n) + (1j*rng.standard_normal(n) if cplx else 0)
+    w = rng.standard_normal(n) + (1j*rng.standard_normal(n) if cplx else 0)
+    qt = actx.from_numpy(q).requires_grad_(True)
+    wt = actx.from_numpy(w)
+    pot = fmm_potential(actx, wr, qt, adj)
+    loss = (torch.conj(wt)*pot).real.sum() if cplx else (wt*pot).sum()
+    loss.backward()
+    grad = actx.to_numpy(qt.grad)
+
+    def L(qq):
+        p, = drive_fmm(actx, wr, [actx.from_numpy(qq)])
+        p = actx.to_numpy(p)
+        return float(np.real(np.vdot(w, p))) if cplx else float(w @ p)
+
+    idx = rng.choice(n, 64, replace=False)
+    h = 1e-6
+    fd_re, fd_im = [], []
+    for j in idx:
+        e = np.zeros(n, q.dtype)
+        e[j] = h
+        fd_re.append((L(q + e) - L(q - e))/(2*h))
+        if cplx:
+            fd_im.append((L(q + 1j*e) - L(q - 1j*e))/(2*h))
+    fd = np.array(fd_re) + (1j*np.array(fd_im) if cplx else 0)
+    assert np.linalg.norm(grad[idx] - fd)/np.linalg.norm(fd) < 1e-5
