@@ -233,6 +233,19 @@ int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt, const int *c
                                       int nvectors, int class_start, double src_rscale,
                                       double k, double *out, void *stream);
 
+/* M2L without translation classes for the Helmholtz kernel, replaces
+ * E2EFromCSR.__call__ (sumpy/e2e.py:164-304) for HelmholtzKernel(3): the
+ * arguments of sumpyhip_m2l_csr plus the real wavenumber k; complex
+ * expansions, interleaved; overwrites the target rows. */
+int sumpyhip_m2l_csr_helmholtz(int order_src, int order_tgt, const int *circ_shape, int nl,
+                               int nm, const int *off_t, const int *off_s,
+                               const int *tgt_degree, int ntgt_boxes, const int *target_boxes,
+                               const int *src_box_starts, const int *src_box_lists,
+                               const double *box_centers, int nboxes, double src_rscale,
+                               double tgt_rscale, double k, const void *src_expansions,
+                               int src_base_ibox, void *tgt_expansions, int tgt_base_ibox,
+                               void *stream);
+
 /* }}} */
 
 /* {{{ M2L through translation classes */

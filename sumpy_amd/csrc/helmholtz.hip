@@ -607,6 +607,89 @@ __global__ __launch_bounds__(256) void p2l_helm3d_moments(
 
 // }}}
 
+// {{{ M2L without translation classes
+
+// One block per target box; per list-2 source box the block writes the
+// scaled derivatives at the pair's vector (as the class-data kernel does for
+// a class) into LDS, then thread t adds sum_s M_s D(t + s) (complex) to its
+// local coefficient; the sums are scaled by (tgt_rscale/src_rscale)^|t| and
+// overwrite the target rows (reference sumpy/e2e.py:164-304).
+__global__ __launch_bounds__(256) void m2l_csr_helm3d(
+    int N, int M0, int M1, int M2, int nl, int nm, const int *off_t, const int *off_s,
+    const int *deg, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const double *centers, int nboxes, double src_rscale, double tgt_rscale, double k,
+    const double2 *src, int sbase, double2 *tgt, int tbase) {
+  extern __shared__ double lds[];
+  const int ncirc = M0*M1*M2, HS = N + 1;
+  double *Dre = lds, *Dim = Dre + ncirc;
+  double2 *Ms = reinterpret_cast<double2 *>(Dim + ncirc);  // 2 ncirc doubles: 16-byte aligned
+  double *ht = reinterpret_cast<double *>(Ms + nm);
+  double *sp = ht + HS*HS, *sg = sp + 3*HS;
+  herm_table(ht, N);
+  const int tid = threadIdx.x;
+  const double rinv = 1.0/src_rscale, ratio = tgt_rscale/src_rscale;
+  for (int ib = blockIdx.x; ib < ntgt; ib += gridDim.x) {
+    const int tb = target_boxes[ib];
+    double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+    for (int is = starts[ib]; is < starts[ib + 1]; ++is) {
+      const int sb = lists[is];
+      __syncthreads();  // the previous pair is done with D, Ms, sp, sg
+      for (int q = tid; q < nm; q += 256) Ms[q] = src[(size_t)(sb - sbase)*nm + q];
+      if (tid < 4) {
+        const double x = (centers[tb] - centers[sb])*rinv,
+                     y = (centers[nboxes + tb] - centers[nboxes + sb])*rinv,
+                     z = (centers[2*(size_t)nboxes + tb] - centers[2*(size_t)nboxes + sb])*rinv;
+        if (tid < 3) {
+          const double xi = tid == 0 ? x : (tid == 1 ? y : z);
+          double pw = 1;
+          for (int e = 0; e <= N; ++e) {
+            sp[tid*HS + e] = pw;
+            pw *= xi;
+          }
+        } else {
+          helm_radial(k*src_rscale, sqrt(x*x + y*y + z*z), N, sg, sg + HS, 1);
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < ncirc; e += 256) {
+        const int mz = e % M2, my = (e/M2) % M1, mx = e/(M2*M1);
+        double re = 0, im = 0;
+        if (mx + my + mz <= N) {
+          const cplx d = helm_deriv(mx, my, mz, sp, sp + HS, sp + 2*HS, 1, sg, sg + HS, 1,
+                                    ht, HS);
+          re = d.re*rinv;
+          im = d.im*rinv;
+        }
+        Dre[e] = re;
+        Dim[e] = im;
+      }
+      __syncthreads();
+      for (int q = 0; q < 4; ++q) {
+        const int row = tid + 256*q;
+        if (row >= nl) break;
+        const int ot = off_t[row];
+        double sr = 0, si = 0;
+        for (int kk = 0; kk < nm; ++kk) {
+          const double2 m = Ms[kk];
+          const double dr = Dre[ot + off_s[kk]], di = Dim[ot + off_s[kk]];
+          sr += m.x*dr - m.y*di;
+          si += m.x*di + m.y*dr;
+        }
+        ar[q] += sr;
+        ai[q] += si;
+      }
+    }
+    for (int q = 0; q < 4; ++q) {
+      const int row = tid + 256*q;
+      if (row >= nl) break;
+      const double f = pow(ratio, (double)deg[row]);
+      tgt[(size_t)(tb - tbase)*nl + row] = make_double2(ar[q]*f, ai[q]*f);
+    }
+  }
+}
+
+// }}}
+
 int check_helm_expn(const sumpyhip_expn *e, const char *what) {
   SUMPY_REQUIRE(e != nullptr && e->stored_ids != nullptr, "%s: null expansion tables", what);
   SUMPY_REQUIRE(e->order >= 0 && e->order + 2 <= HELM_MAXN, "%s: order %d out of range",
@@ -749,4 +832,32 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int n
         box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, c0,
         nullptr, nullptr, nullptr, rscale, (double *)tgt_expansions, tgt_base_ibox);
   return check_launch("p2l_helmholtz");
+}
+
+extern "C" int sumpyhip_m2l_csr_helmholtz(int order_src, int order_tgt, const int *circ_shape,
+                                          int nl, int nm, const int *off_t, const int *off_s,
+                                          const int *tgt_degree, int ntgt_boxes,
+                                          const int *target_boxes, const int *src_box_starts,
+                                          const int *src_box_lists, const double *box_centers,
+                                          int nboxes, double src_rscale, double tgt_rscale,
+                                          double k, const void *src_expansions,
+                                          int src_base_ibox, void *tgt_expansions,
+                                          int tgt_base_ibox, void *stream) {
+  SUMPY_REQUIRE(circ_shape != nullptr, "m2l_csr_helmholtz: null shape");
+  const int N = order_src + order_tgt;
+  SUMPY_REQUIRE(N >= 0 && N <= HELM_MAXN, "m2l_csr_helmholtz: order %d > %d", N, HELM_MAXN);
+  SUMPY_REQUIRE(nl <= 1024, "m2l_csr_helmholtz: too many target coefficients");
+  if (ntgt_boxes == 0) return SUMPYHIP_OK;
+  const size_t ncirc = (size_t)circ_shape[0]*circ_shape[1]*circ_shape[2];
+  const size_t HS = N + 1;
+  const size_t lds = sizeof(double)*(2*ncirc + 2*(size_t)nm + HS*HS + 5*HS);
+  SUMPY_REQUIRE(lds <= 64*1024, "m2l_csr_helmholtz: %zu bytes of LDS needed (order too high)",
+                lds);
+  const int grid = ntgt_boxes < 8192 ? ntgt_boxes : 8192;
+  m2l_csr_helm3d<<<grid, 256, lds, (hipStream_t)stream>>>(
+      N, circ_shape[0], circ_shape[1], circ_shape[2], nl, nm, off_t, off_s, tgt_degree,
+      ntgt_boxes, target_boxes, src_box_starts, src_box_lists, box_centers, nboxes,
+      src_rscale, tgt_rscale, k, static_cast<const double2 *>(src_expansions), src_base_ibox,
+      static_cast<double2 *>(tgt_expansions), tgt_base_ibox);
+  return check_launch("m2l_csr_helmholtz");
 }

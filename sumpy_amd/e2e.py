@@ -419,10 +419,6 @@ class E2EFromCSR(E2EBase):
         if not (isinstance(self.src_expansion, MultipoleExpansionBase)
                 and isinstance(self.tgt_expansion, LocalExpansionBase)):
             raise NotImplementedError("E2EFromCSR implements multipole-to-local only")
-        if self.tgt_expansion.is_complex_valued:
-            raise NotImplementedError(
-                "Helmholtz M2L runs through translation classes "
-                "(M2LUsingTranslationClassesDependentData)")
         m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
@@ -430,6 +426,16 @@ class E2EFromCSR(E2EBase):
         lists = as_i32(kwargs["src_box_lists"])
         src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
         shape = _capi.np_i32(m["shape"])
+        if self.tgt_expansion.is_complex_valued:
+            _capi.call("sumpyhip_m2l_csr_helmholtz", self.src_expansion.order,
+                       self.tgt_expansion.order, shape.ctypes.data, m["nl"], m["nm"],
+                       _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), _capi.ptr(m["tgt_degree"]),
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(centers), int(centers.shape[1]), float(kwargs["src_rscale"]),
+                       float(kwargs["tgt_rscale"]), expansion_k(self.tgt_expansion.kernel, kwargs),
+                       _capi.ptr(src), int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
+                       int(kwargs["tgt_base_ibox"]), actx.stream_handle)
+            return tgt
         _capi.call("sumpyhip_m2l_csr", kernel_id(self.tgt_expansion.kernel),
                    self.src_expansion.order, self.tgt_expansion.order,
                    shape.ctypes.data, m["nl"], m["nm"],

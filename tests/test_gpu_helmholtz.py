@@ -377,3 +377,31 @@ def test_full_taylor_helmholtz_fmm(actx):
     e = {key: rel(v, direct) for key, v in pots.items()}
     assert e["full", 6] < e["full", 3]/10 and e["full", 6] < 1e-3, e
     assert rel(pots["full", 6], pots["compressed", 6]) < 3*e["compressed", 6], e
+
+
+def test_fmm_without_translation_classes(actx):
+    """M2L by E2EFromCSR (per-pair derivatives, reference e2e.py:164-304)
+    equals the translation-class pipeline on the same tree."""
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl, k = HelmholtzKernel(3), 2.5
+    src, q = sphere(2500, 9)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    pots = []
+    for disable in (False, True):
+        ti = SumpyTreeIndependentDataForWrangler(
+            actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+            partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl), [knl],
+            exclude_self=True)
+        wr = SumpyExpansionWrangler(
+            ti, trav, np.complex128, fmm_level_to_order=lambda kn, a, t, lev: 7,
+            kernel_extra_kwargs={"k": k}, _disable_translation_classes=disable,
+            self_extra_kwargs={"target_to_source": actx.from_numpy(
+                np.arange(src.shape[1], dtype=np.int32))})
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        pots.append(actx.to_numpy(pot))
+    assert rel(pots[1], pots[0]) < 1e-11
