@@ -215,6 +215,9 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
   constexpr int DIM = KNL == KLAP3 ? 3 : 2;
   constexpr bool F32 = sizeof(T) == 4;
   constexpr int CH = Chunk<T, DIP>::N;
+  // FP32: chunks hold CH - 128 sources so that the G runs of odd length L
+  // (G L < nn + 2 G <= nn + 128) fit, padded with zero-weight far sources
+  constexpr int CHN = F32 ? CH - 128 : CH;
   constexpr int FOLD = F32 ? 16 : (1 << 20);
   __shared__ v4 s_src[CH];
   __shared__ v4 s_dip[DIP ? CH : 1];
@@ -286,9 +289,21 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
               if ((unsigned)(self - s_bs[k]) < (unsigned)s_bn[k]) pself = s_boff[k] + self - s_bs[k];
           }
         }
-        for (int c0 = 0; c0 < total; c0 += CH) {
-          const int nn = min(CH, total - c0);
+        for (int c0 = 0; c0 < total; c0 += CHN) {
+          const int nn = min(CHN, total - c0);
+          // FP32: lane sub of a target takes the contiguous run [sub L,
+          // sub L + L) of the chunk, L = ceil(nn / G) rounded up to odd (the
+          // G runs of a wave then start on different LDS banks)
+          const int L = ((nn + G - 1)/G) | 1;
           if (c0 > 0) __syncthreads();
+          if constexpr (F32) {
+            // the run tails past nn: sources 1e18 away with zero weights
+            // (every term of the pair underflows to or multiplies zero)
+            for (int p = nn + tid; p < G*L; p += NB) {
+              s_src[p] = v4{T(1e18), T(1e18), T(1e18), T(0)};
+              if constexpr (DIP) s_dip[p] = v4{0, 0, 0, 0};
+            }
+          }
           for (int p = tid; p < nn; p += NB) {
             const int q = c0 + p;
             int lo = 0, hi = nbx;
@@ -305,34 +320,34 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
           __syncthreads();
           const int jself = pself - c0;
           if constexpr (F32) {
-            // FP32: lane sub takes a contiguous run of the chunk's sources,
-            // [sub L, sub L + cnt), L = ceil(nn / G) rounded up to odd (the G
-            // runs of a wave then start on different LDS banks), so the
-            // unrolled loop reads at constant offsets.  The lane's own
-            // (excluded) pair is skipped by splitting the loop there (no
-            // per-pair mask; the FP32 loop is VALU-issue bound).
-            const int L = ((nn + G - 1)/G) | 1;
-            const int cnt = max(0, min(L, nn - sub*L));
-            const int iself = (SELF && jself >= sub*L && jself < sub*L + cnt) ? jself - sub*L : -1;
+            // every lane runs L pairs (wave-uniform loop control), the
+            // folds of 16 unrolled at constant LDS offsets; the excluded
+            // pair is masked with a select only in the folds where some
+            // lane of the wave holds it
+            const int iself = (SELF && jself >= 0 && jself < nn && jself >= sub*L &&
+                               jself < sub*L + L) ? jself - sub*L : -1;
             const v4 *bs = s_src + sub*L;
             const v4 *bd = s_dip + (DIP ? sub*L : 0);
-            for (int i0 = 0; i0 < cnt; i0 += FOLD) {
-              const int i1 = min(cnt, i0 + FOLD);
+            auto pair = [&](int i, bool skip, T &tp, T &t0g, T &t1g, T &t2g) {
+              const v4 sv = bs[i];
+              v4 dv = v4{0, 0, 0, 0};
+              if constexpr (DIP) dv = bd[i];
+              pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y, DIM == 3 ? pz - sv.z : T(0),
+                                             sv.w, dv.x, dv.y, dv.z, skip, tp, t0g, t1g, t2g);
+            };
+            for (int i0 = 0; i0 < L; i0 += FOLD) {
               T tp = 0, t0g = 0, t1g = 0, t2g = 0;
-              const int is = (iself >= i0 && iself < i1) ? iself : i1;
-              auto run = [&](int ib, int ie) {
-                #pragma unroll 4
-                for (int i = ib; i < ie; ++i) {
-                  const v4 sv = bs[i];
-                  v4 dv = v4{0, 0, 0, 0};
-                  if constexpr (DIP) dv = bd[i];
-                  pair_masked<KNL, DIP, GRAD, T>(px - sv.x, py - sv.y,
-                                                 DIM == 3 ? pz - sv.z : T(0), sv.w, dv.x, dv.y,
-                                                 dv.z, false, tp, t0g, t1g, t2g);
+              if (i0 + FOLD <= L) {
+                if (SELF && __any(iself >= i0 && iself < i0 + FOLD)) {
+                  #pragma unroll
+                  for (int k = 0; k < FOLD; ++k) pair(i0 + k, i0 + k == iself, tp, t0g, t1g, t2g);
+                } else {
+                  #pragma unroll
+                  for (int k = 0; k < FOLD; ++k) pair(i0 + k, false, tp, t0g, t1g, t2g);
                 }
-              };
-              run(i0, is);
-              run(is + 1, i1);
+              } else {
+                for (int i = i0; i < L; ++i) pair(i, SELF && i == iself, tp, t0g, t1g, t2g);
+              }
               pot += double(tp);
               if (GRAD) { g0 += double(t0g); g1 += double(t1g); g2 += double(t2g); }
             }
