@@ -151,13 +151,12 @@ constexpr int NB = 256;     // threads per block
 constexpr int MAXB = 256;   // source boxes staged per pass
 
 // Sources per LDS chunk: (x, y, z, w0) [+ (wx, wy, wz, 0)] vectors, <= 48 KB.
-// FP32 charges: 1152 (1024 + the run padding; 18 KB), about the neighbourhood
-// of a 64-particle leaf, so that six blocks of a CU hide each other's list and
-// staging latency.
+// Charges: 1152 in FP32 (1024 + the run padding; 18 KB), 640 in FP64 (20 KB),
+// about the neighbourhood of a 32-64-particle leaf, so that several blocks of
+// a CU hide each other's list and staging latency.
 template <typename T, bool DIP>
 struct Chunk {
-  static constexpr int N = sizeof(T) == 4 && !DIP
-                               ? 1152 : (DIP ? 48 : 32)*1024/(int)(sizeof(T)*4*(DIP ? 2 : 1));
+  static constexpr int N = DIP ? 48*1024/(int)(sizeof(T)*8) : (sizeof(T) == 4 ? 1152 : 640);
 };
 
 // Pair value with the exclude-self pair masked (skip): every term carries a
