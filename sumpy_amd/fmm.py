@@ -308,16 +308,23 @@ class SumpyExpansionWrangler:
         return self._per_level(self.m2l_work_array_level_starts(), clear=True)
 
     def _per_level(self, starts, clear):
-        """Per-level views of one allocation (cleared with one fill)."""
+        """Per-level views of one allocation (cleared with one fill); the
+        allocation and its level starts are kept as ``flat`` and ``starts``."""
         alloc = torch.zeros if clear else torch.empty
         flat = alloc(int(starts[-1]), dtype=torch_dtype(self.preprocessed_mpole_dtype),
                      device=self.tree_indep._setup_actx.device)
-        result = []
+        result = _LevelViews()
         for lev in range(self.tree.nlevels):
             b0, b1 = self.tree_level_start_box_nrs[lev:lev + 2]
             result.append(flat[int(starts[lev]):int(starts[lev + 1])].reshape(
                 int(b1 - b0), -1))
+        result.flat, result.starts = flat, starts
         return result
+
+    def _levels_view(self, flat, starts, l0, l1):
+        """Boxes of levels l0..l1 (of one expansion order) as one 2D view."""
+        b0, b1 = self.tree_level_start_box_nrs[l0], self.tree_level_start_box_nrs[l1 + 1]
+        return flat[int(starts[l0]):int(starts[l1 + 1])].reshape(int(b1 - b0), -1)
 
     def _cached_zeros(self, name, make):
         """A zero result that is never written, kept for the stages that have
@@ -473,11 +480,13 @@ class SumpyExpansionWrangler:
 
     def multipole_to_local(self, actx, level_start_target_box_nrs, target_boxes,
                            src_box_starts, src_box_lists, mpole_exps):
-        """M2L of every level.  With translation classes the levels are
-        preprocessed one by one, contracted in a single MFMA launch
-        (:func:`sumpy_amd.e2e.m2l_apply_levels`, levels of equal order
-        batched) and postprocessed one by one; the result equals the
-        reference's per-level loop (``fmm.py:777-920``)."""
+        """M2L of every level.  With translation classes, each run of
+        consecutive levels of one order is preprocessed in one launch, the
+        levels of an order are contracted in a single MFMA launch
+        (:func:`sumpy_amd.e2e.m2l_apply_levels`) and each run is
+        postprocessed in one launch (source and target of an M2L share the
+        level, so the postprocessing scale ratio is 1 on every level); the
+        result equals the reference's per-level loop (``fmm.py:777-920``)."""
         from sumpy_amd.e2e import m2l_apply_levels
         local_exps = self.local_expansion_zeros(actx)
         starts = _host(level_start_target_box_nrs)
@@ -495,10 +504,6 @@ class SumpyExpansionWrangler:
             if use_pre:
                 if tr_classes[lev] == tr_classes[lev + 1]:
                     continue
-                _, smv = self.multipole_expansions_view(mpole_exps, lev)
-                self.tree_indep.m2l_preprocess_mpole_kernel(order, order)(
-                    actx, src_expansions=smv, preprocessed_src_expansions=pre_mpoles[lev],
-                    src_rscale=self.level_to_rscale(lev), **self.kernel_extra_kwargs)
                 sbase, sview = self.m2l_preproc_mpole_expansions_view(pre_mpoles, lev)
                 tbase, tview = self.m2l_work_array_view(work, lev)
             else:
@@ -521,6 +526,14 @@ class SumpyExpansionWrangler:
             else:
                 m2l(actx, **kwargs)
         for order, levs in batches.items():
+            runs = _consecutive_runs([lev for lev, _, _ in levs])
+            mstarts = self.multipole_expansions_level_starts()
+            for l0, l1 in runs:
+                self.tree_indep.m2l_preprocess_mpole_kernel(order, order)(
+                    actx, src_expansions=self._levels_view(mpole_exps, mstarts, l0, l1),
+                    preprocessed_src_expansions=self._levels_view(
+                        pre_mpoles.flat, pre_mpoles.starts, l0, l1),
+                    src_rscale=self.level_to_rscale(l0), **self.kernel_extra_kwargs)
             ents = [entry for _, entry, _ in levs]
             policy = None
             if self.tree_indep.autotune:
@@ -528,11 +541,13 @@ class SumpyExpansionWrangler:
                     actx, ents, str(self.tree_indep.get_base_kernel()), order,
                     self.tree_indep.m2l_translation.precision)
             m2l_apply_levels(actx, ents, policy=policy)
-            for lev, _, tview in levs:
-                _, lview = self.local_expansions_view(local_exps, lev)
+            lstarts = self.local_expansions_level_starts()
+            for l0, l1 in runs:
                 self.tree_indep.m2l_postprocess_local_kernel(order, order)(
-                    actx, tgt_expansions=lview, tgt_expansions_before_postprocessing=tview,
-                    src_rscale=self.level_to_rscale(lev), tgt_rscale=self.level_to_rscale(lev),
+                    actx, tgt_expansions=self._levels_view(local_exps, lstarts, l0, l1),
+                    tgt_expansions_before_postprocessing=self._levels_view(
+                        work.flat, work.starts, l0, l1),
+                    src_rscale=self.level_to_rscale(l0), tgt_rscale=self.level_to_rscale(l0),
                     **self.kernel_extra_kwargs)
         return local_exps
 
@@ -652,6 +667,22 @@ class SumpyExpansionWrangler:
         return host[name] if same else None
 
     # }}}
+
+
+class _LevelViews(list):
+    """Per-level 2D views of one flat allocation (``flat``, with per-level
+    entry offsets ``starts``)."""
+
+
+def _consecutive_runs(levels):
+    """Sorted levels -> [(first, last)] of runs of consecutive levels."""
+    runs = []
+    for lev in sorted(levels):
+        if runs and lev == runs[-1][1] + 1:
+            runs[-1][1] = lev
+        else:
+            runs.append([lev, lev])
+    return [tuple(r) for r in runs]
 
 
 def _host(a):
