@@ -344,13 +344,22 @@ class SumpyExpansionWrangler:
         return obj_array([self._zeros(self.tree.ntargets, self.dtype)
                           for _ in self.tree_indep.target_kernels])
 
+    # (the permutations as int64 once, for index_select's gather)
+    @cached_property
+    def _user_source_ids_long(self):
+        return self.tree.user_source_ids.long()
+
+    @cached_property
+    def _sorted_target_ids_long(self):
+        return self.tree.sorted_target_ids.long()
+
     def reorder_sources(self, source_array):
-        return source_array[self.tree.user_source_ids.long()]
+        return torch.index_select(source_array, 0, self._user_source_ids_long)
 
     def reorder_potentials(self, potentials):
         assert isinstance(potentials, np.ndarray) and potentials.dtype.char == "O"
-        idx = self.tree.sorted_target_ids.long()
-        return obj_array([p[idx] for p in potentials])
+        idx = self._sorted_target_ids_long
+        return obj_array([torch.index_select(p, 0, idx) for p in potentials])
 
     def distribute_source_weights(self, actx, src_weight_vecs, src_idx_all_ranks):
         return src_weight_vecs
