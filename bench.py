@@ -424,8 +424,8 @@ def helmholtz_batched(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=2**20)
     ap.add_argument("--order", type=int, default=10)
     ap.add_argument("--max-particles", type=int, default=64)

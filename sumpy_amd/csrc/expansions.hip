@@ -427,7 +427,7 @@ __global__ __launch_bounds__(WAVE) void l2l_lap3d(
 
 // {{{ L2P
 
-constexpr int L2P_NW = 4;  // waves per block, each on its own target box
+constexpr int L2P_NWMAX = 8;  // waves per block (8, or 4 where 8 exceed the LDS), each on its own target box
 
 
 // Per block (once): the Horner-ordered factors h[pos] = rho^-|m| / m! of
@@ -465,19 +465,19 @@ __device__ __forceinline__ void horner_t(const double *lh, double bx, double by,
 }
 
 template <bool GRAD, int KMAX, int ORD = -1>
-__global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
+__global__ __launch_bounds__(64*L2P_NWMAX) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
     const double *ty, const double *tz, double rscale, double scale,
     const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
   extern __shared__ double sh[];
   const int nf = e.nfull, nst = e.nstored, nnz = e.proj_nnz;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
   double *hs = sh;                          // [nfull] factors
   double *pv = hs + nf;                     // [nnz] values of P (Horner row order)
   double *lh = pv + nnz + w*(nf + nst);     // per wave: [nfull] Horner coefficients
   double *Ls = lh + nf;                     //           [nstored] L of the box
-  int *rs = reinterpret_cast<int *>(pv + nnz + L2P_NW*(nf + nst));  // [nfull + 1]
+  int *rs = reinterpret_cast<int *>(pv + nnz + nw*(nf + nst));  // [nfull + 1]
   int *pc = rs + nf + 1;                    // [nnz] columns of P
   const int p = e.order;  // p + 1 <= KMAX
   const double rinv = 1.0/rscale;
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
     if (lane == 63) rs[nf] = incl;
   }
   __syncthreads();
-  for (int pos = tid; pos < nf; pos += 64*L2P_NW) {
+  for (int pos = tid; pos < nf; pos += blockDim.x) {
     const int f = e.horner_order[pos];
     const int i = e.full_ids[f], j = e.full_ids[nf + f], k = e.full_ids[2*nf + f];
     double h = inv_fact(i)*inv_fact(j)*inv_fact(k);
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(64*L2P_NW) void l2p_lap3d(
     }
   }
   __syncthreads();
-  for (int ib = blockIdx.x*L2P_NW + w; ib < ntgt; ib += gridDim.x*L2P_NW) {
+  for (int ib = blockIdx.x*nw + w; ib < ntgt; ib += gridDim.x*nw) {
     const int tb = target_boxes[ib];
     const double *L = src + (size_t)(tb - sbase)*nst;
     wave_sync();
@@ -1102,15 +1102,21 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   SUMPY_REQUIRE(pot || gx, "l2p: no output");
   SUMPY_REQUIRE(!gx || (gy && gz), "l2p: incomplete gradient outputs");
   SUMPY_REQUIRE(expn->proj_nnz > 0, "l2p: empty projection");
-  const size_t lds = sizeof(double)*(expn->nfull + expn->proj_nnz +
-                                     L2P_NW*(size_t)(expn->nfull + expn->nstored)) +
-                     sizeof(int)*(expn->nfull + 1 + (size_t)expn->proj_nnz);
+  // P (values, columns, row starts) and the factors are staged once per
+  // block and shared by its waves: 8 waves per block where that fits in 64 KB
+  auto lds_for = [&](int nw) {
+    return sizeof(double)*(expn->nfull + expn->proj_nnz +
+                           nw*(size_t)(expn->nfull + expn->nstored)) +
+           sizeof(int)*(expn->nfull + 1 + (size_t)expn->proj_nnz);
+  };
+  const int nw = lds_for(L2P_NWMAX) <= 64*1024 ? L2P_NWMAX : 4;
+  const size_t lds = lds_for(nw);
   SUMPY_REQUIRE(lds <= 64*1024, "l2p: order too large");
-  const int grid = (ntgt_boxes + L2P_NW - 1)/L2P_NW < 8*num_cus()
-                       ? (ntgt_boxes + L2P_NW - 1)/L2P_NW : 8*num_cus();
+  const int grid = (ntgt_boxes + nw - 1)/nw < 8*num_cus() ? (ntgt_boxes + nw - 1)/nw
+                                                          : 8*num_cus();
   hipStream_t st = (hipStream_t)stream;
 #define SUMPY_L2P_LAUNCH(G, K, ...)                                                        \
-  l2p_lap3d<G, K, ##__VA_ARGS__><<<grid, 64*L2P_NW, lds, st>>>(                            \
+  l2p_lap3d<G, K, ##__VA_ARGS__><<<grid, 64*nw, lds, st>>>(                            \
       *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,      \
       box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,  \
       gx, gy, gz)
