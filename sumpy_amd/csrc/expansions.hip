@@ -434,7 +434,7 @@ constexpr int L2P_NWMAX = 8;  // waves per block (8, or 4 where 8 exceed the LDS
 // L_full = P L (sumpy/expansion/local.py:284-309) and P's rows in Horner
 // order, in LDS.  Each wave then takes its own boxes: L (stored) to LDS,
 // lh[pos] = h[pos] (P L)[m(pos)], and a triple Horner per target (one lane
-// per target) with its gradient.  Four boxes per block are in flight at once.
+// per target) with its gradient.  Every wave of a block has its own box.
 // Triple Horner of the local expansion at (bx, by, bz) over the Horner-ordered
 // coefficients lh (runs of n = p - i - j + 1 coefficients for i = p..0,
 // j = p - i..0), with its gradient; ORD = p at compile time, so every
