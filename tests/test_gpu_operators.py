@@ -161,31 +161,35 @@ def test_p2p_csr(actx, lap3d, small_tree, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("dim", [3, 2])
-@pytest.mark.parametrize("dipole", [False, True])
-def test_p2p_csr_many_chunks(actx, precision, dim, dipole):
+@pytest.mark.parametrize("dipole,separate", [(False, False), (True, False), (True, True)])
+def test_p2p_csr_many_chunks(actx, precision, dim, dipole, separate):
     """Leaves of up to 400 particles: a box's source neighbourhood (up to
-    4455 sources in 3D, 2536 in 2D here) spans several LDS chunks of the CSR
+    4455 sources in 3D, 2536 in 2D with self-interaction) spans several LDS chunks of the CSR
     kernel (640 FP64 / 1024 FP32 sources, the FP32 runs padded past each
     chunk), and boxes of more than 256 targets take several passes of the
     block; charge or charge + dipole sources, potential and gradient, self
-    excluded, against the oracle's CSR loop."""
+    excluded (or separate targets: sources moved by 1e-2, nothing
+    excluded), against the oracle's CSR loop."""
     from sumpy_amd import P2PFromCSR
     from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative, LaplaceKernel
     from sumpy_amd.tree import build_tree_and_traversal
     knl = LaplaceKernel(dim)
     rng = np.random.default_rng(21)
     pts = rng.standard_normal((dim, 12000))*0.3
-    tree, trav = build_tree_and_traversal(actx, pts, None, max_particles_in_box=400)
+    tgts = pts + 1e-2*rng.standard_normal(pts.shape) if separate else None
+    tree, trav = build_tree_and_traversal(actx, pts, tgts, max_particles_in_box=400)
     th, vh = tree.host, trav.host
     ns = th["sources"].shape[1]
     q1, q2 = rng.standard_normal(ns), rng.standard_normal(ns)
     nrm = rng.standard_normal((dim, ns))
     t2s = np.arange(ns, dtype=np.int32)
     skn = [knl, DirectionalSourceDerivative(knl, "dir")] if dipole else [knl]
-    p2p = P2PFromCSR([knl, AxisTargetDerivative(0, knl)], exclude_self=True,
+    p2p = P2PFromCSR([knl, AxisTargetDerivative(0, knl)], exclude_self=not separate,
                      source_kernels=skn, precision=precision)
     extra = {"dir": dev(actx, nrm)} if dipole else {}
-    nmax = int(th["box_source_counts_nonchild"].max())
+    if not separate:
+        extra["target_to_source"] = dev(actx, t2s)
+    nmax = int(max(th["box_source_counts_nonchild"].max(), th["box_target_counts_nonchild"].max()))
     pot, gx = p2p(actx, targets=tree.targets, sources=tree.sources,
                   target_boxes=trav.target_boxes,
                   source_box_starts=trav.neighbor_source_boxes_starts,
@@ -195,26 +199,28 @@ def test_p2p_csr_many_chunks(actx, precision, dim, dipole):
                   box_target_starts=tree.box_target_starts,
                   box_target_counts_nonchild=tree.box_target_counts_nonchild,
                   strength=[dev(actx, q1), dev(actx, q2)][:len(skn)],
-                  max_nsources_in_one_box=nmax, max_ntargets_in_one_box=nmax,
-                  target_to_source=dev(actx, t2s), **extra)
+                  max_nsources_in_one_box=nmax, max_ntargets_in_one_box=nmax, **extra)
     # neighbourhoods reach several chunks
     st = vh["neighbor_source_boxes_starts"]
     cnt = th["box_source_counts_nonchild"]
     hood = max(int(cnt[vh["neighbor_source_boxes_lists"][st[i]:st[i + 1]]].sum())
                for i in range(len(vh["target_boxes"])))
-    assert hood > 2*(640 if precision == "fp64" else 1024)
+    assert hood > (640 if precision == "fp64" else 1024)
     wd = [-q2*nrm[d] for d in range(dim)] if dipole else None
     ref, grad = oracle.p2p_csr(dim, th, vh["target_boxes"], st, vh["neighbor_source_boxes_lists"],
-                               th["targets"], th["sources"], q1, wd=wd, t2s=t2s,
-                               want_grad=True)
+                               th["targets"], th["sources"], q1, wd=wd,
+                               t2s=None if separate else t2s, want_grad=True)
     # 2D FP32: the d / r^2 (gradient, dipole potential) and d d / r^4
     # (dipole gradient) terms of close pairs round at FP32; an FP32 numpy
     # restatement of these sums (box-relative coordinates, FP32 sums of 16
     # folded into FP64) is off the FP64 oracle on this tree by 1.6e-5
     # (gradient), 1.5e-5 (dipole potential) and 5.7e-5 (dipole gradient)
+    # Separate targets 1e-2 from their sources: the restatement of the 3D
+    # dipole potential is 1.46e-5 off (cancellation among the large terms of
+    # the nearest pairs), so those cases get the 2D bounds.
     if precision == "fp64":
         tol_pot = tol_grad = 1e-12
-    elif dim == 3:
+    elif dim == 3 and not separate:
         tol_pot = tol_grad = 1e-5
     else:
         tol_pot, tol_grad = 5e-5, (2e-4 if dipole else 5e-5)
