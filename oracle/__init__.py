@@ -192,9 +192,9 @@ def m2m(src_order, tgt_order, target_boxes, box_child_ids, centers, src_rscale,
     return tgt_expansions
 
 
-def m2l_class_data(order_src, order_tgt, vectors, src_rscale):
+def m2l_class_data(order_src, order_tgt, vectors, src_rscale, compressed=True):
     """Per-class circulant data (``ntranslation_classes x ncirc``)."""
-    T = tables.m2l_tables(order_src, order_tgt)
+    T = tables.m2l_tables(order_src, order_tgt, compressed)
     w2 = T["w2"]
     P2 = f64(w2.projection(src_rscale))
     out = np.zeros((vectors.shape[1], T["ncirc"]))
@@ -213,8 +213,8 @@ def m2l_class_data(order_src, order_tgt, vectors, src_rscale):
 
 def m2l(order_src, order_tgt, target_boxes, src_box_starts, src_box_lists,
         class_lists, class_level_start, class_data, src_rscale, tgt_rscale,
-        src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox):
-    T = tables.m2l_tables(order_src, order_tgt)
+        src_expansions, src_base_ibox, tgt_expansions, tgt_base_ibox, compressed=True):
+    T = tables.m2l_tables(order_src, order_tgt, compressed)
     tb = i32(target_boxes)
     lib().orc_m2l(ctypes.c_int(T["ncirc"]), ctypes.c_int(len(T["src_circ_idx"])),
                   _i(T["src_circ_idx"]), ctypes.c_int(len(T["tgt_circ_idx"])),

@@ -10,7 +10,8 @@ items per CU, at least 12 slots per split) suits dense trees; sparse trees and
 other orders may prefer others.  :class:`M2LSplitTuner` times the contraction
 of a pass under a few policies and keeps the fastest, per
 
-    (kernel, expansion order, precision, GPU architecture, schedule digest)
+    (library digest, kernel, expansion order, precision, GPU architecture,
+     schedule digest)
 
 so that a second run on the same kind of tree reads the choice from disk.
 The cache lives in ``$SUMPY_AMD_CACHE_DIR`` (default ``~/.cache/sumpy_amd``)
@@ -66,8 +67,28 @@ class TuneCache:
             pass  # read-only home: the choice is kept for this process only
 
 
-def make_key(kernel, order, precision, arch, digest):
-    return f"m2l_apply|{kernel}|{order}|{precision}|{arch}|{digest}"
+_CODE_VERSION = None
+
+
+def code_version() -> str:
+    """Digest of the built HIP library (``libsumpyhip.so``): a rebuilt or
+    changed contraction kernel gets new keys, so choices timed on old code are
+    never reused (the role of ``KERNEL_VERSION`` in the reference's cache
+    keys, ``sumpy/tools.py:483-488``)."""
+    global _CODE_VERSION
+    if _CODE_VERSION is None:
+        from sumpy_amd import _capi
+        h = hashlib.sha1()
+        with open(_capi.LIB_PATH, "rb") as f:
+            for chunk in iter(lambda: f.read(1 << 20), b""):
+                h.update(chunk)
+        _CODE_VERSION = h.hexdigest()[:12]
+    return _CODE_VERSION
+
+
+def make_key(kernel, order, precision, arch, digest, version=None):
+    version = code_version() if version is None else version
+    return f"m2l_apply|{version}|{kernel}|{order}|{precision}|{arch}|{digest}"
 
 
 def schedule_digest(entries):
@@ -142,4 +163,4 @@ class M2LSplitTuner:
 
 
 __all__ = ["DEFAULT_POLICY", "M2LSplitTuner", "SPLIT_POLICIES", "TuneCache", "cache_dir",
-           "schedule_digest"]
+           "code_version", "make_key", "schedule_digest"]

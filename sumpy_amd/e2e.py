@@ -68,17 +68,30 @@ class _OctantTranslation(E2EBase):
                  use_octant_matrices=True):
         super().__init__(src_expansion, tgt_expansion, name)
         self.use_octant_matrices = use_octant_matrices
-        self._host = {}
-        self._plans = {}
         self._mats = {}
+        #: whether the last call took the octant-matrix (MFMA) path
+        self.last_call_used_octant_matrices = False
 
-    def _host_copy(self, t):
-        key = (t.data_ptr(), t.numel(), t._version, tuple(t.shape), str(t.dtype))
-        v = self._host.get(key)
+    @staticmethod
+    def _host_copy(t):
+        """Host copy of the tree/traversal tensor *t*, held while the tensor
+        (or the tensor it is a view of) lives."""
+        per = _tensor_cache(_owner(t))
+        key = ("host", t.data_ptr(), t.numel(), t._version, tuple(t.shape), str(t.dtype))
+        v = per.get(key)
         if v is None:
             v = t.cpu().numpy()
-            self._host[key] = v
+            per[key] = v
         return v
+
+    def _plan_cache(self, tb_t, others):
+        """Plans of a level, held with the target-box tensor's owner (a
+        traversal array; the tree arrays in *others* live as long as the
+        traversal that refers to the tree)."""
+        per = _tensor_cache(_owner(tb_t))
+        key = (self.span_name,) + tuple(
+            (t.data_ptr(), t.numel(), t._version) for t in (tb_t, *others))
+        return per, key
 
     def _matrices(self, actx, offsets, present, src_rscale, tgt_rscale, k=0.0):
         key = (offsets.tobytes(), present.tobytes(), float(src_rscale), float(tgt_rscale),
@@ -208,9 +221,9 @@ class E2EFromChildren(_OctantTranslation):
                    _capi.ptr(out), int(par[0]), actx.stream_handle)
 
     def _plan(self, tb_t, child_t, centers_t):
-        key = tuple((t.data_ptr(), t.numel(), t._version) for t in (tb_t, child_t, centers_t))
-        if key in self._plans:
-            return self._plans[key]
+        plans, key = self._plan_cache(tb_t, (child_t, centers_t))
+        if key in plans:
+            return plans[key]
         tb = self._host_copy(tb_t).astype(np.int64)
         child = self._host_copy(child_t)
         centers = self._host_copy(centers_t)
@@ -232,7 +245,7 @@ class E2EFromChildren(_OctantTranslation):
             mask = ((inputs >= 0).any(axis=2)*(1 << np.arange(8))[None, :]).sum(axis=1)
             plan = dict(ngroups=ng, targets=targets, inputs=inputs, mask=mask,
                         offsets=common[0], present=common[1], npairs=int(has.sum()))
-        self._plans[key] = plan
+        plans[key] = plan
         return plan
 
     def _apply(self, actx, kwargs, src, tgt, k):
@@ -240,9 +253,11 @@ class E2EFromChildren(_OctantTranslation):
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
         child = as_i32(kwargs["box_child_ids"])
+        self.last_call_used_octant_matrices = False
         if self.use_octant_matrices and tb.numel():
             plan = self._plan(tb, child, centers)
             if plan is not None:
+                self.last_call_used_octant_matrices = True
                 if "dev" not in plan:
                     plan["dev"] = dict(zip(("targets", "inputs"), _groups(
                         actx, plan["targets"], plan["inputs"], None)),
@@ -287,9 +302,9 @@ class E2EFromParent(_OctantTranslation):
                    _capi.ptr(out), int(chd[0]), actx.stream_handle)
 
     def _plan(self, tb_t, parent_t, centers_t):
-        key = tuple((t.data_ptr(), t.numel(), t._version) for t in (tb_t, parent_t, centers_t))
-        if key in self._plans:
-            return self._plans[key]
+        plans, key = self._plan_cache(tb_t, (parent_t, centers_t))
+        if key in plans:
+            return plans[key]
         tb = self._host_copy(tb_t).astype(np.int64)
         parent = self._host_copy(parent_t).astype(np.int64)
         centers = self._host_copy(centers_t)
@@ -317,7 +332,7 @@ class E2EFromParent(_OctantTranslation):
             plan = dict(ngroups=len(tg), targets=np.concatenate(tg),
                         inputs=np.stack(ip), mask=np.array(mk), offsets=common[0],
                         present=common[1], npairs=len(tb))
-        self._plans[key] = plan
+        plans[key] = plan
         return plan
 
     def _apply(self, actx, kwargs, src, tgt, k):
@@ -325,9 +340,11 @@ class E2EFromParent(_OctantTranslation):
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
         parents = as_i32(kwargs["box_parent_ids"])
+        self.last_call_used_octant_matrices = False
         if self.use_octant_matrices and tb.numel():
             plan = self._plan(tb, parents, centers)
             if plan is not None:
+                self.last_call_used_octant_matrices = True
                 if "dev" not in plan:
                     plan["dev"] = dict(zip(("targets", "inputs"), _groups(
                         actx, plan["targets"], plan["inputs"], None)),
@@ -355,17 +372,24 @@ def _m2l_tabs(src_expansion, tgt_expansion, device):
 
 
 _BF16_TABS = {}
-_SCHEDULES = {}  # id(list tensor) -> (weak ref, {key: M2L tile schedule})
+_PER_TENSOR = {}  # id(tensor) -> (weak ref, {key: data derived from the tensor})
 
 
-def _schedule_cache(lists):
-    """The schedules made for the list tensor *lists*, dropped with it."""
-    k = id(lists)
-    ent = _SCHEDULES.get(k)
-    if ent is None or ent[0]() is not lists:
-        ent = (weakref.ref(lists), {})
-        _SCHEDULES[k] = ent
-        weakref.finalize(lists, _SCHEDULES.pop, k, None)
+def _owner(t):
+    """The tensor whose storage *t* views (``t`` itself if it is no view)."""
+    return t._base if t._base is not None else t
+
+
+def _tensor_cache(t):
+    """A dict of data derived from the tensor object *t* (M2L tile schedules
+    of a list tensor, packed class data of a class-data tensor), dropped when
+    *t* is freed, so a later tensor at the same address or id never sees it."""
+    k = id(t)
+    ent = _PER_TENSOR.get(k)
+    if ent is None or ent[0]() is not t:
+        ent = (weakref.ref(t), {})
+        _PER_TENSOR[k] = ent
+        weakref.finalize(t, _PER_TENSOR.pop, k, None)
     return ent[1]
 
 
@@ -518,17 +542,15 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
 
     default_name = "m2l_using_translation_classes_dependent_data"
 
-    def __init__(self, src_expansion, tgt_expansion, name=None):
-        super().__init__(src_expansion, tgt_expansion, name)
-        self._schedules = {}
-        self._packed_data = {}
-
     def packed_class_data(self, actx, data):
         """The split-BF16 form's class data: one 32-bit word bf16(x) |
-        bf16(x - bf16(x)) << 16 per value, at the strides of the LDS layout,
-        made once per class-data tensor (and version) and cached."""
-        key = (data.data_ptr(), data.numel(), data._version)
-        out = self._packed_data.get(key)
+        bf16(x - bf16(x)) << 16 per value, at the strides of the LDS layout.
+        Made once per class-data tensor object (the wrangler's memoized class
+        data, written once) and held while that tensor lives."""
+        key = ("bf16x3_packed", self.src_expansion.order, self.tgt_expansion.order,
+               self.src_expansion.compressed, data.data_ptr(), tuple(data.shape))
+        per = _tensor_cache(_owner(data))
+        out = per.get(key)
         if out is None:
             tabs = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)
             ncls = int(data.shape[0])
@@ -536,9 +558,7 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             _capi.call("sumpyhip_m2l_pack_bf16_class_data", ncls, tabs["shape"].ctypes.data,
                        *tabs["strides"], tabs["ncirc"], _capi.ptr(data), _capi.ptr(out),
                        actx.stream_handle)
-            if len(self._packed_data) >= 64:
-                self._packed_data.clear()
-            self._packed_data[key] = out
+            per[key] = out
         return out
 
     def schedule(self, actx, target_boxes, src_box_starts, src_box_lists, class_lists,
@@ -550,7 +570,7 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             + (int(class_level_start), int(nclasses), tile, str(actx.device))
         # shared by every operator instance (the schedule depends on the lists
         # only), held while the traversal's list tensor lives
-        per_lists = _schedule_cache(src_box_lists)
+        per_lists = _tensor_cache(_owner(src_box_lists))
         sched = per_lists.get(key)
         if sched is None:
             sched = build_m2l_schedule(
@@ -588,8 +608,9 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         slots of Ar (class 2c) and Ai (2c + 1), fed by the source rows
         3 (s - sbase) + (0: Mr, 1: Mi, 2: -Mi):
         Lr = Ar Mr + Ai (-Mi), Li = Ar Mi + Ai Mr."""
-        key = ("complex", id(sched), sbase, tbase)
-        out = self._schedules.get(key)
+        # held in the real schedule itself, so it lives and dies with it
+        key = ("complex", sbase, tbase)
+        out = sched.get(key)
         if out is not None:
             return out
         h = sched["host"]
@@ -617,7 +638,7 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
                    slot_class=dev(vc.reshape(-1)), slot_sources=dev(vs.reshape(-1)),
                    nslots=nslots, npairs=4*sched["npairs"],
                    nsplit=choose_nsplit(ntiles, nslots, actx.device))
-        self._schedules[key] = out
+        sched[key] = out
         return out
 
     def __call__(self, actx, **kwargs):

@@ -33,15 +33,19 @@ def helm_m2l(order, target_boxes, starts, lists, class_lists, c0, data, mp, mbas
         loc[tb - lbase] = acc
 
 
-def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
+def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0, compressed=True):
     """*k* != 0: Helmholtz-3D with complex weights (complex expansions; the
-    table-driven stages run on the real and imaginary planes)."""
+    table-driven stages run on the real and imaginary planes).
+    *compressed* = False: full Taylor expansions (Laplace), every multi-index
+    of degree <= order stored."""
     if k:
+        assert compressed
         return _oracle_fmm_helm(tree, trav, weights, order, exclude_self, k)
     th, vh = tree.host, trav.host
     lbs = th["level_start_box_nrs"]
     nlev = tree.nlevels
-    ncoef = (order + 1)**2
+    cmp = {"compressed": compressed}
+    ncoef = len(tables.wrangler(order, 3, compressed).stored)
     q = np.asarray(weights)[th["user_source_ids"]]
     centers = th["box_centers"]
     nboxes = len(th["box_levels"])
@@ -56,7 +60,7 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         sb = vh["source_boxes"][lsb[lev]:lsb[lev + 1]]
         if len(sb):
             oracle.p2m(order, sb, th, centers, th["sources"], q, None,
-                       rscale(tree, lev, order), view(mp, lev), int(lbs[lev]))
+                       rscale(tree, lev, order), view(mp, lev), int(lbs[lev]), **cmp)
     # M2M
     lsp = vh["level_start_source_parent_box_nrs"]
     for sl in range(nlev - 1, 2, -1):
@@ -65,7 +69,7 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         if len(pb):
             oracle.m2m(order, order, pb, th["box_child_ids"], centers,
                        rscale(tree, sl, order), rscale(tree, tl, order), view(mp, sl),
-                       int(lbs[sl]), view(mp, tl), int(lbs[tl]))
+                       int(lbs[sl]), view(mp, tl), int(lbs[tl]), **cmp)
     # P2P (list 1)
     t2s = np.argsort(th["user_source_ids"]).astype(np.int32)  # unused unless exclude_self
     nt = th["targets"].shape[1]
@@ -85,16 +89,16 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         if c0 == c1 or s0 == s1:
             continue
         rs = rscale(tree, lev, order)
-        data = oracle.m2l_class_data(order, order, vecs[:, c0:c1], rs)
+        data = oracle.m2l_class_data(order, order, vecs[:, c0:c1], rs, **cmp)
         oracle.m2l(order, order, totp[s0:s1], vh["from_sep_siblings_starts"][s0:s1 + 1],
                    vh["from_sep_siblings_lists"], vh["from_sep_siblings_translation_classes"],
                    c0, data, rs, rs, view(mp, lev), int(lbs[lev]), view(loc, lev),
-                   int(lbs[lev]))
+                   int(lbs[lev]), **cmp)
     # M2P (list 3)
     for slev, (tbs, st, li) in enumerate(vh["sep_smaller"]):
         if len(tbs):
             oracle.m2p(order, tbs, st, li, th, centers, th["targets"],
-                       rscale(tree, slev, order), view(mp, slev), int(lbs[slev]), pot)
+                       rscale(tree, slev, order), view(mp, slev), int(lbs[slev]), pot, **cmp)
     # P2L (list 4)
     loc4 = np.zeros(nboxes*ncoef)
     st4, li4 = vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"]
@@ -103,7 +107,7 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         if s0 == s1 or st4[s1] == st4[s0]:
             continue
         oracle.p2l(order, totp[s0:s1], st4[s0:s1 + 1], li4, th, centers, th["sources"], q,
-                   None, rscale(tree, lev, order), view(loc4, lev), int(lbs[lev]))
+                   None, rscale(tree, lev, order), view(loc4, lev), int(lbs[lev]), **cmp)
     loc += loc4
     # L2L
     for tl in range(1, nlev):
@@ -113,7 +117,7 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         sl = tl - 1
         oracle.l2l(order, order, totp[s0:s1], th["box_parent_ids"], centers,
                    rscale(tree, sl, order), rscale(tree, tl, order), view(loc, sl),
-                   int(lbs[sl]), view(loc, tl), int(lbs[tl]))
+                   int(lbs[sl]), view(loc, tl), int(lbs[tl]), **cmp)
     # L2P
     ltb = vh["level_start_target_box_nrs"]
     lpot = np.zeros(nt)
@@ -121,7 +125,7 @@ def oracle_fmm(tree, trav, weights, order, exclude_self=False, k=0.0):
         tb = vh["target_boxes"][ltb[lev]:ltb[lev + 1]]
         if len(tb):
             oracle.l2p(order, tb, th, centers, th["targets"], rscale(tree, lev, order),
-                       view(loc, lev), int(lbs[lev]), lpot)
+                       view(loc, lev), int(lbs[lev]), lpot, **cmp)
     pot = pot + lpot
     return pot[th["sorted_target_ids"]]
 

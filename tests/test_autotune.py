@@ -40,6 +40,23 @@ def test_policy_read_from_disk_without_timing(tmp_path):
     assert tuner.policy(Actx(), ents, "LapKnl3D", 10, "fp32") == (8, 24)
 
 
+def test_key_carries_library_digest(tmp_path):
+    """A choice stored under another build of the library is not read back."""
+    class Actx:
+        device = "cpu"
+    ents = _entries((10, 100, 1000))
+    dig = autotune.schedule_digest(ents)
+    assert autotune.make_key("K", 10, "fp32", "cpu", dig) == \
+        autotune.make_key("K", 10, "fp32", "cpu", dig, autotune.code_version())
+    cache = autotune.TuneCache(str(tmp_path))
+    cache.put(autotune.make_key("K", 10, "fp32", "cpu", dig, "0123456789ab"),
+              {"policy": [8, 24]})
+    tuner = autotune.M2LSplitTuner(cache=autotune.TuneCache(str(tmp_path)))
+    tuner._memo = {}
+    key = autotune.make_key("K", 10, "fp32", "cpu", dig)
+    assert tuner.cache.get(key) is None
+
+
 @pytest.mark.gpu
 def test_autotuned_pass_matches_default(actx, tmp_path, monkeypatch):
     from functools import partial
@@ -73,6 +90,6 @@ def test_autotuned_pass_matches_default(actx, tmp_path, monkeypatch):
     assert np.linalg.norm(pots[1] - pots[0])/np.linalg.norm(pots[0]) < 1e-6
     data = json.load(open(tmp_path/"autotune.json"))
     (key, val), = data.items()
-    assert key.startswith("m2l_apply|LapKnl3D|8|fp32|")
+    assert key.startswith(f"m2l_apply|{autotune.code_version()}|LapKnl3D|8|fp32|")
     assert tuple(val["policy"]) in autotune.SPLIT_POLICIES
     assert val["ms"] <= val["default_ms"]

@@ -207,3 +207,83 @@ def test_laplace3d_fmm_2e14_order6_vs_direct(actx):
     err = rel(pot, direct)
     print(f"rel. L2 error, order 6: {err:.3e}")
     assert err < 3e-4
+
+
+def test_full_taylor_laplace_fmm_matches_oracle_fmm(actx):
+    """Full Taylor expansions (``VolumeTaylorMultipoleExpansion`` /
+    ``VolumeTaylorLocalExpansion``, reference ``multipole.py:426-567``,
+    ``local.py:522-704``) for the Laplace kernel: every multi-index of degree
+    <= p stored; the GPU FMM equals the oracle FMM run with full-Taylor tables
+    (``compressed=False``) on the same tree, with and without translation
+    classes."""
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import VolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import VolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = LaplaceKernel(3)
+    rng = np.random.default_rng(3)
+    n, order = 3000, 4
+    src, q = rng.random((3, n)), rng.standard_normal(n)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    ref = oracle_fmm(tree, trav, q, order, exclude_self=True, compressed=False)
+    for disable in (False, True):
+        tr = VolumeTaylorM2LWithDenseClassMatrices("fp64")
+        ti = SumpyTreeIndependentDataForWrangler(
+            actx, partial(VolumeTaylorMultipoleExpansion, knl),
+            partial(VolumeTaylorLocalExpansion, knl, m2l_translation_override=tr),
+            [knl], exclude_self=True)
+        wr = SumpyExpansionWrangler(
+            ti, trav, np.float64, fmm_level_to_order=lambda kn, a, t, lev: order,
+            _disable_translation_classes=disable,
+            self_extra_kwargs={"target_to_source": actx.from_numpy(
+                np.arange(n, dtype=np.int32))})
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        assert rel(actx.to_numpy(pot), ref) < 1e-12, disable
+
+
+@pytest.mark.parametrize("m2l_precision", ["fp64", "bf16x3"])
+def test_tree_indep_reused_across_trees(actx, m2l_precision):
+    """One SumpyTreeIndependentDataForWrangler serves wranglers of different
+    trees one after the other (each freed before the next is built, so device
+    buffers and object ids get reused); every result equals a fresh
+    tree_indep's on the same tree.  Guards the per-tensor caches of tile
+    schedules and packed class data."""
+    import gc
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = LaplaceKernel(3)
+
+    def tree_indep():
+        tr = VolumeTaylorM2LWithDenseClassMatrices(m2l_precision)
+        return SumpyTreeIndependentDataForWrangler(
+            actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+            partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                    m2l_translation_override=tr), [knl], exclude_self=True)
+
+    def run(ti, src, q):
+        tree, trav = build_tree_and_traversal(actx, src, None, 30)
+        wr = SumpyExpansionWrangler(
+            ti, trav, np.float64, fmm_level_to_order=lambda k, a, t, lev: 6,
+            self_extra_kwargs={"target_to_source": actx.from_numpy(
+                np.arange(src.shape[1], dtype=np.int32))})
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        return actx.to_numpy(pot).copy()
+
+    shared = tree_indep()
+    for seed, scale in ((0, 1.0), (1, 3.0), (2, 0.5)):
+        rng = np.random.default_rng(seed)
+        src, q = rng.random((3, 4000))*scale, rng.standard_normal(4000)
+        got = run(shared, src, q)
+        gc.collect()
+        ref = run(tree_indep(), src, q)
+        gc.collect()
+        assert rel(got, ref) < 1e-13, (seed, rel(got, ref))

@@ -405,3 +405,45 @@ def test_fmm_without_translation_classes(actx):
         pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
         pots.append(actx.to_numpy(pot))
     assert rel(pots[1], pots[0]) < 1e-11
+
+
+def test_helmholtz_tree_indep_reused_across_trees(actx):
+    """One tree_indep serves Helmholtz wranglers of different trees one after
+    the other (the complex M2L schedule is derived per tree); each result
+    equals a fresh tree_indep's on the same tree."""
+    import gc
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = HelmholtzKernel(3)
+
+    def tree_indep():
+        tr = VolumeTaylorM2LWithDenseClassMatrices("fp64")
+        return SumpyTreeIndependentDataForWrangler(
+            actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+            partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
+                    m2l_translation_override=tr), [knl], exclude_self=True)
+
+    def run(ti, src, q):
+        tree, trav = build_tree_and_traversal(actx, src, None, 30)
+        wr = SumpyExpansionWrangler(
+            ti, trav, np.complex128, fmm_level_to_order=lambda kn, a, t, lev: 6,
+            kernel_extra_kwargs={"k": 3.0},
+            self_extra_kwargs={"target_to_source": actx.from_numpy(
+                np.arange(src.shape[1], dtype=np.int32))})
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        return actx.to_numpy(pot).copy()
+
+    shared = tree_indep()
+    for seed, n in ((0, 3000), (1, 4500), (2, 2000)):
+        src, q = sphere(n, seed)
+        got = run(shared, src, q)
+        gc.collect()
+        ref = run(tree_indep(), src, q)
+        gc.collect()
+        assert rel(got, ref) < 1e-13, (seed, rel(got, ref))

@@ -1,7 +1,7 @@
 """Matrix generators (reference ``sumpy/p2p.py:292-477``, ``sumpy/qbx.py:
 342-539``): the full matrix times a strength vector equals the direct sum of
-the applier (P2P / LayerPotential, themselves pinned to the oracle); single
-columns equal the oracle's direct sum for a unit source; the subset generator
+the applier (P2P / LayerPotential); columns equal the oracle's direct sum
+(line-Taylor sum for QBX) of a unit source, for every target kernel; the subset generator
 returns the full matrix's entries at the requested pairs; excluded self pairs
 are zero.  Cases follow the reference's tests (``test_matrixgen.py``):
 Laplace 2D/3D and Helmholtz 3D, source and target derivatives, a coordinate
@@ -41,6 +41,31 @@ def _kernels(name):
         {"k": 5.0 + 0.5j}
 
 
+def _oracle_columns(name, tgt, src, j, exclude_self):
+    """Column j of each target kernel's matrix of :func:`_kernels`'s case, by
+    the oracle's direct sum with unit strength at source j for every source
+    kernel (dipole weight -1 along the derivative axis: d/ds_a G(t - s) =
+    -d/dt_a G)."""
+    n = src.shape[1]
+    e = np.zeros(n)
+    e[j] = 1
+    t2s = np.arange(n, dtype=np.int32) if exclude_self else None
+    if name == "lap3d":
+        pot, grad = oracle.p2p_dense(3, tgt, src, e, t2s=t2s, want_grad=True)
+        return [pot, grad[0], tgt[2]*pot]
+    if name == "lap3d_dipole":
+        pot, grad = oracle.p2p_dense(3, tgt, src, np.zeros(n), wd=[0*e, -e, 0*e], t2s=t2s,
+                                     want_grad=True)
+        return [pot, grad[2]]
+    if name == "lap2d":
+        pot, grad = oracle.p2p_dense(2, tgt, src, e, t2s=t2s, want_grad=True)
+        return [pot, grad[1]]
+    ec = e.astype(np.complex128)
+    pot, grad = oracle.p2p_dense(13, tgt, src, ec, wd=[-ec, 0*ec, 0*ec], t2s=t2s,
+                                 k=5.0 + 0.5j, want_grad=True)
+    return [pot, grad[1]]
+
+
 @pytest.mark.parametrize("name", ["lap3d", "lap3d_dipole", "lap2d", "helm3d"])
 @pytest.mark.parametrize("exclude_self", [False, True])
 def test_p2p_matrix(actx, name, exclude_self):
@@ -69,15 +94,12 @@ def test_p2p_matrix(actx, name, exclude_self):
     if exclude_self:
         for m in mats:
             assert np.all(np.diag(actx.to_numpy(m)) == 0)
-    # one column against the oracle (plain potential, charge sources)
-    if name in ("lap3d", "lap2d"):
-        j = 17
-        w = np.zeros(n)
-        w[j] = 1
-        knl = 3 if dim == 3 else 2
-        col, _ = oracle.p2p_dense(knl, tgt, src, w,
-                                  t2s=np.arange(n) if exclude_self else None)
-        assert rel(actx.to_numpy(mats[0])[:, j], col) < 1e-13
+    # columns against the oracle's direct sum of a unit source, for every
+    # target kernel of the case
+    for j in (0, 17, n - 1):
+        cols = _oracle_columns(name, tgt, src, j, exclude_self)
+        for m, col in zip(mats, cols):
+            assert rel(actx.to_numpy(m)[:, j], col) < 1e-13, (name, j)
     # subset entries
     ti = rng.integers(0, nt, 500).astype(np.int32)
     si = rng.integers(0, n, 500).astype(np.int32)
@@ -117,6 +139,14 @@ def test_layer_potential_matrix(actx, kind):
         *args, [actx.from_numpy(dens)], actx.from_numpy(radii), **kw)
     for m, a in zip(mats, applied):
         assert rel(actx.to_numpy(m) @ dens, actx.to_numpy(a)) < 1e-12
+    # columns against the oracle's line-Taylor sum of a unit density
+    srcmode = 0 if kind == "slp" else 1
+    for j in (0, 123, 399):
+        e = np.zeros(400)
+        e[j] = 1
+        for m, axis in zip(mats, (-1, 0)):
+            col = oracle.qbx2d(6, srcmode, axis, nodes, centers, radii, nodes, normals, e)
+            assert rel(actx.to_numpy(m)[:, j], col) < 1e-12, (kind, axis, j)
     rng = np.random.default_rng(2)
     ti = rng.integers(0, 400, 300).astype(np.int32)
     si = rng.integers(0, 400, 300).astype(np.int32)

@@ -293,7 +293,7 @@ def test_m2m_l2l(actx, lap3d, small_tree, order, octant):
         tgt_expansions=out, tgt_base_ibox=int(lbs[tl]),
         target_boxes=dev(actx, pb), box_child_ids=tree.box_child_ids,
         centers=tree.box_centers, src_rscale=rs_s, tgt_rscale=rs_t)
-    assert bool(m2m._plans and next(iter(m2m._plans.values())) is not None) == octant
+    assert m2m.last_call_used_octant_matrices == octant
     ref = tgt0.copy()
     oracle.m2m(order, order, pb, th["box_child_ids"], th["box_centers"], rs_s, rs_t, src,
                int(lbs[sl]), ref, int(lbs[tl]))
@@ -311,7 +311,7 @@ def test_m2m_l2l(actx, lap3d, small_tree, order, octant):
         tgt_expansions=out, tgt_base_ibox=int(lbs[sl]),
         target_boxes=dev(actx, cb), box_parent_ids=tree.box_parent_ids,
         centers=tree.box_centers, src_rscale=rs_t, tgt_rscale=rs_s)
-    assert bool(l2l._plans and next(iter(l2l._plans.values())) is not None) == octant
+    assert l2l.last_call_used_octant_matrices == octant
     ref = ltgt0.copy()
     oracle.l2l(order, order, cb, th["box_parent_ids"], th["box_centers"], rs_t, rs_s, lsrc,
                int(lbs[tl]), ref, int(lbs[sl]))

@@ -315,6 +315,24 @@ def test_oracle_fmm_converges_to_direct(order):
     direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(3000, dtype=np.int32))
     assert rel(pot, direct) < {4: 2e-3, 8: 5e-5}[order]
 
+
+def test_oracle_fmm_full_taylor_equals_compressed():
+    """Full Taylor expansions (every multi-index of degree <= p stored, the
+    ``VolumeTaylor*`` path) and the Laplace-compressed ones represent the same
+    harmonic polynomials, so the two oracle FMMs agree to rounding; a
+    full-Taylor chain with a dropped or mis-indexed coefficient would not."""
+    import sys
+    sys.path.insert(0, HERE)
+    from oracle_fmm import oracle_fmm
+    tree, trav, rng = _tree(n=3000, seed=0)
+    src = tree.host["sources"][:, tree.host["sorted_target_ids"]]
+    q = rng.standard_normal(3000)
+    full = oracle_fmm(tree, trav, q, 4, exclude_self=True, compressed=False)
+    comp = oracle_fmm(tree, trav, q, 4, exclude_self=True)
+    assert rel(full, comp) < 1e-12
+    direct, _ = oracle.p2p_dense(3, src, src, q, t2s=np.arange(3000, dtype=np.int32))
+    assert rel(full, direct) < 2e-3
+
 # }}}
 
 
