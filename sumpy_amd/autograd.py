@@ -15,13 +15,12 @@ additions on top of its operators, built from the same HIP kernels:
   ``sumpyhip_qbx2d_adjoint`` (density, sources, normals, targets, centres).
 
 * :func:`fmm_potential` -- a whole FMM pass (``drive_fmm`` over a
-  :class:`~sumpy_amd.fmm.SumpyExpansionWrangler`) for sources = targets with
-  self-interaction excluded.  The interaction matrix is then symmetric
-  (G(t - s) = G(s - t)), so the adjoint for the strengths is one more FMM
-  pass with the conjugate kernel (the same wrangler for Laplace; k -> -k for
-  a real Helmholtz wavenumber, conj G_k = G_{-k}); every expansion is
-  recomputed by that pass, nothing is stored.  It agrees with the exact
-  transpose of the discrete FMM operator to within the FMM's accuracy.
+  :class:`~sumpy_amd.fmm.SumpyExpansionWrangler`).  The strength adjoint is
+  the FMM over the same boxes with sources and targets exchanged and the
+  conjugate kernel, which is the exact transpose of the discrete forward pass
+  (each stage transposes into its exchanged counterpart); source- and
+  target-position gradients are the gradients of the adjoint and forward
+  fields.  Every expansion is recomputed by the backward passes.
 
 Complex outputs follow PyTorch's convention: for a real loss L the incoming
 gradient is dL/dRe y + i dL/dIm y, and the gradient of a real input x is
@@ -212,66 +211,218 @@ def qbx_layer_potential(targets, sources, centers, density, order, normals=None)
     return _QBXPotential.apply(density, sources, targets, centers, normals, int(order), scale)
 
 
-__all__ = ["p2p_potential", "qbx_layer_potential"]
+__all__ = ["adjoint_wrangler", "fmm_potential", "p2p_potential", "qbx_layer_potential"]
 
 
-# {{{ FMM (strength gradient by the adjoint pass)
+# {{{ FMM (adjoints by the FMM with sources and targets exchanged)
+
+def _check_adjoint_config(wrangler):
+    """The adjoint pass computes A^T for the plain single-layer operator:
+    one target kernel equal to the base kernel, charge sources with their own
+    strength, and self-exclusion only for sources = targets with the identity
+    map.  Anything else (derivative kernels, several strengths, another
+    target-to-source map) has a different transpose: refuse it."""
+    ti = wrangler.tree_indep
+    base = ti.get_base_kernel()
+    if len(ti.target_kernels) != 1 or ti.target_kernels[0] != base:
+        raise NotImplementedError("fmm_potential: one target kernel, the base kernel itself "
+                                  f"(got {[str(k) for k in ti.target_kernels]})")
+    if ti.source_kernels is not None and list(ti.source_kernels) != [base]:
+        raise NotImplementedError("fmm_potential: charge sources only (source_kernels None "
+                                  "or [base kernel])")
+    if ti.strength_usage is not None and list(ti.strength_usage) != [0]:
+        raise NotImplementedError("fmm_potential: one strength (strength_usage None or [0])")
+    tree = wrangler.tree
+    if ti.exclude_self:
+        t2s = wrangler.self_extra_kwargs.get("target_to_source")
+        if not tree.sources_are_targets or t2s is None:
+            raise NotImplementedError("fmm_potential: exclude_self needs sources = targets")
+        ident = torch.arange(tree.ntargets, dtype=t2s.dtype, device=t2s.device)
+        if not torch.equal(t2s, ident):
+            raise NotImplementedError("fmm_potential: target_to_source must be the identity")
+
+
+def _conjugate_kernel_kwargs(wrangler):
+    """Kernel arguments of the conjugate kernel: Laplace unchanged; Helmholtz
+    with a real wavenumber k -> -k (conj e^{ikr}/r = e^{-ikr}/r)."""
+    kw = dict(wrangler.kernel_extra_kwargs)
+    base = wrangler.tree_indep.get_base_kernel()
+    if base.is_complex_valued:
+        name = base.get_args()[0]
+        k = complex(kw[name])
+        if k.imag:
+            raise NotImplementedError("the adjoint FMM needs a real wavenumber")
+        kw[name] = -k.real
+    return kw
+
+
+def adjoint_wrangler(wrangler, target_kernels=None):
+    """A wrangler for the adjoint (transposed, conjugate-kernel) FMM of
+    *wrangler*: the same boxes with sources and targets exchanged
+    (:func:`sumpy_amd.tree.swapped_traversal`), the same expansion orders,
+    precisions and options, the conjugate kernel, and *target_kernels*
+    (default: the base kernel; :class:`AxisTargetDerivative` kernels give the
+    gradient of the adjoint field, i.e. the source-position gradient).
+
+    Exact transpose: with the roles exchanged every stage is the transpose of
+    its forward counterpart up to the parity (-1)^|m| of the coefficients --
+    L2P^T = S P2M', M2M^T = S L2L' S, M2L^T = S M2L' S (class matrices
+    A_c[t, s] = D(t + s) are symmetric, D_{-d} = S D_d S), M2P^T = S P2L',
+    P2P^T = P2P' -- and the signs cancel along every path of the pass, so the
+    swapped pass computes A^T g to rounding (for real-valued kernels; with the
+    conjugate kernel, A^H g).  Cached on *wrangler* per target-kernel set."""
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.tree import swapped_traversal
+    ti = wrangler.tree_indep
+    base = ti.get_base_kernel()
+    tk = tuple(target_kernels) if target_kernels else (base,)
+    key = ("adjoint_wrangler", tk)
+    got = wrangler._memo.get(key)
+    if got is not None:
+        return got
+    if tk == tuple(ti.target_kernels) and ti.source_kernels in (None, [base]) and \
+            ti.strength_usage in (None, [0]):
+        ati = ti
+    else:
+        ati = ti._memo.get(("adjoint_tree_indep", tk))
+        if ati is None:
+            ati = SumpyTreeIndependentDataForWrangler(
+                ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory,
+                list(tk), exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
+                p2p_precision=ti.p2p_precision, autotune=ti.autotune)
+            ti._memo[("adjoint_tree_indep", tk)] = ati
+    trav = swapped_traversal(ti._setup_actx, wrangler.traversal)
+    orders = list(wrangler.level_orders)
+    got = SumpyExpansionWrangler(
+        ati, trav, wrangler.dtype, fmm_level_to_order=lambda kn, a, t, lev: orders[lev],
+        kernel_extra_kwargs=_conjugate_kernel_kwargs(wrangler),
+        self_extra_kwargs=wrangler.self_extra_kwargs,
+        translation_classes_data=None if trav is not wrangler.traversal
+        else wrangler.translation_classes_data,
+        preprocessed_mpole_dtype=wrangler.preprocessed_mpole_dtype,
+        _disable_translation_classes=not wrangler.supports_translation_classes)
+    wrangler._memo[key] = got
+    return got
+
+
+_adjoint_wrangler = adjoint_wrangler
+
+
+def _gradient_wrangler(wrangler):
+    """*wrangler* with the target gradient (AxisTargetDerivative per axis) as
+    its target kernels: the target-position gradient."""
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import AxisTargetDerivative
+    key = ("gradient_wrangler",)
+    got = wrangler._memo.get(key)
+    if got is not None:
+        return got
+    ti = wrangler.tree_indep
+    base = ti.get_base_kernel()
+    tk = [AxisTargetDerivative(a, base) for a in range(wrangler.tree.dimensions)]
+    gti = SumpyTreeIndependentDataForWrangler(
+        ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory, tk,
+        exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
+        p2p_precision=ti.p2p_precision, autotune=ti.autotune)
+    orders = list(wrangler.level_orders)
+    got = SumpyExpansionWrangler(
+        gti, wrangler.traversal, wrangler.dtype,
+        fmm_level_to_order=lambda kn, a, t, lev: orders[lev],
+        kernel_extra_kwargs=wrangler.kernel_extra_kwargs,
+        self_extra_kwargs=wrangler.self_extra_kwargs,
+        translation_classes_data=wrangler.translation_classes_data,
+        preprocessed_mpole_dtype=wrangler.preprocessed_mpole_dtype,
+        _disable_translation_classes=not wrangler.supports_translation_classes)
+    wrangler._memo[key] = got
+    return got
+
+
+def _re_conj_mul(a, b):
+    """Re(conj(a) b), real when both are real."""
+    if a.is_complex() or b.is_complex():
+        return (torch.conj(a)*b).real if a.is_complex() else a*b.real
+    return a*b
+
 
 class _FMMPotential(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, strength, actx, wrangler, adjoint_wrangler):
+    def forward(ctx, strength, sources, targets, actx, wrangler, adjoint):
         from sumpy_amd.drive import drive_fmm
-        ctx.actx, ctx.adjoint = actx, adjoint_wrangler
+        ctx.actx, ctx.wrangler, ctx.adjoint = actx, wrangler, adjoint
         ctx.real_input = not strength.is_complex()
-        pot, = drive_fmm(actx, wrangler, [strength.detach().contiguous()])
+        q = strength.detach().contiguous()
+        ctx.save_for_backward(q)
+        pot, = drive_fmm(actx, wrangler, [q])
         return pot
 
     @staticmethod
     def backward(ctx, g):
         from sumpy_amd.drive import drive_fmm
-        grad, = drive_fmm(ctx.actx, ctx.adjoint, [g.contiguous()])
-        if ctx.real_input and grad.is_complex():
-            grad = grad.real
-        return grad.contiguous(), None, None, None
+        from sumpy_amd.kernel import AxisTargetDerivative
+        q, = ctx.saved_tensors
+        wr, actx = ctx.wrangler, ctx.actx
+        g = g.contiguous()
+        need_q, need_s, need_t = ctx.needs_input_grad[:3]
+        dq = ds = dt = None
+        if need_q or need_s:
+            if need_s:
+                base = wr.tree_indep.get_base_kernel()
+                dim = wr.tree.dimensions
+                awr = adjoint_wrangler(
+                    wr, [base] + [AxisTargetDerivative(a, base) for a in range(dim)])
+            else:
+                awr = ctx.adjoint
+            out = drive_fmm(actx, awr, [g])
+            if need_q:
+                dq = out[0]
+                if ctx.real_input and dq.is_complex():
+                    dq = dq.real
+                dq = dq.contiguous()
+            if need_s:
+                # dL/ds_j = Re(conj(q_j) grad phi(s_j)), phi = sum_t conj(G)(x - t) g_t
+                ds = torch.stack([_re_conj_mul(q, out[1 + a]) for a in range(dim)])
+        if need_t:
+            gout = drive_fmm(actx, _gradient_wrangler(wr), [q])
+            dt = torch.stack([_re_conj_mul(d, g) for d in gout])
+        return dq, ds, dt, None, None, None
 
 
-def fmm_potential(actx, wrangler, strength, adjoint_wrangler=None):
-    """Differentiable FMM potential of *strength* (user order) for one target
-    kernel, sources = targets, self excluded (the wrangler's
-    ``target_to_source``).  *adjoint_wrangler* evaluates the adjoint pass; by
-    default the wrangler itself (real kernels).  For the Helmholtz kernel pass
-    a wrangler over the same tree with the wavenumber negated
-    (:func:`helmholtz_adjoint_wrangler`)."""
+def fmm_potential(actx, wrangler, strength, adjoint_wrangler=None, sources=None,
+                  targets=None):
+    """Differentiable FMM potential of *strength* (the caller's order) for
+    the wrangler's single target kernel.
+
+    Gradients flow to *strength* and, when given, to *sources* and *targets*:
+    ``(dim, n)`` tensors equal to the tree's source / target points in the
+    caller's order (the tree itself stays fixed; pass the same tensor as both
+    when the sources are the targets to get the total derivative).
+
+    * strengths: the adjoint FMM, i.e. the FMM with sources and targets
+      exchanged and the conjugate kernel (:func:`adjoint_wrangler`), the exact
+      transpose of the discrete forward pass;
+    * source positions: q_j times the gradient of that adjoint field at s_j
+      (the same adjoint pass with target-gradient kernels);
+    * target positions: g_t times the gradient of the forward field at t
+      (the forward pass with target-gradient kernels).
+
+    Every expansion is recomputed by the backward passes; only the strengths
+    are kept.  *adjoint_wrangler*: a wrangler for the strength-only adjoint
+    pass (default :func:`adjoint_wrangler` of *wrangler*)."""
+    from sumpy_amd.tree import user_order_points
+    _check_adjoint_config(wrangler)
     tree = wrangler.tree
-    if not tree.sources_are_targets:
-        raise NotImplementedError("fmm_potential: sources and targets must coincide")
-    if len(wrangler.tree_indep.target_kernels) != 1:
-        raise ValueError("fmm_potential: one target kernel")
+    for name, x in (("sources", sources), ("targets", targets)):
+        if x is not None and not torch.equal(
+                x.detach().to(torch.float64), user_order_points(tree, name)):
+            raise ValueError(f"fmm_potential: {name} differ from the tree's {name}")
     if adjoint_wrangler is None:
-        if wrangler.tree_indep.get_base_kernel().is_complex_valued:
-            raise ValueError("fmm_potential: complex kernels need an adjoint wrangler")
-        adjoint_wrangler = wrangler
-    return _FMMPotential.apply(strength, actx, wrangler, adjoint_wrangler)
+        adjoint_wrangler = _adjoint_wrangler(wrangler)
+    return _FMMPotential.apply(strength, sources, targets, actx, wrangler, adjoint_wrangler)
 
 
 def helmholtz_adjoint_wrangler(wrangler):
-    """A wrangler for the adjoint pass of a real-wavenumber Helmholtz FMM:
-    same tree, traversal, orders and options, with k -> -k (its translation
-    class data are generated for -k)."""
-    from sumpy_amd.fmm import SumpyExpansionWrangler
-    kw = dict(wrangler.kernel_extra_kwargs)
-    name = wrangler.tree_indep.get_base_kernel().get_args()[0]
-    k = complex(kw[name])
-    if k.imag:
-        raise NotImplementedError("the adjoint wrangler needs a real wavenumber")
-    kw[name] = -k.real
-    orders = list(wrangler.level_orders)
-    return SumpyExpansionWrangler(
-        wrangler.tree_indep, wrangler.traversal, wrangler.dtype,
-        fmm_level_to_order=lambda kn, a, t, lev: orders[lev],
-        source_extra_kwargs=wrangler.source_extra_kwargs, kernel_extra_kwargs=kw,
-        self_extra_kwargs=wrangler.self_extra_kwargs,
-        translation_classes_data=wrangler.translation_classes_data,
-        preprocessed_mpole_dtype=wrangler.preprocessed_mpole_dtype)
+    """The strength-adjoint wrangler of a real-wavenumber Helmholtz FMM
+    (k -> -k, sources and targets exchanged); :func:`adjoint_wrangler`."""
+    return adjoint_wrangler(wrangler)
 
 # }}}

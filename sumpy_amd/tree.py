@@ -261,7 +261,56 @@ def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=3
                              max_levels=None):
     """Builds tree + traversal + translation classes in one host pass."""
     sc, th, vh = _build_host(sources, targets, max_particles_in_box, max_levels)
-    return _wrap(actx, sc, th, vh)
+    tree, trav = _wrap(actx, sc, th, vh)
+    tree.__dict__["_build_params"] = (max_particles_in_box, max_levels)
+    return tree, trav
+
+
+def user_order_points(tree, which):
+    """The tree's ``"sources"`` or ``"targets"`` in the caller's order, on
+    the tree's device (``(dim, n)``)."""
+    pts = getattr(tree, which)
+    if which == "sources":
+        ids = tree.host["user_source_ids"]
+    else:
+        # sorted_target_ids[u] is the tree position of user target u
+        return pts[:, torch.from_numpy(tree.host["sorted_target_ids"].astype(np.int64))
+                   .to(pts.device)]
+    out = torch.empty_like(pts)
+    out[:, torch.from_numpy(ids.astype(np.int64)).to(pts.device)] = pts
+    return out
+
+
+def swapped_traversal(actx, traversal):
+    """The traversal of the same boxes with the roles of sources and targets
+    exchanged: its targets are this tree's sources and its sources this
+    tree's targets.  The builder splits on the joint particle count and
+    numbers boxes by position only, so the swapped tree has the same boxes
+    (checked); every interaction list of the swapped traversal is then the
+    transpose of this one's (list 1 of the swapped target box b holds the
+    boxes whose list 1 holds b, list 2 likewise, lists 3 and 4 exchange).
+    Held on the traversal after the first call."""
+    tree = traversal.tree
+    if tree.sources_are_targets:
+        return traversal
+    got = traversal.__dict__.get("_swapped")
+    if got is not None:
+        return got
+    params = tree.__dict__.get("_build_params")
+    if params is None:
+        raise NotImplementedError("swapped_traversal: the tree was not built by "
+                                  "build_tree_and_traversal (a forest?)")
+    src = user_order_points(tree, "sources").cpu().numpy()
+    tgt = user_order_points(tree, "targets").cpu().numpy()
+    stree, strav = build_tree_and_traversal(actx, tgt, src, *params)
+    for name in ("box_levels", "box_parent_ids", "box_child_ids", "level_start_box_nrs"):
+        if not np.array_equal(stree.host[name], tree.host[name]):
+            raise RuntimeError(f"swapped_traversal: boxes differ ({name})")
+    if not np.array_equal(stree.host["box_centers"], tree.host["box_centers"]) or \
+            stree.root_extent != tree.root_extent:
+        raise RuntimeError("swapped_traversal: box geometry differs")
+    traversal.__dict__["_swapped"] = strav
+    return strav
 
 
 # {{{ forests: many independent problems in one tree

@@ -161,8 +161,10 @@ def test_fmm_strength_adjoint_fd(actx, kernel):
     excluded) by the adjoint pass, against central differences of the FMM
     itself on 64 random strengths.  The FMM is linear in the strengths, so
     the differences are its exact transpose applied to the loss weights; the
-    adjoint pass (the FMM of the conjugate kernel) equals it to within the
-    FMM's accuracy at order 10 (tolerance 1e-5 relative)."""
+    adjoint pass (the FMM of the conjugate kernel over the exchanged roles) is
+    that transpose to rounding; the differences themselves carry the
+    rounding of L (~1e-16 |L| / h), about 1e-8 relative here (tolerance
+    1e-7)."""
     from functools import partial
 
     from sumpy_amd.autograd import fmm_potential, helmholtz_adjoint_wrangler
@@ -211,4 +213,176 @@ def test_fmm_strength_adjoint_fd(actx, kernel):
         if cplx:
             fd_im.append((L(q + 1j*e) - L(q - 1j*e))/(2*h))
     fd = np.array(fd_re) + (1j*np.array(fd_im) if cplx else 0)
-    assert np.linalg.norm(grad[idx] - fd)/np.linalg.norm(fd) < 1e-5
+    assert np.linalg.norm(grad[idx] - fd)/np.linalg.norm(fd) < 1e-7
+
+
+def _fmm_setup(actx, kernel, src, tgt, order, source_kernels=None, target_kernels=None,
+               tree_trav=None):
+    from functools import partial
+
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = LaplaceKernel(3) if kernel == "laplace" else HelmholtzKernel(3)
+    cplx = kernel == "helmholtz"
+    tree, trav = tree_trav or build_tree_and_traversal(actx, src, tgt, 24)
+    same = tree.sources_are_targets   # then the self pair is excluded
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+        partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl),
+        target_kernels or [knl], source_kernels=source_kernels, exclude_self=same)
+    wr = SumpyExpansionWrangler(
+        ti, trav, np.complex128 if cplx else np.float64,
+        fmm_level_to_order=lambda kn, a, t, lev: order,
+        kernel_extra_kwargs={"k": 2.5} if cplx else None,
+        self_extra_kwargs={"target_to_source": actx.from_numpy(
+            np.arange(tree.ntargets, dtype=np.int32))} if same else None)
+    return knl, tree, trav, wr
+
+
+def _separate(seed, ns=4000, nt=1500):
+    """Clustered sources (lists 3 and 4 populated) and separate uniform
+    targets."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.25, 0.75, (4, 3))
+    src = np.clip(c[rng.integers(0, 4, ns)].T + 0.05*rng.standard_normal((3, ns)), 0, 1)
+    return rng, src, rng.random((3, nt))
+
+
+@pytest.mark.parametrize("kernel", ["laplace", "helmholtz"])
+def test_fmm_adjoint_is_exact_transpose_separate_targets(actx, kernel):
+    """Sources != targets: the adjoint pass (roles exchanged, conjugate
+    kernel) satisfies <g, A q> = <A^H g, q> to rounding, on a tree whose
+    lists 3 and 4 are populated; and the strength gradient matches central
+    differences of the FMM on 64 strengths."""
+    from sumpy_amd.autograd import fmm_potential
+    from sumpy_amd.drive import drive_fmm
+    rng, src, tgt = _separate(21)
+    knl, tree, trav, wr = _fmm_setup(actx, kernel, src, tgt, 8)
+    assert sum(len(x[2]) for x in trav.host["sep_smaller"]) > 0
+    assert len(trav.host["from_sep_bigger_lists"]) > 0
+    cplx = kernel == "helmholtz"
+    ns, nt = src.shape[1], tgt.shape[1]
+    q = rng.standard_normal(ns) + (1j*rng.standard_normal(ns) if cplx else 0)
+    g = rng.standard_normal(nt) + (1j*rng.standard_normal(nt) if cplx else 0)
+    qt = actx.from_numpy(q).requires_grad_(True)
+    pot = fmm_potential(actx, wr, qt)
+    gt = actx.from_numpy(g)
+    loss = (torch.conj(gt)*pot).real.sum()
+    loss.backward()
+    aq = actx.to_numpy(pot.detach())
+    ahg = actx.to_numpy(qt.grad)          # A^H g (real part only for real q)
+    lhs, rhs = np.vdot(g, aq), np.vdot(ahg, q)
+    assert abs(lhs - rhs) <= 1e-12*np.linalg.norm(g)*np.linalg.norm(aq), (lhs, rhs)
+
+    def L(qq):
+        p, = drive_fmm(actx, wr, [actx.from_numpy(qq)])
+        return float(np.real(np.vdot(g, actx.to_numpy(p))))
+    idx = rng.choice(ns, 64, replace=False)
+    h = 1e-6
+    fd = []
+    for j in idx:
+        e = np.zeros(ns, q.dtype)
+        e[j] = h
+        v = (L(q + e) - L(q - e))/(2*h)
+        if cplx:
+            v = v + 1j*(L(q + 1j*e) - L(q - 1j*e))/(2*h)
+        fd.append(v)
+    fd = np.array(fd)
+    # the differences carry the rounding of L over 2h (~1e-8 relative)
+    assert np.linalg.norm(ahg[idx] - fd)/np.linalg.norm(fd) < 1e-7
+
+
+def _tree_index(tree, which):
+    """tree position of each point in the caller's order"""
+    if which == "sources":
+        inv = np.empty_like(tree.host["user_source_ids"])
+        inv[tree.host["user_source_ids"]] = np.arange(len(inv))
+        return inv
+    return tree.host["sorted_target_ids"]
+
+
+@pytest.mark.parametrize("kernel,same", [("laplace", False), ("laplace", True),
+                                         ("helmholtz", False)])
+def test_fmm_position_gradients_fd(actx, kernel, same):
+    """Source- and target-position gradients of sum_t Re(conj(w_t) pot_t)
+    through the whole FMM against central differences (h = 1e-6) of the FMM
+    with the points moved in place (the tree fixed), on 64 coordinates;
+    sources = targets passes one tensor for both (total derivative)."""
+    from sumpy_amd.autograd import fmm_potential
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.tree import user_order_points
+    rng, src, tgt = _separate(22)
+    knl, tree, trav, wr = _fmm_setup(actx, kernel, src, None if same else tgt, 8)
+    cplx = kernel == "helmholtz"
+    ns, nt = tree.nsources, tree.ntargets
+    q = rng.standard_normal(ns) + (1j*rng.standard_normal(ns) if cplx else 0)
+    w = rng.standard_normal(nt) + (1j*rng.standard_normal(nt) if cplx else 0)
+    S = user_order_points(tree, "sources").clone().requires_grad_(True)
+    T = S if same else user_order_points(tree, "targets").clone().requires_grad_(True)
+    pot = fmm_potential(actx, wr, actx.from_numpy(q), sources=S, targets=T)
+    (torch.conj(actx.from_numpy(w))*pot).real.sum().backward()
+    gS, gT = actx.to_numpy(S.grad), actx.to_numpy(T.grad)
+
+    def L():
+        p, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        return float(np.real(np.vdot(w, actx.to_numpy(p))))
+    h = 1e-6
+    checks = [("sources", gS)] + ([] if same else [("targets", gT)])
+    for which, grad in checks:
+        pts = getattr(tree, which)        # device, tree order (moved in place)
+        pos = _tree_index(tree, which)
+        fd, got = [], []
+        for _ in range(64 // len(checks)):
+            a, u = int(rng.integers(0, 3)), int(rng.integers(0, pts.shape[1]))
+            x0 = float(pts[a, pos[u]])
+            pts[a, pos[u]] = x0 + h
+            lp = L()
+            pts[a, pos[u]] = x0 - h
+            lm = L()
+            pts[a, pos[u]] = x0
+            fd.append((lp - lm)/(2*h))
+            got.append(grad[a, u])
+        fd, got = np.array(fd), np.array(got)
+        # central differences err by ~(h/r)^2 for a point whose nearest
+        # neighbour is r away (r >~ 1e-3 here)
+        assert np.linalg.norm(got - fd)/np.linalg.norm(fd) < 1e-5, which
+
+
+def test_fmm_source_gradient_matches_source_derivative_fmm(actx):
+    """The source-position gradient is q_j (D_a^T g)_j with D_a the matrix of
+    the AxisSourceDerivative(a) FMM (reference ``kernel.py:2288-2327``): for
+    random w, sum_j w_j grad_s[a, j] / q_j equals <g, FMM_{d/dy_a}(w)>; the
+    target-position gradient is g_t times the AxisTargetDerivative(a) FMM."""
+    from sumpy_amd.autograd import fmm_potential
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.kernel import AxisSourceDerivative, AxisTargetDerivative
+    from sumpy_amd.tree import user_order_points
+    rng, src, tgt = _separate(23)
+    knl, tree, trav, wr = _fmm_setup(actx, "laplace", src, tgt, 8)
+    ns, nt = tree.nsources, tree.ntargets
+    q = 1.0 + rng.random(ns)
+    g = rng.standard_normal(nt)
+    S = user_order_points(tree, "sources").clone().requires_grad_(True)
+    T = user_order_points(tree, "targets").clone().requires_grad_(True)
+    pot = fmm_potential(actx, wr, actx.from_numpy(q), sources=S, targets=T)
+    (actx.from_numpy(g)*pot).sum().backward()
+    gS, gT = actx.to_numpy(S.grad), actx.to_numpy(T.grad)
+    w = rng.standard_normal(ns)
+    for a in range(3):
+        _, _, _, wa = _fmm_setup(actx, "laplace", src, tgt, 8,
+                                 source_kernels=[AxisSourceDerivative(a, knl)],
+                                 tree_trav=(tree, trav))
+        fa, = drive_fmm(actx, wa, [actx.from_numpy(w)])
+        lhs = float(np.sum(w*gS[a]/q))
+        rhs = float(g @ actx.to_numpy(fa))
+        assert abs(lhs - rhs) <= 1e-12*(abs(rhs) + np.linalg.norm(g)*np.linalg.norm(
+            actx.to_numpy(fa))), (a, lhs, rhs)
+        _, _, _, ta = _fmm_setup(actx, "laplace", src, tgt, 8,
+                                 target_kernels=[AxisTargetDerivative(a, knl)],
+                                 tree_trav=(tree, trav))
+        da, = drive_fmm(actx, ta, [actx.from_numpy(q)])
+        assert np.linalg.norm(gT[a] - g*actx.to_numpy(da)) <= \
+            1e-13*np.linalg.norm(g*actx.to_numpy(da))

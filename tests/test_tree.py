@@ -117,3 +117,48 @@ def test_boxtree_interface_names(built):
     v2, _ = FMMTraversalBuilder(actx)(actx, t2)
     tc, _ = TranslationClassesBuilder(actx)(actx, v2, t2)
     assert tc.from_sep_siblings_translation_classes.shape[0] == len(v2.host["from_sep_siblings_lists"])
+
+
+def _pairs(vh):
+    """(target box, source box) pairs of lists 1, 2, 3 and 4."""
+    totp = vh["target_or_target_parent_boxes"]
+
+    def csr(boxes, st, li):
+        return {(int(t), int(s)) for i, t in enumerate(boxes) for s in li[st[i]:st[i + 1]]}
+    l1 = csr(vh["target_boxes"], vh["neighbor_source_boxes_starts"],
+             vh["neighbor_source_boxes_lists"])
+    l2 = csr(totp, vh["from_sep_siblings_starts"], vh["from_sep_siblings_lists"])
+    l3 = set()
+    for tb, st, li in vh["sep_smaller"]:
+        l3 |= csr(tb, st, li)
+    l4 = csr(totp, vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"])
+    return l1, l2, l3, l4
+
+
+def test_swapped_traversal_transposes_every_list():
+    """Sources and targets exchanged on the same boxes: lists 1 and 2 are
+    transposed, and lists 3 and 4 exchange (the dual lists of the adaptive
+    FMM), which makes the FMM with swapped roles the exact transpose of the
+    forward FMM."""
+    from sumpy_amd.tree import swapped_traversal, user_order_points
+    rng = np.random.default_rng(5)
+    c = rng.uniform(0.2, 0.8, (4, 3))
+    src = np.clip(c[rng.integers(0, 4, 3000)].T + 0.05*rng.standard_normal((3, 3000)), 0, 1)
+    tgt = rng.random((3, 1200))
+    actx = TorchArrayContext("cpu")
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 20)
+    assert np.array_equal(user_order_points(tree, "sources").numpy(), src)
+    assert np.array_equal(user_order_points(tree, "targets").numpy(), tgt)
+    strav = swapped_traversal(actx, trav)
+    assert swapped_traversal(actx, trav) is strav
+    st = strav.tree
+    assert np.array_equal(st.host["box_centers"], tree.host["box_centers"])
+    assert np.array_equal(user_order_points(st, "sources").numpy(), tgt)
+    l1, l2, l3, l4 = _pairs(trav.host)
+    s1, s2, s3, s4 = _pairs(strav.host)
+    assert len(l3) > 0 and len(l4) > 0
+
+    def tr(p):
+        return {(b, a) for a, b in p}
+    assert s1 == tr(l1) and s2 == tr(l2)
+    assert s4 == tr(l3) and s3 == tr(l4)
