@@ -304,6 +304,23 @@ int sumpyhip_m2l_apply(int precision, int nlevels, const sumpyhip_m2l_level *lev
                        int nl, int nm, int npad, int kpad, const int *off_t,
                        const int *off_s, int ncirc, void *stream);
 
+/* Bytes of the fixed device scratch of sumpyhip_m2l_apply_noalloc. */
+#define SUMPYHIP_M2L_NOALLOC_SCRATCH_BYTES (64ull << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sumpyhip_m2l_apply with the same arguments and results,
+ * except that levels[l].partial is ignored -- the split partial sums of all
+ * levels live in a fixed device scratch of SUMPYHIP_M2L_NOALLOC_SCRATCH_BYTES
+ * per GPU (held until the process ends), and a call whose partial sums need
+ * more (sum over levels with nsplit > 1 of (nsplit - 1) * ntiles * 64 * npad
+ * values of the compute precision, each level rounded up to 256 bytes) fails
+ * with an argument error.  Allocates nothing.  Calls that overlap on one GPU
+ * (other streams or host threads) race on the scratch. */
+int sumpyhip_m2l_apply_noalloc(int precision, int nlevels, const sumpyhip_m2l_level *levels,
+                               int nl, int nm, int npad, int kpad, const int *off_t,
+                               const int *off_s, int ncirc, void *stream);
+
 /* Split-BF16 form (precision 2) support, no reference counterpart (a layout
  * of this library's split-BF16 contraction): the class data of nclasses
  * classes (circulant shape circ_shape, as written by sumpyhip_m2l_class_data)

@@ -85,6 +85,7 @@ _SIGNATURES = {
     "sumpyhip_m2l_class_data": [_I, _I, _I, _P, _I, _P, _I, _I, _D, _P, _P],
     "sumpyhip_m2l_preprocess": [_I, _I, _I, _P, _I, _P, _P, _P],
     "sumpyhip_m2l_apply": [_I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _P],
+    "sumpyhip_m2l_apply_noalloc": [_I, _I, _P, _I, _I, _I, _I, _P, _P, _I, _P],
     "sumpyhip_m2l_postprocess": [_I, _I, _I, _I, _P, _P, _P, _D, _P, _P],
     "sumpyhip_m2l_csr": [_I, _I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P,
                          _I, _D, _D, _P, _I, _P, _I, _P],

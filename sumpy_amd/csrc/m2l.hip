@@ -18,6 +18,7 @@
 // v_mfma_f64_16x16x4 (plain form) or v_mfma_f32_16x16x4 (FP32 form), A
 // fragments gathered straight from data_c in LDS.  Roofline: MFMA.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -811,6 +812,45 @@ extern "C" int sumpyhip_m2l_postprocess(int precision, int nboxes, int nl, int n
   return check_launch("m2l_postprocess");
 }
 
+namespace sumpyhip {
+namespace {
+
+// Fixed device scratch of sumpyhip_m2l_apply_noalloc: the split partial sums
+// of one call.  Every GPU the library runs on holds one copy until the
+// process ends; concurrent calls on one GPU share it (and race).
+constexpr size_t M2L_SCRATCH_BYTES = (size_t)SUMPYHIP_M2L_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[M2L_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[MAX_DEVICES];
+
+// The device address of the current device's copy of scratch (the host-side
+// name is only a stand-in), looked up once per device with it current.
+int scratch_address(void **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return set_error(SUMPYHIP_EHIP, "m2l_apply_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  SUMPY_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "m2l_apply_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(SUMPYHIP_EHIP, "m2l_apply_noalloc: hipGetSymbolAddress(scratch): %s",
+                       hipGetErrorString(e));
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return SUMPYHIP_OK;
+}
+
+size_t precision_bytes(int precision) { return precision == 0 ? sizeof(double) : sizeof(float); }
+
+}  // namespace
+}  // namespace sumpyhip
+
 extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
                                   const sumpyhip_m2l_level *levels, int nl, int nm, int npad,
                                   int kpad, const int *off_t, const int *off_s, int ncirc,
@@ -846,6 +886,50 @@ extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
   hipStream_t st = (hipStream_t)stream;
   if (precision == 2) return dispatch_apply_bf16(a, st);
   return precision == 0 ? dispatch_apply<double>(a, st) : dispatch_apply<float>(a, st);
+}
+
+extern "C" int sumpyhip_m2l_apply_noalloc(int precision, int nlevels,
+                                          const sumpyhip_m2l_level *levels, int nl, int nm,
+                                          int npad, int kpad, const int *off_t,
+                                          const int *off_s, int ncirc, void *stream) {
+  SUMPY_REQUIRE(precision >= 0 && precision <= 2, "m2l_apply_noalloc: unknown precision %d",
+                precision);
+  SUMPY_REQUIRE(nlevels >= 0 && nlevels <= M2L_MAXLEV, "m2l_apply_noalloc: %d levels (at most %d)",
+                nlevels, M2L_MAXLEV);
+  SUMPY_REQUIRE(nlevels == 0 || levels != nullptr, "m2l_apply_noalloc: null levels");
+  // the split partial sums of every level, carved out of the fixed scratch
+  // (256-byte aligned regions); every entry is written before it is read, so
+  // nothing needs clearing between calls
+  sumpyhip_m2l_level lv[M2L_MAXLEV];
+  size_t need = 0;
+  for (int l = 0; l < nlevels; ++l) {
+    lv[l] = levels[l];
+    lv[l].partial = nullptr;
+    if (lv[l].ntiles > 0 && lv[l].nsplit > 1) {
+      const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*TW*(size_t)npad*
+                           precision_bytes(precision);
+      need += (bytes + 255) & ~(size_t)255;
+    }
+  }
+  SUMPY_REQUIRE(need <= M2L_SCRATCH_BYTES,
+                "m2l_apply_noalloc: the split partial sums need %zu bytes, the fixed scratch "
+                "holds %zu (use fewer splits or sumpyhip_m2l_apply)", need, M2L_SCRATCH_BYTES);
+  if (need) {
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    size_t off = 0;
+    for (int l = 0; l < nlevels; ++l) {
+      if (lv[l].ntiles > 0 && lv[l].nsplit > 1) {
+        lv[l].partial = static_cast<unsigned char *>(base) + off;
+        const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*TW*(size_t)npad*
+                             precision_bytes(precision);
+        off += (bytes + 255) & ~(size_t)255;
+      }
+    }
+  }
+  return sumpyhip_m2l_apply(precision, nlevels, lv, nl, nm, npad, kpad, off_t, off_s, ncirc,
+                            stream);
 }
 
 extern "C" int sumpyhip_m2l_csr(int kernel_id, int order_src, int order_tgt,

@@ -648,11 +648,13 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         return kwargs["tgt_expansions"]
 
 
-def m2l_apply_levels(actx, entries, policy=None):
+def m2l_apply_levels(actx, entries, policy=None, noalloc=False):
     """One launch of the MFMA contraction over several levels (same
     translation operator orders), largest level first.  *policy* = (items
     per CU, minimum slots per split) overrides the split rule of the
-    schedules (:mod:`sumpy_amd.autotune`)."""
+    schedules (:mod:`sumpy_amd.autotune`).  *noalloc*: the split partial sums
+    go to the library's fixed device scratch (``sumpyhip_m2l_apply_noalloc``)
+    instead of buffers from torch's allocator."""
     entries = [e for e in entries if e is not None]
     if not entries:
         return
@@ -676,7 +678,7 @@ def m2l_apply_levels(actx, entries, policy=None):
         if int(work.shape[1]) != npad or int(e["src"].shape[1]) != kpad:
             raise ValueError("m2l_apply_levels: inconsistent row padding")
         partial = None
-        if sched["nsplit"] > 1:
+        if sched["nsplit"] > 1 and not noalloc:
             partial = torch.empty((sched["nsplit"] - 1)*sched["ntiles"]*TILE*npad,
                                   dtype=work.dtype, device=work.device)
             keep.append(partial)
@@ -700,8 +702,9 @@ def m2l_apply_levels(actx, entries, policy=None):
     else:
         geom = (m["nl"], m["nm"], npad, kpad, _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]),
                 m["ncirc"])
+    entry_point = "sumpyhip_m2l_apply_noalloc" if noalloc else "sumpyhip_m2l_apply"
     with kernel_span(f"m2l_apply_{tr.precision}", flops):
-        _capi.call("sumpyhip_m2l_apply", tr.precision_id, len(levels), ctypes.addressof(arr),
+        _capi.call(entry_point, tr.precision_id, len(levels), ctypes.addressof(arr),
                    *geom, actx.stream_handle)
 
 
