@@ -193,6 +193,37 @@ int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                  int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
                  void *stream);
 
+/* M2P from list 3 over source boxes of several tree levels in one launch,
+ * replaces the per-level loop of SumpyExpansionWrangler.eval_multipoles
+ * (sumpy/fmm.py:922-961) over E2PFromCSR.__call__ (sumpy/e2p.py:233-343)
+ * when every level has the same expansion order: source box sb has the
+ * rscale level_rscale[box_levels[sb]] (nlevels <= 32) and its multipole at
+ * row sb of src_expansions (one row per tree box).  The source lists of a
+ * target box are summed in list order; accumulates like sumpyhip_m2p. */
+int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                        const int *target_boxes, const int *source_box_starts,
+                        const int *source_box_lists, const int *box_target_starts,
+                        const int *box_target_counts_nonchild, const double *box_centers,
+                        int nboxes, const double *tx, const double *ty, const double *tz,
+                        const int *box_levels, int nlevels, const double *level_rscale,
+                        double scale, const double *src_expansions, double *pot, double *gx,
+                        double *gy, double *gz, void *stream);
+
+/* P2L from list 4 for target boxes of several tree levels in one launch,
+ * replaces the per-level loop of SumpyExpansionWrangler.form_locals
+ * (sumpy/fmm.py:963-1002) over P2EFromCSR.__call__ (sumpy/p2e.py:268-388)
+ * when every level has the same expansion order: target box tb has the
+ * rscale level_rscale[box_levels[tb]] and its local at row tb of
+ * tgt_expansions (one row per tree box); overwrites like sumpyhip_p2l. */
+int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                        const int *target_boxes, const int *source_box_starts,
+                        const int *source_box_lists, const int *box_source_starts,
+                        const int *box_source_counts_nonchild, const double *box_centers,
+                        int nboxes, const double *sx, const double *sy, const double *sz,
+                        const double *w0, const double *wx, const double *wy,
+                        const double *wz, const int *box_levels, int nlevels,
+                        const double *level_rscale, double *tgt_expansions, void *stream);
+
 /* }}} */
 
 /* {{{ Helmholtz-3D (G = e^{ik|d|}/|d|, sumpy/kernel.py:754-844), real k */

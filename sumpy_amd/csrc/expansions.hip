@@ -688,11 +688,19 @@ __device__ __forceinline__ double shift_x(const double *cur, int mx, int my) {
 
 constexpr int LW = 4;  // waves per block of the list 3 / 4 kernels
 
+// rscale per tree level, for launches that span several levels (a box's
+// rscale is r[box_levels[box]]); unused when box_levels is null
+constexpr int LIST_MAXLEV = 32;
+struct LevelScales {
+  double r[LIST_MAXLEV];
+};
+
 // M2P consumer: P += sum_m M(m) D(m) (shells <= ORD); with GRAD also
 // G_ax += sum_m M(m) D(m + e_ax) from the next shell.
 template <int ORD, bool GRAD>
 struct M2PSum {
   const double *M;  // this lane's source box, shell order (LDS)
+  double gs = 1;    // 1 / rscale of that box (the gradient's extra derivative)
   double P = 0, G[3] = {0, 0, 0};
   template <int N>
   __device__ __forceinline__ void shell(const double *cur, const double *prev) {
@@ -706,7 +714,7 @@ struct M2PSum {
       for (int mx = 0; mx <= 1; ++mx) {
         #pragma unroll
         for (int my = 0; my <= n - mx; ++my) {
-          const double m = M[n*n + sidx(n, mx, my)];
+          const double m = gs*M[n*n + sidx(n, mx, my)];
           G[0] += m*shift_x<N>(cur, mx, my);
           G[1] += m*cur[sidx(N, mx, my + 1)];
           G[2] += m*cur[sidx(N, mx, my)];
@@ -729,7 +737,7 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
     const double *__restrict__ centers, int nboxes, const double *__restrict__ tx,
     const double *__restrict__ ty, const double *__restrict__ tz, double rscale, double scale,
     const double *__restrict__ src, int sbase, double *pot, double *gx, double *gy,
-    double *gz) {
+    double *gz, const int *__restrict__ box_levels, LevelScales ls) {
   constexpr int NS = (ORD + 1)*(ORD + 1);
   constexpr int NC = GRAD ? 4 : 1;
   __shared__ int sPos[NS];
@@ -762,8 +770,10 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
         wave_sync();
         if (base + grp < is1) {
           const int sb = slists[base + grp];
+          const double rs = box_levels ? ls.r[box_levels[sb]] : rscale;
+          f.gs = 1.0/rs;
           const LapGeo g(x - centers[sb], y - centers[nboxes + sb],
-                         z - centers[2*(size_t)nboxes + sb], rscale);
+                         z - centers[2*(size_t)nboxes + sb], rs);
           lap3d_shells<ORD + (GRAD ? 1 : 0)>(g, f);
         }
       }
@@ -781,7 +791,7 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
         const int tt = t0 + tc + tid;
         if (pot) pot[tt] += scale*v[0];
         if (GRAD) {
-          gx[tt] += scale*v[1]/rscale; gy[tt] += scale*v[2]/rscale; gz[tt] += scale*v[3]/rscale;
+          gx[tt] += scale*v[1]; gy[tt] += scale*v[2]; gz[tt] += scale*v[3];
         }
       }
       __syncthreads();
@@ -865,8 +875,8 @@ __global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
     const double *__restrict__ centers, int nboxes, const double *__restrict__ sx,
     const double *__restrict__ sy, const double *__restrict__ sz,
     const double *__restrict__ w0, const double *__restrict__ wx,
-    const double *__restrict__ wy, const double *__restrict__ wz, double rscale, double *out,
-    int base) {
+    const double *__restrict__ wy, const double *__restrict__ wz, double rscale_level,
+    double *out, int base, const int *__restrict__ box_levels, LevelScales ls) {
   using Sum = P2LSum<ORD, DIP>;
   constexpr int NS = Sum::NS;
   __shared__ int sStored[NS];  // shell position -> stored index
@@ -881,6 +891,7 @@ __global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
   for (int ib = blockIdx.x*P2L_LW + wave; ib < ntgt; ib += gridDim.x*P2L_LW) {
     const int tb = target_boxes[ib];
     const int is0 = sstarts[ib], is1 = sstarts[ib + 1];
+    const double rscale = box_levels ? ls.r[box_levels[tb]] : rscale_level;
     double *o = out + (size_t)(tb - base)*NS;
     if (is0 == is1) {
       // empty list 4: the expansion is overwritten with zeros
@@ -1145,15 +1156,14 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   return check_launch("l2p");
 }
 
-extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
-                            const int *target_boxes, const int *source_box_starts,
-                            const int *source_box_lists, const int *box_target_starts,
-                            const int *box_target_counts_nonchild,
-                            const double *box_centers, int nboxes, const double *tx,
-                            const double *ty, const double *tz, double rscale,
-                            double scale, const double *src_expansions, int src_base_ibox,
-                            double *pot, double *gx, double *gy, double *gz,
-                            void *stream) {
+static int m2p_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                    const int *target_boxes, const int *source_box_starts,
+                    const int *source_box_lists, const int *box_target_starts,
+                    const int *box_target_counts_nonchild, const double *box_centers,
+                    int nboxes, const double *tx, const double *ty, const double *tz,
+                    double rscale, double scale, const double *src_expansions,
+                    int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
+                    const int *box_levels, const LevelScales &ls, void *stream) {
   SUMPY_REQUIRE(kernel_id == 3, "m2p: only the Laplace 3D kernel is implemented");
   int rc = check_expn(expn, "m2p");
   if (rc) return rc;
@@ -1170,12 +1180,12 @@ extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
       m2p_lap3d<ORD, true><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                         \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
-          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);                    \
+          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz, box_levels, ls);    \
     else                                                                                     \
       m2p_lap3d<ORD, false><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                        \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
-          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz);                    \
+          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz, box_levels, ls);    \
     break;
   switch (expn->order) {
     SUMPY_M2P(0) SUMPY_M2P(1) SUMPY_M2P(2) SUMPY_M2P(3) SUMPY_M2P(4) SUMPY_M2P(5)
@@ -1186,15 +1196,14 @@ extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   return check_launch("m2p");
 }
 
-extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
-                            const int *target_boxes, const int *source_box_starts,
-                            const int *source_box_lists, const int *box_source_starts,
-                            const int *box_source_counts_nonchild,
-                            const double *box_centers, int nboxes, const double *sx,
-                            const double *sy, const double *sz, const double *w0,
-                            const double *wx, const double *wy, const double *wz,
-                            double rscale, double *tgt_expansions, int tgt_base_ibox,
-                            void *stream) {
+static int p2l_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                    const int *target_boxes, const int *source_box_starts,
+                    const int *source_box_lists, const int *box_source_starts,
+                    const int *box_source_counts_nonchild, const double *box_centers,
+                    int nboxes, const double *sx, const double *sy, const double *sz,
+                    const double *w0, const double *wx, const double *wy, const double *wz,
+                    double rscale, double *tgt_expansions, int tgt_base_ibox,
+                    const int *box_levels, const LevelScales &ls, void *stream) {
   SUMPY_REQUIRE(kernel_id == 3, "p2l: only the Laplace 3D kernel is implemented");
   int rc = check_expn(expn, "p2l");
   if (rc) return rc;
@@ -1211,12 +1220,12 @@ extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
       p2l_lap3d<ORD, true><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                         \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
-          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
+          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox, box_levels, ls);            \
     else                                                                                     \
       p2l_lap3d<ORD, false><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                        \
           *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
           box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
-          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox);                            \
+          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox, box_levels, ls);            \
     break;
   switch (expn->order) {
     SUMPY_P2L(0) SUMPY_P2L(1) SUMPY_P2L(2) SUMPY_P2L(3) SUMPY_P2L(4) SUMPY_P2L(5)
@@ -1225,4 +1234,83 @@ extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   }
 #undef SUMPY_P2L
   return check_launch("p2l");
+}
+
+static int level_scales(int nlevels, const double *level_rscale, LevelScales &ls,
+                        const char *what) {
+  SUMPY_REQUIRE(nlevels >= 1 && nlevels <= LIST_MAXLEV && level_rscale,
+                "%s: 1..%d level scales needed, got %d", what, LIST_MAXLEV, nlevels);
+  for (int l = 0; l < LIST_MAXLEV; ++l) ls.r[l] = level_rscale[l < nlevels ? l : nlevels - 1];
+  return SUMPYHIP_OK;
+}
+
+extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *source_box_starts,
+                            const int *source_box_lists, const int *box_target_starts,
+                            const int *box_target_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *tx,
+                            const double *ty, const double *tz, double rscale,
+                            double scale, const double *src_expansions, int src_base_ibox,
+                            double *pot, double *gx, double *gy, double *gz,
+                            void *stream) {
+  LevelScales ls{};
+  return m2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
+                  source_box_lists, box_target_starts, box_target_counts_nonchild, box_centers,
+                  nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot, gx, gy,
+                  gz, nullptr, ls, stream);
+}
+
+extern "C" int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                                   const int *target_boxes, const int *source_box_starts,
+                                   const int *source_box_lists, const int *box_target_starts,
+                                   const int *box_target_counts_nonchild,
+                                   const double *box_centers, int nboxes, const double *tx,
+                                   const double *ty, const double *tz, const int *box_levels,
+                                   int nlevels, const double *level_rscale, double scale,
+                                   const double *src_expansions, double *pot, double *gx,
+                                   double *gy, double *gz, void *stream) {
+  SUMPY_REQUIRE(box_levels, "m2p_levels: box_levels needed");
+  LevelScales ls;
+  int rc = level_scales(nlevels, level_rscale, ls, "m2p_levels");
+  if (rc) return rc;
+  return m2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
+                  source_box_lists, box_target_starts, box_target_counts_nonchild, box_centers,
+                  nboxes, tx, ty, tz, level_rscale[0], scale, src_expansions, 0, pot, gx, gy, gz,
+                  box_levels, ls, stream);
+}
+
+extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *source_box_starts,
+                            const int *source_box_lists, const int *box_source_starts,
+                            const int *box_source_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *sx,
+                            const double *sy, const double *sz, const double *w0,
+                            const double *wx, const double *wy, const double *wz,
+                            double rscale, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  LevelScales ls{};
+  return p2l_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
+                  source_box_lists, box_source_starts, box_source_counts_nonchild, box_centers,
+                  nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox,
+                  nullptr, ls, stream);
+}
+
+extern "C" int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                                   const int *target_boxes, const int *source_box_starts,
+                                   const int *source_box_lists, const int *box_source_starts,
+                                   const int *box_source_counts_nonchild,
+                                   const double *box_centers, int nboxes, const double *sx,
+                                   const double *sy, const double *sz, const double *w0,
+                                   const double *wx, const double *wy, const double *wz,
+                                   const int *box_levels, int nlevels,
+                                   const double *level_rscale, double *tgt_expansions,
+                                   void *stream) {
+  SUMPY_REQUIRE(box_levels, "p2l_levels: box_levels needed");
+  LevelScales ls;
+  int rc = level_scales(nlevels, level_rscale, ls, "p2l_levels");
+  if (rc) return rc;
+  return p2l_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
+                  source_box_lists, box_source_starts, box_source_counts_nonchild, box_centers,
+                  nboxes, sx, sy, sz, w0, wx, wy, wz, level_rscale[0], tgt_expansions, 0,
+                  box_levels, ls, stream);
 }

@@ -439,19 +439,23 @@ __global__ void m2l_preprocess_bf16_kernel(int nboxes, int nstored, const double
 
 // Same block/item structure as m2l_apply_kernel.  Inputs come pre-split:
 // class data as packed (hi | lo) words, multipole rows as kpad hi then kpad lo
-// bf16, so staging is a plain copy.  The next two slots are fetched into two
-// register sets while a slot is contracted (each fetch has two slots of MFMA
-// work to land); per slot the class words and the tile's 64 hi / lo columns
-// (k contiguous) are staged in LDS, and every 32-deep k step of a wave
-// gathers 8 class words per A row, regroups their halves with byte permutes
-// and issues 3 bf16 MFMAs per (row block, column block).  Operand layout of
-// v_mfma_f32_16x16x32_bf16: A row = lane % 16, B column = lane % 16, the 8 k
-// of a lane are 8 (lane / 16) + e in both; accumulator register r = row
-// 4 (lane / 16) + r.
+// bf16, so staging is a plain copy.  The next slot's class words and columns
+// are fetched into registers while a slot is contracted; per slot they are
+// staged in LDS (the tile's 64 hi / lo columns k contiguous).  Every 32-deep
+// k step of a wave gathers 8 class words per A row, regroups their halves
+// with byte permutes and issues 3 bf16 MFMAs per (row block, column block).
+// Within a wave the LDS reads run ahead of the MFMAs that need them (LDS
+// returns in order): the B fragments of the first column block, then the A
+// words of the next k step, then per column block the B fragments of the
+// next one; the kernel is held to 3 waves per SIMD (its register budget).
+// Operand layout of v_mfma_f32_16x16x32_bf16: A row = lane % 16, B column =
+// lane % 16, the 8 k of a lane are 8 (lane / 16) + e in both; accumulator
+// register r = row 4 (lane / 16) + r.
 constexpr int BF16_MAXCIRC = 2560;  // packed class words per slot (LDS)
 
 template <int RBW>
-__global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void m2l_apply_bf16_kernel(ApplyArgs a) {
   constexpr int DPT = (BF16_MAXCIRC + 255)/256;
   extern __shared__ unsigned char lds_raw[];
   unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
@@ -480,8 +484,6 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
     const int row = (wave + 4*q)*16 + (lane & 15);
     offT[q] = (row < a.nl) ? a.off_t[row] : 0;
   }
-  // one slot's staging data: DPT class words and, for column `lane`, the hi
-  // and lo 8-chunks m = wave + 4 i (i < 4)
   // one slot's multipole columns: for column `lane`, the hi and lo 8-chunks
   // m = wave + 4 i (i < 4)
   struct XRegs {
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
   const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
   const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
-  // source box of this lane's column in a slot (loaded two fetches ahead)
+  // source box of this lane's column in a slot (loaded one fetch ahead)
   auto meta = [&](int slot) { return L.slot_sources[(size_t)min(slot, s1 - 1)*TW + lane]; };
   auto fetch_d = [&](int slot, unsigned (&d)[DPT]) {
     const unsigned *D = cwords + (size_t)L.slot_class[min(slot, s1 - 1)]*a.ncirc;
@@ -531,82 +533,83 @@ __global__ __launch_bounds__(256) void m2l_apply_bf16_kernel(ApplyArgs a) {
   for (int q = 0; q < RBW; ++q)
     for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
   const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
-  auto contract = [&]() {
-    // (the run-time bound keeps the compiler from hoisting later steps'
-    // fragments, which would cost a wave per SIMD)
+  const int nks = a.kpad/KB;
+  auto gather = [&](int k0, unsigned (&w)[RBW][8]) {
+    int os[8];
     #pragma unroll
-    for (int k0 = 0; k0 < 128; k0 += KB) {
-      if (k0 >= a.kpad) break;
-      int os[8];
+    for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
+    #pragma unroll
+    for (int q = 0; q < RBW; ++q)
       #pragma unroll
-      for (int e = 0; e < 8; ++e) os[e] = sOff[k0 + kg + e];
-      bf16x8 ah[RBW], al[RBW];
+      for (int e = 0; e < 8; ++e) w[q][e] = sD[offT[q] + os[e]];
+  };
+  auto to_frag = [&](const unsigned (&w)[RBW][8], bf16x8 (&ah)[RBW], bf16x8 (&al)[RBW]) {
+    #pragma unroll
+    for (int q = 0; q < RBW; ++q) {
+      u4 hw, lw;
       #pragma unroll
-      for (int q = 0; q < RBW; ++q) {
-        unsigned w[8];
-        #pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = sD[offT[q] + os[e]];
-        u4 hw, lw;
-        #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          hw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x05040100u);
-          lw[e] = __builtin_amdgcn_perm(w[2*e + 1], w[2*e], 0x07060302u);
-        }
-        ah[q] = __builtin_bit_cast(bf16x8, hw);
-        al[q] = __builtin_bit_cast(bf16x8, lw);
+      for (int e = 0; e < 4; ++e) {
+        hw[e] = __builtin_amdgcn_perm(w[q][2*e + 1], w[q][2*e], 0x05040100u);
+        lw[e] = __builtin_amdgcn_perm(w[q][2*e + 1], w[q][2*e], 0x07060302u);
       }
+      ah[q] = __builtin_bit_cast(bf16x8, hw);
+      al[q] = __builtin_bit_cast(bf16x8, lw);
+    }
+  };
+  auto contract = [&]() {
+    unsigned wn[RBW][8];
+    bf16x8 ah[RBW], al[RBW];
+    gather(0, wn);
+    to_frag(wn, ah, al);
+    #pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      if (ks >= nks) break;
+      const int k0 = ks*KB;
+      // B of column block 0, then the A words of the next k step, then per
+      // column block: B of the next block in flight while its MFMAs run
+      bf16x8 bh[2], bl[2];
+      bh[0] = *reinterpret_cast<const bf16x8 *>(sXh + (lane & 15)*XB + k0 + kg);
+      bl[0] = *reinterpret_cast<const bf16x8 *>(sXl + (lane & 15)*XB + k0 + kg);
+      if (ks + 1 < nks) gather(k0 + KB, wn);
       #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        const int col = cb*16 + (lane & 15);
-        const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
+        if (cb + 1 < 4) {
+          const int col = (cb + 1)*16 + (lane & 15);
+          bh[(cb + 1) & 1] = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
+          bl[(cb + 1) & 1] = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
+        }
         #pragma unroll
         for (int q = 0; q < RBW; ++q) {
-          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh, acc[q][cb], 0, 0, 0);
-          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl, acc[q][cb], 0, 0, 0);
-          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh, acc[q][cb], 0, 0, 0);
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[q], bh[cb & 1], acc[q][cb], 0, 0, 0);
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bl[cb & 1], acc[q][cb], 0, 0, 0);
+          acc[q][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[q], bh[cb & 1], acc[q][cb], 0, 0, 0);
         }
       }
+      if (ks + 1 < nks) to_frag(wn, ah, al);
     }
   };
   if (s0 < s1) {
-    // the multipole columns (from HBM) are fetched two slots ahead, the class
-    // words (few, shared by the tiles, from L2) one slot ahead; each slot's
-    // class words are issued before the columns of the slot after next, so
-    // that waiting for them leaves those columns in flight
+    // the class words and columns of the next slot are fetched (clamped, so
+    // fetches past the range reload the last slot) while this one runs
     unsigned dn[DPT];
-    XRegs ga, gb;
+    XRegs ga;
     fetch_d(s0, dn);
     fetch_x(meta(s0), ga);
     stage(dn, ga);
     __syncthreads();
-    const int m1 = meta(s0 + 1), m2 = meta(s0 + 2);
+    const int m1 = meta(s0 + 1);
     fetch_d(s0 + 1, dn);
     fetch_x(m1, ga);
-    int m3 = meta(s0 + 3);
-    fetch_x(m2, gb);
-    int m4 = meta(s0 + 4);
-    // at the top of every pair of slots: ga holds the columns of slot + 1, gb
-    // those of slot + 2, dn the class words of slot + 1, and m3 / m4 the
-    // source boxes of slots slot + 3 / slot + 4 (fetches past the range
-    // reload the last slot and are not used)
-    for (int slot = s0; slot < s1; slot += 2) {
+    int m2 = meta(s0 + 2);
+    for (int slot = s0; slot < s1; ++slot) {
       contract();
       __syncthreads();
       if (slot + 1 >= s1) break;
       stage(dn, ga);
       __syncthreads();
       fetch_d(slot + 2, dn);
-      fetch_x(m3, ga);
-      m3 = meta(slot + 5);
-      contract();
-      __syncthreads();
-      if (slot + 2 >= s1) break;
-      stage(dn, gb);
-      __syncthreads();
-      fetch_d(slot + 3, dn);
-      fetch_x(m4, gb);
-      m4 = meta(slot + 6);
+      fetch_x(m2, ga);
+      m2 = meta(slot + 3);
     }
   }
   const int nrb = a.npad >> 4;

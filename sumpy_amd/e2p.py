@@ -99,7 +99,13 @@ class E2PFromSingleBox(E2PBase):
 
 
 class E2PFromCSR(E2PBase):
-    """M2P: ``result[t] += scale * sum_src sum_mi M[mi] rho^|mi| d^mi G(t - c)``."""
+    """M2P: ``result[t] += scale * sum_src sum_mi M[mi] rho^|mi| d^mi G(t - c)``.
+
+    Besides the reference's keywords (one source level, ``rscale``,
+    ``src_base_ibox``) it takes ``box_levels`` (tree box levels) and
+    ``level_rscales`` (rscale per level) instead: the source boxes may then
+    be of any level (one launch for all of list 3), each with the rscale of
+    its level, and ``src_expansions`` holds one row per tree box."""
 
     default_name = "e2p_from_csr"
 
@@ -125,14 +131,27 @@ class E2PFromCSR(E2PBase):
                        *ptrs, actx.stream_handle)
             outs.finish(actx, targets, accumulate=True)
             return result
-        _capi.call("sumpyhip_m2p", kernel_id(self.expansion.kernel), tables.desc,
-                   int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
-                   _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
-                   int(centers.shape[1]), *[t.data_ptr() for t in targets],
-                   float(kwargs["rscale"]),
-                   self.expansion.kernel.get_global_scaling_const(),
-                   _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
-                   *ptrs, actx.stream_handle)
+        if "box_levels" in kwargs:
+            # source boxes of several levels in one launch: each box's rscale
+            # from its level, rows of the whole multipole array by box number
+            levels = as_i32(kwargs["box_levels"])
+            rs = _capi.f64_array(kwargs["level_rscales"])
+            _capi.call("sumpyhip_m2p_levels", kernel_id(self.expansion.kernel), tables.desc,
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
+                       int(centers.shape[1]), *[t.data_ptr() for t in targets],
+                       _capi.ptr(levels), len(kwargs["level_rscales"]), rs,
+                       self.expansion.kernel.get_global_scaling_const(),
+                       _capi.ptr(kwargs["src_expansions"]), *ptrs, actx.stream_handle)
+        else:
+            _capi.call("sumpyhip_m2p", kernel_id(self.expansion.kernel), tables.desc,
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
+                       int(centers.shape[1]), *[t.data_ptr() for t in targets],
+                       float(kwargs["rscale"]),
+                       self.expansion.kernel.get_global_scaling_const(),
+                       _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
+                       *ptrs, actx.stream_handle)
         outs.finish(actx, targets, accumulate=True)
         return result
 

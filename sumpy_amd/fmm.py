@@ -195,6 +195,9 @@ class SumpyExpansionWrangler:
                     actx, traversal, self.tree, is_translation_per_level=True)
             self.supports_translation_classes = True
         self.translation_classes_data = translation_classes_data
+        #: lists 3 and 4 of all levels in one launch each when the levels
+        #: share the expansion order; False: the reference's per-level loop
+        self.all_levels_lists = True
         self._memo = {}
 
     @property
@@ -571,6 +574,19 @@ class SumpyExpansionWrangler:
         pot = self.output_zeros(actx)
         kwargs = dict(self.kernel_extra_kwargs)
         kwargs.update(self.box_target_list_kwargs())
+        merged = self._list3_all_levels(target_boxes_by_source_level)
+        if merged is not None:
+            # one launch for every source level (same order on all levels)
+            tb, st, li = merged
+            m2p = self.tree_indep.m2p(self.level_orders[0])
+            res = m2p(actx, src_expansions=self._one_row_per_box(mpole_exps),
+                      target_boxes=tb, source_box_starts=st, source_box_lists=li,
+                      centers=self.tree.box_centers, result=pot,
+                      box_levels=self.tree.box_levels, level_rscales=self._level_rscales(),
+                      **kwargs)
+            for a, b in zip(pot, res):
+                assert a is b
+            return pot
         for lev, ssn in enumerate(source_boxes_by_level):
             if len(target_boxes_by_source_level[lev]) == 0:
                 continue
@@ -600,6 +616,17 @@ class SumpyExpansionWrangler:
         if self.last_form_locals_was_empty:
             return self._cached_zeros("zero_locals", lambda: self.local_expansion_zeros(actx))
         local_exps = self.local_expansion_zeros(actx)
+        merged = self._list4_all_levels(target_or_target_parent_boxes, starts, host_starts)
+        if merged is not None:
+            # one launch for every level's target boxes (same order on all levels)
+            tb, st = merged
+            p2l = self.tree_indep.p2l(self.level_orders[0])
+            res = p2l(actx, target_boxes=tb, source_box_starts=st, source_box_lists=lists,
+                      centers=self.tree.box_centers, strengths=src_weight_vecs,
+                      tgt_expansions=self._one_row_per_box(local_exps), tgt_base_ibox=0,
+                      rscale=self.level_to_rscale(0), box_levels=self.tree.box_levels,
+                      level_rscales=self._level_rscales(), **kwargs)
+            return local_exps
         for lev in range(self.tree.nlevels):
             start, stop = lstarts[lev:lev + 2]
             if start == stop:
@@ -617,6 +644,75 @@ class SumpyExpansionWrangler:
                       rscale=self.level_to_rscale(lev), **kwargs)
             assert res is view
         return local_exps
+
+    # {{{ lists 3 and 4 over all levels in one launch
+
+    def _all_levels_ok(self):
+        """One launch may span levels when they share the expansion order (one
+        row per tree box in the expansion arrays) and the real Laplace kernels
+        serve the level (the list 3/4 kernels with per-box rscale)."""
+        orders = self.level_orders
+        base = self.tree_indep.get_base_kernel()
+        return (self.all_levels_lists and len(set(orders)) == 1
+                and not base.is_complex_valued and self.tree.dimensions == 3
+                and self.tree.nlevels <= 32
+                and len(self.tree_indep.local_expansion(orders[0])) == (orders[0] + 1)**2)
+
+    def _level_rscales(self):
+        return [self.level_to_rscale(lev) for lev in range(self.tree.nlevels)]
+
+    def _one_row_per_box(self, flat):
+        return flat.reshape(self.tree.nboxes, -1)
+
+    def _list3_all_levels(self, target_boxes_by_source_level):
+        """(target boxes, starts, source boxes) of list 3 over all source
+        levels, a target box's sources level by level (each level's lists in
+        their order), or None when the per-level launches must be used."""
+        trav = self.traversal
+        if not self._all_levels_ok() or "sep_smaller" not in getattr(trav, "host", {}) or \
+                target_boxes_by_source_level is not getattr(
+                    trav, "target_boxes_sep_smaller_by_source_level", None):
+            return None
+        memo = self._memo.get("list3_all_levels")
+        if memo is None:
+            tg, sr = [], []
+            for tbs, st, li in trav.host["sep_smaller"]:
+                if len(tbs):
+                    tg.append(np.repeat(tbs.astype(np.int64), np.diff(st)))
+                    sr.append(li[st[0]:st[-1]])
+            t = np.concatenate(tg)
+            order = np.argsort(t, kind="stable")
+            t, src = t[order], np.concatenate(sr)[order]
+            boxes, counts = np.unique(t, return_counts=True)
+            starts = np.concatenate([[0], np.cumsum(counts)])
+            dev = self.tree_indep._setup_actx.from_numpy
+            memo = tuple(dev(np.ascontiguousarray(a, np.int32)) for a in (boxes, starts, src))
+            self._memo["list3_all_levels"] = memo
+        return memo
+
+    def _list4_all_levels(self, totp, starts, host_starts):
+        """(target boxes, starts) of the boxes with a non-empty list 4 over all
+        levels (their lists stay where they are in the traversal's list), or
+        None when the per-level launches must be used."""
+        trav = self.traversal
+        if not self._all_levels_ok() or host_starts is None or \
+                totp is not getattr(trav, "target_or_target_parent_boxes", None):
+            return None
+        memo = self._memo.get("list4_all_levels")
+        if memo is None:
+            totp_h = trav.host["target_or_target_parent_boxes"]
+            sel = np.nonzero(np.diff(host_starts) > 0)[0]
+            # boxes between two selected ones have empty lists, so the
+            # selected lists are contiguous: starts[sel] closed by the end of
+            # the last one
+            st = np.concatenate([host_starts[sel], host_starts[sel[-1] + 1:sel[-1] + 2]])
+            dev = self.tree_indep._setup_actx.from_numpy
+            memo = (dev(np.ascontiguousarray(totp_h[sel], np.int32)),
+                    dev(np.ascontiguousarray(st, np.int32)))
+            self._memo["list4_all_levels"] = memo
+        return memo
+
+    # }}}
 
     def refine_locals(self, actx, level_start_target_or_target_parent_box_nrs,
                       target_or_target_parent_boxes, local_exps):

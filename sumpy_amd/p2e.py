@@ -92,7 +92,13 @@ class P2EFromSingleBox(P2EBase):
 
 
 class P2EFromCSR(P2EBase):
-    """P2L from list 4 (reference ``p2e.py:268-388``)."""
+    """P2L from list 4 (reference ``p2e.py:268-388``).
+
+    Besides the reference's keywords it takes ``box_levels`` and
+    ``level_rscales`` (rscale per level): the target boxes may then be of any
+    level (one launch for all of list 4), each with the rscale of its level,
+    and ``tgt_expansions`` holds one row per tree box (``rscale`` is then
+    only used for the expansion tables)."""
 
     default_name = "p2e_from_csr"
 
@@ -112,6 +118,18 @@ class P2EFromCSR(P2EBase):
                        int(centers.shape[1]), *[s.data_ptr() for s in sources],
                        *weights.ptrs(3), rscale, _capi.ptr(tgt),
                        int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+            return tgt
+        if "box_levels" in kwargs:
+            # target boxes of several levels in one launch (each with the
+            # rscale of its level; rows of the whole local array by box)
+            levels = as_i32(kwargs.pop("box_levels"))
+            lrs = kwargs.pop("level_rscales")
+            _capi.call("sumpyhip_p2l_levels", kernel_id(self.expansion.kernel), tables.desc,
+                       int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
+                       _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
+                       int(centers.shape[1]), *[s.data_ptr() for s in sources],
+                       *weights.ptrs(3), _capi.ptr(levels), len(lrs), _capi.f64_array(lrs),
+                       _capi.ptr(tgt), actx.stream_handle)
             return tgt
         _capi.call("sumpyhip_p2l", kernel_id(self.expansion.kernel), tables.desc,
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),

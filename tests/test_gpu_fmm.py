@@ -21,7 +21,7 @@ def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
             m2l_precision="fp64", p2p_precision="fp64", targets=None,
             use_classes=True, target_kernels=None, source_kernels=None,
             source_extra_kwargs=None, strength_usage=None, tree_trav=None,
-            return_wrangler=False):
+            return_wrangler=False, all_levels=True):
     from sumpy_amd.drive import drive_fmm
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
     from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
@@ -53,6 +53,7 @@ def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
                                 self_extra_kwargs=self_kw,
                                 source_extra_kwargs=source_extra_kwargs,
                                 _disable_translation_classes=not use_classes)
+    wr.all_levels_lists = all_levels
     ws = weights if isinstance(weights, (list, tuple)) else [weights]
     pots = drive_fmm(actx, wr, [actx.from_numpy(w) for w in ws])
     if return_wrangler:
@@ -76,15 +77,18 @@ def test_fmm_matches_oracle_fmm(actx, uniform_case, use_classes):
     assert rel(pot, ref) < 1e-12
 
 
-def test_fmm_clustered_matches_oracle_fmm(actx):
-    """Uneven tree (lists 3 and 4 populated): 8 Gaussian clusters."""
+@pytest.mark.parametrize("all_levels", [True, False])
+def test_fmm_clustered_matches_oracle_fmm(actx, all_levels):
+    """Uneven tree (lists 3 and 4 populated): 8 Gaussian clusters; lists 3
+    and 4 in one launch over all levels, and level by level."""
     rng = np.random.default_rng(1)
     n = 6000
     c = rng.uniform(0.1, 0.9, (8, 3))
     src = np.clip(c[rng.integers(0, 8, n)].T + 0.02*rng.standard_normal((3, n)), 0, 1)
     q = rng.standard_normal(n)
     order = 5
-    (pot,), tree, trav = run_fmm(actx, src, q, order, max_particles=20)
+    (pot,), tree, trav = run_fmm(actx, src, q, order, max_particles=20,
+                                 all_levels=all_levels)
     assert sum(len(x[2]) for x in trav.host["sep_smaller"]) > 0
     assert len(trav.host["from_sep_bigger_lists"]) > 0
     ref = oracle_fmm(tree, trav, q, order, exclude_self=True)
@@ -287,3 +291,31 @@ def test_tree_indep_reused_across_trees(actx, m2l_precision):
         ref = run(tree_indep(), src, q)
         gc.collect()
         assert rel(got, ref) < 1e-13, (seed, rel(got, ref))
+
+
+def test_fmm_all_level_lists_equal_per_level_with_gradients_and_dipoles(actx):
+    """Lists 3 and 4 in one launch over all levels (per-box rscale) equal the
+    per-level launches to rounding, for potential + gradient targets and
+    charge + dipole sources on a clustered tree."""
+    from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative, LaplaceKernel
+    knl = LaplaceKernel(3)
+    rng = np.random.default_rng(31)
+    n = 5000
+    c = rng.uniform(0.2, 0.8, (6, 3))
+    src = np.clip(c[rng.integers(0, 6, n)].T + 0.03*rng.standard_normal((3, n)), 0, 1)
+    w = [rng.standard_normal(n), rng.standard_normal(n)]
+    dirv = rng.standard_normal((3, n))
+    tk = [knl] + [AxisTargetDerivative(a, knl) for a in range(3)]
+    (p0, *_), tree, trav = run_fmm(actx, src, w[0], 7, max_particles=20, target_kernels=tk)
+    assert sum(len(x[2]) for x in trav.host["sep_smaller"]) > 0
+    assert len(trav.host["from_sep_bigger_lists"]) > 0
+    dk = DirectionalSourceDerivative(knl, "dir_vec")
+    dkw = {"dir_vec": actx.from_numpy(dirv[:, tree.host["user_source_ids"]])}
+    outs = []
+    for all_levels in (True, False):
+        pots, _, _ = run_fmm(actx, src, w, 7, target_kernels=tk, source_kernels=[knl, dk],
+                             strength_usage=[0, 1], source_extra_kwargs=dkw,
+                             tree_trav=(tree, trav), all_levels=all_levels)
+        outs.append(pots)
+    for a, b in zip(*outs):
+        assert rel(a, b) < 1e-13
