@@ -11,8 +11,8 @@ reorderings) with the tree, traversal and translation-class data built
 beforehand.
 
 Three forms are timed: the plain FP64 form and two mixed-precision forms
-(FP32 pair arithmetic with FP64 accumulation in P2P; FP32 or split-BF16 MFMA
-in M2L).  ``value`` is the wall time per pass of the fastest mixed form whose
+(FP32 pair arithmetic with FP64 accumulation in P2P and in the list-3/4 terms
+(M2P, P2L); FP32 or split-BF16 MFMA in M2L).  ``value`` is the wall time per pass of the fastest mixed form whose
 error is within 2x of the FP64 form's (``matched_error``); if none is, the
 FP64 form's time is reported and ``matched_error`` is false.  Errors are
 relative L2 errors against the FP64 direct sum on 4096 sampled targets.  The
@@ -107,7 +107,8 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None)
         actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
         partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
                 m2l_translation_override=tr),
-        [knl], exclude_self=True, p2p_precision=p2p_precision, autotune=True)
+        [knl], exclude_self=True, p2p_precision=p2p_precision, autotune=True,
+        list34_precision=p2p_precision if k is None else "fp64")
     import torch
     t2s = torch.arange(tree.ntargets, dtype=torch.int32, device=actx.device)
     return SumpyExpansionWrangler(ti, trav, np.float64 if k is None else np.complex128,
@@ -700,10 +701,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64",
-                  "mixed_fp32": "f32/f64 mixed (FP32 MFMA M2L, FP32 P2P with FP64 "
-                                "accumulation)",
+                  "mixed_fp32": "f32/f64 mixed (FP32 MFMA M2L, FP32 P2P and list-3/4 terms "
+                                "with FP64 accumulation)",
                   "mixed_bf16x3": "bf16x3/f32/f64 mixed (split-BF16 MFMA M2L with FP32 sums, "
-                                  "FP32 P2P with FP64 accumulation)"}[best],
+                                  "FP32 P2P and list-3/4 terms with FP64 accumulation)"}[best],
         "data": "synthetic",
         "config": {"workload": f"laplace3d_fmm_{args.distribution}", "n": args.n,
                    "order": args.order, "max_particles_in_box": args.max_particles,

@@ -199,7 +199,9 @@ int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
  * when every level has the same expansion order: source box sb has the
  * rscale level_rscale[box_levels[sb]] (nlevels <= 32) and its multipole at
  * row sb of src_expansions (one row per tree box).  The source lists of a
- * target box are summed in list order; accumulates like sumpyhip_m2p. */
+ * target box are summed in list order; accumulates like sumpyhip_m2p.
+ * precision 0: FP64; 1: each (target, source box) term in FP32 (recurrence,
+ * multipole in FP32), the sums over source boxes in FP64. */
 int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                         const int *target_boxes, const int *source_box_starts,
                         const int *source_box_lists, const int *box_target_starts,
@@ -207,14 +209,16 @@ int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes
                         int nboxes, const double *tx, const double *ty, const double *tz,
                         const int *box_levels, int nlevels, const double *level_rscale,
                         double scale, const double *src_expansions, double *pot, double *gx,
-                        double *gy, double *gz, void *stream);
+                        double *gy, double *gz, int precision, void *stream);
 
 /* P2L from list 4 for target boxes of several tree levels in one launch,
  * replaces the per-level loop of SumpyExpansionWrangler.form_locals
  * (sumpy/fmm.py:963-1002) over P2EFromCSR.__call__ (sumpy/p2e.py:268-388)
  * when every level has the same expansion order: target box tb has the
  * rscale level_rscale[box_levels[tb]] and its local at row tb of
- * tgt_expansions (one row per tree box); overwrites like sumpyhip_p2l. */
+ * tgt_expansions (one row per tree box); overwrites like sumpyhip_p2l.
+ * precision 0: FP64; 1: the derivatives and the sums over 64 sources in
+ * FP32, the sums over those batches in FP64. */
 int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                         const int *target_boxes, const int *source_box_starts,
                         const int *source_box_lists, const int *box_source_starts,
@@ -222,7 +226,8 @@ int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes
                         int nboxes, const double *sx, const double *sy, const double *sz,
                         const double *w0, const double *wx, const double *wy,
                         const double *wz, const int *box_levels, int nlevels,
-                        const double *level_rscale, double *tgt_expansions, void *stream);
+                        const double *level_rscale, double *tgt_expansions, int precision,
+                        void *stream);
 
 /* }}} */
 

@@ -71,9 +71,9 @@ _SIGNATURES = {
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
     "sumpyhip_m2p_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P,
-                            _D, _P, _P, _P, _P, _P, _P],
+                            _D, _P, _P, _P, _P, _P, _I, _P],
     "sumpyhip_p2l_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P,
-                            _P, _P, _I, _P, _P, _P],
+                            _P, _P, _I, _P, _P, _I, _P],
     "sumpyhip_m2p_helmholtz": [_P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D,
                                _D, _P, _I, _P, _P, _P, _P, _P],
     "sumpyhip_p2l_helmholtz": [_P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,

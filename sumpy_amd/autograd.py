@@ -289,7 +289,8 @@ def adjoint_wrangler(wrangler, target_kernels=None):
             ati = SumpyTreeIndependentDataForWrangler(
                 ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory,
                 list(tk), exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
-                p2p_precision=ti.p2p_precision, autotune=ti.autotune)
+                p2p_precision=ti.p2p_precision, autotune=ti.autotune,
+                list34_precision=ti.list34_precision)
             ti._memo[("adjoint_tree_indep", tk)] = ati
     trav = swapped_traversal(ti._setup_actx, wrangler.traversal)
     orders = list(wrangler.level_orders)
@@ -323,7 +324,8 @@ def _gradient_wrangler(wrangler):
     gti = SumpyTreeIndependentDataForWrangler(
         ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory, tk,
         exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
-        p2p_precision=ti.p2p_precision, autotune=ti.autotune)
+        p2p_precision=ti.p2p_precision, autotune=ti.autotune,
+        list34_precision=ti.list34_precision)
     orders = list(wrangler.level_orders)
     got = SumpyExpansionWrangler(
         gti, wrangler.traversal, wrangler.dtype,

@@ -589,29 +589,32 @@ __device__ __forceinline__ int shell_pos(int i, int j, int k) {
   return n*n + sidx(n, i, j);
 }
 
+// T: the arithmetic type of the recurrence (double, or float for the mixed
+// form's list-3/4 terms; the offset x is formed in FP64 by the caller)
+template <typename T = double>
 struct LapGeo {
-  double xs[3];     // x / rho
-  double ri2, rinv; // 1 / |x / rho|^2, 1 / |x|
+  T xs[3];     // x / rho
+  T ri2, rinv; // 1 / |x / rho|^2, 1 / |x|
   __device__ LapGeo(double x, double y, double z, double rscale) {
     const double inv = 1.0/rscale;
-    xs[0] = x*inv; xs[1] = y*inv; xs[2] = z*inv;
-    const double r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
-    const double ri = rsqrt(r2s);
+    xs[0] = T(x*inv); xs[1] = T(y*inv); xs[2] = T(z*inv);
+    const T r2s = xs[0]*xs[0] + xs[1]*xs[1] + xs[2]*xs[2];
+    const T ri = rsqrt(r2s);
     ri2 = ri*ri;
-    rinv = ri*inv;
+    rinv = ri*T(inv);
   }
 };
 
 // shell N from shells N - 1 (p1) and N - 2 (p2)
-template <int N>
-__device__ __forceinline__ void lap3d_shell(const LapGeo &g, const double *p1, const double *p2,
-                                            double *cur) {
+template <int N, typename T>
+__device__ __forceinline__ void lap3d_shell(const LapGeo<T> &g, const T *p1, const T *p2,
+                                            T *cur) {
   #pragma unroll
   for (int mx = 0; mx <= 1; ++mx) {
     #pragma unroll
     for (int my = 0; my <= N - mx; ++my) {
       const int mz = N - mx - my;
-      double val;
+      T val;
       if (my <= 1 && mz <= 1) {
         // m in {0,1}^3: closed forms
         if (N == 0) {
@@ -620,24 +623,24 @@ __device__ __forceinline__ void lap3d_shell(const LapGeo &g, const double *p1, c
           val = -g.xs[mx ? 0 : (my ? 1 : 2)]*g.ri2*g.rinv;
         } else if (N == 2) {
           const int a = mx ? 0 : 1, b = mz ? 2 : 1;
-          val = 3.0*g.xs[a]*g.xs[b]*(g.ri2*g.ri2)*g.rinv;
+          val = T(3)*g.xs[a]*g.xs[b]*(g.ri2*g.ri2)*g.rinv;
         } else {
-          val = -15.0*g.xs[0]*g.xs[1]*g.xs[2]*(g.ri2*g.ri2*g.ri2)*g.rinv;
+          val = T(-15)*g.xs[0]*g.xs[1]*g.xs[2]*(g.ri2*g.ri2*g.ri2)*g.rinv;
         }
       } else {
         // d: the first axis with m_d >= 2 (m_x <= 1, so y or z)
         const int d = my >= 2 ? 1 : 2;
-        double acc = 0.0;
-        if (mx == 1) acc += 2.0*g.xs[0]*p1[sidx(N - 1, 0, my)];
+        T acc = 0;
+        if (mx == 1) acc += T(2)*g.xs[0]*p1[sidx(N - 1, 0, my)];
         if (my >= 1) {
-          const double c1 = d == 1 ? 2*my - 1 : 2*my;
-          const double c2 = d == 1 ? (my - 1)*(my - 1) : my*(my - 1);
+          const T c1 = d == 1 ? 2*my - 1 : 2*my;
+          const T c2 = d == 1 ? (my - 1)*(my - 1) : my*(my - 1);
           acc += c1*g.xs[1]*p1[sidx(N - 1, mx, my - 1)];
           if (my >= 2) acc += c2*p2[sidx(N - 2, mx, my - 2)];
         }
         if (mz >= 1) {
-          const double c1 = d == 2 ? 2*mz - 1 : 2*mz;
-          const double c2 = d == 2 ? (mz - 1)*(mz - 1) : mz*(mz - 1);
+          const T c1 = d == 2 ? 2*mz - 1 : 2*mz;
+          const T c2 = d == 2 ? (mz - 1)*(mz - 1) : mz*(mz - 1);
           acc += c1*g.xs[2]*p1[sidx(N - 1, mx, my)];
           if (mz >= 2) acc += c2*p2[sidx(N - 2, mx, my)];
         }
@@ -652,10 +655,10 @@ __device__ __forceinline__ void lap3d_shell(const LapGeo &g, const double *p1, c
 // shell N - 1).
 template <int N, int Q, bool DONE = (N > Q)>
 struct LapShells {
-  template <class F>
-  static __device__ __forceinline__ void run(const LapGeo &g, const double *p2,
-                                             const double *p1, F &f) {
-    double cur[2*N + 1];
+  template <typename T, class F>
+  static __device__ __forceinline__ void run(const LapGeo<T> &g, const T *p2, const T *p1,
+                                             F &f) {
+    T cur[2*N + 1];
     lap3d_shell<N>(g, p1, p2, cur);
     f.template shell<N>(cur, p1);
     LapShells<N + 1, Q>::run(g, p1, cur, f);
@@ -664,21 +667,20 @@ struct LapShells {
 
 template <int N, int Q>
 struct LapShells<N, Q, true> {
-  template <class F>
-  static __device__ __forceinline__ void run(const LapGeo &, const double *, const double *,
-                                             F &) {}
+  template <typename T, class F>
+  static __device__ __forceinline__ void run(const LapGeo<T> &, const T *, const T *, F &) {}
 };
 
-template <int Q, class F>
-__device__ __forceinline__ void lap3d_shells(const LapGeo &g, F &f) {
-  const double none[1] = {0.0};
+template <int Q, typename T, class F>
+__device__ __forceinline__ void lap3d_shells(const LapGeo<T> &g, F &f) {
+  const T none[1] = {T(0)};
   LapShells<0, Q>::run(g, none, none, f);
 }
 
 // D(m + e_x) for m = (mx, my, .) of shell N - 1, from shell N (m_x = 2 by
 // Laplace's equation: D(2, y, z) = -D(0, y + 2, z) - D(0, y, z + 2))
-template <int N>
-__device__ __forceinline__ double shift_x(const double *cur, int mx, int my) {
+template <int N, typename T>
+__device__ __forceinline__ T shift_x(const T *cur, int mx, int my) {
   return mx == 0 ? cur[sidx(N, 1, my)] : -cur[sidx(N, 0, my + 2)] - cur[sidx(N, 0, my)];
 }
 
@@ -697,13 +699,20 @@ struct LevelScales {
 
 // M2P consumer: P += sum_m M(m) D(m) (shells <= ORD); with GRAD also
 // G_ax += sum_m M(m) D(m + e_ax) from the next shell.
-template <int ORD, bool GRAD>
+template <int ORD, bool GRAD, typename T = double>
 struct M2PSum {
-  const double *M;  // this lane's source box, shell order (LDS)
-  double gs = 1;    // 1 / rscale of that box (the gradient's extra derivative)
-  double P = 0, G[3] = {0, 0, 0};
+  const T *M;       // this lane's source box, shell order (LDS)
+  T gs = 1;         // 1 / rscale of that box (the gradient's extra derivative)
+  T P = 0, G[3] = {0, 0, 0};                // this box's sums (type T)
+  double Pt = 0, Gt[3] = {0, 0, 0};         // over the boxes (FP64)
+  __device__ __forceinline__ void flush() {
+    Pt += double(P);
+    P = 0;
+    if (GRAD)
+      for (int c = 0; c < 3; ++c) { Gt[c] += double(G[c]); G[c] = 0; }
+  }
   template <int N>
-  __device__ __forceinline__ void shell(const double *cur, const double *prev) {
+  __device__ __forceinline__ void shell(const T *cur, const T *prev) {
     if (N <= ORD) {
       #pragma unroll
       for (int i = 0; i < 2*N + 1; ++i) P += M[N*N + i]*cur[i];
@@ -714,7 +723,7 @@ struct M2PSum {
       for (int mx = 0; mx <= 1; ++mx) {
         #pragma unroll
         for (int my = 0; my <= n - mx; ++my) {
-          const double m = gs*M[n*n + sidx(n, mx, my)];
+          const T m = gs*M[n*n + sidx(n, mx, my)];
           G[0] += m*shift_x<N>(cur, mx, my);
           G[1] += m*cur[sidx(N, mx, my + 1)];
           G[2] += m*cur[sidx(N, mx, my)];
@@ -729,7 +738,7 @@ struct M2PSum {
 // its own source box.  Every wave stages the multipoles of its current source
 // boxes in LDS in shell order; the partial sums of waves and groups are added
 // in LDS at the end.
-template <int ORD, bool GRAD>
+template <int ORD, bool GRAD, typename T = double>
 __global__ __launch_bounds__(64*LW) void m2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
     const int *__restrict__ sstarts, const int *__restrict__ slists,
@@ -741,7 +750,7 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
   constexpr int NS = (ORD + 1)*(ORD + 1);
   constexpr int NC = GRAD ? 4 : 1;
   __shared__ int sPos[NS];
-  __shared__ double sM[LW][4][NS];
+  __shared__ T sM[LW][4][NS];
   __shared__ double red[NC][LW][64];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -758,28 +767,29 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
     for (int tc = 0; tc < nt; tc += G) {
       const int t = t0 + tc + (tc + tl < nt ? tl : 0);
       const double x = tx[t], y = ty[t], z = tz[t];
-      M2PSum<ORD, GRAD> f;
+      M2PSum<ORD, GRAD, T> f;
       f.M = sM[wave][grp];
       for (int base = is0 + wave*ngrp; base < is1; base += LW*ngrp) {
         wave_sync();
         for (int v = lane; v < ngrp*NS; v += 64) {
           const int g2 = v/NS, s = v - g2*NS;
           if (base + g2 < is1)
-            sM[wave][g2][sPos[s]] = src[(size_t)(slists[base + g2] - sbase)*NS + s];
+            sM[wave][g2][sPos[s]] = T(src[(size_t)(slists[base + g2] - sbase)*NS + s]);
         }
         wave_sync();
         if (base + grp < is1) {
           const int sb = slists[base + grp];
           const double rs = box_levels ? ls.r[box_levels[sb]] : rscale;
-          f.gs = 1.0/rs;
-          const LapGeo g(x - centers[sb], y - centers[nboxes + sb],
-                         z - centers[2*(size_t)nboxes + sb], rs);
+          f.gs = T(1.0/rs);
+          const LapGeo<T> g(x - centers[sb], y - centers[nboxes + sb],
+                            z - centers[2*(size_t)nboxes + sb], rs);
           lap3d_shells<ORD + (GRAD ? 1 : 0)>(g, f);
+          f.flush();
         }
       }
-      red[0][wave][lane] = f.P;
+      red[0][wave][lane] = f.Pt;
       if (GRAD)
-        for (int c = 0; c < 3; ++c) red[1 + c][wave][lane] = f.G[c];
+        for (int c = 0; c < 3; ++c) red[1 + c][wave][lane] = f.Gt[c];
       __syncthreads();
       if (tid < G && tc + tid < nt) {
         double v[NC];
@@ -809,27 +819,27 @@ __global__ __launch_bounds__(64*LW) void m2p_lap3d(
 // rows) and, every 16 coefficients, summed in quarters: lane l adds lanes
 // 16 (l % 4) .. 16 (l % 4) + 15 of coefficient 16 k + l / 4 to its register
 // acc[k].  The four quarters are added once per target box (finish()).
-template <int ORD, bool DIP>
+template <int ORD, bool DIP, typename T = double>
 struct P2LSum {
   static constexpr int NS = (ORD + 1)*(ORD + 1);
   static constexpr int NCH = (NS + 15)/16;
-  static constexpr int BS = 64 + 1;  // LDS row stride (doubles)
+  static constexpr int BS = 64 + 1;  // LDS row stride (values)
   static constexpr int RING = 32;    // rows: two chunks
-  double acc[NCH];
-  double q0, qd[3];
-  double *buf;  // [RING][BS]
+  double acc[NCH];                   // FP64 over the chunks of sources
+  T q0, qd[3];
+  T *buf;  // [RING][BS]
   int lane;
-  __device__ __forceinline__ void put(int p, double v) {
+  __device__ __forceinline__ void put(int p, T v) {
     buf[(p & (RING - 1))*BS + lane] = v;
     if ((p & 15) == 15 || p == NS - 1) {
       // the other chunk's rows were read before this chunk's first write
       // (previous flush), so one wave barrier per flush suffices
       wave_sync();
-      const double *r = buf + ((p & 16) + (lane >> 2))*BS + (lane & 3)*16;
-      double t = 0;
+      const T *r = buf + ((p & 16) + (lane >> 2))*BS + (lane & 3)*16;
+      T t = 0;
       #pragma unroll
       for (int q = 0; q < 16; ++q) t += r[q];
-      acc[p >> 4] += t;
+      acc[p >> 4] += double(t);
     }
   }
   __device__ __forceinline__ double finish(int k) const {
@@ -839,7 +849,7 @@ struct P2LSum {
     return t;
   }
   template <int N>
-  __device__ __forceinline__ void shell(const double *cur, const double *prev) {
+  __device__ __forceinline__ void shell(const T *cur, const T *prev) {
     if (!DIP && N <= ORD) {
       #pragma unroll
       for (int i = 0; i < 2*N + 1; ++i) put(N*N + i, q0*cur[i]);
@@ -867,7 +877,7 @@ struct P2LSum {
 // live derivative shells and (ORD + 1)^2 / 16 accumulators.
 constexpr int P2L_LW = 2;  // waves (target boxes) per block of the P2L kernel
 
-template <int ORD, bool DIP>
+template <int ORD, bool DIP, typename T = double>
 __global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
     sumpyhip_expn e, int ntgt, const int *__restrict__ target_boxes,
     const int *__restrict__ sstarts, const int *__restrict__ slists,
@@ -877,11 +887,11 @@ __global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
     const double *__restrict__ w0, const double *__restrict__ wx,
     const double *__restrict__ wy, const double *__restrict__ wz, double rscale_level,
     double *out, int base, const int *__restrict__ box_levels, LevelScales ls) {
-  using Sum = P2LSum<ORD, DIP>;
+  using Sum = P2LSum<ORD, DIP, T>;
   constexpr int NS = Sum::NS;
   __shared__ int sStored[NS];  // shell position -> stored index
   __shared__ int sOff[P2L_LW][64], sFirst[P2L_LW][64];
-  __shared__ double sBuf[P2L_LW][Sum::RING*Sum::BS];
+  __shared__ T sBuf[P2L_LW][Sum::RING*Sum::BS];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int s = tid; s < NS; s += 64*P2L_LW)
@@ -938,14 +948,14 @@ __global__ __launch_bounds__(64*P2L_LW) void p2l_lap3d(
       for (int j0 = 0; j0 < total; j0 += 64) {
         const bool live = j0 + lane < total;
         const double x = nx, y = ny, z = nz;
-        f.q0 = live ? nq : 0.0;
+        f.q0 = live ? T(nq) : T(0);
         if (DIP) {
-          f.qd[0] = live ? ndx/rscale : 0.0;
-          f.qd[1] = live ? ndy/rscale : 0.0;
-          f.qd[2] = live ? ndz/rscale : 0.0;
+          f.qd[0] = live ? T(ndx/rscale) : T(0);
+          f.qd[1] = live ? T(ndy/rscale) : T(0);
+          f.qd[2] = live ? T(ndz/rscale) : T(0);
         }
         load(source_at(j0 + 64 + lane));
-        const LapGeo g(c0 - x, c1 - y, c2 - z, rscale);
+        const LapGeo<T> g(c0 - x, c1 - y, c2 - z, rscale);
         lap3d_shells<ORD + (DIP ? 1 : 0)>(g, f);
       }
     }
@@ -1163,8 +1173,10 @@ static int m2p_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                     int nboxes, const double *tx, const double *ty, const double *tz,
                     double rscale, double scale, const double *src_expansions,
                     int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
-                    const int *box_levels, const LevelScales &ls, void *stream) {
+                    const int *box_levels, const LevelScales &ls, int precision,
+                    void *stream) {
   SUMPY_REQUIRE(kernel_id == 3, "m2p: only the Laplace 3D kernel is implemented");
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "m2p: precision must be 0 or 1");
   int rc = check_expn(expn, "m2p");
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
@@ -1174,18 +1186,18 @@ static int m2p_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                 "m2p: only PDE-compressed expansions are implemented");
   SUMPY_REQUIRE(!gx || (gy && gz), "m2p: incomplete gradient outputs");
   const int grid = ntgt_boxes < (1 << 30) ? ntgt_boxes : (1 << 30);
+#define SUMPY_M2P_T(ORD, GRAD, T)                                                            \
+  m2p_lap3d<ORD, GRAD, T><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                          \
+      *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
+      box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,        \
+      rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz, box_levels, ls)
 #define SUMPY_M2P(ORD)                                                                       \
   case ORD:                                                                                  \
-    if (gx)                                                                                  \
-      m2p_lap3d<ORD, true><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                         \
-          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
-          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
-          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz, box_levels, ls);    \
-    else                                                                                     \
-      m2p_lap3d<ORD, false><<<grid, 64*LW, 0, (hipStream_t)stream>>>(                        \
-          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
-          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
-          rscale, scale, src_expansions, src_base_ibox, pot, gx, gy, gz, box_levels, ls);    \
+    if (gx) {                                                                                \
+      if (precision) SUMPY_M2P_T(ORD, true, float); else SUMPY_M2P_T(ORD, true, double);    \
+    } else {                                                                                 \
+      if (precision) SUMPY_M2P_T(ORD, false, float); else SUMPY_M2P_T(ORD, false, double);  \
+    }                                                                                        \
     break;
   switch (expn->order) {
     SUMPY_M2P(0) SUMPY_M2P(1) SUMPY_M2P(2) SUMPY_M2P(3) SUMPY_M2P(4) SUMPY_M2P(5)
@@ -1193,6 +1205,7 @@ static int m2p_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
     SUMPY_M2P(12)
   }
 #undef SUMPY_M2P
+#undef SUMPY_M2P_T
   return check_launch("m2p");
 }
 
@@ -1203,8 +1216,10 @@ static int p2l_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                     int nboxes, const double *sx, const double *sy, const double *sz,
                     const double *w0, const double *wx, const double *wy, const double *wz,
                     double rscale, double *tgt_expansions, int tgt_base_ibox,
-                    const int *box_levels, const LevelScales &ls, void *stream) {
+                    const int *box_levels, const LevelScales &ls, int precision,
+                    void *stream) {
   SUMPY_REQUIRE(kernel_id == 3, "p2l: only the Laplace 3D kernel is implemented");
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "p2l: precision must be 0 or 1");
   int rc = check_expn(expn, "p2l");
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
@@ -1214,18 +1229,18 @@ static int p2l_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                 "p2l: only PDE-compressed expansions are implemented");
   SUMPY_REQUIRE(!wx || (wy && wz), "p2l: incomplete dipole weights");
   const int grid = (ntgt_boxes + P2L_LW - 1)/P2L_LW;
+#define SUMPY_P2L_T(ORD, DIP, T)                                                             \
+  p2l_lap3d<ORD, DIP, T><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                       \
+      *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
+      box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0,    \
+      wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox, box_levels, ls)
 #define SUMPY_P2L(ORD)                                                                       \
   case ORD:                                                                                  \
-    if (wx)                                                                                  \
-      p2l_lap3d<ORD, true><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                         \
-          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
-          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
-          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox, box_levels, ls);            \
-    else                                                                                     \
-      p2l_lap3d<ORD, false><<<grid, 64*P2L_LW, 0, (hipStream_t)stream>>>(                        \
-          *expn, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,              \
-          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
-          w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox, box_levels, ls);            \
+    if (wx) {                                                                                \
+      if (precision) SUMPY_P2L_T(ORD, true, float); else SUMPY_P2L_T(ORD, true, double);    \
+    } else {                                                                                 \
+      if (precision) SUMPY_P2L_T(ORD, false, float); else SUMPY_P2L_T(ORD, false, double);  \
+    }                                                                                        \
     break;
   switch (expn->order) {
     SUMPY_P2L(0) SUMPY_P2L(1) SUMPY_P2L(2) SUMPY_P2L(3) SUMPY_P2L(4) SUMPY_P2L(5)
@@ -1233,6 +1248,7 @@ static int p2l_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
     SUMPY_P2L(12)
   }
 #undef SUMPY_P2L
+#undef SUMPY_P2L_T
   return check_launch("p2l");
 }
 
@@ -1257,7 +1273,7 @@ extern "C" int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   return m2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
                   source_box_lists, box_target_starts, box_target_counts_nonchild, box_centers,
                   nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot, gx, gy,
-                  gz, nullptr, ls, stream);
+                  gz, nullptr, ls, 0, stream);
 }
 
 extern "C" int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
@@ -1268,7 +1284,7 @@ extern "C" int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int
                                    const double *ty, const double *tz, const int *box_levels,
                                    int nlevels, const double *level_rscale, double scale,
                                    const double *src_expansions, double *pot, double *gx,
-                                   double *gy, double *gz, void *stream) {
+                                   double *gy, double *gz, int precision, void *stream) {
   SUMPY_REQUIRE(box_levels, "m2p_levels: box_levels needed");
   LevelScales ls;
   int rc = level_scales(nlevels, level_rscale, ls, "m2p_levels");
@@ -1276,7 +1292,7 @@ extern "C" int sumpyhip_m2p_levels(int kernel_id, const sumpyhip_expn *expn, int
   return m2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
                   source_box_lists, box_target_starts, box_target_counts_nonchild, box_centers,
                   nboxes, tx, ty, tz, level_rscale[0], scale, src_expansions, 0, pot, gx, gy, gz,
-                  box_levels, ls, stream);
+                  box_levels, ls, precision, stream);
 }
 
 extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
@@ -1292,7 +1308,7 @@ extern "C" int sumpyhip_p2l(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   return p2l_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
                   source_box_lists, box_source_starts, box_source_counts_nonchild, box_centers,
                   nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions, tgt_base_ibox,
-                  nullptr, ls, stream);
+                  nullptr, ls, 0, stream);
 }
 
 extern "C" int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
@@ -1304,7 +1320,7 @@ extern "C" int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int
                                    const double *wx, const double *wy, const double *wz,
                                    const int *box_levels, int nlevels,
                                    const double *level_rscale, double *tgt_expansions,
-                                   void *stream) {
+                                   int precision, void *stream) {
   SUMPY_REQUIRE(box_levels, "p2l_levels: box_levels needed");
   LevelScales ls;
   int rc = level_scales(nlevels, level_rscale, ls, "p2l_levels");
@@ -1312,5 +1328,5 @@ extern "C" int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int
   return p2l_impl(kernel_id, expn, ntgt_boxes, target_boxes, source_box_starts,
                   source_box_lists, box_source_starts, box_source_counts_nonchild, box_centers,
                   nboxes, sx, sy, sz, w0, wx, wy, wz, level_rscale[0], tgt_expansions, 0,
-                  box_levels, ls, stream);
+                  box_levels, ls, precision, stream);
 }

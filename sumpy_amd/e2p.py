@@ -105,7 +105,9 @@ class E2PFromCSR(E2PBase):
     ``src_base_ibox``) it takes ``box_levels`` (tree box levels) and
     ``level_rscales`` (rscale per level) instead: the source boxes may then
     be of any level (one launch for all of list 3), each with the rscale of
-    its level, and ``src_expansions`` holds one row per tree box."""
+    its level, and ``src_expansions`` holds one row per tree box;
+    ``list34_precision="fp32"`` then evaluates each (target, source box)
+    term in FP32 and sums the boxes in FP64."""
 
     default_name = "e2p_from_csr"
 
@@ -136,13 +138,14 @@ class E2PFromCSR(E2PBase):
             # from its level, rows of the whole multipole array by box number
             levels = as_i32(kwargs["box_levels"])
             rs = _capi.f64_array(kwargs["level_rscales"])
+            prec = {"fp64": 0, "fp32": 1}[kwargs.get("list34_precision", "fp64")]
             _capi.call("sumpyhip_m2p_levels", kernel_id(self.expansion.kernel), tables.desc,
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                        _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
                        int(centers.shape[1]), *[t.data_ptr() for t in targets],
                        _capi.ptr(levels), len(kwargs["level_rscales"]), rs,
                        self.expansion.kernel.get_global_scaling_const(),
-                       _capi.ptr(kwargs["src_expansions"]), *ptrs, actx.stream_handle)
+                       _capi.ptr(kwargs["src_expansions"]), *ptrs, prec, actx.stream_handle)
         else:
             _capi.call("sumpyhip_m2p", kernel_id(self.expansion.kernel), tables.desc,
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),

@@ -47,6 +47,9 @@ class SumpyTreeIndependentDataForWrangler:
     Extra keywords: ``p2p_precision``, ``"fp64"`` (plain) or ``"fp32"`` for
     the near field (the M2L precision is chosen by the local expansion's
     ``m2l_translation``, ``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``);
+    ``list34_precision``, ``"fp64"`` or ``"fp32"`` for the terms of lists 3
+    and 4 (M2P and P2L; FP32 terms, FP64 sums; used where all levels share
+    the expansion order);
     ``autotune``, time the M2L contraction's split policies on the first pass
     of each kind of tree and keep the fastest (cached on disk,
     :mod:`sumpy_amd.autotune`).
@@ -55,7 +58,9 @@ class SumpyTreeIndependentDataForWrangler:
     def __init__(self, array_context, multipole_expansion_factory,
                  local_expansion_factory, target_kernels, exclude_self=False,
                  use_rscale=None, strength_usage=None, source_kernels=None,
-                 p2p_precision="fp64", autotune=False):
+                 p2p_precision="fp64", autotune=False, list34_precision="fp64"):
+        if list34_precision not in ("fp64", "fp32"):
+            raise ValueError(f"list34_precision: 'fp64' or 'fp32', got {list34_precision!r}")
         self._setup_actx = array_context
         self.multipole_expansion_factory = multipole_expansion_factory
         self.local_expansion_factory = local_expansion_factory
@@ -65,6 +70,7 @@ class SumpyTreeIndependentDataForWrangler:
         self.use_rscale = use_rscale
         self.strength_usage = strength_usage
         self.p2p_precision = p2p_precision
+        self.list34_precision = list34_precision
         self.autotune = autotune
         self._memo = {}
 
@@ -583,7 +589,7 @@ class SumpyExpansionWrangler:
                       target_boxes=tb, source_box_starts=st, source_box_lists=li,
                       centers=self.tree.box_centers, result=pot,
                       box_levels=self.tree.box_levels, level_rscales=self._level_rscales(),
-                      **kwargs)
+                      list34_precision=self.tree_indep.list34_precision, **kwargs)
             for a, b in zip(pot, res):
                 assert a is b
             return pot
@@ -625,7 +631,8 @@ class SumpyExpansionWrangler:
                       centers=self.tree.box_centers, strengths=src_weight_vecs,
                       tgt_expansions=self._one_row_per_box(local_exps), tgt_base_ibox=0,
                       rscale=self.level_to_rscale(0), box_levels=self.tree.box_levels,
-                      level_rscales=self._level_rscales(), **kwargs)
+                      level_rscales=self._level_rscales(),
+                      list34_precision=self.tree_indep.list34_precision, **kwargs)
             return local_exps
         for lev in range(self.tree.nlevels):
             start, stop = lstarts[lev:lev + 2]

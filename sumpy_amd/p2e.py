@@ -98,13 +98,16 @@ class P2EFromCSR(P2EBase):
     ``level_rscales`` (rscale per level): the target boxes may then be of any
     level (one launch for all of list 4), each with the rscale of its level,
     and ``tgt_expansions`` holds one row per tree box (``rscale`` is then
-    only used for the expansion tables)."""
+    only used for the expansion tables); ``list34_precision="fp32"`` then
+    forms the terms and their sums over 64 sources in FP32 and adds those
+    sums in FP64."""
 
     default_name = "p2e_from_csr"
 
     def __call__(self, actx, **kwargs):
         if not isinstance(self.expansion, LocalExpansionBase):
             raise TypeError("P2EFromCSR forms local expansions")
+        prec = {"fp64": 0, "fp32": 1}[kwargs.pop("list34_precision", "fp64")]
         sources, centers, rscale, weights, tgt, tables = self._common(actx, kwargs)
         tb = as_i32(kwargs.pop("target_boxes"))
         starts = as_i32(kwargs.pop("source_box_starts"))
@@ -129,7 +132,7 @@ class P2EFromCSR(P2EBase):
                        _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
                        int(centers.shape[1]), *[s.data_ptr() for s in sources],
                        *weights.ptrs(3), _capi.ptr(levels), len(lrs), _capi.f64_array(lrs),
-                       _capi.ptr(tgt), actx.stream_handle)
+                       _capi.ptr(tgt), prec, actx.stream_handle)
             return tgt
         _capi.call("sumpyhip_p2l", kernel_id(self.expansion.kernel), tables.desc,
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
