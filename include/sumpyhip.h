@@ -193,6 +193,36 @@ int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
                  int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
                  void *stream);
 
+/* P2M for source boxes of several tree levels in one launch, replaces the
+ * per-level loop of SumpyExpansionWrangler.form_multipoles
+ * (sumpy/fmm.py:609-641) over P2EFromSingleBox.__call__ (sumpy/p2e.py:
+ * 155-261) for the Laplace 3D kernel when every level has the same expansion
+ * order: box b has the rscale level_rscale[box_levels[b]] and its multipole
+ * at row b of tgt_expansions (one row per tree box); overwrites. */
+int sumpyhip_p2m_levels(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                        const int *source_boxes, const int *box_source_starts,
+                        const int *box_source_counts_nonchild, const double *box_centers,
+                        int nboxes, const double *sx, const double *sy, const double *sz,
+                        const double *w0, const double *wx, const double *wy,
+                        const double *wz, const int *box_levels, int nlevels,
+                        const double *level_rscale, double *tgt_expansions, void *stream);
+
+/* L2P for target boxes of several tree levels in one launch, replaces the
+ * per-level loop of SumpyExpansionWrangler.eval_locals (sumpy/fmm.py:
+ * 1047-1083) over E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226) for the
+ * Laplace 3D kernel when every level has the same expansion order: box tb
+ * has the rscale level_rscale[box_levels[tb]] and its local at row tb of
+ * src_expansions (one row per tree box); overwrites.  precision 0: FP64;
+ * 1: the Horner evaluation in FP32 (potentials, orders 1..12; L_full = P L
+ * formed in FP64). */
+int sumpyhip_l2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                        const int *target_boxes, const int *box_target_starts,
+                        const int *box_target_counts_nonchild, const double *box_centers,
+                        int nboxes, const double *tx, const double *ty, const double *tz,
+                        const int *box_levels, int nlevels, const double *level_rscale,
+                        double scale, const double *src_expansions, double *pot, double *gx,
+                        double *gy, double *gz, int precision, void *stream);
+
 /* M2P from list 3 over source boxes of several tree levels in one launch,
  * replaces the per-level loop of SumpyExpansionWrangler.eval_multipoles
  * (sumpy/fmm.py:922-961) over E2PFromCSR.__call__ (sumpy/e2p.py:233-343)

@@ -70,6 +70,10 @@ _SIGNATURES = {
                      _P, _P, _P, _P, _P],
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
+    "sumpyhip_p2m_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
+                            _I, _P, _P, _P],
+    "sumpyhip_l2p_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _D, _P,
+                            _P, _P, _P, _P, _I, _P],
     "sumpyhip_m2p_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P,
                             _D, _P, _P, _P, _P, _P, _I, _P],
     "sumpyhip_p2l_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P,

@@ -290,7 +290,7 @@ def adjoint_wrangler(wrangler, target_kernels=None):
                 ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory,
                 list(tk), exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
                 p2p_precision=ti.p2p_precision, autotune=ti.autotune,
-                list34_precision=ti.list34_precision)
+                list34_precision=ti.list34_precision, l2p_precision=ti.l2p_precision)
             ti._memo[("adjoint_tree_indep", tk)] = ati
     trav = swapped_traversal(ti._setup_actx, wrangler.traversal)
     orders = list(wrangler.level_orders)
@@ -325,7 +325,8 @@ def _gradient_wrangler(wrangler):
         ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory, tk,
         exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
         p2p_precision=ti.p2p_precision, autotune=ti.autotune,
-        list34_precision=ti.list34_precision)
+        list34_precision=ti.list34_precision,
+        l2p_precision=ti.l2p_precision)
     orders = list(wrangler.level_orders)
     got = SumpyExpansionWrangler(
         gti, wrangler.traversal, wrangler.dtype,

@@ -47,6 +47,13 @@ __device__ __forceinline__ void wave_sync() {
 
 // {{{ P2M
 
+// rscale per tree level, for launches that span several levels (a box's
+// rscale is r[box_levels[box]]); unused when box_levels is null
+constexpr int LIST_MAXLEV = 32;
+struct LevelScales {
+  double r[LIST_MAXLEV];
+};
+
 constexpr int P2M_NT = 128;   // threads per block, one source box at a time
 constexpr int P2M_SCH = 32;   // sources per LDS chunk
 
@@ -96,7 +103,8 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
     sumpyhip_expn e, int nboxes_list, const int *source_boxes, const int *bss,
     const int *bsc, const double *centers, int nboxes, const double *sx,
     const double *sy, const double *sz, const double *w0, const double *wx,
-    const double *wy, const double *wz, double rscale, double *out, int base) {
+    const double *wy, const double *wz, double rscale, double *out, int base,
+    const int *box_levels, LevelScales ls) {
   extern __shared__ double lds[];
   const int p = e.order, P1 = p + 1;
   const int PW = P1 + KT;                 // power rows zero padded by KT
@@ -106,9 +114,9 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
   const int tid = threadIdx.x;
   const int ntasks = p2m_ntasks(p, KT);
   const int nrounds = (ntasks + P2M_NT - 1)/P2M_NT;
-  const double rinv = 1.0/rscale;
   for (int ib = blockIdx.x; ib < nboxes_list; ib += gridDim.x) {
     const int b = source_boxes[ib];
+    const double rinv = 1.0/(box_levels ? ls.r[box_levels[b]] : rscale);
     const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
     const int s0 = bss[b], ns = bsc[b];
     for (int round = 0; round < nrounds; ++round) {
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
     const double *__restrict__ sy, const double *__restrict__ sz,
     const double *__restrict__ w0, const double *__restrict__ wx,
     const double *__restrict__ wy, const double *__restrict__ wz, double rscale, double *out,
-    int base) {
+    int base, const int *__restrict__ box_levels, LevelScales ls) {
   constexpr int P1 = ORD + 1;
   constexpr int NP = P1*(P1 + 1)/2;  // pairs i + j <= ORD
   constexpr int PPL = (NP + 63)/64;  // pairs per lane
@@ -212,9 +220,9 @@ __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
     pj[r] = d <= ORD ? d - q : 0;
     if (d > ORD) pi[r] = -1;
   }
-  const double rinv = 1.0/rscale;
   for (int ib = blockIdx.x*P2M_NW + w; ib < nboxes_list; ib += gridDim.x*P2M_NW) {
     const int b = source_boxes[ib];
+    const double rinv = 1.0/(box_levels ? ls.r[box_levels[b]] : rscale);
     const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
     const int s0 = bss[b], ns = bsc[b];
     double acc[PPL][P1];
@@ -439,17 +447,17 @@ constexpr int L2P_NWMAX = 8;  // waves per block (8, or 4 where 8 exceed the LDS
 // coefficients lh (runs of n = p - i - j + 1 coefficients for i = p..0,
 // j = p - i..0), with its gradient; ORD = p at compile time, so every
 // coefficient read has a constant offset and no row is padded.
-template <int ORD, bool GRAD>
-__device__ __forceinline__ void horner_t(const double *lh, double bx, double by, double bz,
-                                         double &P, double &Px, double &Py, double &Pz) {
+template <int ORD, bool GRAD, typename T>
+__device__ __forceinline__ void horner_t(const T *lh, T bx, T by, T bz, T &P, T &Px, T &Py,
+                                         T &Pz) {
   int pos = 0;
   #pragma unroll
   for (int i = ORD; i >= 0; --i) {
-    double S = 0, Sy = 0, Sz = 0;
+    T S = 0, Sy = 0, Sz = 0;
     #pragma unroll
     for (int j = ORD - i; j >= 0; --j) {
       const int n = ORD - i - j + 1;
-      double R = 0, Rz = 0;
+      T R = 0, Rz = 0;
       #pragma unroll
       for (int q = 0; q < n; ++q) {
         if (GRAD) Rz = Rz*bz + R;
@@ -464,23 +472,27 @@ __device__ __forceinline__ void horner_t(const double *lh, double bx, double by,
   }
 }
 
-template <bool GRAD, int KMAX, int ORD = -1>
+// T: the type of the Horner coefficients and evaluation (double, or float
+// for the mixed form; L_full = P L is formed in FP64).  The expansion is
+// evaluated in b / rho (rho of the box: box_levels and ls when several levels
+// share the launch), the gradient scaled back by 1 / rho.
+template <bool GRAD, int KMAX, int ORD = -1, typename T = double>
 __global__ __launch_bounds__(64*L2P_NWMAX) void l2p_lap3d(
     sumpyhip_expn e, int ntgt, const int *target_boxes, const int *bts,
     const int *btc, const double *centers, int nboxes, const double *tx,
     const double *ty, const double *tz, double rscale, double scale,
-    const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
+    const double *src, int sbase, double *pot, double *gx, double *gy, double *gz,
+    const int *box_levels, LevelScales ls) {
   extern __shared__ double sh[];
   const int nf = e.nfull, nst = e.nstored, nnz = e.proj_nnz;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, nw = blockDim.x >> 6;
-  double *hs = sh;                          // [nfull] factors
+  double *hs = sh;                          // [nfull] factors 1 / m!
   double *pv = hs + nf;                     // [nnz] values of P (Horner row order)
-  double *lh = pv + nnz + w*(nf + nst);     // per wave: [nfull] Horner coefficients
-  double *Ls = lh + nf;                     //           [nstored] L of the box
+  double *Ls = pv + nnz + w*(nf + nst);     // per wave: [nstored] L of the box
+  T *lh = reinterpret_cast<T *>(Ls + nst);  //           [nfull] Horner coefficients
   int *rs = reinterpret_cast<int *>(pv + nnz + nw*(nf + nst));  // [nfull + 1]
   int *pc = rs + nf + 1;                    // [nnz] columns of P
   const int p = e.order;  // p + 1 <= KMAX
-  const double rinv = 1.0/rscale;
   if (w == 0) {  // row starts of P in Horner order: each lane scans a contiguous range
     const int ch = (nf + 63)/64, p0 = min(nf, lane*ch), p1 = min(nf, p0 + ch);
     int loc = 0;
@@ -505,9 +517,7 @@ __global__ __launch_bounds__(64*L2P_NWMAX) void l2p_lap3d(
   for (int pos = tid; pos < nf; pos += blockDim.x) {
     const int f = e.horner_order[pos];
     const int i = e.full_ids[f], j = e.full_ids[nf + f], k = e.full_ids[2*nf + f];
-    double h = inv_fact(i)*inv_fact(j)*inv_fact(k);
-    for (int q = 0; q < i + j + k; ++q) h *= rinv;
-    hs[pos] = h;
+    hs[pos] = inv_fact(i)*inv_fact(j)*inv_fact(k);
     int d = rs[pos];
     for (int q = e.proj_row_starts[f]; q < e.proj_row_starts[f + 1]; ++q, ++d) {
       pv[d] = e.proj_vals[q];
@@ -524,29 +534,30 @@ __global__ __launch_bounds__(64*L2P_NWMAX) void l2p_lap3d(
     for (int pos = lane; pos < nf; pos += 64) {
       double v = 0;
       for (int q = rs[pos]; q < rs[pos + 1]; ++q) v += pv[q]*Ls[pc[q]];
-      lh[pos] = v*hs[pos];
+      lh[pos] = T(v*hs[pos]);
     }
     wave_sync();
     const double c0 = centers[tb], c1 = centers[nboxes + tb], c2 = centers[2*(size_t)nboxes + tb];
+    const double rinv = 1.0/(box_levels ? ls.r[box_levels[tb]] : rscale);
     const int t0 = bts[tb], nt = btc[tb];
     for (int tc = 0; tc < nt; tc += 64) {
       const bool valid = tc + lane < nt;
       const int t = t0 + tc + (valid ? lane : 0);
-      const double bx = tx[t] - c0, by = ty[t] - c1, bz = tz[t] - c2;
-      double P = 0, Px = 0, Py = 0, Pz = 0;
+      const T bx = T((tx[t] - c0)*rinv), by = T((ty[t] - c1)*rinv), bz = T((tz[t] - c2)*rinv);
+      T P = 0, Px = 0, Py = 0, Pz = 0;
       if constexpr (ORD >= 0) {
         horner_t<ORD, GRAD>(lh, bx, by, bz, P, Px, Py, Pz);
       } else {
       int pos = 0;
       for (int i = p; i >= 0; --i) {
-        double S = 0, Sy = 0, Sz = 0;
+        T S = 0, Sy = 0, Sz = 0;
         for (int j = p - i; j >= 0; --j) {
           // the (i, j) run of n coefficients: KMAX loads issued together
           const int n = p - i - j + 1;
-          double c[KMAX];
+          T c[KMAX];
           #pragma unroll
           for (int q = 0; q < KMAX; ++q) c[q] = lh[pos + (q < n ? q : n - 1)];
-          double R = 0, Rz = 0;
+          T R = 0, Rz = 0;
           #pragma unroll
           for (int q = 0; q < KMAX; ++q) {
             if (q < n) {
@@ -563,8 +574,11 @@ __global__ __launch_bounds__(64*L2P_NWMAX) void l2p_lap3d(
       }
       }
       if (valid) {
-        if (pot) pot[t] = scale*P;
-        if (GRAD) { gx[t] = scale*Px; gy[t] = scale*Py; gz[t] = scale*Pz; }
+        if (pot) pot[t] = scale*double(P);
+        if (GRAD) {
+          const double gs = scale*rinv;
+          gx[t] = gs*double(Px); gy[t] = gs*double(Py); gz[t] = gs*double(Pz);
+        }
       }
     }
   }
@@ -690,12 +704,7 @@ __device__ __forceinline__ T shift_x(const T *cur, int mx, int my) {
 
 constexpr int LW = 4;  // waves per block of the list 3 / 4 kernels
 
-// rscale per tree level, for launches that span several levels (a box's
-// rscale is r[box_levels[box]]); unused when box_levels is null
-constexpr int LIST_MAXLEV = 32;
-struct LevelScales {
-  double r[LIST_MAXLEV];
-};
+
 
 // M2P consumer: P += sum_m M(m) D(m) (shells <= ORD); with GRAD also
 // G_ax += sum_m M(m) D(m + e_ax) from the next shell.
@@ -990,14 +999,13 @@ int check_expn(const sumpyhip_expn *e, const char *what) {
 
 using namespace sumpyhip;
 
-extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
-                            const int *source_boxes, const int *box_source_starts,
-                            const int *box_source_counts_nonchild,
-                            const double *box_centers, int nboxes, const double *sx,
-                            const double *sy, const double *sz, const double *w0,
-                            const double *wx, const double *wy, const double *wz,
-                            double rscale, double *tgt_expansions, int tgt_base_ibox,
-                            void *stream) {
+static int p2m_impl(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                    const int *source_boxes, const int *box_source_starts,
+                    const int *box_source_counts_nonchild, const double *box_centers,
+                    int nboxes, const double *sx, const double *sy, const double *sz,
+                    const double *w0, const double *wx, const double *wy, const double *wz,
+                    double rscale, double *tgt_expansions, int tgt_base_ibox,
+                    const int *box_levels, const LevelScales &ls, void *stream) {
   // Helmholtz (13): one real plane of complex coefficients, with projection
   // tables built for (k rscale)^2 -- the kernel itself is table-driven
   SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
@@ -1017,12 +1025,12 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
       p2m_lap3d<true, KT><<<grid, P2M_NT, lds, st>>>(                                    \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild, \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,       \
-          tgt_base_ibox);                                                                \
+          tgt_base_ibox, box_levels, ls);                                                \
     else                                                                                 \
       p2m_lap3d<false, KT><<<grid, P2M_NT, lds, st>>>(                                   \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild, \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,       \
-          tgt_base_ibox);                                                                \
+          tgt_base_ibox, box_levels, ls);                                                \
   } while (0)
   if (expn->nfull == (expn->order + 1)*(expn->order + 2)*(expn->order + 3)/6) {
     // pair kernel for the usual orders
@@ -1033,12 +1041,12 @@ extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_b
       p2m_lap3d_pairs<O, true><<<g2, 64*P2M_NW, 0, st>>>(                                    \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
-          tgt_base_ibox);                                                                    \
+          tgt_base_ibox, box_levels, ls);                                                    \
     else                                                                                     \
       p2m_lap3d_pairs<O, false><<<g2, 64*P2M_NW, 0, st>>>(                                   \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
-          tgt_base_ibox);                                                                    \
+          tgt_base_ibox, box_levels, ls);                                                    \
     return check_launch("p2m");
     switch (expn->order) {
       SUMPY_P2M_PAIRS(1) SUMPY_P2M_PAIRS(2) SUMPY_P2M_PAIRS(3) SUMPY_P2M_PAIRS(4)
@@ -1105,14 +1113,15 @@ extern "C" int sumpyhip_l2l(int kernel_id, const sumpyhip_expn *src_expn,
   return check_launch("l2l");
 }
 
-extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
-                            const int *target_boxes, const int *box_target_starts,
-                            const int *box_target_counts_nonchild,
-                            const double *box_centers, int nboxes, const double *tx,
-                            const double *ty, const double *tz, double rscale,
-                            double scale, const double *src_expansions, int src_base_ibox,
-                            double *pot, double *gx, double *gy, double *gz,
-                            void *stream) {
+static int l2p_impl(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                    const int *target_boxes, const int *box_target_starts,
+                    const int *box_target_counts_nonchild, const double *box_centers,
+                    int nboxes, const double *tx, const double *ty, const double *tz,
+                    double rscale, double scale, const double *src_expansions,
+                    int src_base_ibox, double *pot, double *gx, double *gy, double *gz,
+                    const int *box_levels, const LevelScales &ls, int precision,
+                    void *stream) {
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "l2p: precision must be 0 or 1");
   // Helmholtz (13): one real plane of complex coefficients, with projection
   // tables built for (k rscale)^2 -- the kernel itself is table-driven
   SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
@@ -1136,14 +1145,15 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
   const int grid = (ntgt_boxes + nw - 1)/nw < 8*num_cus() ? (ntgt_boxes + nw - 1)/nw
                                                           : 8*num_cus();
   hipStream_t st = (hipStream_t)stream;
-#define SUMPY_L2P_LAUNCH(G, K, ...)                                                        \
-  l2p_lap3d<G, K, ##__VA_ARGS__><<<grid, 64*nw, lds, st>>>(                            \
+#define SUMPY_L2P_LAUNCH_T(G, K, O, T)                                                     \
+  l2p_lap3d<G, K, O, T><<<grid, 64*nw, lds, st>>>(                                       \
       *expn, ntgt_boxes, target_boxes, box_target_starts, box_target_counts_nonchild,      \
       box_centers, nboxes, tx, ty, tz, rscale, scale, src_expansions, src_base_ibox, pot,  \
-      gx, gy, gz)
+      gx, gy, gz, box_levels, ls)
 #define SUMPY_L2P_ORD(O)                                                                    \
   case O:                                                                                   \
-    SUMPY_L2P_LAUNCH(false, 8, O);                                                          \
+    if (precision) SUMPY_L2P_LAUNCH_T(false, 8, O, float);                                  \
+    else SUMPY_L2P_LAUNCH_T(false, 8, O, double);                                           \
     return check_launch("l2p");
   // potentials: compile-time Horner for the usual orders (with gradients the
   // unrolled form would hold every coefficient in registers); otherwise the
@@ -1155,14 +1165,20 @@ extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_b
     default: break;
   }
 #undef SUMPY_L2P_ORD
+  // the run-time-order loop: FP64 only
+  SUMPY_REQUIRE(precision == 0 || (!gx && expn->order >= 1 && expn->order <= 12),
+                "l2p: FP32 evaluation is built for potentials of orders 1..12");
   if (expn->order < 8) {
-    if (gx) SUMPY_L2P_LAUNCH(true, 8); else SUMPY_L2P_LAUNCH(false, 8);
+    if (gx) SUMPY_L2P_LAUNCH_T(true, 8, -1, double);
+    else SUMPY_L2P_LAUNCH_T(false, 8, -1, double);
   } else if (expn->order < 12) {
-    if (gx) SUMPY_L2P_LAUNCH(true, 12); else SUMPY_L2P_LAUNCH(false, 12);
+    if (gx) SUMPY_L2P_LAUNCH_T(true, 12, -1, double);
+    else SUMPY_L2P_LAUNCH_T(false, 12, -1, double);
   } else {
-    if (gx) SUMPY_L2P_LAUNCH(true, 25); else SUMPY_L2P_LAUNCH(false, 25);
+    if (gx) SUMPY_L2P_LAUNCH_T(true, 25, -1, double);
+    else SUMPY_L2P_LAUNCH_T(false, 25, -1, double);
   }
-#undef SUMPY_L2P_LAUNCH
+#undef SUMPY_L2P_LAUNCH_T
   return check_launch("l2p");
 }
 
@@ -1329,4 +1345,69 @@ extern "C" int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int
                   source_box_lists, box_source_starts, box_source_counts_nonchild, box_centers,
                   nboxes, sx, sy, sz, w0, wx, wy, wz, level_rscale[0], tgt_expansions, 0,
                   box_levels, ls, precision, stream);
+}
+
+extern "C" int sumpyhip_p2m(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                            const int *source_boxes, const int *box_source_starts,
+                            const int *box_source_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *sx,
+                            const double *sy, const double *sz, const double *w0,
+                            const double *wx, const double *wy, const double *wz,
+                            double rscale, double *tgt_expansions, int tgt_base_ibox,
+                            void *stream) {
+  LevelScales ls{};
+  return p2m_impl(kernel_id, expn, nsrc_boxes, source_boxes, box_source_starts,
+                  box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz,
+                  rscale, tgt_expansions, tgt_base_ibox, nullptr, ls, stream);
+}
+
+extern "C" int sumpyhip_p2m_levels(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
+                                   const int *source_boxes, const int *box_source_starts,
+                                   const int *box_source_counts_nonchild,
+                                   const double *box_centers, int nboxes, const double *sx,
+                                   const double *sy, const double *sz, const double *w0,
+                                   const double *wx, const double *wy, const double *wz,
+                                   const int *box_levels, int nlevels,
+                                   const double *level_rscale, double *tgt_expansions,
+                                   void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "p2m_levels: the Laplace 3D kernel only (rscale-free tables)");
+  SUMPY_REQUIRE(box_levels, "p2m_levels: box_levels needed");
+  LevelScales ls;
+  int rc = level_scales(nlevels, level_rscale, ls, "p2m_levels");
+  if (rc) return rc;
+  return p2m_impl(kernel_id, expn, nsrc_boxes, source_boxes, box_source_starts,
+                  box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz,
+                  level_rscale[0], tgt_expansions, 0, box_levels, ls, stream);
+}
+
+extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                            const int *target_boxes, const int *box_target_starts,
+                            const int *box_target_counts_nonchild,
+                            const double *box_centers, int nboxes, const double *tx,
+                            const double *ty, const double *tz, double rscale,
+                            double scale, const double *src_expansions, int src_base_ibox,
+                            double *pot, double *gx, double *gy, double *gz,
+                            void *stream) {
+  LevelScales ls{};
+  return l2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, box_target_starts,
+                  box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz, rscale, scale,
+                  src_expansions, src_base_ibox, pot, gx, gy, gz, nullptr, ls, 0, stream);
+}
+
+extern "C" int sumpyhip_l2p_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
+                                   const int *target_boxes, const int *box_target_starts,
+                                   const int *box_target_counts_nonchild,
+                                   const double *box_centers, int nboxes, const double *tx,
+                                   const double *ty, const double *tz, const int *box_levels,
+                                   int nlevels, const double *level_rscale, double scale,
+                                   const double *src_expansions, double *pot, double *gx,
+                                   double *gy, double *gz, int precision, void *stream) {
+  SUMPY_REQUIRE(kernel_id == 3, "l2p_levels: the Laplace 3D kernel only (rscale-free tables)");
+  SUMPY_REQUIRE(box_levels, "l2p_levels: box_levels needed");
+  LevelScales ls;
+  int rc = level_scales(nlevels, level_rscale, ls, "l2p_levels");
+  if (rc) return rc;
+  return l2p_impl(kernel_id, expn, ntgt_boxes, target_boxes, box_target_starts,
+                  box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz, level_rscale[0],
+                  scale, src_expansions, 0, pot, gx, gy, gz, box_levels, ls, precision, stream);
 }

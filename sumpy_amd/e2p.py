@@ -53,7 +53,13 @@ class E2PBase:
 
 
 class E2PFromSingleBox(E2PBase):
-    """L2P: ``result[t] = scale * sum_mi L_full[mi] (b/rho)^mi/mi!``."""
+    """L2P: ``result[t] = scale * sum_mi L_full[mi] (b/rho)^mi/mi!``.
+
+    Besides the reference's keywords it takes ``box_levels`` and
+    ``level_rscales``: the target boxes may then be of any level (one launch
+    for all of them), each with the rscale of its level, and
+    ``src_expansions`` holds one row per tree box (Laplace kernel);
+    ``l2p_precision="fp32"`` then evaluates the expansion in FP32."""
 
     default_name = "e2p_from_single_box"
 
@@ -61,7 +67,7 @@ class E2PFromSingleBox(E2PBase):
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
-        rscale = float(kwargs["rscale"])
+        rscale = float(kwargs["rscale"] if "rscale" in kwargs else kwargs["level_rscales"][0])
         k = expansion_k(self.expansion.kernel, kwargs)
         tables = self.expansion.tables_for(actx.device, k, rscale)
         tb = as_i32(kwargs["target_boxes"])
@@ -70,6 +76,20 @@ class E2PFromSingleBox(E2PBase):
         src = kwargs["src_expansions"]
 
         def l2p(loc, out_ptrs):
+            if "box_levels" in kwargs:
+                # target boxes of several levels in one launch (each with the
+                # rscale of its level; one row per tree box)
+                lrs = kwargs["level_rscales"]
+                prec = {"fp64": 0, "fp32": 1}[kwargs.get("l2p_precision", "fp64")]
+                _capi.call("sumpyhip_l2p_levels", kernel_id(self.expansion.kernel),
+                           tables.desc, int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts),
+                           _capi.ptr(btc), _capi.ptr(centers), int(centers.shape[1]),
+                           *[t.data_ptr() for t in targets],
+                           _capi.ptr(as_i32(kwargs["box_levels"])), len(lrs),
+                           _capi.f64_array(lrs),
+                           self.expansion.kernel.get_global_scaling_const(), _capi.ptr(loc),
+                           *out_ptrs, prec, actx.stream_handle)
+                return
             _capi.call("sumpyhip_l2p", kernel_id(self.expansion.kernel), tables.desc,
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts), _capi.ptr(btc),
                        _capi.ptr(centers), int(centers.shape[1]),
@@ -80,6 +100,8 @@ class E2PFromSingleBox(E2PBase):
 
         if not src.is_complex():
             l2p(src, ptrs)
+        elif "box_levels" in kwargs:
+            raise NotImplementedError("L2P over several levels: real (Laplace) expansions")
         else:
             # complex locals (Helmholtz): evaluation is real-linear in the
             # coefficients with a real projection; each plane of the locals

@@ -48,8 +48,9 @@ class SumpyTreeIndependentDataForWrangler:
     the near field (the M2L precision is chosen by the local expansion's
     ``m2l_translation``, ``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``);
     ``list34_precision``, ``"fp64"`` or ``"fp32"`` for the terms of lists 3
-    and 4 (M2P and P2L; FP32 terms, FP64 sums; used where all levels share
-    the expansion order);
+    and 4 (M2P and P2L; FP32 terms, FP64 sums) and ``l2p_precision`` for the
+    evaluation of the local expansions at the targets (potentials), both
+    used where all levels share the expansion order;
     ``autotune``, time the M2L contraction's split policies on the first pass
     of each kind of tree and keep the fastest (cached on disk,
     :mod:`sumpy_amd.autotune`).
@@ -58,9 +59,12 @@ class SumpyTreeIndependentDataForWrangler:
     def __init__(self, array_context, multipole_expansion_factory,
                  local_expansion_factory, target_kernels, exclude_self=False,
                  use_rscale=None, strength_usage=None, source_kernels=None,
-                 p2p_precision="fp64", autotune=False, list34_precision="fp64"):
-        if list34_precision not in ("fp64", "fp32"):
-            raise ValueError(f"list34_precision: 'fp64' or 'fp32', got {list34_precision!r}")
+                 p2p_precision="fp64", autotune=False, list34_precision="fp64",
+                 l2p_precision="fp64"):
+        for name, v in (("list34_precision", list34_precision),
+                        ("l2p_precision", l2p_precision)):
+            if v not in ("fp64", "fp32"):
+                raise ValueError(f"{name}: 'fp64' or 'fp32', got {v!r}")
         self._setup_actx = array_context
         self.multipole_expansion_factory = multipole_expansion_factory
         self.local_expansion_factory = local_expansion_factory
@@ -71,6 +75,7 @@ class SumpyTreeIndependentDataForWrangler:
         self.strength_usage = strength_usage
         self.p2p_precision = p2p_precision
         self.list34_precision = list34_precision
+        self.l2p_precision = l2p_precision
         self.autotune = autotune
         self._memo = {}
 
@@ -407,6 +412,15 @@ class SumpyExpansionWrangler:
         starts = _host(level_start_source_box_nrs)
         kwargs = dict(self.extra_kwargs)
         kwargs.update(self.box_source_list_kwargs())
+        if self._all_levels_ok() and source_boxes is getattr(self.traversal, "source_boxes",
+                                                              None):
+            # one launch for every level's source boxes (same order on all levels)
+            p2m = self.tree_indep.p2m(self.level_orders[0])
+            p2m(actx, source_boxes=source_boxes, centers=self.tree.box_centers,
+                strengths=src_weight_vecs, tgt_expansions=self._one_row_per_box(mpoles),
+                tgt_base_ibox=0, rscale=self.level_to_rscale(0),
+                box_levels=self.tree.box_levels, level_rscales=self._level_rscales(), **kwargs)
+            return mpoles
         for lev in range(self.tree.nlevels):
             start, stop = starts[lev:lev + 2]
             if start == stop:
@@ -665,6 +679,13 @@ class SumpyExpansionWrangler:
                 and self.tree.nlevels <= 32
                 and len(self.tree_indep.local_expansion(orders[0])) == (orders[0] + 1)**2)
 
+    def _l2p_precision(self):
+        """FP32 L2P where it is built: potentials (no target derivatives)."""
+        from sumpy_amd.kernel import remove_target_transforms
+        ti = self.tree_indep
+        plain = all(remove_target_transforms(k) == k for k in ti.target_kernels)
+        return ti.l2p_precision if plain else "fp64"
+
     def _level_rscales(self):
         return [self.level_to_rscale(lev) for lev in range(self.tree.nlevels)]
 
@@ -749,6 +770,17 @@ class SumpyExpansionWrangler:
         starts = _host(level_start_target_box_nrs)
         kwargs = dict(self.kernel_extra_kwargs)
         kwargs.update(self.box_target_list_kwargs())
+        if self._all_levels_ok() and target_boxes is getattr(self.traversal, "target_boxes",
+                                                              None):
+            # one launch for every level's target boxes (same order on all levels)
+            l2p = self.tree_indep.l2p(self.level_orders[0])
+            res = l2p(actx, src_expansions=self._one_row_per_box(local_exps),
+                      target_boxes=target_boxes, centers=self.tree.box_centers, result=pot,
+                      box_levels=self.tree.box_levels, level_rscales=self._level_rscales(),
+                      l2p_precision=self._l2p_precision(), **kwargs)
+            for a, b in zip(pot, res):
+                assert a is b
+            return pot
         for lev in range(self.tree.nlevels):
             start, stop = starts[lev:lev + 2]
             if start == stop:

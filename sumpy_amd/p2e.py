@@ -58,7 +58,12 @@ class P2EBase(KernelComputation):
 
 
 class P2EFromSingleBox(P2EBase):
-    """P2M (reference ``p2e.py:155-261``)."""
+    """P2M (reference ``p2e.py:155-261``).
+
+    Besides the reference's keywords it takes ``box_levels`` and
+    ``level_rscales``: the source boxes may then be of any level (one launch
+    for all of them), each with the rscale of its level, and
+    ``tgt_expansions`` holds one row per tree box (Laplace kernel)."""
 
     default_name = "p2e_from_single_box"
 
@@ -69,7 +74,20 @@ class P2EFromSingleBox(P2EBase):
         bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
         base = int(kwargs.pop("tgt_base_ibox"))
 
+        levels = kwargs.pop("box_levels", None)
+        lrs = kwargs.pop("level_rscales", None)
+
         def p2m(wptrs, out):
+            if levels is not None:
+                # source boxes of several levels in one launch (each with the
+                # rscale of its level; one row per tree box)
+                _capi.call("sumpyhip_p2m_levels", kernel_id(self.expansion.kernel),
+                           tables.desc, int(source_boxes.shape[0]), _capi.ptr(source_boxes),
+                           _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
+                           int(centers.shape[1]), *[s.data_ptr() for s in sources], *wptrs,
+                           _capi.ptr(as_i32(levels)), len(lrs), _capi.f64_array(lrs),
+                           _capi.ptr(out), actx.stream_handle)
+                return
             _capi.call("sumpyhip_p2m", kernel_id(self.expansion.kernel), tables.desc,
                        int(source_boxes.shape[0]), _capi.ptr(source_boxes), _capi.ptr(bss),
                        _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
@@ -79,6 +97,8 @@ class P2EFromSingleBox(P2EBase):
         if not tgt.is_complex():
             p2m(weights.ptrs(3), tgt)
             return tgt
+        if levels is not None:
+            raise NotImplementedError("P2M over several levels: real (Laplace) expansions")
         # complex strengths (Helmholtz): the moments are real-linear in the
         # weights and the projection is real, so the real and imaginary
         # planes are formed one after the other
