@@ -71,16 +71,95 @@ __device__ __forceinline__ void pair_real(T dx, T dy, T dz, T w0, T wx, T wy, T 
   }
 }
 
-__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
-__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
+// FP64 sin and cos of one argument by Cody-Waite reduction (Cody and Waite,
+// "Software Manual for the Elementary Functions", 1980) with pi/2 split into
+// four doubles, the first three of 26 significant bits, so that n p_i is
+// exact for |n| < 2^27 (the split is pi/2 rounded in steps, computed with
+// 400-bit arithmetic), then the Taylor series of sin and cos on
+// [-pi/4, pi/4] (through r^17 / 17! and r^16 / 16!; the first omitted terms
+// are below 1e-17 relative).  The reduction is exact to rounding for
+// |x| < 2^27 pi/2 (k r up to 2.1e8, far beyond any expansion's range);
+// larger arguments give NaN rather than an inaccurate value.
+__device__ __forceinline__ void sincos_cw(double x, double *sn, double *cs) {
+  constexpr double two_over_pi = 0.6366197723675814;
+  constexpr double p1 = 0x1.921fb58000000p+0, p2 = -0x1.dde9740000000p-27,
+                   p3 = 0x1.1a62630000000p-54, p4 = 0x1.8a2e03707344ap-81;
+  const double n = rint(x*two_over_pi);
+  double r = fma(-n, p1, x);
+  r = fma(-n, p2, r);
+  r = fma(-n, p3, r);
+  r = fma(-n, p4, r);
+  r = fabs(n) < 134217728.0 ? r : __builtin_nan("");
+  const double r2 = r*r;
+  // sin r = r sum_k (-1)^k r^2k / (2k+1)!, cos r = sum_k (-1)^k r^2k / (2k)!
+  double ps = -1.0/355687428096000.0;           // -1/17!
+  ps = fma(ps, r2, 1.0/1307674368000.0);         //  1/15!
+  ps = fma(ps, r2, -1.0/6227020800.0);           // -1/13!
+  ps = fma(ps, r2, 1.0/39916800.0);              //  1/11!
+  ps = fma(ps, r2, -1.0/362880.0);               // -1/9!
+  ps = fma(ps, r2, 1.0/5040.0);                  //  1/7!
+  ps = fma(ps, r2, -1.0/120.0);                  // -1/5!
+  ps = fma(ps, r2, 1.0/6.0);                     //  1/3!  (sign below)
+  const double sr = fma(-r*r2, ps, r);
+  double pc = 1.0/20922789888000.0;             //  1/16!
+  pc = fma(pc, r2, -1.0/87178291200.0);          // -1/14!
+  pc = fma(pc, r2, 1.0/479001600.0);             //  1/12!
+  pc = fma(pc, r2, -1.0/3628800.0);              // -1/10!
+  pc = fma(pc, r2, 1.0/40320.0);                 //  1/8!
+  pc = fma(pc, r2, -1.0/720.0);                  // -1/6!
+  pc = fma(pc, r2, 1.0/24.0);                    //  1/4!
+  pc = fma(pc, r2, -0.5);                        // -1/2!
+  const double cr = fma(pc, r2, 1.0);
+  const int q = int(n) & 3;  // |n| < 2^27 where the result is finite
+  const double s1 = (q & 1) ? cr : sr, c1 = (q & 1) ? sr : cr;
+  *sn = (q & 2) ? -s1 : s1;
+  *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// FP32 form of the same: pi/2 in four floats, the first three of 8
+// significant bits (n p_i exact for |n| < 2^16, |x| < 1e5; larger arguments
+// give NaN), Taylor series through r^9 / 9! and r^10 / 10! (first omitted
+// terms below 2e-9).
+__device__ __forceinline__ void sincos_cw(float x, float *sn, float *cs) {
+  constexpr float two_over_pi = 0.63661975f;
+  constexpr float p1 = 0x1.92p+0f, p2 = 0x1.fcp-12f, p3 = -0x1.58p-21f, p4 = 0x1.10b462p-30f;
+  const float n = rintf(x*two_over_pi);
+  float r = fmaf(-n, p1, x);
+  r = fmaf(-n, p2, r);
+  r = fmaf(-n, p3, r);
+  r = fmaf(-n, p4, r);
+  r = fabsf(n) < 65536.0f ? r : __builtin_nanf("");
+  const float r2 = r*r;
+  float ps = 1.0f/362880.0f;                    //  1/9!
+  ps = fmaf(ps, r2, -1.0f/5040.0f);              // -1/7!
+  ps = fmaf(ps, r2, 1.0f/120.0f);                //  1/5!
+  ps = fmaf(ps, r2, -1.0f/6.0f);                 // -1/3!
+  const float sr = fmaf(r*r2, ps, r);
+  float pc = -1.0f/3628800.0f;                  // -1/10!
+  pc = fmaf(pc, r2, 1.0f/40320.0f);              //  1/8!
+  pc = fmaf(pc, r2, -1.0f/720.0f);               // -1/6!
+  pc = fmaf(pc, r2, 1.0f/24.0f);                 //  1/4!
+  pc = fmaf(pc, r2, -0.5f);                      // -1/2!
+  const float cr = fmaf(pc, r2, 1.0f);
+  const int q = int(n) & 3;
+  const float s1 = (q & 1) ? cr : sr, c1 = (q & 1) ? sr : cr;
+  *sn = (q & 2) ? -s1 : s1;
+  *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos_cw(x, s, c); }
+__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincos_cw(x, s, c); }
 
 // Helmholtz 3D: G = exp(i k r)/r, complex k = kr + i ki, complex weights.
+// skip: the exclude-self pair, masked through 1/r (every term carries it)
 template <bool DIP, bool GRAD, typename T>
 __device__ __forceinline__ void pair_helm(T dx, T dy, T dz, T kre, T kim,
-                                          T w0r, T w0i, const T *wd, T *acc) {
+                                          T w0r, T w0i, const T *wd, T *acc,
+                                          bool skip = false) {
   // acc: pot(re,im), g0(re,im), g1, g2
   T r2 = dx*dx + dy*dy + dz*dz;
   T ri = rsqrt_t(r2);
+  ri = skip ? T(0) : ri;
   T r = r2*ri;
   T s, c;
   sincos_t(kre*r, &s, &c);
@@ -399,69 +478,147 @@ __global__ __launch_bounds__(NB) void p2p_csr_real(P2PArgs a) {
 
 // {{{ CSR near field, Helmholtz (complex weights and outputs)
 
-template <bool DIP, bool GRAD, typename T>
-__global__ __launch_bounds__(WAVE) void p2p_csr_helm(P2PArgs a) {
-  constexpr bool F32 = sizeof(T) == 4;
-  __shared__ T s_c[3][WAVE];
-  __shared__ T s_w[DIP ? 8 : 2][WAVE];
-  __shared__ int s_idx[WAVE];
-  const int lane = threadIdx.x;
+// Sources per LDS chunk: (x, y, z, Re w0) and Im w0 [+ the dipole weights],
+// about 20 KB, as for the real kernels.
+template <typename T, bool DIP>
+struct HChunk {
+  static constexpr int N = DIP ? (sizeof(T) == 4 ? 512 : 256) : (sizeof(T) == 4 ? 1024 : 512);
+};
+
+// The structure of p2p_csr_real: one 256-thread block per target box, its
+// list's source boxes staged MAXB at a time (starts, counts, offsets), their
+// particles loaded into LDS in chunks (binary search of the box: independent
+// loads), G = floor(256 / nt) lanes per target taking every G-th source,
+// lane sums added through LDS; the exclude-self pair masked with a select at
+// its position in the concatenation.  FP32: coordinates relative to the box
+// centre, FP32 sums of 16 pairs folded into FP64.
+template <bool DIP, bool GRAD, bool SELF, typename T>
+__global__ __launch_bounds__(NB) void p2p_csr_helm(P2PArgs a) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  constexpr int CH = HChunk<T, DIP>::N;
+  constexpr int NACC = GRAD ? 8 : 2;
+  constexpr int FOLD = sizeof(T) == 4 ? 16 : (1 << 20);
+  __shared__ v4 s_src[CH];                 // x, y, z, Re w0
+  __shared__ T s_wi[CH];                   // Im w0
+  __shared__ v4 s_dxy[DIP ? CH : 1];       // Re/Im wx, Re/Im wy
+  __shared__ v2 s_dz[DIP ? CH : 1];        // Re/Im wz
+  __shared__ int s_bs[MAXB], s_bn[MAXB], s_boff[MAXB + 1];
+  __shared__ int s_ktb;
+  __shared__ double s_red[NACC][NB];
+  const int tid = threadIdx.x;
   const T kre = T(a.kre), kim = T(a.kim);
   for (int ib = blockIdx.x; ib < a.ntgt_boxes; ib += gridDim.x) {
     const int tb = a.target_boxes[ib];
     const int t0 = a.box_tgt_starts[tb], nt = a.box_tgt_counts[tb];
     if (nt == 0) continue;
     const int sb0 = a.src_box_starts[ib], sb1 = a.src_box_starts[ib + 1];
-    int G = 1;
-    while (G < WAVE && nt*(2*G) <= WAVE) G *= 2;
-    const int tpc = WAVE/G;
+    const int G = max(1, min(64, NB/nt));
+    const int tpp = NB/G;
     double org[3] = {0, 0, 0};
-    if (F32) {
-      const double *tc[3] = {a.tx, a.ty, a.tz};
+    if (sizeof(T) == 4) {
+      const double *tcrd[3] = {a.tx, a.ty, a.tz};
       for (int d = 0; d < 3; ++d)
-        org[d] = a.centers ? a.centers[(size_t)d*a.nboxes + tb] : tc[d][t0];
+        org[d] = a.centers ? a.centers[(size_t)d*a.nboxes + tb] : tcrd[d][t0];
     }
-    for (int tc = 0; tc < nt; tc += tpc) {
-      const int lt = lane/G, sub = lane % G;
+    for (int tc = 0; tc < nt; tc += tpp) {
+      const int lt = tid/G, sub = tid % G;
       const bool tvalid = tc + lt < nt;
       const int t = t0 + tc + (tvalid ? lt : 0);
       const T px = T(a.tx[t] - org[0]), py = T(a.ty[t] - org[1]), pz = T(a.tz[t] - org[2]);
-      const int self = a.t2s ? a.t2s[t] : -1;
-      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int isb = sb0; isb < sb1; ++isb) {
-        const int sbx = a.src_box_lists[isb];
-        const int s0 = a.box_src_starts[sbx], ns = a.box_src_counts[sbx];
-        for (int sc = 0; sc < ns; sc += WAVE) {
-          __syncthreads();
-          if (lane < ns - sc) {
-            const int s = s0 + sc + lane;
-            s_c[0][lane] = T(a.sx[s] - org[0]);
-            s_c[1][lane] = T(a.sy[s] - org[1]);
-            s_c[2][lane] = T(a.sz[s] - org[2]);
-            s_w[0][lane] = T(a.w0[2*s]);
-            s_w[1][lane] = T(a.w0[2*s + 1]);
-            if constexpr (DIP) {
-              s_w[2][lane] = T(a.wx[2*s]); s_w[3][lane] = T(a.wx[2*s + 1]);
-              s_w[4][lane] = T(a.wy[2*s]); s_w[5][lane] = T(a.wy[2*s + 1]);
-              s_w[6][lane] = T(a.wz[2*s]); s_w[7][lane] = T(a.wz[2*s + 1]);
+      const int self = SELF ? a.t2s[t] : -1;
+      double acc[NACC];
+      for (int q = 0; q < NACC; ++q) acc[q] = 0;
+      for (int bb = sb0; bb < sb1; bb += MAXB) {
+        const int nbx = min(MAXB, sb1 - bb);
+        __syncthreads();
+        if (tid == 0) s_ktb = -1;
+        __syncthreads();
+        if (tid < nbx) {
+          const int sbx = a.src_box_lists[bb + tid];
+          s_bs[tid] = a.box_src_starts[sbx];
+          s_bn[tid] = a.box_src_counts[sbx];
+          if (SELF && sbx == tb) s_ktb = tid;
+        }
+        __syncthreads();
+        if (tid < 64) {  // exclusive scan of the counts (4 per lane)
+          int v[4], loc = 0;
+          for (int i = 0; i < 4; ++i) {
+            const int k = 4*tid + i;
+            v[i] = k < nbx ? s_bn[k] : 0;
+            loc += v[i];
+          }
+          int incl = loc;
+          for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (tid >= o) incl += y;
+          }
+          int run = incl - loc;
+          for (int i = 0; i < 4; ++i) { s_boff[4*tid + i] = run; run += v[i]; }
+          if (tid == 63) s_boff[MAXB] = run;
+        }
+        __syncthreads();
+        const int total = s_boff[nbx];
+        int pself = -1;
+        if (SELF) {
+          const int kt = s_ktb;
+          if (kt >= 0 && (unsigned)(self - s_bs[kt]) < (unsigned)s_bn[kt]) {
+            pself = s_boff[kt] + self - s_bs[kt];
+          } else {
+            for (int k = 0; k < nbx; ++k)
+              if ((unsigned)(self - s_bs[k]) < (unsigned)s_bn[k]) pself = s_boff[k] + self - s_bs[k];
+          }
+        }
+        for (int c0 = 0; c0 < total; c0 += CH) {
+          const int nn = min(CH, total - c0);
+          if (c0 > 0) __syncthreads();
+          for (int p = tid; p < nn; p += NB) {
+            const int q = c0 + p;
+            int lo = 0, hi = nbx;
+            while (hi - lo > 1) {
+              const int mid = (lo + hi) >> 1;
+              if (s_boff[mid] <= q) lo = mid; else hi = mid;
             }
-            s_idx[lane] = s;
+            const int s = s_bs[lo] + q - s_boff[lo];
+            s_src[p] = v4{T(a.sx[s] - org[0]), T(a.sy[s] - org[1]), T(a.sz[s] - org[2]),
+                          T(a.w0[2*s])};
+            s_wi[p] = T(a.w0[2*s + 1]);
+            if constexpr (DIP) {
+              s_dxy[p] = v4{T(a.wx[2*s]), T(a.wx[2*s + 1]), T(a.wy[2*s]), T(a.wy[2*s + 1])};
+              s_dz[p] = v2{T(a.wz[2*s]), T(a.wz[2*s + 1])};
+            }
           }
           __syncthreads();
-          const int nn = min(WAVE, ns - sc);
-          T tacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-          for (int j = sub; j < nn; j += G) {
-            if (s_idx[j] == self) continue;
-            T wd[6];
-            if (DIP) for (int q = 0; q < 6; ++q) wd[q] = s_w[2 + q][j];
-            pair_helm<DIP, GRAD, T>(px - s_c[0][j], py - s_c[1][j], pz - s_c[2][j], kre,
-                                    kim, s_w[0][j], s_w[1][j], wd, tacc);
+          const int jself = pself - c0;
+          for (int j0 = sub; j0 < nn; j0 += G*FOLD) {
+            const int jend = min(nn, j0 + G*FOLD);
+            T tacc[NACC];
+            for (int q = 0; q < NACC; ++q) tacc[q] = 0;
+            for (int j = j0; j < jend; j += G) {
+              const v4 sv = s_src[j];
+              T wd[6] = {0, 0, 0, 0, 0, 0};
+              if constexpr (DIP) {
+                const v4 d1 = s_dxy[j];
+                const v2 d2 = s_dz[j];
+                wd[0] = d1.x; wd[1] = d1.y; wd[2] = d1.z; wd[3] = d1.w; wd[4] = d2.x; wd[5] = d2.y;
+              }
+              pair_helm<DIP, GRAD, T>(px - sv.x, py - sv.y, pz - sv.z, kre, kim, sv.w, s_wi[j],
+                                      wd, tacc, SELF && j == jself);
+            }
+            for (int q = 0; q < NACC; ++q) acc[q] += double(tacc[q]);
           }
-          for (int q = 0; q < (GRAD ? 8 : 2); ++q) acc[q] += double(tacc[q]);
         }
       }
-      for (int q = 0; q < (GRAD ? 8 : 2); ++q) acc[q] = group_sum(acc[q], G);
+      __syncthreads();  // the previous pass's sums have been read
+      for (int q = 0; q < NACC; ++q) s_red[q][tid] = acc[q];
+      __syncthreads();
       if (sub == 0 && tvalid) {
+        const int q0 = lt*G;
+        for (int q = 0; q < NACC; ++q) {
+          double v = 0;
+          for (int l = q0; l < q0 + G; ++l) v += s_red[q][l];
+          acc[q] = v;
+        }
         if (a.pot) { a.pot[2*t] = a.scale*acc[0]; a.pot[2*t + 1] = a.scale*acc[1]; }
         if (GRAD) {
           double *g[3] = {a.gx, a.gy, a.gz};
@@ -577,12 +734,12 @@ __global__ __launch_bounds__(DB) void p2p_dense_helm(P2PArgs a) {
     __syncthreads();
     const int nn = min(DB, a.nsources - sc);
     T tacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int jself = self - sc;  // the excluded pair of this tile, masked
     for (int j = 0; j < nn; ++j) {
-      if (sc + j == self) continue;
-      T wd[6];
+      T wd[6] = {0, 0, 0, 0, 0, 0};
       if (DIP) for (int q = 0; q < 6; ++q) wd[q] = s_w[2 + q][j];
       pair_helm<DIP, GRAD, T>(px - s_c[0][j], py - s_c[1][j], pz - s_c[2][j], kre, kim,
-                              s_w[0][j], s_w[1][j], wd, tacc);
+                              s_w[0][j], s_w[1][j], wd, tacc, j == jself);
     }
     for (int q = 0; q < (GRAD ? 8 : 2); ++q) acc[q] += double(tacc[q]);
   }
@@ -620,8 +777,14 @@ int launch_csr(int knl, int prec, const P2PArgs &a, hipStream_t st) {
     if (self) launch_csr_real<KLAP2, DIP, GRAD, true>(prec, a, st);
     else launch_csr_real<KLAP2, DIP, GRAD, false>(prec, a, st);
   } else {
-    if (prec == 0) p2p_csr_helm<DIP, GRAD, double><<<grid, WAVE, 0, st>>>(a);
-    else p2p_csr_helm<DIP, GRAD, float><<<grid, WAVE, 0, st>>>(a);
+    const int hgrid = a.ntgt_boxes;  // one box per block (as the real kernels)
+    if (self) {
+      if (prec == 0) p2p_csr_helm<DIP, GRAD, true, double><<<hgrid, NB, 0, st>>>(a);
+      else p2p_csr_helm<DIP, GRAD, true, float><<<hgrid, NB, 0, st>>>(a);
+    } else {
+      if (prec == 0) p2p_csr_helm<DIP, GRAD, false, double><<<hgrid, NB, 0, st>>>(a);
+      else p2p_csr_helm<DIP, GRAD, false, float><<<hgrid, NB, 0, st>>>(a);
+    }
   }
   return check_launch("p2p_csr");
 }
