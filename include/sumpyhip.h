@@ -485,6 +485,65 @@ int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *
 
 /* }}} */
 
+/* {{{ tree and traversal on the GPU
+
+ * Building blocks of the device tree / traversal builder
+ * (sumpy_amd/tree_gpu.py orchestrates them: sorts, scans, compaction).  The
+ * reference takes its tree, lists and translation classes from boxtree
+ * (TreeBuilder, FMMTraversalBuilder, TranslationClassesBuilder, called at
+ * sumpy/test/test_fmm.py:182-196 and read at sumpy/fmm.py:357-592); these
+ * produce the same arrays as the host builder, array for array.  Integer
+ * arrays are int32, keys and Morton prefixes uint64, coordinates FP64 SoA. */
+
+/* Morton keys of n points (x, y, z: device arrays; z unused in 2D): cell
+ * c_d = floor((x_d - lo[d]) / ext * 2^maxdepth) clamped to the grid, axis d
+ * bit b at key bit dim*b + dim-1-d.  lo: host array of dim doubles. */
+int sumpytree_gpu_morton(int dim, long n, const double *x, const double *y, const double *z,
+                         const double *lo, double ext, int maxdepth, void *keys, void *stream);
+
+/* Children of nsplit split boxes (Morton prefix, sorted source and target
+ * ranges): for child c, the ranges of prefix (P << dim) | c among the sorted
+ * keys (key >> shift), and whether it holds any particle.  tkeys NULL: the
+ * targets are the sources.  Outputs have nsplit << dim entries. */
+int sumpytree_gpu_children(int dim, int nsplit, const void *prefix, const int *slo,
+                           const int *shi, const int *tlo, const int *thi, const void *skeys,
+                           const void *tkeys, int shift, int *cslo, int *cshi, int *ctlo,
+                           int *cthi, int *exists, void *stream);
+
+/* Box centres (dim x nboxes, lo + (ix + 1/2) ext / 2^level) and integer cell
+ * coordinates (dim x nboxes) from the boxes' Morton prefixes and levels. */
+int sumpytree_gpu_geometry(int dim, int nboxes, const void *prefix, const int *level,
+                           const double *lo, double ext, double *centers, int *ix,
+                           void *stream);
+
+/* Same-level neighbours of every box, ascending, -1 padded to 26 (3D) or 8
+ * (2D) per box. */
+int sumpytree_gpu_colleagues(int dim, int nboxes, int nlevels, const int *level,
+                             const int *ix, const int *child, const int *parent,
+                             const int *leaf, const int *level_start, const int *src_cumul,
+                             const int *src_nonchild, const void *prefix, int *coll,
+                             void *stream);
+
+/* Interaction list `which` (1: neighbour leaves, 2: well-separated children of
+ * the parent's colleagues, 3: separated smaller boxes per source level, 4:
+ * separated coarser leaves) of nb boxes.  counts != NULL: the counting pass
+ * (list 3: nlevels counts per box); else the writing pass at starts (list 3:
+ * offsets per (level, box)), with the translation slot of each list-2 pair in
+ * slots and the level's used slots flagged in used. */
+int sumpytree_gpu_list(int which, int dim, int nboxes, int nlevels, const int *level,
+                       const int *ix, const int *child, const int *parent, const int *leaf,
+                       const int *level_start, const int *src_cumul, const int *src_nonchild,
+                       const void *prefix, const int *coll, int nb, const int *boxes,
+                       int *counts, const int *starts, int *out, int *slots, int *used,
+                       void *stream);
+
+/* Translation class of each of n list-2 pairs:
+ * cls[e] = slot_to_cls[entry_level[e] * nslots + slot[e]]. */
+int sumpytree_gpu_list2_classes(int n, const int *entry_level, const int *slot,
+                                const int *slot_to_cls, int nslots, int *cls, void *stream);
+
+/* }}} */
+
 #ifdef __cplusplus
 }
 #endif

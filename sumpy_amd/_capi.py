@@ -70,6 +70,13 @@ _SIGNATURES = {
                      _P, _P, _P, _P, _P],
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
+    "sumpytree_gpu_morton": [_I, _L, _P, _P, _P, _P, _D, _I, _P, _P],
+    "sumpytree_gpu_children": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    "sumpytree_gpu_geometry": [_I, _I, _P, _P, _P, _D, _P, _P, _P],
+    "sumpytree_gpu_colleagues": [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "sumpytree_gpu_list": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P,
+                           _P, _P, _P, _P, _P],
+    "sumpytree_gpu_list2_classes": [_I, _P, _P, _P, _I, _P, _P],
     "sumpyhip_p2m_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
                             _I, _P, _P, _P],
     "sumpyhip_l2p_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _D, _P,

@@ -257,6 +257,36 @@ def _wrap(actx, sc, th, vh):
     return tree, trav
 
 
+def _wrap_device(actx, sc, th_dev, vh_dev):
+    """Tree / traversal / translation-class objects over device arrays
+    (built on the GPU), with host copies of every array."""
+    def host(t):
+        return t.cpu().numpy()
+    th = {k: host(v) for k, v in th_dev.items() if not (sc["same"] and k == "targets")}
+    if sc["same"]:
+        th["targets"] = th["sources"]
+    vh = {k: host(v) for k, v in vh_dev.items() if k != "sep_smaller"}
+    vh["sep_smaller"] = [tuple(host(a) for a in trip) for trip in vh_dev["sep_smaller"]]
+    dim, nlevels = sc["dim"], sc["nlevels"]
+    tree = Tree(dimensions=dim, nlevels=nlevels, root_extent=sc["root_extent"],
+                bounding_box=(sc["bb_min"], sc["bb_min"] + sc["root_extent"]),
+                sources_are_targets=sc["same"], host=th)
+    tree.__dict__["_dev"] = dict(th_dev)
+    trav = FMMTraversalInfo(tree=tree, host=vh)
+    tdev = {k: v for k, v in vh_dev.items()
+            if not k.startswith("from_sep_siblings_translation") and k != "sep_smaller"}
+    tdev["target_boxes_sep_smaller_by_source_level"] = [t for t, _, _ in vh_dev["sep_smaller"]]
+    tdev["from_sep_smaller_by_level"] = [_LevelCSR(s, li) for _, s, li in vh_dev["sep_smaller"]]
+    trav.__dict__["_dev"] = tdev
+    trav.__dict__["_translation_classes"] = TranslationClassesData(
+        from_sep_siblings_translation_classes=vh_dev["from_sep_siblings_translation_classes"],
+        from_sep_siblings_translation_class_to_distance_vector=vh_dev[
+            "from_sep_siblings_translation_class_to_distance_vector"],
+        from_sep_siblings_translation_classes_level_starts=vh_dev[
+            "from_sep_siblings_translation_classes_level_starts"])
+    return tree, trav
+
+
 def build_tree_and_traversal(actx, sources, targets=None, max_particles_in_box=30,
                              max_levels=None):
     """Builds tree + traversal + translation classes in one host pass."""

@@ -1,5 +1,6 @@
 """The C-ABI library loads without a GPU and exports every symbol that
-include/sumpyhip.h declares, with matching argument counts."""
+include/sumpyhip.h declares (sumpyhip_* and sumpytree_gpu_*), with matching
+argument counts."""
 
 import os
 import re
@@ -11,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_prototypes():
     hdr = open(os.path.join(ROOT, "include", "sumpyhip.h")).read()
-    return re.findall(r"^(?:const char \*|int )(sumpyhip_\w+)\(([^;]*)\);", hdr, re.M)
+    return re.findall(r"^(?:const char \*|int )(sumpy(?:hip|tree_gpu)_\w+)\(([^;]*)\);", hdr,
+                      re.M)
 
 
 def test_library_exports_header_symbols():
