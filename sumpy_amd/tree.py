@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+from collections.abc import MutableMapping
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -101,17 +102,21 @@ class Tree:
             return dev[name]
         raise AttributeError(name)
 
+    def _shape(self, name):
+        dev = self.__dict__.get("_dev")
+        return tuple((dev if dev is not None and name in dev else self.host)[name].shape)
+
     @property
     def nsources(self):
-        return self.host["sources"].shape[1]
+        return self._shape("sources")[1]
 
     @property
     def ntargets(self):
-        return self.host["targets"].shape[1]
+        return self._shape("targets")[1]
 
     @property
     def nboxes(self):
-        return len(self.host["box_levels"])
+        return self._shape("box_levels")[0]
 
     @property
     def aligned_nboxes(self):
@@ -257,16 +262,63 @@ def _wrap(actx, sc, th, vh):
     return tree, trav
 
 
+class _HostCopies(MutableMapping):
+    """Host (numpy) copies of device arrays, each made on its first access
+    (a device synchronisation) and kept: a tree built on the GPU hands the
+    host only what its users read.  ``aliases`` maps a name to another
+    whose copy it shares; ``"sep_smaller"`` holds per-level triples."""
+
+    def __init__(self, dev, aliases=None):
+        self._dev = dict(dev)
+        self._aliases = dict(aliases or {})
+        self._copies = {}
+
+    def __getitem__(self, name):
+        got = self._copies.get(name)
+        if got is None:
+            if name in self._aliases:
+                got = self[self._aliases[name]]
+            else:
+                v = self._dev[name]
+                got = ([tuple(a.cpu().numpy() for a in trip) for trip in v]
+                       if name == "sep_smaller" else v.cpu().numpy())
+            self._copies[name] = got
+        return got
+
+    def __setitem__(self, name, value):
+        self._dev.pop(name, None)
+        self._aliases.pop(name, None)
+        self._copies[name] = value
+
+    def __delitem__(self, name):
+        found = False
+        for d in (self._dev, self._aliases, self._copies):
+            if name in d:
+                del d[name]
+                found = True
+        if not found:
+            raise KeyError(name)
+
+    def _names(self):
+        return list(dict.fromkeys([*self._dev, *self._aliases, *self._copies]))
+
+    def __iter__(self):
+        return iter(self._names())
+
+    def __len__(self):
+        return len(self._names())
+
+    def __contains__(self, name):
+        return name in self._dev or name in self._aliases or name in self._copies
+
+
 def _wrap_device(actx, sc, th_dev, vh_dev):
     """Tree / traversal / translation-class objects over device arrays
-    (built on the GPU), with host copies of every array."""
-    def host(t):
-        return t.cpu().numpy()
-    th = {k: host(v) for k, v in th_dev.items() if not (sc["same"] and k == "targets")}
-    if sc["same"]:
-        th["targets"] = th["sources"]
-    vh = {k: host(v) for k, v in vh_dev.items() if k != "sep_smaller"}
-    vh["sep_smaller"] = [tuple(host(a) for a in trip) for trip in vh_dev["sep_smaller"]]
+    (built on the GPU); host copies are made on first access."""
+    th = _HostCopies({k: v for k, v in th_dev.items()
+                      if not (sc["same"] and k == "targets")},
+                     {"targets": "sources"} if sc["same"] else None)
+    vh = _HostCopies(vh_dev)
     dim, nlevels = sc["dim"], sc["nlevels"]
     tree = Tree(dimensions=dim, nlevels=nlevels, root_extent=sc["root_extent"],
                 bounding_box=(sc["bb_min"], sc["bb_min"] + sc["root_extent"]),
@@ -330,9 +382,14 @@ def swapped_traversal(actx, traversal):
     if params is None:
         raise NotImplementedError("swapped_traversal: the tree was not built by "
                                   "build_tree_and_traversal (a forest?)")
-    src = user_order_points(tree, "sources").cpu().numpy()
-    tgt = user_order_points(tree, "targets").cpu().numpy()
-    stree, strav = build_tree_and_traversal(actx, tgt, src, *params)
+    src = user_order_points(tree, "sources")
+    tgt = user_order_points(tree, "targets")
+    if tree.__dict__.get("_built_on_device"):
+        from sumpy_amd.tree_gpu import build_tree_and_traversal_gpu
+        stree, strav = build_tree_and_traversal_gpu(actx, tgt, src, *params)
+    else:
+        stree, strav = build_tree_and_traversal(actx, tgt.cpu().numpy(), src.cpu().numpy(),
+                                                *params)
     for name in ("box_levels", "box_parent_ids", "box_child_ids", "level_start_box_nrs"):
         if not np.array_equal(stree.host[name], tree.host[name]):
             raise RuntimeError(f"swapped_traversal: boxes differ ({name})")

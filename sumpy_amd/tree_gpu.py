@@ -316,6 +316,7 @@ def build_tree_and_traversal_gpu(actx, sources, targets=None, max_particles_in_b
     t0 = time.perf_counter()
     tree, trav = _wrap_device(actx, sc, th, vh)
     tree.__dict__["_build_params"] = (max_particles_in_box, max_levels)
+    tree.__dict__["_built_on_device"] = True
     if timing is not None:
         timing["host_copies"] = time.perf_counter() - t0
     return tree, trav
