@@ -283,6 +283,58 @@ def build_trees(actx, pts, max_particles, host_builder=True):
     return tree, trav, t_gpu, t_host
 
 
+def h2d_bench(actx, n=2**18, k=20.0, order=20, max_particles=64, steps=5, nsample=1024):
+    """The 2D Helmholtz FMM with the H2D (Hankel/Bessel) expansions: n
+    uniform points in the unit square (about 3 wavelengths across at
+    k = 20), sources = targets, self excluded, complex N(0,1) strengths,
+    order 20, translation classes; pass time (HIP-graph replay when bitwise
+    equal to the eager pass) and the error against the FP64 direct sum on
+    *nsample* targets."""
+    from functools import partial
+
+    import torch
+
+    from sumpy_amd import P2P
+    from sumpy_amd.drive import capture_drive_fmm, drive_fmm
+    from sumpy_amd.expansion.local import H2DLocalExpansion
+    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree_gpu import build_tree_and_traversal_gpu
+    rng = np.random.default_rng(7)
+    pts = rng.random((2, n))
+    q = rng.standard_normal(n) + 1j*rng.standard_normal(n)
+    tree, trav = build_tree_and_traversal_gpu(actx, torch.as_tensor(pts, device=actx.device),
+                                              None, max_particles)
+    knl = HelmholtzKernel(2)
+    ti = SumpyTreeIndependentDataForWrangler(actx, partial(H2DMultipoleExpansion, knl),
+                                             partial(H2DLocalExpansion, knl), [knl],
+                                             exclude_self=True)
+    t2s = actx.from_numpy(np.arange(n, dtype=np.int32))
+    wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: order,
+                                kernel_extra_kwargs={"k": k},
+                                self_extra_kwargs={"target_to_source": t2s})
+    qd = actx.from_numpy(q)
+    wr.multipole_to_local_precompute()
+    pot, = drive_fmm(actx, wr, [qd])
+    ms, _, _ = time_passes(actx, lambda: drive_fmm(actx, wr, [qd]), steps, 1)
+    graph = capture_drive_fmm(actx, wr, [qd])
+    gms, _, gout = time_passes(actx, graph.replay, steps, 1)
+    same = bool(torch.equal(gout[0], pot))
+    sample = np.sort(rng.choice(n, size=nsample, replace=False))
+    direct, = P2P([knl], exclude_self=True)(
+        actx, actx.from_numpy(pts[:, sample]), actx.from_numpy(pts), [qd], k=k,
+        target_to_source=actx.from_numpy(sample.astype(np.int32)))
+    direct = actx.to_numpy(direct)
+    err = float(np.linalg.norm(actx.to_numpy(pot)[sample] - direct)/np.linalg.norm(direct))
+    out = {"n": n, "k": k, "order": order, "nboxes": tree.nboxes, "nlevels": tree.nlevels,
+           "ms": round(min(ms, gms) if same else ms, 3), "eager_ms": round(ms, 3),
+           "graph_ok": same, "rel_l2_err": err}
+    log(f"H2D 2D Helmholtz FMM n={n} k={k} p={order}: {out['ms']:.3f} ms/pass, "
+        f"rel. L2 error {err:.2e}")
+    return out
+
+
 def gaussian8_bench(actx, args, steps=10):
     """The same FMM pass (N, order, 64 per box, self excluded) on the mixture
     of 8 Gaussians (sigma = 0.02): the uneven tree with lists 3 and 4.  Every
@@ -542,6 +594,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gaussian8", action="store_true",
                     help="skip the extra 8-Gaussian-mixture measurement of the uniform run")
+    ap.add_argument("--no-h2d", action="store_true",
+                    help="skip the extra 2D Helmholtz (H2D expansions) FMM measurement")
     ap.add_argument("--nsample", type=int, default=4096)
     ap.add_argument("--p2p-dense-n", type=int, default=2**18,
                     help="dense P2P size (sources = targets; 0 skips it)")
@@ -701,6 +755,7 @@ def main():
     qbx = qbx_bench(actx, 3)
     gauss = gaussian8_bench(actx, args) if args.distribution == "uniform" and \
         not args.no_gaussian8 else None
+    h2d = None if args.no_h2d else h2d_bench(actx)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -758,6 +813,7 @@ def main():
         "p2p_dense": dense,
         "qbx_starfish_dlp": qbx,
         "gaussian8": gauss,
+        "helmholtz2d_h2d": h2d,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,611 @@
+// 2D Helmholtz FMM with Hankel/Bessel expansions (H2D).
+//
+// G(d) = (i/4) H0^(1)(k |d|), real k > 0 (sumpy/kernel.py:754-844, 2D).  The
+// reference's H2DMultipoleExpansion / H2DLocalExpansion
+// (sumpy/expansion/multipole.py:426-550, sumpy/expansion/local.py:522-682)
+// and its Fourier-Bessel M2L (sumpy/expansion/m2l.py:1008-1115) store 2p+1
+// complex coefficients c = -p..p at index c + p:
+//   P2M  M_c = sum_s q_s J_c(k|a|) rscale^-|c| e^{-i c th_s}
+//   M2P  u(t) = (i/4) sum_c M_c H_c(k|b|) rscale^|c| e^{i c th_b}
+//   P2L  L_c = sum_s q_s H_c(k|a|) rscale^|c| e^{i c th_s}
+//   L2P  u(t) = (i/4) sum_c L_c J_c(k|b|) rscale^-|c| e^{-i c th_b}
+//   M2M  M'_j = sum_m M_m J_{m-j}(k|d|) rs^|m| rt^-|j| e^{i (m-j) th_d}
+//   L2L  L'_j = sum_m L_m J_{m-j}(k|d|) rs^-|m| rt^|j| e^{-i (m-j) th_d}
+//   M2L  L_j = (-1)^j rt^|j| sum_m D[m + j + ps + pt] rs^|m| M_m,
+//        D[n + ps + pt] = H_n(k|d|) e^{i n th_d}  (the translation-class data)
+// with a = centre - source, b = target - centre, d = new centre - old
+// centre, th_s the angle of the source about the centre.
+//
+// Bessel functions: J_0..J_n by Miller's backward recurrence normalised with
+// J_0 + 2 sum_k J_2k = 1; Y_0 and Y_1 from their Neumann series in the J_n
+// (Abramowitz & Stegun 9.1.88), Y_n by the upward recurrence (stable for Y);
+// H_n = J_n + i Y_n, J_-n = (-1)^n J_n, H_-n = (-1)^n H_n.
+//
+// Layout: one wave (64 lanes) per box; lane c + p owns coefficient c
+// (2p+1 <= 63), so the orders go up to 31.  Per-source terms are summed per
+// lane and reduced over the wave with a fixed butterfly (deterministic).
+#include "common.h"
+
+#include <cmath>
+#include <cstdint>
+
+namespace sumpyhip {
+namespace {
+
+constexpr int PMAX = 31;
+constexpr int NSEQ = 2*PMAX + 2;  // sequences of orders 0..2 PMAX (+1 for Y_1)
+constexpr int WAVE = 64;
+constexpr double PI = 3.14159265358979323846;
+constexpr double EULER_GAMMA = 0.57721566490153286061;
+constexpr double BIG = 1e250, SMALL = 1e-250;
+
+__device__ __forceinline__ cplx conj(cplx a) { return {a.re, -a.im}; }
+__device__ __forceinline__ cplx cfma(cplx a, cplx b, cplx acc) {
+  return {acc.re + a.re*b.re - a.im*b.im, acc.im + a.re*b.im + a.im*b.re};
+}
+
+// J_0..J_n(x) and, when Y != null, Y_0..Y_n(x) (x > 0 then; n >= 1).
+// x = 0: J_0 = 1, J_n = 0 (Y not requested there).
+__device__ void bessel_jy(double x, int n, double *J, double *Y) {
+  if (x == 0.0) {
+    J[0] = 1.0;
+    for (int m = 1; m <= n; ++m) J[m] = 0.0;
+    return;
+  }
+  int M = (n > (int)x ? n : (int)x) + 20 + (int)(16.0*cbrt(x));
+  M += M & 1;
+  const double tox = 2.0/x;
+  double jp1 = 0.0, j = 1.0, norm = 0.0, sy0 = 0.0, sy1 = 0.0;
+  for (int m = M; m >= 1; --m) {
+    // j = J_m, jp1 = J_{m+1} (a common unknown factor)
+    if (m <= n) J[m] = j;
+    if ((m & 1) == 0) {
+      const int kk = m >> 1;
+      norm += 2.0*j;
+      sy0 += ((kk & 1) ? -j : j)/(double)kk;
+    } else if (m >= 3) {
+      const int kk = (m - 1) >> 1;
+      sy1 += ((kk & 1) ? -j : j)*(double)(2*kk + 1)/((double)kk*(double)(kk + 1));
+    }
+    const double jm1 = (double)m*tox*j - jp1;
+    jp1 = j;
+    j = jm1;
+    if (fabs(j) > BIG) {
+      j *= SMALL; jp1 *= SMALL; norm *= SMALL; sy0 *= SMALL; sy1 *= SMALL;
+      for (int q = m; q <= n; ++q) J[q] *= SMALL;
+    }
+  }
+  norm += j;
+  J[0] = j;
+  const double inv = 1.0/norm;
+  for (int m = 0; m <= n; ++m) J[m] *= inv;
+  if (!Y) return;
+  sy0 *= inv;
+  sy1 *= inv;
+  const double L = log(0.5*x) + EULER_GAMMA;
+  Y[0] = (2.0/PI)*L*J[0] - (4.0/PI)*sy0;
+  Y[1] = -2.0*J[0]/(PI*x) + (2.0/PI)*(L - 1.0)*J[1] - (2.0/PI)*sy1;
+  for (int m = 1; m < n; ++m) Y[m + 1] = (double)m*tox*Y[m] - Y[m - 1];
+}
+
+__device__ __forceinline__ double wave_sum(double v) { return group_sum(v, WAVE); }
+
+__device__ __forceinline__ cplx load_c(const double *p, long i) { return {p[2*i], p[2*i + 1]}; }
+__device__ __forceinline__ void store_c(double *p, long i, cplx v) {
+  p[2*i] = v.re;
+  p[2*i + 1] = v.im;
+}
+
+// rscale^e for 0 <= e <= 2 PMAX (e multiplications: the same on every lane
+// for one e, no pow)
+__device__ __forceinline__ double ipow(double r, int e) {
+  double v = 1.0;
+  for (int i = 0; i < e; ++i) v *= r;
+  return v;
+}
+
+// unit vector of v (1, 0 for v = 0) and |v|
+__device__ __forceinline__ double unit(double vx, double vy, double &ux, double &uy) {
+  const double r = sqrt(vx*vx + vy*vy);
+  if (r > 0.0) {
+    ux = vx/r;
+    uy = vy/r;
+  } else {
+    ux = 1.0;
+    uy = 0.0;
+  }
+  return r;
+}
+
+// Per-lane accumulation of sum_c w Z_c(k rho) e^{i s c th} over c = -p..p
+// into acc[c + p] (Z = J, or H = J + iY when Y is given); the phase
+// e^{i th} is (ux, uy) and s = sign.
+__device__ void add_terms(int p, cplx w, const double *J, const double *Y, double ux,
+                          double uy, int sign, cplx *acc) {
+  const cplx e1 = {ux, sign*uy};
+  cplx ph = {1.0, 0.0};  // e^{i s c th}
+  for (int c = 0; c <= p; ++c) {
+    const cplx z = {J[c], Y ? Y[c] : 0.0};
+    const cplx t = cmul(w, cmul(z, ph));
+    acc[p + c] = cadd(acc[p + c], t);
+    if (c > 0) {
+      // c -> -c: Z_-c = (-1)^c Z_c, e^{-i s c th} = conj(ph)
+      const cplx tn = cmul(w, cmul(z, conj(ph)));
+      acc[p - c] = cadd(acc[p - c], (c & 1) ? cscale(tn, -1.0) : tn);
+    }
+    ph = cmul(ph, e1);
+  }
+}
+
+// {{{ P2M / P2L: one wave per box, lanes over its sources
+
+template <bool LOCAL>
+__global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
+    int p, double k, int nboxes, const int *boxes, const int *starts, const int *lists,
+    const int *bss, const int *bsc, const double *centers, int ncenters, const double *sx,
+    const double *sy, const double *q, double rscale, double *out, int base) {
+  const int ib = blockIdx.x;
+  if (ib >= nboxes) return;
+  const int box = boxes[ib];
+  const int lane = threadIdx.x;
+  const double cx = centers[box], cy = centers[ncenters + box];
+  cplx acc[2*PMAX + 1];
+  for (int c = 0; c < 2*p + 1; ++c) acc[c] = {0.0, 0.0};
+  double J[PMAX + 2], Y[PMAX + 2];
+  // P2M: the box's own sources; P2L: the sources of its list-4 boxes
+  const int e0 = LOCAL ? starts[ib] : 0, e1 = LOCAL ? starts[ib + 1] : 1;
+  for (int e = e0; e < e1; ++e) {
+    const int sb = LOCAL ? lists[e] : box;
+    const int s0 = bss[sb], ns = bsc[sb];
+    for (int i = lane; i < ns; i += WAVE) {
+      const int s = s0 + i;
+      double ux, uy;  // e^{i th_s}: the source about the centre
+      const double rho = unit(sx[s] - cx, sy[s] - cy, ux, uy);
+      const cplx w = {q[2*s], q[2*s + 1]};
+      if (LOCAL) {
+        bessel_jy(k*rho, p > 0 ? p : 1, J, Y);
+        add_terms(p, w, J, Y, ux, uy, 1, acc);
+      } else {
+        bessel_jy(k*rho, p, J, nullptr);
+        add_terms(p, w, J, nullptr, ux, uy, -1, acc);
+      }
+    }
+  }
+  double mine_re = 0.0, mine_im = 0.0;
+  for (int c = 0; c < 2*p + 1; ++c) {
+    const double re = wave_sum(acc[c].re), im = wave_sum(acc[c].im);
+    if (lane == c) {
+      mine_re = re;
+      mine_im = im;
+    }
+  }
+  if (lane < 2*p + 1) {
+    const int ac = lane > p ? lane - p : p - lane;
+    const double s = LOCAL ? ipow(rscale, ac) : 1.0/ipow(rscale, ac);
+    store_c(out, (long)(box - base)*(2*p + 1) + lane, {mine_re*s, mine_im*s});
+  }
+}
+
+// }}}
+
+// {{{ M2M / L2L: one wave per target box
+
+// J_n(k|d|) e^{i n th_d} for n = 0..N into LDS (lane 0), the same wave
+__device__ void shift_data(double k, double dx, double dy, int N, double *Js, cplx *ph) {
+  if (threadIdx.x == 0) {
+    double ux, uy;
+    const double dl = unit(dx, dy, ux, uy);
+    double J[NSEQ];
+    bessel_jy(k*dl, N, J, nullptr);
+    cplx e = {1.0, 0.0};
+    for (int n = 0; n <= N; ++n) {
+      Js[n] = J[n];
+      ph[n] = e;
+      e = cmul(e, cplx{ux, uy});
+    }
+  }
+  __syncthreads();
+}
+
+// J_n e^{i s n th} for any sign of n from the n >= 0 tables
+__device__ __forceinline__ cplx shift_term(int n, const double *Js, const cplx *ph, int sign) {
+  const int an = n < 0 ? -n : n;
+  double j = Js[an];
+  if (n < 0 && (an & 1)) j = -j;
+  cplx e = ph[an];
+  if ((n < 0) != (sign < 0)) e = conj(e);
+  return cscale(e, j);
+}
+
+template <bool LOCAL>
+__global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
+    int ps, int pt, double k, int ntgt, const int *target_boxes, const int *relatives,
+    int nboxes, const double *centers, int ncenters, double src_rscale, double tgt_rscale,
+    const double *src, int sbase, double *tgt, int tbase) {
+  __shared__ double Js[NSEQ];
+  __shared__ cplx ph[NSEQ];
+  __shared__ cplx S[2*PMAX + 1];
+  const int it = blockIdx.x;
+  if (it >= ntgt) return;
+  const int tb = target_boxes[it];
+  const int lane = threadIdx.x;
+  const int cj = lane - pt;
+  cplx acc = {0.0, 0.0};
+  // M2M: the (up to 4) children; L2L: the parent
+  const int nrel = LOCAL ? 1 : 4;
+  for (int r = 0; r < nrel; ++r) {
+    const int sb = LOCAL ? relatives[tb] : relatives[(long)r*nboxes + tb];
+    if (!LOCAL && sb == 0) continue;  // same on every lane
+    const double dx = centers[tb] - centers[sb], dy = centers[ncenters + tb] - centers[ncenters + sb];
+    shift_data(k, dx, dy, ps + pt, Js, ph);
+    if (lane < 2*ps + 1) {
+      const int am = lane > ps ? lane - ps : ps - lane;
+      const double s = LOCAL ? 1.0/ipow(src_rscale, am) : ipow(src_rscale, am);
+      S[lane] = cscale(load_c(src, (long)(sb - sbase)*(2*ps + 1) + lane), s);
+    }
+    __syncthreads();
+    if (lane < 2*pt + 1)
+      for (int mi = 0; mi < 2*ps + 1; ++mi)
+        acc = cfma(S[mi], shift_term(mi - ps - cj, Js, ph, LOCAL ? -1 : 1), acc);
+    __syncthreads();
+  }
+  if (lane < 2*pt + 1) {
+    const int aj = cj < 0 ? -cj : cj;
+    const double s = LOCAL ? ipow(tgt_rscale, aj) : 1.0/ipow(tgt_rscale, aj);
+    const long o = (long)(tb - tbase)*(2*pt + 1) + lane;
+    store_c(tgt, o, cadd(load_c(tgt, o), cscale(acc, s)));
+  }
+}
+
+// }}}
+
+// {{{ M2L
+
+// H_n(k|d|) e^{i n th_d}, n = -N..N, into D[n + N] (one thread)
+__device__ void hankel_row(double k, double dx, double dy, int N, cplx *D) {
+  double ux, uy;
+  const double dl = unit(dx, dy, ux, uy);
+  double J[NSEQ], Y[NSEQ];
+  bessel_jy(k*dl, N > 0 ? N : 1, J, Y);
+  cplx e = {1.0, 0.0};
+  for (int n = 0; n <= N; ++n) {
+    const cplx h = cmul(cplx{J[n], Y[n]}, e);
+    D[N + n] = h;
+    // H_-n e^{-i n th} = (-1)^n H_n conj(e)
+    const cplx hn = cmul(cplx{J[n], Y[n]}, conj(e));
+    D[N - n] = (n & 1) ? cscale(hn, -1.0) : hn;
+    e = cmul(e, cplx{ux, uy});
+  }
+}
+
+__global__ void h2d_m2l_data_kernel(int N, double k, int ncls, const double *vecs, int nvecs,
+                                    int cls_start, double *data) {
+  const int c = blockIdx.x*blockDim.x + threadIdx.x;
+  if (c >= ncls) return;
+  cplx D[2*NSEQ];
+  hankel_row(k, vecs[cls_start + c], vecs[nvecs + cls_start + c], N, D);
+  for (int n = 0; n < 2*N + 1; ++n) store_c(data, (long)c*(2*N + 1) + n, D[n]);
+}
+
+// one wave per target box; class data from `data` (rows by class - cls_start)
+// or, when data is null, the Hankel row of each pair made in LDS
+__global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
+    int ps, int pt, double k, int ntgt, const int *target_boxes, const int *starts,
+    const int *lists, const int *classes, int cls_start, const double *data,
+    const double *centers, int ncenters, double src_rscale, double tgt_rscale,
+    const double *src, int sbase, double *tgt, int tbase) {
+  __shared__ cplx Dl[2*NSEQ];
+  __shared__ cplx S[2*PMAX + 1];
+  const int it = blockIdx.x;
+  if (it >= ntgt) return;
+  const int tb = target_boxes[it];
+  const int lane = threadIdx.x;
+  const int N = ps + pt;
+  cplx acc = {0.0, 0.0};
+  for (int e = starts[it]; e < starts[it + 1]; ++e) {
+    const int sb = lists[e];
+    if (!data && lane == 0)
+      hankel_row(k, centers[tb] - centers[sb], centers[ncenters + tb] - centers[ncenters + sb],
+                 N, Dl);
+    if (lane < 2*ps + 1) {
+      const int am = lane > ps ? lane - ps : ps - lane;
+      S[lane] = cscale(load_c(src, (long)(sb - sbase)*(2*ps + 1) + lane), ipow(src_rscale, am));
+    }
+    __syncthreads();
+    if (lane < 2*pt + 1) {
+      const long row = data ? (long)(classes[e] - cls_start)*(2*N + 1) : 0;
+      for (int mi = 0; mi < 2*ps + 1; ++mi) {
+        const cplx d = data ? load_c(data, row + lane + mi) : Dl[lane + mi];
+        acc = cfma(d, S[mi], acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < 2*pt + 1) {
+    const int cj = lane - pt, aj = cj < 0 ? -cj : cj;
+    const double s = ipow(tgt_rscale, aj)*((aj & 1) ? -1.0 : 1.0);
+    store_c(tgt, (long)(tb - tbase)*(2*pt + 1) + lane, cscale(acc, s));
+  }
+}
+
+// }}}
+
+// {{{ L2P / M2P: one wave per target box, lanes over its targets
+
+// (i/4) z
+__device__ __forceinline__ cplx scale_i4(cplx z) { return {-0.25*z.im, 0.25*z.re}; }
+
+template <bool MPOLE>
+__global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
+    int p, double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const int *bts, const int *btc, const double *centers, int ncenters, const double *tx,
+    const double *ty, double rscale, const double *coeffs, int base, double *out) {
+  __shared__ cplx S[2*PMAX + 1];
+  const int it = blockIdx.x;
+  if (it >= ntgt) return;
+  const int tb = target_boxes[it];
+  const int lane = threadIdx.x;
+  const int t0 = bts[tb], nt = btc[tb];
+  const int e0 = MPOLE ? starts[it] : 0, e1 = MPOLE ? starts[it + 1] : 1;
+  double J[PMAX + 2], Y[PMAX + 2];
+  for (int c0 = 0; c0 < nt; c0 += WAVE) {
+    const int t = t0 + c0 + lane;
+    const bool active = c0 + lane < nt;
+    cplx acc = {0.0, 0.0};
+    for (int e = e0; e < e1; ++e) {
+      const int sb = MPOLE ? lists[e] : tb;
+      if (lane < 2*p + 1) {
+        const int ac = lane > p ? lane - p : p - lane;
+        const double s = MPOLE ? ipow(rscale, ac) : 1.0/ipow(rscale, ac);
+        S[lane] = cscale(load_c(coeffs, (long)(sb - base)*(2*p + 1) + lane), s);
+      }
+      __syncthreads();
+      if (active) {
+        double ux, uy;
+        const double rho = unit(tx[t] - centers[sb], ty[t] - centers[ncenters + sb], ux, uy);
+        // M2P: e^{i c th}, H_c; L2P: e^{-i c th}, J_c
+        const cplx e1u = {ux, MPOLE ? uy : -uy};
+        bessel_jy(k*rho, p > 0 ? p : 1, J, MPOLE ? Y : nullptr);
+        cplx ph = {1.0, 0.0};
+        for (int c = 0; c <= p; ++c) {
+          const cplx z = {J[c], MPOLE ? Y[c] : 0.0};
+          acc = cfma(S[p + c], cmul(z, ph), acc);
+          if (c > 0) {
+            const cplx tn = cmul(S[p - c], cmul(z, conj(ph)));
+            acc = (c & 1) ? cplx{acc.re - tn.re, acc.im - tn.im} : cadd(acc, tn);
+          }
+          ph = cmul(ph, e1u);
+        }
+      }
+      __syncthreads();
+    }
+    if (active) {
+      const cplx v = scale_i4(acc);
+      // M2P adds to the targets' values; L2P writes them
+      store_c(out, t, MPOLE ? cadd(load_c(out, t), v) : v);
+    }
+  }
+}
+
+// }}}
+
+// {{{ P2P
+
+// (i/4) H0(k r) for r > 0
+__device__ __forceinline__ cplx green(double k, double r) {
+  double J[2], Y[2];
+  bessel_jy(k*r, 1, J, Y);
+  return {-0.25*Y[0], 0.25*J[0]};
+}
+
+// one wave per target box, lanes over its targets; sources of list 1
+__global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
+    double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    const int *bss, const int *bsc, const int *bts, const int *btc, const double *tx,
+    const double *ty, const double *sx, const double *sy, const double *q, const int *t2s,
+    double *out) {
+  const int it = blockIdx.x;
+  if (it >= ntgt) return;
+  const int tb = target_boxes[it];
+  const int t0 = bts[tb], nt = btc[tb];
+  for (int i = threadIdx.x; i < nt; i += WAVE) {
+    const int t = t0 + i;
+    const double x = tx[t], y = ty[t];
+    const int self = t2s ? t2s[t] : -1;
+    cplx acc = {0.0, 0.0};
+    for (int e = starts[it]; e < starts[it + 1]; ++e) {
+      const int sb = lists[e];
+      const int s0 = bss[sb], s1 = s0 + bsc[sb];
+      for (int s = s0; s < s1; ++s) {
+        const double dx = x - sx[s], dy = y - sy[s];
+        const double r = sqrt(dx*dx + dy*dy);
+        if (r == 0.0 || s == self) continue;
+        acc = cfma(green(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+      }
+    }
+    store_c(out, t, acc);
+  }
+}
+
+__global__ void h2d_p2p_dense_kernel(double k, int nt, const double *tx, const double *ty,
+                                     int ns, const double *sx, const double *sy,
+                                     const double *q, const int *t2s, double *out) {
+  const int t = blockIdx.x*blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const double x = tx[t], y = ty[t];
+  const int self = t2s ? t2s[t] : -1;
+  cplx acc = {0.0, 0.0};
+  for (int s = 0; s < ns; ++s) {
+    const double dx = x - sx[s], dy = y - sy[s];
+    const double r = sqrt(dx*dx + dy*dy);
+    if (r == 0.0 || s == self) continue;
+    acc = cfma(green(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+  }
+  store_c(out, t, acc);
+}
+
+// }}}
+
+int check_orders(int ps, int pt) {
+  SUMPY_REQUIRE(ps >= 0 && ps <= PMAX && pt >= 0 && pt <= PMAX,
+                "h2d: orders must be in 0..%d (got %d, %d)", PMAX, ps, pt);
+  return SUMPYHIP_OK;
+}
+
+int check_k(double k) {
+  SUMPY_REQUIRE(k > 0.0 && std::isfinite(k), "h2d: the wavenumber must be real and > 0");
+  return SUMPYHIP_OK;
+}
+
+}  // namespace
+}  // namespace sumpyhip
+
+using namespace sumpyhip;
+
+#define H2D_CHECK(expr)              \
+  do {                               \
+    const int rc_ = (expr);          \
+    if (rc_ != SUMPYHIP_OK) return rc_; \
+  } while (0)
+
+extern "C" int sumpyhip_h2d_p2m(int order, double k, int nboxes, const int *source_boxes,
+                                const int *box_source_starts, const int *box_source_counts,
+                                const double *centers, int ncenters, const double *sx,
+                                const double *sy, const void *weights, double rscale,
+                                void *out, int base, void *stream) {
+  H2D_CHECK(check_orders(order, order));
+  H2D_CHECK(check_k(k));
+  if (nboxes == 0) return SUMPYHIP_OK;
+  h2d_p2e_kernel<false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  return check_launch("h2d_p2m");
+}
+
+extern "C" int sumpyhip_h2d_p2l(int order, double k, int ntgt, const int *target_boxes,
+                                const int *starts, const int *lists,
+                                const int *box_source_starts, const int *box_source_counts,
+                                const double *centers, int ncenters, const double *sx,
+                                const double *sy, const void *weights, double rscale,
+                                void *out, int base, void *stream) {
+  H2D_CHECK(check_orders(order, order));
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_p2e_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  return check_launch("h2d_p2l");
+}
+
+extern "C" int sumpyhip_h2d_m2m(int src_order, int tgt_order, double k, int ntgt,
+                                const int *target_boxes, const int *box_child_ids, int nboxes,
+                                const double *centers, int ncenters, double src_rscale,
+                                double tgt_rscale, const void *src, int src_base, void *tgt,
+                                int tgt_base, void *stream) {
+  H2D_CHECK(check_orders(src_order, tgt_order));
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_shift_kernel<false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  return check_launch("h2d_m2m");
+}
+
+extern "C" int sumpyhip_h2d_l2l(int src_order, int tgt_order, double k, int ntgt,
+                                const int *target_boxes, const int *box_parent_ids,
+                                const double *centers, int ncenters, double src_rscale,
+                                double tgt_rscale, const void *src, int src_base, void *tgt,
+                                int tgt_base, void *stream) {
+  H2D_CHECK(check_orders(src_order, tgt_order));
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_shift_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  return check_launch("h2d_l2l");
+}
+
+extern "C" int sumpyhip_h2d_m2l_data(int src_order, int tgt_order, double k, int ncls,
+                                     const double *vecs, int nvecs, int cls_start, void *data,
+                                     void *stream) {
+  H2D_CHECK(check_orders(src_order, tgt_order));
+  H2D_CHECK(check_k(k));
+  if (ncls == 0) return SUMPYHIP_OK;
+  SUMPY_REQUIRE(cls_start >= 0 && cls_start + ncls <= nvecs, "h2d_m2l_data: class range");
+  h2d_m2l_data_kernel<<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
+      src_order + tgt_order, k, ncls, vecs, nvecs, cls_start, (double *)data);
+  return check_launch("h2d_m2l_data");
+}
+
+extern "C" int sumpyhip_h2d_m2l(int src_order, int tgt_order, double k, int ntgt,
+                                const int *target_boxes, const int *starts, const int *lists,
+                                const int *classes, int cls_start, const void *data,
+                                const double *centers, int ncenters, double src_rscale,
+                                double tgt_rscale, const void *src, int src_base, void *tgt,
+                                int tgt_base, void *stream) {
+  H2D_CHECK(check_orders(src_order, tgt_order));
+  H2D_CHECK(check_k(k));
+  SUMPY_REQUIRE(!data || classes, "h2d_m2l: class data without the class list");
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_m2l_kernel<<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, starts, lists, classes, cls_start,
+      (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
+      src_base, (double *)tgt, tgt_base);
+  return check_launch("h2d_m2l");
+}
+
+extern "C" int sumpyhip_h2d_l2p(int order, double k, int ntgt, const int *target_boxes,
+                                const int *box_target_starts, const int *box_target_counts,
+                                const double *centers, int ncenters, const double *tx,
+                                const double *ty, double rscale, const void *loc, int base,
+                                void *out, void *stream) {
+  H2D_CHECK(check_orders(order, order));
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_e2p_kernel<false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
+  return check_launch("h2d_l2p");
+}
+
+extern "C" int sumpyhip_h2d_m2p(int order, double k, int ntgt, const int *target_boxes,
+                                const int *starts, const int *lists,
+                                const int *box_target_starts, const int *box_target_counts,
+                                const double *centers, int ncenters, const double *tx,
+                                const double *ty, double rscale, const void *mpoles, int base,
+                                void *out, void *stream) {
+  H2D_CHECK(check_orders(order, order));
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_e2p_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
+  return check_launch("h2d_m2p");
+}
+
+extern "C" int sumpyhip_h2d_p2p_csr(double k, int ntgt, const int *target_boxes,
+                                    const int *starts, const int *lists,
+                                    const int *box_source_starts, const int *box_source_counts,
+                                    const int *box_target_starts, const int *box_target_counts,
+                                    const double *tx, const double *ty, const double *sx,
+                                    const double *sy, const void *weights, const int *t2s,
+                                    void *out, void *stream) {
+  H2D_CHECK(check_k(k));
+  if (ntgt == 0) return SUMPYHIP_OK;
+  h2d_p2p_csr_kernel<<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
+      (double *)out);
+  return check_launch("h2d_p2p_csr");
+}
+
+extern "C" int sumpyhip_h2d_p2p_dense(double k, int ntargets, const double *tx,
+                                      const double *ty, int nsources, const double *sx,
+                                      const double *sy, const void *weights, const int *t2s,
+                                      void *out, void *stream) {
+  H2D_CHECK(check_k(k));
+  if (ntargets == 0) return SUMPYHIP_OK;
+  h2d_p2p_dense_kernel<<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
+      k, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+  return check_launch("h2d_p2p_dense");
+}

@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import torch
 
-from sumpy_amd import _capi
+from sumpy_amd import _capi, h2d
 from sumpy_amd.array_context import obj_array
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.tools import (
@@ -64,6 +64,8 @@ class E2PFromSingleBox(E2PBase):
     default_name = "e2p_from_single_box"
 
     def __call__(self, actx, **kwargs):
+        if h2d.is_h2d(self.expansion):
+            return h2d.l2p(self, actx, kwargs)
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
@@ -134,6 +136,8 @@ class E2PFromCSR(E2PBase):
     default_name = "e2p_from_csr"
 
     def __call__(self, actx, **kwargs):
+        if h2d.is_h2d(self.expansion):
+            return h2d.m2p(self, actx, kwargs)
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)

@@ -60,7 +60,49 @@ class LinearPDEConformingVolumeTaylorLocalExpansion(_VolumeTaylorLocalBase):
     compressed = True
 
 
+@dataclass(frozen=True)
+class FourierBesselLocalExpansionMixin(LocalExpansionBase):
+    """2D Bessel local (``local.py:522-651``): coefficients ``c = -p..p`` at
+    storage index ``c + p``; M2L by :class:`~sumpy_amd.expansion.m2l.FourierBesselM2LTranslation`."""
+
+    m2l_translation_override: object = field(default=None, kw_only=True)
+
+    def get_coefficient_identifiers(self):
+        return [(i,) for i in range(-self.order, self.order + 1)]
+
+    def get_storage_index(self, mi):
+        ind, = mi
+        return self.order + ind
+
+    @property
+    def m2l_translation(self):
+        if self.m2l_translation_override is not None:
+            return self.m2l_translation_override
+        from sumpy_amd.expansion.m2l import DefaultM2LTranslationClassFactory
+        return DefaultM2LTranslationClassFactory().get_m2l_translation_class(
+            self.kernel, type(self))()
+
+
+@dataclass(frozen=True)
+class H2DLocalExpansion(FourierBesselLocalExpansionMixin):
+    """For the 2D Helmholtz kernel (``local.py:654-682``)."""
+
+    def __post_init__(self):
+        from sumpy_amd.kernel import HelmholtzKernel
+        base = self.kernel.get_base_kernel()
+        if not (isinstance(base, HelmholtzKernel) and base.dim == 2):
+            raise TypeError(f"{type(self).__name__} can only be applied to 2D "
+                            f"HelmholtzKernel: {self.kernel!r}")
+
+    @property
+    def mpole_expn_class(self):
+        from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
+        return H2DMultipoleExpansion
+
+
 __all__ = [
+    "FourierBesselLocalExpansionMixin",
+    "H2DLocalExpansion",
     "LineTaylorLocalExpansion",
     "LinearPDEConformingVolumeTaylorLocalExpansion",
     "LocalExpansionBase",

@@ -90,19 +90,71 @@ def complex_rows(expansion):
     return (3, 2) if expansion.is_complex_valued else (1, 1)
 
 
+class FourierBesselM2LTranslation(M2LTranslationBase):
+    """M2L of the 2D Bessel locals (reference ``m2l.py:1008-1115``): the
+    class data are the Hankel row ``H_n(k|d|) e^{i n th_d}``,
+    ``n = -(ps+pt)..(ps+pt)`` (``2 (ps + pt) + 1`` complex values, stored
+    as FP64 (re, im) pairs); the multipole pre-scaling ``rs^|m|`` and the
+    local post-scaling ``(-1)^j rt^|j|`` are folded into the contraction
+    kernel, so no separate pre/postprocessing arrays are used."""
+
+    use_fft = False
+    use_preprocessing = False
+    precision = "fp64"
+
+    def preprocessed_dtype(self, dtype):
+        return np.dtype(dtype)
+
+    def translation_classes_dependent_ndata(self, tgt_expansion, src_expansion):
+        return 2*(2*tgt_expansion.order + 2*src_expansion.order + 1)
+
+    def preprocess_multipole_nexprs(self, tgt_expansion, src_expansion):
+        return 2*src_expansion.order + 1
+
+    def postprocess_local_nexprs(self, tgt_expansion, src_expansion):
+        return 2*tgt_expansion.order + 1
+
+
+class FourierBesselM2LWithFFT(FourierBesselM2LTranslation):
+    """Not supported, as in the reference (``m2l.py:1199-1215``)."""
+
+    use_fft = True
+
+    def __init__(self):
+        raise ValueError("Bessel based expansions with FFT are not supported yet.")
+
+
+def _is_fourier_bessel(local_expansion_class):
+    from sumpy_amd.expansion.local import FourierBesselLocalExpansionMixin
+    return isinstance(local_expansion_class, type) and \
+        issubclass(local_expansion_class, FourierBesselLocalExpansionMixin)
+
+
 class M2LTranslationClassFactoryBase:
+    """Volume-Taylor locals: the dense class-matrix M2L; 2D Bessel locals:
+    :class:`FourierBesselM2LTranslation` (reference ``m2l.py:100-173``)."""
+
+    fourier_bessel = FourierBesselM2LTranslation
+
     def get_m2l_translation_class(self, base_kernel, local_expansion_class):
+        if _is_fourier_bessel(local_expansion_class):
+            return self.fourier_bessel
         return VolumeTaylorM2LWithDenseClassMatrices
+
+
+class FFTM2LTranslationClassFactory(M2LTranslationClassFactoryBase):
+    fourier_bessel = FourierBesselM2LWithFFT
 
 
 DefaultM2LTranslationClassFactory = M2LTranslationClassFactoryBase
 NonFFTM2LTranslationClassFactory = M2LTranslationClassFactoryBase
-FFTM2LTranslationClassFactory = M2LTranslationClassFactoryBase
 
 __all__ = [
     "DefaultM2LTranslationClassFactory",
     "complex_rows",
     "FFTM2LTranslationClassFactory",
+    "FourierBesselM2LTranslation",
+    "FourierBesselM2LWithFFT",
     "M2LTranslationBase",
     "NonFFTM2LTranslationClassFactory",
     "VolumeTaylorM2LWithDenseClassMatrices",

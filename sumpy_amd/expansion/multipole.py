@@ -32,7 +32,35 @@ class LinearPDEConformingVolumeTaylorMultipoleExpansion(
     compressed = True
 
 
+@dataclass(frozen=True)
+class HankelBased2DMultipoleExpansion(MultipoleExpansionBase):
+    """2D Hankel multipole (``multipole.py:424-497``): coefficients
+    ``c = -p..p`` at storage index ``c + p``; the operators run the HIP
+    kernels of ``csrc/h2d.hip`` (:mod:`sumpy_amd.h2d`)."""
+
+    def get_coefficient_identifiers(self):
+        return [(i,) for i in range(-self.order, self.order + 1)]
+
+    def get_storage_index(self, mi):
+        ind, = mi
+        return self.order + ind
+
+
+@dataclass(frozen=True)
+class H2DMultipoleExpansion(HankelBased2DMultipoleExpansion):
+    """For the 2D Helmholtz kernel (``multipole.py:532-550``)."""
+
+    def __post_init__(self):
+        from sumpy_amd.kernel import HelmholtzKernel
+        base = self.kernel.get_base_kernel()
+        if not (isinstance(base, HelmholtzKernel) and base.dim == 2):
+            raise TypeError(f"{type(self).__name__} can only be applied to 2D "
+                            f"HelmholtzKernel: {self.kernel!r}")
+
+
 __all__ = [
+    "H2DMultipoleExpansion",
+    "HankelBased2DMultipoleExpansion",
     "LinearPDEConformingVolumeTaylorMultipoleExpansion",
     "MultipoleExpansionBase",
     "VolumeTaylorMultipoleExpansion",

@@ -522,7 +522,10 @@ class SumpyExpansionWrangler:
         from sumpy_amd.e2e import m2l_apply_levels
         local_exps = self.local_expansion_zeros(actx)
         starts = _host(level_start_target_box_nrs)
-        use_pre = self.supports_translation_classes
+        # translations without preprocessing (the 2D Bessel locals) take the
+        # classes' data in the per-level M2L operator itself
+        use_pre = (self.supports_translation_classes
+                   and self.tree_indep.m2l_translation.use_preprocessing)
         if use_pre:
             pre_mpoles = self.m2l_preproc_mpole_expansion_zeros(actx, mpole_exps)
             work = self.m2l_work_array_zeros(actx, local_exps)

@@ -77,25 +77,27 @@ class LaplaceKernel(ScalarKernel):
 
 @dataclass(frozen=True)
 class HelmholtzKernel(ScalarKernel):
-    """``G = exp(i k |d|)/|d|`` (3D) with global scaling ``1/(4 pi)``
-    (reference ``kernel.py:754-844``).  ``k`` is passed at call time under
-    :attr:`helmholtz_k_name`; it is real unless *allow_evanescent*."""
+    """``G = exp(i k |d|)/|d|`` (3D) with global scaling ``1/(4 pi)``, or
+    ``G = H0^(1)(k |d|)`` (2D) with scaling ``i/4`` (reference
+    ``kernel.py:754-844``).  ``k`` is passed at call time under
+    :attr:`helmholtz_k_name`; it is real unless *allow_evanescent*.  The 2D
+    kernel runs with the H2D expansions and its own P2P
+    (:mod:`sumpy_amd.h2d`)."""
 
     dim: int
     helmholtz_k_name: str = "k"
     allow_evanescent: bool = False
 
     def __post_init__(self):
-        if self.dim != 3:
-            raise NotImplementedError(
-                "only the 3D Helmholtz kernel is implemented on the GPU")
+        if self.dim not in (2, 3):
+            raise NotImplementedError(f"unsupported dimension: '{self.dim}'")
 
     @property
     def is_complex_valued(self) -> bool:
         return True
 
-    def get_global_scaling_const(self) -> float:
-        return 1.0 / (4.0 * math.pi)
+    def get_global_scaling_const(self):
+        return 1.0 / (4.0 * math.pi) if self.dim == 3 else 0.25j
 
     def get_args(self):
         return [self.helmholtz_k_name]
@@ -299,7 +301,7 @@ def kernel_id(knl: ScalarKernel) -> int:
     if isinstance(base, LaplaceKernel):
         return 3 if base.dim == 3 else 2
     if isinstance(base, HelmholtzKernel):
-        return 13
+        return 13 if base.dim == 3 else 12
     raise NotImplementedError(f"unsupported kernel: {knl}")
 
 

@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import torch
 
-from sumpy_amd import _capi
+from sumpy_amd import _capi, h2d
 from sumpy_amd.array_context import obj_array
 from sumpy_amd.kernel import (
     kernel_id,
@@ -56,6 +56,10 @@ class P2PBase(KernelComputation):
         self.precision = precision
         self.base_kernel = next(iter(base_src))
 
+    def _h2d_precision(self):
+        if self.precision != "fp64":
+            raise NotImplementedError("the 2D Helmholtz P2P runs in FP64 only")
+
     def get_cache_key(self):
         return (type(self).__name__, tuple(self.target_kernels), self.exclude_self,
                 tuple(self.strength_usage), tuple(self.value_dtypes),
@@ -87,6 +91,10 @@ class P2P(P2PBase):
         tgt, src, weights, t2s, k = self._prepare(actx, targets, sources, strength, kwargs)
         nt, ns = tgt[0].shape[0], src[0].shape[0]
         results = new_results(actx, nt, self.value_dtypes)
+        if h2d.is_h2d_kernel(self.base_kernel):
+            self._h2d_precision()
+            return h2d.p2p(actx, self.base_kernel, kwargs, tgt, src, weights, t2s,
+                           self.target_kernels, results)
         outs = TargetOutputs(self.target_kernels, self.dim)
         dtype = results[0].dtype
         out_ptrs = outs.buffers(list(results), nt, dtype, actx.device)
@@ -118,6 +126,13 @@ class P2PFromCSR(P2PBase):
         nt = tgt[0].shape[0]
         if result is None:
             result = new_results(actx, nt, self.value_dtypes)
+        if h2d.is_h2d_kernel(self.base_kernel):
+            self._h2d_precision()
+            return h2d.p2p(actx, self.base_kernel, kwargs, tgt, src, weights, t2s,
+                           self.target_kernels, result,
+                           csr=(target_boxes, source_box_starts, source_box_lists,
+                                box_source_starts, box_source_counts_nonchild,
+                                box_target_starts, box_target_counts_nonchild))
         outs = TargetOutputs(self.target_kernels, self.dim)
         out_ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
         tp = [t.data_ptr() for t in tgt] + [None]*(3 - self.dim)

@@ -224,3 +224,83 @@ def _oracle_fmm_helm(tree, trav, weights, order, exclude_self, k):
                        np.ascontiguousarray(lv.imag), int(lbs[lev]), lim, k=k)
     pot = pot + lre + 1j*lim
     return pot[th["sorted_target_ids"]]
+
+
+def oracle_fmm_h2d(tree, trav, weights, order, k, exclude_self=False):
+    """The 2D Helmholtz FMM with the H2D (Hankel/Bessel) expansions, every
+    operator from ``oracle.h2d`` (scipy's Bessel functions), per box, in
+    the stage order of ``drive_fmm``; the M2L through the translation-class
+    Hankel rows (``sumpy/expansion/m2l.py:1008-1115``)."""
+    from oracle import h2d
+    th, vh = tree.host, trav.host
+    nlev = tree.nlevels
+    q = np.asarray(weights, np.complex128)[th["user_source_ids"]]
+    C = th["box_centers"]
+    S, T = th["sources"], th["targets"]
+    nboxes = len(th["box_levels"])
+    lev_of = th["box_levels"]
+    bss, bsc = th["box_source_starts"], th["box_source_counts_nonchild"]
+    bts, btc = th["box_target_starts"], th["box_target_counts_nonchild"]
+    ncoef = 2*order + 1
+
+    def rs(box):
+        return rscale(tree, int(lev_of[box]), order)
+
+    def srcs(b):
+        return slice(int(bss[b]), int(bss[b] + bsc[b]))
+
+    def tgts(b):
+        return slice(int(bts[b]), int(bts[b] + btc[b]))
+
+    mp = np.zeros((nboxes, ncoef), np.complex128)
+    for b in vh["source_boxes"]:
+        mp[b] = h2d.p2m(order, k, C[:, b], S[:, srcs(b)], q[srcs(b)], rs(b))
+    lsp = vh["level_start_source_parent_box_nrs"]
+    for sl in range(nlev - 1, 2, -1):
+        for pb in vh["source_parent_boxes"][lsp[sl - 1]:lsp[sl]]:
+            for ch in th["box_child_ids"][:, pb]:
+                if ch:
+                    mp[pb] += h2d.m2m(order, order, k, mp[ch], C[:, ch], C[:, pb], rs(ch),
+                                      rs(pb))
+    nt = T.shape[1]
+    pot = np.zeros(nt, np.complex128)
+    st1, li1 = vh["neighbor_source_boxes_starts"], vh["neighbor_source_boxes_lists"]
+    for i, tb in enumerate(vh["target_boxes"]):
+        sl = np.concatenate([np.arange(bss[s], bss[s] + bsc[s]) for s in li1[st1[i]:st1[i + 1]]]
+                            + [np.zeros(0, np.int64)]).astype(np.int64)
+        tt = np.arange(bts[tb], bts[tb] + btc[tb])
+        g = np.zeros((len(tt), len(sl)), np.complex128)
+        if len(sl):
+            r = np.hypot(T[0, tt][:, None] - S[0, sl][None, :], T[1, tt][:, None] - S[1, sl][None, :])
+            nz = r > 0
+            if exclude_self:
+                nz &= tt[:, None] != sl[None, :]
+            from scipy import special
+            g[nz] = h2d.SCALE*special.hankel1(0, k*r[nz])
+        pot[tt] = g @ q[sl]
+    loc = np.zeros((nboxes, ncoef), np.complex128)
+    totp = vh["target_or_target_parent_boxes"]
+    st2, li2 = vh["from_sep_siblings_starts"], vh["from_sep_siblings_lists"]
+    cls = vh["from_sep_siblings_translation_classes"]
+    vecs = vh["from_sep_siblings_translation_class_to_distance_vector"]
+    for i, tb in enumerate(totp):
+        for e in range(st2[i], st2[i + 1]):
+            sb = li2[e]
+            data = h2d.m2l_class_data(order, order, k, vecs[:, cls[e]])
+            loc[tb] += h2d.m2l(order, order, k, mp[sb], None, rs(sb), rs(tb), data=data)
+    for tbs, st, li in vh["sep_smaller"]:
+        for i, tb in enumerate(tbs):
+            for sb in li[st[i]:st[i + 1]]:
+                pot[tgts(tb)] += h2d.m2p(order, k, C[:, sb], mp[sb], T[:, tgts(tb)], rs(sb))
+    st4, li4 = vh["from_sep_bigger_starts"], vh["from_sep_bigger_lists"]
+    for i, tb in enumerate(totp):
+        for sb in li4[st4[i]:st4[i + 1]]:
+            loc[tb] += h2d.p2l(order, k, C[:, tb], S[:, srcs(sb)], q[srcs(sb)], rs(tb))
+    lst = vh["level_start_target_or_target_parent_box_nrs"]
+    for tl in range(1, nlev):
+        for tb in totp[lst[tl]:lst[tl + 1]]:
+            pb = th["box_parent_ids"][tb]
+            loc[tb] += h2d.l2l(order, order, k, loc[pb], C[:, pb], C[:, tb], rs(pb), rs(tb))
+    for tb in vh["target_boxes"]:
+        pot[tgts(tb)] += h2d.l2p(order, k, C[:, tb], loc[tb], T[:, tgts(tb)], rs(tb))
+    return pot[th["sorted_target_ids"]]

@@ -1,0 +1,231 @@
+"""2D Helmholtz (H2D Hankel/Bessel expansions) on the GPU against the oracle.
+
+* The dense P2P equals ``oracle.h2d.p2p`` (scipy's H0) to 1e-13 relative,
+  also at k |d| up to ~60 (the device's Miller/Neumann Bessel routine
+  against scipy).
+* The expansion chains of the reference's ``test_translations``
+  (``sumpy/test/test_kernels.py:493-640``) through the wrangler's operators
+  equal the oracle's chains to 1e-12.
+* The full FMM on the reference's ``test_sumpy_fmm`` H2D setup
+  (``sumpy/test/test_fmm.py:78-299``: 1000 normal sources, 300 targets
+  offset by 0.1, weights ``default_rng(44).random``, 30 per box, k = 0.05,
+  orders 4 and 5, with and without translation classes) equals the
+  oracle-driven FMM (``tests/oracle_fmm.py:oracle_fmm_h2d``) to 1e-12 and
+  converges to the direct sum.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import h2d as oh
+from oracle_fmm import oracle_fmm_h2d
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b)/np.linalg.norm(b))
+
+
+def _knl():
+    from sumpy_amd.kernel import HelmholtzKernel
+    return HelmholtzKernel(2)
+
+
+@pytest.mark.parametrize("k", [0.05, 1.0, 15.0])
+def test_h2d_p2p_dense_matches_oracle(actx, k):
+    from sumpy_amd import P2P
+    rng = np.random.default_rng(5)
+    src, tgt = 4*rng.random((2, 700)), 4*rng.random((2, 300))
+    q = rng.standard_normal(700) + 1j*rng.standard_normal(700)
+    pot, = P2P([_knl()], exclude_self=False)(actx, actx.from_numpy(tgt), actx.from_numpy(src),
+                                              [actx.from_numpy(q)], k=k)
+    err = rel(actx.to_numpy(pot), oh.p2p(k, tgt, src, q))
+    print(f"k={k}: P2P vs oracle {err:.2e}")
+    assert err < 1e-13
+
+
+def test_h2d_p2p_exclude_self(actx):
+    from sumpy_amd import P2P
+    rng = np.random.default_rng(6)
+    pts = rng.random((2, 400))
+    q = rng.random(400)
+    t2s = np.arange(400, dtype=np.int32)
+    pot, = P2P([_knl()], exclude_self=True)(
+        actx, actx.from_numpy(pts), actx.from_numpy(pts), [actx.from_numpy(q)], k=2.0,
+        target_to_source=actx.from_numpy(t2s))
+    assert rel(actx.to_numpy(pot), oh.p2p(2.0, pts, pts, q, exclude_self_t2s=t2s)) < 1e-13
+
+
+def _chain_geometry():
+    rng = np.random.default_rng(18)
+    origin = np.array([2.0, 1.0])
+    src = 0.7*(-0.5 + rng.random((2, 15))) + origin[:, None]
+    g = np.linspace(-0.15, 0.15, 20)
+    off = np.array([5.5, 0.0])
+    tx, ty = np.meshgrid(g + off[0] + origin[0], g + off[1] + origin[1], indexing="ij")
+    tgt = np.array([tx.ravel(), ty.ravel()])
+    centers = np.array([[0, 0], [-0.2, 0.1], off + [0.3, -0.2], off]).T + origin[:, None]
+    return src, np.ones(15)/15, tgt, centers
+
+
+@pytest.mark.parametrize(("order", "k"), [(3, 0.05), (8, 0.05), (20, 1.3)])
+def test_h2d_operator_chain_matches_oracle(actx, order, k):
+    """Boxes 0..3 = the four centres; box 0 holds the sources, box 3 the
+    targets; P2M(0) -> M2M(0->1) -> M2L(1->2) -> L2L(2->3) -> L2P(3), plus
+    M2P(1) and P2L(3), each through the operators' public calls."""
+    from sumpy_amd import (
+        E2EFromChildren,
+        E2EFromCSR,
+        E2EFromParent,
+        E2PFromCSR,
+        E2PFromSingleBox,
+        P2EFromCSR,
+        P2EFromSingleBox,
+    )
+    from sumpy_amd.expansion.local import H2DLocalExpansion
+    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
+    knl = _knl()
+    mpole, local = H2DMultipoleExpansion(knl, order), H2DLocalExpansion(knl, order)
+    src, q, tgt, c = _chain_geometry()
+    rs = (0.5, 0.25, 0.5, 0.25)
+    dev = actx.from_numpy
+    i32 = lambda a: dev(np.asarray(a, np.int32))  # noqa: E731
+    nb = 4
+    centers = dev(np.ascontiguousarray(c))
+    srcd, tgtd, qd = dev(src), dev(tgt), dev(q.astype(np.complex128))
+    bss, bsc = i32([0, 15, 15, 15]), i32([15, 0, 0, 0])
+    bts, btc = i32([0, 0, 0, 0]), i32([0, 0, 0, 400])
+    ncoef = 2*order + 1
+
+    def zeros():
+        return torch.zeros((nb, ncoef), dtype=torch.complex128, device=actx.device)
+
+    M = zeros()
+    P2EFromSingleBox(mpole)(actx, sources=srcd, centers=centers, strengths=[qd],
+                            tgt_expansions=M, tgt_base_ibox=0, rscale=rs[0], source_boxes=i32([0]),
+                            box_source_starts=bss, box_source_counts_nonchild=bsc, k=k)
+    # M2M with box 1 as the parent of box 0: a child id 0 means "no child",
+    # so the M2M input is a copy with box 0's multipole and centre in row 2
+    child = np.zeros((4, nb), np.int32)
+    child[0, 1] = 2
+    Msrc = zeros()
+    Msrc[2] = M[0]
+    cen2 = c.copy()
+    cen2[:, 2] = c[:, 0]
+    M2 = zeros()
+    E2EFromChildren(mpole, mpole)(actx, src_expansions=Msrc, src_base_ibox=0,
+                                  tgt_expansions=M2, tgt_base_ibox=0, target_boxes=i32([1]),
+                                  box_child_ids=dev(child), centers=dev(np.ascontiguousarray(cen2)),
+                                  src_rscale=rs[0], tgt_rscale=rs[1], k=k)
+    L = zeros()
+    E2EFromCSR(mpole, local)(actx, src_expansions=M2, src_base_ibox=0, tgt_expansions=L,
+                             tgt_base_ibox=0, target_boxes=i32([2]), src_box_starts=i32([0, 1]),
+                             src_box_lists=i32([1]), centers=centers, src_rscale=rs[1],
+                             tgt_rscale=rs[2], k=k)
+    L2 = zeros()
+    E2EFromParent(local, local)(actx, src_expansions=L, src_base_ibox=0, tgt_expansions=L2,
+                                tgt_base_ibox=0, target_boxes=i32([3]),
+                                box_parent_ids=i32([0, 0, 0, 2]), centers=centers,
+                                src_rscale=rs[2], tgt_rscale=rs[3], k=k)
+    pot_l2p, = E2PFromSingleBox(local, [knl])(
+        actx, src_expansions=L2, src_base_ibox=0, target_boxes=i32([3]), centers=centers,
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[3], k=k)
+    pot_m2p, = E2PFromCSR(mpole, [knl])(
+        actx, src_expansions=M2, src_base_ibox=0, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([1]), centers=centers,
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[1], k=k)
+    L4 = zeros()
+    P2EFromCSR(local)(actx, sources=srcd, centers=centers, strengths=[qd], tgt_expansions=L4,
+                      tgt_base_ibox=0, rscale=rs[3], target_boxes=i32([3]),
+                      source_box_starts=i32([0, 1]), source_box_lists=i32([0]),
+                      box_source_starts=bss, box_source_counts_nonchild=bsc, k=k)
+
+    m1 = oh.p2m(order, k, c[:, 0], src, q, rs[0])
+    m2 = oh.m2m(order, order, k, m1, c[:, 0], c[:, 1], rs[0], rs[1])
+    l1 = oh.m2l(order, order, k, m2, c[:, 2] - c[:, 1], rs[1], rs[2])
+    l2 = oh.l2l(order, order, k, l1, c[:, 2], c[:, 3], rs[2], rs[3])
+    l4 = oh.p2l(order, k, c[:, 3], src, q, rs[3])
+    got = {n: actx.to_numpy(v) for n, v in (("p2m", M[0]), ("m2m", M2[1]), ("m2l", L[2]),
+                                             ("l2l", L2[3]), ("p2l", L4[3]))}
+    want = {"p2m": m1, "m2m": m2, "m2l": l1, "l2l": l2, "p2l": l4}
+    for n in want:
+        e = rel(got[n], want[n])
+        print(f"p={order} k={k} {n}: {e:.2e}")
+        assert e < 1e-12, n
+    assert rel(actx.to_numpy(pot_l2p), oh.l2p(order, k, c[:, 3], l2, tgt, rs[3])) < 1e-12
+    assert rel(actx.to_numpy(pot_m2p), oh.m2p(order, k, c[:, 1], m2, tgt, rs[1])) < 1e-12
+
+
+def _fmm_case():
+    src = np.random.default_rng(15).standard_normal((2, 1000))
+    tgt = np.array([[0.1], [0.0]]) + np.random.default_rng(18).standard_normal((2, 300))
+    q = np.random.default_rng(44).random(1000)
+    return src, tgt, q
+
+
+def _run_fmm(actx, tree, trav, q, order, k, use_classes):
+    from functools import partial
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import H2DLocalExpansion
+    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    knl = _knl()
+    ti = SumpyTreeIndependentDataForWrangler(actx, partial(H2DMultipoleExpansion, knl),
+                                             partial(H2DLocalExpansion, knl), [knl])
+    wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: order,
+                                kernel_extra_kwargs={"k": k},
+                                _disable_translation_classes=not use_classes)
+    pot, = drive_fmm(actx, wr, [actx.from_numpy(q.astype(np.complex128))])
+    return actx.to_numpy(pot)
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_h2d_fmm_matches_oracle_fmm_and_converges(actx, use_classes):
+    from sumpy_amd.tree import build_tree_and_traversal
+    src, tgt, q = _fmm_case()
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    direct = oh.p2p(0.05, tgt, src, q)
+    errs = []
+    for order in (4, 5):
+        pot = _run_fmm(actx, tree, trav, q, order, 0.05, use_classes)
+        ref = oracle_fmm_h2d(tree, trav, q, order, 0.05)
+        e_or = rel(pot, ref)
+        errs.append(float(np.max(np.abs(pot - direct))/np.max(np.abs(direct))))
+        print(f"classes={use_classes} p={order}: vs oracle FMM {e_or:.2e}, "
+              f"rel-inf vs direct {errs[-1]:.2e}")
+        assert e_or < 1e-12
+    assert errs[1] < errs[0] < 1e-3
+
+
+def test_h2d_fmm_sources_are_targets_exclude_self_high_order(actx):
+    """Sources = targets, self excluded, order 16, k = 6 (k h up to ~6 at
+    the top levels), against the oracle FMM and the direct sum."""
+    from functools import partial
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import H2DLocalExpansion
+    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.tree import build_tree_and_traversal
+    rng = np.random.default_rng(2)
+    pts = rng.random((2, 3000))
+    q = rng.standard_normal(3000) + 1j*rng.standard_normal(3000)
+    tree, trav = build_tree_and_traversal(actx, pts, None, 40)
+    knl = _knl()
+    ti = SumpyTreeIndependentDataForWrangler(actx, partial(H2DMultipoleExpansion, knl),
+                                             partial(H2DLocalExpansion, knl), [knl],
+                                             exclude_self=True)
+    t2s = np.arange(3000, dtype=np.int32)
+    wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: 16,
+                                kernel_extra_kwargs={"k": 6.0},
+                                self_extra_kwargs={"target_to_source": actx.from_numpy(t2s)})
+    pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+    pot = actx.to_numpy(pot)
+    ref = oracle_fmm_h2d(tree, trav, q, 16, 6.0, exclude_self=True)
+    direct = oh.p2p(6.0, pts, pts, q, exclude_self_t2s=t2s)
+    print(f"p=16 k=6: vs oracle FMM {rel(pot, ref):.2e}, vs direct {rel(pot, direct):.2e}")
+    assert rel(pot, ref) < 1e-11
+    assert rel(pot, direct) < 1e-8
