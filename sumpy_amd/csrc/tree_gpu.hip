@@ -274,7 +274,10 @@ __global__ void list2_kernel(TreeView t, int nb, const int *boxes, const int *co
             for (int d = 0; d < t.dim; ++d)
               slot = slot*7 + (int)(t.ix[(size_t)d*t.nboxes + b] - t.ix[(size_t)d*t.nboxes + ch] + 3);
             slot_out[e] = slot;
-            atomicOr(&used[(size_t)l*nslots + slot], 1);
+            // most pairs find their slot already flagged: read before the
+            // atomic (a stale 0 only costs one more atomic)
+            int *u = &used[(size_t)l*nslots + slot];
+            if (*u == 0) atomicOr(u, 1);
           }
           ++n;
         }

@@ -50,6 +50,23 @@ def _level_starts(levels_of_boxes, nlevels):
     return _excl(cnt[:nlevels])
 
 
+def _bounds(x):
+    """(2, dim): per-axis minimum and maximum of the points *x* (dim, n); in
+    two stages over rows of 1024, which keeps the reduction wide."""
+    dim, n = x.shape
+    m = n//1024*1024
+    parts = []
+    if m:
+        h = x[:, :m].reshape(dim, -1, 1024)
+        parts.append(torch.stack([h.amin(2).amin(1), h.amax(2).amax(1)]))
+    if m < n:
+        t = x[:, m:]
+        parts.append(torch.stack([t.amin(1), t.amax(1)]))
+    b = parts[0] if len(parts) == 1 else torch.stack(
+        [torch.minimum(parts[0][0], parts[1][0]), torch.maximum(parts[0][1], parts[1][1])])
+    return b
+
+
 class _Timer:
     def __init__(self, sync):
         self.sync, self.marks, self.t0 = sync, {}, time.perf_counter()
@@ -86,8 +103,8 @@ def build_device(actx, sources, targets=None, max_particles_in_box=30, max_level
     nch = 1 << dim
 
     # {{{ bounding box and keys
-    pts = src if same else torch.cat([src, tgt], 1)
-    lohi = torch.stack([pts.amin(1), pts.amax(1)]).cpu().numpy()
+    lohi = torch.stack([_bounds(src)] if same else [_bounds(src), _bounds(tgt)])
+    lohi = torch.stack([lohi[:, 0].amin(0), lohi[:, 1].amax(0)]).cpu().numpy()
     lo, hi = lohi[0], lohi[1]
     ext = 0.0
     for d in range(dim):
