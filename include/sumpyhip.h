@@ -485,28 +485,29 @@ int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *
 
 /* }}} */
 
-/* {{{ Helmholtz-2D Hankel/Bessel expansions (H2D), G = (i/4) H0^(1)(k|d|)
+/* {{{ 2D Hankel/Bessel expansions: Helmholtz (H2D, kind 0, G = (i/4) H0^(1)(k|d|))
+ * and Yukawa (Y2D, kind 1, G = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi))
 
- * The reference's H2DMultipoleExpansion / H2DLocalExpansion
- * (sumpy/expansion/multipole.py:426-550, sumpy/expansion/local.py:522-682)
+ * The reference's H2D / Y2D multipole and local expansions
+ * (sumpy/expansion/multipole.py:426-567, sumpy/expansion/local.py:522-704)
  * with the Fourier-Bessel M2L (sumpy/expansion/m2l.py:1008-1115), behind the
  * same operators as the 3D expansions (P2EFromSingleBox, P2EFromCSR,
  * E2EFromChildren, E2EFromParent, E2EFromCSR /
  * M2LUsingTranslationClassesDependentData, E2PFromSingleBox, E2PFromCSR,
- * P2PFromCSR, P2P: sumpy/p2e.py, e2e.py, e2p.py, p2p.py).  Real k > 0, orders
- * 0..31, FP64.  Expansions, weights and outputs are complex128 (interleaved
+ * P2PFromCSR, P2P: sumpy/p2e.py, e2e.py, e2p.py, p2p.py).  `k` is the Helmholtz
+ * k or the Yukawa lambda (real, > 0), orders 0..31, FP64.  Expansions, weights and outputs are complex128 (interleaved
  * re, im); an expansion row holds the 2p+1 coefficients c = -p..p at c + p.
  * Coordinates are SoA; centers is 2 x ncenters.  The evaluations include the
  * kernel's i/4; M2P adds to `out`, L2P and the P2P forms write it. */
 
 /* P2M of each source box from its own sources (rows box - base of out). */
-int sumpyhip_h2d_p2m(int order, double k, int nboxes, const int *source_boxes,
+int sumpyhip_h2d_p2m(int kind, int order, double k, int nboxes, const int *source_boxes,
                      const int *box_source_starts, const int *box_source_counts,
                      const double *centers, int ncenters, const double *sx, const double *sy,
                      const void *weights, double rscale, void *out, int base, void *stream);
 
 /* P2L of each target box from the sources of its list-4 boxes (CSR). */
-int sumpyhip_h2d_p2l(int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_p2l(int kind, int order, double k, int ntgt, const int *target_boxes,
                      const int *starts, const int *lists, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
                      const double *sx, const double *sy, const void *weights, double rscale,
@@ -514,14 +515,14 @@ int sumpyhip_h2d_p2l(int order, double k, int ntgt, const int *target_boxes,
 
 /* M2M: each target box's children (box_child_ids, 4 x nboxes, 0 = none)
  * translated and added to its row. */
-int sumpyhip_h2d_m2m(int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, int ntgt,
                      const int *target_boxes, const int *box_child_ids, int nboxes,
                      const double *centers, int ncenters, double src_rscale,
                      double tgt_rscale, const void *src, int src_base, void *tgt,
                      int tgt_base, void *stream);
 
 /* L2L: each target box's parent local translated and added to its row. */
-int sumpyhip_h2d_l2l(int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, int ntgt,
                      const int *target_boxes, const int *box_parent_ids,
                      const double *centers, int ncenters, double src_rscale,
                      double tgt_rscale, const void *src, int src_base, void *tgt,
@@ -530,14 +531,14 @@ int sumpyhip_h2d_l2l(int src_order, int tgt_order, double k, int ntgt,
 /* Translation-class data of ncls classes (distance vectors vecs, 2 x nvecs,
  * from column cls_start): row c holds H_n(k|d|) e^{i n th_d},
  * n = -(src+tgt)..(src+tgt), as complex128. */
-int sumpyhip_h2d_m2l_data(int src_order, int tgt_order, double k, int ncls,
+int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, int ncls,
                           const double *vecs, int nvecs, int cls_start, void *data,
                           void *stream);
 
 /* M2L over list 2 (CSR); with `data` (rows by classes[e] - cls_start) or,
  * when data is NULL, the Hankel row of each pair computed in place.  Writes
  * each target box's row. */
-int sumpyhip_h2d_m2l(int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, int ntgt,
                      const int *target_boxes, const int *starts, const int *lists,
                      const int *classes, int cls_start, const void *data,
                      const double *centers, int ncenters, double src_rscale,
@@ -545,14 +546,14 @@ int sumpyhip_h2d_m2l(int src_order, int tgt_order, double k, int ntgt,
                      int tgt_base, void *stream);
 
 /* L2P of each target box at its own targets (writes). */
-int sumpyhip_h2d_l2p(int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_l2p(int kind, int order, double k, int ntgt, const int *target_boxes,
                      const int *box_target_starts, const int *box_target_counts,
                      const double *centers, int ncenters, const double *tx,
                      const double *ty, double rscale, const void *loc, int base, void *out,
                      void *stream);
 
 /* M2P from the list-3 boxes of each target box (adds). */
-int sumpyhip_h2d_m2p(int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_m2p(int kind, int order, double k, int ntgt, const int *target_boxes,
                      const int *starts, const int *lists, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
                      const double *tx, const double *ty, double rscale, const void *mpoles,
@@ -560,7 +561,7 @@ int sumpyhip_h2d_m2p(int order, double k, int ntgt, const int *target_boxes,
 
 /* Near field over list 1 (CSR); t2s (or NULL): the source index of each
  * target to skip; pairs at zero distance add nothing. */
-int sumpyhip_h2d_p2p_csr(double k, int ntgt, const int *target_boxes, const int *starts,
+int sumpyhip_h2d_p2p_csr(int kind, double k, int ntgt, const int *target_boxes, const int *starts,
                          const int *lists, const int *box_source_starts,
                          const int *box_source_counts, const int *box_target_starts,
                          const int *box_target_counts, const double *tx, const double *ty,
@@ -568,7 +569,7 @@ int sumpyhip_h2d_p2p_csr(double k, int ntgt, const int *target_boxes, const int 
                          const int *t2s, void *out, void *stream);
 
 /* Dense direct sum. */
-int sumpyhip_h2d_p2p_dense(double k, int ntargets, const double *tx, const double *ty,
+int sumpyhip_h2d_p2p_dense(int kind, double k, int ntargets, const double *tx, const double *ty,
                            int nsources, const double *sx, const double *sy,
                            const void *weights, const int *t2s, void *out, void *stream);
 

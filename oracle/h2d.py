@@ -1,5 +1,13 @@
-"""CPU oracle for the 2D Helmholtz (H2D) Hankel/Bessel expansions -- TEST
+"""CPU oracle for the 2D Hankel/Bessel expansions (H2D, Y2D) -- TEST
 INFRASTRUCTURE ONLY (imported by ``tests/`` alone).
+
+Every function takes the Bessel argument scale ``k``: the Helmholtz
+wavenumber (H2D), or ``1j*lam`` for the Yukawa kernel (Y2D), whose
+expansions are the H2D formulas at the scaled argument
+(``sumpy/expansion/multipole.py:551-567``, ``local.py:683-704``) and whose
+Green's function is (i/4) H0(i lam r) = K0(lam r)/(2 pi)
+(``sumpy/kernel.py:866-897``); scipy evaluates ``jv`` and ``hankel1`` at
+complex arguments.
 
 A numpy/scipy restatement of the reference's formulas, one function per
 operation, each citing the reference file:line it follows.  Coefficients

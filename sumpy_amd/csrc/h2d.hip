@@ -1,10 +1,12 @@
-// 2D Helmholtz FMM with Hankel/Bessel expansions (H2D).
+// 2D Helmholtz and Yukawa FMMs with Hankel/Bessel expansions (H2D, Y2D).
 //
-// G(d) = (i/4) H0^(1)(k |d|), real k > 0 (sumpy/kernel.py:754-844, 2D).  The
-// reference's H2DMultipoleExpansion / H2DLocalExpansion
-// (sumpy/expansion/multipole.py:426-550, sumpy/expansion/local.py:522-682)
+// Helmholtz: G(d) = (i/4) H0^(1)(k |d|), real k > 0 (sumpy/kernel.py:754-844,
+// 2D); Yukawa: G(d) = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi), lam > 0
+// (sumpy/kernel.py:848-897).  The reference's H2D / Y2D expansions
+// (sumpy/expansion/multipole.py:426-567, sumpy/expansion/local.py:522-704)
 // and its Fourier-Bessel M2L (sumpy/expansion/m2l.py:1008-1115) store 2p+1
-// complex coefficients c = -p..p at index c + p:
+// complex coefficients c = -p..p at index c + p; with the Bessel argument
+// scale s = k (Helmholtz) or i lam (Yukawa, multipole.py:552-567):
 //   P2M  M_c = sum_s q_s J_c(k|a|) rscale^-|c| e^{-i c th_s}
 //   M2P  u(t) = (i/4) sum_c M_c H_c(k|b|) rscale^|c| e^{i c th_b}
 //   P2L  L_c = sum_s q_s H_c(k|a|) rscale^|c| e^{i c th_s}
@@ -14,12 +16,18 @@
 //   M2L  L_j = (-1)^j rt^|j| sum_m D[m + j + ps + pt] rs^|m| M_m,
 //        D[n + ps + pt] = H_n(k|d|) e^{i n th_d}  (the translation-class data)
 // with a = centre - source, b = target - centre, d = new centre - old
-// centre, th_s the angle of the source about the centre.
+// centre, th_s the angle of the source about the centre, and J_c, H_c taken
+// at s |.| (the formulas above are written for s = k).
 //
-// Bessel functions: J_0..J_n by Miller's backward recurrence normalised with
-// J_0 + 2 sum_k J_2k = 1; Y_0 and Y_1 from their Neumann series in the J_n
-// (Abramowitz & Stegun 9.1.88), Y_n by the upward recurrence (stable for Y);
-// H_n = J_n + i Y_n, J_-n = (-1)^n J_n, H_-n = (-1)^n H_n.
+// Bessel functions (KIND 0, Helmholtz): J_0..J_n by Miller's backward
+// recurrence normalised with J_0 + 2 sum_k J_2k = 1; Y_0 and Y_1 from their
+// Neumann series in the J_n (Abramowitz & Stegun 9.1.88), Y_n by the upward
+// recurrence (stable for Y); H_n = J_n + i Y_n.  (KIND 1, Yukawa) with
+// x = lam r: J_n(i x) = i^n I_n(x), H_n(i x) = (2/pi) i^(-n-1) K_n(x); I_n by
+// Miller's recurrence normalised with I_0 + 2 sum_k I_k = e^x; K_0, K_1 by the
+// trapezoidal rule on K_nu(x) = int_0^inf e^{-x cosh t} cosh(nu t) dt (step
+// min(1/4, 1/(2 sqrt x)), exponentially convergent for this integrand), K_n
+// by the upward recurrence.  Both kinds: Z_-n = (-1)^n Z_n for Z = J, H.
 //
 // Layout: one wave (64 lanes) per box; lane c + p owns coefficient c
 // (2p+1 <= 63), so the orders go up to 31.  Per-source terms are summed per
@@ -88,6 +96,85 @@ __device__ void bessel_jy(double x, int n, double *J, double *Y) {
   for (int m = 1; m < n; ++m) Y[m + 1] = (double)m*tox*Y[m] - Y[m - 1];
 }
 
+// I_0..I_n(x) and, when K != null, K_0..K_n(x) (x > 0; n >= 1 with K).
+// x = 0: I_0 = 1, I_n = 0 (K not requested there).
+__device__ void bessel_ik(double x, int n, double *I, double *K) {
+  if (x == 0.0) {
+    I[0] = 1.0;
+    for (int m = 1; m <= n; ++m) I[m] = 0.0;
+    return;
+  }
+  int M = (n > (int)x ? n : (int)x) + 20 + (int)(16.0*cbrt(x));
+  M += M & 1;
+  const double tox = 2.0/x;
+  double ip1 = 0.0, v = 1.0, norm = 0.0;
+  for (int m = M; m >= 1; --m) {
+    if (m <= n) I[m] = v;
+    norm += 2.0*v;
+    const double im1 = (double)m*tox*v + ip1;
+    ip1 = v;
+    v = im1;
+    if (v > BIG) {
+      v *= SMALL; ip1 *= SMALL; norm *= SMALL;
+      for (int q = m; q <= n; ++q) I[q] *= SMALL;
+    }
+  }
+  norm += v;
+  I[0] = v;
+  const double f = exp(x)/norm;
+  for (int m = 0; m <= n; ++m) I[m] *= f;
+  if (!K) return;
+  const double h = x <= 4.0 ? 0.25 : 0.5/sqrt(x);
+  const double e0 = exp(-x);
+  double s0 = 0.5*e0, s1 = 0.5*e0;
+  for (int j = 1; j < 4096; ++j) {
+    const double ct = cosh(j*h), e = exp(-x*ct);
+    s0 += e;
+    s1 += e*ct;
+    if (x*(ct - 1.0) > 40.0) break;
+  }
+  K[0] = h*s0;
+  K[1] = h*s1;
+  for (int m = 1; m < n; ++m) K[m + 1] = (double)m*tox*K[m] + K[m - 1];
+}
+
+// The two function sequences of one kernel kind at x = |s| r, as complex
+// values Z^J_c = J_c(s r) and Z^H_c = H_c(s r), c = 0..n (n >= 1 for H).
+template <int KIND>
+__device__ __forceinline__ void seq_j(double x, int n, cplx *Z) {
+  double A[NSEQ];
+  if (KIND == 0) bessel_jy(x, n, A, nullptr);
+  else bessel_ik(x, n, A, nullptr);
+  for (int c = 0; c <= n; ++c) {
+    if (KIND == 0) {
+      Z[c] = {A[c], 0.0};
+    } else {
+      // i^c I_c
+      const int q = c & 3;
+      Z[c] = q == 0 ? cplx{A[c], 0.0} : q == 1 ? cplx{0.0, A[c]}
+                    : q == 2 ? cplx{-A[c], 0.0} : cplx{0.0, -A[c]};
+    }
+  }
+}
+
+template <int KIND>
+__device__ __forceinline__ void seq_h(double x, int n, cplx *Z) {
+  double A[NSEQ], B[NSEQ];
+  if (KIND == 0) bessel_jy(x, n, A, B);
+  else bessel_ik(x, n, A, B);
+  for (int c = 0; c <= n; ++c) {
+    if (KIND == 0) {
+      Z[c] = {A[c], B[c]};
+    } else {
+      // (2/pi) i^(-c-1) K_c
+      const double v = (2.0/PI)*B[c];
+      const int q = c & 3;
+      Z[c] = q == 0 ? cplx{0.0, -v} : q == 1 ? cplx{-v, 0.0}
+                    : q == 2 ? cplx{0.0, v} : cplx{v, 0.0};
+    }
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double v) { return group_sum(v, WAVE); }
 
 __device__ __forceinline__ cplx load_c(const double *p, long i) { return {p[2*i], p[2*i + 1]}; }
@@ -117,15 +204,15 @@ __device__ __forceinline__ double unit(double vx, double vy, double &ux, double 
   return r;
 }
 
-// Per-lane accumulation of sum_c w Z_c(k rho) e^{i s c th} over c = -p..p
-// into acc[c + p] (Z = J, or H = J + iY when Y is given); the phase
-// e^{i th} is (ux, uy) and s = sign.
-__device__ void add_terms(int p, cplx w, const double *J, const double *Y, double ux,
-                          double uy, int sign, cplx *acc) {
+// Per-lane accumulation of sum_c w Z_c e^{i s c th} over c = -p..p into
+// acc[c + p] (Z: the sequence, c >= 0); the phase e^{i th} is (ux, uy) and
+// s = sign.
+__device__ void add_terms(int p, cplx w, const cplx *Z, double ux, double uy, int sign,
+                          cplx *acc) {
   const cplx e1 = {ux, sign*uy};
   cplx ph = {1.0, 0.0};  // e^{i s c th}
   for (int c = 0; c <= p; ++c) {
-    const cplx z = {J[c], Y ? Y[c] : 0.0};
+    const cplx z = Z[c];
     const cplx t = cmul(w, cmul(z, ph));
     acc[p + c] = cadd(acc[p + c], t);
     if (c > 0) {
@@ -139,7 +226,7 @@ __device__ void add_terms(int p, cplx w, const double *J, const double *Y, doubl
 
 // {{{ P2M / P2L: one wave per box, lanes over its sources
 
-template <bool LOCAL>
+template <int KIND, bool LOCAL>
 __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
     int p, double k, int nboxes, const int *boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const double *centers, int ncenters, const double *sx,
@@ -151,7 +238,7 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
   const double cx = centers[box], cy = centers[ncenters + box];
   cplx acc[2*PMAX + 1];
   for (int c = 0; c < 2*p + 1; ++c) acc[c] = {0.0, 0.0};
-  double J[PMAX + 2], Y[PMAX + 2];
+  cplx Z[PMAX + 2];
   // P2M: the box's own sources; P2L: the sources of its list-4 boxes
   const int e0 = LOCAL ? starts[ib] : 0, e1 = LOCAL ? starts[ib + 1] : 1;
   for (int e = e0; e < e1; ++e) {
@@ -163,11 +250,11 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
       const double rho = unit(sx[s] - cx, sy[s] - cy, ux, uy);
       const cplx w = {q[2*s], q[2*s + 1]};
       if (LOCAL) {
-        bessel_jy(k*rho, p > 0 ? p : 1, J, Y);
-        add_terms(p, w, J, Y, ux, uy, 1, acc);
+        seq_h<KIND>(k*rho, p > 0 ? p : 1, Z);
+        add_terms(p, w, Z, ux, uy, 1, acc);
       } else {
-        bessel_jy(k*rho, p, J, nullptr);
-        add_terms(p, w, J, nullptr, ux, uy, -1, acc);
+        seq_j<KIND>(k*rho, p, Z);
+        add_terms(p, w, Z, ux, uy, -1, acc);
       }
     }
   }
@@ -190,16 +277,17 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
 
 // {{{ M2M / L2L: one wave per target box
 
-// J_n(k|d|) e^{i n th_d} for n = 0..N into LDS (lane 0), the same wave
-__device__ void shift_data(double k, double dx, double dy, int N, double *Js, cplx *ph) {
+// J_n(s|d|) and e^{i n th_d} for n = 0..N into LDS (lane 0), the same wave
+template <int KIND>
+__device__ void shift_data(double k, double dx, double dy, int N, cplx *Js, cplx *ph) {
   if (threadIdx.x == 0) {
     double ux, uy;
     const double dl = unit(dx, dy, ux, uy);
-    double J[NSEQ];
-    bessel_jy(k*dl, N, J, nullptr);
+    cplx Z[NSEQ];
+    seq_j<KIND>(k*dl, N, Z);
     cplx e = {1.0, 0.0};
     for (int n = 0; n <= N; ++n) {
-      Js[n] = J[n];
+      Js[n] = Z[n];
       ph[n] = e;
       e = cmul(e, cplx{ux, uy});
     }
@@ -207,22 +295,22 @@ __device__ void shift_data(double k, double dx, double dy, int N, double *Js, cp
   __syncthreads();
 }
 
-// J_n e^{i s n th} for any sign of n from the n >= 0 tables
-__device__ __forceinline__ cplx shift_term(int n, const double *Js, const cplx *ph, int sign) {
+// J_n e^{i s n th} for any sign of n from the n >= 0 tables (J_-n = (-1)^n J_n)
+__device__ __forceinline__ cplx shift_term(int n, const cplx *Js, const cplx *ph, int sign) {
   const int an = n < 0 ? -n : n;
-  double j = Js[an];
-  if (n < 0 && (an & 1)) j = -j;
+  cplx j = Js[an];
+  if (n < 0 && (an & 1)) j = cscale(j, -1.0);
   cplx e = ph[an];
   if ((n < 0) != (sign < 0)) e = conj(e);
-  return cscale(e, j);
+  return cmul(e, j);
 }
 
-template <bool LOCAL>
+template <int KIND, bool LOCAL>
 __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
     int ps, int pt, double k, int ntgt, const int *target_boxes, const int *relatives,
     int nboxes, const double *centers, int ncenters, double src_rscale, double tgt_rscale,
     const double *src, int sbase, double *tgt, int tbase) {
-  __shared__ double Js[NSEQ];
+  __shared__ cplx Js[NSEQ];
   __shared__ cplx ph[NSEQ];
   __shared__ cplx S[2*PMAX + 1];
   const int it = blockIdx.x;
@@ -237,7 +325,7 @@ __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
     const int sb = LOCAL ? relatives[tb] : relatives[(long)r*nboxes + tb];
     if (!LOCAL && sb == 0) continue;  // same on every lane
     const double dx = centers[tb] - centers[sb], dy = centers[ncenters + tb] - centers[ncenters + sb];
-    shift_data(k, dx, dy, ps + pt, Js, ph);
+    shift_data<KIND>(k, dx, dy, ps + pt, Js, ph);
     if (lane < 2*ps + 1) {
       const int am = lane > ps ? lane - ps : ps - lane;
       const double s = LOCAL ? 1.0/ipow(src_rscale, am) : ipow(src_rscale, am);
@@ -261,34 +349,37 @@ __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
 
 // {{{ M2L
 
-// H_n(k|d|) e^{i n th_d}, n = -N..N, into D[n + N] (one thread)
+// H_n(s|d|) e^{i n th_d}, n = -N..N, into D[n + N] (one thread)
+template <int KIND>
 __device__ void hankel_row(double k, double dx, double dy, int N, cplx *D) {
   double ux, uy;
   const double dl = unit(dx, dy, ux, uy);
-  double J[NSEQ], Y[NSEQ];
-  bessel_jy(k*dl, N > 0 ? N : 1, J, Y);
+  cplx Z[NSEQ];
+  seq_h<KIND>(k*dl, N > 0 ? N : 1, Z);
   cplx e = {1.0, 0.0};
   for (int n = 0; n <= N; ++n) {
-    const cplx h = cmul(cplx{J[n], Y[n]}, e);
+    const cplx h = cmul(Z[n], e);
     D[N + n] = h;
     // H_-n e^{-i n th} = (-1)^n H_n conj(e)
-    const cplx hn = cmul(cplx{J[n], Y[n]}, conj(e));
+    const cplx hn = cmul(Z[n], conj(e));
     D[N - n] = (n & 1) ? cscale(hn, -1.0) : hn;
     e = cmul(e, cplx{ux, uy});
   }
 }
 
+template <int KIND>
 __global__ void h2d_m2l_data_kernel(int N, double k, int ncls, const double *vecs, int nvecs,
                                     int cls_start, double *data) {
   const int c = blockIdx.x*blockDim.x + threadIdx.x;
   if (c >= ncls) return;
   cplx D[2*NSEQ];
-  hankel_row(k, vecs[cls_start + c], vecs[nvecs + cls_start + c], N, D);
+  hankel_row<KIND>(k, vecs[cls_start + c], vecs[nvecs + cls_start + c], N, D);
   for (int n = 0; n < 2*N + 1; ++n) store_c(data, (long)c*(2*N + 1) + n, D[n]);
 }
 
 // one wave per target box; class data from `data` (rows by class - cls_start)
 // or, when data is null, the Hankel row of each pair made in LDS
+template <int KIND>
 __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
     int ps, int pt, double k, int ntgt, const int *target_boxes, const int *starts,
     const int *lists, const int *classes, int cls_start, const double *data,
@@ -305,8 +396,8 @@ __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
   for (int e = starts[it]; e < starts[it + 1]; ++e) {
     const int sb = lists[e];
     if (!data && lane == 0)
-      hankel_row(k, centers[tb] - centers[sb], centers[ncenters + tb] - centers[ncenters + sb],
-                 N, Dl);
+      hankel_row<KIND>(k, centers[tb] - centers[sb],
+                       centers[ncenters + tb] - centers[ncenters + sb], N, Dl);
     if (lane < 2*ps + 1) {
       const int am = lane > ps ? lane - ps : ps - lane;
       S[lane] = cscale(load_c(src, (long)(sb - sbase)*(2*ps + 1) + lane), ipow(src_rscale, am));
@@ -335,7 +426,7 @@ __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
 // (i/4) z
 __device__ __forceinline__ cplx scale_i4(cplx z) { return {-0.25*z.im, 0.25*z.re}; }
 
-template <bool MPOLE>
+template <int KIND, bool MPOLE>
 __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
     int p, double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bts, const int *btc, const double *centers, int ncenters, const double *tx,
@@ -347,7 +438,7 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
   const int lane = threadIdx.x;
   const int t0 = bts[tb], nt = btc[tb];
   const int e0 = MPOLE ? starts[it] : 0, e1 = MPOLE ? starts[it + 1] : 1;
-  double J[PMAX + 2], Y[PMAX + 2];
+  cplx Z[PMAX + 2];
   for (int c0 = 0; c0 < nt; c0 += WAVE) {
     const int t = t0 + c0 + lane;
     const bool active = c0 + lane < nt;
@@ -365,10 +456,11 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
         const double rho = unit(tx[t] - centers[sb], ty[t] - centers[ncenters + sb], ux, uy);
         // M2P: e^{i c th}, H_c; L2P: e^{-i c th}, J_c
         const cplx e1u = {ux, MPOLE ? uy : -uy};
-        bessel_jy(k*rho, p > 0 ? p : 1, J, MPOLE ? Y : nullptr);
+        if (MPOLE) seq_h<KIND>(k*rho, p > 0 ? p : 1, Z);
+        else seq_j<KIND>(k*rho, p, Z);
         cplx ph = {1.0, 0.0};
         for (int c = 0; c <= p; ++c) {
-          const cplx z = {J[c], MPOLE ? Y[c] : 0.0};
+          const cplx z = Z[c];
           acc = cfma(S[p + c], cmul(z, ph), acc);
           if (c > 0) {
             const cplx tn = cmul(S[p - c], cmul(z, conj(ph)));
@@ -411,9 +503,15 @@ __device__ __forceinline__ void h0_series(double x, double &j0, double &y0) {
 }
 
 // (i/4) H0(k r) for r > 0: the power series up to k r = 4, the Miller /
-// Neumann route above it
+// Neumann route above it; Yukawa: K0(lam r) / (2 pi)
+template <int KIND>
 __device__ __forceinline__ cplx green(double k, double r) {
   const double x = k*r;
+  if (KIND == 1) {
+    double I[2], K[2];
+    bessel_ik(x, 1, I, K);
+    return {K[0]/(2.0*PI), 0.0};
+  }
   double j0, y0;
   if (x <= 4.0) {
     h0_series(x, j0, y0);
@@ -427,6 +525,7 @@ __device__ __forceinline__ cplx green(double k, double r) {
 }
 
 // one wave per target box, lanes over its targets; sources of list 1
+template <int KIND>
 __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
     double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const int *bts, const int *btc, const double *tx,
@@ -448,13 +547,14 @@ __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
         const double dx = x - sx[s], dy = y - sy[s];
         const double r = sqrt(dx*dx + dy*dy);
         if (r == 0.0 || s == self) continue;
-        acc = cfma(green(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+        acc = cfma(green<KIND>(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
       }
     }
     store_c(out, t, acc);
   }
 }
 
+template <int KIND>
 __global__ void h2d_p2p_dense_kernel(double k, int nt, const double *tx, const double *ty,
                                      int ns, const double *sx, const double *sy,
                                      const double *q, const int *t2s, double *out) {
@@ -467,7 +567,7 @@ __global__ void h2d_p2p_dense_kernel(double k, int nt, const double *tx, const d
     const double dx = x - sx[s], dy = y - sy[s];
     const double r = sqrt(dx*dx + dy*dy);
     if (r == 0.0 || s == self) continue;
-    acc = cfma(green(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+    acc = cfma(green<KIND>(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
   }
   store_c(out, t, acc);
 }
@@ -480,8 +580,10 @@ int check_orders(int ps, int pt) {
   return SUMPYHIP_OK;
 }
 
-int check_k(double k) {
-  SUMPY_REQUIRE(k > 0.0 && std::isfinite(k), "h2d: the wavenumber must be real and > 0");
+int check_k(int kind, double k) {
+  SUMPY_REQUIRE(kind == 0 || kind == 1, "h2d: kind must be 0 (Helmholtz) or 1 (Yukawa)");
+  SUMPY_REQUIRE(k > 0.0 && std::isfinite(k),
+                "h2d: the Helmholtz k / Yukawa lambda must be real and > 0");
   return SUMPYHIP_OK;
 }
 
@@ -496,144 +598,174 @@ using namespace sumpyhip;
     if (rc_ != SUMPYHIP_OK) return rc_; \
   } while (0)
 
-extern "C" int sumpyhip_h2d_p2m(int order, double k, int nboxes, const int *source_boxes,
+extern "C" int sumpyhip_h2d_p2m(int kind, int order, double k, int nboxes, const int *source_boxes,
                                 const int *box_source_starts, const int *box_source_counts,
                                 const double *centers, int ncenters, const double *sx,
                                 const double *sy, const void *weights, double rscale,
                                 void *out, int base, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (nboxes == 0) return SUMPYHIP_OK;
-  h2d_p2e_kernel<false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_p2e_kernel<1, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  else h2d_p2e_kernel<0, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
       order, k, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   return check_launch("h2d_p2m");
 }
 
-extern "C" int sumpyhip_h2d_p2l(int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_p2l(int kind, int order, double k, int ntgt, const int *target_boxes,
                                 const int *starts, const int *lists,
                                 const int *box_source_starts, const int *box_source_counts,
                                 const double *centers, int ncenters, const double *sx,
                                 const double *sy, const void *weights, double rscale,
                                 void *out, int base, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_p2e_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_p2e_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  else h2d_p2e_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       order, k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   return check_launch("h2d_p2l");
 }
 
-extern "C" int sumpyhip_h2d_m2m(int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, int ntgt,
                                 const int *target_boxes, const int *box_child_ids, int nboxes,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_shift_kernel<false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_shift_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  else h2d_shift_kernel<0, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       src_order, tgt_order, k, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_m2m");
 }
 
-extern "C" int sumpyhip_h2d_l2l(int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, int ntgt,
                                 const int *target_boxes, const int *box_parent_ids,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_shift_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_shift_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  else h2d_shift_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       src_order, tgt_order, k, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_l2l");
 }
 
-extern "C" int sumpyhip_h2d_m2l_data(int src_order, int tgt_order, double k, int ncls,
+extern "C" int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, int ncls,
                                      const double *vecs, int nvecs, int cls_start, void *data,
                                      void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ncls == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(cls_start >= 0 && cls_start + ncls <= nvecs, "h2d_m2l_data: class range");
-  h2d_m2l_data_kernel<<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_m2l_data_kernel<1><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
+      src_order + tgt_order, k, ncls, vecs, nvecs, cls_start, (double *)data);
+  else h2d_m2l_data_kernel<0><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
       src_order + tgt_order, k, ncls, vecs, nvecs, cls_start, (double *)data);
   return check_launch("h2d_m2l_data");
 }
 
-extern "C" int sumpyhip_h2d_m2l(int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, int ntgt,
                                 const int *target_boxes, const int *starts, const int *lists,
                                 const int *classes, int cls_start, const void *data,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   SUMPY_REQUIRE(!data || classes, "h2d_m2l: class data without the class list");
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_m2l_kernel<<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_m2l_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, k, ntgt, target_boxes, starts, lists, classes, cls_start,
+      (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
+      src_base, (double *)tgt, tgt_base);
+  else h2d_m2l_kernel<0><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       src_order, tgt_order, k, ntgt, target_boxes, starts, lists, classes, cls_start,
       (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
       src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_m2l");
 }
 
-extern "C" int sumpyhip_h2d_l2p(int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_l2p(int kind, int order, double k, int ntgt, const int *target_boxes,
                                 const int *box_target_starts, const int *box_target_counts,
                                 const double *centers, int ncenters, const double *tx,
                                 const double *ty, double rscale, const void *loc, int base,
                                 void *out, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_e2p_kernel<false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_e2p_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
+  else h2d_e2p_kernel<0, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       order, k, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
   return check_launch("h2d_l2p");
 }
 
-extern "C" int sumpyhip_h2d_m2p(int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_m2p(int kind, int order, double k, int ntgt, const int *target_boxes,
                                 const int *starts, const int *lists,
                                 const int *box_target_starts, const int *box_target_counts,
                                 const double *centers, int ncenters, const double *tx,
                                 const double *ty, double rscale, const void *mpoles, int base,
                                 void *out, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_e2p_kernel<true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_e2p_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, k, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
+  else h2d_e2p_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       order, k, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
   return check_launch("h2d_m2p");
 }
 
-extern "C" int sumpyhip_h2d_p2p_csr(double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_p2p_csr(int kind, double k, int ntgt, const int *target_boxes,
                                     const int *starts, const int *lists,
                                     const int *box_source_starts, const int *box_source_counts,
                                     const int *box_target_starts, const int *box_target_counts,
                                     const double *tx, const double *ty, const double *sx,
                                     const double *sy, const void *weights, const int *t2s,
                                     void *out, void *stream) {
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntgt == 0) return SUMPYHIP_OK;
-  h2d_p2p_csr_kernel<<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_p2p_csr_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
+      (double *)out);
+  else h2d_p2p_csr_kernel<0><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
       k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
       (double *)out);
   return check_launch("h2d_p2p_csr");
 }
 
-extern "C" int sumpyhip_h2d_p2p_dense(double k, int ntargets, const double *tx,
+extern "C" int sumpyhip_h2d_p2p_dense(int kind, double k, int ntargets, const double *tx,
                                       const double *ty, int nsources, const double *sx,
                                       const double *sy, const void *weights, const int *t2s,
                                       void *out, void *stream) {
-  H2D_CHECK(check_k(k));
+  H2D_CHECK(check_k(kind, k));
   if (ntargets == 0) return SUMPYHIP_OK;
-  h2d_p2p_dense_kernel<<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
+  if (kind) h2d_p2p_dense_kernel<1><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
+      k, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+  else h2d_p2p_dense_kernel<0><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
       k, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
   return check_launch("h2d_p2p_dense");
 }

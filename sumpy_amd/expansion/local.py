@@ -100,9 +100,27 @@ class H2DLocalExpansion(FourierBesselLocalExpansionMixin):
         return H2DMultipoleExpansion
 
 
+@dataclass(frozen=True)
+class Y2DLocalExpansion(FourierBesselLocalExpansionMixin):
+    """For the 2D Yukawa kernel (``local.py:683-704``)."""
+
+    def __post_init__(self):
+        from sumpy_amd.kernel import YukawaKernel
+        base = self.kernel.get_base_kernel()
+        if not (isinstance(base, YukawaKernel) and base.dim == 2):
+            raise TypeError(f"{type(self).__name__} can only be applied to 2D "
+                            f"YukawaKernel: {self.kernel!r}")
+
+    @property
+    def mpole_expn_class(self):
+        from sumpy_amd.expansion.multipole import Y2DMultipoleExpansion
+        return Y2DMultipoleExpansion
+
+
 __all__ = [
     "FourierBesselLocalExpansionMixin",
     "H2DLocalExpansion",
+    "Y2DLocalExpansion",
     "LineTaylorLocalExpansion",
     "LinearPDEConformingVolumeTaylorLocalExpansion",
     "LocalExpansionBase",

@@ -1,17 +1,20 @@
-"""Host side of the 2D Helmholtz Hankel/Bessel (H2D) expansions.
+"""Host side of the 2D Hankel/Bessel expansions: Helmholtz (H2D) and
+Yukawa (Y2D).
 
 The operators of :mod:`sumpy_amd.p2e`, :mod:`~sumpy_amd.e2e`,
 :mod:`~sumpy_amd.e2p` and :mod:`~sumpy_amd.p2p` hand their call here when
-their expansion is an :class:`~sumpy_amd.expansion.multipole.H2DMultipoleExpansion`
-/ :class:`~sumpy_amd.expansion.local.H2DLocalExpansion` (or their kernel the
-2D Helmholtz kernel), with the reference's keyword contract
+their expansion is an H2D / Y2D multipole or local expansion
+(:mod:`sumpy_amd.expansion.multipole`, :mod:`~sumpy_amd.expansion.local`),
+or their kernel the 2D Helmholtz or Yukawa kernel, with the reference's
+keyword contract
 (``sumpy/p2e.py:155-388``, ``sumpy/e2e.py:164-1090``, ``sumpy/e2p.py:123-343``,
 ``sumpy/p2p.py:215-833``).  The HIP kernels are ``csrc/h2d.hip``; the
 mathematics is that of ``sumpy/expansion/multipole.py:426-550``,
-``sumpy/expansion/local.py:522-682`` and ``sumpy/expansion/m2l.py:1008-1115``.
+``sumpy/expansion/local.py:522-704`` and ``sumpy/expansion/m2l.py:1008-1115``
+(Yukawa: the Bessel argument scaled by ``i lam``, ``multipole.py:551-567``).
 
 Supported: potentials (no target derivatives) of charge weights (no source
-derivatives), real ``k > 0``, orders 0..31, FP64.
+derivatives), real ``k > 0`` / ``lam > 0``, orders 0..31, FP64.
 """
 
 from __future__ import annotations
@@ -31,10 +34,20 @@ from sumpy_amd.tools import (
 PMAX = 31
 
 
-def is_h2d_kernel(kernel) -> bool:
-    from sumpy_amd.kernel import HelmholtzKernel
+def kind_of(kernel) -> int:
+    """0: the 2D Helmholtz kernel, 1: the 2D Yukawa kernel, -1: neither."""
+    from sumpy_amd.kernel import HelmholtzKernel, YukawaKernel
     base = kernel.get_base_kernel()
-    return isinstance(base, HelmholtzKernel) and base.dim == 2
+    if isinstance(base, HelmholtzKernel) and base.dim == 2:
+        return 0
+    if isinstance(base, YukawaKernel) and base.dim == 2:
+        return 1
+    return -1
+
+
+def is_h2d_kernel(kernel) -> bool:
+    """The kernel runs with the 2D Bessel machinery (Helmholtz or Yukawa)."""
+    return kind_of(kernel) >= 0
 
 
 def is_h2d(expansion) -> bool:
@@ -45,9 +58,11 @@ def is_h2d(expansion) -> bool:
 
 
 def wavenumber(kernel, kwargs) -> float:
+    """The Helmholtz k or the Yukawa lambda from the call's keywords."""
     k = kernel_k(kernel, kwargs)
     if k.imag != 0 or not k.real > 0:
-        raise NotImplementedError("H2D expansions on the GPU need a real wavenumber k > 0")
+        raise NotImplementedError("the 2D Bessel expansions on the GPU need a real "
+                                  "k > 0 (Helmholtz) or lam > 0 (Yukawa)")
     return float(k.real)
 
 
@@ -98,7 +113,8 @@ def p2m(op, actx, kwargs):
     sb = as_i32(kwargs.pop("source_boxes"))
     bss = as_i32(kwargs.pop("box_source_starts"))
     bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
-    _capi.call("sumpyhip_h2d_p2m", _order(op.expansion), k, int(sb.shape[0]), _capi.ptr(sb),
+    _capi.call("sumpyhip_h2d_p2m", kind_of(op.expansion.kernel), _order(op.expansion), k,
+               int(sb.shape[0]), _capi.ptr(sb),
                _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
                src[0].data_ptr(), src[1].data_ptr(), w.data_ptr(),
                _rscale(op.expansion, kwargs.pop("rscale")), _capi.ptr(tgt),
@@ -121,7 +137,8 @@ def p2l(op, actx, kwargs):
     lists = as_i32(kwargs.pop("source_box_lists"))
     bss = as_i32(kwargs.pop("box_source_starts"))
     bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
-    _capi.call("sumpyhip_h2d_p2l", _order(op.expansion), k, int(tb.shape[0]), _capi.ptr(tb),
+    _capi.call("sumpyhip_h2d_p2l", kind_of(op.expansion.kernel), _order(op.expansion), k,
+               int(tb.shape[0]), _capi.ptr(tb),
                _capi.ptr(starts), _capi.ptr(lists), _capi.ptr(bss), _capi.ptr(bsc),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(),
                src[1].data_ptr(), w.data_ptr(), _rscale(op.expansion, kwargs.pop("rscale")),
@@ -142,9 +159,10 @@ def _shift(op, actx, kwargs, local):
     ps, pt = _order(op.src_expansion), _order(op.tgt_expansion)
     sr = _rscale(op.src_expansion, kwargs["src_rscale"])
     tr = _rscale(op.tgt_expansion, kwargs["tgt_rscale"])
+    kind = kind_of(op.tgt_expansion.kernel)
     if local:
         par = as_i32(kwargs["box_parent_ids"])
-        _capi.call("sumpyhip_h2d_l2l", ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_l2l", kind, ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(par), _capi.ptr(centers), int(centers.shape[1]), sr, tr,
                    _capi.ptr(src), int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
                    int(kwargs["tgt_base_ibox"]), actx.stream_handle)
@@ -152,7 +170,7 @@ def _shift(op, actx, kwargs, local):
         child = as_i32(kwargs["box_child_ids"])
         if child.dim() != 2 or child.shape[0] != 4:
             raise ValueError("box_child_ids must have shape (4, nboxes)")
-        _capi.call("sumpyhip_h2d_m2m", ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_m2m", kind, ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(child), int(child.shape[1]), _capi.ptr(centers),
                    int(centers.shape[1]), sr, tr, _capi.ptr(src), int(kwargs["src_base_ibox"]),
                    _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
@@ -185,7 +203,8 @@ def m2l_data(op, actx, kwargs):
         raise ValueError(f"class data must be contiguous FP64 of shape ({ncls}, {n})")
     if int(vecs.shape[0]) != 2:
         raise ValueError("translation vectors must have shape (2, n)")
-    _capi.call("sumpyhip_h2d_m2l_data", ps, pt, wavenumber(op.tgt_expansion.kernel, kwargs),
+    _capi.call("sumpyhip_h2d_m2l_data", kind_of(op.tgt_expansion.kernel), ps, pt,
+               wavenumber(op.tgt_expansion.kernel, kwargs),
                ncls, _capi.ptr(vecs), int(vecs.shape[1]),
                int(kwargs["translation_classes_level_start"]), _capi.ptr(out),
                actx.stream_handle)
@@ -210,7 +229,8 @@ def m2l(op, actx, kwargs, with_classes):
         c0 = int(kwargs["translation_classes_level_start"])
         if data.shape[1] != 2*(2*(ps + pt) + 1):
             raise ValueError("class data of the wrong orders")
-    _capi.call("sumpyhip_h2d_m2l", ps, pt, wavenumber(op.tgt_expansion.kernel, kwargs),
+    _capi.call("sumpyhip_h2d_m2l", kind_of(op.tgt_expansion.kernel), ps, pt,
+               wavenumber(op.tgt_expansion.kernel, kwargs),
                int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                _capi.ptr(classes), c0, _capi.ptr(data), _capi.ptr(centers),
                int(centers.shape[1]), _rscale(op.src_expansion, kwargs["src_rscale"]),
@@ -241,16 +261,18 @@ def _e2p(op, actx, kwargs, mpole):
     btc = as_i32(kwargs["box_target_counts_nonchild"])
     p = _order(op.expansion)
     rs = _rscale(op.expansion, kwargs["rscale"])
+    kind = kind_of(op.expansion.kernel)
     common = (_capi.ptr(centers), int(centers.shape[1]), targets[0].data_ptr(),
               targets[1].data_ptr(), rs, _capi.ptr(coeffs), int(kwargs["src_base_ibox"]), out,
               actx.stream_handle)
     if mpole:
         starts = as_i32(kwargs["source_box_starts"])
         lists = as_i32(kwargs["source_box_lists"])
-        _capi.call("sumpyhip_h2d_m2p", p, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_m2p", kind, p, k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(starts), _capi.ptr(lists), _capi.ptr(bts), _capi.ptr(btc), *common)
     else:
-        _capi.call("sumpyhip_h2d_l2p", p, k, int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(bts),
+        _capi.call("sumpyhip_h2d_l2p", kind, p, k, int(tb.shape[0]), _capi.ptr(tb),
+                   _capi.ptr(bts),
                    _capi.ptr(btc), *common)
     outs.finish(actx, targets, accumulate=mpole)
     return result
@@ -287,11 +309,12 @@ def p2p(actx, kernel, kwargs, targets, sources, weights, t2s, target_kernels, re
     xy = (targets[0].data_ptr(), targets[1].data_ptr())
     sxy = (sources[0].data_ptr(), sources[1].data_ptr())
     if csr is None:
-        _capi.call("sumpyhip_h2d_p2p_dense", k, nt, *xy, int(sources[0].shape[0]), *sxy,
+        _capi.call("sumpyhip_h2d_p2p_dense", kind_of(kernel), k, nt, *xy,
+                   int(sources[0].shape[0]), *sxy,
                    w.data_ptr(), _capi.ptr(t2s), out, actx.stream_handle)
     else:
         ints = [as_i32(a) for a in csr]
-        _capi.call("sumpyhip_h2d_p2p_csr", k, int(ints[0].shape[0]),
+        _capi.call("sumpyhip_h2d_p2p_csr", kind_of(kernel), k, int(ints[0].shape[0]),
                    *[_capi.ptr(a) for a in ints], *xy, *sxy, w.data_ptr(), _capi.ptr(t2s),
                    out, actx.stream_handle)
     outs.finish(actx, targets)
@@ -300,5 +323,5 @@ def p2p(actx, kernel, kwargs, targets, sources, weights, t2s, target_kernels, re
 # }}}
 
 
-__all__ = ["is_h2d", "is_h2d_kernel", "l2l", "l2p", "m2l", "m2l_data", "m2m", "m2p",
+__all__ = ["is_h2d", "is_h2d_kernel", "kind_of", "l2l", "l2p", "m2l", "m2l_data", "m2m", "m2p",
            "p2l", "p2m", "p2p"]

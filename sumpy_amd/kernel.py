@@ -106,6 +106,36 @@ class HelmholtzKernel(ScalarKernel):
         return f"HelmKnl{self.dim}D({self.helmholtz_k_name})"
 
 
+@dataclass(frozen=True)
+class YukawaKernel(ScalarKernel):
+    """``G = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi)`` (2D; reference
+    ``kernel.py:848-923``, which writes it through the Hankel function with
+    the scaling ``i/4``).  ``lam`` is passed at call time under
+    :attr:`yukawa_lambda_name`.  Complex-valued as in the reference.  Runs
+    with the Y2D expansions and its own P2P (:mod:`sumpy_amd.h2d`); the 3D
+    kernel is not built."""
+
+    dim: int
+    yukawa_lambda_name: str = "lam"
+
+    def __post_init__(self):
+        if self.dim != 2:
+            raise NotImplementedError("only the 2D Yukawa kernel is implemented on the GPU")
+
+    @property
+    def is_complex_valued(self) -> bool:
+        return True
+
+    def get_global_scaling_const(self):
+        return 0.25j
+
+    def get_args(self):
+        return [self.yukawa_lambda_name]
+
+    def __str__(self):
+        return f"YukKnl{self.dim}D({self.yukawa_lambda_name})"
+
+
 class KernelWrapper(ScalarKernel):
     inner_kernel: ScalarKernel
 
@@ -223,7 +253,7 @@ def source_terms(knl: ScalarKernel) -> list[SourceTerm]:
     (reference ``kernel.py:2308-2318``); ``n . grad_y`` gives ``-n_a dG/dd_a``
     (``kernel.py:2474-2490``).  Only first source derivatives are supported.
     """
-    if isinstance(knl, (LaplaceKernel, HelmholtzKernel)):
+    if isinstance(knl, (LaplaceKernel, HelmholtzKernel, YukawaKernel)):
         return [SourceTerm(-1, 1.0)]
     if isinstance(knl, AxisSourceDerivative):
         inner = knl.inner_kernel
@@ -265,7 +295,7 @@ def target_terms(knl: ScalarKernel) -> list[TargetTerm]:
     ``d/dx_a (x_j u) = delta_aj u + x_j du/dx_a`` (reference
     ``kernel.py:2548-2570`` with ``AxisTargetDerivative`` applied outside).
     """
-    if isinstance(knl, (LaplaceKernel, HelmholtzKernel)):
+    if isinstance(knl, (LaplaceKernel, HelmholtzKernel, YukawaKernel)):
         return [TargetTerm(-1, 1.0)]
     if isinstance(knl, TargetPointMultiplier):
         inner = target_terms(knl.inner_kernel)
@@ -302,6 +332,8 @@ def kernel_id(knl: ScalarKernel) -> int:
         return 3 if base.dim == 3 else 2
     if isinstance(base, HelmholtzKernel):
         return 13 if base.dim == 3 else 12
+    if isinstance(base, YukawaKernel):
+        return 22
     raise NotImplementedError(f"unsupported kernel: {knl}")
 
 

@@ -1,4 +1,6 @@
-"""2D Helmholtz (H2D Hankel/Bessel expansions) on the GPU against the oracle.
+"""2D Helmholtz and Yukawa (H2D / Y2D Hankel/Bessel expansions) on the GPU
+against the oracle.  The Yukawa cases run with the Bessel argument i lam in
+the oracle (scipy at complex arguments) and through I_n, K_n on the device.
 
 * The dense P2P equals ``oracle.h2d.p2p`` (scipy's H0) to 1e-13 relative,
   also at k |d| up to ~60 (the device's Miller/Neumann Bessel routine
@@ -31,6 +33,32 @@ def rel(a, b):
 def _knl():
     from sumpy_amd.kernel import HelmholtzKernel
     return HelmholtzKernel(2)
+
+
+def _kernel(kind):
+    """(kernel, multipole class, local class, keyword name, oracle argument
+    scale from the parameter)."""
+    from sumpy_amd.expansion.local import H2DLocalExpansion, Y2DLocalExpansion
+    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion, Y2DMultipoleExpansion
+    from sumpy_amd.kernel import HelmholtzKernel, YukawaKernel
+    if kind == "helmholtz":
+        return (HelmholtzKernel(2), H2DMultipoleExpansion, H2DLocalExpansion, "k",
+                lambda v: v)
+    return YukawaKernel(2), Y2DMultipoleExpansion, Y2DLocalExpansion, "lam", lambda v: 1j*v
+
+
+@pytest.mark.parametrize("lam", [0.05, 2.0, 20.0])
+def test_y2d_p2p_dense_matches_oracle(actx, lam):
+    from sumpy_amd import P2P
+    knl = _kernel("yukawa")[0]
+    rng = np.random.default_rng(8)
+    src, tgt = 4*rng.random((2, 700)), 4*rng.random((2, 300))
+    q = rng.standard_normal(700) + 1j*rng.standard_normal(700)
+    pot, = P2P([knl], exclude_self=False)(actx, actx.from_numpy(tgt), actx.from_numpy(src),
+                                           [actx.from_numpy(q)], lam=lam)
+    err = rel(actx.to_numpy(pot), oh.p2p(1j*lam, tgt, src, q))
+    print(f"lam={lam}: Yukawa P2P vs oracle {err:.2e}")
+    assert err < 1e-13
 
 
 @pytest.mark.parametrize("k", [0.05, 1.0, 15.0])
@@ -70,8 +98,10 @@ def _chain_geometry():
     return src, np.ones(15)/15, tgt, centers
 
 
-@pytest.mark.parametrize(("order", "k"), [(3, 0.05), (8, 0.05), (20, 1.3)])
-def test_h2d_operator_chain_matches_oracle(actx, order, k):
+@pytest.mark.parametrize(("kind", "order", "k"), [
+    ("helmholtz", 3, 0.05), ("helmholtz", 8, 0.05), ("helmholtz", 20, 1.3),
+    ("yukawa", 8, 2.0), ("yukawa", 20, 0.5)])
+def test_h2d_operator_chain_matches_oracle(actx, kind, order, k):
     """Boxes 0..3 = the four centres; box 0 holds the sources, box 3 the
     targets; P2M(0) -> M2M(0->1) -> M2L(1->2) -> L2L(2->3) -> L2P(3), plus
     M2P(1) and P2L(3), each through the operators' public calls."""
@@ -84,10 +114,10 @@ def test_h2d_operator_chain_matches_oracle(actx, order, k):
         P2EFromCSR,
         P2EFromSingleBox,
     )
-    from sumpy_amd.expansion.local import H2DLocalExpansion
-    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
-    knl = _knl()
-    mpole, local = H2DMultipoleExpansion(knl, order), H2DLocalExpansion(knl, order)
+    knl, mcls, lcls, kname, arg = _kernel(kind)
+    mpole, local = mcls(knl, order), lcls(knl, order)
+    kw = {kname: k}
+    ka = arg(k)
     src, q, tgt, c = _chain_geometry()
     rs = (0.5, 0.25, 0.5, 0.25)
     dev = actx.from_numpy
@@ -105,7 +135,7 @@ def test_h2d_operator_chain_matches_oracle(actx, order, k):
     M = zeros()
     P2EFromSingleBox(mpole)(actx, sources=srcd, centers=centers, strengths=[qd],
                             tgt_expansions=M, tgt_base_ibox=0, rscale=rs[0], source_boxes=i32([0]),
-                            box_source_starts=bss, box_source_counts_nonchild=bsc, k=k)
+                            box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
     # M2M with box 1 as the parent of box 0: a child id 0 means "no child",
     # so the M2M input is a copy with box 0's multipole and centre in row 2
     child = np.zeros((4, nb), np.int32)
@@ -118,44 +148,44 @@ def test_h2d_operator_chain_matches_oracle(actx, order, k):
     E2EFromChildren(mpole, mpole)(actx, src_expansions=Msrc, src_base_ibox=0,
                                   tgt_expansions=M2, tgt_base_ibox=0, target_boxes=i32([1]),
                                   box_child_ids=dev(child), centers=dev(np.ascontiguousarray(cen2)),
-                                  src_rscale=rs[0], tgt_rscale=rs[1], k=k)
+                                  src_rscale=rs[0], tgt_rscale=rs[1], **kw)
     L = zeros()
     E2EFromCSR(mpole, local)(actx, src_expansions=M2, src_base_ibox=0, tgt_expansions=L,
                              tgt_base_ibox=0, target_boxes=i32([2]), src_box_starts=i32([0, 1]),
                              src_box_lists=i32([1]), centers=centers, src_rscale=rs[1],
-                             tgt_rscale=rs[2], k=k)
+                             tgt_rscale=rs[2], **kw)
     L2 = zeros()
     E2EFromParent(local, local)(actx, src_expansions=L, src_base_ibox=0, tgt_expansions=L2,
                                 tgt_base_ibox=0, target_boxes=i32([3]),
                                 box_parent_ids=i32([0, 0, 0, 2]), centers=centers,
-                                src_rscale=rs[2], tgt_rscale=rs[3], k=k)
+                                src_rscale=rs[2], tgt_rscale=rs[3], **kw)
     pot_l2p, = E2PFromSingleBox(local, [knl])(
         actx, src_expansions=L2, src_base_ibox=0, target_boxes=i32([3]), centers=centers,
-        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[3], k=k)
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[3], **kw)
     pot_m2p, = E2PFromCSR(mpole, [knl])(
         actx, src_expansions=M2, src_base_ibox=0, target_boxes=i32([3]),
         source_box_starts=i32([0, 1]), source_box_lists=i32([1]), centers=centers,
-        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[1], k=k)
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[1], **kw)
     L4 = zeros()
     P2EFromCSR(local)(actx, sources=srcd, centers=centers, strengths=[qd], tgt_expansions=L4,
                       tgt_base_ibox=0, rscale=rs[3], target_boxes=i32([3]),
                       source_box_starts=i32([0, 1]), source_box_lists=i32([0]),
-                      box_source_starts=bss, box_source_counts_nonchild=bsc, k=k)
+                      box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
 
-    m1 = oh.p2m(order, k, c[:, 0], src, q, rs[0])
-    m2 = oh.m2m(order, order, k, m1, c[:, 0], c[:, 1], rs[0], rs[1])
-    l1 = oh.m2l(order, order, k, m2, c[:, 2] - c[:, 1], rs[1], rs[2])
-    l2 = oh.l2l(order, order, k, l1, c[:, 2], c[:, 3], rs[2], rs[3])
-    l4 = oh.p2l(order, k, c[:, 3], src, q, rs[3])
+    m1 = oh.p2m(order, ka, c[:, 0], src, q, rs[0])
+    m2 = oh.m2m(order, order, ka, m1, c[:, 0], c[:, 1], rs[0], rs[1])
+    l1 = oh.m2l(order, order, ka, m2, c[:, 2] - c[:, 1], rs[1], rs[2])
+    l2 = oh.l2l(order, order, ka, l1, c[:, 2], c[:, 3], rs[2], rs[3])
+    l4 = oh.p2l(order, ka, c[:, 3], src, q, rs[3])
     got = {n: actx.to_numpy(v) for n, v in (("p2m", M[0]), ("m2m", M2[1]), ("m2l", L[2]),
                                              ("l2l", L2[3]), ("p2l", L4[3]))}
     want = {"p2m": m1, "m2m": m2, "m2l": l1, "l2l": l2, "p2l": l4}
     for n in want:
         e = rel(got[n], want[n])
-        print(f"p={order} k={k} {n}: {e:.2e}")
+        print(f"{kind} p={order} k={k} {n}: {e:.2e}")
         assert e < 1e-12, n
-    assert rel(actx.to_numpy(pot_l2p), oh.l2p(order, k, c[:, 3], l2, tgt, rs[3])) < 1e-12
-    assert rel(actx.to_numpy(pot_m2p), oh.m2p(order, k, c[:, 1], m2, tgt, rs[1])) < 1e-12
+    assert rel(actx.to_numpy(pot_l2p), oh.l2p(order, ka, c[:, 3], l2, tgt, rs[3])) < 1e-12
+    assert rel(actx.to_numpy(pot_m2p), oh.m2p(order, ka, c[:, 1], m2, tgt, rs[1])) < 1e-12
 
 
 def _fmm_case():
@@ -165,18 +195,16 @@ def _fmm_case():
     return src, tgt, q
 
 
-def _run_fmm(actx, tree, trav, q, order, k, use_classes):
+def _run_fmm(actx, tree, trav, q, order, k, use_classes, kind="helmholtz"):
     from functools import partial
 
     from sumpy_amd.drive import drive_fmm
-    from sumpy_amd.expansion.local import H2DLocalExpansion
-    from sumpy_amd.expansion.multipole import H2DMultipoleExpansion
     from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
-    knl = _knl()
-    ti = SumpyTreeIndependentDataForWrangler(actx, partial(H2DMultipoleExpansion, knl),
-                                             partial(H2DLocalExpansion, knl), [knl])
+    knl, mcls, lcls, kname, _ = _kernel(kind)
+    ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                             [knl])
     wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: order,
-                                kernel_extra_kwargs={"k": k},
+                                kernel_extra_kwargs={kname: k},
                                 _disable_translation_classes=not use_classes)
     pot, = drive_fmm(actx, wr, [actx.from_numpy(q.astype(np.complex128))])
     return actx.to_numpy(pot)
@@ -229,3 +257,23 @@ def test_h2d_fmm_sources_are_targets_exclude_self_high_order(actx):
     print(f"p=16 k=6: vs oracle FMM {rel(pot, ref):.2e}, vs direct {rel(pot, direct):.2e}")
     assert rel(pot, ref) < 1e-11
     assert rel(pot, direct) < 1e-8
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_y2d_fmm_matches_oracle_fmm_and_converges(actx, use_classes):
+    """The reference's ``test_sumpy_fmm`` Y2D case (``sumpy/test/test_fmm.py:106-107,233-241``:
+    lam = 2, orders 10 and 12) on the same particles as the H2D case."""
+    from sumpy_amd.tree import build_tree_and_traversal
+    src, tgt, q = _fmm_case()
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    direct = oh.p2p(2j, tgt, src, q)
+    errs = []
+    for order in (10, 12):
+        pot = _run_fmm(actx, tree, trav, q, order, 2.0, use_classes, kind="yukawa")
+        ref = oracle_fmm_h2d(tree, trav, q, order, 2j)
+        e_or = rel(pot, ref)
+        errs.append(float(np.max(np.abs(pot - direct))/np.max(np.abs(direct))))
+        print(f"Y2D classes={use_classes} p={order}: vs oracle FMM {e_or:.2e}, "
+              f"rel-inf vs direct {errs[-1]:.2e}")
+        assert e_or < 1e-12
+    assert errs[1] < errs[0] < 1e-6

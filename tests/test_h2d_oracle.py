@@ -92,3 +92,25 @@ def test_h2d_p2p_matches_hankel_definition():
     from scipy import special
     v = h2d.p2p(2.0, np.array([[0.3], [0.4]]), np.zeros((2, 1)), np.array([1.0]))[0]
     assert abs(v - 0.25j*(special.j0(1.0) + 1j*special.y0(1.0))) < 1e-15
+
+
+def test_y2d_p2p_is_k0_over_2pi():
+    """(i/4) H0(i lam r) = K0(lam r)/(2 pi) for the Yukawa kernel
+    (``sumpy/kernel.py:879-884``)."""
+    from scipy import special
+    r = np.array([0.01, 0.3, 1.0, 7.5])
+    v = np.array([h2d.p2p(2j, np.array([[x], [0.0]]), np.zeros((2, 1)), np.array([1.0]))[0]
+                  for x in r])
+    assert np.allclose(v, special.k0(2*r)/(2*np.pi), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("lam", [0.05, 2.0])
+def test_y2d_chains_converge_to_direct_sum(lam):
+    """The Yukawa expansions (Bessel argument i lam) through the same chains;
+    the M2L's Hankel row grows like K_n, so the rscales keep the terms in
+    range."""
+    errs = [_chains(p, 1j*lam)[0] for p in (2, 4, 8, 14)]
+    for name in errs[0]:
+        e = [x[name] for x in errs]
+        assert all(b < 0.5*a for a, b in zip(e, e[1:]) if a > 1e-13), (name, e)
+        assert e[-1] < 1e-11, (name, e)
