@@ -119,9 +119,10 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None)
 
 
 def tuned_policies(wr):
-    """The M2L split policies (items per CU, slots per split) the autotuner
-    chose for this wrangler's passes."""
-    return sorted({tuple(v) for v in wr.tree_indep.m2l_tuner._memo.values()})
+    """The M2L launch shapes the autotuner chose for this wrangler's passes:
+    (tile width, (items per CU, slots per split))."""
+    return sorted({(v[0], tuple(v[1])) if isinstance(v[1], tuple) else tuple(v)
+                   for v in wr.tree_indep.m2l_tuner._memo.values()})
 
 
 def time_passes(actx, fn, steps, warmup):

@@ -345,22 +345,24 @@ int sumpyhip_m2l_preprocess(int precision, int nboxes, int nstored,
  * entries of the compute precision. */
 typedef struct sumpyhip_m2l_level {
   int ntiles, nsplit;
-  const int *tile_targets;      /* ntiles x 64 target boxes (-1: none) */
+  const int *tile_targets;      /* ntiles x tile_width target boxes (-1: none) */
   const int *tile_slot_starts;  /* ntiles + 1 */
   const int *slot_class;        /* nslots: class of the slot (level-relative) */
-  const int *slot_sources;      /* nslots x 64 source boxes (-1: none) */
+  const int *slot_sources;      /* nslots x tile_width source boxes (-1: none) */
   const void *class_data;
   const void *packed_src;
   int src_base_ibox;
   void *work;
   int tgt_base_ibox;
   void *partial;
+  int tile_width;               /* targets per tile: 64 (0 means 64) or 32; the
+                                   levels of one call share it */
 } sumpyhip_m2l_level;
 
 /* Translation, replaces M2LUsingTranslationClassesDependentData.__call__
  * (sumpy/e2e.py:310-558) as a class-batched dense contraction on MFMA, for up
- * to 32 levels of the same orders in one launch: for each tile of 64 target
- * boxes and each class slot of the tile,
+ * to 32 levels of the same orders in one launch: for each tile of 64 (or 32,
+ * the levels' tile_width) target boxes and each class slot of the tile,
  * Y += A_c X with A_c[t, s] = data_c[off_t[t] + off_s[s]] and X the stacked
  * packed multipoles of the tile's sources.  Overwrites the work rows of the
  * tile targets.  The slots of a tile are split over nsplit blocks; splits

@@ -45,6 +45,7 @@ class M2LLevel(ctypes.Structure):
         ("class_data", ctypes.c_void_p), ("packed_src", ctypes.c_void_p),
         ("src_base_ibox", ctypes.c_int), ("work", ctypes.c_void_p),
         ("tgt_base_ibox", ctypes.c_int), ("partial", ctypes.c_void_p),
+        ("tile_width", ctypes.c_int),
     ]
 
 

@@ -3,12 +3,15 @@ cache (the analogue of sumpy's ``code_cache``, ``sumpy/__init__.py:69-70``,
 ``tools.py:474-514``, for tuning results instead of generated code).
 
 The class-batched M2L contraction (``sumpyhip_m2l_apply``) runs one block per
-(tile, split) item; how many splits a tile's class slots are cut into is a
-free choice (any choice gives the same sums up to the order of the split
-additions).  The default rule (:func:`sumpy_amd.e2e.choose_nsplit`: about 4
-items per CU, at least 12 slots per split) suits dense trees; sparse trees and
-other orders may prefer others.  :class:`M2LSplitTuner` times the contraction
-of a pass under a few policies and keeps the fastest, per
+(tile, split) item.  Two shapes are free choices: the tile width (64 or 32
+target boxes per tile: narrower tiles hold fewer empty columns on sparse
+levels, wider ones reuse each gathered class fragment across more columns),
+and how many splits a tile's class slots are cut into (any choice gives the
+same sums up to the order of the split additions).  The default (64 columns;
+:func:`sumpy_amd.e2e.choose_nsplit`: about 4 items per CU, at least 12 slots
+per split) suits dense trees; sparse trees and other orders may prefer
+others.  :class:`M2LSplitTuner` times the contraction of a pass under every
+tile width and split policy and keeps the fastest, per
 
     (library digest, kernel, expansion order, precision, GPU architecture,
      schedule digest)
@@ -91,6 +94,11 @@ def make_key(kernel, order, precision, arch, digest, version=None):
     return f"m2l_apply|{version}|{kernel}|{order}|{precision}|{arch}|{digest}"
 
 
+def make_shape_key(kernel, order, precision, arch, digest, version=None):
+    """The key of a (tile width, split policy) choice (:meth:`M2LSplitTuner.choose`)."""
+    return make_key(kernel, order, precision, arch, digest, version) + "|tile_width"
+
+
 def schedule_digest(entries):
     """Digest of the work shape of a batch of levels: tiles, slots and pairs
     per level, rounded to two significant digits so that trees of the same
@@ -161,6 +169,59 @@ class M2LSplitTuner:
                              "default_ms": round(times[DEFAULT_POLICY], 4)})
         return best
 
+    def _time(self, actx, entries, pol):
+        import torch
+
+        from sumpy_amd.e2e import m2l_apply_levels
+        stream = torch.cuda.current_stream(actx.device)
+        m2l_apply_levels(actx, entries, policy=pol)  # warm-up (and LDS limits)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(self.repeats):
+            m2l_apply_levels(actx, entries, policy=pol)
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop)/self.repeats
+
+    def choose(self, actx, make_entries, kernel, order, precision):
+        """(tile width, split policy) for a batch of levels: *make_entries*
+        (tile width) gives the batch's level entries for that width; every
+        width of :data:`sumpy_amd.e2e.TILE_WIDTHS` is timed under every split
+        policy.  Keyed by the default (64-column) schedule's digest."""
+        import torch
+
+        from sumpy_amd.e2e import TILE, TILE_WIDTHS
+        base = [e for e in make_entries(TILE) if e is not None]
+        if not base:
+            return TILE, DEFAULT_POLICY
+        key = make_shape_key(kernel, order, precision, device_arch(actx.device),
+                             schedule_digest(base))
+        got = self._memo.get(key)
+        if got is None:
+            stored = self.cache.get(key)
+            if stored is not None:
+                got = (int(stored["tile_width"]), tuple(stored["policy"]))
+        if got is not None:
+            self._memo[key] = got
+            return got
+        if torch.cuda.is_current_stream_capturing():
+            return TILE, DEFAULT_POLICY  # never time inside a graph capture
+        times = {}
+        for tw in TILE_WIDTHS:
+            ents = base if tw == TILE else [e for e in make_entries(tw) if e is not None]
+            for pol in SPLIT_POLICIES:
+                times[tw, pol] = self._time(actx, ents, pol)
+        best = min(times, key=times.get)
+        self._memo[key] = best
+        by_width = {str(tw): round(min(t for (w, _), t in times.items() if w == tw), 4)
+                    for tw in TILE_WIDTHS}
+        self.cache.put(key, {"tile_width": best[0], "policy": list(best[1]),
+                             "ms": round(times[best], 4),
+                             "default_ms": round(times[TILE, DEFAULT_POLICY], 4),
+                             "best_ms_by_tile_width": by_width})
+        return best
+
 
 __all__ = ["DEFAULT_POLICY", "M2LSplitTuner", "SPLIT_POLICIES", "TuneCache", "cache_dir",
-           "code_version", "make_key", "schedule_digest"]
+           "code_version", "make_key", "make_shape_key", "schedule_digest"]

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void m2l_postprocess_kernel(
 
 // {{{ the MFMA contraction
 
-constexpr int TW = 64;       // targets per tile
+constexpr int TW = 64;       // targets per tile (the widest tile; 32 is the other width)
 constexpr int MAXRBW = 4;    // row blocks of 16 per wave (npad <= 256)
 
 template <typename T>
@@ -179,6 +179,7 @@ constexpr int M2L_MAXLEV = 32;
 struct ApplyArgs {
   int nlev;
   int nl, nm, npad, kpad, ncirc;
+  int tw;               // targets per tile: 64 or 32 (every level of a launch)
   const int *off_t, *off_s;
   LevelArgs lv[M2L_MAXLEV];
 };
@@ -209,11 +210,12 @@ struct Limits {
   static constexpr int DPT = (MAXCIRC + 255)/256;   // class-data values per thread
 };
 
-template <typename T, int RBW>
+template <typename T, int RBW, int NCB>
 struct Apply {
   static constexpr int VN = Vec16<T>::N;
+  static constexpr int TWL = 16*NCB;  // targets per tile (column blocks of 16)
   // padded LDS row of X (FP64: 4 doubles, so that two blocks fit in one CU)
-  static constexpr int XS = TW + (sizeof(T) == 4 ? 16 : 4);
+  static constexpr int XS = TWL + (sizeof(T) == 4 ? 16 : 4);
   static constexpr int XPT = (Limits<RBW>::MAXK/VN*TW + 255)/256;  // X vectors per thread
   typedef typename Vec16<T>::type vec_t;
 };
@@ -231,10 +233,11 @@ struct Apply {
 // The next slot's data are loaded into registers before the MFMA loop of the
 // current slot, so global-memory latency overlaps the contraction; LDS is
 // rewritten between two barriers.
-template <typename T, int RBW>
+template <typename T, int RBW, int NCB>
 __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   typedef typename Mfma<T>::acc_t acc_t;
-  typedef Apply<T, RBW> C;
+  typedef Apply<T, RBW, NCB> C;
+  constexpr int TWL = C::TWL;
   typedef typename C::vec_t vec_t;
   constexpr int DPT = Limits<RBW>::DPT;
   extern __shared__ unsigned char lds_raw[];
@@ -280,7 +283,7 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
       const int e = tid + 256*i;
       dreg[i] = D[e < a.ncirc ? e : 0];
     }
-    const int sb = L.slot_sources[(size_t)slot*TW + lane];
+    const int sb = lane < TWL ? L.slot_sources[(size_t)slot*TWL + lane] : -1;
     const T *row = packed + (size_t)((sb >= 0 ? sb : L.sbase) - L.sbase)*a.kpad;
     for (int i = 0; i < C::XPT; ++i) {
       const int q = wave + 4*i;
@@ -297,7 +300,7 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
     }
     for (int i = 0; i < C::XPT; ++i) {
       const int q = wave + 4*i;
-      if (q < kq) {
+      if (q < kq && lane < TWL) {
         const T *xv = reinterpret_cast<const T *>(&xreg[i]);
         for (int j = 0; j < C::VN; ++j) sX[(q*C::VN + j)*C::XS + lane] = xv[j];
       }
@@ -308,9 +311,9 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
   const int *olane = sOff + (lane >> 4);
   {
     const int tile = li/L.nsplit, split = li - tile*L.nsplit;
-    acc_t acc[RBW][4];
+    acc_t acc[RBW][NCB];
     for (int r = 0; r < RBW; ++r)
-      for (int cb = 0; cb < 4; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
+      for (int cb = 0; cb < NCB; ++cb) acc[r][cb] = acc_t{0, 0, 0, 0};
     const int t0 = L.tile_slot_starts[tile], nsl = L.tile_slot_starts[tile + 1] - t0;
     const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
     const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
@@ -330,14 +333,14 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
         const int os = os_next;
         os_next = olane[k0 + 4 < a.kpad ? k0 + 4 : 0];
         const T *xr = xlane + k0*C::XS;
-        T bfr[4];
+        T bfr[NCB];
         #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) bfr[cb] = xr[cb*16];
+        for (int cb = 0; cb < NCB; ++cb) bfr[cb] = xr[cb*16];
         #pragma unroll
         for (int r = 0; r < RBW; ++r) {
           const T af = sD[offT[r] + os];
           #pragma unroll
-          for (int cb = 0; cb < 4; ++cb) acc[r][cb] = Mfma<T>::mma(af, bfr[cb], acc[r][cb]);
+          for (int cb = 0; cb < NCB; ++cb) acc[r][cb] = Mfma<T>::mma(af, bfr[cb], acc[r][cb]);
         }
       }
       __syncthreads();
@@ -351,14 +354,14 @@ __global__ __launch_bounds__(256) void m2l_apply_kernel(ApplyArgs a) {
     for (int r = 0; r < RBW; ++r) {
       const int rb = wave + 4*r;
       if (rb >= nrb) break;
-      for (int cb = 0; cb < 4; ++cb) {
+      for (int cb = 0; cb < NCB; ++cb) {
         const int col = cb*16 + (lane & 15);
-        const int tb = L.tile_targets[(size_t)tile*TW + col];
+        const int tb = L.tile_targets[(size_t)tile*TWL + col];
         if (tb < 0) continue;
         T *dst = split == 0
             ? work + (size_t)(tb - L.tbase)*a.npad
             : reinterpret_cast<T *>(L.partial) +
-                  ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
+                  ((size_t)(split - 1)*L.ntiles*TWL + (size_t)tile*TWL + col)*a.npad;
         for (int i = 0; i < 4; ++i) dst[rb*16 + Mfma<T>::row(lane, i)] = acc[r][cb][i];
       }
     }
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(256) void m2l_reduce_splits(ApplyArgs a) {
   const int npad = a.npad;
   const T *partial = reinterpret_cast<const T *>(L.partial);
   T *work = reinterpret_cast<T *>(L.work);
-  const size_t npos = (size_t)L.ntiles*TW, n = npos*npad;
+  const size_t npos = (size_t)L.ntiles*a.tw, n = npos*npad;
   for (size_t i = blockIdx.x*(size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x*blockDim.x) {
     const size_t pos = i/npad;
@@ -453,15 +456,16 @@ __global__ void m2l_preprocess_bf16_kernel(int nboxes, int nstored, const double
 // register r = row 4 (lane / 16) + r.
 constexpr int BF16_MAXCIRC = 2560;  // packed class words per slot (LDS)
 
-template <int RBW>
+template <int RBW, int NCB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 void m2l_apply_bf16_kernel(ApplyArgs a) {
+  constexpr int TWL = 16*NCB;  // targets per tile
   constexpr int DPT = (BF16_MAXCIRC + 255)/256;
   extern __shared__ unsigned char lds_raw[];
   unsigned *sD = reinterpret_cast<unsigned *>(lds_raw);                  // [ncirc]
   __bf16 *sXh = reinterpret_cast<__bf16 *>(sD + ((a.ncirc + 3) & ~3));  // [64][XB]
-  __bf16 *sXl = sXh + TW*XB;                                             // [64][XB]
-  int *sOff = reinterpret_cast<int *>(sXl + TW*XB);                      // [kpad]
+  __bf16 *sXl = sXh + TWL*XB;                                            // [TWL][XB]
+  int *sOff = reinterpret_cast<int *>(sXl + TWL*XB);                     // [kpad]
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kq = a.kpad/8;  // 8-wide k chunks per row
@@ -497,7 +501,9 @@ void m2l_apply_bf16_kernel(ApplyArgs a) {
   const int s0 = t0 + (int)((long long)nsl*split/L.nsplit);
   const int s1 = t0 + (int)((long long)nsl*(split + 1)/L.nsplit);
   // source box of this lane's column in a slot (loaded one fetch ahead)
-  auto meta = [&](int slot) { return L.slot_sources[(size_t)min(slot, s1 - 1)*TW + lane]; };
+  auto meta = [&](int slot) {
+    return lane < TWL ? L.slot_sources[(size_t)min(slot, s1 - 1)*TWL + lane] : -1;
+  };
   auto fetch_d = [&](int slot, unsigned (&d)[DPT]) {
     const unsigned *D = cwords + (size_t)L.slot_class[min(slot, s1 - 1)]*a.ncirc;
     for (int i = 0; i < DPT; ++i) {
@@ -522,16 +528,16 @@ void m2l_apply_bf16_kernel(ApplyArgs a) {
     }
     for (int i = 0; i < 4; ++i) {
       const int m = wave + 4*i;
-      if (m < kq) {
+      if (m < kq && lane < TWL) {
         *reinterpret_cast<u4 *>(sXh + lane*XB + m*8) = g.xh[i];
         *reinterpret_cast<u4 *>(sXl + lane*XB + m*8) = g.xl[i];
       }
     }
   };
   typedef float v4f_t __attribute__((ext_vector_type(4)));
-  v4f_t acc[RBW][4];
+  v4f_t acc[RBW][NCB];
   for (int q = 0; q < RBW; ++q)
-    for (int cb = 0; cb < 4; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
+    for (int cb = 0; cb < NCB; ++cb) acc[q][cb] = v4f_t{0, 0, 0, 0};
   const int kg = (lane >> 4)*8;  // first k of this lane inside a 32-deep step
   const int nks = a.kpad/KB;
   auto gather = [&](int k0, unsigned (&w)[RBW][8]) {
@@ -572,8 +578,8 @@ void m2l_apply_bf16_kernel(ApplyArgs a) {
       bl[0] = *reinterpret_cast<const bf16x8 *>(sXl + (lane & 15)*XB + k0 + kg);
       if (ks + 1 < nks) gather(k0 + KB, wn);
       #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        if (cb + 1 < 4) {
+      for (int cb = 0; cb < NCB; ++cb) {
+        if (cb + 1 < NCB) {
           const int col = (cb + 1)*16 + (lane & 15);
           bh[(cb + 1) & 1] = *reinterpret_cast<const bf16x8 *>(sXh + col*XB + k0 + kg);
           bl[(cb + 1) & 1] = *reinterpret_cast<const bf16x8 *>(sXl + col*XB + k0 + kg);
@@ -616,40 +622,43 @@ void m2l_apply_bf16_kernel(ApplyArgs a) {
   for (int q = 0; q < RBW; ++q) {
     const int rb = wave + 4*q;
     if (rb >= nrb) break;
-    for (int cb = 0; cb < 4; ++cb) {
+    for (int cb = 0; cb < NCB; ++cb) {
       const int col = cb*16 + (lane & 15);
-      const int tb = L.tile_targets[(size_t)tile*TW + col];
+      const int tb = L.tile_targets[(size_t)tile*TWL + col];
       if (tb < 0) continue;
       float *dst = split == 0
           ? work + (size_t)(tb - L.tbase)*a.npad
           : reinterpret_cast<float *>(L.partial) +
-                ((size_t)(split - 1)*L.ntiles*TW + (size_t)tile*TW + col)*a.npad;
+                ((size_t)(split - 1)*L.ntiles*TWL + (size_t)tile*TWL + col)*a.npad;
       for (int i = 0; i < 4; ++i) dst[rb*16 + 4*(lane >> 4) + i] = acc[q][cb][i];
     }
   }
 }
 
-template <int RBW>
+template <int RBW, int NCB>
 int launch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
   SUMPY_REQUIRE(a.kpad % KB == 0 && a.kpad <= 128,
                 "m2l_apply (bf16x3): kpad %d must be a multiple of 32 and <= 128", a.kpad);
   const size_t lds = sizeof(unsigned)*((size_t)((a.ncirc + 3) & ~3)) +
-                     2*sizeof(__bf16)*TW*XB + sizeof(int)*a.kpad;
+                     2*sizeof(__bf16)*(16*NCB)*XB + sizeof(int)*a.kpad;
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
-  int rc = set_lds_limit((const void *)m2l_apply_bf16_kernel<RBW>, lds);
+  int rc = set_lds_limit((const void *)m2l_apply_bf16_kernel<RBW, NCB>, lds);
   if (rc) return rc;
   long long per_xcd = 0;
   for (int l = 0; l < a.nlev; ++l)
     per_xcd += (a.lv[l].item_end - (l ? a.lv[l - 1].item_end : 0) + 7)/8;
-  m2l_apply_bf16_kernel<RBW><<<(int)(8*per_xcd), 256, lds, st>>>(a);
+  m2l_apply_bf16_kernel<RBW, NCB><<<(int)(8*per_xcd), 256, lds, st>>>(a);
   return check_launch("m2l_apply_bf16");
 }
 
 int dispatch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
   const int nrb = a.npad/16;
   int rc;
-  if (nrb <= 4 && a.ncirc <= BF16_MAXCIRC) rc = launch_apply_bf16<1>(a, st);
-  else if (nrb <= 8 && a.ncirc <= BF16_MAXCIRC) rc = launch_apply_bf16<2>(a, st);
+  const bool narrow = a.tw == 32;
+  if (nrb <= 4 && a.ncirc <= BF16_MAXCIRC)
+    rc = narrow ? launch_apply_bf16<1, 2>(a, st) : launch_apply_bf16<1, 4>(a, st);
+  else if (nrb <= 8 && a.ncirc <= BF16_MAXCIRC)
+    rc = narrow ? launch_apply_bf16<2, 2>(a, st) : launch_apply_bf16<2, 4>(a, st);
   else
     return set_error(SUMPYHIP_ENOTIMPL,
                      "m2l_apply (bf16x3): npad %d / ncirc %d exceed the supported sizes",
@@ -657,7 +666,7 @@ int dispatch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
   if (rc) return rc;
   size_t nmax = 0;
   for (int l = 0; l < a.nlev; ++l)
-    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*TW*a.npad);
+    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*a.tw*a.npad);
   if (nmax == 0) return SUMPYHIP_OK;
   const int rgrid = (int)((nmax + 255)/256 < 8192 ? (nmax + 255)/256 : 8192);
   m2l_reduce_splits<float><<<dim3(rgrid, a.nlev), 256, 0, st>>>(a);
@@ -666,40 +675,46 @@ int dispatch_apply_bf16(const ApplyArgs &a, hipStream_t st) {
 
 // }}}
 
-template <typename T, int RBW>
+template <typename T, int RBW, int NCB>
 int launch_apply(const ApplyArgs &a, hipStream_t st) {
-  const size_t lds = sizeof(T)*((size_t)((a.ncirc + 3) & ~3) + (size_t)a.kpad*Apply<T, RBW>::XS) +
+  const size_t lds = sizeof(T)*((size_t)((a.ncirc + 3) & ~3) +
+                                (size_t)a.kpad*Apply<T, RBW, NCB>::XS) +
                      sizeof(int)*a.kpad;
   SUMPY_REQUIRE(lds <= 160*1024, "m2l_apply: %zu bytes of LDS needed (order too high)", lds);
-  int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW>, lds);
+  int rc = set_lds_limit((const void *)m2l_apply_kernel<T, RBW, NCB>, lds);
   if (rc) return rc;
   long long per_xcd = 0;  // items of XCD 0 (the largest share)
   for (int l = 0; l < a.nlev; ++l)
     per_xcd += (a.lv[l].item_end - (l ? a.lv[l - 1].item_end : 0) + 7)/8;
   const int grid = (int)(8*per_xcd);
-  m2l_apply_kernel<T, RBW><<<grid, 256, lds, st>>>(a);
+  m2l_apply_kernel<T, RBW, NCB><<<grid, 256, lds, st>>>(a);
   rc = check_launch("m2l_apply");
   if (rc) return rc;
   size_t nmax = 0;
   for (int l = 0; l < a.nlev; ++l)
-    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*TW*a.npad);
+    if (a.lv[l].nsplit > 1) nmax = std::max(nmax, (size_t)a.lv[l].ntiles*a.tw*a.npad);
   if (nmax == 0) return SUMPYHIP_OK;
   const int rgrid = (int)((nmax + 255)/256 < 8192 ? (nmax + 255)/256 : 8192);
   m2l_reduce_splits<T><<<dim3(rgrid, a.nlev), 256, 0, st>>>(a);
   return check_launch("m2l_reduce_splits");
 }
 
+template <typename T, int RBW>
+int launch_apply_tw(const ApplyArgs &a, hipStream_t st) {
+  return a.tw == 32 ? launch_apply<T, RBW, 2>(a, st) : launch_apply<T, RBW, 4>(a, st);
+}
+
 template <typename T>
 int dispatch_apply(const ApplyArgs &a, hipStream_t st) {
   const int nrb = a.npad/16;
   if (nrb <= 4 && a.kpad <= Limits<1>::MAXK && a.ncirc <= Limits<1>::MAXCIRC)
-    return launch_apply<T, 1>(a, st);
+    return launch_apply_tw<T, 1>(a, st);
   if (nrb <= 8 && a.kpad <= Limits<2>::MAXK && a.ncirc <= Limits<2>::MAXCIRC)
-    return launch_apply<T, 2>(a, st);
+    return launch_apply_tw<T, 2>(a, st);
   if (nrb <= 12 && a.kpad <= Limits<3>::MAXK && a.ncirc <= Limits<3>::MAXCIRC)
-    return launch_apply<T, 3>(a, st);
+    return launch_apply_tw<T, 3>(a, st);
   if (nrb <= 16 && a.kpad <= Limits<4>::MAXK && a.ncirc <= Limits<4>::MAXCIRC)
-    return launch_apply<T, 4>(a, st);
+    return launch_apply_tw<T, 4>(a, st);
   return set_error(SUMPYHIP_ENOTIMPL,
                    "m2l_apply: npad %d / kpad %d / ncirc %d exceed the supported sizes",
                    a.npad, a.kpad, a.ncirc);
@@ -868,10 +883,15 @@ extern "C" int sumpyhip_m2l_apply(int precision, int nlevels,
   ApplyArgs a{};
   a.nl = nl; a.nm = nm; a.npad = npad; a.kpad = kpad; a.ncirc = ncirc;
   a.off_t = off_t; a.off_s = off_s;
+  a.tw = 0;
   long long items = 0;
   for (int l = 0; l < nlevels; ++l) {
     const sumpyhip_m2l_level &in = levels[l];
     if (in.ntiles == 0) continue;
+    const int tw = in.tile_width ? in.tile_width : TW;
+    SUMPY_REQUIRE(tw == 64 || tw == 32, "m2l_apply: tile width %d (64 or 32)", tw);
+    SUMPY_REQUIRE(a.tw == 0 || a.tw == tw, "m2l_apply: levels of one launch share the tile width");
+    a.tw = tw;
     SUMPY_REQUIRE(in.ntiles > 0 && in.nsplit >= 1, "m2l_apply: bad tiles/splits");
     SUMPY_REQUIRE(in.nsplit == 1 || in.partial != nullptr,
                   "m2l_apply: nsplit > 1 needs a partial buffer");
@@ -909,7 +929,8 @@ extern "C" int sumpyhip_m2l_apply_noalloc(int precision, int nlevels,
     lv[l] = levels[l];
     lv[l].partial = nullptr;
     if (lv[l].ntiles > 0 && lv[l].nsplit > 1) {
-      const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*TW*(size_t)npad*
+      const size_t tw = lv[l].tile_width ? lv[l].tile_width : TW;
+      const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*tw*(size_t)npad*
                            precision_bytes(precision);
       need += (bytes + 255) & ~(size_t)255;
     }
@@ -925,7 +946,8 @@ extern "C" int sumpyhip_m2l_apply_noalloc(int precision, int nlevels,
     for (int l = 0; l < nlevels; ++l) {
       if (lv[l].ntiles > 0 && lv[l].nsplit > 1) {
         lv[l].partial = static_cast<unsigned char *>(base) + off;
-        const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*TW*(size_t)npad*
+        const size_t tw = lv[l].tile_width ? lv[l].tile_width : TW;
+        const size_t bytes = (size_t)(lv[l].nsplit - 1)*(size_t)lv[l].ntiles*tw*(size_t)npad*
                              precision_bytes(precision);
         off += (bytes + 255) & ~(size_t)255;
       }

@@ -24,7 +24,8 @@ from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_
 from sumpy_amd.profiling import kernel_span
 from sumpy_amd.tools import as_i32, expansion_k, planes, set_planes
 
-TILE = 64  # target boxes per M2L tile (matches the HIP kernel)
+TILE = 64  # target boxes per M2L tile by default (the HIP kernel's widest tile)
+TILE_WIDTHS = (64, 32)  # the tile widths the contraction kernels are built for
 GROUP = 16  # target boxes per octant-translation group (matches the HIP kernel)
 MIN_SLOTS_PER_SPLIT = 12
 
@@ -568,9 +569,13 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         return out
 
     def schedule(self, actx, target_boxes, src_box_starts, src_box_lists, class_lists,
-                 class_level_start, nclasses, centers):
-        """Tile schedule of one level, built on the host and cached."""
-        tile = TILE//2 if self.tgt_expansion.is_complex_valued else TILE
+                 class_level_start, nclasses, centers, tile_width=TILE):
+        """Tile schedule of one level, built on the host and cached;
+        *tile_width* columns per tile of the contraction (64 or 32; a complex
+        target takes two)."""
+        if tile_width not in TILE_WIDTHS:
+            raise ValueError(f"tile width {tile_width}: one of {TILE_WIDTHS}")
+        tile = tile_width//2 if self.tgt_expansion.is_complex_valued else tile_width
         key = tuple((t.data_ptr(), t.numel(), t._version) for t in
                     (target_boxes, src_box_starts, src_box_lists, class_lists, centers)) \
             + (int(class_level_start), int(nclasses), tile, str(actx.device))
@@ -586,30 +591,32 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
             per_lists[key] = sched
         return sched
 
-    def level_entry(self, actx, **kwargs):
+    def level_entry(self, actx, tile_width=TILE, **kwargs):
         """The launch description of one level (for :func:`m2l_apply_levels`),
-        or None when the level has no target tiles."""
+        or None when the level has no target tiles; *tile_width*: columns
+        per tile of the contraction (64 or 32)."""
         data = kwargs["m2l_translation_classes_dependent_data"]
         src, work = kwargs["src_expansions"], kwargs["tgt_expansions"]
         sched = self.schedule(
             actx, kwargs["target_boxes"], kwargs["src_box_starts"], kwargs["src_box_lists"],
             kwargs["m2l_translation_classes_lists"],
-            kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"])
+            kwargs["translation_classes_level_start"], data.shape[0], kwargs["centers"],
+            tile_width=tile_width)
         if sched["ntiles"] == 0:
             return None
         sbase, tbase = int(kwargs["src_base_ibox"]), int(kwargs["tgt_base_ibox"])
         if self.tgt_expansion.is_complex_valued:
-            sched = self._complex_schedule(actx, sched, sbase, tbase)
+            sched = self._complex_schedule(actx, sched, sbase, tbase, tile_width)
             nb_s, nb_t = int(src.shape[0]), int(work.shape[0])
             return dict(op=self, sched=sched, data=data.contiguous().view(2*data.shape[0], -1),
                         src=src.view(3*nb_s, -1), sbase=0, work=work.view(2*nb_t, -1),
-                        tbase=0)
+                        tbase=0, tw=tile_width)
         return dict(op=self, sched=sched, data=data.contiguous(), src=src,
-                    sbase=sbase, work=work, tbase=tbase)
+                    sbase=sbase, work=work, tbase=tbase, tw=tile_width)
 
-    def _complex_schedule(self, actx, sched, sbase, tbase):
-        """The schedule of the real embedding.  A tile of TILE/2 complex
-        targets becomes one tile of TILE columns: column 2j is the row Lr
+    def _complex_schedule(self, actx, sched, sbase, tbase, tile_width=TILE):
+        """The schedule of the real embedding.  A tile of tile_width/2 complex
+        targets becomes one tile of tile_width columns: column 2j is the row Lr
         (2 (t_j - tbase)), column 2j + 1 the row Li.  Class slot c becomes the
         slots of Ar (class 2c) and Ai (2c + 1), fed by the source rows
         3 (s - sbase) + (0: Mr, 1: Mi, 2: -Mi):
@@ -620,7 +627,7 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         if out is not None:
             return out
         h = sched["host"]
-        half = TILE//2
+        half = tile_width//2
         tt = h["tile_targets"].astype(np.int64).reshape(-1, half)
         ss = h["slot_sources"].astype(np.int64).reshape(-1, half)
         sc = h["slot_class"].astype(np.int64)
@@ -630,7 +637,7 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
                          mult*(ids[..., None] - base) + np.array(offs), -1)
             return r.reshape(ids.shape[0], -1)
 
-        vt = rows(tt, tbase, 2, (0, 1))                      # (ntiles, TILE)
+        vt = rows(tt, tbase, 2, (0, 1))                      # (ntiles, tile_width)
         vs = np.stack([rows(ss, sbase, 3, (0, 1)),           # Ar <- (Mr, Mi)
                        rows(ss, sbase, 3, (2, 0))], axis=1)  # Ai <- (-Mi, Mr)
         vc = np.stack([2*sc, 2*sc + 1], axis=1)
@@ -676,6 +683,9 @@ def m2l_apply_levels(actx, entries, policy=None, noalloc=False):
             raise ValueError("m2l_apply_levels: levels must share orders and precision")
     entries = sorted(entries, key=lambda e: -e["sched"]["npairs"])
     npad, kpad = int(entries[0]["work"].shape[1]), int(entries[0]["src"].shape[1])
+    tw = entries[0].get("tw", TILE)
+    if any(e.get("tw", TILE) != tw for e in entries):
+        raise ValueError("m2l_apply_levels: levels of one launch share the tile width")
     nsplits = [e["sched"]["nsplit"] if policy is None else
                choose_nsplit(e["sched"]["ntiles"], e["sched"]["nslots"], actx.device, *policy)
                for e in entries]
@@ -687,7 +697,7 @@ def m2l_apply_levels(actx, entries, policy=None, noalloc=False):
             raise ValueError("m2l_apply_levels: inconsistent row padding")
         partial = None
         if sched["nsplit"] > 1 and not noalloc:
-            partial = torch.empty((sched["nsplit"] - 1)*sched["ntiles"]*TILE*npad,
+            partial = torch.empty((sched["nsplit"] - 1)*sched["ntiles"]*tw*npad,
                                   dtype=work.dtype, device=work.device)
             keep.append(partial)
         npairs += sched["npairs"]
@@ -698,7 +708,7 @@ def m2l_apply_levels(actx, entries, policy=None, noalloc=False):
             sched["ntiles"], sched["nsplit"], _capi.ptr(sched["tile_targets"]),
             _capi.ptr(sched["tile_slot_starts"]), _capi.ptr(sched["slot_class"]),
             _capi.ptr(sched["slot_sources"]), _capi.ptr(data), _capi.ptr(e["src"]),
-            e["sbase"], _capi.ptr(work), e["tbase"], _capi.ptr(partial)))
+            e["sbase"], _capi.ptr(work), e["tbase"], _capi.ptr(partial), tw))
     arr = (_capi.M2LLevel*len(levels))(*levels)
     flops = 2.0*m["nl"]*m["nm"]*npairs
     if tr.precision == "bf16x3":

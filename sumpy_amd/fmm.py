@@ -519,7 +519,7 @@ class SumpyExpansionWrangler:
         postprocessed in one launch (source and target of an M2L share the
         level, so the postprocessing scale ratio is 1 on every level); the
         result equals the reference's per-level loop (``fmm.py:777-920``)."""
-        from sumpy_amd.e2e import m2l_apply_levels
+        from sumpy_amd.e2e import TILE, m2l_apply_levels
         local_exps = self.local_expansion_zeros(actx)
         starts = _host(level_start_target_box_nrs)
         # translations without preprocessing (the 2D Bessel locals) take the
@@ -556,12 +556,11 @@ class SumpyExpansionWrangler:
                     kwargs["m2l_translation_classes_dependent_data"].numel() == 0:
                 continue
             if use_pre:
-                batches.setdefault(order, []).append(
-                    (lev, m2l.level_entry(actx, **kwargs), tview))
+                batches.setdefault(order, []).append((lev, m2l, kwargs, tview))
             else:
                 m2l(actx, **kwargs)
         for order, levs in batches.items():
-            runs = _consecutive_runs([lev for lev, _, _ in levs])
+            runs = _consecutive_runs([lev for lev, _, _, _ in levs])
             mstarts = self.multipole_expansions_level_starts()
             for l0, l1 in runs:
                 self.tree_indep.m2l_preprocess_mpole_kernel(order, order)(
@@ -569,13 +568,17 @@ class SumpyExpansionWrangler:
                     preprocessed_src_expansions=self._levels_view(
                         pre_mpoles.flat, pre_mpoles.starts, l0, l1),
                     src_rscale=self.level_to_rscale(l0), **self.kernel_extra_kwargs)
-            ents = [entry for _, entry, _ in levs]
-            policy = None
+            def entries(tile_width, levs=levs):
+                return [op.level_entry(actx, tile_width=tile_width, **kw)
+                        for _, op, kw, _ in levs]
+
+            policy, tile_width = None, TILE
             if self.tree_indep.autotune:
-                policy = self.tree_indep.m2l_tuner.policy(
-                    actx, ents, str(self.tree_indep.get_base_kernel()), order,
+                # the split policy and the tile width of the order's batch
+                tile_width, policy = self.tree_indep.m2l_tuner.choose(
+                    actx, entries, str(self.tree_indep.get_base_kernel()), order,
                     self.tree_indep.m2l_translation.precision)
-            m2l_apply_levels(actx, ents, policy=policy)
+            m2l_apply_levels(actx, entries(tile_width), policy=policy)
             lstarts = self.local_expansions_level_starts()
             for l0, l1 in runs:
                 self.tree_indep.m2l_postprocess_local_kernel(order, order)(

@@ -40,6 +40,26 @@ def test_policy_read_from_disk_without_timing(tmp_path):
     assert tuner.policy(Actx(), ents, "LapKnl3D", 10, "fp32") == (8, 24)
 
 
+def test_shape_choice_read_from_disk_without_timing(tmp_path):
+    """A stored (tile width, split policy) choice is returned without building
+    the 32-column entries or timing anything."""
+    class Actx:
+        device = "cpu"
+    ents = _entries((10, 100, 1000))
+    key = autotune.make_shape_key("LapKnl3D", 10, "bf16x3", "cpu",
+                                  autotune.schedule_digest(ents))
+    cache = autotune.TuneCache(str(tmp_path))
+    cache.put(key, {"tile_width": 32, "policy": [2, 24]})
+    tuner = autotune.M2LSplitTuner(cache=autotune.TuneCache(str(tmp_path)))
+    asked = []
+
+    def make_entries(tw):
+        asked.append(tw)
+        return ents
+    assert tuner.choose(Actx(), make_entries, "LapKnl3D", 10, "bf16x3") == (32, (2, 24))
+    assert asked == [64]
+
+
 def test_key_carries_library_digest(tmp_path):
     """A choice stored under another build of the library is not read back."""
     class Actx:
@@ -91,5 +111,7 @@ def test_autotuned_pass_matches_default(actx, tmp_path, monkeypatch):
     data = json.load(open(tmp_path/"autotune.json"))
     (key, val), = data.items()
     assert key.startswith(f"m2l_apply|{autotune.code_version()}|LapKnl3D|8|fp32|")
+    assert key.endswith("|tile_width")
     assert tuple(val["policy"]) in autotune.SPLIT_POLICIES
+    assert val["tile_width"] in (64, 32)
     assert val["ms"] <= val["default_ms"]

@@ -60,9 +60,10 @@ def _level(actx, n, order, precision, seed):
     keep = dict(tree=tree, trav=trav, data=data, pre=pre, tb=tb, starts=starts,
                 classes=classes)
 
-    def entry(work):
+    def entry(work, tile_width=64):
         return op.level_entry(
-            actx, src_expansions=pre, src_base_ibox=int(lbs[lev]), tgt_expansions=work,
+            actx, tile_width=tile_width, src_expansions=pre, src_base_ibox=int(lbs[lev]),
+            tgt_expansions=work,
             tgt_base_ibox=int(lbs[lev]), target_boxes=tb, src_box_starts=starts,
             src_box_lists=trav.from_sep_siblings_lists, centers=tree.box_centers,
             src_rscale=rs, tgt_rscale=rs, m2l_translation_classes_dependent_data=data,
