@@ -109,7 +109,8 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None)
                 m2l_translation_override=tr),
         [knl], exclude_self=True, p2p_precision=p2p_precision, autotune=True,
         list34_precision=p2p_precision if k is None else "fp64",
-        l2p_precision=p2p_precision if k is None else "fp64")
+        l2p_precision=p2p_precision if k is None else "fp64",
+        p2m_precision=p2p_precision if k is None else "fp64")
     import torch
     t2s = torch.arange(tree.ntargets, dtype=torch.int32, device=actx.device)
     return SumpyExpansionWrangler(ti, trav, np.float64 if k is None else np.complex128,

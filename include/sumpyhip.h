@@ -198,14 +198,17 @@ int sumpyhip_m2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,
  * (sumpy/fmm.py:609-641) over P2EFromSingleBox.__call__ (sumpy/p2e.py:
  * 155-261) for the Laplace 3D kernel when every level has the same expansion
  * order: box b has the rscale level_rscale[box_levels[b]] and its multipole
- * at row b of tgt_expansions (one row per tree box); overwrites. */
+ * at row b of tgt_expansions (one row per tree box); overwrites.  precision 1:
+ * the source moments and their sums over a box in FP32 (orders 1-12 with the
+ * full Taylor set), projected to the stored coefficients in FP64. */
 int sumpyhip_p2m_levels(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
                         const int *source_boxes, const int *box_source_starts,
                         const int *box_source_counts_nonchild, const double *box_centers,
                         int nboxes, const double *sx, const double *sy, const double *sz,
                         const double *w0, const double *wx, const double *wy,
                         const double *wz, const int *box_levels, int nlevels,
-                        const double *level_rscale, double *tgt_expansions, void *stream);
+                        const double *level_rscale, double *tgt_expansions, int precision,
+                        void *stream);
 
 /* L2P for target boxes of several tree levels in one launch, replaces the
  * per-level loop of SumpyExpansionWrangler.eval_locals (sumpy/fmm.py:

@@ -91,7 +91,7 @@ _SIGNATURES = {
                            _P, _P, _P, _P, _P],
     "sumpytree_gpu_list2_classes": [_I, _P, _P, _P, _I, _P, _P],
     "sumpyhip_p2m_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P,
-                            _I, _P, _P, _P],
+                            _I, _P, _P, _I, _P],
     "sumpyhip_l2p_levels": [_I, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _D, _P,
                             _P, _P, _P, _P, _I, _P],
     "sumpyhip_m2p_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P,

@@ -290,7 +290,8 @@ def adjoint_wrangler(wrangler, target_kernels=None):
                 ti._setup_actx, ti.multipole_expansion_factory, ti.local_expansion_factory,
                 list(tk), exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
                 p2p_precision=ti.p2p_precision, autotune=ti.autotune,
-                list34_precision=ti.list34_precision, l2p_precision=ti.l2p_precision)
+                list34_precision=ti.list34_precision, l2p_precision=ti.l2p_precision,
+                p2m_precision=ti.p2m_precision)
             ti._memo[("adjoint_tree_indep", tk)] = ati
     trav = swapped_traversal(ti._setup_actx, wrangler.traversal)
     orders = list(wrangler.level_orders)
@@ -326,7 +327,7 @@ def _gradient_wrangler(wrangler):
         exclude_self=ti.exclude_self, use_rscale=ti.use_rscale,
         p2p_precision=ti.p2p_precision, autotune=ti.autotune,
         list34_precision=ti.list34_precision,
-        l2p_precision=ti.l2p_precision)
+        l2p_precision=ti.l2p_precision, p2m_precision=ti.p2m_precision)
     orders = list(wrangler.level_orders)
     got = SumpyExpansionWrangler(
         gti, wrangler.traversal, wrangler.dtype,

@@ -190,7 +190,10 @@ __global__ __launch_bounds__(P2M_NT) void p2m_lap3d(
 // coefficients are then M = P^T C (sumpy/expansion/multipole.py:78-114).
 constexpr int P2M_NW = 4;
 
-template <int ORD, bool DIP>
+// T = float: the source tables, the (i, j, k) moments and their sums over a
+// box's sources in FP32 (box-relative scaled coordinates, |a/rho| <~ 1), the
+// projection to the stored coefficients in FP64
+template <int ORD, bool DIP, typename T = double>
 __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
     sumpyhip_expn e, int nboxes_list, const int *__restrict__ source_boxes,
     const int *__restrict__ bss, const int *__restrict__ bsc,
@@ -204,9 +207,9 @@ __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
   constexpr int PPL = (NP + 63)/64;  // pairs per lane
   constexpr int NF = P1*(P1 + 1)*(P1 + 2)/6;
   constexpr int CHK = 32;            // sources per table chunk
-  __shared__ double sX[P2M_NW][CHK][P1], sY[P2M_NW][CHK][P1];
-  __shared__ double sZ[P2M_NW][CHK][DIP ? 4 : 2];  // a_z / rho, w0 (wx, wy / rho)
-  __shared__ double sWz[P2M_NW][DIP ? CHK : 1];    // wz / rho
+  __shared__ T sX[P2M_NW][CHK][P1], sY[P2M_NW][CHK][P1];
+  __shared__ T sZ[P2M_NW][CHK][DIP ? 4 : 2];  // a_z / rho, w0 (wx, wy / rho)
+  __shared__ T sWz[P2M_NW][DIP ? CHK : 1];    // wz / rho
   __shared__ double sC[P2M_NW][NF];
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -225,53 +228,53 @@ __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
     const double rinv = 1.0/(box_levels ? ls.r[box_levels[b]] : rscale);
     const double c0 = centers[b], c1 = centers[nboxes + b], c2 = centers[2*(size_t)nboxes + b];
     const int s0 = bss[b], ns = bsc[b];
-    double acc[PPL][P1];
+    T acc[PPL][P1];
     #pragma unroll
     for (int r = 0; r < PPL; ++r)
       #pragma unroll
-      for (int k = 0; k < P1; ++k) acc[r][k] = 0.0;
+      for (int k = 0; k < P1; ++k) acc[r][k] = T(0);
     for (int sc = 0; sc < ns; sc += CHK) {
       const int nn = min(CHK, ns - sc);
       wave_sync();
       if (lane < nn) {
         const int s = s0 + sc + lane;
-        const double ax = (c0 - sx[s])*rinv, ay = (c1 - sy[s])*rinv;
-        double xv = 1, yv = 1;
+        const T ax = T((c0 - sx[s])*rinv), ay = T((c1 - sy[s])*rinv);
+        T xv = 1, yv = 1;
         #pragma unroll
         for (int q = 0; q < P1; ++q) {
           sX[w][lane][q] = xv;
           sY[w][lane][q] = yv;
-          xv *= ax*(1.0/(q + 1));
-          yv *= ay*(1.0/(q + 1));
+          xv *= ax*T(1.0/(q + 1));
+          yv *= ay*T(1.0/(q + 1));
         }
-        sZ[w][lane][0] = (c2 - sz[s])*rinv;
-        sZ[w][lane][1] = w0[s];
+        sZ[w][lane][0] = T((c2 - sz[s])*rinv);
+        sZ[w][lane][1] = T(w0[s]);
         if (DIP) {
-          sZ[w][lane][2 % (DIP ? 4 : 2)] = wx[s]*rinv;
-          sZ[w][lane][3 % (DIP ? 4 : 2)] = wy[s]*rinv;
-          sWz[w][lane % (DIP ? CHK : 1)] = wz[s]*rinv;
+          sZ[w][lane][2 % (DIP ? 4 : 2)] = T(wx[s]*rinv);
+          sZ[w][lane][3 % (DIP ? 4 : 2)] = T(wy[s]*rinv);
+          sWz[w][lane % (DIP ? CHK : 1)] = T(wz[s]*rinv);
         }
       }
       wave_sync();
       for (int t = 0; t < nn; ++t) {
-        const double az = sZ[w][t][0], q0 = sZ[w][t][1];
+        const T az = sZ[w][t][0], q0 = sZ[w][t][1];
         #pragma unroll
         for (int r = 0; r < PPL; ++r) {
           const int i = pi[r] < 0 ? 0 : pi[r], j = pj[r];
-          const double xi = sX[w][t][i], yj = sY[w][t][j];
-          double bco = q0*xi*yj, cco = 0.0;
+          const T xi = sX[w][t][i], yj = sY[w][t][j];
+          T bco = q0*xi*yj, cco = T(0);
           if (DIP) {
-            const double qx = sZ[w][t][2 % (DIP ? 4 : 2)], qy = sZ[w][t][3 % (DIP ? 4 : 2)];
-            bco += qx*(i ? sX[w][t][i - 1] : 0.0)*yj + qy*xi*(j ? sY[w][t][j - 1] : 0.0);
+            const T qx = sZ[w][t][2 % (DIP ? 4 : 2)], qy = sZ[w][t][3 % (DIP ? 4 : 2)];
+            bco += qx*(i ? sX[w][t][i - 1] : T(0))*yj + qy*xi*(j ? sY[w][t][j - 1] : T(0));
             cco = sWz[w][t % (DIP ? CHK : 1)]*xi*yj;
           }
-          double z = 1.0, zprev = 0.0;
+          T z = T(1), zprev = T(0);
           #pragma unroll
           for (int k = 0; k < P1; ++k) {
             acc[r][k] += bco*z;
             if (DIP) acc[r][k] += cco*zprev;
             zprev = z;
-            z *= az*(1.0/(k + 1));
+            z *= az*T(1.0/(k + 1));
           }
         }
       }
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(64*P2M_NW) void p2m_lap3d_pairs(
       const int i = pi[r], j = pj[r];
       #pragma unroll
       for (int k = 0; k < P1; ++k)
-        if (k <= ORD - i - j) sC[w][e.cube_index[(i*P1 + j)*P1 + k]] = acc[r][k];
+        if (k <= ORD - i - j) sC[w][e.cube_index[(i*P1 + j)*P1 + k]] = double(acc[r][k]);
     }
     wave_sync();
     double *o = out + (size_t)(b - base)*e.nstored;
@@ -1005,7 +1008,8 @@ static int p2m_impl(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
                     int nboxes, const double *sx, const double *sy, const double *sz,
                     const double *w0, const double *wx, const double *wy, const double *wz,
                     double rscale, double *tgt_expansions, int tgt_base_ibox,
-                    const int *box_levels, const LevelScales &ls, void *stream) {
+                    const int *box_levels, const LevelScales &ls, void *stream,
+                    int precision = 0) {
   // Helmholtz (13): one real plane of complex coefficients, with projection
   // tables built for (k rscale)^2 -- the kernel itself is table-driven
   SUMPY_REQUIRE(kernel_id == 3 || kernel_id == 13,
@@ -1035,18 +1039,24 @@ static int p2m_impl(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
   if (expn->nfull == (expn->order + 1)*(expn->order + 2)*(expn->order + 3)/6) {
     // pair kernel for the usual orders
     const int g2 = (nsrc_boxes + P2M_NW - 1)/P2M_NW;
-#define SUMPY_P2M_PAIRS(O)                                                                   \
-  case O:                                                                                    \
+#define SUMPY_P2M_PAIRS_T(O, T)                                                              \
     if (wx)                                                                                  \
-      p2m_lap3d_pairs<O, true><<<g2, 64*P2M_NW, 0, st>>>(                                    \
+      p2m_lap3d_pairs<O, true, T><<<g2, 64*P2M_NW, 0, st>>>(                                 \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
           tgt_base_ibox, box_levels, ls);                                                    \
     else                                                                                     \
-      p2m_lap3d_pairs<O, false><<<g2, 64*P2M_NW, 0, st>>>(                                   \
+      p2m_lap3d_pairs<O, false, T><<<g2, 64*P2M_NW, 0, st>>>(                                \
           *expn, nsrc_boxes, source_boxes, box_source_starts, box_source_counts_nonchild,     \
           box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz, rscale, tgt_expansions,           \
-          tgt_base_ibox, box_levels, ls);                                                    \
+          tgt_base_ibox, box_levels, ls);
+#define SUMPY_P2M_PAIRS(O)                                                                   \
+  case O:                                                                                    \
+    if (precision == 1) {                                                                    \
+      SUMPY_P2M_PAIRS_T(O, float)                                                            \
+    } else {                                                                                 \
+      SUMPY_P2M_PAIRS_T(O, double)                                                           \
+    }                                                                                        \
     return check_launch("p2m");
     switch (expn->order) {
       SUMPY_P2M_PAIRS(1) SUMPY_P2M_PAIRS(2) SUMPY_P2M_PAIRS(3) SUMPY_P2M_PAIRS(4)
@@ -1055,6 +1065,7 @@ static int p2m_impl(int kernel_id, const sumpyhip_expn *expn, int nsrc_boxes,
       default: break;
     }
 #undef SUMPY_P2M_PAIRS
+#undef SUMPY_P2M_PAIRS_T
   }
   switch (kt) {
     case 2: SUMPY_P2M(2); break;
@@ -1369,15 +1380,16 @@ extern "C" int sumpyhip_p2m_levels(int kernel_id, const sumpyhip_expn *expn, int
                                    const double *wx, const double *wy, const double *wz,
                                    const int *box_levels, int nlevels,
                                    const double *level_rscale, double *tgt_expansions,
-                                   void *stream) {
+                                   int precision, void *stream) {
   SUMPY_REQUIRE(kernel_id == 3, "p2m_levels: the Laplace 3D kernel only (rscale-free tables)");
   SUMPY_REQUIRE(box_levels, "p2m_levels: box_levels needed");
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "p2m_levels: precision 0 (FP64) or 1 (FP32)");
   LevelScales ls;
   int rc = level_scales(nlevels, level_rscale, ls, "p2m_levels");
   if (rc) return rc;
   return p2m_impl(kernel_id, expn, nsrc_boxes, source_boxes, box_source_starts,
                   box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, w0, wx, wy, wz,
-                  level_rscale[0], tgt_expansions, 0, box_levels, ls, stream);
+                  level_rscale[0], tgt_expansions, 0, box_levels, ls, stream, precision);
 }
 
 extern "C" int sumpyhip_l2p(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes,

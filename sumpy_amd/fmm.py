@@ -48,7 +48,9 @@ class SumpyTreeIndependentDataForWrangler:
     the near field (the M2L precision is chosen by the local expansion's
     ``m2l_translation``, ``VolumeTaylorM2LWithDenseClassMatrices(precision=...)``);
     ``list34_precision``, ``"fp64"`` or ``"fp32"`` for the terms of lists 3
-    and 4 (M2P and P2L; FP32 terms, FP64 sums) and ``l2p_precision`` for the
+    and 4 (M2P and P2L; FP32 terms, FP64 sums), ``p2m_precision`` for the
+    formation of the multipoles (FP32 moments and box sums, FP64 projection)
+    and ``l2p_precision`` for the
     evaluation of the local expansions at the targets (potentials), both
     used where all levels share the expansion order;
     ``autotune``, time the M2L contraction's split policies on the first pass
@@ -60,9 +62,9 @@ class SumpyTreeIndependentDataForWrangler:
                  local_expansion_factory, target_kernels, exclude_self=False,
                  use_rscale=None, strength_usage=None, source_kernels=None,
                  p2p_precision="fp64", autotune=False, list34_precision="fp64",
-                 l2p_precision="fp64"):
+                 l2p_precision="fp64", p2m_precision="fp64"):
         for name, v in (("list34_precision", list34_precision),
-                        ("l2p_precision", l2p_precision)):
+                        ("l2p_precision", l2p_precision), ("p2m_precision", p2m_precision)):
             if v not in ("fp64", "fp32"):
                 raise ValueError(f"{name}: 'fp64' or 'fp32', got {v!r}")
         self._setup_actx = array_context
@@ -76,6 +78,7 @@ class SumpyTreeIndependentDataForWrangler:
         self.p2p_precision = p2p_precision
         self.list34_precision = list34_precision
         self.l2p_precision = l2p_precision
+        self.p2m_precision = p2m_precision
         self.autotune = autotune
         self._memo = {}
 
@@ -419,7 +422,8 @@ class SumpyExpansionWrangler:
             p2m(actx, source_boxes=source_boxes, centers=self.tree.box_centers,
                 strengths=src_weight_vecs, tgt_expansions=self._one_row_per_box(mpoles),
                 tgt_base_ibox=0, rscale=self.level_to_rscale(0),
-                box_levels=self.tree.box_levels, level_rscales=self._level_rscales(), **kwargs)
+                box_levels=self.tree.box_levels, level_rscales=self._level_rscales(),
+                p2m_precision=self.tree_indep.p2m_precision, **kwargs)
             return mpoles
         for lev in range(self.tree.nlevels):
             start, stop = starts[lev:lev + 2]

@@ -63,7 +63,9 @@ class P2EFromSingleBox(P2EBase):
     Besides the reference's keywords it takes ``box_levels`` and
     ``level_rscales``: the source boxes may then be of any level (one launch
     for all of them), each with the rscale of its level, and
-    ``tgt_expansions`` holds one row per tree box (Laplace kernel)."""
+    ``tgt_expansions`` holds one row per tree box (Laplace kernel);
+    ``p2m_precision="fp32"`` then forms the source moments and their box sums
+    in FP32 (projected to the stored coefficients in FP64)."""
 
     default_name = "p2e_from_single_box"
 
@@ -78,6 +80,9 @@ class P2EFromSingleBox(P2EBase):
 
         levels = kwargs.pop("box_levels", None)
         lrs = kwargs.pop("level_rscales", None)
+        prec = {"fp64": 0, "fp32": 1}[kwargs.pop("p2m_precision", "fp64")]
+        if prec and levels is None:
+            raise ValueError("p2m_precision='fp32' needs box_levels/level_rscales")
 
         def p2m(wptrs, out):
             if levels is not None:
@@ -88,7 +93,7 @@ class P2EFromSingleBox(P2EBase):
                            _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
                            int(centers.shape[1]), *[s.data_ptr() for s in sources], *wptrs,
                            _capi.ptr(as_i32(levels)), len(lrs), _capi.f64_array(lrs),
-                           _capi.ptr(out), actx.stream_handle)
+                           _capi.ptr(out), prec, actx.stream_handle)
                 return
             _capi.call("sumpyhip_p2m", kernel_id(self.expansion.kernel), tables.desc,
                        int(source_boxes.shape[0]), _capi.ptr(source_boxes), _capi.ptr(bss),

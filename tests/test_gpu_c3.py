@@ -6,7 +6,7 @@ cube and for the mixture of 8 Gaussians (sigma = 0.02, centres
 
 * The plain FP64 form equals the oracle-driven FMM (same tree, same stages)
   to 1e-12 relative.
-* The mixed forms (FP32 near field, list-3/4 terms and L2P; FP32 or
+* The mixed forms (FP32 near field, P2M moments, list-3/4 terms and L2P; FP32 or
   split-BF16 M2L) stay within 2x of the FP64 form's relative L2 error against the FP64
   direct sum on 4096 sampled targets (reference
   ``sumpy/test/test_fmm.py:78-299`` checks FMM results against the direct
@@ -65,7 +65,7 @@ def test_c3_mixed_error_within_twice_fp64(actx, c3, m2l_precision):
     e64 = rel(c3["p64"][s], c3["direct"])
     (pm,), _, _ = run_fmm(actx, c3["pts"], c3["q"], ORDER, m2l_precision=m2l_precision,
                           p2p_precision="fp32", list34_precision="fp32",
-                          l2p_precision="fp32",
+                          l2p_precision="fp32", p2m_precision="fp32",
                           tree_trav=(c3["tree"], c3["trav"]))
     em = rel(pm[s], c3["direct"])
     print(f"{c3['name']}: rel. L2 error vs direct, FP64 {e64:.3e}, "

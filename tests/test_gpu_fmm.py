@@ -22,7 +22,7 @@ def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
             use_classes=True, target_kernels=None, source_kernels=None,
             source_extra_kwargs=None, strength_usage=None, tree_trav=None,
             return_wrangler=False, all_levels=True, list34_precision="fp64",
-            l2p_precision="fp64"):
+            l2p_precision="fp64", p2m_precision="fp64"):
     from sumpy_amd.drive import drive_fmm
     from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
     from sumpy_amd.expansion.m2l import VolumeTaylorM2LWithDenseClassMatrices
@@ -45,7 +45,8 @@ def run_fmm(actx, sources, weights, order, exclude_self=True, max_particles=30,
                 m2l_translation_override=tr),
         target_kernels or [knl], exclude_self=exclude_self, source_kernels=source_kernels,
         strength_usage=strength_usage, p2p_precision=p2p_precision,
-        list34_precision=list34_precision, l2p_precision=l2p_precision)
+        list34_precision=list34_precision, l2p_precision=l2p_precision,
+        p2m_precision=p2m_precision)
     self_kw = {}
     if exclude_self:
         self_kw["target_to_source"] = actx.from_numpy(
@@ -103,6 +104,10 @@ def test_fmm_clustered_matches_oracle_fmm(actx, all_levels):
         assert 1e-12 < rel(p32, ref) < 2e-6
         (p32,), _, _ = run_fmm(actx, src, q, order, tree_trav=(tree, trav),
                                list34_precision="fp32", l2p_precision="fp32")
+        assert 1e-12 < rel(p32, ref) < 2e-6
+        (p32,), _, _ = run_fmm(actx, src, q, order, tree_trav=(tree, trav),
+                               list34_precision="fp32", l2p_precision="fp32",
+                               p2m_precision="fp32")
         assert 1e-12 < rel(p32, ref) < 2e-6
 
 
