@@ -471,6 +471,41 @@ int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntargets,
                    const double *sy, const double *nx, const double *ny,
                    const double *density, double scale, double *out, void *stream);
 
+/* Line-Taylor layer potential for the Laplace, Helmholtz and Yukawa kernels in
+ * 2D and 3D, replaces LayerPotential.__call__ (sumpy/qbx.py:250-335) with
+ * LineTaylorLocalExpansion (sumpy/expansion/local.py:116-198) for those
+ * kernels: result[t] (complex, re/im interleaved) = scale * sum_s sigma_s
+ * sum_{i<=order} [tau^i] D_s K(a + tau b), a = c_t - s, b = t - c_t.
+ * kernel_kind: 0 Laplace 2D (log r), 1 Laplace 3D (1/r), 2 Helmholtz 2D
+ * (H0(k r), real k), 3 Helmholtz 3D (e^{ikr}/r, k = k_re + i k_im),
+ * 4 Yukawa 2D (H0(i lam r)), 5 Yukawa 3D (e^{-lam r}/r); lam = k_re.
+ * src_kind: 0 none, 1 -n.grad_a (DirectionalSourceDerivative, normals
+ * nx/ny/nz), 2 -d/da_src_axis (AxisSourceDerivative); tgt_axis >= 0 applies
+ * d/da_tgt_axis (AxisTargetDerivative).  tz, cz, sz, nz: NULL in 2D.
+ * order <= 12.  accumulate = 1 adds to result. */
+int sumpyhip_qbx_line(int kernel_kind, int order, int src_kind, int src_axis, int tgt_axis,
+                      double k_re, double k_im, int ntargets, const double *tx,
+                      const double *ty, const double *tz, const double *cx, const double *cy,
+                      const double *cz, int nsources, const double *sx, const double *sy,
+                      const double *sz, const double *nx, const double *ny, const double *nz,
+                      const double *density, double scale_re, double scale_im, int accumulate,
+                      double *result, void *stream);
+
+/* Matrix entries of sumpyhip_qbx_line (unit density), replaces
+ * LayerPotentialMatrixGenerator / LayerPotentialMatrixSubsetGenerator
+ * (sumpy/qbx.py:342-539) for the same kernels: entry e of the pair
+ * (tgt_idx[e], src_idx[e]), or of the full row-major matrix (index arrays
+ * NULL, nentries = ntargets * nsources); complex, re/im interleaved. */
+int sumpyhip_qbx_line_matrix(int kernel_kind, int order, int src_kind, int src_axis,
+                             int tgt_axis, double k_re, double k_im, int ntargets,
+                             const double *tx, const double *ty, const double *tz,
+                             const double *cx, const double *cy, const double *cz,
+                             int nsources, const double *sx, const double *sy, const double *sz,
+                             const double *nx, const double *ny, const double *nz,
+                             long long nentries, const int *tgt_idx, const int *src_idx,
+                             double scale_re, double scale_im, int accumulate, double *out,
+                             void *stream);
+
 /* Adjoint of sumpyhip_qbx2d for the single (src_kind 0) and double (1) layer
  * without target derivatives: with g = grad_out (per target), the gradients of
  * sum_t g_t out_t with respect to density, source positions, source normals

@@ -123,6 +123,10 @@ _SIGNATURES = {
                               _D, _I, _P, _P],
     "sumpyhip_axpy_weights": [_I, _I, _D, _P, _P, _P, _P],
     "sumpyhip_combine_outputs": [_I, _I, _I, _P, _P, _P, _P, _I, _P],
+    "sumpyhip_qbx_line": [_I, _I, _I, _I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P,
+                          _P, _P, _P, _P, _P, _D, _D, _I, _P, _P],
+    "sumpyhip_qbx_line_matrix": [_I, _I, _I, _I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I,
+                                 _P, _P, _P, _P, _P, _P, _L, _P, _P, _D, _D, _I, _P, _P],
     "sumpyhip_qbx2d": [_I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P,
                        _D, _P, _P],
 }

@@ -108,26 +108,28 @@ class HelmholtzKernel(ScalarKernel):
 
 @dataclass(frozen=True)
 class YukawaKernel(ScalarKernel):
-    """``G = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi)`` (2D; reference
-    ``kernel.py:848-923``, which writes it through the Hankel function with
-    the scaling ``i/4``).  ``lam`` is passed at call time under
-    :attr:`yukawa_lambda_name`.  Complex-valued as in the reference.  Runs
-    with the Y2D expansions and its own P2P (:mod:`sumpy_amd.h2d`); the 3D
-    kernel is not built."""
+    """``G = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi)`` (2D) or
+    ``G = exp(-lam |d|)/(4 pi |d|)`` (3D) (reference ``kernel.py:848-923``,
+    which writes the 2D kernel through the Hankel function with the scaling
+    ``i/4``).  ``lam`` is passed at call time under
+    :attr:`yukawa_lambda_name`.  Complex-valued as in the reference.  The 2D
+    kernel runs with the Y2D expansions and its own P2P (:mod:`sumpy_amd.h2d`);
+    the 3D kernel is built for the line-Taylor layer potential
+    (:mod:`sumpy_amd.qbx`) only."""
 
     dim: int
     yukawa_lambda_name: str = "lam"
 
     def __post_init__(self):
-        if self.dim != 2:
-            raise NotImplementedError("only the 2D Yukawa kernel is implemented on the GPU")
+        if self.dim not in (2, 3):
+            raise NotImplementedError(f"unsupported dimension: '{self.dim}'")
 
     @property
     def is_complex_valued(self) -> bool:
         return True
 
     def get_global_scaling_const(self):
-        return 0.25j
+        return 0.25j if self.dim == 2 else 1.0 / (4.0 * math.pi)
 
     def get_args(self):
         return [self.yukawa_lambda_name]
@@ -332,7 +334,7 @@ def kernel_id(knl: ScalarKernel) -> int:
         return 3 if base.dim == 3 else 2
     if isinstance(base, HelmholtzKernel):
         return 13 if base.dim == 3 else 12
-    if isinstance(base, YukawaKernel):
+    if isinstance(base, YukawaKernel) and base.dim == 2:
         return 22
     raise NotImplementedError(f"unsupported kernel: {knl}")
 
