@@ -525,6 +525,93 @@ int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *
 
 /* }}} */
 
+/* {{{ 2D volume-Taylor expansions: Laplace (kind 0, G = log r, real rows) and
+ * Helmholtz (kind 1, G = H0(k r), real k, complex rows), full and
+ * PDE-compressed (VolumeTaylor* / LinearPDEConformingVolumeTaylor*
+ * expansions of sumpy/expansion/multipole.py:70-419, local.py:200-510 in
+ * 2D).  Expansions carry no scaling constant; M2P and L2P multiply by
+ * scale_re + i scale_im.  Orders 0..12. */
+
+/* One expansion: its order, whether it is PDE-compressed (stored m_x <= 1,
+ * the rest by u_{mx,my} = -u_{mx-2,my+2} - (k rscale)^2 u_{mx-2,my}) and its
+ * stored multi-indices in storage order (device arrays of nstored ints). */
+typedef struct sumpyhip_t2d_expn {
+  int order, compressed, nstored;
+  const int *stored_mx, *stored_my;
+} sumpyhip_t2d_expn;
+
+/* P2M, replaces P2EFromSingleBox.__call__ (sumpy/p2e.py:155-261): overwrites
+ * the rows of the source boxes (row box - tgt_base_ibox). */
+int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, int nboxes,
+                     const int *source_boxes, const int *box_source_starts,
+                     const int *box_source_counts, const double *centers, int ncenters,
+                     const double *sx, const double *sy, const double *weights, double rscale,
+                     double *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226): writes the
+ * potentials of the target boxes' targets. */
+int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+                     const int *target_boxes, const int *box_target_starts,
+                     const int *box_target_counts, const double *centers, int ncenters,
+                     const double *tx, const double *ty, double rscale,
+                     const double *src_expansions, int src_base_ibox, double scale_re,
+                     double scale_im, double *result, void *stream);
+
+/* M2P, replaces E2PFromCSR.__call__ (sumpy/e2p.py:233-343): adds the list-3
+ * multipoles' potentials. */
+int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+                     const int *target_boxes, const int *source_box_starts,
+                     const int *source_box_lists, const int *box_target_starts,
+                     const int *box_target_counts, const double *centers, int ncenters,
+                     const double *tx, const double *ty, double rscale,
+                     const double *src_expansions, int src_base_ibox, double scale_re,
+                     double scale_im, double *result, void *stream);
+
+/* P2L, replaces P2EFromCSR.__call__ (sumpy/p2e.py:268-388): overwrites the
+ * target boxes' rows from their list-4 sources. */
+int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+                     const int *target_boxes, const int *source_box_starts,
+                     const int *source_box_lists, const int *box_source_starts,
+                     const int *box_source_counts, const double *centers, int ncenters,
+                     const double *sx, const double *sy, const double *weights, double rscale,
+                     double *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* M2M, replaces E2EFromChildren.__call__ (sumpy/e2e.py:866-978): adds the
+ * translated children (box_child_ids: 4 x child_stride, 0 = none). */
+int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
+                     double k, int ntgt, const int *target_boxes, const int *box_child_ids,
+                     int child_stride, const double *centers, int ncenters, double src_rscale,
+                     double tgt_rscale, const double *src_expansions, int src_base_ibox,
+                     double *tgt_expansions, int tgt_base_ibox, void *stream);
+
+/* L2L, replaces E2EFromParent.__call__ (sumpy/e2e.py:985-1090): adds the
+ * translated parent local. */
+int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
+                     double k, int ntgt, const int *target_boxes, const int *box_parent_ids,
+                     const double *centers, int ncenters, double src_rscale, double tgt_rscale,
+                     const double *src_expansions, int src_base_ibox, double *tgt_expansions,
+                     int tgt_base_ibox, void *stream);
+
+/* M2L class data, replaces M2LGenerateTranslationClassesDependentData
+ * (sumpy/e2e.py:565-669): row c holds rscale^|m| d^m G(vec[class_start + c])
+ * for |m| <= order_sum at index |m|(|m|+1)/2 + m_y (complex: re, im). */
+int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, int ncls, const double *vecs,
+                          int nvecs, int class_start, double rscale, double *out, void *stream);
+
+/* M2L, replaces M2LUsingTranslationClassesDependentData.__call__ (class data
+ * given, the class of list entry e in classes[e]) and E2EFromCSR.__call__
+ * (class data NULL; derivatives per pair) (sumpy/e2e.py:164-558): overwrites
+ * each target box's row. */
+int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
+                     double k, int ntgt, const int *target_boxes, const int *src_box_starts,
+                     const int *src_box_lists, const int *classes, int class_start,
+                     const double *class_data, const double *centers, int ncenters,
+                     double src_rscale, double tgt_rscale, const double *src_expansions,
+                     int src_base_ibox, double *tgt_expansions, int tgt_base_ibox,
+                     void *stream);
+
+/* }}} */
+
 /* {{{ 2D Hankel/Bessel expansions: Helmholtz (H2D, kind 0, G = (i/4) H0^(1)(k|d|))
  * and Yukawa (Y2D, kind 1, G = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi))
 

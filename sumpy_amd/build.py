@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["misc.hip", "p2p.hip", "expansions.hip", "m2l.hip", "octant.hip", "helmholtz.hip",
-               "tree_gpu.hip", "h2d.hip", "qbx.hip"]
+               "tree_gpu.hip", "h2d.hip", "qbx.hip",
+               "taylor2d.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

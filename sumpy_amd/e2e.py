@@ -18,7 +18,7 @@ import weakref
 import numpy as np
 import torch
 
-from sumpy_amd import _capi, h2d
+from sumpy_amd import _capi, h2d, taylor2d
 from sumpy_amd.expansion import m2l_device_tables
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.profiling import kernel_span
@@ -107,6 +107,9 @@ class _OctantTranslation(E2EBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.tgt_expansion):
             return (h2d.l2l if self.span_name == "l2l" else h2d.m2m)(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.tgt_expansion):
+            return (taylor2d.l2l if self.span_name == "l2l" else taylor2d.m2m)(
+                self, actx, kwargs)
         src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
         k = expansion_k(self.tgt_expansion.kernel, kwargs)
         if not tgt.is_complex():
@@ -448,6 +451,8 @@ class E2EFromCSR(E2EBase):
             raise NotImplementedError("E2EFromCSR implements multipole-to-local only")
         if h2d.is_h2d(self.tgt_expansion):
             return h2d.m2l(self, actx, kwargs, with_classes=False)
+        if taylor2d.is_taylor2d(self.tgt_expansion):
+            return taylor2d.m2l(self, actx, kwargs, with_classes=False)
         m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
         centers = kwargs["centers"].contiguous()
         tb = as_i32(kwargs["target_boxes"])
@@ -485,6 +490,8 @@ class M2LGenerateTranslationClassesDependentData(E2EBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.tgt_expansion):
             return h2d.m2l_data(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.tgt_expansion):
+            return taylor2d.m2l_data(self, actx, kwargs)
         vecs = kwargs["m2l_translation_vectors"].contiguous()
         out = kwargs["m2l_translation_classes_dependent_data"]
         m = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)
@@ -657,6 +664,8 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.tgt_expansion):
             return h2d.m2l(self, actx, kwargs, with_classes=True)
+        if taylor2d.is_taylor2d(self.tgt_expansion):
+            return taylor2d.m2l(self, actx, kwargs, with_classes=True)
         entry = self.level_entry(actx, **kwargs)
         if entry is not None:
             m2l_apply_levels(actx, [entry])

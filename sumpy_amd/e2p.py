@@ -7,7 +7,7 @@ from __future__ import annotations
 
 import torch
 
-from sumpy_amd import _capi, h2d
+from sumpy_amd import _capi, h2d, taylor2d
 from sumpy_amd.array_context import obj_array
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.tools import (
@@ -66,6 +66,8 @@ class E2PFromSingleBox(E2PBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.expansion):
             return h2d.l2p(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.expansion):
+            return taylor2d.l2p(self, actx, kwargs)
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
@@ -138,6 +140,8 @@ class E2PFromCSR(E2PBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.expansion):
             return h2d.m2p(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.expansion):
+            return taylor2d.m2p(self, actx, kwargs)
         targets, centers, result, outs = self._setup(actx, kwargs)
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)

@@ -122,9 +122,9 @@ class VolumeTaylorExpansionMixin:
 
     @property
     def table(self) -> TermsTable:
-        if self.kernel.dim != 3:
+        if self.kernel.dim not in (2, 3):
             raise NotImplementedError(
-                "volume Taylor expansions are implemented for 3D kernels")
+                "volume Taylor expansions are implemented for 2D and 3D kernels")
         if not isinstance(self.kernel.get_base_kernel(), (LaplaceKernel, HelmholtzKernel)):
             raise NotImplementedError(
                 "volume Taylor expansions on the GPU are implemented for the "

@@ -115,6 +115,31 @@ class FourierBesselM2LTranslation(M2LTranslationBase):
         return 2*tgt_expansion.order + 1
 
 
+class VolumeTaylor2DM2LTranslation(M2LTranslationBase):
+    """M2L of the 2D Volume-Taylor locals: the class data are the scaled
+    derivatives ``rho^|m| d^m G(d_c)``, ``|m| <= ps + pt``, at index
+    ``|m|(|m|+1)/2 + m_y`` (complex values as (re, im) pairs), the
+    Toeplitz form ``L_t = sum_s M_s D[t + s]`` of ``m2l.py:363-389`` in 2D;
+    no separate pre/postprocessing arrays (:mod:`sumpy_amd.taylor2d`)."""
+
+    use_fft = False
+    use_preprocessing = False
+    precision = "fp64"
+
+    def preprocessed_dtype(self, dtype):
+        return np.dtype(dtype)
+
+    def translation_classes_dependent_ndata(self, tgt_expansion, src_expansion):
+        from sumpy_amd.taylor2d import class_ndata
+        return class_ndata(tgt_expansion, src_expansion)
+
+    def preprocess_multipole_nexprs(self, tgt_expansion, src_expansion):
+        return len(src_expansion)
+
+    def postprocess_local_nexprs(self, tgt_expansion, src_expansion):
+        return len(tgt_expansion)
+
+
 class FourierBesselM2LWithFFT(FourierBesselM2LTranslation):
     """Not supported, as in the reference (``m2l.py:1199-1215``)."""
 
@@ -139,6 +164,8 @@ class M2LTranslationClassFactoryBase:
     def get_m2l_translation_class(self, base_kernel, local_expansion_class):
         if _is_fourier_bessel(local_expansion_class):
             return self.fourier_bessel
+        if base_kernel.dim == 2:
+            return VolumeTaylor2DM2LTranslation
         return VolumeTaylorM2LWithDenseClassMatrices
 
 
@@ -157,5 +184,6 @@ __all__ = [
     "FourierBesselM2LWithFFT",
     "M2LTranslationBase",
     "NonFFTM2LTranslationClassFactory",
+    "VolumeTaylor2DM2LTranslation",
     "VolumeTaylorM2LWithDenseClassMatrices",
 ]

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import torch
 
-from sumpy_amd import _capi, h2d
+from sumpy_amd import _capi, h2d, taylor2d
 from sumpy_amd.expansion.local import LocalExpansionBase
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.tools import (
@@ -72,6 +72,8 @@ class P2EFromSingleBox(P2EBase):
     def __call__(self, actx, **kwargs):
         if h2d.is_h2d(self.expansion):
             return h2d.p2m(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.expansion):
+            return taylor2d.p2m(self, actx, kwargs)
         sources, centers, rscale, weights, tgt, tables = self._common(actx, kwargs)
         source_boxes = as_i32(kwargs.pop("source_boxes"))
         bss = as_i32(kwargs.pop("box_source_starts"))
@@ -136,6 +138,8 @@ class P2EFromCSR(P2EBase):
             raise TypeError("P2EFromCSR forms local expansions")
         if h2d.is_h2d(self.expansion):
             return h2d.p2l(self, actx, kwargs)
+        if taylor2d.is_taylor2d(self.expansion):
+            return taylor2d.p2l(self, actx, kwargs)
         prec = {"fp64": 0, "fp32": 1}[kwargs.pop("list34_precision", "fp64")]
         sources, centers, rscale, weights, tgt, tables = self._common(actx, kwargs)
         tb = as_i32(kwargs.pop("target_boxes"))

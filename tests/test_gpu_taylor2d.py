@@ -1,0 +1,226 @@
+"""GPU parity of the 2D volume-Taylor expansions (Laplace and Helmholtz, full
+and PDE-compressed; ``csrc/taylor2d.hip``) against the oracle
+(``oracle/taylor2d.py``) operator by operator, and the 2D FMM against the
+direct sum: the reference's ``test_sumpy_fmm`` cases with 2D kernels
+(``sumpy/test/test_fmm.py:78-299``: 1000 normal sources, 300 targets offset
+by 0.1, 30 per box, orders 1-3, k = 0.05, with and without translation
+classes; error decreasing with the order) and its exclude-self case
+(``test_fmm.py:538-598``: 500 sources, full Taylor order 10, rel. error
+< 1e-7).
+"""
+
+from functools import partial
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import taylor2d as t2
+
+pytestmark = pytest.mark.gpu
+
+K = 0.05
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b))/np.linalg.norm(np.asarray(b)))
+
+
+def _classes(kind, compressed):
+    from sumpy_amd.expansion.local import (
+        LinearPDEConformingVolumeTaylorLocalExpansion,
+        VolumeTaylorLocalExpansion,
+    )
+    from sumpy_amd.expansion.multipole import (
+        LinearPDEConformingVolumeTaylorMultipoleExpansion,
+        VolumeTaylorMultipoleExpansion,
+    )
+    from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
+    knl = LaplaceKernel(2) if kind == "lap" else HelmholtzKernel(2)
+    if compressed:
+        return (knl, LinearPDEConformingVolumeTaylorMultipoleExpansion,
+                LinearPDEConformingVolumeTaylorLocalExpansion)
+    return knl, VolumeTaylorMultipoleExpansion, VolumeTaylorLocalExpansion
+
+
+def _chain_geometry():
+    rng = np.random.default_rng(18)
+    origin = np.array([2.0, 1.0])
+    src = 0.7*(-0.5 + rng.random((2, 15))) + origin[:, None]
+    q = np.ones(15)/15
+    g = np.linspace(-0.15, 0.15, 6)
+    off = np.array([5.5, 0.0])
+    tx, ty = np.meshgrid(g + off[0] + origin[0], g + off[1] + origin[1], indexing="ij")
+    tgt = np.array([tx.ravel(), ty.ravel()])
+    c = np.array([[0, 0], [-0.2, 0.1], off + [0.3, -0.2], off]).T + origin[:, None]
+    return src, q, tgt, c
+
+
+@pytest.mark.parametrize("kind", ["lap", "helm"])
+@pytest.mark.parametrize("compressed", [True, False])
+@pytest.mark.parametrize("order", [3, 7])
+def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
+    """Boxes 0..3 = the four centres; box 0 holds the sources, box 3 the
+    targets; P2M(0) -> M2M(0->1) -> M2L(1->2, with and without class data)
+    -> L2L(2->3) -> L2P(3), plus M2P(1) and P2L(3), through the operators'
+    public calls."""
+    from sumpy_amd import (
+        E2EFromChildren,
+        E2EFromCSR,
+        E2EFromParent,
+        E2PFromCSR,
+        E2PFromSingleBox,
+        M2LGenerateTranslationClassesDependentData,
+        M2LUsingTranslationClassesDependentData,
+        P2EFromCSR,
+        P2EFromSingleBox,
+    )
+    knl, mcls, lcls = _classes(kind, compressed)
+    mpole, local = mcls(knl, order), lcls(knl, order)
+    kw = {"k": K} if kind == "helm" else {}
+    cdt = torch.complex128 if kind == "helm" else torch.float64
+    src, q, tgt, c = _chain_geometry()
+    rs = (0.5, 0.25, 0.25, 0.125)
+    dev = actx.from_numpy
+    i32 = lambda a: dev(np.asarray(a, np.int32))  # noqa: E731
+    nb = 4
+    centers = dev(np.ascontiguousarray(c))
+    srcd, tgtd = dev(src), dev(tgt)
+    qd = dev(q.astype(np.complex128 if kind == "helm" else np.float64))
+    bss, bsc = i32([0, 15, 15, 15]), i32([15, 0, 0, 0])
+    bts, btc = i32([0, 0, 0, 0]), i32([0, 0, 0, tgt.shape[1]])
+    ncoef = len(mpole)
+
+    def zeros():
+        return torch.zeros((nb, ncoef), dtype=cdt, device=actx.device)
+
+    M = zeros()
+    P2EFromSingleBox(mpole)(actx, sources=srcd, centers=centers, strengths=[qd],
+                            tgt_expansions=M, tgt_base_ibox=0, rscale=rs[0],
+                            source_boxes=i32([0]), box_source_starts=bss,
+                            box_source_counts_nonchild=bsc, **kw)
+    child = np.zeros((4, nb), np.int32)
+    child[0, 1] = 2
+    Msrc = zeros()
+    Msrc[2] = M[0]
+    cen2 = c.copy()
+    cen2[:, 2] = c[:, 0]
+    M2 = zeros()
+    E2EFromChildren(mpole, mpole)(actx, src_expansions=Msrc, src_base_ibox=0,
+                                  tgt_expansions=M2, tgt_base_ibox=0, target_boxes=i32([1]),
+                                  box_child_ids=dev(child),
+                                  centers=dev(np.ascontiguousarray(cen2)),
+                                  src_rscale=rs[0], tgt_rscale=rs[1], **kw)
+    L = zeros()
+    E2EFromCSR(mpole, local)(actx, src_expansions=M2, src_base_ibox=0, tgt_expansions=L,
+                             tgt_base_ibox=0, target_boxes=i32([2]), src_box_starts=i32([0, 1]),
+                             src_box_lists=i32([1]), centers=centers, src_rscale=rs[1],
+                             tgt_rscale=rs[2], **kw)
+    # the same M2L through the translation-class operators (one class)
+    vec = dev(np.ascontiguousarray((c[:, 2] - c[:, 1])[:, None]))
+    nd = local.m2l_translation.translation_classes_dependent_ndata(local, mpole)
+    data = torch.zeros((1, nd), dtype=torch.float64, device=actx.device)
+    M2LGenerateTranslationClassesDependentData(mpole, local)(
+        actx, src_rscale=rs[1], translation_classes_level_start=0, ntranslation_classes=1,
+        m2l_translation_classes_dependent_data=data, m2l_translation_vectors=vec,
+        ntranslation_vectors=1, **kw)
+    Lc = zeros()
+    M2LUsingTranslationClassesDependentData(mpole, local)(
+        actx, src_expansions=M2, src_base_ibox=0, tgt_expansions=Lc, tgt_base_ibox=0,
+        target_boxes=i32([2]), src_box_starts=i32([0, 1]), src_box_lists=i32([1]),
+        centers=centers, src_rscale=rs[1], tgt_rscale=rs[2],
+        m2l_translation_classes_dependent_data=data, translation_classes_level_start=0,
+        m2l_translation_classes_lists=i32([0]), **kw)
+    L2 = zeros()
+    E2EFromParent(local, local)(actx, src_expansions=L, src_base_ibox=0, tgt_expansions=L2,
+                                tgt_base_ibox=0, target_boxes=i32([3]),
+                                box_parent_ids=i32([0, 0, 0, 2]), centers=centers,
+                                src_rscale=rs[2], tgt_rscale=rs[3], **kw)
+    pot_l2p, = E2PFromSingleBox(local, [knl])(
+        actx, src_expansions=L2, src_base_ibox=0, target_boxes=i32([3]), centers=centers,
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[3],
+        **kw)
+    pot_m2p, = E2PFromCSR(mpole, [knl])(
+        actx, src_expansions=M2, src_base_ibox=0, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([1]), centers=centers,
+        targets=tgtd, box_target_starts=bts, box_target_counts_nonchild=btc, rscale=rs[1],
+        **kw)
+    L4 = zeros()
+    P2EFromCSR(local)(actx, sources=srcd, centers=centers, strengths=[qd], tgt_expansions=L4,
+                      tgt_base_ibox=0, rscale=rs[3], target_boxes=i32([3]),
+                      source_box_starts=i32([0, 1]), source_box_lists=i32([0]),
+                      box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
+
+    m1 = t2.p2m(order, compressed, kind, K, c[:, 0], src, q, rs[0])
+    m2 = t2.m2m(order, order, compressed, kind, K, m1, c[:, 0], c[:, 1], rs[0], rs[1])
+    l1 = t2.m2l(order, order, compressed, kind, K, m2, c[:, 2] - c[:, 1], rs[1])
+    l2 = t2.l2l(order, order, compressed, kind, K, l1, c[:, 2], c[:, 3], rs[2], rs[3])
+    l4 = t2.p2l(order, compressed, kind, K, c[:, 3], src, q, rs[3])
+    got = {n: actx.to_numpy(v) for n, v in (("p2m", M[0]), ("m2m", M2[1]), ("m2l", L[2]),
+                                             ("m2l_classes", Lc[2]), ("l2l", L2[3]),
+                                             ("p2l", L4[3]))}
+    want = {"p2m": m1, "m2m": m2, "m2l": l1, "m2l_classes": l1, "l2l": l2, "p2l": l4}
+    for n in want:
+        e = rel(got[n], want[n])
+        print(f"{kind} compressed={compressed} p={order} {n}: {e:.2e}")
+        assert e < 1e-12, n
+    ref = t2.l2p(order, compressed, kind, K, c[:, 3], l2, tgt, rs[3])
+    assert rel(actx.to_numpy(pot_l2p), ref) < 1e-12
+    ref = t2.m2p(order, compressed, kind, K, c[:, 1], m2, tgt, rs[1])
+    assert rel(actx.to_numpy(pot_m2p), ref) < 1e-12
+    # (the chain itself converges to the direct sum)
+    assert rel(actx.to_numpy(pot_l2p), t2.p2p(kind, K, tgt, src, q)) < 0.2**order
+
+
+def _fmm(actx, tree, trav, q, kind, compressed, order, use_classes, exclude_self=False):
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    knl, mcls, lcls = _classes(kind, compressed)
+    ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                             [knl], exclude_self=exclude_self)
+    self_kw = {}
+    if exclude_self:
+        self_kw["target_to_source"] = actx.from_numpy(np.arange(tree.ntargets, dtype=np.int32))
+    dtype = np.complex128 if kind == "helm" else np.float64
+    wr = SumpyExpansionWrangler(ti, trav, dtype, lambda *a: order,
+                                kernel_extra_kwargs={"k": K} if kind == "helm" else None,
+                                self_extra_kwargs=self_kw,
+                                _disable_translation_classes=not use_classes)
+    print(f"fmm {kind} compressed={compressed} p={order} classes={use_classes}", flush=True)
+    pot, = drive_fmm(actx, wr, [actx.from_numpy(q.astype(dtype))])
+    return actx.to_numpy(pot)
+
+
+@pytest.mark.parametrize("kind", ["lap", "helm"])
+@pytest.mark.parametrize("compressed", [True, False])
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_taylor2d_fmm_converges(actx, kind, compressed, use_classes):
+    from sumpy_amd.tree import build_tree_and_traversal
+    src = np.random.default_rng(15).standard_normal((2, 1000))
+    tgt = np.array([[0.1], [0.0]]) + np.random.default_rng(18).standard_normal((2, 300))
+    q = np.random.default_rng(44).random(1000)
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    direct = t2.p2p(kind, K, tgt, src, q)
+    errs = []
+    for order in (1, 2, 3):
+        pot = _fmm(actx, tree, trav, q, kind, compressed, order, use_classes)
+        errs.append(float(np.max(np.abs(pot - direct))/np.max(np.abs(direct))))
+    print(f"{kind} compressed={compressed} classes={use_classes}: rel-inf errors {errs}")
+    assert errs[2] < errs[1] < errs[0] < 0.1
+    # order of convergence: about one digit per order at 30 per box
+    assert errs[2] < 0.05*errs[0]
+
+
+@pytest.mark.parametrize("compressed", [True, False])
+def test_taylor2d_fmm_exclude_self(actx, compressed):
+    from sumpy_amd.tree import build_tree_and_traversal
+    src = np.random.default_rng(15).standard_normal((2, 500))
+    q = np.random.default_rng(44).random(500)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    pot = _fmm(actx, tree, trav, q, "lap", compressed, 10, True, exclude_self=True)
+    r = np.hypot(src[0][:, None] - src[0][None, :], src[1][:, None] - src[1][None, :])
+    np.fill_diagonal(r, 1.0)
+    direct = -1/(2*np.pi)*(np.log(r) @ q)
+    e = rel(pot, direct)
+    print(f"exclude self, order 10, compressed={compressed}: rel. l2 error {e:.2e}")
+    assert e < 1e-7
