@@ -477,7 +477,7 @@ int sumpyhip_qbx2d(int order, int src_kind, int tgt_axis, int ntargets,
  * kernels: result[t] (complex, re/im interleaved) = scale * sum_s sigma_s
  * sum_{i<=order} [tau^i] D_s K(a + tau b), a = c_t - s, b = t - c_t.
  * kernel_kind: 0 Laplace 2D (log r), 1 Laplace 3D (1/r), 2 Helmholtz 2D
- * (H0(k r), real k), 3 Helmholtz 3D (e^{ikr}/r, k = k_re + i k_im),
+ * (H0(k r), k = k_re + i k_im, k_im >= 0), 3 Helmholtz 3D (e^{ikr}/r),
  * 4 Yukawa 2D (H0(i lam r)), 5 Yukawa 3D (e^{-lam r}/r); lam = k_re.
  * src_kind: 0 none, 1 -n.grad_a (DirectionalSourceDerivative, normals
  * nx/ny/nz), 2 -d/da_src_axis (AxisSourceDerivative); tgt_axis >= 0 applies
@@ -526,7 +526,8 @@ int sumpyhip_qbx2d_adjoint(int order, int src_kind, int ntargets, const double *
 /* }}} */
 
 /* {{{ 2D volume-Taylor expansions: Laplace (kind 0, G = log r, real rows) and
- * Helmholtz (kind 1, G = H0(k r), real k, complex rows), full and
+ * Helmholtz (kind 1, G = H0(k r), k = k + i k_im with k_im >= 0, complex
+ * rows; Laplace takes k_im = 0), full and
  * PDE-compressed (VolumeTaylor* / LinearPDEConformingVolumeTaylor*
  * expansions of sumpy/expansion/multipole.py:70-419, local.py:200-510 in
  * 2D).  Expansions carry no scaling constant; M2P and L2P multiply by
@@ -542,7 +543,7 @@ typedef struct sumpyhip_t2d_expn {
 
 /* P2M, replaces P2EFromSingleBox.__call__ (sumpy/p2e.py:155-261): overwrites
  * the rows of the source boxes (row box - tgt_base_ibox). */
-int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, int nboxes,
+int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int nboxes,
                      const int *source_boxes, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
                      const double *sx, const double *sy, const double *weights, double rscale,
@@ -550,7 +551,7 @@ int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, int nboxes
 
 /* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226): writes the
  * potentials of the target boxes' targets. */
-int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
                      const double *tx, const double *ty, double rscale,
@@ -559,7 +560,7 @@ int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
 
 /* M2P, replaces E2PFromCSR.__call__ (sumpy/e2p.py:233-343): adds the list-3
  * multipoles' potentials. */
-int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *source_box_starts,
                      const int *source_box_lists, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
@@ -569,7 +570,7 @@ int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
 
 /* P2L, replaces P2EFromCSR.__call__ (sumpy/p2e.py:268-388): overwrites the
  * target boxes' rows from their list-4 sources. */
-int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *source_box_starts,
                      const int *source_box_lists, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
@@ -579,7 +580,7 @@ int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
 /* M2M, replaces E2EFromChildren.__call__ (sumpy/e2e.py:866-978): adds the
  * translated children (box_child_ids: 4 x child_stride, 0 = none). */
 int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
-                     double k, int ntgt, const int *target_boxes, const int *box_child_ids,
+                     double k, double k_im, int ntgt, const int *target_boxes, const int *box_child_ids,
                      int child_stride, const double *centers, int ncenters, double src_rscale,
                      double tgt_rscale, const double *src_expansions, int src_base_ibox,
                      double *tgt_expansions, int tgt_base_ibox, void *stream);
@@ -587,7 +588,7 @@ int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t
 /* L2L, replaces E2EFromParent.__call__ (sumpy/e2e.py:985-1090): adds the
  * translated parent local. */
 int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
-                     double k, int ntgt, const int *target_boxes, const int *box_parent_ids,
+                     double k, double k_im, int ntgt, const int *target_boxes, const int *box_parent_ids,
                      const double *centers, int ncenters, double src_rscale, double tgt_rscale,
                      const double *src_expansions, int src_base_ibox, double *tgt_expansions,
                      int tgt_base_ibox, void *stream);
@@ -595,7 +596,7 @@ int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t
 /* M2L class data, replaces M2LGenerateTranslationClassesDependentData
  * (sumpy/e2e.py:565-669): row c holds rscale^|m| d^m G(vec[class_start + c])
  * for |m| <= order_sum at index |m|(|m|+1)/2 + m_y (complex: re, im). */
-int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, int ncls, const double *vecs,
+int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, double k_im, int ncls, const double *vecs,
                           int nvecs, int class_start, double rscale, double *out, void *stream);
 
 /* M2L, replaces M2LUsingTranslationClassesDependentData.__call__ (class data
@@ -603,7 +604,7 @@ int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, int ncls, const dou
  * (class data NULL; derivatives per pair) (sumpy/e2e.py:164-558): overwrites
  * each target box's row. */
 int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t2d_expn *tgt_ex,
-                     double k, int ntgt, const int *target_boxes, const int *src_box_starts,
+                     double k, double k_im, int ntgt, const int *target_boxes, const int *src_box_starts,
                      const int *src_box_lists, const int *classes, int class_start,
                      const double *class_data, const double *centers, int ncenters,
                      double src_rscale, double tgt_rscale, const double *src_expansions,
@@ -621,20 +622,22 @@ int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t
  * same operators as the 3D expansions (P2EFromSingleBox, P2EFromCSR,
  * E2EFromChildren, E2EFromParent, E2EFromCSR /
  * M2LUsingTranslationClassesDependentData, E2PFromSingleBox, E2PFromCSR,
- * P2PFromCSR, P2P: sumpy/p2e.py, e2e.py, e2p.py, p2p.py).  `k` is the Helmholtz
- * k or the Yukawa lambda (real, > 0), orders 0..31, FP64.  Expansions, weights and outputs are complex128 (interleaved
+ * P2PFromCSR, P2P: sumpy/p2e.py, e2e.py, e2p.py, p2p.py).  `k` + i `k_im` is the
+ * Helmholtz wavenumber (k_im >= 0: damped waves; k_im = 0: real k > 0, which
+ * takes the real-argument Bessel route) or `k` the Yukawa lambda (real, > 0,
+ * k_im = 0), orders 0..31, FP64.  Expansions, weights and outputs are complex128 (interleaved
  * re, im); an expansion row holds the 2p+1 coefficients c = -p..p at c + p.
  * Coordinates are SoA; centers is 2 x ncenters.  The evaluations include the
  * kernel's i/4; M2P adds to `out`, L2P and the P2P forms write it. */
 
 /* P2M of each source box from its own sources (rows box - base of out). */
-int sumpyhip_h2d_p2m(int kind, int order, double k, int nboxes, const int *source_boxes,
+int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes, const int *source_boxes,
                      const int *box_source_starts, const int *box_source_counts,
                      const double *centers, int ncenters, const double *sx, const double *sy,
                      const void *weights, double rscale, void *out, int base, void *stream);
 
 /* P2L of each target box from the sources of its list-4 boxes (CSR). */
-int sumpyhip_h2d_p2l(int kind, int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                      const int *starts, const int *lists, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
                      const double *sx, const double *sy, const void *weights, double rscale,
@@ -642,14 +645,14 @@ int sumpyhip_h2d_p2l(int kind, int order, double k, int ntgt, const int *target_
 
 /* M2M: each target box's children (box_child_ids, 4 x nboxes, 0 = none)
  * translated and added to its row. */
-int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *box_child_ids, int nboxes,
                      const double *centers, int ncenters, double src_rscale,
                      double tgt_rscale, const void *src, int src_base, void *tgt,
                      int tgt_base, void *stream);
 
 /* L2L: each target box's parent local translated and added to its row. */
-int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *box_parent_ids,
                      const double *centers, int ncenters, double src_rscale,
                      double tgt_rscale, const void *src, int src_base, void *tgt,
@@ -658,14 +661,14 @@ int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, int ntgt,
 /* Translation-class data of ncls classes (distance vectors vecs, 2 x nvecs,
  * from column cls_start): row c holds H_n(k|d|) e^{i n th_d},
  * n = -(src+tgt)..(src+tgt), as complex128. */
-int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, int ncls,
+int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, double k_im, int ncls,
                           const double *vecs, int nvecs, int cls_start, void *data,
                           void *stream);
 
 /* M2L over list 2 (CSR); with `data` (rows by classes[e] - cls_start) or,
  * when data is NULL, the Hankel row of each pair computed in place.  Writes
  * each target box's row. */
-int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, int ntgt,
+int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *starts, const int *lists,
                      const int *classes, int cls_start, const void *data,
                      const double *centers, int ncenters, double src_rscale,
@@ -673,14 +676,14 @@ int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, int ntgt,
                      int tgt_base, void *stream);
 
 /* L2P of each target box at its own targets (writes). */
-int sumpyhip_h2d_l2p(int kind, int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                      const int *box_target_starts, const int *box_target_counts,
                      const double *centers, int ncenters, const double *tx,
                      const double *ty, double rscale, const void *loc, int base, void *out,
                      void *stream);
 
 /* M2P from the list-3 boxes of each target box (adds). */
-int sumpyhip_h2d_m2p(int kind, int order, double k, int ntgt, const int *target_boxes,
+int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                      const int *starts, const int *lists, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
                      const double *tx, const double *ty, double rscale, const void *mpoles,
@@ -688,7 +691,7 @@ int sumpyhip_h2d_m2p(int kind, int order, double k, int ntgt, const int *target_
 
 /* Near field over list 1 (CSR); t2s (or NULL): the source index of each
  * target to skip; pairs at zero distance add nothing. */
-int sumpyhip_h2d_p2p_csr(int kind, double k, int ntgt, const int *target_boxes, const int *starts,
+int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt, const int *target_boxes, const int *starts,
                          const int *lists, const int *box_source_starts,
                          const int *box_source_counts, const int *box_target_starts,
                          const int *box_target_counts, const double *tx, const double *ty,
@@ -696,7 +699,7 @@ int sumpyhip_h2d_p2p_csr(int kind, double k, int ntgt, const int *target_boxes, 
                          const int *t2s, void *out, void *stream);
 
 /* Dense direct sum. */
-int sumpyhip_h2d_p2p_dense(int kind, double k, int ntargets, const double *tx, const double *ty,
+int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets, const double *tx, const double *ty,
                            int nsources, const double *sx, const double *sy,
                            const void *weights, const int *t2s, void *out, void *stream);
 

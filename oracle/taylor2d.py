@@ -21,7 +21,8 @@ kernel derivatives come from the radial form G(r) = h(r^2/2) (Hermite-type
 expansion of the Cartesian derivatives, h^(n) from DLMF 10.6.6 and scipy's
 Hankel functions) -- independent of the product's W-operator recurrences.
 Scaling constants: ``-1/(2 pi)`` (Laplace),
-``i/4`` (Helmholtz), applied by the evaluations only.
+``i/4`` (Helmholtz), applied by the evaluations only.  The Helmholtz k may be
+complex (scipy's Hankel functions and the projection take complex k).
 """
 
 from __future__ import annotations
@@ -69,7 +70,10 @@ def _wr(order, compressed, kind):
 
 
 def _proj(order, compressed, kind, rscale, k):
-    return _wr(order, compressed, kind).projection(rscale, float(k) if kind == "helm" else 0.0)
+    if kind != "helm":
+        return _wr(order, compressed, kind).projection(rscale, 0.0)
+    return _wr(order, compressed, kind).projection(
+        rscale, complex(k) if np.iscomplexobj(k) and complex(k).imag else float(np.real(k)))
 
 
 def _mono(mi, v, rscale):

@@ -71,18 +71,18 @@ _SIGNATURES = {
                      _P, _P, _P, _P, _P],
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
-    "sumpyhip_h2d_p2m": [_I, _I, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
-    "sumpyhip_h2d_p2l": [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
-    "sumpyhip_h2d_m2m": [_I, _I, _I, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
-    "sumpyhip_h2d_l2l": [_I, _I, _I, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
-    "sumpyhip_h2d_m2l_data": [_I, _I, _I, _D, _I, _P, _I, _I, _P, _P],
-    "sumpyhip_h2d_m2l": [_I, _I, _I, _D, _I, _P, _P, _P, _P, _I, _P, _P, _I, _D, _D, _P, _I, _P,
+    "sumpyhip_h2d_p2m": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_h2d_p2l": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_h2d_m2m": [_I, _I, _I, _D, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_h2d_l2l": [_I, _I, _I, _D, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_h2d_m2l_data": [_I, _I, _I, _D, _D, _I, _P, _I, _I, _P, _P],
+    "sumpyhip_h2d_m2l": [_I, _I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _I, _D, _D, _P, _I, _P,
                          _I, _P],
-    "sumpyhip_h2d_l2p": [_I, _I, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
-    "sumpyhip_h2d_m2p": [_I, _I, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
-    "sumpyhip_h2d_p2p_csr": [_I, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+    "sumpyhip_h2d_l2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
+    "sumpyhip_h2d_m2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
+    "sumpyhip_h2d_p2p_csr": [_I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                              _P],
-    "sumpyhip_h2d_p2p_dense": [_I, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    "sumpyhip_h2d_p2p_dense": [_I, _D, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P],
     "sumpytree_gpu_morton": [_I, _L, _P, _P, _P, _P, _D, _I, _P, _P],
     "sumpytree_gpu_children": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
     "sumpytree_gpu_geometry": [_I, _I, _P, _P, _P, _D, _P, _P, _P],
@@ -123,17 +123,17 @@ _SIGNATURES = {
                               _D, _I, _P, _P],
     "sumpyhip_axpy_weights": [_I, _I, _D, _P, _P, _P, _P],
     "sumpyhip_combine_outputs": [_I, _I, _I, _P, _P, _P, _P, _I, _P],
-    "sumpyhip_t2d_p2m": [_I, _P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
-    "sumpyhip_t2d_l2p": [_I, _P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D, _D, _P,
+    "sumpyhip_t2d_p2m": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_t2d_l2p": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D, _D, _P,
                          _P],
-    "sumpyhip_t2d_m2p": [_I, _P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D,
+    "sumpyhip_t2d_m2p": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D,
                          _D, _P, _P],
-    "sumpyhip_t2d_p2l": [_I, _P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I,
+    "sumpyhip_t2d_p2l": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I,
                          _P],
-    "sumpyhip_t2d_m2m": [_I, _P, _P, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
-    "sumpyhip_t2d_l2l": [_I, _P, _P, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
-    "sumpyhip_t2d_m2l_data": [_I, _I, _D, _I, _P, _I, _I, _D, _P, _P],
-    "sumpyhip_t2d_m2l": [_I, _P, _P, _D, _I, _P, _P, _P, _P, _I, _P, _P, _I, _D, _D, _P, _I,
+    "sumpyhip_t2d_m2m": [_I, _P, _P, _D, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_t2d_l2l": [_I, _P, _P, _D, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_t2d_m2l_data": [_I, _I, _D, _D, _I, _P, _I, _I, _D, _P, _P],
+    "sumpyhip_t2d_m2l": [_I, _P, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _I, _D, _D, _P, _I,
                          _P, _I, _P],
     "sumpyhip_qbx_line": [_I, _I, _I, _I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P,
                           _P, _P, _P, _P, _P, _D, _D, _I, _P, _P],

@@ -1,7 +1,7 @@
 // 2D Helmholtz and Yukawa FMMs with Hankel/Bessel expansions (H2D, Y2D).
 //
-// Helmholtz: G(d) = (i/4) H0^(1)(k |d|), real k > 0 (sumpy/kernel.py:754-844,
-// 2D); Yukawa: G(d) = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi), lam > 0
+// Helmholtz: G(d) = (i/4) H0^(1)(k |d|), real k > 0 or complex k with
+// Im k > 0 (damped waves; sumpy/kernel.py:754-844, 2D); Yukawa: G(d) = (i/4) H0^(1)(i lam |d|) = K0(lam |d|)/(2 pi), lam > 0
 // (sumpy/kernel.py:848-897).  The reference's H2D / Y2D expansions
 // (sumpy/expansion/multipole.py:426-567, sumpy/expansion/local.py:522-704)
 // and its Fourier-Bessel M2L (sumpy/expansion/m2l.py:1008-1115) store 2p+1
@@ -27,7 +27,10 @@
 // Miller's recurrence normalised with I_0 + 2 sum_k I_k = e^x; K_0, K_1 by the
 // trapezoidal rule on K_nu(x) = int_0^inf e^{-x cosh t} cosh(nu t) dt (step
 // min(1/4, 1/(2 sqrt x)), exponentially convergent for this integrand), K_n
-// by the upward recurrence.  Both kinds: Z_-n = (-1)^n Z_n for Z = J, H.
+// by the upward recurrence.  (KIND 2, complex k) the same Miller / Neumann
+// route in complex arithmetic (bessel_jy_c); H_n = J_n + i Y_n then loses
+// about e^{2 Im(k) r} relative to |H_n|, which limits it to moderate Im(k) r.
+// All kinds: Z_-n = (-1)^n Z_n for Z = J, H.
 //
 // Layout: one wave (64 lanes) per box; lane c + p owns coefficient c
 // (2p+1 <= 63), so the orders go up to 31.  Per-source terms are summed per
@@ -50,13 +53,19 @@ __device__ __forceinline__ cplx cfma(cplx a, cplx b, cplx acc) {
   return {acc.re + a.re*b.re - a.im*b.im, acc.im + a.re*b.im + a.im*b.re};
 }
 
-// The two function sequences of one kernel kind at x = |s| r, as complex
-// values Z^J_c = J_c(s r) and Z^H_c = H_c(s r), c = 0..n (n >= 1 for H).
+// The two function sequences of one kernel kind at distance r, as complex
+// values Z^J_c = J_c(s r) and Z^H_c = H_c(s r), c = 0..n (n >= 1 for H), with
+// the Bessel argument scale s = k (KIND 0, real k; KIND 2, complex k) or
+// i lam (KIND 1; kk = (lam, 0)).
 template <int KIND>
-__device__ __forceinline__ void seq_j(double x, int n, cplx *Z) {
+__device__ __forceinline__ void seq_j(cplx kk, double r, int n, cplx *Z) {
+  if (KIND == 2) {
+    bessel_jy_c(cmul(kk, cplx{r, 0.0}), n, Z, nullptr);
+    return;
+  }
   double A[NSEQ];
-  if (KIND == 0) bessel_jy(x, n, A, nullptr);
-  else bessel_ik(x, n, A, nullptr);
+  if (KIND == 0) bessel_jy(kk.re*r, n, A, nullptr);
+  else bessel_ik(kk.re*r, n, A, nullptr);
   for (int c = 0; c <= n; ++c) {
     if (KIND == 0) {
       Z[c] = {A[c], 0.0};
@@ -70,10 +79,16 @@ __device__ __forceinline__ void seq_j(double x, int n, cplx *Z) {
 }
 
 template <int KIND>
-__device__ __forceinline__ void seq_h(double x, int n, cplx *Z) {
+__device__ __forceinline__ void seq_h(cplx kk, double r, int n, cplx *Z) {
+  if (KIND == 2) {
+    cplx Y[NSEQ];
+    bessel_jy_c(cmul(kk, cplx{r, 0.0}), n, Z, Y);
+    for (int c = 0; c <= n; ++c) Z[c] = {Z[c].re - Y[c].im, Z[c].im + Y[c].re};  // J + i Y
+    return;
+  }
   double A[NSEQ], B[NSEQ];
-  if (KIND == 0) bessel_jy(x, n, A, B);
-  else bessel_ik(x, n, A, B);
+  if (KIND == 0) bessel_jy(kk.re*r, n, A, B);
+  else bessel_ik(kk.re*r, n, A, B);
   for (int c = 0; c <= n; ++c) {
     if (KIND == 0) {
       Z[c] = {A[c], B[c]};
@@ -140,7 +155,7 @@ __device__ void add_terms(int p, cplx w, const cplx *Z, double ux, double uy, in
 
 template <int KIND, bool LOCAL>
 __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
-    int p, double k, int nboxes, const int *boxes, const int *starts, const int *lists,
+    int p, cplx kk, int nboxes, const int *boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const double *centers, int ncenters, const double *sx,
     const double *sy, const double *q, double rscale, double *out, int base) {
   const int ib = blockIdx.x;
@@ -162,10 +177,10 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
       const double rho = unit(sx[s] - cx, sy[s] - cy, ux, uy);
       const cplx w = {q[2*s], q[2*s + 1]};
       if (LOCAL) {
-        seq_h<KIND>(k*rho, p > 0 ? p : 1, Z);
+        seq_h<KIND>(kk, rho, p > 0 ? p : 1, Z);
         add_terms(p, w, Z, ux, uy, 1, acc);
       } else {
-        seq_j<KIND>(k*rho, p, Z);
+        seq_j<KIND>(kk, rho, p, Z);
         add_terms(p, w, Z, ux, uy, -1, acc);
       }
     }
@@ -191,12 +206,12 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
 
 // J_n(s|d|) and e^{i n th_d} for n = 0..N into LDS (lane 0), the same wave
 template <int KIND>
-__device__ void shift_data(double k, double dx, double dy, int N, cplx *Js, cplx *ph) {
+__device__ void shift_data(cplx kk, double dx, double dy, int N, cplx *Js, cplx *ph) {
   if (threadIdx.x == 0) {
     double ux, uy;
     const double dl = unit(dx, dy, ux, uy);
     cplx Z[NSEQ];
-    seq_j<KIND>(k*dl, N, Z);
+    seq_j<KIND>(kk, dl, N, Z);
     cplx e = {1.0, 0.0};
     for (int n = 0; n <= N; ++n) {
       Js[n] = Z[n];
@@ -219,7 +234,7 @@ __device__ __forceinline__ cplx shift_term(int n, const cplx *Js, const cplx *ph
 
 template <int KIND, bool LOCAL>
 __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
-    int ps, int pt, double k, int ntgt, const int *target_boxes, const int *relatives,
+    int ps, int pt, cplx kk, int ntgt, const int *target_boxes, const int *relatives,
     int nboxes, const double *centers, int ncenters, double src_rscale, double tgt_rscale,
     const double *src, int sbase, double *tgt, int tbase) {
   __shared__ cplx Js[NSEQ];
@@ -237,7 +252,7 @@ __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
     const int sb = LOCAL ? relatives[tb] : relatives[(long)r*nboxes + tb];
     if (!LOCAL && sb == 0) continue;  // same on every lane
     const double dx = centers[tb] - centers[sb], dy = centers[ncenters + tb] - centers[ncenters + sb];
-    shift_data<KIND>(k, dx, dy, ps + pt, Js, ph);
+    shift_data<KIND>(kk, dx, dy, ps + pt, Js, ph);
     if (lane < 2*ps + 1) {
       const int am = lane > ps ? lane - ps : ps - lane;
       const double s = LOCAL ? 1.0/ipow(src_rscale, am) : ipow(src_rscale, am);
@@ -263,11 +278,11 @@ __global__ __launch_bounds__(WAVE) void h2d_shift_kernel(
 
 // H_n(s|d|) e^{i n th_d}, n = -N..N, into D[n + N] (one thread)
 template <int KIND>
-__device__ void hankel_row(double k, double dx, double dy, int N, cplx *D) {
+__device__ void hankel_row(cplx kk, double dx, double dy, int N, cplx *D) {
   double ux, uy;
   const double dl = unit(dx, dy, ux, uy);
   cplx Z[NSEQ];
-  seq_h<KIND>(k*dl, N > 0 ? N : 1, Z);
+  seq_h<KIND>(kk, dl, N > 0 ? N : 1, Z);
   cplx e = {1.0, 0.0};
   for (int n = 0; n <= N; ++n) {
     const cplx h = cmul(Z[n], e);
@@ -280,12 +295,12 @@ __device__ void hankel_row(double k, double dx, double dy, int N, cplx *D) {
 }
 
 template <int KIND>
-__global__ void h2d_m2l_data_kernel(int N, double k, int ncls, const double *vecs, int nvecs,
+__global__ void h2d_m2l_data_kernel(int N, cplx kk, int ncls, const double *vecs, int nvecs,
                                     int cls_start, double *data) {
   const int c = blockIdx.x*blockDim.x + threadIdx.x;
   if (c >= ncls) return;
   cplx D[2*NSEQ];
-  hankel_row<KIND>(k, vecs[cls_start + c], vecs[nvecs + cls_start + c], N, D);
+  hankel_row<KIND>(kk, vecs[cls_start + c], vecs[nvecs + cls_start + c], N, D);
   for (int n = 0; n < 2*N + 1; ++n) store_c(data, (long)c*(2*N + 1) + n, D[n]);
 }
 
@@ -293,7 +308,7 @@ __global__ void h2d_m2l_data_kernel(int N, double k, int ncls, const double *vec
 // or, when data is null, the Hankel row of each pair made in LDS
 template <int KIND>
 __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
-    int ps, int pt, double k, int ntgt, const int *target_boxes, const int *starts,
+    int ps, int pt, cplx kk, int ntgt, const int *target_boxes, const int *starts,
     const int *lists, const int *classes, int cls_start, const double *data,
     const double *centers, int ncenters, double src_rscale, double tgt_rscale,
     const double *src, int sbase, double *tgt, int tbase) {
@@ -308,7 +323,7 @@ __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
   for (int e = starts[it]; e < starts[it + 1]; ++e) {
     const int sb = lists[e];
     if (!data && lane == 0)
-      hankel_row<KIND>(k, centers[tb] - centers[sb],
+      hankel_row<KIND>(kk, centers[tb] - centers[sb],
                        centers[ncenters + tb] - centers[ncenters + sb], N, Dl);
     if (lane < 2*ps + 1) {
       const int am = lane > ps ? lane - ps : ps - lane;
@@ -340,7 +355,7 @@ __device__ __forceinline__ cplx scale_i4(cplx z) { return {-0.25*z.im, 0.25*z.re
 
 template <int KIND, bool MPOLE>
 __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
-    int p, double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    int p, cplx kk, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bts, const int *btc, const double *centers, int ncenters, const double *tx,
     const double *ty, double rscale, const double *coeffs, int base, double *out) {
   __shared__ cplx S[2*PMAX + 1];
@@ -368,8 +383,8 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
         const double rho = unit(tx[t] - centers[sb], ty[t] - centers[ncenters + sb], ux, uy);
         // M2P: e^{i c th}, H_c; L2P: e^{-i c th}, J_c
         const cplx e1u = {ux, MPOLE ? uy : -uy};
-        if (MPOLE) seq_h<KIND>(k*rho, p > 0 ? p : 1, Z);
-        else seq_j<KIND>(k*rho, p, Z);
+        if (MPOLE) seq_h<KIND>(kk, rho, p > 0 ? p : 1, Z);
+        else seq_j<KIND>(kk, rho, p, Z);
         cplx ph = {1.0, 0.0};
         for (int c = 0; c <= p; ++c) {
           const cplx z = Z[c];
@@ -415,10 +430,17 @@ __device__ __forceinline__ void h0_series(double x, double &j0, double &y0) {
 }
 
 // (i/4) H0(k r) for r > 0: the power series up to k r = 4, the Miller /
-// Neumann route above it; Yukawa: K0(lam r) / (2 pi)
+// Neumann route above it (complex k: the complex Miller / Neumann route);
+// Yukawa: K0(lam r) / (2 pi)
 template <int KIND>
-__device__ __forceinline__ cplx green(double k, double r) {
-  const double x = k*r;
+__device__ __forceinline__ cplx green(cplx kk, double r) {
+  if (KIND == 2) {
+    cplx J[2], Y[2];
+    bessel_jy_c(cmul(kk, cplx{r, 0.0}), 1, J, Y);
+    const cplx h = {J[0].re - Y[0].im, J[0].im + Y[0].re};  // H0 = J0 + i Y0
+    return {-0.25*h.im, 0.25*h.re};
+  }
+  const double x = kk.re*r;
   if (KIND == 1) {
     double I[2], K[2];
     bessel_ik(x, 1, I, K);
@@ -439,7 +461,7 @@ __device__ __forceinline__ cplx green(double k, double r) {
 // one wave per target box, lanes over its targets; sources of list 1
 template <int KIND>
 __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
-    double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
+    cplx kk, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const int *bts, const int *btc, const double *tx,
     const double *ty, const double *sx, const double *sy, const double *q, const int *t2s,
     double *out) {
@@ -459,7 +481,7 @@ __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
         const double dx = x - sx[s], dy = y - sy[s];
         const double r = sqrt(dx*dx + dy*dy);
         if (r == 0.0 || s == self) continue;
-        acc = cfma(green<KIND>(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+        acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
       }
     }
     store_c(out, t, acc);
@@ -467,7 +489,7 @@ __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
 }
 
 template <int KIND>
-__global__ void h2d_p2p_dense_kernel(double k, int nt, const double *tx, const double *ty,
+__global__ void h2d_p2p_dense_kernel(cplx kk, int nt, const double *tx, const double *ty,
                                      int ns, const double *sx, const double *sy,
                                      const double *q, const int *t2s, double *out) {
   const int t = blockIdx.x*blockDim.x + threadIdx.x;
@@ -479,7 +501,7 @@ __global__ void h2d_p2p_dense_kernel(double k, int nt, const double *tx, const d
     const double dx = x - sx[s], dy = y - sy[s];
     const double r = sqrt(dx*dx + dy*dy);
     if (r == 0.0 || s == self) continue;
-    acc = cfma(green<KIND>(k, r), cplx{q[2*s], q[2*s + 1]}, acc);
+    acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
   }
   store_c(out, t, acc);
 }
@@ -492,10 +514,13 @@ int check_orders(int ps, int pt) {
   return SUMPYHIP_OK;
 }
 
-int check_k(int kind, double k) {
+int check_k(int kind, double k, double k_im) {
   SUMPY_REQUIRE(kind == 0 || kind == 1, "h2d: kind must be 0 (Helmholtz) or 1 (Yukawa)");
-  SUMPY_REQUIRE(k > 0.0 && std::isfinite(k),
-                "h2d: the Helmholtz k / Yukawa lambda must be real and > 0");
+  SUMPY_REQUIRE(std::isfinite(k) && std::isfinite(k_im), "h2d: non-finite k");
+  SUMPY_REQUIRE(kind == 0 || k_im == 0.0, "h2d: the Yukawa lambda must be real");
+  SUMPY_REQUIRE(k_im == 0.0 ? k > 0.0 : (k_im > 0.0 && k >= 0.0),
+                "h2d: the Helmholtz k must be > 0 or have Re k >= 0 < Im k, the Yukawa "
+                "lambda > 0");
   return SUMPYHIP_OK;
 }
 
@@ -510,174 +535,214 @@ using namespace sumpyhip;
     if (rc_ != SUMPYHIP_OK) return rc_; \
   } while (0)
 
-extern "C" int sumpyhip_h2d_p2m(int kind, int order, double k, int nboxes, const int *source_boxes,
+extern "C" int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes, const int *source_boxes,
                                 const int *box_source_starts, const int *box_source_counts,
                                 const double *centers, int ncenters, const double *sx,
                                 const double *sy, const void *weights, double rscale,
                                 void *out, int base, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (nboxes == 0) return SUMPYHIP_OK;
-  if (kind) h2d_p2e_kernel<1, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
+  if (kind == 1) h2d_p2e_kernel<1, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  else if (k_im != 0.0) h2d_p2e_kernel<2, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   else h2d_p2e_kernel<0, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
+      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   return check_launch("h2d_p2m");
 }
 
-extern "C" int sumpyhip_h2d_p2l(int kind, int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                                 const int *starts, const int *lists,
                                 const int *box_source_starts, const int *box_source_counts,
                                 const double *centers, int ncenters, const double *sx,
                                 const double *sy, const void *weights, double rscale,
                                 void *out, int base, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_p2e_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+  if (kind == 1) h2d_p2e_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  else if (k_im != 0.0) h2d_p2e_kernel<2, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   else h2d_p2e_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
   return check_launch("h2d_p2l");
 }
 
-extern "C" int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_m2m(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *box_child_ids, int nboxes,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_shift_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
+  if (kind == 1) h2d_shift_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  else if (k_im != 0.0) h2d_shift_kernel<2, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   else h2d_shift_kernel<0, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
+      src_order, tgt_order, kk, ntgt, target_boxes, box_child_ids, nboxes, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_m2m");
 }
 
-extern "C" int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_l2l(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *box_parent_ids,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_shift_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
+  if (kind == 1) h2d_shift_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
+      src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
+  else if (k_im != 0.0) h2d_shift_kernel<2, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   else h2d_shift_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
+      src_order, tgt_order, kk, ntgt, target_boxes, box_parent_ids, 0, centers, ncenters,
       src_rscale, tgt_rscale, (const double *)src, src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_l2l");
 }
 
-extern "C" int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, int ncls,
+extern "C" int sumpyhip_h2d_m2l_data(int kind, int src_order, int tgt_order, double k, double k_im, int ncls,
                                      const double *vecs, int nvecs, int cls_start, void *data,
                                      void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ncls == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(cls_start >= 0 && cls_start + ncls <= nvecs, "h2d_m2l_data: class range");
-  if (kind) h2d_m2l_data_kernel<1><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
-      src_order + tgt_order, k, ncls, vecs, nvecs, cls_start, (double *)data);
+  if (kind == 1) h2d_m2l_data_kernel<1><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
+      src_order + tgt_order, kk, ncls, vecs, nvecs, cls_start, (double *)data);
+  else if (k_im != 0.0) h2d_m2l_data_kernel<2><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
+      src_order + tgt_order, kk, ncls, vecs, nvecs, cls_start, (double *)data);
   else h2d_m2l_data_kernel<0><<<(ncls + 63)/64, 64, 0, (hipStream_t)stream>>>(
-      src_order + tgt_order, k, ncls, vecs, nvecs, cls_start, (double *)data);
+      src_order + tgt_order, kk, ncls, vecs, nvecs, cls_start, (double *)data);
   return check_launch("h2d_m2l_data");
 }
 
-extern "C" int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, int ntgt,
+extern "C" int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *starts, const int *lists,
                                 const int *classes, int cls_start, const void *data,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const void *src, int src_base, void *tgt,
                                 int tgt_base, void *stream) {
   H2D_CHECK(check_orders(src_order, tgt_order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   SUMPY_REQUIRE(!data || classes, "h2d_m2l: class data without the class list");
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_m2l_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, starts, lists, classes, cls_start,
+  if (kind == 1) h2d_m2l_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, starts, lists, classes, cls_start,
+      (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
+      src_base, (double *)tgt, tgt_base);
+  else if (k_im != 0.0) h2d_m2l_kernel<2><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      src_order, tgt_order, kk, ntgt, target_boxes, starts, lists, classes, cls_start,
       (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
       src_base, (double *)tgt, tgt_base);
   else h2d_m2l_kernel<0><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      src_order, tgt_order, k, ntgt, target_boxes, starts, lists, classes, cls_start,
+      src_order, tgt_order, kk, ntgt, target_boxes, starts, lists, classes, cls_start,
       (const double *)data, centers, ncenters, src_rscale, tgt_rscale, (const double *)src,
       src_base, (double *)tgt, tgt_base);
   return check_launch("h2d_m2l");
 }
 
-extern "C" int sumpyhip_h2d_l2p(int kind, int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                                 const int *box_target_starts, const int *box_target_counts,
                                 const double *centers, int ncenters, const double *tx,
                                 const double *ty, double rscale, const void *loc, int base,
                                 void *out, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_e2p_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
+  if (kind == 1) h2d_e2p_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
+  else if (k_im != 0.0) h2d_e2p_kernel<2, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
   else h2d_e2p_kernel<0, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
+      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
   return check_launch("h2d_l2p");
 }
 
-extern "C" int sumpyhip_h2d_m2p(int kind, int order, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
                                 const int *starts, const int *lists,
                                 const int *box_target_starts, const int *box_target_counts,
                                 const double *centers, int ncenters, const double *tx,
                                 const double *ty, double rscale, const void *mpoles, int base,
                                 void *out, void *stream) {
   H2D_CHECK(check_orders(order, order));
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_e2p_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
+  if (kind == 1) h2d_e2p_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
+      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
+  else if (k_im != 0.0) h2d_e2p_kernel<2, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
   else h2d_e2p_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, k, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
+      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
       centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
   return check_launch("h2d_m2p");
 }
 
-extern "C" int sumpyhip_h2d_p2p_csr(int kind, double k, int ntgt, const int *target_boxes,
+extern "C" int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt, const int *target_boxes,
                                     const int *starts, const int *lists,
                                     const int *box_source_starts, const int *box_source_counts,
                                     const int *box_target_starts, const int *box_target_counts,
                                     const double *tx, const double *ty, const double *sx,
                                     const double *sy, const void *weights, const int *t2s,
                                     void *out, void *stream) {
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind) h2d_p2p_csr_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+  if (kind == 1) h2d_p2p_csr_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
+      (double *)out);
+  else if (k_im != 0.0) h2d_p2p_csr_kernel<2><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
+      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
       (double *)out);
   else h2d_p2p_csr_kernel<0><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      k, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
+      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
       box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
       (double *)out);
   return check_launch("h2d_p2p_csr");
 }
 
-extern "C" int sumpyhip_h2d_p2p_dense(int kind, double k, int ntargets, const double *tx,
+extern "C" int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets, const double *tx,
                                       const double *ty, int nsources, const double *sx,
                                       const double *sy, const void *weights, const int *t2s,
                                       void *out, void *stream) {
-  H2D_CHECK(check_k(kind, k));
+  H2D_CHECK(check_k(kind, k, k_im));
+  const cplx kk = {k, k_im};
   if (ntargets == 0) return SUMPYHIP_OK;
-  if (kind) h2d_p2p_dense_kernel<1><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
-      k, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+  if (kind == 1) h2d_p2p_dense_kernel<1><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
+      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+  else if (k_im != 0.0) h2d_p2p_dense_kernel<2><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
+      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
   else h2d_p2p_dense_kernel<0><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
-      k, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
   return check_launch("h2d_p2p_dense");
 }

@@ -94,11 +94,15 @@ __device__ void radial_taylor(double r0, cplx kk, int ng, cplx *g) {
       g[n] = cmul(e, acc);
     }
   } else {
-    // H0(s r), s = k (real) or i lam; Z[m] = H_m(s r0)
+    // H0(s r), s = k (real or complex) or i lam; Z[m] = H_m(s r0)
     double A[QS + 1], Bv[QS + 1];
     cplx Z[QS + 1];
     const int nz = ng < 1 ? 1 : ng;
-    if (KK == HELM2) {
+    if (KK == HELM2 && kk.im != 0.0) {
+      cplx Jc[QS + 1], Yc[QS + 1];
+      bessel_jy_c(cscale(kk, r0), nz, Jc, Yc);
+      for (int m = 0; m <= nz; ++m) Z[m] = {Jc[m].re - Yc[m].im, Jc[m].im + Yc[m].re};
+    } else if (KK == HELM2) {
       bessel_jy(kk.re*r0, nz, A, Bv);
       for (int m = 0; m <= nz; ++m) Z[m] = {A[m], Bv[m]};
     } else {
@@ -112,7 +116,7 @@ __device__ void radial_taylor(double r0, cplx kk, int ng, cplx *g) {
     }
     // s^n / (2^n n!)
     cplx sp = {1.0, 0.0};
-    const cplx s = KK == HELM2 ? cplx{kk.re, 0.0} : cplx{0.0, kk.re};
+    const cplx s = KK == HELM2 ? kk : cplx{0.0, kk.re};
     for (int n = 0; n <= ng; ++n) {
       cplx acc = {0.0, 0.0};
       double binom = 1.0;  // C(n, j)
@@ -371,9 +375,12 @@ int fill_line_args(LineArgs &g, int kernel_kind, int order, int src_kind, int sr
   SUMPY_REQUIRE(nsources == 0 || (sx && sy && (dim == 2 || sz)), "qbx_line: null sources");
   SUMPY_REQUIRE(src_kind != 1 || nsources == 0 || (nx && ny && (dim == 2 || nz)),
                 "qbx_line: the directional source derivative needs the directions");
-  if (kernel_kind == HELM2 || kernel_kind == YUK2 || kernel_kind == YUK3)
+  if (kernel_kind == YUK2 || kernel_kind == YUK3)
     SUMPY_REQUIRE(k_im == 0.0 && k_re > 0.0,
                   "qbx_line: kernel kind %d needs a real positive parameter", kernel_kind);
+  if (kernel_kind == HELM2)
+    SUMPY_REQUIRE(k_im > 0.0 ? k_re >= 0.0 : (k_im == 0.0 && k_re > 0.0),
+                  "qbx_line: 2D Helmholtz needs k > 0 or Re k >= 0 < Im k");
   const double *T[3] = {tx, ty, tz}, *Cc[3] = {cx, cy, cz}, *S[3] = {sx, sy, sz},
                *N[3] = {nx, ny, nz};
   g.p = order;

@@ -98,9 +98,10 @@ __device__ __forceinline__ void addv(double *p, size_t i, cplx v) {
 
 // Wp[n] = W^n G, Wm[n] = Wbar^n G at the scaled vector (dx, dy) (= d/rho),
 // n = 0..N, for KIND 0 (Laplace, lrho = log rho enters the n = 0 term) or
-// KIND 1 (Helmholtz, kr = k rho)
+// KIND 1 (Helmholtz, kr = k rho; complex k: Bessel functions of complex
+// argument)
 template <int KIND>
-__device__ void wseq(double dx, double dy, double kr, double lrho, int N, cplx *Wp, cplx *Wm) {
+__device__ void wseq(double dx, double dy, cplx kr, double lrho, int N, cplx *Wp, cplx *Wm) {
   const double r2 = dx*dx + dy*dy, r = sqrt(r2);
   if (KIND == 0) {
     Wp[0] = Wm[0] = {log(r) + lrho, 0.0};
@@ -115,24 +116,31 @@ __device__ void wseq(double dx, double dy, double kr, double lrho, int N, cplx *
       f *= 2.0*n;
     }
   } else {
-    double J[2*TPMAX + 2], Y[2*TPMAX + 2];
-    bessel_jy(kr*r, N < 1 ? 1 : N, J, Y);
-    const cplx e1 = {dx/r, dy/r};
-    cplx e = {1.0, 0.0};
-    double c = 1.0;
+    // H_n = J_n + i Y_n into Wp first
+    if (kr.im == 0.0) {
+      double J[2*TPMAX + 2], Y[2*TPMAX + 2];
+      bessel_jy(kr.re*r, N < 1 ? 1 : N, J, Y);
+      for (int n = 0; n <= N; ++n) Wp[n] = {J[n], Y[n]};
+    } else {
+      cplx J[2*TPMAX + 2], Y[2*TPMAX + 2];
+      bessel_jy_c(cscale(kr, r), N < 1 ? 1 : N, J, Y);
+      for (int n = 0; n <= N; ++n) Wp[n] = {J[n].re - Y[n].im, J[n].im + Y[n].re};
+    }
+    const cplx e1 = {dx/r, dy/r}, mkr = {-kr.re, -kr.im};
+    cplx e = {1.0, 0.0}, c = {1.0, 0.0};
     for (int n = 0; n <= N; ++n) {
-      const cplx h = {J[n]*c, Y[n]*c};
+      const cplx h = cmul(Wp[n], c);
       Wp[n] = cmul(h, e);
       Wm[n] = cmul(h, cplx{e.re, -e.im});
       e = cmul(e, e1);
-      c *= -kr;
+      c = cmul(c, mkr);
     }
   }
 }
 
 // d_x^mx d_y^my G from the W sequences; mk2 = -(k rho)^2 (Helmholtz)
 template <int KIND>
-__device__ cplx deriv(int mx, int my, const cplx *Wp, const cplx *Wm, double mk2) {
+__device__ cplx deriv(int mx, int my, const cplx *Wp, const cplx *Wm, cplx mk2) {
   const int n = mx + my;
   cplx acc = {0.0, 0.0};
   double cx = 1.0;  // C(mx, i)
@@ -143,9 +151,9 @@ __device__ cplx deriv(int mx, int my, const cplx *Wp, const cplx *Wm, double mk2
       const double c = ((my - j) & 1) ? -cx*cy : cx*cy;
       if (KIND == 1) {
         const int lo = a < b ? a : b;
-        double f = 1.0;
-        for (int q = 0; q < lo; ++q) f *= mk2;
-        acc = cfma_s(a >= b ? Wp[a - b] : Wm[b - a], c*f, acc);
+        cplx f = {c, 0.0};
+        for (int q = 0; q < lo; ++q) f = cmul(f, mk2);
+        acc = cfma(a >= b ? Wp[a - b] : Wm[b - a], f, acc);
       } else if (a == 0 || b == 0) {
         acc = cfma_s(a >= b ? Wp[a] : Wm[b], c, acc);
       }
@@ -163,18 +171,19 @@ __device__ cplx deriv(int mx, int my, const cplx *Wp, const cplx *Wm, double mk2
 }
 
 // stored -> full local coefficients in F (stored positions already set)
-__device__ void proj_fwd(int p, double kap, cplx *F) {
+__device__ void proj_fwd(int p, cplx kap, cplx *F) {
   for (int mx = 2; mx <= p; ++mx) {
     for (int my = threadIdx.x; my <= p - mx; my += blockDim.x) {
       const cplx a = F[tri(mx - 2, my + 2)], b = F[tri(mx - 2, my)];
-      F[tri(mx, my)] = {-a.re - kap*b.re, -a.im - kap*b.im};
+      const cplx kb = cmul(kap, b);
+      F[tri(mx, my)] = {-a.re - kb.re, -a.im - kb.im};
     }
     __syncthreads();
   }
 }
 
 // full multipole moments in F -> P^T F at the stored positions
-__device__ void proj_bwd(int p, double kap, cplx *F) {
+__device__ void proj_bwd(int p, cplx kap, cplx *F) {
   for (int mx = p; mx >= 2; --mx) {
     for (int my = threadIdx.x; my <= p - mx; my += blockDim.x) {
       const cplx v = F[tri(mx, my)];
@@ -185,7 +194,8 @@ __device__ void proj_bwd(int p, double kap, cplx *F) {
     for (int my = threadIdx.x; my <= p - mx; my += blockDim.x) {
       const cplx v = F[tri(mx, my)];
       cplx &o = F[tri(mx - 2, my)];
-      o = {o.re - kap*v.re, o.im - kap*v.im};
+      const cplx kv = cmul(kap, v);
+      o = {o.re - kv.re, o.im - kv.im};
     }
     __syncthreads();
   }
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(TTH) void t2_p2m(T2E ex, const int *source_boxes,
                                               const int *bss, const int *bsc,
                                               const double *centers, int ncenters,
                                               const double *sx, const double *sy,
-                                              const double *w, double rscale, double kap,
+                                              const double *w, double rscale, cplx kap,
                                               double *out, int base) {
   __shared__ cplx F[TNF];
   __shared__ double px[TCH][TPMAX + 1], py[TCH][TPMAX + 1];
@@ -252,7 +262,7 @@ template <bool CX>
 __global__ __launch_bounds__(TTH) void t2_l2p(T2E ex, const int *target_boxes, const int *bts,
                                               const int *btc, const double *centers,
                                               int ncenters, const double *tx, const double *ty,
-                                              double rscale, double kap, const double *src,
+                                              double rscale, cplx kap, const double *src,
                                               int base, cplx scale, double *out) {
   __shared__ cplx F[TNF];
   const int box = target_boxes[blockIdx.x], p = ex.order;
@@ -287,13 +297,14 @@ __global__ __launch_bounds__(TTH) void t2_m2p(T2E ex, const int *target_boxes, c
                                               const int *lists, const int *bts, const int *btc,
                                               const double *centers, int ncenters,
                                               const double *tx, const double *ty,
-                                              double rscale, double k, const double *src,
+                                              double rscale, cplx k, const double *src,
                                               int base, cplx scale, double *out) {
   constexpr bool CX = KIND == 1;
   __shared__ cplx M[TNF];
   const int it = blockIdx.x, box = target_boxes[it];
   const int t0 = bts[box], nt = btc[box];
-  const double kr = k*rscale, mk2 = -kr*kr, lrho = log(rscale);
+  const cplx kr = cscale(k, rscale), mk2 = cscale(cmul(kr, kr), -1.0);
+  const double lrho = log(rscale);
   // targets of this thread (the box has at most a few hundred)
   for (int tb = 0; tb < nt; tb += TTH) {
     const int t = t0 + tb + threadIdx.x;
@@ -325,14 +336,15 @@ __global__ __launch_bounds__(TTH) void t2_p2l(T2E ex, const int *target_boxes, c
                                               const int *lists, const int *bss, const int *bsc,
                                               const double *centers, int ncenters,
                                               const double *sx, const double *sy,
-                                              const double *w, double rscale, double k,
+                                              const double *w, double rscale, cplx k,
                                               double *out, int base) {
   constexpr bool CX = KIND == 1;
   __shared__ cplx Wp[TCH][TPMAX + 1], Wm[TCH][TPMAX + 1];
   __shared__ cplx sw[TCH];
   const int it = blockIdx.x, box = target_boxes[it];
   const double cx = centers[box], cy = centers[ncenters + box];
-  const double kr = k*rscale, mk2 = -kr*kr, lrho = log(rscale);
+  const cplx kr = cscale(k, rscale), mk2 = cscale(cmul(kr, kr), -1.0);
+  const double lrho = log(rscale);
   const int j = threadIdx.x;
   const bool act = j < ex.nstored;
   const int mx = act ? ex.smx[j] : 0, my = act ? ex.smy[j] : 0;
@@ -367,7 +379,7 @@ __global__ __launch_bounds__(TTH) void t2_m2m(T2E es, T2E et, const int *target_
                                               const int *child_ids, int child_stride,
                                               const double *centers, int ncenters,
                                               double src_rscale, double tgt_rscale,
-                                              double kap_t, const double *src, int sbase,
+                                              cplx kap_t, const double *src, int sbase,
                                               double *tgt, int tbase) {
   __shared__ cplx G[TNF], Ms[TNF];
   __shared__ double dxp[TPMAX + 1], dyp[TPMAX + 1], rr[TPMAX + 1];
@@ -419,7 +431,7 @@ template <bool CX>
 __global__ __launch_bounds__(TTH) void t2_l2l(T2E es, T2E et, const int *target_boxes,
                                               const int *parent_ids, const double *centers,
                                               int ncenters, double src_rscale,
-                                              double tgt_rscale, double kap_s,
+                                              double tgt_rscale, cplx kap_s,
                                               const double *src, int sbase, double *tgt,
                                               int tbase) {
   __shared__ cplx F[TNF];
@@ -460,21 +472,21 @@ __global__ __launch_bounds__(TTH) void t2_l2l(T2E es, T2E et, const int *target_
 
 // D[tri(m)] = rho^|m| d^m G(d), |m| <= N, block-cooperative (ends with a barrier)
 template <int KIND>
-__device__ void deriv_table(double dx, double dy, double rscale, double k, int N, cplx *Wp,
+__device__ void deriv_table(double dx, double dy, double rscale, cplx k, int N, cplx *Wp,
                             cplx *Wm, cplx *D) {
-  const double kr = k*rscale;
+  const cplx kr = cscale(k, rscale), mk2 = cscale(cmul(kr, kr), -1.0);
   if (threadIdx.x == 0) wseq<KIND>(dx/rscale, dy/rscale, kr, log(rscale), N, Wp, Wm);
   __syncthreads();
   for (int e = threadIdx.x; e < nfull(N); e += blockDim.x) {
     int mx, my;
     untri(e, mx, my);
-    D[e] = deriv<KIND>(mx, my, Wp, Wm, -kr*kr);
+    D[e] = deriv<KIND>(mx, my, Wp, Wm, mk2);
   }
   __syncthreads();
 }
 
 template <int KIND>
-__global__ __launch_bounds__(TTH) void t2_m2l_data(int N, double k, int ncls, const double *vecs,
+__global__ __launch_bounds__(TTH) void t2_m2l_data(int N, cplx k, int ncls, const double *vecs,
                                                    int nvecs, int class_start, double rscale,
                                                    double *out) {
   constexpr bool CX = KIND == 1;
@@ -486,7 +498,7 @@ __global__ __launch_bounds__(TTH) void t2_m2l_data(int N, double k, int ncls, co
 }
 
 template <int KIND>
-__global__ __launch_bounds__(TTH) void t2_m2l(T2E es, T2E et, double k, const int *target_boxes,
+__global__ __launch_bounds__(TTH) void t2_m2l(T2E es, T2E et, cplx k, const int *target_boxes,
                                               const int *starts, const int *lists,
                                               const int *classes, int class_start,
                                               const double *data, const double *centers,
@@ -537,6 +549,12 @@ int check_expn(const sumpyhip_t2d_expn *e, const char *what) {
   return SUMPYHIP_OK;
 }
 
+// (k rho)^2 on the host
+cplx ksq(cplx k, double rho) {
+  const double a = k.re*rho, b = k.im*rho;
+  return {a*a - b*b, 2.0*a*b};
+}
+
 T2E to_t2e(const sumpyhip_t2d_expn *e) {
   return T2E{e->order, e->compressed, e->nstored, e->stored_mx, e->stored_my};
 }
@@ -547,9 +565,13 @@ T2E to_t2e(const sumpyhip_t2d_expn *e) {
 using namespace sumpyhip;
 
 #define T2_KIND(kind, what)                                                          \
-  SUMPY_REQUIRE(kind == 0 || kind == 1, "%s: kind %d (0 Laplace, 1 Helmholtz)", what, kind)
+  SUMPY_REQUIRE(kind == 0 || kind == 1, "%s: kind %d (0 Laplace, 1 Helmholtz)", what, kind); \
+  SUMPY_REQUIRE(kind == 0 ? k_im == 0.0 : (k_im >= 0.0 && (k > 0.0 || k_im > 0.0)),     \
+                "%s: wavenumber %g%+gi (Helmholtz: Im k >= 0, k != 0; Laplace: real)", what, k, \
+                k_im);                                                                   \
+  const cplx kk = {kind ? k : 0.0, kind ? k_im : 0.0}
 
-extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, int nboxes,
+extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int nboxes,
                                 const int *source_boxes, const int *box_source_starts,
                                 const int *box_source_counts, const double *centers,
                                 int ncenters, const double *sx, const double *sy,
@@ -562,7 +584,7 @@ extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k,
   SUMPY_REQUIRE(nboxes > 0 && source_boxes && box_source_starts && box_source_counts && centers &&
                     sx && sy && weights && tgt_expansions && rscale > 0,
                 "t2d_p2m: bad arguments");
-  const double kap = kind ? (k*rscale)*(k*rscale) : 0.0;
+  const cplx kap = ksq(kk, rscale);
   hipStream_t st = (hipStream_t)stream;
   if (kind)
     t2_p2m<true><<<nboxes, TTH, 0, st>>>(to_t2e(ex), source_boxes, box_source_starts,
@@ -575,7 +597,7 @@ extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k,
   return check_launch("t2d_p2m");
 }
 
-extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *box_target_starts,
                                 const int *box_target_counts, const double *centers,
                                 int ncenters, const double *tx, const double *ty, double rscale,
@@ -588,7 +610,7 @@ extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k,
   SUMPY_REQUIRE(ntgt > 0 && target_boxes && box_target_starts && box_target_counts && centers &&
                     tx && ty && src_expansions && result && rscale > 0,
                 "t2d_l2p: bad arguments");
-  const double kap = kind ? (k*rscale)*(k*rscale) : 0.0;
+  const cplx kap = ksq(kk, rscale);
   const cplx sc = {scale_re, scale_im};
   hipStream_t st = (hipStream_t)stream;
   if (kind)
@@ -602,7 +624,7 @@ extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k,
   return check_launch("t2d_l2p");
 }
 
-extern "C" int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+extern "C" int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *source_box_starts,
                                 const int *source_box_lists, const int *box_target_starts,
                                 const int *box_target_counts, const double *centers,
@@ -622,17 +644,17 @@ extern "C" int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k,
   if (kind)
     t2_m2p<1><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_target_starts, box_target_counts,
-                                    centers, ncenters, tx, ty, rscale, k, src_expansions,
+                                    centers, ncenters, tx, ty, rscale, kk, src_expansions,
                                     src_base_ibox, sc, result);
   else
     t2_m2p<0><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_target_starts, box_target_counts,
-                                    centers, ncenters, tx, ty, rscale, k, src_expansions,
+                                    centers, ncenters, tx, ty, rscale, kk, src_expansions,
                                     src_base_ibox, sc, result);
   return check_launch("t2d_m2p");
 }
 
-extern "C" int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, int ntgt,
+extern "C" int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *source_box_starts,
                                 const int *source_box_lists, const int *box_source_starts,
                                 const int *box_source_counts, const double *centers,
@@ -651,18 +673,18 @@ extern "C" int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k,
   if (kind)
     t2_p2l<1><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_source_starts, box_source_counts,
-                                    centers, ncenters, sx, sy, weights, rscale, k,
+                                    centers, ncenters, sx, sy, weights, rscale, kk,
                                     tgt_expansions, tgt_base_ibox);
   else
     t2_p2l<0><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_source_starts, box_source_counts,
-                                    centers, ncenters, sx, sy, weights, rscale, k,
+                                    centers, ncenters, sx, sy, weights, rscale, kk,
                                     tgt_expansions, tgt_base_ibox);
   return check_launch("t2d_p2l");
 }
 
 extern "C" int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex,
-                                const sumpyhip_t2d_expn *tgt_ex, double k, int ntgt,
+                                const sumpyhip_t2d_expn *tgt_ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *box_child_ids,
                                 int child_stride, const double *centers, int ncenters,
                                 double src_rscale, double tgt_rscale,
@@ -676,7 +698,7 @@ extern "C" int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex,
   SUMPY_REQUIRE(ntgt > 0 && target_boxes && box_child_ids && centers && src_expansions &&
                     tgt_expansions && src_rscale > 0 && tgt_rscale > 0,
                 "t2d_m2m: bad arguments");
-  const double kap = kind ? (k*tgt_rscale)*(k*tgt_rscale) : 0.0;
+  const cplx kap = ksq(kk, tgt_rscale);
   hipStream_t st = (hipStream_t)stream;
   if (kind)
     t2_m2m<true><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), target_boxes,
@@ -692,7 +714,7 @@ extern "C" int sumpyhip_t2d_m2m(int kind, const sumpyhip_t2d_expn *src_ex,
 }
 
 extern "C" int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex,
-                                const sumpyhip_t2d_expn *tgt_ex, double k, int ntgt,
+                                const sumpyhip_t2d_expn *tgt_ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *box_parent_ids,
                                 const double *centers, int ncenters, double src_rscale,
                                 double tgt_rscale, const double *src_expansions,
@@ -706,7 +728,7 @@ extern "C" int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex,
   SUMPY_REQUIRE(ntgt > 0 && target_boxes && box_parent_ids && centers && src_expansions &&
                     tgt_expansions && src_rscale > 0 && tgt_rscale > 0,
                 "t2d_l2l: bad arguments");
-  const double kap = kind ? (k*src_rscale)*(k*src_rscale) : 0.0;
+  const cplx kap = ksq(kk, src_rscale);
   hipStream_t st = (hipStream_t)stream;
   if (kind)
     t2_l2l<true><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), target_boxes,
@@ -721,7 +743,7 @@ extern "C" int sumpyhip_t2d_l2l(int kind, const sumpyhip_t2d_expn *src_ex,
   return check_launch("t2d_l2l");
 }
 
-extern "C" int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, int ncls,
+extern "C" int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, double k_im, int ncls,
                                      const double *vecs, int nvecs, int class_start,
                                      double rscale, double *out, void *stream) {
   T2_KIND(kind, "t2d_m2l_data");
@@ -733,16 +755,16 @@ extern "C" int sumpyhip_t2d_m2l_data(int kind, int order_sum, double k, int ncls
                 "t2d_m2l_data: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   if (kind)
-    t2_m2l_data<1><<<ncls, TTH, 0, st>>>(order_sum, k, ncls, vecs, nvecs, class_start, rscale,
+    t2_m2l_data<1><<<ncls, TTH, 0, st>>>(order_sum, kk, ncls, vecs, nvecs, class_start, rscale,
                                          out);
   else
-    t2_m2l_data<0><<<ncls, TTH, 0, st>>>(order_sum, k, ncls, vecs, nvecs, class_start, rscale,
+    t2_m2l_data<0><<<ncls, TTH, 0, st>>>(order_sum, kk, ncls, vecs, nvecs, class_start, rscale,
                                          out);
   return check_launch("t2d_m2l_data");
 }
 
 extern "C" int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex,
-                                const sumpyhip_t2d_expn *tgt_ex, double k, int ntgt,
+                                const sumpyhip_t2d_expn *tgt_ex, double k, double k_im, int ntgt,
                                 const int *target_boxes, const int *src_box_starts,
                                 const int *src_box_lists, const int *classes, int class_start,
                                 const double *class_data, const double *centers, int ncenters,
@@ -760,13 +782,13 @@ extern "C" int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex,
   SUMPY_REQUIRE(!class_data || classes, "t2d_m2l: class data without class lists");
   hipStream_t st = (hipStream_t)stream;
   if (kind)
-    t2_m2l<1><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), k, target_boxes,
+    t2_m2l<1><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), kk, target_boxes,
                                     src_box_starts, src_box_lists, classes, class_start,
                                     class_data, centers, ncenters, src_rscale, tgt_rscale,
                                     src_expansions, src_base_ibox, tgt_expansions,
                                     tgt_base_ibox);
   else
-    t2_m2l<0><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), k, target_boxes,
+    t2_m2l<0><<<ntgt, TTH, 0, st>>>(to_t2e(src_ex), to_t2e(tgt_ex), kk, target_boxes,
                                     src_box_starts, src_box_lists, classes, class_start,
                                     class_data, centers, ncenters, src_rscale, tgt_rscale,
                                     src_expansions, src_base_ibox, tgt_expansions,

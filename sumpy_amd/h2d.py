@@ -14,7 +14,9 @@ mathematics is that of ``sumpy/expansion/multipole.py:426-550``,
 (Yukawa: the Bessel argument scaled by ``i lam``, ``multipole.py:551-567``).
 
 Supported: potentials (no target derivatives) of charge weights (no source
-derivatives), real ``k > 0`` / ``lam > 0``, orders 0..31, FP64.
+derivatives), Helmholtz ``k > 0`` or complex ``k`` with ``Im k > 0`` (the
+Bessel functions of complex argument then come from the complex recurrences
+of ``csrc/bessel.h``), Yukawa ``lam > 0``, orders 0..31, FP64.
 """
 
 from __future__ import annotations
@@ -57,13 +59,24 @@ def is_h2d(expansion) -> bool:
                                   FourierBesselLocalExpansionMixin))
 
 
-def wavenumber(kernel, kwargs) -> float:
-    """The Helmholtz k or the Yukawa lambda from the call's keywords."""
+def wavenumber(kernel, kwargs) -> tuple[float, float]:
+    """(Re, Im) of the Helmholtz k, or (lam, 0) for Yukawa, from the call's
+    keywords.  A complex Helmholtz k (damped waves, Im k > 0) needs the
+    kernel's ``allow_evanescent``, as in the reference's
+    ``HelmholtzKernel`` (``sumpy/kernel.py``)."""
     k = kernel_k(kernel, kwargs)
+    if kind_of(kernel) == 0 and k.imag != 0:
+        if not getattr(kernel.get_base_kernel(), "allow_evanescent", False):
+            raise ValueError("a complex Helmholtz k needs HelmholtzKernel(..., "
+                             "allow_evanescent=True)")
+        if not (k.imag > 0 and k.real >= 0):
+            raise NotImplementedError("the 2D Bessel expansions on the GPU take "
+                                      "Re k >= 0 and Im k > 0")
+        return float(k.real), float(k.imag)
     if k.imag != 0 or not k.real > 0:
-        raise NotImplementedError("the 2D Bessel expansions on the GPU need a real "
-                                  "k > 0 (Helmholtz) or lam > 0 (Yukawa)")
-    return float(k.real)
+        raise NotImplementedError("the 2D Bessel expansions on the GPU need k > 0 or "
+                                  "Re k >= 0 < Im k (Helmholtz), lam > 0 (Yukawa)")
+    return float(k.real), 0.0
 
 
 def _order(expansion):
@@ -113,7 +126,7 @@ def p2m(op, actx, kwargs):
     sb = as_i32(kwargs.pop("source_boxes"))
     bss = as_i32(kwargs.pop("box_source_starts"))
     bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
-    _capi.call("sumpyhip_h2d_p2m", kind_of(op.expansion.kernel), _order(op.expansion), k,
+    _capi.call("sumpyhip_h2d_p2m", kind_of(op.expansion.kernel), _order(op.expansion), *k,
                int(sb.shape[0]), _capi.ptr(sb),
                _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
                src[0].data_ptr(), src[1].data_ptr(), w.data_ptr(),
@@ -137,7 +150,7 @@ def p2l(op, actx, kwargs):
     lists = as_i32(kwargs.pop("source_box_lists"))
     bss = as_i32(kwargs.pop("box_source_starts"))
     bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
-    _capi.call("sumpyhip_h2d_p2l", kind_of(op.expansion.kernel), _order(op.expansion), k,
+    _capi.call("sumpyhip_h2d_p2l", kind_of(op.expansion.kernel), _order(op.expansion), *k,
                int(tb.shape[0]), _capi.ptr(tb),
                _capi.ptr(starts), _capi.ptr(lists), _capi.ptr(bss), _capi.ptr(bsc),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(),
@@ -162,7 +175,7 @@ def _shift(op, actx, kwargs, local):
     kind = kind_of(op.tgt_expansion.kernel)
     if local:
         par = as_i32(kwargs["box_parent_ids"])
-        _capi.call("sumpyhip_h2d_l2l", kind, ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_l2l", kind, ps, pt, *k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(par), _capi.ptr(centers), int(centers.shape[1]), sr, tr,
                    _capi.ptr(src), int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
                    int(kwargs["tgt_base_ibox"]), actx.stream_handle)
@@ -170,7 +183,7 @@ def _shift(op, actx, kwargs, local):
         child = as_i32(kwargs["box_child_ids"])
         if child.dim() != 2 or child.shape[0] != 4:
             raise ValueError("box_child_ids must have shape (4, nboxes)")
-        _capi.call("sumpyhip_h2d_m2m", kind, ps, pt, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_m2m", kind, ps, pt, *k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(child), int(child.shape[1]), _capi.ptr(centers),
                    int(centers.shape[1]), sr, tr, _capi.ptr(src), int(kwargs["src_base_ibox"]),
                    _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
@@ -204,7 +217,7 @@ def m2l_data(op, actx, kwargs):
     if int(vecs.shape[0]) != 2:
         raise ValueError("translation vectors must have shape (2, n)")
     _capi.call("sumpyhip_h2d_m2l_data", kind_of(op.tgt_expansion.kernel), ps, pt,
-               wavenumber(op.tgt_expansion.kernel, kwargs),
+               *wavenumber(op.tgt_expansion.kernel, kwargs),
                ncls, _capi.ptr(vecs), int(vecs.shape[1]),
                int(kwargs["translation_classes_level_start"]), _capi.ptr(out),
                actx.stream_handle)
@@ -230,7 +243,7 @@ def m2l(op, actx, kwargs, with_classes):
         if data.shape[1] != 2*(2*(ps + pt) + 1):
             raise ValueError("class data of the wrong orders")
     _capi.call("sumpyhip_h2d_m2l", kind_of(op.tgt_expansion.kernel), ps, pt,
-               wavenumber(op.tgt_expansion.kernel, kwargs),
+               *wavenumber(op.tgt_expansion.kernel, kwargs),
                int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                _capi.ptr(classes), c0, _capi.ptr(data), _capi.ptr(centers),
                int(centers.shape[1]), _rscale(op.src_expansion, kwargs["src_rscale"]),
@@ -268,10 +281,10 @@ def _e2p(op, actx, kwargs, mpole):
     if mpole:
         starts = as_i32(kwargs["source_box_starts"])
         lists = as_i32(kwargs["source_box_lists"])
-        _capi.call("sumpyhip_h2d_m2p", kind, p, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_m2p", kind, p, *k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(starts), _capi.ptr(lists), _capi.ptr(bts), _capi.ptr(btc), *common)
     else:
-        _capi.call("sumpyhip_h2d_l2p", kind, p, k, int(tb.shape[0]), _capi.ptr(tb),
+        _capi.call("sumpyhip_h2d_l2p", kind, p, *k, int(tb.shape[0]), _capi.ptr(tb),
                    _capi.ptr(bts),
                    _capi.ptr(btc), *common)
     outs.finish(actx, targets, accumulate=mpole)
@@ -309,12 +322,12 @@ def p2p(actx, kernel, kwargs, targets, sources, weights, t2s, target_kernels, re
     xy = (targets[0].data_ptr(), targets[1].data_ptr())
     sxy = (sources[0].data_ptr(), sources[1].data_ptr())
     if csr is None:
-        _capi.call("sumpyhip_h2d_p2p_dense", kind_of(kernel), k, nt, *xy,
+        _capi.call("sumpyhip_h2d_p2p_dense", kind_of(kernel), *k, nt, *xy,
                    int(sources[0].shape[0]), *sxy,
                    w.data_ptr(), _capi.ptr(t2s), out, actx.stream_handle)
     else:
         ints = [as_i32(a) for a in csr]
-        _capi.call("sumpyhip_h2d_p2p_csr", kind_of(kernel), k, int(ints[0].shape[0]),
+        _capi.call("sumpyhip_h2d_p2p_csr", kind_of(kernel), *k, int(ints[0].shape[0]),
                    *[_capi.ptr(a) for a in ints], *xy, *sxy, w.data_ptr(), _capi.ptr(t2s),
                    out, actx.stream_handle)
     outs.finish(actx, targets)

@@ -125,8 +125,9 @@ class LayerPotentialBase(KernelComputation):
             k = complex(kwargs[base.helmholtz_k_name])
             if k.imag != 0 and not base.allow_evanescent:
                 raise ValueError("complex k needs HelmholtzKernel(allow_evanescent=True)")
-            if self.dim == 2 and k.imag != 0:
-                raise NotImplementedError("the 2D Helmholtz layer potential needs a real k")
+            if self.dim == 2 and (k.imag < 0 or (k.imag > 0 and k.real < 0)):
+                raise NotImplementedError("the 2D Helmholtz layer potential takes k > 0 "
+                                          "or Re k >= 0 < Im k")
             return k.real, k.imag
         if isinstance(base, YukawaKernel):
             return float(kwargs[base.yukawa_lambda_name]), 0.0

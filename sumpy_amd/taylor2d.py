@@ -12,7 +12,8 @@ and the PDE projection of ``sumpy/expansion/__init__.py:563-886`` in 2D.
 
 Supported: potentials (no target derivatives) of charge weights (no source
 derivatives), Laplace (``log r``, real coefficients) and Helmholtz
-(``H0(k r)``, real ``k``, complex coefficients), orders 0..12, FP64.
+(``H0(k r)``, real ``k`` or complex ``k`` with ``Im k > 0``, complex
+coefficients), orders 0..12, FP64.
 """
 
 from __future__ import annotations
@@ -61,13 +62,21 @@ def is_taylor2d(expansion) -> bool:
     return isinstance(expansion, VolumeTaylorExpansionMixin) and expansion.kernel.dim == 2
 
 
-def _k(expansion, kwargs) -> float:
+def _k(expansion, kwargs) -> tuple[float, float]:
+    """(Re k, Im k) of the call (0, 0 for Laplace); a complex Helmholtz k
+    needs the kernel's ``allow_evanescent`` and ``Im k > 0``."""
     if kind_of(expansion.kernel) == 0:
-        return 0.0
+        return 0.0, 0.0
     k = kernel_k(expansion.kernel, kwargs)
     if k.imag != 0:
-        raise NotImplementedError("the 2D Taylor expansions on the GPU need a real k")
-    return float(k.real)
+        if not getattr(expansion.kernel.get_base_kernel(), "allow_evanescent", False):
+            raise ValueError("a complex Helmholtz k needs HelmholtzKernel(..., "
+                             "allow_evanescent=True)")
+        if k.imag < 0:
+            raise NotImplementedError("the 2D Taylor expansions on the GPU take Im k >= 0")
+    elif k.real == 0:
+        raise ValueError("Helmholtz k must not be zero")
+    return float(k.real), float(k.imag)
 
 
 @lru_cache(maxsize=None)
@@ -129,7 +138,7 @@ def p2m(op, actx, kwargs):
     w = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
     sb = as_i32(kwargs.pop("source_boxes"))
     d = _desc(ex, actx.device)
-    _capi.call("sumpyhip_t2d_p2m", kind, _ref(d), k, int(sb.shape[0]), _capi.ptr(sb),
+    _capi.call("sumpyhip_t2d_p2m", kind, _ref(d), *k, int(sb.shape[0]), _capi.ptr(sb),
                _capi.ptr(as_i32(kwargs.pop("box_source_starts"))),
                _capi.ptr(as_i32(kwargs.pop("box_source_counts_nonchild"))),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(), src[1].data_ptr(),
@@ -152,7 +161,7 @@ def p2l(op, actx, kwargs):
     w = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
     tb = as_i32(kwargs.pop("target_boxes"))
     d = _desc(ex, actx.device)
-    _capi.call("sumpyhip_t2d_p2l", kind, _ref(d), k, int(tb.shape[0]), _capi.ptr(tb),
+    _capi.call("sumpyhip_t2d_p2l", kind, _ref(d), *k, int(tb.shape[0]), _capi.ptr(tb),
                _capi.ptr(as_i32(kwargs.pop("source_box_starts"))),
                _capi.ptr(as_i32(kwargs.pop("source_box_lists"))),
                _capi.ptr(as_i32(kwargs.pop("box_source_starts"))),
@@ -179,13 +188,13 @@ def _shift(op, actx, kwargs, local):
               _capi.ptr(tgt), int(kwargs["tgt_base_ibox"]), actx.stream_handle)
     if local:
         par = as_i32(kwargs["box_parent_ids"])
-        _capi.call("sumpyhip_t2d_l2l", kind_of(te.kernel), _ref(ds), _ref(dt), _k(te, kwargs),
+        _capi.call("sumpyhip_t2d_l2l", kind_of(te.kernel), _ref(ds), _ref(dt), *_k(te, kwargs),
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(par), *common)
     else:
         child = as_i32(kwargs["box_child_ids"])
         if child.dim() != 2 or child.shape[0] != 4:
             raise ValueError("box_child_ids must have shape (4, nboxes)")
-        _capi.call("sumpyhip_t2d_m2m", kind_of(te.kernel), _ref(ds), _ref(dt), _k(te, kwargs),
+        _capi.call("sumpyhip_t2d_m2m", kind_of(te.kernel), _ref(ds), _ref(dt), *_k(te, kwargs),
                    int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(child), int(child.shape[1]),
                    *common)
     return tgt
@@ -224,7 +233,7 @@ def m2l_data(op, actx, kwargs):
     if int(vecs.shape[0]) != 2:
         raise ValueError("translation vectors must have shape (2, n)")
     _capi.call("sumpyhip_t2d_m2l_data", kind_of(te.kernel), te.order + se.order,
-               _k(te, kwargs), ncls, _capi.ptr(vecs), int(vecs.shape[1]),
+               *_k(te, kwargs), ncls, _capi.ptr(vecs), int(vecs.shape[1]),
                int(kwargs["translation_classes_level_start"]),
                _rscale(se, kwargs["src_rscale"]), _capi.ptr(out), actx.stream_handle)
     return out
@@ -246,7 +255,7 @@ def m2l(op, actx, kwargs, with_classes):
         if data.shape[1] != class_ndata(te, se):
             raise ValueError("class data of the wrong orders")
     ds, dt = _desc(se, actx.device), _desc(te, actx.device)
-    _capi.call("sumpyhip_t2d_m2l", kind_of(te.kernel), _ref(ds), _ref(dt), _k(te, kwargs),
+    _capi.call("sumpyhip_t2d_m2l", kind_of(te.kernel), _ref(ds), _ref(dt), *_k(te, kwargs),
                int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(as_i32(kwargs["src_box_starts"])),
                _capi.ptr(as_i32(kwargs["src_box_lists"])), _capi.ptr(classes), c0,
                _capi.ptr(data), _capi.ptr(centers), int(centers.shape[1]),
@@ -286,11 +295,11 @@ def _e2p(op, actx, kwargs, mpole):
               _rscale(ex, kwargs["rscale"]), _capi.ptr(coeffs), int(kwargs["src_base_ibox"]),
               scale.real, scale.imag, out, actx.stream_handle)
     if mpole:
-        _capi.call("sumpyhip_t2d_m2p", kind, _ref(d), _k(ex, kwargs), int(tb.shape[0]),
+        _capi.call("sumpyhip_t2d_m2p", kind, _ref(d), *_k(ex, kwargs), int(tb.shape[0]),
                    _capi.ptr(tb), _capi.ptr(as_i32(kwargs["source_box_starts"])),
                    _capi.ptr(as_i32(kwargs["source_box_lists"])), *common)
     else:
-        _capi.call("sumpyhip_t2d_l2p", kind, _ref(d), _k(ex, kwargs), int(tb.shape[0]),
+        _capi.call("sumpyhip_t2d_l2p", kind, _ref(d), *_k(ex, kwargs), int(tb.shape[0]),
                    _capi.ptr(tb), *common)
     outs.finish(actx, targets, accumulate=mpole)
     return result

@@ -41,9 +41,9 @@ def _kernel(kind):
     from sumpy_amd.expansion.local import H2DLocalExpansion, Y2DLocalExpansion
     from sumpy_amd.expansion.multipole import H2DMultipoleExpansion, Y2DMultipoleExpansion
     from sumpy_amd.kernel import HelmholtzKernel, YukawaKernel
-    if kind == "helmholtz":
-        return (HelmholtzKernel(2), H2DMultipoleExpansion, H2DLocalExpansion, "k",
-                lambda v: v)
+    if kind in ("helmholtz", "helmholtz_c"):
+        return (HelmholtzKernel(2, allow_evanescent=kind == "helmholtz_c"),
+                H2DMultipoleExpansion, H2DLocalExpansion, "k", lambda v: v)
     return YukawaKernel(2), Y2DMultipoleExpansion, Y2DLocalExpansion, "lam", lambda v: 1j*v
 
 
@@ -74,6 +74,28 @@ def test_h2d_p2p_dense_matches_oracle(actx, k):
     assert err < 1e-13
 
 
+@pytest.mark.parametrize("k", [0.2*(0.707 + 0.707j), 3.0 + 1.0j, 2.0j])
+def test_h2d_p2p_dense_complex_k_matches_oracle(actx, k):
+    """Damped waves: H0 of complex argument (J + i Y by the complex
+    recurrences); the first k is the reference's evanescent value
+    (``sumpy/test/test_kernels.py:168``)."""
+    from sumpy_amd import P2P
+    rng = np.random.default_rng(5)
+    src, tgt = 4*rng.random((2, 700)), 4*rng.random((2, 300))
+    q = rng.standard_normal(700) + 1j*rng.standard_normal(700)
+    pot, = P2P([_kernel("helmholtz_c")[0]], exclude_self=False)(
+        actx, actx.from_numpy(tgt), actx.from_numpy(src), [actx.from_numpy(q)], k=k)
+    err = rel(actx.to_numpy(pot), oh.p2p(k, tgt, src, q))
+    print(f"k={k}: P2P vs oracle {err:.2e}")
+    # H0 = J0 + i Y0 cancels: |J0|, |Y0| ~ e^{Im k r} against |H0| ~ e^{-Im k r},
+    # so the relative accuracy of a pair is about 1e-16 e^{2 Im k r}
+    # (k = 2i, r up to 5.7 here)
+    assert err < (1e-13 if k.imag < 1.5 else 2e-12)
+    with pytest.raises(ValueError, match="allow_evanescent"):
+        P2P([_knl()], exclude_self=False)(
+            actx, actx.from_numpy(tgt), actx.from_numpy(src), [actx.from_numpy(q)], k=k)
+
+
 def test_h2d_p2p_exclude_self(actx):
     from sumpy_amd import P2P
     rng = np.random.default_rng(6)
@@ -100,6 +122,7 @@ def _chain_geometry():
 
 @pytest.mark.parametrize(("kind", "order", "k"), [
     ("helmholtz", 3, 0.05), ("helmholtz", 8, 0.05), ("helmholtz", 20, 1.3),
+    ("helmholtz_c", 8, 0.2*(0.707 + 0.707j)), ("helmholtz_c", 20, 1.3 + 0.4j),
     ("yukawa", 8, 2.0), ("yukawa", 20, 0.5)])
 def test_h2d_operator_chain_matches_oracle(actx, kind, order, k):
     """Boxes 0..3 = the four centres; box 0 holds the sources, box 3 the
@@ -277,3 +300,25 @@ def test_y2d_fmm_matches_oracle_fmm_and_converges(actx, use_classes):
               f"rel-inf vs direct {errs[-1]:.2e}")
         assert e_or < 1e-12
     assert errs[1] < errs[0] < 1e-6
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_h2d_fmm_complex_k(actx, use_classes):
+    """The 2D Helmholtz FMM with the reference's evanescent k = 0.2 (0.707 +
+    0.707 i) against the oracle FMM (same tree and lists) and the direct
+    sum."""
+    from sumpy_amd.tree import build_tree_and_traversal
+    k = 0.2*(0.707 + 0.707j)
+    src, tgt, q = _fmm_case()
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    direct = oh.p2p(k, tgt, src, q)
+    errs = []
+    for order in (4, 8):
+        pot = _run_fmm(actx, tree, trav, q, order, k, use_classes, kind="helmholtz_c")
+        ref = oracle_fmm_h2d(tree, trav, q, order, k)
+        e_or = rel(pot, ref)
+        errs.append(float(np.max(np.abs(pot - direct))/np.max(np.abs(direct))))
+        print(f"complex k classes={use_classes} p={order}: vs oracle FMM {e_or:.2e}, "
+              f"rel-inf vs direct {errs[-1]:.2e}")
+        assert e_or < 1e-12
+    assert errs[1] < 0.1*errs[0] and errs[0] < 1e-3

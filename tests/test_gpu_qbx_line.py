@@ -38,7 +38,7 @@ def _dev(x):
     return torch.as_tensor(np.ascontiguousarray(x), dtype=torch.float64, device="cuda")
 
 
-def _call_line(kind, order, mode, tgt, ctr, src, nrm, dens, accumulate=0, out=None):
+def _call_line(kind, order, mode, tgt, ctr, src, nrm, dens, accumulate=0, out=None, k=None):
     from sumpy_amd import _capi
     skind, sax, tax = MODES[mode]
     dim = tgt.shape[0]
@@ -50,7 +50,8 @@ def _call_line(kind, order, mode, tgt, ctr, src, nrm, dens, accumulate=0, out=No
         out = torch.zeros((nt, 2), dtype=torch.float64, device="cuda")
     sc = complex(ql.SCALE[kind])
     print(f"qbx_line {kind} order {order} mode {mode} nt {nt} ns {src.shape[1]}", flush=True)
-    _capi.call("sumpyhip_qbx_line", KIND_ID[kind], order, skind, sax, tax, K[kind], 0.0, nt,
+    k = complex(K[kind] if k is None else k)
+    _capi.call("sumpyhip_qbx_line", KIND_ID[kind], order, skind, sax, tax, k.real, k.imag, nt,
                *[_capi.ptr(x) for x in t], *[_capi.ptr(x) for x in c], src.shape[1],
                *[_capi.ptr(x) for x in s], *[_capi.ptr(x) for x in n], _capi.ptr(d),
                sc.real, sc.imag, accumulate, _capi.ptr(out), 0)
@@ -93,6 +94,47 @@ def test_qbx_line_accumulates_and_rejects_bad_input():
     with pytest.raises(ValueError, match="order 13"):
         _capi.call("sumpyhip_qbx_line", 3, 13, 0, -1, -1, 1.0, 0.0, 1, *[None]*6, 0,
                    *[None]*6, None, 1.0, 0.0, 0, None, 0)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_qbx_line_helm2_complex_k_matches_oracle(mode):
+    """2D Helmholtz with a complex (damped) k: H_n of complex argument."""
+    k = 1.3 + 0.4j
+    tgt, ctr, src, nrm, dens = _case(2, 31)
+    got = _call_line("helm2", 6, mode, tgt, ctr, src, nrm, dens, k=k)
+    skind, sax, tax = MODES[mode]
+    ref = ql.layer_potential("helm2", k, 6, tgt, ctr, src, dens,
+                             normals=nrm if skind == 1 else None, tgt_axis=tax,
+                             src_axis=sax if skind == 2 else -1)
+    err = np.abs(got - ref).max()/np.abs(ref).max()
+    print(f"helm2 k={k} {mode}: max rel. err {err:.2e}")
+    assert err < 1e-12
+
+
+def test_layer_potential_helm2_complex_k(actx):
+    """LayerPotential with HelmholtzKernel(2, allow_evanescent=True) and the
+    reference's evanescent k (``sumpy/test/test_kernels.py:168``); without
+    allow_evanescent a complex k is refused."""
+    from sumpy_amd.expansion.local import LineTaylorLocalExpansion
+    from sumpy_amd.kernel import DirectionalSourceDerivative, HelmholtzKernel
+    from sumpy_amd.qbx import LayerPotential
+    k = 0.2*(0.707 + 0.707j)
+    tgt, ctr, src, nrm, dens = _case(2, 12, nt=4, ns=6)
+    T, C, S, N = (_dev(x) for x in (tgt, ctr, src, nrm))
+    for evanescent in (True, False):
+        base = HelmholtzKernel(2, allow_evanescent=evanescent)
+        lp = LayerPotential(LineTaylorLocalExpansion(base, 5),
+                            source_kernels=[DirectionalSourceDerivative(base)],
+                            target_kernels=[base], value_dtypes=np.complex128)
+        if not evanescent:
+            with pytest.raises(ValueError, match="allow_evanescent"):
+                lp(actx, T, S, C, [_dev(dens)], _dev(np.full(4, 0.3)), src_derivative_dir=N,
+                   k=k)
+            continue
+        got, = lp(actx, T, S, C, [_dev(dens)], _dev(np.full(4, 0.3)), src_derivative_dir=N,
+                  k=k)
+        ref = ql.layer_potential("helm2", k, 5, tgt, ctr, src, dens, normals=nrm)
+        assert np.abs(got.cpu().numpy() - ref).max() <= 1e-12*np.abs(ref).max()
 
 
 def _layer_potential(kind, dim):

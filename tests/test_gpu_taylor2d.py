@@ -20,6 +20,15 @@ from oracle import taylor2d as t2
 pytestmark = pytest.mark.gpu
 
 K = 0.05
+KC = 0.2*(0.707 + 0.707j)   # the reference's evanescent k (sumpy/test/test_kernels.py:168)
+
+
+def _k(kind):
+    return KC if kind == "helmc" else K
+
+
+def _okind(kind):
+    return "helm" if kind.startswith("helm") else "lap"
 
 
 def rel(a, b):
@@ -36,7 +45,8 @@ def _classes(kind, compressed):
         VolumeTaylorMultipoleExpansion,
     )
     from sumpy_amd.kernel import HelmholtzKernel, LaplaceKernel
-    knl = LaplaceKernel(2) if kind == "lap" else HelmholtzKernel(2)
+    knl = LaplaceKernel(2) if kind == "lap" else HelmholtzKernel(
+        2, allow_evanescent=kind == "helmc")
     if compressed:
         return (knl, LinearPDEConformingVolumeTaylorMultipoleExpansion,
                 LinearPDEConformingVolumeTaylorLocalExpansion)
@@ -56,7 +66,7 @@ def _chain_geometry():
     return src, q, tgt, c
 
 
-@pytest.mark.parametrize("kind", ["lap", "helm"])
+@pytest.mark.parametrize("kind", ["lap", "helm", "helmc"])
 @pytest.mark.parametrize("compressed", [True, False])
 @pytest.mark.parametrize("order", [3, 7])
 def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
@@ -76,9 +86,10 @@ def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
         P2EFromSingleBox,
     )
     knl, mcls, lcls = _classes(kind, compressed)
+    ok, kv = _okind(kind), _k(kind)
     mpole, local = mcls(knl, order), lcls(knl, order)
-    kw = {"k": K} if kind == "helm" else {}
-    cdt = torch.complex128 if kind == "helm" else torch.float64
+    kw = {"k": _k(kind)} if kind != "lap" else {}
+    cdt = torch.complex128 if kind != "lap" else torch.float64
     src, q, tgt, c = _chain_geometry()
     rs = (0.5, 0.25, 0.25, 0.125)
     dev = actx.from_numpy
@@ -86,7 +97,7 @@ def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
     nb = 4
     centers = dev(np.ascontiguousarray(c))
     srcd, tgtd = dev(src), dev(tgt)
-    qd = dev(q.astype(np.complex128 if kind == "helm" else np.float64))
+    qd = dev(q.astype(np.complex128 if kind != "lap" else np.float64))
     bss, bsc = i32([0, 15, 15, 15]), i32([15, 0, 0, 0])
     bts, btc = i32([0, 0, 0, 0]), i32([0, 0, 0, tgt.shape[1]])
     ncoef = len(mpole)
@@ -151,11 +162,11 @@ def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
                       source_box_starts=i32([0, 1]), source_box_lists=i32([0]),
                       box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
 
-    m1 = t2.p2m(order, compressed, kind, K, c[:, 0], src, q, rs[0])
-    m2 = t2.m2m(order, order, compressed, kind, K, m1, c[:, 0], c[:, 1], rs[0], rs[1])
-    l1 = t2.m2l(order, order, compressed, kind, K, m2, c[:, 2] - c[:, 1], rs[1])
-    l2 = t2.l2l(order, order, compressed, kind, K, l1, c[:, 2], c[:, 3], rs[2], rs[3])
-    l4 = t2.p2l(order, compressed, kind, K, c[:, 3], src, q, rs[3])
+    m1 = t2.p2m(order, compressed, ok, kv, c[:, 0], src, q, rs[0])
+    m2 = t2.m2m(order, order, compressed, ok, kv, m1, c[:, 0], c[:, 1], rs[0], rs[1])
+    l1 = t2.m2l(order, order, compressed, ok, kv, m2, c[:, 2] - c[:, 1], rs[1])
+    l2 = t2.l2l(order, order, compressed, ok, kv, l1, c[:, 2], c[:, 3], rs[2], rs[3])
+    l4 = t2.p2l(order, compressed, ok, kv, c[:, 3], src, q, rs[3])
     got = {n: actx.to_numpy(v) for n, v in (("p2m", M[0]), ("m2m", M2[1]), ("m2l", L[2]),
                                              ("m2l_classes", Lc[2]), ("l2l", L2[3]),
                                              ("p2l", L4[3]))}
@@ -164,12 +175,12 @@ def test_taylor2d_operator_chain_matches_oracle(actx, kind, compressed, order):
         e = rel(got[n], want[n])
         print(f"{kind} compressed={compressed} p={order} {n}: {e:.2e}")
         assert e < 1e-12, n
-    ref = t2.l2p(order, compressed, kind, K, c[:, 3], l2, tgt, rs[3])
+    ref = t2.l2p(order, compressed, ok, kv, c[:, 3], l2, tgt, rs[3])
     assert rel(actx.to_numpy(pot_l2p), ref) < 1e-12
-    ref = t2.m2p(order, compressed, kind, K, c[:, 1], m2, tgt, rs[1])
+    ref = t2.m2p(order, compressed, ok, kv, c[:, 1], m2, tgt, rs[1])
     assert rel(actx.to_numpy(pot_m2p), ref) < 1e-12
     # (the chain itself converges to the direct sum)
-    assert rel(actx.to_numpy(pot_l2p), t2.p2p(kind, K, tgt, src, q)) < 0.2**order
+    assert rel(actx.to_numpy(pot_l2p), t2.p2p(ok, kv, tgt, src, q)) < 0.2**order
 
 
 def _fmm(actx, tree, trav, q, kind, compressed, order, use_classes, exclude_self=False):
@@ -181,9 +192,9 @@ def _fmm(actx, tree, trav, q, kind, compressed, order, use_classes, exclude_self
     self_kw = {}
     if exclude_self:
         self_kw["target_to_source"] = actx.from_numpy(np.arange(tree.ntargets, dtype=np.int32))
-    dtype = np.complex128 if kind == "helm" else np.float64
+    dtype = np.complex128 if kind != "lap" else np.float64
     wr = SumpyExpansionWrangler(ti, trav, dtype, lambda *a: order,
-                                kernel_extra_kwargs={"k": K} if kind == "helm" else None,
+                                kernel_extra_kwargs={"k": _k(kind)} if kind != "lap" else None,
                                 self_extra_kwargs=self_kw,
                                 _disable_translation_classes=not use_classes)
     print(f"fmm {kind} compressed={compressed} p={order} classes={use_classes}", flush=True)
@@ -191,7 +202,7 @@ def _fmm(actx, tree, trav, q, kind, compressed, order, use_classes, exclude_self
     return actx.to_numpy(pot)
 
 
-@pytest.mark.parametrize("kind", ["lap", "helm"])
+@pytest.mark.parametrize("kind", ["lap", "helm", "helmc"])
 @pytest.mark.parametrize("compressed", [True, False])
 @pytest.mark.parametrize("use_classes", [True, False])
 def test_taylor2d_fmm_converges(actx, kind, compressed, use_classes):
@@ -200,7 +211,7 @@ def test_taylor2d_fmm_converges(actx, kind, compressed, use_classes):
     tgt = np.array([[0.1], [0.0]]) + np.random.default_rng(18).standard_normal((2, 300))
     q = np.random.default_rng(44).random(1000)
     tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
-    direct = t2.p2p(kind, K, tgt, src, q)
+    direct = t2.p2p(_okind(kind), _k(kind), tgt, src, q)
     errs = []
     for order in (1, 2, 3):
         pot = _fmm(actx, tree, trav, q, kind, compressed, order, use_classes)

@@ -114,3 +114,15 @@ def test_y2d_chains_converge_to_direct_sum(lam):
         e = [x[name] for x in errs]
         assert all(b < 0.5*a for a, b in zip(e, e[1:]) if a > 1e-13), (name, e)
         assert e[-1] < 1e-11, (name, e)
+
+
+@pytest.mark.parametrize("k", [0.2*(0.707 + 0.707j), 1.3 + 0.4j])
+def test_h2d_chains_converge_complex_k(k):
+    """Damped waves (complex k; the first value is the reference's evanescent
+    k, ``sumpy/test/test_kernels.py:168``): Graf's addition theorem holds for
+    complex arguments, so the chains still converge to the direct sum."""
+    errs = [_chains(p, k)[0] for p in (2, 4, 8, 16)]
+    for name in errs[0]:
+        e = [x[name] for x in errs]
+        assert all(b < 0.5*a for a, b in zip(e, e[1:])), (name, e)
+        assert e[-1] < 1e-10, (name, e)

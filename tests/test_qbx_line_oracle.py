@@ -102,3 +102,16 @@ def test_yuk2_is_k0_over_2pi_and_helm2_is_hankel():
     assert abs(v - special.k0(2.0*0.5)/(2*np.pi)) < 1e-14
     v = ql.SCALE["helm2"]*ql.direct_value("helm2", 2.0, d)
     assert abs(v - 0.25j*special.hankel1(0, 1.0)) < 1e-14
+
+
+@pytest.mark.parametrize("mode", ["slp", "dlp_dy"])
+def test_helm2_complex_k_converges_to_kernel_at_target(mode):
+    """2D Helmholtz with a complex (damped) k: the line expansion still
+    converges to the kernel at the target."""
+    k = 1.3 + 0.4j
+    a, b, n = _pair(2, 0.15, 7)
+    nrm = n if mode.startswith("dlp") else None
+    ax = -1 if mode == "slp" else 1
+    ref = ql.direct_value("helm2", k, a + b, nrm, ax)
+    errs = [abs(ql.pair_value("helm2", k, p, a, b, nrm, ax) - ref)/abs(ref) for p in (2, 5, 9)]
+    assert errs[-1] < 1e-3*errs[0] and errs[-1] < 1e-6, errs

@@ -21,6 +21,7 @@ import pytest
 from oracle import taylor2d as t2
 
 K = 0.05
+KC = 0.2*(0.707 + 0.707j)   # the reference's evanescent k (test_kernels.py:168)
 
 
 def test_derivatives_match_closed_forms():
@@ -37,24 +38,26 @@ def test_derivatives_match_closed_forms():
     assert abs(lap + 4.0*t2.deriv("helm", 2.0, d, 0, 0)) < 1e-13
 
 
-@pytest.mark.parametrize(("kind", "nmax"), [("lap", 8), ("helm", 4)])
-def test_derivatives_match_sympy(kind, nmax):
+@pytest.mark.parametrize(("kind", "nmax", "k"), [("lap", 8, 0), ("helm", 4, 1.3),
+                                              ("helm", 3, 1.3 + 0.4j)])
+def test_derivatives_match_sympy(kind, nmax, k):
     """Exact differentiation of log r / H0(k r) (sympy) at every order up to
-    nmax."""
+    nmax, for a real and a complex (damped) k."""
     import sympy as sp
     x, y = sp.symbols("x y", real=True)
     r = sp.sqrt(x**2 + y**2)
-    g = sp.log(r) if kind == "lap" else sp.hankel1(0, sp.Rational(13, 10)*r)
+    ks = sp.Rational(13, 10) + (sp.Rational(2, 5)*sp.I if isinstance(k, complex) else 0)
+    g = sp.log(r) if kind == "lap" else sp.hankel1(0, ks*r)
     d = np.array([0.7, -0.4])
     for n in range(nmax + 1):
         for my in range(n + 1):
             e = sp.diff(g, x, n - my, y, my) if n else g
             ref = complex(sp.N(e.subs({x: sp.Rational(7, 10), y: sp.Rational(-4, 10)}), 20))
-            got = t2.deriv(kind, 1.3, d, n - my, my)
+            got = t2.deriv(kind, k, d, n - my, my)
             assert abs(got - ref) <= 1e-12*max(1.0, abs(ref)), (n, my, got, ref)
 
 
-def _setup(kind):
+def _setup(kind, k=K):
     rng = np.random.default_rng(18)
     origin = np.array([2.0, 1.0])
     src = 0.7*(-0.5 + rng.random((2, 15))) + origin[:, None]
@@ -64,11 +67,11 @@ def _setup(kind):
     tx, ty = np.meshgrid(g + off[0] + origin[0], g + off[1] + origin[1], indexing="ij")
     tgt = np.array([tx.ravel(), ty.ravel()])
     c = np.array([[0, 0], [-0.2, 0.1], off + [0.3, -0.2], off]).T + origin[:, None]
-    return src, q, tgt, c, t2.p2p(kind, K, tgt, src, q)
+    return src, q, tgt, c, t2.p2p(kind, k, tgt, src, q)
 
 
-def _chains(kind, compressed, p, rs=(0.5, 0.25, 0.5, 0.25)):
-    src, q, tgt, c, ref = _setup(kind)
+def _chains(kind, compressed, p, rs=(0.5, 0.25, 0.5, 0.25), K=K):
+    src, q, tgt, c, ref = _setup(kind, K)
     m1r, m2r, l1r, l2r = rs
     m1 = t2.p2m(p, compressed, kind, K, c[:, 0], src, q, m1r)
     m2 = t2.m2m(p, p, compressed, kind, K, m1, c[:, 0], c[:, 1], m1r, m2r)
@@ -113,3 +116,14 @@ def test_rscale_cancels(kind):
     b = _chains(kind, True, 3, rs=(1.0, 0.1, 0.1, 2.0))
     for name in a:
         assert abs(a[name] - b[name]) <= 1e-6*a[name] + 1e-14, name
+
+
+@pytest.mark.parametrize("compressed", [True, False])
+def test_chains_converge_complex_k(compressed):
+    """Damped waves (complex k, the reference's evanescent value): the same
+    convergence, and the compressed projection with (k rho)^2 complex."""
+    errs = [_chains("helm", compressed, p, rs=(0.5, 0.25, 0.25, 0.25), K=KC) for p in (1, 3, 5)]
+    for name in errs[0]:
+        e = [x[name] for x in errs]
+        assert all(b < 0.2*a for a, b in zip(e, e[1:])), (name, e)
+        assert e[-1] < 1e-5, (name, e)
