@@ -100,7 +100,7 @@ def test_qbx_line_accumulates_and_rejects_bad_input():
 def test_qbx_line_helm2_complex_k_matches_oracle(mode):
     """2D Helmholtz with a complex (damped) k: H_n of complex argument."""
     k = 1.3 + 0.4j
-    tgt, ctr, src, nrm, dens = _case(2, 31)
+    tgt, ctr, src, nrm, dens = _case(2, 31, nt=2, ns=3)
     got = _call_line("helm2", 6, mode, tgt, ctr, src, nrm, dens, k=k)
     skind, sax, tax = MODES[mode]
     ref = ql.layer_potential("helm2", k, 6, tgt, ctr, src, dens,
