@@ -628,20 +628,31 @@ int sumpyhip_t2d_m2l(int kind, const sumpyhip_t2d_expn *src_ex, const sumpyhip_t
  * k_im = 0), orders 0..31, FP64.  Expansions, weights and outputs are complex128 (interleaved
  * re, im); an expansion row holds the 2p+1 coefficients c = -p..p at c + p.
  * Coordinates are SoA; centers is 2 x ncenters.  The evaluations include the
- * kernel's i/4; M2P adds to `out`, L2P and the P2P forms write it. */
+ * kernel's i/4; M2P adds to `out`, L2P and the P2P forms write it.  Source
+ * dipoles (DirectionalSourceDerivative) and target gradients
+ * (AxisTargetDerivative) use the shifts W Z_c(s r) e^{icth} =
+ * -s Z_{c+1} e^{i(c+1)th}, Wbar = s Z_{c-1} e^{i(c-1)th} (W = d_x + i d_y,
+ * s = k or i lam). */
 
-/* P2M of each source box from its own sources (rows box - base of out). */
-int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes, const int *source_boxes,
+/* P2M of each source box from its own sources (rows box - base of out).
+ * weights: charges; weights_dx / weights_dy (either may be NULL): dipole
+ * weights u contributing u . grad_d G(d), d = t - s (as in the 3D P2P; a
+ * DirectionalSourceDerivative source of strength q along n has u = -q n). */
+int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes,
+                     const int *source_boxes, const int *box_source_starts,
+                     const int *box_source_counts, const double *centers, int ncenters,
+                     const double *sx, const double *sy, const void *weights,
+                     const void *weights_dx, const void *weights_dy, double rscale, void *out,
+                     int base, void *stream);
+
+/* P2L of each target box from the sources of its list-4 boxes (CSR); weights
+ * as in P2M. */
+int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt,
+                     const int *target_boxes, const int *starts, const int *lists,
                      const int *box_source_starts, const int *box_source_counts,
                      const double *centers, int ncenters, const double *sx, const double *sy,
-                     const void *weights, double rscale, void *out, int base, void *stream);
-
-/* P2L of each target box from the sources of its list-4 boxes (CSR). */
-int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                     const int *starts, const int *lists, const int *box_source_starts,
-                     const int *box_source_counts, const double *centers, int ncenters,
-                     const double *sx, const double *sy, const void *weights, double rscale,
-                     void *out, int base, void *stream);
+                     const void *weights, const void *weights_dx, const void *weights_dy,
+                     double rscale, void *out, int base, void *stream);
 
 /* M2M: each target box's children (box_child_ids, 4 x nboxes, 0 = none)
  * translated and added to its row. */
@@ -675,33 +686,40 @@ int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k, double k_
                      double tgt_rscale, const void *src, int src_base, void *tgt,
                      int tgt_base, void *stream);
 
-/* L2P of each target box at its own targets (writes). */
-int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                     const int *box_target_starts, const int *box_target_counts,
-                     const double *centers, int ncenters, const double *tx,
-                     const double *ty, double rscale, const void *loc, int base, void *out,
-                     void *stream);
-
-/* M2P from the list-3 boxes of each target box (adds). */
-int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                     const int *starts, const int *lists, const int *box_target_starts,
+/* L2P of each target box at its own targets (writes): the potential into out,
+ * d/dx and d/dy (AxisTargetDerivative) into out_dx, out_dy; any of the three
+ * may be NULL. */
+int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt,
+                     const int *target_boxes, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
-                     const double *tx, const double *ty, double rscale, const void *mpoles,
-                     int base, void *out, void *stream);
+                     const double *tx, const double *ty, double rscale, const void *loc,
+                     int base, void *out, void *out_dx, void *out_dy, void *stream);
+
+/* M2P from the list-3 boxes of each target box (adds); outputs as in L2P. */
+int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt,
+                     const int *target_boxes, const int *starts, const int *lists,
+                     const int *box_target_starts, const int *box_target_counts,
+                     const double *centers, int ncenters, const double *tx, const double *ty,
+                     double rscale, const void *mpoles, int base, void *out, void *out_dx,
+                     void *out_dy, void *stream);
 
 /* Near field over list 1 (CSR); t2s (or NULL): the source index of each
- * target to skip; pairs at zero distance add nothing. */
-int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt, const int *target_boxes, const int *starts,
-                         const int *lists, const int *box_source_starts,
+ * target to skip; pairs at zero distance add nothing.  Weights as in P2M,
+ * outputs as in L2P (written). */
+int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt, const int *target_boxes,
+                         const int *starts, const int *lists, const int *box_source_starts,
                          const int *box_source_counts, const int *box_target_starts,
                          const int *box_target_counts, const double *tx, const double *ty,
                          const double *sx, const double *sy, const void *weights,
-                         const int *t2s, void *out, void *stream);
+                         const void *weights_dx, const void *weights_dy, const int *t2s,
+                         void *out, void *out_dx, void *out_dy, void *stream);
 
-/* Dense direct sum. */
-int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets, const double *tx, const double *ty,
-                           int nsources, const double *sx, const double *sy,
-                           const void *weights, const int *t2s, void *out, void *stream);
+/* Dense direct sum; weights and outputs as in the CSR form. */
+int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets, const double *tx,
+                           const double *ty, int nsources, const double *sx, const double *sy,
+                           const void *weights, const void *weights_dx, const void *weights_dy,
+                           const int *t2s, void *out, void *out_dx, void *out_dy,
+                           void *stream);
 
 /* }}} */
 

@@ -35,23 +35,72 @@ def _ids(order):
     return np.arange(-order, order + 1)
 
 
-def p2p(k, targets, sources, weights, exclude_self_t2s=None):
-    """Direct sum (i/4) H0(k |t - s|) q_s; pairs at zero distance add nothing."""
+def p2p(k, targets, sources, weights, exclude_self_t2s=None, dipoles=None, grad=False):
+    """Direct sum (i/4) H0(k |t - s|) q_s; pairs at zero distance add nothing.
+
+    dipoles = (q n_x, q n_y): ``DirectionalSourceDerivative`` sources,
+    n . grad_s G(t - s) (``sumpy/kernel.py:2288-2385``), added to the
+    charges.  grad: return (potential, d/dt_x, d/dt_y)
+    (``AxisTargetDerivative``), from the Hessian G'' d d^T/r^2 +
+    G'/r (I - d d^T/r^2) with G' = -(i/4) k H1(k r), G'' = -(i/4) k^2 H1'(k r)
+    (scipy's ``h1vp``)."""
     t = np.asarray(targets, np.float64)
     s = np.asarray(sources, np.float64)
-    r = np.hypot(t[0][:, None] - s[0][None, :], t[1][:, None] - s[1][None, :])
-    g = np.zeros_like(r, dtype=np.complex128)
+    dx = t[0][:, None] - s[0][None, :]
+    dy = t[1][:, None] - s[1][None, :]
+    r = np.hypot(dx, dy)
     nz = r > 0
-    g[nz] = SCALE*special.hankel1(0, k*r[nz])
     if exclude_self_t2s is not None:
-        g[np.arange(len(exclude_self_t2s)), exclude_self_t2s] = 0
-    return g @ np.asarray(weights, np.complex128)
+        nz[np.arange(len(exclude_self_t2s)), exclude_self_t2s] = False
+    rr = np.where(nz, r, 1.0)
+    g = np.where(nz, SCALE*special.hankel1(0, k*rr), 0)
+    g1 = np.where(nz, -SCALE*k*special.hankel1(1, k*rr), 0)        # G'
+    g2 = np.where(nz, -SCALE*k*k*special.h1vp(1, k*rr), 0)         # G''
+    q = np.asarray(weights, np.complex128)
+    ux, uy = dx/rr, dy/rr
+    gx, gy = g1*ux, g1*uy                                          # grad_t G
+    pot = g @ q
+    hxx = g2*ux*ux + g1/rr*(1 - ux*ux)
+    hyy = g2*uy*uy + g1/rr*(1 - uy*uy)
+    hxy = (g2 - g1/rr)*ux*uy
+    outx, outy = gx @ q, gy @ q
+    if dipoles is not None:
+        wx, wy = (np.asarray(w, np.complex128) for w in dipoles)
+        # n . grad_s G(t - s) = -n . grad_t G; its target gradient -H n
+        pot = pot - (gx @ wx + gy @ wy)
+        outx = outx - (hxx @ wx + hxy @ wy)
+        outy = outy - (hxy @ wx + hyy @ wy)
+    return (pot, outx, outy) if grad else pot
 
 
-def p2m(order, k, center, sources, weights, rscale):
+def _basis(order, k, vec, sign, hankel, rscale_pow):
+    """Values and x/y derivatives (with respect to the components of vec) of
+    F_c(v) = Z_c(k|v|) e^{i sign c theta_v} rscale_pow(c), c = -order..order,
+    Z = H^(1) or J, by the chain rule in polar coordinates (d/drho through
+    scipy's ``h1vp`` / ``jvp``, d/dtheta = i sign c)."""
+    c = _ids(order)[:, None]
+    v = np.asarray(vec, np.float64)
+    rho, th = np.hypot(v[0], v[1])[None, :], np.arctan2(v[1], v[0])[None, :]
+    z, zp = ((special.hankel1(c, k*rho), special.h1vp(c, k*rho)) if hankel
+             else (special.jv(c, k*rho), special.jvp(c, k*rho)))
+    ph = np.exp(1j*sign*c*th)*rscale_pow(c)
+    f = z*ph
+    frho, fth = k*zp*ph, 1j*sign*c*f
+    fx = np.cos(th)*frho - np.sin(th)/rho*fth
+    fy = np.sin(th)*frho + np.cos(th)/rho*fth
+    return f, fx, fy
+
+
+def p2m(order, k, center, sources, weights, rscale, dipoles=None):
     """``HankelBased2DMultipoleExpansion.coefficients_from_source``
     (``sumpy/expansion/multipole.py:441-470``): J_c(k|a|) / rscale^|c|
-    e^{-i c theta_s}, theta_s = atan2(-a_y, -a_x), summed with the weights."""
+    e^{-i c theta_s}, theta_s = atan2(-a_y, -a_x), summed with the weights;
+    dipoles = (q n_x, q n_y) add n . grad_s of the same terms."""
+    if dipoles is not None:
+        v = np.asarray(sources, np.float64) - np.asarray(center, np.float64)[:, None]
+        f, fx, fy = _basis(order, k, v, -1, False, lambda c: rscale**(-np.abs(c)))
+        return (f @ np.asarray(weights, np.complex128) + fx @ np.asarray(dipoles[0], complex)
+                + fy @ np.asarray(dipoles[1], complex))
     c = _ids(order)
     a = np.asarray(center, np.float64)[:, None] - np.asarray(sources, np.float64)
     rho = np.hypot(a[0], a[1])
@@ -61,9 +110,15 @@ def p2m(order, k, center, sources, weights, rscale):
     return terms @ np.asarray(weights, np.complex128)
 
 
-def m2p(order, k, center, coeffs, targets, rscale):
+def m2p(order, k, center, coeffs, targets, rscale, grad=False):
     """``HankelBased2DMultipoleExpansion.evaluate`` (``multipole.py:472-497``)
-    times the kernel scaling: sum_c M_c H_c(k|b|) rscale^|c| e^{i c theta_b}."""
+    times the kernel scaling: sum_c M_c H_c(k|b|) rscale^|c| e^{i c theta_b};
+    grad: (potential, d/dx, d/dy) at the targets."""
+    if grad:
+        v = np.asarray(targets, np.float64) - np.asarray(center, np.float64)[:, None]
+        f, fx, fy = _basis(order, k, v, 1, True, lambda c: rscale**np.abs(c))
+        cf = np.asarray(coeffs)
+        return SCALE*(cf @ f), SCALE*(cf @ fx), SCALE*(cf @ fy)
     c = _ids(order)
     b = np.asarray(targets, np.float64) - np.asarray(center, np.float64)[:, None]
     rho = np.hypot(b[0], b[1])
@@ -86,10 +141,15 @@ def m2m(src_order, tgt_order, k, coeffs, src_center, tgt_center, src_rscale, tgt
     return T @ np.asarray(coeffs)
 
 
-def p2l(order, k, center, sources, weights, rscale):
+def p2l(order, k, center, sources, weights, rscale, dipoles=None):
     """``FourierBesselLocalExpansionMixin.coefficients_from_source``
     (``sumpy/expansion/local.py:560-583``): H_c(k|a|) rscale^|c|
-    e^{i c theta_s}, theta_s = atan2(-a_y, -a_x)."""
+    e^{i c theta_s}, theta_s = atan2(-a_y, -a_x); dipoles as in p2m."""
+    if dipoles is not None:
+        v = np.asarray(sources, np.float64) - np.asarray(center, np.float64)[:, None]
+        f, fx, fy = _basis(order, k, v, 1, True, lambda c: rscale**np.abs(c))
+        return (f @ np.asarray(weights, np.complex128) + fx @ np.asarray(dipoles[0], complex)
+                + fy @ np.asarray(dipoles[1], complex))
     c = _ids(order)
     a = np.asarray(center, np.float64)[:, None] - np.asarray(sources, np.float64)
     rho = np.hypot(a[0], a[1])
@@ -99,10 +159,15 @@ def p2l(order, k, center, sources, weights, rscale):
     return terms @ np.asarray(weights, np.complex128)
 
 
-def l2p(order, k, center, coeffs, targets, rscale):
+def l2p(order, k, center, coeffs, targets, rscale, grad=False):
     """``FourierBesselLocalExpansionMixin.evaluate`` (``local.py:585-605``)
     times the kernel scaling: sum_c L_c J_c(k|b|) / rscale^|c|
-    e^{-i c theta_b}."""
+    e^{-i c theta_b}; grad as in m2p."""
+    if grad:
+        v = np.asarray(targets, np.float64) - np.asarray(center, np.float64)[:, None]
+        f, fx, fy = _basis(order, k, v, -1, False, lambda c: rscale**(-np.abs(c)))
+        cf = np.asarray(coeffs)
+        return SCALE*(cf @ f), SCALE*(cf @ fx), SCALE*(cf @ fy)
     c = _ids(order)
     b = np.asarray(targets, np.float64) - np.asarray(center, np.float64)[:, None]
     rho = np.hypot(b[0], b[1])

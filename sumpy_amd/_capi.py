@@ -71,18 +71,20 @@ _SIGNATURES = {
                      _P, _P, _P, _P, _P],
     "sumpyhip_m2p": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _D,
                      _P, _I, _P, _P, _P, _P, _P],
-    "sumpyhip_h2d_p2m": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
-    "sumpyhip_h2d_p2l": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_h2d_p2m": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_h2d_p2l": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I,
+                         _P],
     "sumpyhip_h2d_m2m": [_I, _I, _I, _D, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_h2d_l2l": [_I, _I, _I, _D, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_h2d_m2l_data": [_I, _I, _I, _D, _D, _I, _P, _I, _I, _P, _P],
     "sumpyhip_h2d_m2l": [_I, _I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _I, _D, _D, _P, _I, _P,
                          _I, _P],
-    "sumpyhip_h2d_l2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
-    "sumpyhip_h2d_m2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P],
-    "sumpyhip_h2d_p2p_csr": [_I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                             _P],
-    "sumpyhip_h2d_p2p_dense": [_I, _D, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P],
+    "sumpyhip_h2d_l2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P, _P, _P],
+    "sumpyhip_h2d_m2p": [_I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _P, _P,
+                         _P, _P],
+    "sumpyhip_h2d_p2p_csr": [_I, _D, _D, _I] + [_P]*19,
+    "sumpyhip_h2d_p2p_dense": [_I, _D, _D, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                               _P],
     "sumpytree_gpu_morton": [_I, _L, _P, _P, _P, _P, _D, _I, _P, _P],
     "sumpytree_gpu_children": [_I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P],
     "sumpytree_gpu_geometry": [_I, _I, _P, _P, _P, _D, _P, _P, _P],

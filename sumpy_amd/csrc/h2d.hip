@@ -151,13 +151,55 @@ __device__ void add_terms(int p, cplx w, const cplx *Z, double ux, double uy, in
   }
 }
 
+// Bessel argument scale s: k (Helmholtz, real or complex) or i lam (Yukawa)
+template <int KIND>
+__device__ __forceinline__ cplx sarg(cplx kk) {
+  return KIND == 1 ? cplx{0.0, kk.re} : kk;
+}
+
+// acc[p + c] += v for |c| <= p
+__device__ __forceinline__ void acc_add(int p, int c, cplx v, cplx *acc) {
+  if (c >= -p && c <= p) acc[p + c] = cadd(acc[p + c], v);
+}
+
+// Derivatives of F_c = Z_c(s r) e^{i sg c th} (sg = +-1) by the shifts
+// W F_c = -s F_{c+1}, Wbar F_c = s F_{c-1} (W = d_x + i d_y):
+//   d_x F_c = (s/2) (F_{c-1} - F_{c+1}),  d_y F_c = (i sg s/2) (F_{c-1} + F_{c+1}),
+// so n . grad F_c = (s/2) ((n_x + i sg n_y) F_{c-1} - (n_x - i sg n_y) F_{c+1}).
+// Per-lane accumulation of w F_c + a F_{c-1} - b F_{c+1} over c = -p..p; Z
+// holds Z_0..Z_{p+1}.  A dipole weight u (contributing u . grad_d G(t - s),
+// i.e. -u . grad_s) gives a = -(s/2)(u_x + i sg u_y), b = -(s/2)(u_x - i sg u_y).
+__device__ void add_terms_der(int p, cplx w, cplx a, cplx b, const cplx *Z, double ux,
+                              double uy, int sign, cplx *acc) {
+  const cplx e1 = {ux, sign*uy};
+  const cplx mb = {-b.re, -b.im};
+  cplx ph = {1.0, 0.0};
+  for (int m = 0; m <= p + 1; ++m) {
+    const cplx f = cmul(Z[m], ph);
+    acc_add(p, m, cmul(w, f), acc);
+    acc_add(p, m + 1, cmul(a, f), acc);
+    acc_add(p, m - 1, cmul(mb, f), acc);
+    if (m > 0) {
+      // F_-m = (-1)^m Z_m e^{-i sg m th}
+      cplx fn = cmul(Z[m], conj(ph));
+      if (m & 1) fn = cscale(fn, -1.0);
+      acc_add(p, -m, cmul(w, fn), acc);
+      acc_add(p, -m + 1, cmul(a, fn), acc);
+      acc_add(p, -m - 1, cmul(mb, fn), acc);
+    }
+    ph = cmul(ph, e1);
+  }
+}
+
 // {{{ P2M / P2L: one wave per box, lanes over its sources
 
-template <int KIND, bool LOCAL>
+// DER: dipole weights qdx, qdy (either may be null) beside the charges
+template <int KIND, bool LOCAL, bool DER>
 __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
     int p, cplx kk, int nboxes, const int *boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const double *centers, int ncenters, const double *sx,
-    const double *sy, const double *q, double rscale, double *out, int base) {
+    const double *sy, const double *q, const double *qdx, const double *qdy, double rscale,
+    double *out, int base) {
   const int ib = blockIdx.x;
   if (ib >= nboxes) return;
   const int box = boxes[ib];
@@ -176,7 +218,17 @@ __global__ __launch_bounds__(WAVE) void h2d_p2e_kernel(
       double ux, uy;  // e^{i th_s}: the source about the centre
       const double rho = unit(sx[s] - cx, sy[s] - cy, ux, uy);
       const cplx w = {q[2*s], q[2*s + 1]};
-      if (LOCAL) {
+      if (DER) {
+        const int sg = LOCAL ? 1 : -1;
+        const cplx dx = qdx ? cplx{qdx[2*s], qdx[2*s + 1]} : cplx{0.0, 0.0};
+        const cplx dy = qdy ? cplx{qdy[2*s], qdy[2*s + 1]} : cplx{0.0, 0.0};
+        const cplx h = cscale(sarg<KIND>(kk), -0.5);
+        const cplx idy = {-sg*dy.im, sg*dy.re};  // i sg dy
+        const cplx a = cmul(h, cadd(dx, idy)), b = cmul(h, cplx{dx.re - idy.re, dx.im - idy.im});
+        if (LOCAL) seq_h<KIND>(kk, rho, p + 1, Z);
+        else seq_j<KIND>(kk, rho, p + 1, Z);
+        add_terms_der(p, w, a, b, Z, ux, uy, sg, acc);
+      } else if (LOCAL) {
         seq_h<KIND>(kk, rho, p > 0 ? p : 1, Z);
         add_terms(p, w, Z, ux, uy, 1, acc);
       } else {
@@ -353,11 +405,13 @@ __global__ __launch_bounds__(WAVE) void h2d_m2l_kernel(
 // (i/4) z
 __device__ __forceinline__ cplx scale_i4(cplx z) { return {-0.25*z.im, 0.25*z.re}; }
 
-template <int KIND, bool MPOLE>
+// DER: the gradient into gx, gy and the potential into out, any of them null
+template <int KIND, bool MPOLE, bool DER>
 __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
     int p, cplx kk, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bts, const int *btc, const double *centers, int ncenters, const double *tx,
-    const double *ty, double rscale, const double *coeffs, int base, double *out) {
+    const double *ty, double rscale, const double *coeffs, int base, double *out, double *gx,
+    double *gy) {
   __shared__ cplx S[2*PMAX + 1];
   const int it = blockIdx.x;
   if (it >= ntgt) return;
@@ -369,7 +423,7 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
   for (int c0 = 0; c0 < nt; c0 += WAVE) {
     const int t = t0 + c0 + lane;
     const bool active = c0 + lane < nt;
-    cplx acc = {0.0, 0.0};
+    cplx acc = {0.0, 0.0}, accx = {0.0, 0.0}, accy = {0.0, 0.0};
     for (int e = e0; e < e1; ++e) {
       const int sb = MPOLE ? lists[e] : tb;
       if (lane < 2*p + 1) {
@@ -383,6 +437,31 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
         const double rho = unit(tx[t] - centers[sb], ty[t] - centers[ncenters + sb], ux, uy);
         // M2P: e^{i c th}, H_c; L2P: e^{-i c th}, J_c
         const cplx e1u = {ux, MPOLE ? uy : -uy};
+        if (DER) {
+          // pot = sum_c S_c F_c; grad via the shifts of F (add_terms_der):
+          // d_x -> (s/2) sum_m F_m (S_{m+1} - S_{m-1}),
+          // d_y -> (i sg s/2) sum_m F_m (S_{m+1} + S_{m-1})
+          if (MPOLE) seq_h<KIND>(kk, rho, p + 1, Z);
+          else seq_j<KIND>(kk, rho, p + 1, Z);
+          auto sget = [&](int c) { return (c >= -p && c <= p) ? S[p + c] : cplx{0.0, 0.0}; };
+          cplx ph = {1.0, 0.0};
+          for (int m = 0; m <= p + 1; ++m) {
+            const cplx f = cmul(Z[m], ph);
+            acc = cfma(sget(m), f, acc);
+            const cplx up = sget(m + 1), dn = sget(m - 1);
+            accx = cfma(f, cplx{up.re - dn.re, up.im - dn.im}, accx);
+            accy = cfma(f, cadd(up, dn), accy);
+            if (m > 0) {
+              cplx fn = cmul(Z[m], conj(ph));
+              if (m & 1) fn = cscale(fn, -1.0);
+              acc = cfma(sget(-m), fn, acc);
+              const cplx up2 = sget(-m + 1), dn2 = sget(-m - 1);
+              accx = cfma(fn, cplx{up2.re - dn2.re, up2.im - dn2.im}, accx);
+              accy = cfma(fn, cadd(up2, dn2), accy);
+            }
+            ph = cmul(ph, e1u);
+          }
+        } else {
         if (MPOLE) seq_h<KIND>(kk, rho, p > 0 ? p : 1, Z);
         else seq_j<KIND>(kk, rho, p, Z);
         cplx ph = {1.0, 0.0};
@@ -395,13 +474,22 @@ __global__ __launch_bounds__(WAVE) void h2d_e2p_kernel(
           }
           ph = cmul(ph, e1u);
         }
+        }
       }
       __syncthreads();
     }
     if (active) {
-      const cplx v = scale_i4(acc);
       // M2P adds to the targets' values; L2P writes them
-      store_c(out, t, MPOLE ? cadd(load_c(out, t), v) : v);
+      auto put = [&](double *o, cplx v) {
+        if (o) store_c(o, t, MPOLE ? cadd(load_c(o, t), v) : v);
+      };
+      put(out, scale_i4(acc));
+      if (DER) {
+        const cplx h = cscale(sarg<KIND>(kk), 0.5);
+        const cplx ih = MPOLE ? cplx{-h.im, h.re} : cplx{h.im, -h.re};  // i sg s/2
+        put(gx, scale_i4(cmul(h, accx)));
+        put(gy, scale_i4(cmul(ih, accy)));
+      }
     }
   }
 }
@@ -458,13 +546,61 @@ __device__ __forceinline__ cplx green(cplx kk, double r) {
   return {-0.25*y0, 0.25*j0};
 }
 
-// one wave per target box, lanes over its targets; sources of list 1
+// H0(s r), H1(s r) for r > 0 (s = k, complex k or i lam)
 template <int KIND>
+__device__ __forceinline__ void hankel01(cplx kk, double r, cplx &h0, cplx &h1) {
+  if (KIND == 2) {
+    cplx J[2], Y[2];
+    bessel_jy_c(cmul(kk, cplx{r, 0.0}), 1, J, Y);
+    h0 = {J[0].re - Y[0].im, J[0].im + Y[0].re};
+    h1 = {J[1].re - Y[1].im, J[1].im + Y[1].re};
+  } else if (KIND == 1) {
+    // H_n(i x) = (2/pi) i^(-n-1) K_n(x)
+    double I[2], K[2];
+    bessel_ik(kk.re*r, 1, I, K);
+    h0 = {0.0, -(2.0/PI)*K[0]};
+    h1 = {-(2.0/PI)*K[1], 0.0};
+  } else {
+    double J[2], Y[2];
+    bessel_jy(kk.re*r, 1, J, Y);
+    h0 = {J[0], Y[0]};
+    h1 = {J[1], Y[1]};
+  }
+}
+
+// One pair with dipole weights u and the target gradient, d = t - src, G(r) =
+// (i/4) H0(s r): the dipole term u . grad_d G = G'(r) (u . d)/r, the target
+// gradient of a charge G'(r) d/r and of a dipole
+// (G'' - G'/r) d (u . d)/r^2 + G' u/r, with G' = -(i/4) s H1(s r) and
+// G'' - G'/r = -(i/4) s^2 H0 + (i/2) s H1/r.
+template <int KIND>
+__device__ __forceinline__ void pair_der(cplx kk, double dx, double dy, double r, cplx w,
+                                         cplx wx, cplx wy, cplx &pot, cplx &gx, cplx &gy) {
+  cplx h0, h1;
+  hankel01<KIND>(kk, r, h0, h1);
+  const cplx s = sarg<KIND>(kk);
+  const cplx i4 = {0.0, 0.25};
+  const cplx G = cmul(i4, h0);
+  const cplx Gp = cscale(cmul(i4, cmul(s, h1)), -1.0);
+  const cplx K2 = cadd(cscale(cmul(i4, cmul(cmul(s, s), h0)), -1.0),
+                       cscale(cmul(cplx{0.0, 0.5}, cmul(s, h1)), 1.0/r));
+  const double ir = 1.0/r;
+  const cplx nd = cadd(cscale(wx, dx), cscale(wy, dy));  // u . d
+  const cplx gpr = cscale(Gp, ir);                          // G'/r
+  pot = cadd(pot, cadd(cmul(w, G), cmul(gpr, nd)));
+  const cplx k2nd = cscale(cmul(K2, nd), ir*ir);
+  gx = cadd(gx, cadd(cscale(cmul(w, gpr), dx), cadd(cscale(k2nd, dx), cmul(gpr, wx))));
+  gy = cadd(gy, cadd(cscale(cmul(w, gpr), dy), cadd(cscale(k2nd, dy), cmul(gpr, wy))));
+}
+
+// one wave per target box, lanes over its targets; sources of list 1.  DER:
+// dipole weights qdx, qdy and the gradient outputs gx, gy (any may be null)
+template <int KIND, bool DER>
 __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
     cplx kk, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const int *bts, const int *btc, const double *tx,
-    const double *ty, const double *sx, const double *sy, const double *q, const int *t2s,
-    double *out) {
+    const double *ty, const double *sx, const double *sy, const double *q, const double *qdx,
+    const double *qdy, const int *t2s, double *out, double *gx, double *gy) {
   const int it = blockIdx.x;
   if (it >= ntgt) return;
   const int tb = target_boxes[it];
@@ -473,7 +609,7 @@ __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
     const int t = t0 + i;
     const double x = tx[t], y = ty[t];
     const int self = t2s ? t2s[t] : -1;
-    cplx acc = {0.0, 0.0};
+    cplx acc = {0.0, 0.0}, ax = {0.0, 0.0}, ay = {0.0, 0.0};
     for (int e = starts[it]; e < starts[it + 1]; ++e) {
       const int sb = lists[e];
       const int s0 = bss[sb], s1 = s0 + bsc[sb];
@@ -481,29 +617,44 @@ __global__ __launch_bounds__(WAVE) void h2d_p2p_csr_kernel(
         const double dx = x - sx[s], dy = y - sy[s];
         const double r = sqrt(dx*dx + dy*dy);
         if (r == 0.0 || s == self) continue;
-        acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
+        if (DER)
+          pair_der<KIND>(kk, dx, dy, r, cplx{q[2*s], q[2*s + 1]},
+                         qdx ? cplx{qdx[2*s], qdx[2*s + 1]} : cplx{0.0, 0.0},
+                         qdy ? cplx{qdy[2*s], qdy[2*s + 1]} : cplx{0.0, 0.0}, acc, ax, ay);
+        else
+          acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
       }
     }
-    store_c(out, t, acc);
+    if (out) store_c(out, t, acc);
+    if (DER && gx) store_c(gx, t, ax);
+    if (DER && gy) store_c(gy, t, ay);
   }
 }
 
-template <int KIND>
+template <int KIND, bool DER>
 __global__ void h2d_p2p_dense_kernel(cplx kk, int nt, const double *tx, const double *ty,
                                      int ns, const double *sx, const double *sy,
-                                     const double *q, const int *t2s, double *out) {
+                                     const double *q, const double *qdx, const double *qdy,
+                                     const int *t2s, double *out, double *gx, double *gy) {
   const int t = blockIdx.x*blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const double x = tx[t], y = ty[t];
   const int self = t2s ? t2s[t] : -1;
-  cplx acc = {0.0, 0.0};
+  cplx acc = {0.0, 0.0}, ax = {0.0, 0.0}, ay = {0.0, 0.0};
   for (int s = 0; s < ns; ++s) {
     const double dx = x - sx[s], dy = y - sy[s];
     const double r = sqrt(dx*dx + dy*dy);
     if (r == 0.0 || s == self) continue;
-    acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
+    if (DER)
+      pair_der<KIND>(kk, dx, dy, r, cplx{q[2*s], q[2*s + 1]},
+                     qdx ? cplx{qdx[2*s], qdx[2*s + 1]} : cplx{0.0, 0.0},
+                     qdy ? cplx{qdy[2*s], qdy[2*s + 1]} : cplx{0.0, 0.0}, acc, ax, ay);
+    else
+      acc = cfma(green<KIND>(kk, r), cplx{q[2*s], q[2*s + 1]}, acc);
   }
-  store_c(out, t, acc);
+  if (out) store_c(out, t, acc);
+  if (DER && gx) store_c(gx, t, ax);
+  if (DER && gy) store_c(gy, t, ay);
 }
 
 // }}}
@@ -535,46 +686,59 @@ using namespace sumpyhip;
     if (rc_ != SUMPYHIP_OK) return rc_; \
   } while (0)
 
-extern "C" int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes, const int *source_boxes,
-                                const int *box_source_starts, const int *box_source_counts,
-                                const double *centers, int ncenters, const double *sx,
-                                const double *sy, const void *weights, double rscale,
-                                void *out, int base, void *stream) {
+// KIND (0 real k, 1 Yukawa, 2 complex k) x derivative variant of a launch
+#define H2D_LAUNCH(KCALL, der)                   \
+  do {                                           \
+    if (kind == 1) {                             \
+      if (der) KCALL(1, true); else KCALL(1, false); \
+    } else if (k_im != 0.0) {                    \
+      if (der) KCALL(2, true); else KCALL(2, false); \
+    } else {                                     \
+      if (der) KCALL(0, true); else KCALL(0, false); \
+    }                                            \
+  } while (0)
+
+extern "C" int sumpyhip_h2d_p2m(int kind, int order, double k, double k_im, int nboxes,
+                                const int *source_boxes, const int *box_source_starts,
+                                const int *box_source_counts, const double *centers,
+                                int ncenters, const double *sx, const double *sy,
+                                const void *weights, const void *weights_dx,
+                                const void *weights_dy, double rscale, void *out, int base,
+                                void *stream) {
   H2D_CHECK(check_orders(order, order));
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (nboxes == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_p2e_kernel<1, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
-  else if (k_im != 0.0) h2d_p2e_kernel<2, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
-  else h2d_p2e_kernel<0, false><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  SUMPY_REQUIRE(weights, "h2d_p2m: null charge weights");
+#define P2M_CALL(KD, DR)                                                                    \
+  h2d_p2e_kernel<KD, false, DR><<<nboxes, WAVE, 0, (hipStream_t)stream>>>(                 \
+      order, kk, nboxes, source_boxes, nullptr, nullptr, box_source_starts,                \
+      box_source_counts, centers, ncenters, sx, sy, (const double *)weights,               \
+      (const double *)weights_dx, (const double *)weights_dy, rscale, (double *)out, base)
+  H2D_LAUNCH(P2M_CALL, weights_dx || weights_dy);
+#undef P2M_CALL
   return check_launch("h2d_p2m");
 }
 
-extern "C" int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                                const int *starts, const int *lists,
+extern "C" int sumpyhip_h2d_p2l(int kind, int order, double k, double k_im, int ntgt,
+                                const int *target_boxes, const int *starts, const int *lists,
                                 const int *box_source_starts, const int *box_source_counts,
                                 const double *centers, int ncenters, const double *sx,
-                                const double *sy, const void *weights, double rscale,
-                                void *out, int base, void *stream) {
+                                const double *sy, const void *weights, const void *weights_dx,
+                                const void *weights_dy, double rscale, void *out, int base,
+                                void *stream) {
   H2D_CHECK(check_orders(order, order));
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_p2e_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
-  else if (k_im != 0.0) h2d_p2e_kernel<2, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
-  else h2d_p2e_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      centers, ncenters, sx, sy, (const double *)weights, rscale, (double *)out, base);
+  SUMPY_REQUIRE(weights, "h2d_p2l: null charge weights");
+#define P2L_CALL(KD, DR)                                                                    \
+  h2d_p2e_kernel<KD, true, DR><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(                    \
+      order, kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,  \
+      centers, ncenters, sx, sy, (const double *)weights, (const double *)weights_dx,      \
+      (const double *)weights_dy, rscale, (double *)out, base)
+  H2D_LAUNCH(P2L_CALL, weights_dx || weights_dy);
+#undef P2L_CALL
   return check_launch("h2d_p2l");
 }
 
@@ -663,86 +827,86 @@ extern "C" int sumpyhip_h2d_m2l(int kind, int src_order, int tgt_order, double k
   return check_launch("h2d_m2l");
 }
 
-extern "C" int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                                const int *box_target_starts, const int *box_target_counts,
-                                const double *centers, int ncenters, const double *tx,
-                                const double *ty, double rscale, const void *loc, int base,
-                                void *out, void *stream) {
+extern "C" int sumpyhip_h2d_l2p(int kind, int order, double k, double k_im, int ntgt,
+                                const int *target_boxes, const int *box_target_starts,
+                                const int *box_target_counts, const double *centers,
+                                int ncenters, const double *tx, const double *ty, double rscale,
+                                const void *loc, int base, void *out, void *out_dx,
+                                void *out_dy, void *stream) {
   H2D_CHECK(check_orders(order, order));
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_e2p_kernel<1, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
-  else if (k_im != 0.0) h2d_e2p_kernel<2, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
-  else h2d_e2p_kernel<0, false><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out);
+#define L2P_CALL(KD, DR)                                                                    \
+  h2d_e2p_kernel<KD, false, DR><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(                   \
+      order, kk, ntgt, target_boxes, nullptr, nullptr, box_target_starts, box_target_counts, \
+      centers, ncenters, tx, ty, rscale, (const double *)loc, base, (double *)out,         \
+      (double *)out_dx, (double *)out_dy)
+  H2D_LAUNCH(L2P_CALL, out_dx || out_dy || !out);
+#undef L2P_CALL
   return check_launch("h2d_l2p");
 }
 
-extern "C" int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt, const int *target_boxes,
-                                const int *starts, const int *lists,
+extern "C" int sumpyhip_h2d_m2p(int kind, int order, double k, double k_im, int ntgt,
+                                const int *target_boxes, const int *starts, const int *lists,
                                 const int *box_target_starts, const int *box_target_counts,
                                 const double *centers, int ncenters, const double *tx,
                                 const double *ty, double rscale, const void *mpoles, int base,
-                                void *out, void *stream) {
+                                void *out, void *out_dx, void *out_dy, void *stream) {
   H2D_CHECK(check_orders(order, order));
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_e2p_kernel<1, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
-  else if (k_im != 0.0) h2d_e2p_kernel<2, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
-  else h2d_e2p_kernel<0, true><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,
-      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out);
+#define M2P_CALL(KD, DR)                                                                    \
+  h2d_e2p_kernel<KD, true, DR><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(                    \
+      order, kk, ntgt, target_boxes, starts, lists, box_target_starts, box_target_counts,  \
+      centers, ncenters, tx, ty, rscale, (const double *)mpoles, base, (double *)out,      \
+      (double *)out_dx, (double *)out_dy)
+  H2D_LAUNCH(M2P_CALL, out_dx || out_dy || !out);
+#undef M2P_CALL
   return check_launch("h2d_m2p");
 }
 
-extern "C" int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt, const int *target_boxes,
-                                    const int *starts, const int *lists,
+extern "C" int sumpyhip_h2d_p2p_csr(int kind, double k, double k_im, int ntgt,
+                                    const int *target_boxes, const int *starts, const int *lists,
                                     const int *box_source_starts, const int *box_source_counts,
                                     const int *box_target_starts, const int *box_target_counts,
                                     const double *tx, const double *ty, const double *sx,
-                                    const double *sy, const void *weights, const int *t2s,
-                                    void *out, void *stream) {
+                                    const double *sy, const void *weights,
+                                    const void *weights_dx, const void *weights_dy,
+                                    const int *t2s, void *out, void *out_dx, void *out_dy,
+                                    void *stream) {
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (ntgt == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_p2p_csr_kernel<1><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
-      (double *)out);
-  else if (k_im != 0.0) h2d_p2p_csr_kernel<2><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
-      (double *)out);
-  else h2d_p2p_csr_kernel<0><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(
-      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,
-      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights, t2s,
-      (double *)out);
+  SUMPY_REQUIRE(weights, "h2d_p2p_csr: null charge weights");
+#define CSR_CALL(KD, DR)                                                                    \
+  h2d_p2p_csr_kernel<KD, DR><<<ntgt, WAVE, 0, (hipStream_t)stream>>>(                      \
+      kk, ntgt, target_boxes, starts, lists, box_source_starts, box_source_counts,         \
+      box_target_starts, box_target_counts, tx, ty, sx, sy, (const double *)weights,       \
+      (const double *)weights_dx, (const double *)weights_dy, t2s, (double *)out,          \
+      (double *)out_dx, (double *)out_dy)
+  H2D_LAUNCH(CSR_CALL, weights_dx || weights_dy || out_dx || out_dy || !out);
+#undef CSR_CALL
   return check_launch("h2d_p2p_csr");
 }
 
-extern "C" int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets, const double *tx,
-                                      const double *ty, int nsources, const double *sx,
-                                      const double *sy, const void *weights, const int *t2s,
-                                      void *out, void *stream) {
+extern "C" int sumpyhip_h2d_p2p_dense(int kind, double k, double k_im, int ntargets,
+                                      const double *tx, const double *ty, int nsources,
+                                      const double *sx, const double *sy, const void *weights,
+                                      const void *weights_dx, const void *weights_dy,
+                                      const int *t2s, void *out, void *out_dx, void *out_dy,
+                                      void *stream) {
   H2D_CHECK(check_k(kind, k, k_im));
   const cplx kk = {k, k_im};
   if (ntargets == 0) return SUMPYHIP_OK;
-  if (kind == 1) h2d_p2p_dense_kernel<1><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
-      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
-  else if (k_im != 0.0) h2d_p2p_dense_kernel<2><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
-      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
-  else h2d_p2p_dense_kernel<0><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(
-      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights, t2s, (double *)out);
+  SUMPY_REQUIRE(weights, "h2d_p2p_dense: null charge weights");
+#define DENSE_CALL(KD, DR)                                                                  \
+  h2d_p2p_dense_kernel<KD, DR><<<(ntargets + 255)/256, 256, 0, (hipStream_t)stream>>>(     \
+      kk, ntargets, tx, ty, nsources, sx, sy, (const double *)weights,                     \
+      (const double *)weights_dx, (const double *)weights_dy, t2s, (double *)out,          \
+      (double *)out_dx, (double *)out_dy)
+  H2D_LAUNCH(DENSE_CALL, weights_dx || weights_dy || out_dx || out_dy || !out);
+#undef DENSE_CALL
   return check_launch("h2d_p2p_dense");
 }

@@ -13,8 +13,10 @@ mathematics is that of ``sumpy/expansion/multipole.py:426-550``,
 ``sumpy/expansion/local.py:522-704`` and ``sumpy/expansion/m2l.py:1008-1115``
 (Yukawa: the Bessel argument scaled by ``i lam``, ``multipole.py:551-567``).
 
-Supported: potentials (no target derivatives) of charge weights (no source
-derivatives), Helmholtz ``k > 0`` or complex ``k`` with ``Im k > 0`` (the
+Supported: charges and ``DirectionalSourceDerivative`` / ``AxisSourceDerivative``
+dipoles at the sources, potentials and ``AxisTargetDerivative`` gradients at
+the targets (with multipliers and sums of those through
+:class:`~sumpy_amd.tools.TargetOutputs`), Helmholtz ``k > 0`` or complex ``k`` with ``Im k > 0`` (the
 Bessel functions of complex argument then come from the complex recurrences
 of ``csrc/bessel.h``), Yukawa ``lam > 0``, orders 0..31, FP64.
 """
@@ -97,19 +99,25 @@ def _complex(t, what):
 
 
 def _weights(actx, op, kwargs, strengths):
+    """Device pointers (charges, dipole x, dipole y; None without dipoles) and
+    the tensors that hold them."""
     w = SourceWeights(actx, op.source_kernels, op.strength_usage,
                       as_strength_list(strengths, op.strength_count), kwargs, 2, True)
-    if w.wd is not None:
-        raise NotImplementedError("H2D: source derivatives are not implemented")
-    return w.w0.contiguous()
+    return _weight_ptrs(w)
+
+
+def _weight_ptrs(w):
+    keep = [w.w0.contiguous()] + ([x.contiguous() for x in w.wd] if w.wd is not None else [])
+    ptrs = [t.data_ptr() for t in keep] + [None]*(3 - len(keep))
+    return ptrs, keep
 
 
 def _outputs(kernels, result, nt, device):
+    """TargetOutputs over potential and (d/dx, d/dy) buffers, and their
+    pointers [pot, dx, dy] (None where no kernel needs one)."""
     outs = TargetOutputs(kernels, 2)
-    if outs.need_grad or not outs.direct:
-        raise NotImplementedError("H2D: only potentials (no target derivatives)")
     ptrs = outs.buffers(list(result), nt, torch.complex128, device)
-    return outs, ptrs[0]
+    return outs, ptrs[:3]
 
 
 # {{{ P2M, P2L
@@ -122,14 +130,14 @@ def p2m(op, actx, kwargs):
     centers = kwargs.pop("centers").contiguous()
     tgt = _complex(kwargs.pop("tgt_expansions"), "tgt_expansions")
     k = wavenumber(op.expansion.kernel, kwargs)
-    w = _weights(actx, op, kwargs, kwargs.pop("strengths"))
+    w, _keep = _weights(actx, op, kwargs, kwargs.pop("strengths"))
     sb = as_i32(kwargs.pop("source_boxes"))
     bss = as_i32(kwargs.pop("box_source_starts"))
     bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
     _capi.call("sumpyhip_h2d_p2m", kind_of(op.expansion.kernel), _order(op.expansion), *k,
                int(sb.shape[0]), _capi.ptr(sb),
                _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers), int(centers.shape[1]),
-               src[0].data_ptr(), src[1].data_ptr(), w.data_ptr(),
+               src[0].data_ptr(), src[1].data_ptr(), *w,
                _rscale(op.expansion, kwargs.pop("rscale")), _capi.ptr(tgt),
                int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
     return tgt
@@ -144,7 +152,7 @@ def p2l(op, actx, kwargs):
     centers = kwargs.pop("centers").contiguous()
     tgt = _complex(kwargs.pop("tgt_expansions"), "tgt_expansions")
     k = wavenumber(op.expansion.kernel, kwargs)
-    w = _weights(actx, op, kwargs, kwargs.pop("strengths"))
+    w, _keep = _weights(actx, op, kwargs, kwargs.pop("strengths"))
     tb = as_i32(kwargs.pop("target_boxes"))
     starts = as_i32(kwargs.pop("source_box_starts"))
     lists = as_i32(kwargs.pop("source_box_lists"))
@@ -154,7 +162,7 @@ def p2l(op, actx, kwargs):
                int(tb.shape[0]), _capi.ptr(tb),
                _capi.ptr(starts), _capi.ptr(lists), _capi.ptr(bss), _capi.ptr(bsc),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(),
-               src[1].data_ptr(), w.data_ptr(), _rscale(op.expansion, kwargs.pop("rscale")),
+               src[1].data_ptr(), *w, _rscale(op.expansion, kwargs.pop("rscale")),
                _capi.ptr(tgt), int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
     return tgt
 
@@ -276,7 +284,7 @@ def _e2p(op, actx, kwargs, mpole):
     rs = _rscale(op.expansion, kwargs["rscale"])
     kind = kind_of(op.expansion.kernel)
     common = (_capi.ptr(centers), int(centers.shape[1]), targets[0].data_ptr(),
-              targets[1].data_ptr(), rs, _capi.ptr(coeffs), int(kwargs["src_base_ibox"]), out,
+              targets[1].data_ptr(), rs, _capi.ptr(coeffs), int(kwargs["src_base_ibox"]), *out,
               actx.stream_handle)
     if mpole:
         starts = as_i32(kwargs["source_box_starts"])
@@ -313,23 +321,21 @@ def p2p(actx, kernel, kwargs, targets, sources, weights, t2s, target_kernels, re
     boxes, starts, lists, box source starts, counts, box target starts,
     counts), ``P2PFromCSR`` (``sumpy/p2p.py:484-833``) over list 1; writes
     the potentials."""
-    if weights.wd is not None:
-        raise NotImplementedError("H2D: source derivatives are not implemented")
     k = wavenumber(kernel, kwargs)
     nt = targets[0].shape[0]
     outs, out = _outputs(target_kernels, result, nt, actx.device)
-    w = weights.w0.contiguous()
+    w, _keep = _weight_ptrs(weights)
     xy = (targets[0].data_ptr(), targets[1].data_ptr())
     sxy = (sources[0].data_ptr(), sources[1].data_ptr())
     if csr is None:
         _capi.call("sumpyhip_h2d_p2p_dense", kind_of(kernel), *k, nt, *xy,
-                   int(sources[0].shape[0]), *sxy,
-                   w.data_ptr(), _capi.ptr(t2s), out, actx.stream_handle)
+                   int(sources[0].shape[0]), *sxy, *w, _capi.ptr(t2s), *out,
+                   actx.stream_handle)
     else:
         ints = [as_i32(a) for a in csr]
         _capi.call("sumpyhip_h2d_p2p_csr", kind_of(kernel), *k, int(ints[0].shape[0]),
-                   *[_capi.ptr(a) for a in ints], *xy, *sxy, w.data_ptr(), _capi.ptr(t2s),
-                   out, actx.stream_handle)
+                   *[_capi.ptr(a) for a in ints], *xy, *sxy, *w, _capi.ptr(t2s),
+                   *out, actx.stream_handle)
     outs.finish(actx, targets)
     return result
 

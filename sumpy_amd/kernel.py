@@ -259,13 +259,13 @@ def source_terms(knl: ScalarKernel) -> list[SourceTerm]:
         return [SourceTerm(-1, 1.0)]
     if isinstance(knl, AxisSourceDerivative):
         inner = knl.inner_kernel
-        if not isinstance(inner, (LaplaceKernel, HelmholtzKernel)):
+        if not isinstance(inner, (LaplaceKernel, HelmholtzKernel, YukawaKernel)):
             raise NotImplementedError(
                 "only first-order source derivatives are implemented")
         return [SourceTerm(knl.axis, -1.0)]
     if isinstance(knl, DirectionalSourceDerivative):
         inner = knl.inner_kernel
-        if not isinstance(inner, (LaplaceKernel, HelmholtzKernel)):
+        if not isinstance(inner, (LaplaceKernel, HelmholtzKernel, YukawaKernel)):
             raise NotImplementedError(
                 "only first-order source derivatives are implemented")
         return [SourceTerm(a, -1.0, knl.dir_vec_name) for a in range(knl.dim)]

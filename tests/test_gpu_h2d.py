@@ -322,3 +322,141 @@ def test_h2d_fmm_complex_k(actx, use_classes):
               f"rel-inf vs direct {errs[-1]:.2e}")
         assert e_or < 1e-12
     assert errs[1] < 0.1*errs[0] and errs[0] < 1e-3
+
+
+def _der_kernels(kind):
+    from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative
+    knl, mcls, lcls, kname, arg = _kernel(kind)
+    skn = [knl, DirectionalSourceDerivative(knl, "dir_vec")]
+    tkn = [knl, AxisTargetDerivative(0, knl), AxisTargetDerivative(1, knl)]
+    return knl, mcls, lcls, kname, arg, skn, tkn
+
+
+@pytest.mark.parametrize(("kind", "k"), [("helmholtz", 1.3), ("helmholtz_c", 0.2*(0.707 + 0.707j)),
+                                         ("yukawa", 2.0)])
+def test_h2d_p2p_dipoles_and_gradients_match_oracle(actx, kind, k):
+    """P2P (dense and list-1 CSR) with charge + DirectionalSourceDerivative
+    sources (strength_usage [0, 1]) and potential + d/dx + d/dy targets
+    against the oracle."""
+    from sumpy_amd import P2P, P2PFromCSR
+    knl, _, _, kname, arg, skn, tkn = _der_kernels(kind)
+    rng = np.random.default_rng(12)
+    src, tgt = 4*rng.random((2, 500)), 4*rng.random((2, 200))
+    q0 = rng.standard_normal(500) + 1j*rng.standard_normal(500)
+    q1 = rng.standard_normal(500) + 1j*rng.standard_normal(500)
+    n = rng.standard_normal((2, 500))
+    ref = oh.p2p(arg(k), tgt, src, q0, dipoles=(q1*n[0], q1*n[1]), grad=True)
+    d = actx.from_numpy
+    print(f"h2d p2p derivatives {kind} k={k}", flush=True)
+    got = P2P(tkn, exclude_self=False, source_kernels=skn, strength_usage=[0, 1])(
+        actx, d(tgt), d(src), [d(q0), d(q1)], dir_vec=d(n), **{kname: k})
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+    # CSR: one target box (all targets) with one source box (all sources)
+    i32 = lambda a: d(np.asarray(a, np.int32))  # noqa: E731
+    got = P2PFromCSR(tkn, exclude_self=False, source_kernels=skn, strength_usage=[0, 1])(
+        actx, d(tgt), d(src), target_boxes=i32([0]), source_box_starts=i32([0, 1]),
+        source_box_lists=i32([0]), box_source_starts=i32([0]),
+        box_source_counts_nonchild=i32([500]), box_target_starts=i32([0]),
+        box_target_counts_nonchild=i32([200]), strength=[d(q0), d(q1)], dir_vec=d(n),
+        **{kname: k})
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+
+
+@pytest.mark.parametrize(("kind", "order", "k"), [
+    ("helmholtz", 10, 0.05), ("helmholtz_c", 12, 0.2*(0.707 + 0.707j)), ("yukawa", 10, 2.0)])
+def test_h2d_dipole_p2e_and_gradient_e2p_match_oracle(actx, kind, order, k):
+    """P2M and P2L with charge + dipole sources, M2P and L2P with potential +
+    gradient targets, through the operators, against the oracle."""
+    from sumpy_amd import E2PFromCSR, E2PFromSingleBox, P2EFromCSR, P2EFromSingleBox
+    knl, mcls, lcls, kname, arg, skn, tkn = _der_kernels(kind)
+    mpole, local = mcls(knl, order), lcls(knl, order)
+    kw = {kname: k}
+    ka = arg(k)
+    src, _, tgt, c = _chain_geometry()
+    rng = np.random.default_rng(3)
+    q0 = rng.standard_normal(15) + 1j*rng.standard_normal(15)
+    q1 = rng.standard_normal(15) + 1j*rng.standard_normal(15)
+    n = rng.standard_normal((2, 15))
+    dip = (q1*n[0], q1*n[1])
+    d = actx.from_numpy
+    i32 = lambda a: d(np.asarray(a, np.int32))  # noqa: E731
+    centers = d(np.ascontiguousarray(c))
+    bss, bsc = i32([0, 15, 15, 15]), i32([15, 0, 0, 0])
+    bts, btc = i32([0, 0, 0, 0]), i32([0, 0, 0, 400])
+    ncoef = 2*order + 1
+    print(f"h2d dipole/gradient operators {kind} p={order}", flush=True)
+    M = torch.zeros((4, ncoef), dtype=torch.complex128, device=actx.device)
+    P2EFromSingleBox(mpole, kernels=skn, strength_usage=[0, 1])(
+        actx, sources=d(src), centers=centers, strengths=[d(q0), d(q1)], dir_vec=d(n),
+        tgt_expansions=M, tgt_base_ibox=0, rscale=0.5, source_boxes=i32([0]),
+        box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
+    m_ref = oh.p2m(order, ka, c[:, 0], src, q0, 0.5, dipoles=dip)
+    assert rel(actx.to_numpy(M[0]), m_ref) < 1e-12
+    L = torch.zeros((4, ncoef), dtype=torch.complex128, device=actx.device)
+    P2EFromCSR(local, kernels=skn, strength_usage=[0, 1])(
+        actx, sources=d(src), centers=centers, strengths=[d(q0), d(q1)], dir_vec=d(n),
+        tgt_expansions=L, tgt_base_ibox=0, rscale=0.25, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([0]), box_source_starts=bss,
+        box_source_counts_nonchild=bsc, **kw)
+    l_ref = oh.p2l(order, ka, c[:, 3], src, q0, 0.25, dipoles=dip)
+    assert rel(actx.to_numpy(L[3]), l_ref) < 1e-12
+    got = E2PFromSingleBox(local, tkn)(
+        actx, src_expansions=L, src_base_ibox=0, target_boxes=i32([3]), centers=centers,
+        targets=d(tgt), box_target_starts=bts, box_target_counts_nonchild=btc, rscale=0.25,
+        **kw)
+    ref = oh.l2p(order, ka, c[:, 3], l_ref, tgt, 0.25, grad=True)
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+    Mb = torch.zeros_like(M)
+    Mb[1] = M[0]
+    cen = c.copy()
+    cen[:, 1] = c[:, 0]
+    got = E2PFromCSR(mpole, tkn)(
+        actx, src_expansions=Mb, src_base_ibox=0, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([1]),
+        centers=d(np.ascontiguousarray(cen)), targets=d(tgt), box_target_starts=bts,
+        box_target_counts_nonchild=btc, rscale=0.5, **kw)
+    ref = oh.m2p(order, ka, c[:, 0], m_ref, tgt, 0.5, grad=True)
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+    # the expansions reproduce the direct sum at these distances
+    direct = oh.p2p(ka, tgt, src, q0, dipoles=dip, grad=True)
+    for g, r in zip(got, direct):
+        assert rel(actx.to_numpy(g), r) < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["helmholtz", "yukawa"])
+def test_h2d_fmm_dipoles_and_gradients(actx, kind):
+    """The 2D Bessel FMM with charge + dipole sources and potential +
+    gradient targets (the reference's test_sumpy_fmm setup) against the
+    direct sum, converging with the order."""
+    from functools import partial
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl, mcls, lcls, kname, arg, skn, tkn = _der_kernels(kind)
+    k = 0.05 if kind == "helmholtz" else 2.0
+    src, tgt, q = _fmm_case()
+    rng = np.random.default_rng(21)
+    q1 = rng.standard_normal(1000)
+    n = rng.standard_normal((2, 1000))
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    ids = tree.host["user_source_ids"]
+    ref = oh.p2p(arg(k), tgt, src, q, dipoles=(q1*n[0], q1*n[1]), grad=True)
+    errs = []
+    for order in (6, 10):
+        ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                                 tkn, source_kernels=skn, strength_usage=[0, 1])
+        wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: order,
+                                    kernel_extra_kwargs={kname: k},
+                                    source_extra_kwargs={"dir_vec": actx.from_numpy(
+                                        np.ascontiguousarray(n[:, ids]))})
+        print(f"h2d fmm derivatives {kind} p={order}", flush=True)
+        pots = drive_fmm(actx, wr, [actx.from_numpy(q.astype(np.complex128)),
+                                    actx.from_numpy(q1.astype(np.complex128))])
+        errs.append(max(rel(actx.to_numpy(p), r) for p, r in zip(pots, ref)))
+    print(f"{kind}: dipole/gradient FMM errors {errs}")
+    assert errs[1] < 0.2*errs[0] and errs[0] < 1e-3

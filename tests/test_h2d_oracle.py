@@ -126,3 +126,70 @@ def test_h2d_chains_converge_complex_k(k):
         e = [x[name] for x in errs]
         assert all(b < 0.5*a for a, b in zip(e, e[1:])), (name, e)
         assert e[-1] < 1e-10, (name, e)
+
+
+@pytest.mark.parametrize("k", [1.3, 0.2*(0.707 + 0.707j), 2.0j])
+def test_h2d_dipoles_and_gradients_are_derivatives(k):
+    """The dipole (n . grad_source) and target-gradient forms of P2P, P2M,
+    P2L, M2P and L2P equal central differences (h = 1e-5) of the charge /
+    potential forms (k real, complex and the Yukawa scale 2i)."""
+    rng = np.random.default_rng(4)
+    src = rng.random((2, 6))
+    tgt = np.array([[3.1, 2.8, 3.4], [0.2, -0.3, 0.5]])
+    q = rng.standard_normal(6) + 1j*rng.standard_normal(6)
+    n = rng.standard_normal((2, 6))
+    n /= np.linalg.norm(n, axis=0)
+    h = 1e-5
+    c0, c1 = np.array([0.5, 0.5]), np.array([3.2, 0.1])
+
+    def fd_src(f):
+        return (f(src + h*n) - f(src - h*n))/(2*h)
+
+    dip = (q*n[0], q*n[1])
+    zero = np.zeros(6)
+    # P2P: the dipole potential, and the gradients
+    pot, gx, gy = h2d.p2p(k, tgt, src, zero, dipoles=dip, grad=True)
+    ref = fd_src(lambda s: h2d.p2p(k, tgt, s, q))
+    assert np.allclose(pot, ref, rtol=1e-7, atol=1e-9)
+    for ax, g in enumerate((gx, gy)):
+        e = np.eye(2)[:, ax:ax + 1]
+        ref = (h2d.p2p(k, tgt + h*e, src, zero, dipoles=dip)
+               - h2d.p2p(k, tgt - h*e, src, zero, dipoles=dip))/(2*h)
+        assert np.allclose(g, ref, rtol=1e-6, atol=1e-8), ax
+    # P2M / P2L dipoles
+    for fn, c in ((h2d.p2m, c0), (h2d.p2l, c1)):
+        got = fn(6, k, c, src, zero, 0.7, dipoles=dip)
+        ref = fd_src(lambda s: fn(6, k, c, s, q, 0.7))
+        assert np.allclose(got, ref, rtol=1e-7, atol=1e-9), fn.__name__
+    # M2P / L2P gradients
+    coeffs = rng.standard_normal(13) + 1j*rng.standard_normal(13)
+    for fn, c, t in ((h2d.m2p, c0, tgt), (h2d.l2p, c1, tgt)):
+        pot, gx, gy = fn(6, k, c, coeffs, t, 0.7, grad=True)
+        assert np.allclose(pot, fn(6, k, c, coeffs, t, 0.7), rtol=1e-14)
+        for ax, g in enumerate((gx, gy)):
+            e = np.eye(2)[:, ax:ax + 1]
+            ref = (fn(6, k, c, coeffs, t + h*e, 0.7) - fn(6, k, c, coeffs, t - h*e, 0.7))/(2*h)
+            assert np.allclose(g, ref, rtol=1e-7, atol=1e-9), (fn.__name__, ax)
+
+
+@pytest.mark.parametrize("k", [0.05, 0.2*(0.707 + 0.707j)])
+def test_h2d_dipole_gradient_chain_converges(k):
+    """Dipole sources through P2M -> M2M -> M2L -> L2L -> L2P with target
+    gradients converge to the direct dipole-gradient sum (the reference's
+    ``test_p2e2p`` with source derivative and the x derivative)."""
+    src, q, tgt, c, _ = _setup(k)
+    rng = np.random.default_rng(9)
+    n = rng.standard_normal((2, src.shape[1]))
+    n /= np.linalg.norm(n, axis=0)
+    dip = (q*n[0], q*n[1])
+    zero = np.zeros_like(q)
+    ref = h2d.p2p(k, tgt, src, zero, dipoles=dip, grad=True)
+    errs = []
+    for order in (4, 8, 12):
+        m1 = h2d.p2m(order, k, c[:, 0], src, zero, 0.5, dipoles=dip)
+        m2 = h2d.m2m(order, order, k, m1, c[:, 0], c[:, 1], 0.5, 0.25)
+        l1 = h2d.m2l(order, order, k, m2, c[:, 2] - c[:, 1], 0.25, 0.5)
+        l2 = h2d.l2l(order, order, k, l1, c[:, 2], c[:, 3], 0.5, 0.25)
+        got = h2d.l2p(order, k, c[:, 3], l2, tgt, 0.25, grad=True)
+        errs.append(max(np.linalg.norm(g - r)/np.linalg.norm(r) for g, r in zip(got, ref)))
+    assert errs[2] < 1e-3*errs[0] and errs[2] < 1e-9, errs
