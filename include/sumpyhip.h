@@ -542,39 +542,46 @@ typedef struct sumpyhip_t2d_expn {
 } sumpyhip_t2d_expn;
 
 /* P2M, replaces P2EFromSingleBox.__call__ (sumpy/p2e.py:155-261): overwrites
- * the rows of the source boxes (row box - tgt_base_ibox). */
+ * the rows of the source boxes (row box - tgt_base_ibox).  weights_dx /
+ * weights_dy (NULL: none): dipole weights u contributing u . grad_d G(d),
+ * d = t - s, as in the H2D functions. */
 int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int nboxes,
                      const int *source_boxes, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
-                     const double *sx, const double *sy, const double *weights, double rscale,
+                     const double *sx, const double *sy, const double *weights,
+                     const double *weights_dx, const double *weights_dy, double rscale,
                      double *tgt_expansions, int tgt_base_ibox, void *stream);
 
 /* L2P, replaces E2PFromSingleBox.__call__ (sumpy/e2p.py:123-226): writes the
- * potentials of the target boxes' targets. */
+ * potentials (result) and d/dx, d/dy (result_dx, result_dy) of the target
+ * boxes' targets; any of the three may be NULL. */
 int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
                      const double *tx, const double *ty, double rscale,
                      const double *src_expansions, int src_base_ibox, double scale_re,
-                     double scale_im, double *result, void *stream);
+                     double scale_im, double *result, double *result_dx, double *result_dy,
+                     void *stream);
 
 /* M2P, replaces E2PFromCSR.__call__ (sumpy/e2p.py:233-343): adds the list-3
- * multipoles' potentials. */
+ * multipoles' potentials (and gradients, as in L2P). */
 int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *source_box_starts,
                      const int *source_box_lists, const int *box_target_starts,
                      const int *box_target_counts, const double *centers, int ncenters,
                      const double *tx, const double *ty, double rscale,
                      const double *src_expansions, int src_base_ibox, double scale_re,
-                     double scale_im, double *result, void *stream);
+                     double scale_im, double *result, double *result_dx, double *result_dy,
+                     void *stream);
 
 /* P2L, replaces P2EFromCSR.__call__ (sumpy/p2e.py:268-388): overwrites the
- * target boxes' rows from their list-4 sources. */
+ * target boxes' rows from their list-4 sources (dipole weights as in P2M). */
 int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k, double k_im, int ntgt,
                      const int *target_boxes, const int *source_box_starts,
                      const int *source_box_lists, const int *box_source_starts,
                      const int *box_source_counts, const double *centers, int ncenters,
-                     const double *sx, const double *sy, const double *weights, double rscale,
+                     const double *sx, const double *sy, const double *weights,
+                     const double *weights_dx, const double *weights_dy, double rscale,
                      double *tgt_expansions, int tgt_base_ibox, void *stream);
 
 /* M2M, replaces E2EFromChildren.__call__ (sumpy/e2e.py:866-978): adds the

@@ -80,37 +80,71 @@ def _mono(mi, v, rscale):
     return np.prod([(v[i]/rscale)**mi[i]/factorial(mi[i]) for i in range(2)])
 
 
-def p2m(order, compressed, kind, k, center, sources, weights, rscale):
+def _mono_d(mi, v, rscale, ax):
+    """d/dv_ax of (v/rscale)^mi/mi!."""
+    if mi[ax] == 0:
+        return 0.0
+    m = list(mi)
+    m[ax] -= 1
+    return _mono(m, v, rscale)/rscale
+
+
+def p2m(order, compressed, kind, k, center, sources, weights, rscale, dipoles=None):
+    """dipoles = (q n_x, q n_y): n . grad_s of the moments (a = c - s, so
+    -n . grad_a)."""
     w = _wr(order, compressed, kind)
+    srcs = np.asarray(sources).T
     full = np.array([sum(q*_mono(mi, np.asarray(center) - s, rscale)
-                         for s, q in zip(np.asarray(sources).T, weights))
+                         for s, q in zip(srcs, weights))
                      for mi in w.full], dtype=np.complex128)
+    if dipoles is not None:
+        full -= np.array([sum(dx*_mono_d(mi, np.asarray(center) - s, rscale, 0)
+                              + dy*_mono_d(mi, np.asarray(center) - s, rscale, 1)
+                              for s, dx, dy in zip(srcs, *dipoles))
+                          for mi in w.full], dtype=np.complex128)
     return _proj(order, compressed, kind, rscale, k).T @ full
 
 
-def m2p(order, compressed, kind, k, center, coeffs, targets, rscale):
+def m2p(order, compressed, kind, k, center, coeffs, targets, rscale, grad=False):
+    """grad: (potential, d/dx, d/dy): rho^|m| d^(m + e) G."""
     w = _wr(order, compressed, kind)
     out = []
     for t in np.asarray(targets).T:
         b = t - np.asarray(center)
-        out.append(sum(c*rscale**sum(mi)*deriv(kind, k, b, *mi)
-                       for mi, c in zip(w.stored, coeffs)))
-    return SCALE[kind]*np.array(out)
+        out.append([sum(c*rscale**sum(mi)*deriv(kind, k, b, mi[0] + ex, mi[1] + ey)
+                        for mi, c in zip(w.stored, coeffs))
+                    for ex, ey in ((0, 0), (1, 0), (0, 1))[:3 if grad else 1]])
+    out = SCALE[kind]*np.array(out)
+    return tuple(out.T) if grad else out[:, 0]
 
 
-def p2l(order, compressed, kind, k, center, sources, weights, rscale):
+def p2l(order, compressed, kind, k, center, sources, weights, rscale, dipoles=None):
+    """dipoles as in p2m: n . grad_s of rho^|m| d^m G(c - s)."""
     w = _wr(order, compressed, kind)
-    return np.array([sum(q*rscale**sum(mi)*deriv(kind, k, np.asarray(center) - s, *mi)
-                         for s, q in zip(np.asarray(sources).T, weights))
-                     for mi in w.stored], dtype=np.complex128)
+    srcs = np.asarray(sources).T
+    out = np.array([sum(q*rscale**sum(mi)*deriv(kind, k, np.asarray(center) - s, *mi)
+                        for s, q in zip(srcs, weights))
+                    for mi in w.stored], dtype=np.complex128)
+    if dipoles is not None:
+        out -= np.array([sum(rscale**sum(mi)*(
+            dx*deriv(kind, k, np.asarray(center) - s, mi[0] + 1, mi[1])
+            + dy*deriv(kind, k, np.asarray(center) - s, mi[0], mi[1] + 1))
+            for s, dx, dy in zip(srcs, *dipoles)) for mi in w.stored], dtype=np.complex128)
+    return out
 
 
-def l2p(order, compressed, kind, k, center, coeffs, targets, rscale):
+def l2p(order, compressed, kind, k, center, coeffs, targets, rscale, grad=False):
+    """grad: (potential, d/dx, d/dy) of the local polynomial."""
     w = _wr(order, compressed, kind)
     full = _proj(order, compressed, kind, rscale, k) @ np.asarray(coeffs)
-    out = [sum(c*_mono(mi, t - np.asarray(center), rscale) for mi, c in zip(w.full, full))
-           for t in np.asarray(targets).T]
-    return SCALE[kind]*np.array(out)
+    ts = np.asarray(targets).T
+    out = np.array([sum(c*_mono(mi, t - np.asarray(center), rscale) for mi, c in zip(w.full, full))
+                    for t in ts])
+    if not grad:
+        return SCALE[kind]*out
+    g = [np.array([sum(c*_mono_d(mi, t - np.asarray(center), rscale, ax)
+                       for mi, c in zip(w.full, full)) for t in ts]) for ax in (0, 1)]
+    return SCALE[kind]*out, SCALE[kind]*g[0], SCALE[kind]*g[1]
 
 
 def m2m(src_order, tgt_order, compressed, kind, k, coeffs, src_center, tgt_center,
@@ -161,15 +195,33 @@ def stored_ids(order, compressed, kind="lap"):
     return list(_wr(order, compressed, kind).stored)
 
 
-def p2p(kind, k, targets, sources, weights):
+def p2p(kind, k, targets, sources, weights, dipoles=None, grad=False):
+    """Direct sum; dipoles = (q n_x, q n_y) add n . grad_s G(t - s); grad:
+    (potential, d/dt_x, d/dt_y).  Helmholtz: ``oracle.h2d.p2p``; Laplace:
+    log r with grad log r = d/r^2 and the Hessian I/r^2 - 2 d d^T/r^4."""
+    if kind != "lap":
+        from . import h2d
+        return h2d.p2p(k, targets, sources, weights, dipoles=dipoles, grad=grad)
     t, s = np.asarray(targets), np.asarray(sources)
-    r = np.hypot(t[0][:, None] - s[0][None, :], t[1][:, None] - s[1][None, :])
-    if kind == "lap":
-        g = np.where(r > 0, np.log(np.where(r > 0, r, 1.0)), 0.0)
-    else:
-        from scipy.special import hankel1
-        g = np.where(r > 0, hankel1(0, k*np.where(r > 0, r, 1.0)), 0.0)
-    return SCALE[kind]*(g @ np.asarray(weights))
+    dx = t[0][:, None] - s[0][None, :]
+    dy = t[1][:, None] - s[1][None, :]
+    r2 = dx*dx + dy*dy
+    nz = r2 > 0
+    rr2 = np.where(nz, r2, 1.0)
+    g = np.where(nz, 0.5*np.log(rr2), 0.0)
+    gx, gy = np.where(nz, dx/rr2, 0.0), np.where(nz, dy/rr2, 0.0)
+    hxx = np.where(nz, 1/rr2 - 2*dx*dx/rr2**2, 0.0)
+    hyy = np.where(nz, 1/rr2 - 2*dy*dy/rr2**2, 0.0)
+    hxy = np.where(nz, -2*dx*dy/rr2**2, 0.0)
+    q = np.asarray(weights)
+    pot, ox, oy = g @ q, gx @ q, gy @ q
+    if dipoles is not None:
+        wx, wy = (np.asarray(w) for w in dipoles)
+        pot = pot - (gx @ wx + gy @ wy)
+        ox = ox - (hxx @ wx + hxy @ wy)
+        oy = oy - (hxy @ wx + hyy @ wy)
+    out = SCALE[kind]*pot, SCALE[kind]*ox, SCALE[kind]*oy
+    return out if grad else out[0]
 
 
 __all__ = ["SCALE", "deriv", "l2l", "l2p", "m2l", "m2m", "m2p", "p2l", "p2m", "p2p",

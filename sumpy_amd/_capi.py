@@ -125,13 +125,14 @@ _SIGNATURES = {
                               _D, _I, _P, _P],
     "sumpyhip_axpy_weights": [_I, _I, _D, _P, _P, _P, _P],
     "sumpyhip_combine_outputs": [_I, _I, _I, _P, _P, _P, _P, _I, _P],
-    "sumpyhip_t2d_p2m": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I, _P],
+    "sumpyhip_t2d_p2m": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P, _I,
+                         _P],
     "sumpyhip_t2d_l2p": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D, _D, _P,
-                         _P],
+                         _P, _P, _P],
     "sumpyhip_t2d_m2p": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _D, _P, _I, _D,
-                         _D, _P, _P],
-    "sumpyhip_t2d_p2l": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D, _P, _I,
-                         _P],
+                         _D, _P, _P, _P, _P],
+    "sumpyhip_t2d_p2l": [_I, _P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D,
+                         _P, _I, _P],
     "sumpyhip_t2d_m2m": [_I, _P, _P, _D, _D, _I, _P, _P, _I, _P, _I, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_t2d_l2l": [_I, _P, _P, _D, _D, _I, _P, _P, _P, _I, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_t2d_m2l_data": [_I, _I, _D, _D, _I, _P, _I, _I, _D, _P, _P],

@@ -209,16 +209,21 @@ __device__ __forceinline__ double ipow(double x, int n) {
 
 // {{{ P2M
 
+// Dipole weights u (wdx, wdy; NULL: none) contribute u . grad_d G(t - s): the
+// derivative of the moments in a = c - s, (u_x/rho) (a/rho)^(m - e_x)/(m - e_x)!
+// and the same in y.
 template <bool CX>
 __global__ __launch_bounds__(TTH) void t2_p2m(T2E ex, const int *source_boxes,
                                               const int *bss, const int *bsc,
                                               const double *centers, int ncenters,
                                               const double *sx, const double *sy,
-                                              const double *w, double rscale, cplx kap,
+                                              const double *w, const double *wdx,
+                                              const double *wdy, double rscale, cplx kap,
                                               double *out, int base) {
   __shared__ cplx F[TNF];
   __shared__ double px[TCH][TPMAX + 1], py[TCH][TPMAX + 1];
-  __shared__ cplx sw[TCH];
+  __shared__ cplx sw[TCH], swx[TCH], swy[TCH];
+  const bool dip = wdx || wdy;
   const int box = source_boxes[blockIdx.x], p = ex.order, nf = nfull(p);
   const double cx = centers[box], cy = centers[ncenters + box];
   const int s0 = bss[box], ns = bsc[box];
@@ -239,11 +244,21 @@ __global__ __launch_bounds__(TTH) void t2_p2m(T2E ex, const int *source_boxes,
         fy *= ay/(i + 1);
       }
       sw[threadIdx.x] = ld<CX>(w, s);
+      if (dip) {
+        swx[threadIdx.x] = wdx ? cscale(ld<CX>(wdx, s), 1.0/rscale) : cplx{0.0, 0.0};
+        swy[threadIdx.x] = wdy ? cscale(ld<CX>(wdy, s), 1.0/rscale) : cplx{0.0, 0.0};
+      }
     }
     __syncthreads();
     const int cnt = min(TCH, ns - c0);
-    if (e < nf)
+    if (e < nf) {
       for (int j = 0; j < cnt; ++j) acc = cfma_s(sw[j], px[j][mx]*py[j][my], acc);
+      if (dip)
+        for (int j = 0; j < cnt; ++j) {
+          if (mx > 0) acc = cfma_s(swx[j], px[j][mx - 1]*py[j][my], acc);
+          if (my > 0) acc = cfma_s(swy[j], px[j][mx]*py[j][my - 1], acc);
+        }
+    }
   }
   for (int i = e; i < TNF; i += TTH) F[i] = {0.0, 0.0};
   __syncthreads();
@@ -258,12 +273,15 @@ __global__ __launch_bounds__(TTH) void t2_p2m(T2E ex, const int *source_boxes,
 
 // {{{ L2P, M2P
 
+// out (potential), gx, gy (d/dx, d/dy: the local polynomial differentiated,
+// (1/rho) F_{m + e} (b/rho)^m/m!); any may be NULL
 template <bool CX>
 __global__ __launch_bounds__(TTH) void t2_l2p(T2E ex, const int *target_boxes, const int *bts,
                                               const int *btc, const double *centers,
                                               int ncenters, const double *tx, const double *ty,
                                               double rscale, cplx kap, const double *src,
-                                              int base, cplx scale, double *out) {
+                                              int base, cplx scale, double *out, double *gx,
+                                              double *gy) {
   __shared__ cplx F[TNF];
   const int box = target_boxes[blockIdx.x], p = ex.order;
   for (int i = threadIdx.x; i < TNF; i += TTH) F[i] = {0.0, 0.0};
@@ -280,25 +298,37 @@ __global__ __launch_bounds__(TTH) void t2_l2p(T2E ex, const int *target_boxes, c
     double fy[TPMAX + 1];
     fy[0] = 1.0;
     for (int i = 1; i <= p; ++i) fy[i] = fy[i - 1]*by/i;
-    cplx acc = {0.0, 0.0};
-    double fx = 1.0;
+    cplx acc = {0.0, 0.0}, ax = {0.0, 0.0}, ay = {0.0, 0.0};
+    double fx = 1.0, fxm = 0.0;  // (b_x/rho)^mx/mx!, the same at mx - 1
     for (int mx = 0; mx <= p; ++mx) {
-      cplx s = {0.0, 0.0};
-      for (int my = 0; my <= p - mx; ++my) s = cfma_s(F[tri(mx, my)], fy[my], s);
+      cplx s = {0.0, 0.0}, sy = {0.0, 0.0};
+      for (int my = 0; my <= p - mx; ++my) {
+        s = cfma_s(F[tri(mx, my)], fy[my], s);
+        if (my > 0) sy = cfma_s(F[tri(mx, my)], fy[my - 1], sy);
+      }
       acc = cfma_s(s, fx, acc);
+      ax = cfma_s(s, fxm, ax);
+      ay = cfma_s(sy, fx, ay);
+      fxm = fx;
       fx *= bx/(mx + 1);
     }
-    stv<CX>(out, t, cmul(acc, scale));
+    if (out) stv<CX>(out, t, cmul(acc, scale));
+    if (gx) stv<CX>(gx, t, cmul(cscale(ax, 1.0/rscale), scale));
+    if (gy) stv<CX>(gy, t, cmul(cscale(ay, 1.0/rscale), scale));
   }
 }
 
+// out, gx, gy as in t2_l2p (added to): d/dx of rho^|m| d^m G is
+// rho^|m| d^(m + e_x) G
 template <int KIND>
 __global__ __launch_bounds__(TTH) void t2_m2p(T2E ex, const int *target_boxes, const int *starts,
                                               const int *lists, const int *bts, const int *btc,
                                               const double *centers, int ncenters,
                                               const double *tx, const double *ty,
                                               double rscale, cplx k, const double *src,
-                                              int base, cplx scale, double *out) {
+                                              int base, cplx scale, double *out, double *gx,
+                                              double *gy) {
+  const bool grad = gx || gy;
   constexpr bool CX = KIND == 1;
   __shared__ cplx M[TNF];
   const int it = blockIdx.x, box = target_boxes[it];
@@ -309,7 +339,7 @@ __global__ __launch_bounds__(TTH) void t2_m2p(T2E ex, const int *target_boxes, c
   for (int tb = 0; tb < nt; tb += TTH) {
     const int t = t0 + tb + threadIdx.x;
     const bool act = tb + threadIdx.x < nt;
-    cplx acc = {0.0, 0.0};
+    cplx acc = {0.0, 0.0}, ax = {0.0, 0.0}, ay = {0.0, 0.0};
     for (int q = starts[it]; q < starts[it + 1]; ++q) {
       const int sb = lists[q];
       __syncthreads();
@@ -317,13 +347,23 @@ __global__ __launch_bounds__(TTH) void t2_m2p(T2E ex, const int *target_boxes, c
       for (int j = threadIdx.x; j < ex.nstored; j += TTH) M[j] = ld<CX>(row, j);
       __syncthreads();
       if (!act) continue;
-      cplx Wp[TPMAX + 1], Wm[TPMAX + 1];
+      cplx Wp[TPMAX + 2], Wm[TPMAX + 2];
       wseq<KIND>((tx[t] - centers[sb])/rscale, (ty[t] - centers[ncenters + sb])/rscale, kr,
-                 lrho, ex.order, Wp, Wm);
-      for (int j = 0; j < ex.nstored; ++j)
-        acc = cfma(M[j], deriv<KIND>(ex.smx[j], ex.smy[j], Wp, Wm, mk2), acc);
+                 lrho, ex.order + (grad ? 1 : 0), Wp, Wm);
+      for (int j = 0; j < ex.nstored; ++j) {
+        const int mx = ex.smx[j], my = ex.smy[j];
+        if (out) acc = cfma(M[j], deriv<KIND>(mx, my, Wp, Wm, mk2), acc);
+        if (grad) {
+          ax = cfma(M[j], deriv<KIND>(mx + 1, my, Wp, Wm, mk2), ax);
+          ay = cfma(M[j], deriv<KIND>(mx, my + 1, Wp, Wm, mk2), ay);
+        }
+      }
     }
-    if (act) addv<CX>(out, t, cmul(acc, scale));
+    if (act) {
+      if (out) addv<CX>(out, t, cmul(acc, scale));
+      if (gx) addv<CX>(gx, t, cmul(cscale(ax, 1.0/rscale), scale));
+      if (gy) addv<CX>(gy, t, cmul(cscale(ay, 1.0/rscale), scale));
+    }
   }
 }
 
@@ -331,16 +371,20 @@ __global__ __launch_bounds__(TTH) void t2_m2p(T2E ex, const int *target_boxes, c
 
 // {{{ P2L
 
+// Dipole weights u (NULL: none): L_m += (u . grad) rho^|m| d^m G(c - s), i.e.
+// (u_x rho^|m| d^(m + e_x) G + u_y ...), from the derivatives up to order p + 1
 template <int KIND>
 __global__ __launch_bounds__(TTH) void t2_p2l(T2E ex, const int *target_boxes, const int *starts,
                                               const int *lists, const int *bss, const int *bsc,
                                               const double *centers, int ncenters,
                                               const double *sx, const double *sy,
-                                              const double *w, double rscale, cplx k,
+                                              const double *w, const double *wdx,
+                                              const double *wdy, double rscale, cplx k,
                                               double *out, int base) {
   constexpr bool CX = KIND == 1;
-  __shared__ cplx Wp[TCH][TPMAX + 1], Wm[TCH][TPMAX + 1];
-  __shared__ cplx sw[TCH];
+  __shared__ cplx Wp[TCH][TPMAX + 2], Wm[TCH][TPMAX + 2];
+  __shared__ cplx sw[TCH], swx[TCH], swy[TCH];
+  const bool dip = wdx || wdy;
   const int it = blockIdx.x, box = target_boxes[it];
   const double cx = centers[box], cy = centers[ncenters + box];
   const cplx kr = cscale(k, rscale), mk2 = cscale(cmul(kr, kr), -1.0);
@@ -355,15 +399,24 @@ __global__ __launch_bounds__(TTH) void t2_p2l(T2E ex, const int *target_boxes, c
       __syncthreads();
       if (threadIdx.x < TCH && c0 + threadIdx.x < ns) {
         const int s = s0 + c0 + threadIdx.x;
-        wseq<KIND>((cx - sx[s])/rscale, (cy - sy[s])/rscale, kr, lrho, ex.order,
+        wseq<KIND>((cx - sx[s])/rscale, (cy - sy[s])/rscale, kr, lrho, ex.order + (dip ? 1 : 0),
                    Wp[threadIdx.x], Wm[threadIdx.x]);
         sw[threadIdx.x] = ld<CX>(w, s);
+        if (dip) {
+          swx[threadIdx.x] = wdx ? cscale(ld<CX>(wdx, s), 1.0/rscale) : cplx{0.0, 0.0};
+          swy[threadIdx.x] = wdy ? cscale(ld<CX>(wdy, s), 1.0/rscale) : cplx{0.0, 0.0};
+        }
       }
       __syncthreads();
       const int cnt = min(TCH, ns - c0);
       if (act)
-        for (int s = 0; s < cnt; ++s)
+        for (int s = 0; s < cnt; ++s) {
           acc = cfma(sw[s], deriv<KIND>(mx, my, Wp[s], Wm[s], mk2), acc);
+          if (dip) {
+            acc = cfma(swx[s], deriv<KIND>(mx + 1, my, Wp[s], Wm[s], mk2), acc);
+            acc = cfma(swy[s], deriv<KIND>(mx, my + 1, Wp[s], Wm[s], mk2), acc);
+          }
+        }
     }
   }
   // (the reference's P2L keeps multi-index rows up to TTH >= 91 stored)
@@ -575,8 +628,9 @@ extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k,
                                 const int *source_boxes, const int *box_source_starts,
                                 const int *box_source_counts, const double *centers,
                                 int ncenters, const double *sx, const double *sy,
-                                const double *weights, double rscale, double *tgt_expansions,
-                                int tgt_base_ibox, void *stream) {
+                                const double *weights, const double *weights_dx,
+                                const double *weights_dy, double rscale,
+                                double *tgt_expansions, int tgt_base_ibox, void *stream) {
   T2_KIND(kind, "t2d_p2m");
   int rc = check_expn(ex, "t2d_p2m");
   if (rc) return rc;
@@ -589,11 +643,13 @@ extern "C" int sumpyhip_t2d_p2m(int kind, const sumpyhip_t2d_expn *ex, double k,
   if (kind)
     t2_p2m<true><<<nboxes, TTH, 0, st>>>(to_t2e(ex), source_boxes, box_source_starts,
                                          box_source_counts, centers, ncenters, sx, sy, weights,
-                                         rscale, kap, tgt_expansions, tgt_base_ibox);
+                                         weights_dx, weights_dy, rscale, kap, tgt_expansions,
+                                         tgt_base_ibox);
   else
     t2_p2m<false><<<nboxes, TTH, 0, st>>>(to_t2e(ex), source_boxes, box_source_starts,
                                           box_source_counts, centers, ncenters, sx, sy, weights,
-                                          rscale, kap, tgt_expansions, tgt_base_ibox);
+                                          weights_dx, weights_dy, rscale, kap, tgt_expansions,
+                                          tgt_base_ibox);
   return check_launch("t2d_p2m");
 }
 
@@ -602,13 +658,14 @@ extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k,
                                 const int *box_target_counts, const double *centers,
                                 int ncenters, const double *tx, const double *ty, double rscale,
                                 const double *src_expansions, int src_base_ibox,
-                                double scale_re, double scale_im, double *result, void *stream) {
+                                double scale_re, double scale_im, double *result,
+                                double *result_dx, double *result_dy, void *stream) {
   T2_KIND(kind, "t2d_l2p");
   int rc = check_expn(ex, "t2d_l2p");
   if (rc) return rc;
   if (ntgt == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(ntgt > 0 && target_boxes && box_target_starts && box_target_counts && centers &&
-                    tx && ty && src_expansions && result && rscale > 0,
+                    tx && ty && src_expansions && (result || result_dx || result_dy) && rscale > 0,
                 "t2d_l2p: bad arguments");
   const cplx kap = ksq(kk, rscale);
   const cplx sc = {scale_re, scale_im};
@@ -616,11 +673,13 @@ extern "C" int sumpyhip_t2d_l2p(int kind, const sumpyhip_t2d_expn *ex, double k,
   if (kind)
     t2_l2p<true><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, box_target_starts,
                                        box_target_counts, centers, ncenters, tx, ty, rscale, kap,
-                                       src_expansions, src_base_ibox, sc, result);
+                                       src_expansions, src_base_ibox, sc, result, result_dx,
+                                       result_dy);
   else
     t2_l2p<false><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, box_target_starts,
                                         box_target_counts, centers, ncenters, tx, ty, rscale,
-                                        kap, src_expansions, src_base_ibox, sc, result);
+                                        kap, src_expansions, src_base_ibox, sc, result, result_dx,
+                                        result_dy);
   return check_launch("t2d_l2p");
 }
 
@@ -630,14 +689,15 @@ extern "C" int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k,
                                 const int *box_target_counts, const double *centers,
                                 int ncenters, const double *tx, const double *ty, double rscale,
                                 const double *src_expansions, int src_base_ibox,
-                                double scale_re, double scale_im, double *result, void *stream) {
+                                double scale_re, double scale_im, double *result,
+                                double *result_dx, double *result_dy, void *stream) {
   T2_KIND(kind, "t2d_m2p");
   int rc = check_expn(ex, "t2d_m2p");
   if (rc) return rc;
   if (ntgt == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(ntgt > 0 && target_boxes && source_box_starts && source_box_lists &&
                     box_target_starts && box_target_counts && centers && tx && ty &&
-                    src_expansions && result && rscale > 0,
+                    src_expansions && (result || result_dx || result_dy) && rscale > 0,
                 "t2d_m2p: bad arguments");
   const cplx sc = {scale_re, scale_im};
   hipStream_t st = (hipStream_t)stream;
@@ -645,12 +705,12 @@ extern "C" int sumpyhip_t2d_m2p(int kind, const sumpyhip_t2d_expn *ex, double k,
     t2_m2p<1><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_target_starts, box_target_counts,
                                     centers, ncenters, tx, ty, rscale, kk, src_expansions,
-                                    src_base_ibox, sc, result);
+                                    src_base_ibox, sc, result, result_dx, result_dy);
   else
     t2_m2p<0><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_target_starts, box_target_counts,
                                     centers, ncenters, tx, ty, rscale, kk, src_expansions,
-                                    src_base_ibox, sc, result);
+                                    src_base_ibox, sc, result, result_dx, result_dy);
   return check_launch("t2d_m2p");
 }
 
@@ -659,8 +719,9 @@ extern "C" int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k,
                                 const int *source_box_lists, const int *box_source_starts,
                                 const int *box_source_counts, const double *centers,
                                 int ncenters, const double *sx, const double *sy,
-                                const double *weights, double rscale, double *tgt_expansions,
-                                int tgt_base_ibox, void *stream) {
+                                const double *weights, const double *weights_dx,
+                                const double *weights_dy, double rscale,
+                                double *tgt_expansions, int tgt_base_ibox, void *stream) {
   T2_KIND(kind, "t2d_p2l");
   int rc = check_expn(ex, "t2d_p2l");
   if (rc) return rc;
@@ -673,12 +734,14 @@ extern "C" int sumpyhip_t2d_p2l(int kind, const sumpyhip_t2d_expn *ex, double k,
   if (kind)
     t2_p2l<1><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_source_starts, box_source_counts,
-                                    centers, ncenters, sx, sy, weights, rscale, kk,
+                                    centers, ncenters, sx, sy, weights, weights_dx, weights_dy,
+                                    rscale, kk,
                                     tgt_expansions, tgt_base_ibox);
   else
     t2_p2l<0><<<ntgt, TTH, 0, st>>>(to_t2e(ex), target_boxes, source_box_starts,
                                     source_box_lists, box_source_starts, box_source_counts,
-                                    centers, ncenters, sx, sy, weights, rscale, kk,
+                                    centers, ncenters, sx, sy, weights, weights_dx, weights_dy,
+                                    rscale, kk,
                                     tgt_expansions, tgt_base_ibox);
   return check_launch("t2d_p2l");
 }

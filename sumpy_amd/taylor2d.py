@@ -10,8 +10,10 @@ with the reference's keyword contract (``sumpy/p2e.py:155-388``,
 ``sumpy/expansion/multipole.py:70-419``, ``sumpy/expansion/local.py:200-510``
 and the PDE projection of ``sumpy/expansion/__init__.py:563-886`` in 2D.
 
-Supported: potentials (no target derivatives) of charge weights (no source
-derivatives), Laplace (``log r``, real coefficients) and Helmholtz
+Supported: charges and first source derivatives (``DirectionalSourceDerivative``,
+``AxisSourceDerivative``) at the sources, potentials and ``AxisTargetDerivative``
+gradients (with multipliers, through :class:`~sumpy_amd.tools.TargetOutputs`)
+at the targets, Laplace (``log r``, real coefficients) and Helmholtz
 (``H0(k r)``, real ``k`` or complex ``k`` with ``Im k > 0``, complex
 coefficients), orders 0..12, FP64.
 """
@@ -116,11 +118,12 @@ def _rscale(expansion, r):
 
 
 def _weights(actx, op, kwargs, strengths, kind):
+    """Device pointers (charges, dipole x, dipole y; None without dipoles) and
+    the tensors that hold them."""
     w = SourceWeights(actx, op.source_kernels, op.strength_usage,
                       as_strength_list(strengths, op.strength_count), kwargs, 2, kind == 1)
-    if w.wd is not None:
-        raise NotImplementedError("2D Taylor: source derivatives are not implemented")
-    return w.w0.contiguous()
+    keep = [w.w0.contiguous()] + ([x.contiguous() for x in w.wd] if w.wd is not None else [])
+    return [t.data_ptr() for t in keep] + [None]*(3 - len(keep)), keep
 
 
 # {{{ P2M, P2L
@@ -135,14 +138,14 @@ def p2m(op, actx, kwargs):
     centers = kwargs.pop("centers").contiguous()
     tgt = _rows(kwargs.pop("tgt_expansions"), ex, "tgt_expansions")
     k = _k(ex, kwargs)
-    w = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
+    w, _keep = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
     sb = as_i32(kwargs.pop("source_boxes"))
     d = _desc(ex, actx.device)
     _capi.call("sumpyhip_t2d_p2m", kind, _ref(d), *k, int(sb.shape[0]), _capi.ptr(sb),
                _capi.ptr(as_i32(kwargs.pop("box_source_starts"))),
                _capi.ptr(as_i32(kwargs.pop("box_source_counts_nonchild"))),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(), src[1].data_ptr(),
-               w.data_ptr(), _rscale(ex, kwargs.pop("rscale")), _capi.ptr(tgt),
+               *w, _rscale(ex, kwargs.pop("rscale")), _capi.ptr(tgt),
                int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
     return tgt
 
@@ -158,7 +161,7 @@ def p2l(op, actx, kwargs):
     centers = kwargs.pop("centers").contiguous()
     tgt = _rows(kwargs.pop("tgt_expansions"), ex, "tgt_expansions")
     k = _k(ex, kwargs)
-    w = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
+    w, _keep = _weights(actx, op, kwargs, kwargs.pop("strengths"), kind)
     tb = as_i32(kwargs.pop("target_boxes"))
     d = _desc(ex, actx.device)
     _capi.call("sumpyhip_t2d_p2l", kind, _ref(d), *k, int(tb.shape[0]), _capi.ptr(tb),
@@ -167,7 +170,7 @@ def p2l(op, actx, kwargs):
                _capi.ptr(as_i32(kwargs.pop("box_source_starts"))),
                _capi.ptr(as_i32(kwargs.pop("box_source_counts_nonchild"))),
                _capi.ptr(centers), int(centers.shape[1]), src[0].data_ptr(), src[1].data_ptr(),
-               w.data_ptr(), _rscale(ex, kwargs.pop("rscale")), _capi.ptr(tgt),
+               *w, _rscale(ex, kwargs.pop("rscale")), _capi.ptr(tgt),
                int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
     return tgt
 
@@ -282,9 +285,7 @@ def _e2p(op, actx, kwargs, mpole):
         result = obj_array([torch.zeros(nt, dtype=dtype, device=actx.device)
                             for _ in op.kernels])
     outs = TargetOutputs(op.kernels, 2)
-    if outs.need_grad or not outs.direct:
-        raise NotImplementedError("2D Taylor: only potentials (no target derivatives)")
-    out = outs.buffers(list(result), nt, dtype, actx.device)[0]
+    out = outs.buffers(list(result), nt, dtype, actx.device)[:3]
     coeffs = _rows(kwargs["src_expansions"], ex, "src_expansions")
     tb = as_i32(kwargs["target_boxes"])
     scale = complex(ex.kernel.get_base_kernel().get_global_scaling_const())
@@ -293,7 +294,7 @@ def _e2p(op, actx, kwargs, mpole):
               _capi.ptr(as_i32(kwargs["box_target_counts_nonchild"])), _capi.ptr(centers),
               int(centers.shape[1]), targets[0].data_ptr(), targets[1].data_ptr(),
               _rscale(ex, kwargs["rscale"]), _capi.ptr(coeffs), int(kwargs["src_base_ibox"]),
-              scale.real, scale.imag, out, actx.stream_handle)
+              scale.real, scale.imag, *out, actx.stream_handle)
     if mpole:
         _capi.call("sumpyhip_t2d_m2p", kind, _ref(d), *_k(ex, kwargs), int(tb.shape[0]),
                    _capi.ptr(tb), _capi.ptr(as_i32(kwargs["source_box_starts"])),

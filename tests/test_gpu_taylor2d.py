@@ -235,3 +235,113 @@ def test_taylor2d_fmm_exclude_self(actx, compressed):
     e = rel(pot, direct)
     print(f"exclude self, order 10, compressed={compressed}: rel. l2 error {e:.2e}")
     assert e < 1e-7
+
+
+def _der_kernels(kind, compressed):
+    from sumpy_amd.kernel import AxisTargetDerivative, DirectionalSourceDerivative
+    knl, mcls, lcls = _classes(kind, compressed)
+    skn = [knl, DirectionalSourceDerivative(knl, "dir_vec")]
+    tkn = [knl, AxisTargetDerivative(0, knl), AxisTargetDerivative(1, knl)]
+    return knl, mcls, lcls, skn, tkn
+
+
+@pytest.mark.parametrize("kind", ["lap", "helm", "helmc"])
+@pytest.mark.parametrize("compressed", [True, False])
+def test_taylor2d_dipole_p2e_and_gradient_e2p_match_oracle(actx, kind, compressed):
+    """P2M and P2L with charge + DirectionalSourceDerivative sources and M2P /
+    L2P with potential + d/dx + d/dy targets (the kernels of the reference's
+    test_p2e2p with a source derivative), through the operators, against
+    the oracle."""
+    from sumpy_amd import E2PFromCSR, E2PFromSingleBox, P2EFromCSR, P2EFromSingleBox
+    knl, mcls, lcls, skn, tkn = _der_kernels(kind, compressed)
+    ok, kv = _okind(kind), _k(kind)
+    order = 6
+    mpole, local = mcls(knl, order), lcls(knl, order)
+    kw = {"k": kv} if kind != "lap" else {}
+    cx = kind != "lap"
+    src, _, tgt, c = _chain_geometry()
+    rng = np.random.default_rng(3)
+    q0, q1 = rng.standard_normal(15), rng.standard_normal(15)
+    n = rng.standard_normal((2, 15))
+    dip = (q1*n[0], q1*n[1])
+    d = actx.from_numpy
+    i32 = lambda a: d(np.asarray(a, np.int32))  # noqa: E731
+    wdt = np.complex128 if cx else np.float64
+    cdt = torch.complex128 if cx else torch.float64
+    centers = d(np.ascontiguousarray(c))
+    bss, bsc = i32([0, 15, 15, 15]), i32([15, 0, 0, 0])
+    bts, btc = i32([0, 0, 0, 0]), i32([0, 0, 0, tgt.shape[1]])
+    nst = len(t2.stored_ids(order, compressed, ok))
+    print(f"taylor2d dipole/gradient operators {kind} compressed={compressed}", flush=True)
+    M = torch.zeros((4, nst), dtype=cdt, device=actx.device)
+    P2EFromSingleBox(mpole, kernels=skn, strength_usage=[0, 1])(
+        actx, sources=d(src), centers=centers, strengths=[d(q0.astype(wdt)), d(q1.astype(wdt))],
+        dir_vec=d(n), tgt_expansions=M, tgt_base_ibox=0, rscale=0.5, source_boxes=i32([0]),
+        box_source_starts=bss, box_source_counts_nonchild=bsc, **kw)
+    m_ref = t2.p2m(order, compressed, ok, kv, c[:, 0], src, q0, 0.5, dipoles=dip)
+    assert rel(actx.to_numpy(M[0]), m_ref) < 1e-12
+    L = torch.zeros((4, nst), dtype=cdt, device=actx.device)
+    P2EFromCSR(local, kernels=skn, strength_usage=[0, 1])(
+        actx, sources=d(src), centers=centers, strengths=[d(q0.astype(wdt)), d(q1.astype(wdt))],
+        dir_vec=d(n), tgt_expansions=L, tgt_base_ibox=0, rscale=0.25, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([0]), box_source_starts=bss,
+        box_source_counts_nonchild=bsc, **kw)
+    l_ref = t2.p2l(order, compressed, ok, kv, c[:, 3], src, q0, 0.25, dipoles=dip)
+    assert rel(actx.to_numpy(L[3]), l_ref) < 1e-12
+    got = E2PFromSingleBox(local, tkn)(
+        actx, src_expansions=L, src_base_ibox=0, target_boxes=i32([3]), centers=centers,
+        targets=d(tgt), box_target_starts=bts, box_target_counts_nonchild=btc, rscale=0.25,
+        **kw)
+    ref = t2.l2p(order, compressed, ok, kv, c[:, 3], l_ref, tgt, 0.25, grad=True)
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+    Mb = torch.zeros_like(M)
+    Mb[1] = M[0]
+    cen = c.copy()
+    cen[:, 1] = c[:, 0]
+    got = E2PFromCSR(mpole, tkn)(
+        actx, src_expansions=Mb, src_base_ibox=0, target_boxes=i32([3]),
+        source_box_starts=i32([0, 1]), source_box_lists=i32([1]),
+        centers=d(np.ascontiguousarray(cen)), targets=d(tgt), box_target_starts=bts,
+        box_target_counts_nonchild=btc, rscale=0.5, **kw)
+    ref = t2.m2p(order, compressed, ok, kv, c[:, 0], m_ref, tgt, 0.5, grad=True)
+    for g, r in zip(got, ref):
+        assert rel(actx.to_numpy(g), r) < 1e-12
+    direct = t2.p2p(ok, kv, tgt, src, q0, dipoles=dip, grad=True)
+    for g, r in zip(got, direct):
+        assert rel(actx.to_numpy(g), r) < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["lap", "helm"])
+def test_taylor2d_fmm_dipoles_and_gradients(actx, kind):
+    """The 2D volume FMM (compressed) with charge + dipole sources and
+    potential + gradient targets against the direct sum, converging with the
+    order."""
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl, mcls, lcls, skn, tkn = _der_kernels(kind, True)
+    src = np.random.default_rng(15).standard_normal((2, 1000))
+    tgt = np.array([[0.1], [0.0]]) + np.random.default_rng(18).standard_normal((2, 300))
+    q = np.random.default_rng(44).random(1000)
+    rng = np.random.default_rng(21)
+    q1 = rng.standard_normal(1000)
+    n = rng.standard_normal((2, 1000))
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    ids = tree.host["user_source_ids"]
+    ref = t2.p2p(kind, K, tgt, src, q, dipoles=(q1*n[0], q1*n[1]), grad=True)
+    dtype = np.complex128 if kind != "lap" else np.float64
+    errs = []
+    for order in (3, 6):
+        ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                                 tkn, source_kernels=skn, strength_usage=[0, 1])
+        wr = SumpyExpansionWrangler(ti, trav, dtype, lambda *a: order,
+                                    kernel_extra_kwargs={"k": K} if kind != "lap" else None,
+                                    source_extra_kwargs={"dir_vec": actx.from_numpy(
+                                        np.ascontiguousarray(n[:, ids]))})
+        print(f"taylor2d fmm derivatives {kind} p={order}", flush=True)
+        pots = drive_fmm(actx, wr, [actx.from_numpy(q.astype(dtype)),
+                                    actx.from_numpy(q1.astype(dtype))])
+        errs.append(max(rel(actx.to_numpy(p), r) for p, r in zip(pots, ref)))
+    print(f"{kind}: dipole/gradient FMM errors {errs}")
+    assert errs[1] < 0.1*errs[0] and errs[0] < 1e-2

@@ -127,3 +127,44 @@ def test_chains_converge_complex_k(compressed):
         e = [x[name] for x in errs]
         assert all(b < 0.2*a for a, b in zip(e, e[1:])), (name, e)
         assert e[-1] < 1e-5, (name, e)
+
+
+@pytest.mark.parametrize(("kind", "k"), [("lap", 0.0), ("helm", 1.3), ("helm", KC)])
+@pytest.mark.parametrize("compressed", [True, False])
+def test_dipoles_and_gradients_are_derivatives(kind, k, compressed):
+    """The dipole (n . grad_source) forms of P2P, P2M and P2L and the
+    target-gradient forms of P2P, M2P and L2P equal central differences
+    (h = 1e-5) of the charge / potential forms."""
+    rng = np.random.default_rng(6)
+    src = 0.3*rng.random((2, 5))
+    tgt = np.array([[3.1, 2.8, 3.4], [0.2, -0.3, 0.5]])
+    q = rng.standard_normal(5) + 0j
+    n = rng.standard_normal((2, 5))
+    dip = (q*n[0], q*n[1])
+    zero = np.zeros(5, complex)
+    h = 1e-5
+    c0, c1 = np.array([0.15, 0.15]), np.array([3.1, 0.1])
+
+    def fd_src(f):
+        return (f(src + h*n) - f(src - h*n))/(2*h)
+
+    pot, gx, gy = t2.p2p(kind, k, tgt, src, zero, dipoles=dip, grad=True)
+    assert np.allclose(pot, fd_src(lambda s: t2.p2p(kind, k, tgt, s, q)), rtol=1e-7, atol=1e-9)
+    for ax, g in enumerate((gx, gy)):
+        e = np.eye(2)[:, ax:ax + 1]
+        ref = (t2.p2p(kind, k, tgt + h*e, src, zero, dipoles=dip)
+               - t2.p2p(kind, k, tgt - h*e, src, zero, dipoles=dip))/(2*h)
+        assert np.allclose(g, ref, rtol=1e-6, atol=1e-8), ax
+    for fn, c in ((t2.p2m, c0), (t2.p2l, c1)):
+        got = fn(4, compressed, kind, k, c, src, zero, 0.5, dipoles=dip)
+        ref = fd_src(lambda s: fn(4, compressed, kind, k, c, s, q, 0.5))
+        assert np.allclose(got, ref, rtol=1e-7, atol=1e-9), fn.__name__
+    coeffs = rng.standard_normal(len(t2.stored_ids(4, compressed, kind))) + 0j
+    for fn, c in ((t2.m2p, c0), (t2.l2p, c1)):
+        pot, gx, gy = fn(4, compressed, kind, k, c, coeffs, tgt, 0.5, grad=True)
+        assert np.allclose(pot, fn(4, compressed, kind, k, c, coeffs, tgt, 0.5), rtol=1e-14)
+        for ax, g in enumerate((gx, gy)):
+            e = np.eye(2)[:, ax:ax + 1]
+            ref = (fn(4, compressed, kind, k, c, coeffs, tgt + h*e, 0.5)
+                   - fn(4, compressed, kind, k, c, coeffs, tgt - h*e, 0.5))/(2*h)
+            assert np.allclose(g, ref, rtol=1e-6, atol=1e-8), (fn.__name__, ax)
