@@ -57,8 +57,23 @@ class P2EBase(KernelComputation):
         return sources, centers.contiguous(), rscale, weights, tgt, tables
 
 
+def _ensure_tgt_expansions(op, actx, kwargs):
+    """The reference's optional output: without ``tgt_expansions`` a zero
+    array of ``nboxes`` rows is made and returned (``sumpy/p2e.py:242-262``)."""
+    if kwargs.get("tgt_expansions") is not None:
+        return kwargs
+    if "nboxes" not in kwargs:
+        raise ValueError("either tgt_expansions or nboxes is needed")
+    kwargs = dict(kwargs)
+    dt = torch.complex128 if op.expansion.is_complex_valued else torch.float64
+    kwargs["tgt_expansions"] = torch.zeros((int(kwargs["nboxes"]), len(op.expansion)),
+                                           dtype=dt, device=actx.device)
+    return kwargs
+
+
 class P2EFromSingleBox(P2EBase):
-    """P2M (reference ``p2e.py:155-261``).
+    """P2M (reference ``p2e.py:155-261``); with a local expansion, the local
+    expansion of each box from its own sources.
 
     Besides the reference's keywords it takes ``box_levels`` and
     ``level_rscales``: the source boxes may then be of any level (one launch
@@ -70,6 +85,18 @@ class P2EFromSingleBox(P2EBase):
     default_name = "p2e_from_single_box"
 
     def __call__(self, actx, **kwargs):
+        kwargs = _ensure_tgt_expansions(self, actx, kwargs)
+        kwargs.pop("nboxes", None)
+        if isinstance(self.expansion, LocalExpansionBase):
+            # a local expansion of each box from its own sources: P2L with
+            # the box itself as its only list entry
+            sb = as_i32(kwargs.pop("source_boxes"))
+            kwargs["target_boxes"] = sb
+            kwargs["source_box_lists"] = sb
+            kwargs["source_box_starts"] = torch.arange(int(sb.shape[0]) + 1, dtype=torch.int32,
+                                                       device=sb.device)
+            return P2EFromCSR(self.expansion, self.source_kernels,
+                              strength_usage=self.strength_usage)(actx, **kwargs)
         if h2d.is_h2d(self.expansion):
             return h2d.p2m(self, actx, kwargs)
         if taylor2d.is_taylor2d(self.expansion):
@@ -136,6 +163,8 @@ class P2EFromCSR(P2EBase):
     def __call__(self, actx, **kwargs):
         if not isinstance(self.expansion, LocalExpansionBase):
             raise TypeError("P2EFromCSR forms local expansions")
+        kwargs = _ensure_tgt_expansions(self, actx, kwargs)
+        kwargs.pop("nboxes", None)
         if h2d.is_h2d(self.expansion):
             return h2d.p2l(self, actx, kwargs)
         if taylor2d.is_taylor2d(self.expansion):

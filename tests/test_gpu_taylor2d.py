@@ -345,3 +345,38 @@ def test_taylor2d_fmm_dipoles_and_gradients(actx, kind):
         errs.append(max(rel(actx.to_numpy(p), r) for p, r in zip(pots, ref)))
     print(f"{kind}: dipole/gradient FMM errors {errs}")
     assert errs[1] < 0.1*errs[0] and errs[0] < 1e-2
+
+
+@pytest.mark.parametrize("local", [True, False])
+def test_p2e_multiple_source_kernels_is_linear(actx, local):
+    """The reference's ``test_p2e_multiple`` (``sumpy/test/test_kernels.py:152-260``):
+    2D Laplace, PDE-compressed order 4, 100 sources, P2EFromSingleBox with
+    [DirectionalSourceDerivative, charge] and strength_usage [0, 1] in one
+    call (tgt_expansions made from nboxes) equals the sum of the one-kernel
+    calls to 1e-12."""
+    from sumpy_amd import P2EFromSingleBox
+    from sumpy_amd.kernel import DirectionalSourceDerivative
+    knl, mcls, lcls = _classes("lap", True)
+    expn = (lcls if local else mcls)(knl, 4)
+    skn = [DirectionalSourceDerivative(knl, "dir_vec"), knl]
+    n = 100
+    rng = np.random.default_rng(14)
+    center = np.array([2.0, 1.0])
+    d = actx.from_numpy
+    src = d(0.7*(-0.5 + rng.random((2, n))) + center[:, None])
+    strengths = [d(np.full(n, 1/n)), d(np.full(n, 2/n))]
+    alpha = np.linspace(0, 2*np.pi, n)
+    dir_vec = d(np.vstack([np.cos(alpha), np.sin(alpha)]))
+    c = (np.array([5.5, 0.0]) + center) if local else center
+    kw = dict(source_boxes=d(np.array([0], np.int32)),
+              box_source_starts=d(np.array([0], np.int32)),
+              box_source_counts_nonchild=d(np.array([n], np.int32)),
+              centers=d(c.reshape(2, 1)), sources=src, strengths=strengths, nboxes=1,
+              tgt_base_ibox=0, rscale=0.5)
+    actual = actx.to_numpy(P2EFromSingleBox(expn, kernels=skn, strength_usage=[0, 1])(
+        actx, dir_vec=dir_vec, **kw))
+    expected = sum(
+        actx.to_numpy(P2EFromSingleBox(expn, kernels=[sk], strength_usage=[i])(
+            actx, **kw, **({"dir_vec": dir_vec} if i == 0 else {})))
+        for i, sk in enumerate(skn))
+    assert np.linalg.norm(actual - expected)/np.linalg.norm(expected) < 1e-12
