@@ -53,7 +53,8 @@ class E2PBase:
 
 
 class E2PFromSingleBox(E2PBase):
-    """L2P: ``result[t] = scale * sum_mi L_full[mi] (b/rho)^mi/mi!``.
+    """L2P: ``result[t] = scale * sum_mi L_full[mi] (b/rho)^mi/mi!``; a
+    multipole expansion is evaluated at its own box's targets (M2P).
 
     Besides the reference's keywords it takes ``box_levels`` and
     ``level_rscales``: the target boxes may then be of any level (one launch
@@ -64,6 +65,9 @@ class E2PFromSingleBox(E2PBase):
     default_name = "e2p_from_single_box"
 
     def __call__(self, actx, **kwargs):
+        from sumpy_amd.expansion.local import LocalExpansionBase
+        if not isinstance(self.expansion, LocalExpansionBase):
+            return self._multipole(actx, kwargs)
         if h2d.is_h2d(self.expansion):
             return h2d.l2p(self, actx, kwargs)
         if taylor2d.is_taylor2d(self.expansion):
@@ -122,6 +126,29 @@ class E2PFromSingleBox(E2PBase):
         # of the boxes' targets; plain kernels are written in place
         outs.finish(actx, targets, accumulate=not outs.direct)
         return result
+
+
+    def _multipole(self, actx, kwargs):
+        """A multipole expansion evaluated at its own box's targets (the
+        reference's E2PFromSingleBox evaluates any expansion): the M2P with
+        the box as its only list entry, after clearing those targets (this
+        operator overwrites)."""
+        kwargs = dict(kwargs)
+        tb = as_i32(kwargs["target_boxes"])
+        bts = as_i32(kwargs["box_target_starts"])[tb.long()].long()
+        btc = as_i32(kwargs["box_target_counts_nonchild"])[tb.long()].long()
+        result = kwargs.get("result")
+        if result is not None:
+            first = torch.repeat_interleave(bts, btc)
+            offs = torch.arange(int(btc.sum()), device=tb.device) - torch.repeat_interleave(
+                torch.cumsum(btc, 0) - btc, btc)
+            idx = first + offs
+            for r in result:
+                r[idx] = 0
+        kwargs["source_box_starts"] = torch.arange(int(tb.shape[0]) + 1, dtype=torch.int32,
+                                                   device=tb.device)
+        kwargs["source_box_lists"] = tb
+        return E2PFromCSR(self.expansion, self.kernels)(actx, **kwargs)
 
 
 class E2PFromCSR(E2PBase):
