@@ -380,3 +380,63 @@ def test_p2e_multiple_source_kernels_is_linear(actx, local):
             actx, **kw, **({"dir_vec": dir_vec} if i == 0 else {})))
         for i, sk in enumerate(skn))
     assert np.linalg.norm(actual - expected)/np.linalg.norm(expected) < 1e-12
+
+
+def _fmm_2d_laplace(actx, tree, trav, q, order, target_kernels, source_kernels):
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import VolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import VolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import LaplaceKernel
+    knl = LaplaceKernel(2)
+    ti = SumpyTreeIndependentDataForWrangler(
+        actx, partial(VolumeTaylorMultipoleExpansion, knl),
+        partial(VolumeTaylorLocalExpansion, knl), target_kernels=target_kernels,
+        source_kernels=source_kernels, exclude_self=True)
+    wr = SumpyExpansionWrangler(
+        ti, trav, np.float64, lambda *a: order,
+        self_extra_kwargs={"target_to_source": actx.from_numpy(
+            np.arange(tree.ntargets, dtype=np.int32))})
+    return [actx.to_numpy(p) for p in drive_fmm(actx, wr, [actx.from_numpy(q)])]
+
+
+def test_fmm_axis_source_derivative_2d(actx):
+    """The reference's ``test_sumpy_axis_source_derivative``
+    (``sumpy/test/test_fmm.py:605-665``): 2D Laplace, full Taylor order 10,
+    500 normal sources = targets (self excluded), 30 per box; the FMM of
+    AxisTargetDerivative(0) with charges equals minus the FMM of
+    AxisSourceDerivative(0) (relative l2 < 1e-5)."""
+    from sumpy_amd.kernel import AxisSourceDerivative, AxisTargetDerivative, LaplaceKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = LaplaceKernel(2)
+    src = np.random.default_rng(15).standard_normal((2, 500))
+    q = np.random.default_rng(12).random(500)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    p0, = _fmm_2d_laplace(actx, tree, trav, q, 10, [AxisTargetDerivative(0, knl)], [knl])
+    p1, = _fmm_2d_laplace(actx, tree, trav, q, 10, [knl], [AxisSourceDerivative(0, knl)])
+    e = np.linalg.norm(p0 + p1)/np.linalg.norm(p0)
+    print(f"axis source derivative 2D: {e:.2e}")
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("deriv_axes", [(), (0,), (1,)])
+def test_fmm_target_point_multiplier_2d(actx, deriv_axes):
+    """The reference's ``test_sumpy_target_point_multiplier``
+    (``sumpy/test/test_fmm.py:672-741``): 2D Laplace, full Taylor order 5;
+    [x0 G, G, G] with the derivatives of deriv_axes applied to the first two:
+    x0 pot (+ pot for d/dx0) equals the first (relative l2 < 1e-5)."""
+    from sumpy_amd.kernel import AxisTargetDerivative, LaplaceKernel, TargetPointMultiplier
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl = LaplaceKernel(2)
+    src = np.random.default_rng(15).standard_normal((2, 500))
+    q = np.random.default_rng(12).random(500)
+    tree, trav = build_tree_and_traversal(actx, src, None, 30)
+    tk = [TargetPointMultiplier(0, knl), knl, knl]
+    for axis in deriv_axes:
+        tk[0] = AxisTargetDerivative(axis, tk[0])
+        tk[1] = AxisTargetDerivative(axis, tk[1])
+    p0, p1, p2 = _fmm_2d_laplace(actx, tree, trav, q, 5, tk, [knl])
+    ref = p1*src[0] + (p2 if deriv_axes == (0,) else 0)
+    e = np.linalg.norm(p0 - ref)/np.linalg.norm(ref)
+    print(f"target point multiplier 2D {deriv_axes}: {e:.2e}")
+    assert e < 1e-5
