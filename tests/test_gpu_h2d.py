@@ -460,3 +460,36 @@ def test_h2d_fmm_dipoles_and_gradients(actx, kind):
         errs.append(max(rel(actx.to_numpy(p), r) for p, r in zip(pots, ref)))
     print(f"{kind}: dipole/gradient FMM errors {errs}")
     assert errs[1] < 0.2*errs[0] and errs[0] < 1e-3
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_h2d_fmm_order_varies_with_level(actx, use_classes):
+    """The reference's H2D case with ``order_varies_with_level``
+    (``sumpy/test/test_fmm.py:100-101,268-270``: order + lev % 2): M2M, L2L
+    and M2L between different orders; converging to the direct sum and
+    between the uniform-order results."""
+    from functools import partial
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.tree import build_tree_and_traversal
+    knl, mcls, lcls, kname, _ = _kernel("helmholtz")
+    src, tgt, q = _fmm_case()
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 30)
+    direct = oh.p2p(0.05, tgt, src, q)
+    errs = []
+    for order in (4, 5):
+        ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                                 [knl])
+        wr = SumpyExpansionWrangler(ti, trav, np.complex128,
+                                    lambda kernel, args, tree, lev, o=order: o + lev % 2,
+                                    kernel_extra_kwargs={kname: 0.05},
+                                    _disable_translation_classes=not use_classes)
+        print(f"h2d varying orders {order} + lev % 2, classes={use_classes}", flush=True)
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q.astype(np.complex128))])
+        errs.append(float(np.max(np.abs(actx.to_numpy(pot) - direct))/np.max(np.abs(direct))))
+    uni = [float(np.max(np.abs(_run_fmm(actx, tree, trav, q, o, 0.05, use_classes) - direct))
+                 / np.max(np.abs(direct))) for o in (4, 6)]
+    print(f"varying-order errors {errs}, uniform orders 4 and 6 {uni}")
+    assert errs[1] < errs[0] < 1e-3
+    assert uni[1] <= errs[0] <= uni[0] and uni[1] <= errs[1] <= uni[0]
