@@ -264,13 +264,14 @@ int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes
 
 /* }}} */
 
-/* {{{ Helmholtz-3D (G = e^{ik|d|}/|d|, sumpy/kernel.py:754-844), real k */
+/* {{{ Helmholtz-3D (G = e^{ik|d|}/|d|, sumpy/kernel.py:754-844), k = k + i k_im
+ * (k_im > 0: damped waves, full-Taylor expansions on the Python side) */
 
 /* M2P from list 3 for the Helmholtz kernel, replaces E2PFromCSR.__call__
  * (sumpy/e2p.py:233-343) for HelmholtzKernel(3); accumulates.  Multipoles,
  * pot and the gradient are complex, interleaved (re, im) doubles; expn holds
  * the stored identifiers of the PDE-compressed expansion (at most 256). */
-int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, double k_im, int ntgt_boxes,
                            const int *target_boxes, const int *source_box_starts,
                            const int *source_box_lists, const int *box_target_starts,
                            const int *box_target_counts_nonchild,
@@ -282,7 +283,7 @@ int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
 /* P2L from list 4 for the Helmholtz kernel, replaces P2EFromCSR.__call__
  * (sumpy/p2e.py:268-388) for HelmholtzKernel(3); overwrites the rows of the
  * target boxes.  Weights and expansions are complex, interleaved. */
-int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, double k_im, int ntgt_boxes,
                            const int *target_boxes, const int *source_box_starts,
                            const int *source_box_lists, const int *box_source_starts,
                            const int *box_source_counts_nonchild,
@@ -300,18 +301,19 @@ int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
 int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt, const int *circ_shape,
                                       int nclasses, const double *translation_vectors,
                                       int nvectors, int class_start, double src_rscale,
-                                      double k, double *out, void *stream);
+                                      double k, double k_im, double *out, void *stream);
 
 /* M2L without translation classes for the Helmholtz kernel, replaces
  * E2EFromCSR.__call__ (sumpy/e2e.py:164-304) for HelmholtzKernel(3): the
- * arguments of sumpyhip_m2l_csr plus the real wavenumber k; complex
+ * arguments of sumpyhip_m2l_csr plus the wavenumber k + i k_im; complex
  * expansions, interleaved; overwrites the target rows. */
 int sumpyhip_m2l_csr_helmholtz(int order_src, int order_tgt, const int *circ_shape, int nl,
                                int nm, const int *off_t, const int *off_s,
                                const int *tgt_degree, int ntgt_boxes, const int *target_boxes,
                                const int *src_box_starts, const int *src_box_lists,
                                const double *box_centers, int nboxes, double src_rscale,
-                               double tgt_rscale, double k, const void *src_expansions,
+                               double tgt_rscale, double k, double k_im,
+                               const void *src_expansions,
                                int src_base_ibox, void *tgt_expansions, int tgt_base_ibox,
                                void *stream);
 

@@ -100,13 +100,13 @@ _SIGNATURES = {
                             _D, _P, _P, _P, _P, _P, _I, _P],
     "sumpyhip_p2l_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P,
                             _P, _P, _I, _P, _P, _I, _P],
-    "sumpyhip_m2p_helmholtz": [_P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D,
+    "sumpyhip_m2p_helmholtz": [_P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D,
                                _D, _P, _I, _P, _P, _P, _P, _P],
-    "sumpyhip_p2l_helmholtz": [_P, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
+    "sumpyhip_p2l_helmholtz": [_P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
                                _P, _P, _P, _D, _P, _I, _P],
     "sumpyhip_m2l_csr_helmholtz": [_I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _I, _D,
-                                   _D, _D, _P, _I, _P, _I, _P],
-    "sumpyhip_m2l_class_data_helmholtz": [_I, _I, _P, _I, _P, _I, _I, _D, _D, _P, _P],
+                                   _D, _D, _D, _P, _I, _P, _I, _P],
+    "sumpyhip_m2l_class_data_helmholtz": [_I, _I, _P, _I, _P, _I, _I, _D, _D, _D, _P, _P],
     "sumpyhip_octant_translate": [_I, _P, _P, _P, _I, _P, _I, _I, _P, _P, _I, _P, _I, _P,
                                   _P],
     "sumpyhip_qbx2d_adjoint": [_I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _D, _P,

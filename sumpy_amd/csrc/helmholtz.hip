@@ -44,9 +44,40 @@ __device__ void herm_table(double *tab, int nmax) {
   __syncthreads();
 }
 
-// g_0..g_n of e^{i kappa r}/r at r into (gre, gim)[q*stride]
-__device__ void helm_radial(double kappa, double r, int n, double *gre, double *gim,
-                            int stride) {
+// g_0..g_n of e^{i kappa r}/r at r into (gre, gim)[q*stride]; kappa =
+// kappa + i kappa_im (damped waves, kappa_im > 0: the same recurrence in
+// complex arithmetic)
+__device__ void helm_radial_c(double kappa, double kappa_im, double r, int n, double *gre,
+                              double *gim, int stride) {
+  const double ri = 1.0/r, ri2 = ri*ri;
+  double sn, cs;
+  sincos(kappa*r, &sn, &cs);
+  const double damp = exp(-kappa_im*r);
+  double ar = damp*cs*ri, ai = damp*sn*ri;
+  gre[0] = ar;
+  gim[0] = ai;
+  if (n < 1) return;
+  // e^{i kappa r}(i kappa r - 1)/r^3, i kappa r - 1 = (-kappa_im r - 1) + i kappa r
+  const double ur = -kappa_im*r - 1.0, ui = kappa*r;
+  double br = (ar*ur - ai*ui)*ri2, bi = (ar*ui + ai*ur)*ri2;
+  gre[stride] = br;
+  gim[stride] = bi;
+  const double k2r = kappa*kappa - kappa_im*kappa_im, k2i = 2.0*kappa*kappa_im;
+  for (int q = 2; q <= n; ++q) {
+    const double f = -(2*q - 1);
+    const double cr = (f*br - (k2r*ar - k2i*ai))*ri2, ci = (f*bi - (k2r*ai + k2i*ar))*ri2;
+    gre[q*stride] = cr;
+    gim[q*stride] = ci;
+    ar = br; ai = bi; br = cr; bi = ci;
+  }
+}
+
+__device__ void helm_radial(double kappa, double kappa_im, double r, int n, double *gre,
+                            double *gim, int stride) {
+  if (kappa_im != 0.0) {
+    helm_radial_c(kappa, kappa_im, r, n, gre, gim, stride);
+    return;
+  }
   const double ri = 1.0/r, ri2 = ri*ri;
   double sn, cs;
   sincos(kappa*r, &sn, &cs);
@@ -103,7 +134,7 @@ __device__ __forceinline__ cplx helm_deriv(int a, int b, int c, const double *xp
 // the row-major box (M0, M1, M2); zero elsewhere
 __global__ __launch_bounds__(256) void helm_class_data_kernel(
     int N, int M0, int M1, int M2, int nclasses, const double *vecs, int nvectors,
-    int class_start, double rscale, double k, double *out) {
+    int class_start, double rscale, double k, double kim, double *out) {
   __shared__ double ht[HN*HN];
   __shared__ double sp[3][HN], sg[2][HN];
   herm_table(ht, N);
@@ -123,7 +154,7 @@ __global__ __launch_bounds__(256) void helm_class_data_kernel(
           pw *= xi;
         }
       } else {
-        helm_radial(k*rscale, sqrt(x*x + y*y + z*z), N, sg[0], sg[1], 1);
+        helm_radial(k*rscale, kim*rscale, sqrt(x*x + y*y + z*z), N, sg[0], sg[1], 1);
       }
     }
     __syncthreads();
@@ -154,7 +185,7 @@ __global__ __launch_bounds__(256) void helm_class_data_kernel(
 // (reference sumpy/e2p.py:233-343, accumulates).
 template <bool GRAD>
 __global__ __launch_bounds__(64) void m2p_helm3d(
-    sumpyhip_expn e, double k, int ntgt, const int *target_boxes, const int *starts,
+    sumpyhip_expn e, double k, double kim, int ntgt, const int *target_boxes, const int *starts,
     const int *lists, const int *bts, const int *btc, const double *centers, int nboxes,
     const double *tx, const double *ty, const double *tz, double rscale, double scale,
     const double *src, int sbase, double *pot, double *gx, double *gy, double *gz) {
@@ -191,7 +222,7 @@ __global__ __launch_bounds__(64) void m2p_helm3d(
           xp[u*64] = px; yp[u*64] = py; zp[u*64] = pz;
           px *= x; py *= y; pz *= z;
         }
-        helm_radial(kappa, sqrt(x*x + y*y + z*z), NQ, gre, gim, 64);
+        helm_radial(kappa, kim*rscale, sqrt(x*x + y*y + z*z), NQ, gre, gim, 64);
         __syncthreads();
         for (int s = 0; s < nst; ++s) {
           const int a = sid[s], b = sid[nst + s], c = sid[2*nst + s];
@@ -382,7 +413,7 @@ constexpr int P2LH_CH = 64;  // sources staged per chunk
 // (reference sumpy/p2e.py:268-388; overwrites the box's row).
 template <bool DIP>
 __global__ __launch_bounds__(256) void p2l_helm3d(
-    sumpyhip_expn e, double k, int ntgt, const int *target_boxes, const int *starts,
+    sumpyhip_expn e, double k, double kim, int ntgt, const int *target_boxes, const int *starts,
     const int *lists, const int *bss, const int *bsc, const double *centers, int nboxes,
     const double *sx, const double *sy, const double *sz, const double *w0,
     const double *wx, const double *wy, const double *wz, double rscale, double *tgt,
@@ -423,7 +454,8 @@ __global__ __launch_bounds__(256) void p2l_helm3d(
             tab[(2*(NQ + 1) + u)*P2LH_CH + tid] = pz;
             px *= x; py *= y; pz *= z;
           }
-          helm_radial(kappa, sqrt(x*x + y*y + z*z), NQ, tab + 3*(NQ + 1)*P2LH_CH + tid,
+          helm_radial(kappa, kim*rscale, sqrt(x*x + y*y + z*z), NQ,
+                      tab + 3*(NQ + 1)*P2LH_CH + tid,
                       tab + 4*(NQ + 1)*P2LH_CH + tid, P2LH_CH);
           sw[tid] = w0[2*(size_t)s];
           sw[P2LH_CH + tid] = w0[2*(size_t)s + 1];
@@ -618,7 +650,7 @@ __global__ __launch_bounds__(256) void m2l_csr_helm3d(
     int N, int M0, int M1, int M2, int nl, int nm, const int *off_t, const int *off_s,
     const int *deg, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const double *centers, int nboxes, double src_rscale, double tgt_rscale, double k,
-    const double2 *src, int sbase, double2 *tgt, int tbase) {
+    double kim, const double2 *src, int sbase, double2 *tgt, int tbase) {
   extern __shared__ double lds[];
   const int ncirc = M0*M1*M2, HS = N + 1;
   double *Dre = lds, *Dim = Dre + ncirc;
@@ -647,7 +679,7 @@ __global__ __launch_bounds__(256) void m2l_csr_helm3d(
             pw *= xi;
           }
         } else {
-          helm_radial(k*src_rscale, sqrt(x*x + y*y + z*z), N, sg, sg + HS, 1);
+          helm_radial(k*src_rscale, kim*src_rscale, sqrt(x*x + y*y + z*z), N, sg, sg + HS, 1);
         }
       }
       __syncthreads();
@@ -708,8 +740,8 @@ extern "C" int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt,
                                                  const int *circ_shape, int nclasses,
                                                  const double *translation_vectors,
                                                  int nvectors, int class_start,
-                                                 double src_rscale, double k, double *out,
-                                                 void *stream) {
+                                                 double src_rscale, double k, double k_im,
+                                                 double *out, void *stream) {
   SUMPY_REQUIRE(circ_shape != nullptr, "m2l_class_data_helmholtz: null shape");
   const int N = order_src + order_tgt;
   SUMPY_REQUIRE(N >= 0 && N <= HELM_MAXN, "m2l_class_data_helmholtz: order %d > %d", N,
@@ -721,11 +753,12 @@ extern "C" int sumpyhip_m2l_class_data_helmholtz(int order_src, int order_tgt,
   const int grid = nclasses < 1024 ? nclasses : 1024;
   helm_class_data_kernel<<<grid, 256, 0, (hipStream_t)stream>>>(
       N, circ_shape[0], circ_shape[1], circ_shape[2], nclasses, translation_vectors,
-      nvectors, class_start, src_rscale, k, out);
+      nvectors, class_start, src_rscale, k, k_im, out);
   return check_launch("m2l_class_data_helmholtz");
 }
 
-extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, double k_im,
+                                      int ntgt_boxes,
                                       const int *target_boxes, const int *source_box_starts,
                                       const int *source_box_lists,
                                       const int *box_target_starts,
@@ -748,7 +781,7 @@ extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int n
   const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
   hipStream_t st = (hipStream_t)stream;
   const double *src = static_cast<const double *>(src_expansions);
-  if (!gx && pot && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
+  if (k_im == 0.0 && !gx && pot && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
     // potentials of the usual orders: the polynomial form
 #define SUMPY_M2P_POLY(O)                                                                  \
   case O:                                                                                  \
@@ -768,19 +801,20 @@ extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, int n
   }
   if (gx)
     m2p_helm3d<true><<<grid, 64, lds, st>>>(
-        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        *expn, k, k_im, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
         box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
         rscale, scale, src, src_base_ibox, (double *)pot, (double *)gx, (double *)gy,
         (double *)gz);
   else
     m2p_helm3d<false><<<grid, 64, lds, st>>>(
-        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        *expn, k, k_im, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
         box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,
         rscale, scale, src, src_base_ibox, (double *)pot, nullptr, nullptr, nullptr);
   return check_launch("m2p_helmholtz");
 }
 
-extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int ntgt_boxes,
+extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, double k_im,
+                                      int ntgt_boxes,
                                       const int *target_boxes, const int *source_box_starts,
                                       const int *source_box_lists,
                                       const int *box_source_starts,
@@ -802,7 +836,7 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int n
   const int grid = ntgt_boxes < 65536 ? ntgt_boxes : 65536;
   hipStream_t st = (hipStream_t)stream;
   const double *c0 = static_cast<const double *>(w0);
-  if (!wx && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
+  if (k_im == 0.0 && !wx && expn->nstored == (expn->order + 1)*(expn->order + 1)) {
     // charges at the usual orders: the moment form
 #define SUMPY_P2L_MOM(O)                                                                   \
   case O:                                                                                  \
@@ -822,13 +856,13 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, int n
   }
   if (wx)
     p2l_helm3d<true><<<grid, nt, lds, st>>>(
-        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        *expn, k, k_im, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
         box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, c0,
         (const double *)wx, (const double *)wy, (const double *)wz, rscale,
         (double *)tgt_expansions, tgt_base_ibox);
   else
     p2l_helm3d<false><<<grid, nt, lds, st>>>(
-        *expn, k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
+        *expn, k, k_im, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,
         box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz, c0,
         nullptr, nullptr, nullptr, rscale, (double *)tgt_expansions, tgt_base_ibox);
   return check_launch("p2l_helmholtz");
@@ -840,7 +874,7 @@ extern "C" int sumpyhip_m2l_csr_helmholtz(int order_src, int order_tgt, const in
                                           const int *target_boxes, const int *src_box_starts,
                                           const int *src_box_lists, const double *box_centers,
                                           int nboxes, double src_rscale, double tgt_rscale,
-                                          double k, const void *src_expansions,
+                                          double k, double k_im, const void *src_expansions,
                                           int src_base_ibox, void *tgt_expansions,
                                           int tgt_base_ibox, void *stream) {
   SUMPY_REQUIRE(circ_shape != nullptr, "m2l_csr_helmholtz: null shape");
@@ -857,7 +891,8 @@ extern "C" int sumpyhip_m2l_csr_helmholtz(int order_src, int order_tgt, const in
   m2l_csr_helm3d<<<grid, 256, lds, (hipStream_t)stream>>>(
       N, circ_shape[0], circ_shape[1], circ_shape[2], nl, nm, off_t, off_s, tgt_degree,
       ntgt_boxes, target_boxes, src_box_starts, src_box_lists, box_centers, nboxes,
-      src_rscale, tgt_rscale, k, static_cast<const double2 *>(src_expansions), src_base_ibox,
+      src_rscale, tgt_rscale, k, k_im, static_cast<const double2 *>(src_expansions),
+      src_base_ibox,
       static_cast<double2 *>(tgt_expansions), tgt_base_ibox);
   return check_launch("m2l_csr_helmholtz");
 }

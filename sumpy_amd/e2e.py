@@ -22,7 +22,7 @@ from sumpy_amd import _capi, h2d, taylor2d
 from sumpy_amd.expansion import m2l_device_tables
 from sumpy_amd.kernel import kernel_id, remove_source_transforms, remove_target_transforms
 from sumpy_amd.profiling import kernel_span
-from sumpy_amd.tools import as_i32, expansion_k, planes, set_planes
+from sumpy_amd.tools import as_i32, expansion_k, helmholtz_k, planes, set_planes
 
 TILE = 64  # target boxes per M2L tile by default (the HIP kernel's widest tile)
 TILE_WIDTHS = (64, 32)  # the tile widths the contraction kernels are built for
@@ -111,7 +111,7 @@ class _OctantTranslation(E2EBase):
             return (taylor2d.l2l if self.span_name == "l2l" else taylor2d.m2m)(
                 self, actx, kwargs)
         src, tgt = kwargs["src_expansions"], kwargs["tgt_expansions"]
-        k = expansion_k(self.tgt_expansion.kernel, kwargs)
+        k = expansion_k(self.tgt_expansion.kernel, kwargs, self.tgt_expansion)
         if not tgt.is_complex():
             return self._apply(actx, kwargs, src, tgt, k)
         # complex (Helmholtz) expansions: the translation is real-linear with
@@ -466,7 +466,7 @@ class E2EFromCSR(E2EBase):
                        _capi.ptr(m["off_t"]), _capi.ptr(m["off_s"]), _capi.ptr(m["tgt_degree"]),
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                        _capi.ptr(centers), int(centers.shape[1]), float(kwargs["src_rscale"]),
-                       float(kwargs["tgt_rscale"]), expansion_k(self.tgt_expansion.kernel, kwargs),
+                       float(kwargs["tgt_rscale"]), *helmholtz_k(self.tgt_expansion.kernel, kwargs),
                        _capi.ptr(src), int(kwargs["src_base_ibox"]), _capi.ptr(tgt),
                        int(kwargs["tgt_base_ibox"]), actx.stream_handle)
             return tgt
@@ -506,7 +506,7 @@ class M2LGenerateTranslationClassesDependentData(E2EBase):
                        self.tgt_expansion.order, shape.ctypes.data, ncls, _capi.ptr(vecs),
                        int(vecs.shape[1]), int(kwargs["translation_classes_level_start"]),
                        float(kwargs["src_rscale"]),
-                       expansion_k(self.tgt_expansion.kernel, kwargs), _capi.ptr(out),
+                       *helmholtz_k(self.tgt_expansion.kernel, kwargs), _capi.ptr(out),
                        actx.stream_handle)
             return out
         if out.shape != (ncls, m["ncirc"]):

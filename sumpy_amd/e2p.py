@@ -15,6 +15,7 @@ from sumpy_amd.tools import (
     as_component_list,
     as_i32,
     expansion_k,
+    helmholtz_k,
     planes,
     set_planes,
 )
@@ -76,7 +77,7 @@ class E2PFromSingleBox(E2PBase):
         nt = targets[0].shape[0]
         ptrs = outs.buffers(list(result), nt, result[0].dtype, actx.device)
         rscale = float(kwargs["rscale"] if "rscale" in kwargs else kwargs["level_rscales"][0])
-        k = expansion_k(self.expansion.kernel, kwargs)
+        k = expansion_k(self.expansion.kernel, kwargs, self.expansion)
         tables = self.expansion.tables_for(actx.device, k, rscale)
         tb = as_i32(kwargs["target_boxes"])
         bts = as_i32(kwargs["box_target_starts"])
@@ -180,7 +181,7 @@ class E2PFromCSR(E2PBase):
         btc = as_i32(kwargs["box_target_counts_nonchild"])
         if self.expansion.is_complex_valued:
             _capi.call("sumpyhip_m2p_helmholtz", tables.desc,
-                       expansion_k(self.expansion.kernel, kwargs),
+                       *helmholtz_k(self.expansion.kernel, kwargs),
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                        _capi.ptr(bts), _capi.ptr(btc), _capi.ptr(centers),
                        int(centers.shape[1]), *[t.data_ptr() for t in targets],

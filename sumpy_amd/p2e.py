@@ -18,6 +18,7 @@ from sumpy_amd.tools import (
     as_i32,
     as_strength_list,
     expansion_k,
+    helmholtz_k,
     planes,
     set_planes,
 )
@@ -45,7 +46,8 @@ class P2EBase(KernelComputation):
         centers = kwargs.pop("centers")
         _capi.require_gpu(centers)
         rscale = float(kwargs.pop("rscale"))
-        self._k = expansion_k(self.expansion.kernel, kwargs)
+        self._k = expansion_k(self.expansion.kernel, kwargs, self.expansion)
+        self._kc = helmholtz_k(self.expansion.kernel, kwargs)
         strengths = as_strength_list(kwargs.pop("strengths"), self.strength_count)
         weights = SourceWeights(actx, self.source_kernels, self.strength_usage, strengths,
                                 kwargs, self.dim, self.expansion.is_complex_valued)
@@ -177,7 +179,7 @@ class P2EFromCSR(P2EBase):
         bss = as_i32(kwargs.pop("box_source_starts"))
         bsc = as_i32(kwargs.pop("box_source_counts_nonchild"))
         if tgt.is_complex():
-            _capi.call("sumpyhip_p2l_helmholtz", tables.desc, self._k,
+            _capi.call("sumpyhip_p2l_helmholtz", tables.desc, *self._kc,
                        int(tb.shape[0]), _capi.ptr(tb), _capi.ptr(starts), _capi.ptr(lists),
                        _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
                        int(centers.shape[1]), *[s.data_ptr() for s in sources],

@@ -220,18 +220,42 @@ def kernel_k(kernel, kwargs):
     return complex(k)
 
 
-def expansion_k(kernel, kwargs):
-    """Real Helmholtz parameter of an FMM expansion (0 for Laplace).
+def expansion_k(kernel, kwargs, expansion=None):
+    """Real Helmholtz parameter of an FMM expansion's tables (0 for Laplace).
 
     The Helmholtz expansions here split complex coefficients into a real and
     an imaginary plane that the real translation kernels process one after
     the other; that needs the PDE projection (the ``(k rho)^2`` terms) to be
-    real, hence a real ``k``."""
+    real, hence a real ``k`` for PDE-compressed expansions.  Full-Taylor
+    expansions have no projection: with a complex ``k`` (the kernel's
+    ``allow_evanescent``) their tables are those of ``k = 0``, and the
+    k-dependent operators take :func:`helmholtz_k`."""
     k = kernel_k(kernel, kwargs)
     if k.imag != 0:
-        raise NotImplementedError(
-            "Helmholtz expansions on the GPU need a real wavenumber")
+        if expansion is None or getattr(expansion, "compressed", True):
+            raise NotImplementedError(
+                "PDE-compressed Helmholtz expansions on the GPU need a real wavenumber; "
+                "use the full-Taylor (VolumeTaylor*) expansions for a complex k")
+        _check_evanescent(kernel)
+        return 0.0
     return k.real
+
+
+def _check_evanescent(kernel):
+    if not getattr(kernel.get_base_kernel(), "allow_evanescent", False):
+        raise ValueError("a complex Helmholtz k needs HelmholtzKernel(..., "
+                         "allow_evanescent=True)")
+
+
+def helmholtz_k(kernel, kwargs):
+    """(Re k, Im k) for the k-dependent Helmholtz kernels (class data, M2P,
+    P2L, M2L without classes); Im k >= 0."""
+    k = kernel_k(kernel, kwargs)
+    if k.imag != 0:
+        _check_evanescent(kernel)
+        if k.imag < 0:
+            raise NotImplementedError("complex k needs Im k > 0")
+    return float(k.real), float(k.imag)
 
 
 def planes(z: torch.Tensor):

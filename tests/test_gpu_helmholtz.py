@@ -447,3 +447,53 @@ def test_helmholtz_tree_indep_reused_across_trees(actx):
         ref = run(tree_indep(), src, q)
         gc.collect()
         assert rel(got, ref) < 1e-13, (seed, rel(got, ref))
+
+
+@pytest.mark.parametrize("use_classes", [True, False])
+def test_helmholtz3d_complex_k_full_taylor_fmm(actx, use_classes):
+    """Damped waves in 3D (``HelmholtzKernel(3, allow_evanescent=True)``, k =
+    2 + 0.5i) with the full-Taylor expansions (no PDE projection, so the real
+    translation kernels are those of k = 0; the class data, M2P, P2L and the
+    class-free M2L take the complex k): converging to the direct sum with the
+    order; the PDE-compressed expansions refuse a complex k."""
+    from functools import partial
+
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import (
+        LinearPDEConformingVolumeTaylorLocalExpansion,
+        VolumeTaylorLocalExpansion,
+    )
+    from sumpy_amd.expansion.multipole import (
+        LinearPDEConformingVolumeTaylorMultipoleExpansion,
+        VolumeTaylorMultipoleExpansion,
+    )
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    k = 2.0 + 0.5j
+    knl = HelmholtzKernel(3, allow_evanescent=True)
+    rng = np.random.default_rng(7)
+    src = rng.random((3, 3000))
+    tgt = rng.random((3, 400))
+    q = rng.standard_normal(3000) + 1j*rng.standard_normal(3000)
+    r = np.sqrt(((tgt[:, :, None] - src[:, None, :])**2).sum(axis=0))
+    direct = (np.exp(1j*k*r)/(4*np.pi*r)) @ q
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 40)
+
+    def run(mcls, lcls, order):
+        ti = SumpyTreeIndependentDataForWrangler(actx, partial(mcls, knl), partial(lcls, knl),
+                                                 [knl])
+        wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a: order,
+                                    kernel_extra_kwargs={"k": k},
+                                    _disable_translation_classes=not use_classes)
+        print(f"helmholtz3d complex k full taylor p={order} classes={use_classes}", flush=True)
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        return actx.to_numpy(pot)
+
+    errs = [np.linalg.norm(run(VolumeTaylorMultipoleExpansion, VolumeTaylorLocalExpansion, p)
+                           - direct)/np.linalg.norm(direct) for p in (3, 6)]
+    print(f"complex-k full-Taylor FMM errors {errs}")
+    assert errs[1] < 0.05*errs[0] and errs[1] < 1e-4
+    with pytest.raises(NotImplementedError, match="complex k"):
+        run(LinearPDEConformingVolumeTaylorMultipoleExpansion,
+            LinearPDEConformingVolumeTaylorLocalExpansion, 3)
