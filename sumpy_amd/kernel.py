@@ -114,8 +114,8 @@ class YukawaKernel(ScalarKernel):
     ``i/4``).  ``lam`` is passed at call time under
     :attr:`yukawa_lambda_name`.  Complex-valued as in the reference.  The 2D
     kernel runs with the Y2D expansions and its own P2P (:mod:`sumpy_amd.h2d`);
-    the 3D kernel is built for the line-Taylor layer potential
-    (:mod:`sumpy_amd.qbx`) only."""
+    the 3D kernel is the 3D Helmholtz kernel at k = i lam (P2P, the full-Taylor
+    FMM) and has its own line-Taylor layer potential (:mod:`sumpy_amd.qbx`)."""
 
     dim: int
     yukawa_lambda_name: str = "lam"
@@ -336,6 +336,9 @@ def kernel_id(knl: ScalarKernel) -> int:
         return 13 if base.dim == 3 else 12
     if isinstance(base, YukawaKernel) and base.dim == 2:
         return 22
+    if isinstance(base, YukawaKernel) and base.dim == 3:
+        # e^{-lam r}/(4 pi r): the 3D Helmholtz kernel at k = i lam
+        return 13
     raise NotImplementedError(f"unsupported kernel: {knl}")
 
 

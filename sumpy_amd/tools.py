@@ -217,6 +217,11 @@ def kernel_k(kernel, kwargs):
     k = kwargs[names[0]]
     if isinstance(k, torch.Tensor):
         k = k.item()
+    from sumpy_amd.kernel import YukawaKernel
+    base = kernel.get_base_kernel()
+    if isinstance(base, YukawaKernel) and base.dim == 3:
+        # e^{-lam r}/r = e^{i k r}/r at k = i lam
+        return 1j*complex(k)
     return complex(k)
 
 
@@ -242,6 +247,9 @@ def expansion_k(kernel, kwargs, expansion=None):
 
 
 def _check_evanescent(kernel):
+    from sumpy_amd.kernel import YukawaKernel
+    if isinstance(kernel.get_base_kernel(), YukawaKernel):
+        return
     if not getattr(kernel.get_base_kernel(), "allow_evanescent", False):
         raise ValueError("a complex Helmholtz k needs HelmholtzKernel(..., "
                          "allow_evanescent=True)")

@@ -497,3 +497,41 @@ def test_helmholtz3d_complex_k_full_taylor_fmm(actx, use_classes):
     with pytest.raises(NotImplementedError, match="complex k"):
         run(LinearPDEConformingVolumeTaylorMultipoleExpansion,
             LinearPDEConformingVolumeTaylorLocalExpansion, 3)
+
+
+def test_yukawa3d_p2p_and_full_taylor_fmm(actx):
+    """YukawaKernel(3), e^{-lam r}/(4 pi r) (``sumpy/kernel.py:886-893``), as
+    the 3D Helmholtz kernel at k = i lam: the dense P2P against numpy, and
+    the full-Taylor FMM converging to it."""
+    from functools import partial
+
+    from sumpy_amd import P2P
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import VolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import VolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import YukawaKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    lam = 3.0
+    knl = YukawaKernel(3)
+    rng = np.random.default_rng(8)
+    src, tgt = rng.random((3, 3000)), rng.random((3, 400))
+    q = rng.standard_normal(3000) + 0j
+    r = np.sqrt(((tgt[:, :, None] - src[:, None, :])**2).sum(axis=0))
+    direct = (np.exp(-lam*r)/(4*np.pi*r)) @ q
+    d = actx.from_numpy
+    pot, = P2P([knl], exclude_self=False)(actx, d(tgt), d(src), [d(q)], lam=lam)
+    assert np.linalg.norm(actx.to_numpy(pot) - direct)/np.linalg.norm(direct) < 1e-13
+    tree, trav = build_tree_and_traversal(actx, src, tgt, 40)
+    errs = []
+    for order in (3, 6):
+        ti = SumpyTreeIndependentDataForWrangler(
+            actx, partial(VolumeTaylorMultipoleExpansion, knl),
+            partial(VolumeTaylorLocalExpansion, knl), [knl])
+        wr = SumpyExpansionWrangler(ti, trav, np.complex128, lambda *a, o=order: o,
+                                    kernel_extra_kwargs={"lam": lam})
+        print(f"yukawa3d full taylor p={order}", flush=True)
+        p, = drive_fmm(actx, wr, [d(q)])
+        errs.append(np.linalg.norm(actx.to_numpy(p) - direct)/np.linalg.norm(direct))
+    print(f"Yukawa 3D FMM errors {errs}")
+    assert errs[1] < 0.05*errs[0] and errs[1] < 3e-4
