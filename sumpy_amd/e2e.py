@@ -567,7 +567,10 @@ class M2LUsingTranslationClassesDependentData(E2EBase):
         out = per.get(key)
         if out is None:
             tabs = _bf16_tabs(self.src_expansion, self.tgt_expansion, actx.device)
-            ncls = int(data.shape[0])
+            # complex kernels: the real and imaginary planes of a class are
+            # the classes 2c and 2c + 1 of the real embedding
+            ncirc = _m2l_tabs(self.src_expansion, self.tgt_expansion, actx.device)["ncirc"]
+            ncls = int(data.numel())//int(ncirc)
             out = torch.empty((ncls, tabs["ncirc"]), dtype=torch.int32, device=data.device)
             _capi.call("sumpyhip_m2l_pack_bf16_class_data", ncls, tabs["shape"].ctypes.data,
                        *tabs["strides"], tabs["ncirc"], _capi.ptr(data), _capi.ptr(out),

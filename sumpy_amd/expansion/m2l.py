@@ -62,8 +62,6 @@ class VolumeTaylorM2LWithDenseClassMatrices(M2LTranslationBase):
         """Real for every kernel: complex (Helmholtz) expansions are contracted
         through their real embedding (see :func:`complex_rows`)."""
         dtype = np.dtype(dtype)
-        if dtype.kind == "c" and self.precision == "bf16x3":
-            raise NotImplementedError("the split-BF16 M2L is implemented for real kernels")
         return np.dtype(np.float64 if self.precision == "fp64" else np.float32)
 
     def translation_classes_dependent_ndata(self, tgt_expansion, src_expansion):

@@ -535,3 +535,29 @@ def test_yukawa3d_p2p_and_full_taylor_fmm(actx):
         errs.append(np.linalg.norm(actx.to_numpy(p) - direct)/np.linalg.norm(direct))
     print(f"Yukawa 3D FMM errors {errs}")
     assert errs[1] < 0.05*errs[0] and errs[1] < 3e-4
+
+
+def test_helmholtz_split_bf16_m2l(actx):
+    """The split-BF16 M2L on the real embedding of the complex Helmholtz
+    contraction (rows Mr, Mi, -Mi against the class planes Ar, Ai): at order 8
+    and k = 10 its error against the direct sum stays within 2x that of the
+    FP64 form, and it differs from the FP64 FMM by FP32-level rounding."""
+    rng = np.random.default_rng(3)
+    n = 6000
+    src = rng.random((3, n))
+    q = rng.standard_normal(n) + 1j*rng.standard_normal(n)
+    sel = np.arange(0, n, 15)
+    r = np.sqrt(((src[:, sel, None] - src[:, None, :])**2).sum(axis=0))
+    r[np.arange(len(sel)), sel] = 1.0
+    g = np.exp(10j*r)/(4*np.pi*r)
+    g[np.arange(len(sel)), sel] = 0.0
+    direct = g @ q
+    pots = {}
+    for prec in ("fp64", "bf16x3"):
+        print(f"helmholtz split-BF16 M2L check: {prec}", flush=True)
+        pots[prec] = run_helm(actx, src, q, 8, 10.0, m2l_precision=prec)[0]
+    err = {p: np.linalg.norm(v[sel] - direct)/np.linalg.norm(direct) for p, v in pots.items()}
+    diff = np.linalg.norm(pots["bf16x3"] - pots["fp64"])/np.linalg.norm(pots["fp64"])
+    print(f"errors vs direct {err}, bf16x3 vs fp64 {diff:.2e}")
+    assert err["bf16x3"] < 2*err["fp64"]
+    assert diff < 1e-5
