@@ -108,7 +108,7 @@ def make_wrangler(actx, tree, trav, order, m2l_precision, p2p_precision, k=None)
         partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl,
                 m2l_translation_override=tr),
         [knl], exclude_self=True, p2p_precision=p2p_precision, autotune=True,
-        list34_precision=p2p_precision if k is None else "fp64",
+        list34_precision=p2p_precision,
         l2p_precision=p2p_precision if k is None else "fp64",
         p2m_precision=p2p_precision if k is None else "fp64")
     import torch

@@ -265,7 +265,9 @@ int sumpyhip_p2l_levels(int kernel_id, const sumpyhip_expn *expn, int ntgt_boxes
 /* }}} */
 
 /* {{{ Helmholtz-3D (G = e^{ik|d|}/|d|, sumpy/kernel.py:754-844), k = k + i k_im
- * (k_im > 0: damped waves, full-Taylor expansions on the Python side) */
+ * (k_im > 0: damped waves, full-Taylor expansions on the Python side).  M2P and
+ * P2L take `precision` 0 (FP64) or 1 (FP32 terms with FP64 sums, the
+ * polynomial and moment forms of the PDE-compressed expansions). */
 
 /* M2P from list 3 for the Helmholtz kernel, replaces E2PFromCSR.__call__
  * (sumpy/e2p.py:233-343) for HelmholtzKernel(3); accumulates.  Multipoles,
@@ -278,7 +280,7 @@ int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, double k_im, int
                            const double *box_centers, int nboxes, const double *tx,
                            const double *ty, const double *tz, double rscale,
                            double scale, const void *src_expansions, int src_base_ibox,
-                           void *pot, void *gx, void *gy, void *gz, void *stream);
+                           void *pot, void *gx, void *gy, void *gz, int precision, void *stream);
 
 /* P2L from list 4 for the Helmholtz kernel, replaces P2EFromCSR.__call__
  * (sumpy/p2e.py:268-388) for HelmholtzKernel(3); overwrites the rows of the
@@ -290,7 +292,7 @@ int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, double k_im, int
                            const double *box_centers, int nboxes, const double *sx,
                            const double *sy, const double *sz, const void *w0,
                            const void *wx, const void *wy, const void *wz, double rscale,
-                           void *tgt_expansions, int tgt_base_ibox, void *stream);
+                           void *tgt_expansions, int tgt_base_ibox, int precision, void *stream);
 
 /* Class data for the Helmholtz kernel, replaces
  * M2LGenerateTranslationClassesDependentData.__call__ (sumpy/e2e.py:565-669)

@@ -101,9 +101,9 @@ _SIGNATURES = {
     "sumpyhip_p2l_levels": [_I, _P, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P,
                             _P, _P, _I, _P, _P, _I, _P],
     "sumpyhip_m2p_helmholtz": [_P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _D,
-                               _D, _P, _I, _P, _P, _P, _P, _P],
+                               _D, _P, _I, _P, _P, _P, _P, _I, _P],
     "sumpyhip_p2l_helmholtz": [_P, _D, _D, _I, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
-                               _P, _P, _P, _D, _P, _I, _P],
+                               _P, _P, _P, _D, _P, _I, _I, _P],
     "sumpyhip_m2l_csr_helmholtz": [_I, _I, _P, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _I, _D,
                                    _D, _D, _D, _P, _I, _P, _I, _P],
     "sumpyhip_m2l_class_data_helmholtz": [_I, _I, _P, _I, _P, _I, _I, _D, _D, _D, _P, _P],

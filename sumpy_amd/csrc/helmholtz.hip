@@ -276,6 +276,14 @@ __global__ __launch_bounds__(64) void m2p_helm3d(
 
 __host__ __device__ constexpr int tet(int n) { return n*(n + 1)*(n + 2)/6; }
 
+// 2-vector of T (the complex terms of the FP32 list-3/4 forms)
+template <typename T>
+struct V2T { typedef double2 type; };
+template <>
+struct V2T<float> { typedef float2 type; };
+template <typename T>
+__device__ __forceinline__ typename V2T<T>::type mk2(T a, T b) { return {a, b}; }
+
 template <int ORD>
 struct M2PPoly {
   static constexpr int N0 = tet(ORD + 1);   // coefficients with a = 0
@@ -289,23 +297,22 @@ __device__ __forceinline__ int stored_index(int a, int b, int c) {
   return d*d + (a ? d + 1 + b : b);
 }
 
-template <int ORD>
-__device__ __forceinline__ void m2p_poly_eval(const double2 *C, double x, double y, double z,
-                                              const double *gr, const double *gi,
-                                              double &pr, double &pi) {
+template <int ORD, typename T>
+__device__ __forceinline__ void m2p_poly_eval(const typename V2T<T>::type *C, T x, T y, T z,
+                                              const T *gr, const T *gi, T &pr, T &pi) {
   #pragma unroll
   for (int a = 0; a <= 1; ++a) {
     const int base = a ? M2PPoly<ORD>::N0 : 0;
-    double ar = 0, ai = 0;
+    T ar = 0, ai = 0;
     #pragma unroll
     for (int n = 0; n <= ORD - a; ++n) {
-      double Rr = 0, Ri = 0;
+      T Rr = 0, Ri = 0;
       #pragma unroll
       for (int b = n; b >= 0; --b) {
-        double Sr = 0, Si = 0;
+        T Sr = 0, Si = 0;
         #pragma unroll
         for (int c = n - b; c >= 0; --c) {
-          const double2 v = C[base + tet(n) + b*(n + 1) - b*(b - 1)/2 + c];
+          const auto v = C[base + tet(n) + b*(n + 1) - b*(b - 1)/2 + c];
           Sr = Sr*z + v.x;
           Si = Si*z + v.y;
         }
@@ -319,14 +326,16 @@ __device__ __forceinline__ void m2p_poly_eval(const double2 *C, double x, double
   }
 }
 
-template <int ORD>
+// T = float: the polynomial, its coefficients and g_q in FP32 (g_q from FP64
+// values), each source box's term added to FP64 sums
+template <int ORD, typename T>
 __global__ __launch_bounds__(64) void m2p_helm3d_poly(
     double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bts, const int *btc, const double *centers, int nboxes, const double *tx,
     const double *ty, const double *tz, double rscale, double scale, const double2 *src,
     int sbase, double2 *pot) {
   constexpr int NC = M2PPoly<ORD>::NC, NS = (ORD + 1)*(ORD + 1), HS = ORD + 1;
-  __shared__ double2 sC[NC];
+  __shared__ typename V2T<T>::type sC[NC];
   __shared__ double2 sM[NS];
   __shared__ double ht[HS*HS];
   __shared__ unsigned dec[NC];
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(64) void m2p_helm3d_poly(
             vr += w*m.x;
             vi += w*m.y;
           }
-          sC[i] = make_double2(vr, vi);
+          sC[i] = mk2<T>(T(vr), T(vi));
         }
         const double x = (X - centers[sb])*rinv, y = (Y - centers[nboxes + sb])*rinv,
                      z = (Z - centers[2*(size_t)nboxes + sb])*rinv;
@@ -390,7 +399,17 @@ __global__ __launch_bounds__(64) void m2p_helm3d_poly(
           }
         }
         __syncthreads();
-        m2p_poly_eval<ORD>(sC, x, y, z, gr, gi, pr, pi);
+        if constexpr (sizeof(T) == sizeof(double)) {
+          m2p_poly_eval<ORD, double>(sC, x, y, z, gr, gi, pr, pi);
+        } else {
+          T grt[ORD + 1], git[ORD + 1];
+          #pragma unroll
+          for (int q = 0; q <= ORD; ++q) { grt[q] = T(gr[q]); git[q] = T(gi[q]); }
+          T br = 0, bi = 0;
+          m2p_poly_eval<ORD, T>(sC, T(x), T(y), T(z), grt, git, br, bi);
+          pr += double(br);
+          pi += double(bi);
+        }
       }
       if (valid) {
         pot[t].x += scale*rinv*pr;
@@ -512,7 +531,9 @@ __global__ __launch_bounds__(256) void p2l_helm3d(
 // box's list-4 sources stream through LDS; the coefficients are assembled
 // from the moments at the end.
 
-template <int ORD>
+// T = float: the staged terms (q g_q, powers) and each chunk's moment sums
+// in FP32, added to FP64 totals per chunk of CH sources
+template <int ORD, typename T>
 __global__ __launch_bounds__(256) void p2l_helm3d_moments(
     double k, int ntgt, const int *target_boxes, const int *starts, const int *lists,
     const int *bss, const int *bsc, const double *centers, int nboxes, const double *sx,
@@ -522,13 +543,14 @@ __global__ __launch_bounds__(256) void p2l_helm3d_moments(
   constexpr int CH = P2LH_CH, NP = ORD + 1;
   // the source staging (q g_q, powers) and, after the last source, the
   // moments share one LDS buffer
+  typedef typename V2T<T>::type V2;
   constexpr int STAGE = NP*CH*2 + 2*NP*CH + CH;  // doubles
   constexpr int BUF = STAGE > 2*NC ? STAGE : 2*NC;
   __shared__ __align__(16) double sbuf[BUF];
-  double2 (*sQG)[CH] = reinterpret_cast<double2 (*)[CH]>(sbuf);
-  double (*sY)[CH] = reinterpret_cast<double (*)[CH]>(sbuf + NP*CH*2);
-  double (*sZ)[CH] = reinterpret_cast<double (*)[CH]>(sbuf + NP*CH*3);
-  double *sX = sbuf + NP*CH*4;
+  V2 (*sQG)[CH] = reinterpret_cast<V2 (*)[CH]>(sbuf);
+  T (*sY)[CH] = reinterpret_cast<T (*)[CH]>(sbuf + NP*CH*2);
+  T (*sZ)[CH] = reinterpret_cast<T (*)[CH]>(sbuf + NP*CH*3);
+  T *sX = reinterpret_cast<T *>(sbuf + NP*CH*4);
   double2 *sPhi = reinterpret_cast<double2 *>(sbuf);
   __shared__ double ht[HS*HS];
   herm_table(ht, ORD);
@@ -566,47 +588,69 @@ __global__ __launch_bounds__(256) void p2l_helm3d_moments(
           sincos(kappa*r, &sn, &cs);
           const double2 w = w0[si];
           double ar = cs*ri, ai = sn*ri;
-          sQG[0][tid] = make_double2(w.x*ar - w.y*ai, w.x*ai + w.y*ar);
+          sQG[0][tid] = mk2<T>(T(w.x*ar - w.y*ai), T(w.x*ai + w.y*ar));
           double br = 0, bi = 0;
           if (ORD >= 1) {
             const double kr = kappa*r;
             br = (-cs - sn*kr)*ri2*ri;
             bi = (cs*kr - sn)*ri2*ri;
-            sQG[1][tid] = make_double2(w.x*br - w.y*bi, w.x*bi + w.y*br);
+            sQG[1][tid] = mk2<T>(T(w.x*br - w.y*bi), T(w.x*bi + w.y*br));
           }
           #pragma unroll
           for (int g = 2; g <= ORD; ++g) {
             const double f = -(2*g - 1);
             const double cr = (f*br - kappa*kappa*ar)*ri2, ci = (f*bi - kappa*kappa*ai)*ri2;
-            sQG[g][tid] = make_double2(w.x*cr - w.y*ci, w.x*ci + w.y*cr);
+            sQG[g][tid] = mk2<T>(T(w.x*cr - w.y*ci), T(w.x*ci + w.y*cr));
             ar = br; ai = bi; br = cr; bi = ci;
           }
           double py = 1, pz = 1;
           #pragma unroll
           for (int e = 0; e <= ORD; ++e) {
-            sY[e][tid] = py;
-            sZ[e][tid] = pz;
+            sY[e][tid] = T(py);
+            sZ[e][tid] = T(pz);
             py *= y;
             pz *= z;
           }
-          sX[tid] = x;
+          sX[tid] = T(x);
         }
         __syncthreads();
-        if (tid < NS) {
-          #pragma unroll 2
-          for (int j = 0; j < n; ++j) {
-            const double2 qg = sQG[ra + rn][j];
-            const double m = (ra ? sX[j] : 1.0)*sY[rb][j];
-            const double wr = qg.x*m, wi = qg.y*m;
-            #pragma unroll
-            for (int g = 0; g < NP; ++g) {
-              if (g < rlen) {
-                const double zg = sZ[g][j];
-                pr[g] += wr*zg;
-                pi[g] += wi*zg;
+        if constexpr (sizeof(T) == sizeof(double)) {
+          if (tid < NS) {
+            #pragma unroll 2
+            for (int j = 0; j < n; ++j) {
+              const V2 qg = sQG[ra + rn][j];
+              const double m = (ra ? sX[j] : 1.0)*sY[rb][j];
+              const double wr = qg.x*m, wi = qg.y*m;
+              #pragma unroll
+              for (int g = 0; g < NP; ++g) {
+                if (g < rlen) {
+                  const double zg = sZ[g][j];
+                  pr[g] += wr*zg;
+                  pi[g] += wi*zg;
+                }
               }
             }
           }
+        } else if (tid < NS) {
+          T cr[NP], ci[NP];
+          #pragma unroll
+          for (int g = 0; g < NP; ++g) { cr[g] = 0; ci[g] = 0; }
+          #pragma unroll 2
+          for (int j = 0; j < n; ++j) {
+            const V2 qg = sQG[ra + rn][j];
+            const T m = (ra ? sX[j] : T(1))*sY[rb][j];
+            const T wr = qg.x*m, wi = qg.y*m;
+            #pragma unroll
+            for (int g = 0; g < NP; ++g) {
+              if (g < rlen) {
+                const T zg = sZ[g][j];
+                cr[g] += wr*zg;
+                ci[g] += wi*zg;
+              }
+            }
+          }
+          #pragma unroll
+          for (int g = 0; g < NP; ++g) { pr[g] += double(cr[g]); pi[g] += double(ci[g]); }
         }
       }
     }
@@ -768,8 +812,9 @@ extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, doubl
                                       double rscale, double scale,
                                       const void *src_expansions, int src_base_ibox,
                                       void *pot, void *gx, void *gy, void *gz,
-                                      void *stream) {
+                                      int precision, void *stream) {
   int rc = check_helm_expn(expn, "m2p_helmholtz");
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "m2p_helmholtz: precision %d", precision);
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(pot || gx, "m2p_helmholtz: no output");
@@ -785,11 +830,18 @@ extern "C" int sumpyhip_m2p_helmholtz(const sumpyhip_expn *expn, double k, doubl
     // potentials of the usual orders: the polynomial form
 #define SUMPY_M2P_POLY(O)                                                                  \
   case O:                                                                                  \
-    m2p_helm3d_poly<O><<<grid, 64, 0, st>>>(                                               \
-        k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
-        box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,    \
-        rscale, scale, static_cast<const double2 *>(src_expansions), src_base_ibox,        \
-        static_cast<double2 *>(pot));                                                      \
+    if (precision)                                                                         \
+      m2p_helm3d_poly<O, float><<<grid, 64, 0, st>>>(                                      \
+          k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                \
+          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,  \
+          rscale, scale, static_cast<const double2 *>(src_expansions), src_base_ibox,      \
+          static_cast<double2 *>(pot));                                                    \
+    else                                                                                   \
+      m2p_helm3d_poly<O, double><<<grid, 64, 0, st>>>(                                     \
+          k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                \
+          box_target_starts, box_target_counts_nonchild, box_centers, nboxes, tx, ty, tz,  \
+          rscale, scale, static_cast<const double2 *>(src_expansions), src_base_ibox,      \
+          static_cast<double2 *>(pot));                                                    \
     return check_launch("m2p_helmholtz");
     switch (expn->order) {
       SUMPY_M2P_POLY(1) SUMPY_M2P_POLY(2) SUMPY_M2P_POLY(3) SUMPY_M2P_POLY(4)
@@ -823,8 +875,9 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, doubl
                                       const double *sx, const double *sy, const double *sz,
                                       const void *w0, const void *wx, const void *wy,
                                       const void *wz, double rscale, void *tgt_expansions,
-                                      int tgt_base_ibox, void *stream) {
+                                      int tgt_base_ibox, int precision, void *stream) {
   int rc = check_helm_expn(expn, "p2l_helmholtz");
+  SUMPY_REQUIRE(precision == 0 || precision == 1, "p2l_helmholtz: precision %d", precision);
   if (rc) return rc;
   if (ntgt_boxes == 0) return SUMPYHIP_OK;
   SUMPY_REQUIRE(w0 != nullptr, "p2l_helmholtz: null charge weights");
@@ -840,11 +893,18 @@ extern "C" int sumpyhip_p2l_helmholtz(const sumpyhip_expn *expn, double k, doubl
     // charges at the usual orders: the moment form
 #define SUMPY_P2L_MOM(O)                                                                   \
   case O:                                                                                  \
-    p2l_helm3d_moments<O><<<grid, nt, 0, st>>>(                                            \
-        k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                  \
-        box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,    \
-        static_cast<const double2 *>(w0), rscale, static_cast<double2 *>(tgt_expansions), \
-        tgt_base_ibox);                                                                    \
+    if (precision)                                                                         \
+      p2l_helm3d_moments<O, float><<<grid, nt, 0, st>>>(                                   \
+          k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                \
+          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,  \
+          static_cast<const double2 *>(w0), rscale,                                        \
+          static_cast<double2 *>(tgt_expansions), tgt_base_ibox);                          \
+    else                                                                                   \
+      p2l_helm3d_moments<O, double><<<grid, nt, 0, st>>>(                                  \
+          k, ntgt_boxes, target_boxes, source_box_starts, source_box_lists,                \
+          box_source_starts, box_source_counts_nonchild, box_centers, nboxes, sx, sy, sz,  \
+          static_cast<const double2 *>(w0), rscale,                                        \
+          static_cast<double2 *>(tgt_expansions), tgt_base_ibox);                          \
     return check_launch("p2l_helmholtz");
     switch (expn->order) {
       SUMPY_P2L_MOM(1) SUMPY_P2L_MOM(2) SUMPY_P2L_MOM(3) SUMPY_P2L_MOM(4)

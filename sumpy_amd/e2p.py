@@ -188,7 +188,8 @@ class E2PFromCSR(E2PBase):
                        float(kwargs["rscale"]),
                        self.expansion.kernel.get_global_scaling_const(),
                        _capi.ptr(kwargs["src_expansions"]), int(kwargs["src_base_ibox"]),
-                       *ptrs, actx.stream_handle)
+                       *ptrs, {"fp64": 0, "fp32": 1}[kwargs.get("list34_precision", "fp64")],
+                       actx.stream_handle)
             outs.finish(actx, targets, accumulate=True)
             return result
         if "box_levels" in kwargs:

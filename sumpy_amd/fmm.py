@@ -434,7 +434,8 @@ class SumpyExpansionWrangler:
             res = p2m(actx, source_boxes=source_boxes[start:stop],
                       centers=self.tree.box_centers, strengths=src_weight_vecs,
                       tgt_expansions=view, tgt_base_ibox=base,
-                      rscale=self.level_to_rscale(lev), **kwargs)
+                      rscale=self.level_to_rscale(lev),
+                      list34_precision=self.tree_indep.list34_precision, **kwargs)
             assert res is view
         return mpoles
 
@@ -626,7 +627,8 @@ class SumpyExpansionWrangler:
                       target_boxes=target_boxes_by_source_level[lev],
                       source_box_starts=ssn.starts, source_box_lists=ssn.lists,
                       centers=self.tree.box_centers, result=pot,
-                      rscale=self.level_to_rscale(lev), **kwargs)
+                      rscale=self.level_to_rscale(lev),
+                      list34_precision=self.tree_indep.list34_precision, **kwargs)
             for a, b in zip(pot, res):
                 assert a is b
         return pot
@@ -672,7 +674,8 @@ class SumpyExpansionWrangler:
                       source_box_starts=starts[start:stop + 1], source_box_lists=lists,
                       centers=self.tree.box_centers, strengths=src_weight_vecs,
                       tgt_expansions=view, tgt_base_ibox=base,
-                      rscale=self.level_to_rscale(lev), **kwargs)
+                      rscale=self.level_to_rscale(lev),
+                      list34_precision=self.tree_indep.list34_precision, **kwargs)
             assert res is view
         return local_exps
 

@@ -184,7 +184,7 @@ class P2EFromCSR(P2EBase):
                        _capi.ptr(bss), _capi.ptr(bsc), _capi.ptr(centers),
                        int(centers.shape[1]), *[s.data_ptr() for s in sources],
                        *weights.ptrs(3), rscale, _capi.ptr(tgt),
-                       int(kwargs.pop("tgt_base_ibox")), actx.stream_handle)
+                       int(kwargs.pop("tgt_base_ibox")), prec, actx.stream_handle)
             return tgt
         if "box_levels" in kwargs:
             # target boxes of several levels in one launch (each with the

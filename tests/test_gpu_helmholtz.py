@@ -561,3 +561,47 @@ def test_helmholtz_split_bf16_m2l(actx):
     print(f"errors vs direct {err}, bf16x3 vs fp64 {diff:.2e}")
     assert err["bf16x3"] < 2*err["fp64"]
     assert diff < 1e-5
+
+
+def test_helmholtz_fp32_list34(actx):
+    """FP32 terms with FP64 sums for the Helmholtz M2P (polynomial form) and
+    P2L (moment form) of lists 3 and 4 on a clustered tree: the potential
+    differs from the FP64 form by FP32-level rounding, and the error against
+    the direct sum stays within 2x."""
+    from sumpy_amd.drive import drive_fmm
+    from sumpy_amd.expansion.local import LinearPDEConformingVolumeTaylorLocalExpansion
+    from sumpy_amd.expansion.multipole import LinearPDEConformingVolumeTaylorMultipoleExpansion
+    from sumpy_amd.fmm import SumpyExpansionWrangler, SumpyTreeIndependentDataForWrangler
+    from sumpy_amd.kernel import HelmholtzKernel
+    from sumpy_amd.tree import build_tree_and_traversal
+    rng = np.random.default_rng(31)
+    n = 6000
+    c = rng.uniform(0.2, 0.8, (6, 3))
+    src = np.clip(c[rng.integers(0, 6, n)].T + 0.03*rng.standard_normal((3, n)), 0, 1)
+    q = rng.standard_normal(n) + 1j*rng.standard_normal(n)
+    tree, trav = build_tree_and_traversal(actx, src, None, 20)
+    assert sum(len(x[2]) for x in trav.host["sep_smaller"]) > 0
+    knl = HelmholtzKernel(3)
+    pots = {}
+    for prec in ("fp64", "fp32"):
+        ti = SumpyTreeIndependentDataForWrangler(
+            actx, partial(LinearPDEConformingVolumeTaylorMultipoleExpansion, knl),
+            partial(LinearPDEConformingVolumeTaylorLocalExpansion, knl), [knl],
+            exclude_self=True, list34_precision=prec)
+        wr = SumpyExpansionWrangler(
+            ti, trav, np.complex128, lambda *a: 8, kernel_extra_kwargs={"k": 10.0},
+            self_extra_kwargs={"target_to_source": actx.from_numpy(
+                np.arange(n, dtype=np.int32))})
+        print(f"helmholtz list 3/4 {prec}", flush=True)
+        pot, = drive_fmm(actx, wr, [actx.from_numpy(q)])
+        pots[prec] = actx.to_numpy(pot)
+    sel = np.arange(0, n, 20)
+    r = np.sqrt(((src[:, sel, None] - src[:, None, :])**2).sum(axis=0))
+    r[np.arange(len(sel)), sel] = 1.0
+    g = np.exp(10j*r)/(4*np.pi*r)
+    g[np.arange(len(sel)), sel] = 0.0
+    direct = g @ q
+    err = {p: np.linalg.norm(v[sel] - direct)/np.linalg.norm(direct) for p, v in pots.items()}
+    diff = np.linalg.norm(pots["fp32"] - pots["fp64"])/np.linalg.norm(pots["fp64"])
+    print(f"list 3/4 errors {err}, fp32 vs fp64 {diff:.2e}")
+    assert err["fp32"] < 2*err["fp64"] and diff < 1e-5
